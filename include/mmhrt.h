@@ -1,0 +1,210 @@
+/*
+ * mmhrt — MI355X (gfx950) particle-sphere ray tracer, C ABI.
+ *
+ * This is the drop-in boundary for MegaMol's sphere ray-tracing path. A MegaMol
+ * renderer module (or any other host: ctypes, a benchmark, a test harness) calls
+ * these functions; everything behind them is hand-written HIP for gfx950.
+ *
+ * Reference interfaces replaced (paths relative to the MegaMol source tree):
+ *   - plugins/optix_hpg/src/optix/SphereGeometry.cpp:78-227   (assertData: per-list
+ *     conversion, upload, bounds kernel, optixAccelBuild)        -> mmhrt_set_spheres + mmhrt_build
+ *   - plugins/datatools/src/AddParticleColors.cpp:40-117        -> mmhrt_set_transfer_function
+ *   - plugins/optix_hpg/src/optix/Renderer.cpp:136-172          (camera -> FrameState)
+ *     plugins/optix_hpg/src/optix/AbstractRenderer.cpp:27-111   (Render: change detection, launch)
+ *     plugins/optix_hpg/cuda_resources/raygen.cu:79-278          (raygen + traceRay) -> mmhrt_render
+ *   - frontend/services/cuda/CUDA_Service.cpp:30-56             (device context)    -> mmhrt_create
+ *
+ * Conventions
+ *   - Every function returns an int32 status (MMHRT_OK == 0). On failure the
+ *     context keeps a message, readable through mmhrt_last_error().
+ *   - A context is single-threaded; calls are synchronous unless stated.
+ *   - Inputs passed as host pointers are copied before the call returns, matching
+ *     the lock/unlock lifetime of MultiParticleDataCall frames
+ *     (plugins/moldyn/src/io/MMPLDDataSource.cpp:443-461).
+ *   - Pixel (x, y) has y = 0 at the bottom row, like the reference's CUDA surface
+ *     (raygen.cu:241-242 maps y = 0 to v = -th).
+ */
+#ifndef MMHRT_H
+#define MMHRT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MMHRT_API __attribute__((visibility("default")))
+
+/* ---- status codes ------------------------------------------------------- */
+enum {
+    MMHRT_OK = 0,
+    MMHRT_E_INVALID = 1,     /* bad argument (null pointer, bad enum, size 0 where >0 needed) */
+    MMHRT_E_HIP = 2,         /* a HIP runtime call failed */
+    MMHRT_E_OOM = 3,         /* device allocation failed */
+    MMHRT_E_UNSUPPORTED = 4, /* valid request that this build does not implement */
+    MMHRT_E_STATE = 5        /* call order violated (e.g. render before build) */
+};
+
+/* ---- particle list: mirrors geocalls::SimpleSphericalParticles ----------- */
+/* Vertex types: plugins/geometry_calls/include/geometry_calls/SimpleSphericalParticles.h:27-33 */
+enum {
+    MMHRT_VERTDATA_NONE = 0,
+    MMHRT_VERTDATA_FLOAT_XYZ = 1,
+    MMHRT_VERTDATA_FLOAT_XYZR = 2,
+    MMHRT_VERTDATA_SHORT_XYZ = 3,
+    MMHRT_VERTDATA_DOUBLE_XYZ = 4
+};
+/* Colour types: SimpleSphericalParticles.h:41-50 (note: these are the in-memory enum
+ * values, not the MMPLD file codes of MMPLDDataSource.cpp:117-156). */
+enum {
+    MMHRT_COLDATA_NONE = 0,
+    MMHRT_COLDATA_UINT8_RGB = 1,
+    MMHRT_COLDATA_UINT8_RGBA = 2,
+    MMHRT_COLDATA_FLOAT_RGB = 3,
+    MMHRT_COLDATA_FLOAT_RGBA = 4,
+    MMHRT_COLDATA_FLOAT_I = 5,
+    MMHRT_COLDATA_USHORT_RGBA = 6,
+    MMHRT_COLDATA_DOUBLE_I = 7
+};
+
+/* One particle list, as AccessParticles(i) exposes it
+ * (plugins/geometry_calls/include/geometry_calls/AbstractParticleDataCall.h:61-81).
+ * Strides of 0 mean "tightly packed" (the element size of SimpleSphericalParticles.cpp:7-9). */
+typedef struct mmhrt_sphere_list {
+    int32_t vert_type;
+    int32_t col_type;
+    const void* vert;       /* host pointer to the first vertex */
+    const void* col;        /* host pointer to the first colour (ignored for COLDATA_NONE) */
+    uint64_t vert_stride;   /* bytes between vertices */
+    uint64_t col_stride;    /* bytes between colours */
+    uint64_t count;
+    float global_radius;    /* used unless vert_type == FLOAT_XYZR */
+    uint8_t global_rgba[4]; /* used when the list has no per-particle colour */
+    float col_min;          /* colour-map index range of FLOAT_I / DOUBLE_I lists */
+    float col_max;
+} mmhrt_sphere_list;
+
+/* ---- camera: mirrors core::view::Camera Pose + PerspectiveParameters ----- */
+/* core/include/mmcore/view/Camera.h:41-115; fovy is in radians (Camera.h:367). */
+typedef struct mmhrt_camera {
+    float position[3];
+    float direction[3];
+    float up[3];
+    float fovy;       /* full vertical field of view, radians */
+    float aspect;     /* width / height */
+    float near_plane; /* projection near plane (depth conversion only) */
+    float far_plane;  /* projection far plane (depth conversion only) */
+    float tile_start[2]; /* image-plane tile (Camera.h:94-99), default {0,0} */
+    float tile_end[2];   /* default {1,1} */
+} mmhrt_camera;
+
+/* ---- render parameters: mirrors the optix_hpg Renderer params ------------ */
+/* spp / max bounces / accumulate: plugins/optix_hpg/src/optix/Renderer.cpp:22-39;
+ * AO samples / distance: plugins/mmospray/src/AbstractOSPRayRenderer.cpp:370-377. */
+enum {
+    MMHRT_MODE_LAMBERT = 0, /* primary rays + albedo*(0.2+0.6|N.d|) (raygen.cu:140-142) */
+    MMHRT_MODE_AO = 1,      /* primary rays + 1 cosine-weighted AO ray per sample */
+    MMHRT_MODE_BOUNCE = 2   /* diffuse multi-bounce path of raygen.cu:144-154 */
+};
+enum {
+    MMHRT_FLAG_AUX = 1u << 0,        /* also write fp16 denoiser features (albedo, normal, depth) */
+    MMHRT_FLAG_NO_DOWNLOAD = 1u << 1 /* keep the frame on the device; read it later with mmhrt_download */
+};
+typedef struct mmhrt_render_params {
+    uint32_t width;
+    uint32_t height;
+    uint32_t spp;          /* samples per pixel, >= 1 */
+    int32_t mode;          /* MMHRT_MODE_* */
+    float ao_distance;     /* AO ray length (tmax) */
+    int32_t max_bounces;   /* MMHRT_MODE_BOUNCE only */
+    uint32_t frame_idx;    /* RNG frame index; > 0 with accumulate blends with the previous frame */
+    int32_t accumulate;    /* progressive accumulation (raygen.cu:254-260) */
+    float background[4];   /* RGBA background (CallRender3D::BackgroundColor) */
+    uint32_t flags;        /* MMHRT_FLAG_* */
+} mmhrt_render_params;
+
+/* ---- framebuffer: host destinations (any pointer may be NULL = skip) ------ */
+typedef struct mmhrt_framebuffer {
+    uint32_t* rgba8;   /* width*height RGBA8, as CPUFramebuffer::colorBuffer (CPUFramebuffer.h:21) */
+    float* depth;      /* pseudo depth 0.5*(C/t - A)+0.5, 1.0 on miss (raygen.cu:262-268) */
+    float* color;      /* width*height*4 float RGBA before quantisation */
+    float* t;          /* linear ray distance of the nearest primary hit, +inf on miss */
+    int32_t* hit_id;   /* global particle index of the nearest primary hit, -1 on miss */
+    uint16_t* aux;     /* width*height*8 fp16: albedo rgb, normal xyz, linear depth, visibility */
+} mmhrt_framebuffer;
+
+/* ---- build policy --------------------------------------------------------- */
+enum {
+    MMHRT_BUILD_REBUILD = 0, /* full LBVH build (the reference always builds: SphereGeometry.cpp:197-214) */
+    MMHRT_BUILD_REFIT = 1,   /* keep topology, recompute boxes from moved spheres */
+    MMHRT_BUILD_AUTO = 2     /* refit while the measured cost ratio allows, else rebuild */
+};
+
+typedef struct mmhrt_ctx_s* mmhrt_ctx;
+
+/* Device context. Replaces CUDA_Service::init (frontend/services/cuda/CUDA_Service.cpp:30-56). */
+MMHRT_API int32_t mmhrt_create(int32_t device, mmhrt_ctx* out);
+MMHRT_API int32_t mmhrt_destroy(mmhrt_ctx ctx);
+/* Message of the last failed call on this context ("" if none). ctx may be NULL for create errors. */
+MMHRT_API const char* mmhrt_last_error(mmhrt_ctx ctx);
+
+/* Upload and convert particle lists: SphereGeometry::assertData (SphereGeometry.cpp:94-146).
+ * Lists with count 0 or VERTDATA_NONE are skipped; particle IDs are the running index over
+ * all non-skipped lists in order. data_hash / frame_id mirror MultiParticleDataCall
+ * (AbstractGetDataCall.h:54, AbstractGetData3DCall.h:49) and are reported back unchanged. */
+MMHRT_API int32_t mmhrt_set_spheres(mmhrt_ctx ctx, const mmhrt_sphere_list* lists, uint32_t n_lists,
+                                    uint64_t data_hash, uint32_t frame_id);
+/* Same, with data already in device memory as packed float4 xyzr and optional float4 rgba
+ * (NULL: every sphere takes global_rgba/255). Device pointers are copied on-device. */
+MMHRT_API int32_t mmhrt_set_spheres_device(mmhrt_ctx ctx, const void* xyzr_dev, const void* rgba_dev,
+                                           uint64_t count, const uint8_t global_rgba[4]);
+/* Transfer function for FLOAT_I / DOUBLE_I lists: AddParticleColors::manipulateData
+ * (plugins/datatools/src/AddParticleColors.cpp:40-117). rgba has n*4 floats. If vmin > vmax the
+ * range is taken from the lists' [col_min, col_max] (AddParticleColors.cpp:59-71). Applies to the
+ * next mmhrt_set_spheres; n == 0 removes it (intensity lists then use their global colour,
+ * SphereGeometry.cpp:102-105). */
+MMHRT_API int32_t mmhrt_set_transfer_function(mmhrt_ctx ctx, const float* rgba, uint32_t n, float vmin, float vmax);
+
+/* Build (or refit) the LBVH over the current spheres: replaces optixAccelBuild
+ * (SphereGeometry.cpp:197-214). morton_bits is 30 or 63. */
+MMHRT_API int32_t mmhrt_build(mmhrt_ctx ctx, int32_t policy, int32_t morton_bits);
+
+/* Render one frame: AbstractRenderer::Render + raygen_program (AbstractRenderer.cpp:27-111,
+ * raygen.cu:159-278). out may be NULL together with MMHRT_FLAG_NO_DOWNLOAD. */
+MMHRT_API int32_t mmhrt_render(mmhrt_ctx ctx, const mmhrt_camera* cam, const mmhrt_render_params* params,
+                               mmhrt_framebuffer* out);
+/* Copy the last rendered frame to host buffers (any pointer NULL = skip). */
+MMHRT_API int32_t mmhrt_download(mmhrt_ctx ctx, mmhrt_framebuffer* out);
+
+/* Wait for all queued work on the context's stream. */
+MMHRT_API int32_t mmhrt_sync(mmhrt_ctx ctx);
+
+/* Scene facts: sphere count, scene bounds (6 floats: min xyz, max xyz incl. radii), LBVH node count. */
+MMHRT_API int32_t mmhrt_get_info(mmhrt_ctx ctx, uint64_t* n_spheres, float* bounds6, uint64_t* n_nodes);
+
+/* Copy the LBVH to host for structural checks: nodes (n_nodes*16 floats: child0 lo.xyz hi.xyz,
+ * child1 lo.xyz hi.xyz, then the two child references as int32 bit patterns, 2 pad words),
+ * leaf_ids (n_spheres uint32, particle index of leaf k), keys (n_spheres uint64 Morton keys in
+ * leaf order). Any pointer may be NULL. */
+MMHRT_API int32_t mmhrt_get_bvh(mmhrt_ctx ctx, float* nodes, uint32_t* leaf_ids, uint64_t* keys);
+
+/* Per-kernel timing of the last mmhrt_build / mmhrt_render on the context's stream, measured
+ * with HIP events: times_ms[0..7] = morton, sort, hierarchy, bounds, trace, resolve, total build,
+ * total render. */
+MMHRT_API int32_t mmhrt_get_timings(mmhrt_ctx ctx, float* times_ms, int32_t n);
+
+/* Instrumented trace counters of the last render (when enabled with mmhrt_set_option
+ * "count_visits"): node visits, sphere tests, rays traced. Also refreshes the option
+ * "stack_overflow" (1 if a traversal stack ever exceeded its capacity; 0 expected). */
+MMHRT_API int32_t mmhrt_get_counters(mmhrt_ctx ctx, uint64_t* node_visits, uint64_t* sphere_tests, uint64_t* rays);
+
+/* Tuning knobs (autotuner output): "stack_lds" (LDS stack entries per lane), "tile_w", "tile_h",
+ * "persistent" (0/1), "refill" (ballot refill threshold in lanes), "count_visits" (0/1). */
+MMHRT_API int32_t mmhrt_set_option(mmhrt_ctx ctx, const char* key, int64_t value);
+MMHRT_API int32_t mmhrt_get_option(mmhrt_ctx ctx, const char* key, int64_t* value);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MMHRT_H */

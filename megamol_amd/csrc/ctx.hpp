@@ -1,0 +1,166 @@
+// ctx.hpp — context state of the mmhrt library and the launchers each .hip unit exports.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "mmhrt.h"
+#include "rt_math.hpp"
+
+namespace mmhrt {
+
+// BVH2 node, 64 bytes: both child boxes in one node so that one visit tests two children
+// with four 16-byte loads. child ref >= 0: internal node index; < 0: leaf ~k (k-th sphere in
+// Morton order). w of d holds the subtree height (leaves have height 0).
+struct __align__(16) Node {
+    float4 a;  // c0.lo.x c0.lo.y c0.lo.z c0.hi.x
+    float4 b;  // c0.hi.y c0.hi.z c1.lo.x c1.lo.y
+    float4 c;  // c1.lo.z c1.hi.x c1.hi.y c1.hi.z
+    int4 d;    // child0, child1, parent, height
+};
+
+// Scene facts computed on the device by the bounds reduction.
+struct SceneInfo {
+    float clo[3], chi[3];  // centre bounds
+    float blo[3], bhi[3];  // sphere bounds (centre -+ radius)
+    float absmax;          // max over spheres of max(|x|,|y|,|z|) + r
+    float scale;           // Morton cells per world unit
+    uint32_t height;       // root height after a build
+    uint32_t pad_;
+};
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t cap = 0;  // elements
+};
+
+struct Timers {
+    hipEvent_t ev[12];
+    bool init = false;
+};
+
+}  // namespace mmhrt
+
+struct mmhrt_ctx_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int cu_count = 0;
+
+    // scene (original particle order)
+    uint64_t n = 0;
+    mmhrt::DevBuf<float4> sph, col;
+    mmhrt::DevBuf<uint8_t> staging;
+    uint64_t data_hash = 0;
+    uint32_t frame_id = 0;
+
+    // transfer function (AddParticleColors semantics)
+    std::vector<float> tf;
+    float tf_min = 1.0f, tf_max = 0.0f;
+    mmhrt::DevBuf<float> tf_buf;
+
+    // LBVH
+    int bits = 30;
+    bool built = false;
+    uint64_t built_n = 0;
+    mmhrt::DevBuf<uint64_t> keys64;  // Morton keys in leaf order
+    mmhrt::DevBuf<uint32_t> ka, kb;  // 32-bit key ping-pong
+    mmhrt::DevBuf<uint64_t> kla, klb;  // 64-bit key ping-pong
+    mmhrt::DevBuf<uint32_t> va, vb;  // values ping-pong
+    mmhrt::DevBuf<uint32_t> hist, scan_tmp;
+    mmhrt::DevBuf<float4> leaf_sph;
+    mmhrt::DevBuf<uint32_t> leaf_id;
+    mmhrt::DevBuf<mmhrt::Node> nodes;
+    mmhrt::DevBuf<int32_t> parent;
+    mmhrt::DevBuf<uint32_t> flags;
+    mmhrt::DevBuf<float> partial;
+    mmhrt::DevBuf<mmhrt::SceneInfo> info;
+    mmhrt::SceneInfo info_host{};
+    bool info_valid = false;
+
+    // frame
+    uint32_t fb_w = 0, fb_h = 0;
+    mmhrt::DevBuf<float4> fb_col;
+    mmhrt::DevBuf<float> fb_depth, fb_t;
+    mmhrt::DevBuf<int32_t> fb_id;
+    mmhrt::DevBuf<uint32_t> fb_rgba8;
+    mmhrt::DevBuf<__half> fb_aux;
+    mmhrt::DevBuf<int32_t> spill;
+    mmhrt::DevBuf<uint32_t> counters;  // [0] tile counter, [2..7] visit counters (u64 pairs)
+
+    // options
+    int64_t opt_stack_lds = 16;
+    int64_t opt_persistent = 1;
+    int64_t opt_refill = 0;
+    int64_t opt_count = 0;
+    int64_t opt_blocks_per_cu = 8;
+    int64_t opt_tile_w = 8, opt_tile_h = 8;
+    int64_t opt_refit_period = 4;  // MMHRT_BUILD_AUTO: refits between rebuilds
+    int64_t refits_since_build = 0;
+    int64_t stack_overflow = 0;  // set by mmhrt_get_counters from the trace kernel's flag
+
+    // timing
+    mmhrt::Timers tm;
+    float t_build[5] = {0, 0, 0, 0, 0};  // morton(+bounds), sort, hierarchy, boxes, total
+    float t_render[3] = {0, 0, 0};       // trace, resolve, total
+    uint64_t last_visits = 0, last_tests = 0, last_rays = 0;
+};
+
+namespace mmhrt {
+// error helper: records message, returns code
+int fail(mmhrt_ctx_s* c, int code, const std::string& msg);
+int hip_fail(mmhrt_ctx_s* c, hipError_t e, const char* what);
+
+template <typename T>
+int ensure(mmhrt_ctx_s* c, DevBuf<T>& b, size_t n, const char* what) {
+    if (b.cap >= n && b.p) return 0;
+    if (b.p) {
+        hipError_t e = hipFree(b.p);
+        b.p = nullptr;
+        b.cap = 0;
+        if (e != hipSuccess) return hip_fail(c, e, what);
+    }
+    size_t bytes = (n ? n : 1) * sizeof(T);
+    hipError_t e = hipMalloc((void**)&b.p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        b.p = nullptr;
+        return fail(c, 3, std::string("device allocation failed: ") + what + " (" + std::to_string(bytes) + " bytes)");
+    }
+    b.cap = n ? n : 1;
+    return 0;
+}
+template <typename T>
+void release(DevBuf<T>& b) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+}
+
+// ingest.hip
+struct ListDesc {
+    const uint8_t* vert;
+    const uint8_t* col;
+    uint64_t vstride, cstride, count, out_offset;
+    int32_t vtype, ctype;
+    float gradius;
+    float gcol[4];
+    int32_t use_tf;
+};
+int launch_convert_list(mmhrt_ctx_s* c, const ListDesc& d, const float* tf_dev, uint32_t tf_n, float tf_vmin,
+                        float tf_fac);
+int launch_copy_packed(mmhrt_ctx_s* c, const float4* xyzr, const float4* rgba, uint64_t n, float4 gcol);
+void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t v, uint64_t n);
+
+// lbvh.hip
+int build_lbvh(mmhrt_ctx_s* c, int bits);
+int refit_lbvh(mmhrt_ctx_s* c);
+int reduce_scene(mmhrt_ctx_s* c);
+
+// trace.hip
+int render_frame(mmhrt_ctx_s* c, const FrameConsts& fc);
+}  // namespace mmhrt

@@ -1,0 +1,898 @@
+/*
+ * oracle.c — CPU oracle for MegaMol's particle-sphere ray-tracing path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker that tests, the smoke check and the benchmark's
+ * CPU-baseline leg use. It is never linked into, loaded by or called from the product.
+ *
+ * What each part restates (paths relative to the MegaMol source tree):
+ *   RNG            plugins/optix_hpg/src/optix/random.h:31-62 (TEA<16>, LCG, rnd)
+ *   frame state    plugins/optix_hpg/src/optix/Renderer.cpp:136-172, core/include/mmcore/view/Camera.h:350-397
+ *   primary ray    plugins/optix_hpg/cuda_resources/raygen.cu:182,241-245; camera.h:11-16
+ *   intersection   plugins/optix_hpg/cuda_resources/sphere.cuh:14-41
+ *   shading        plugins/optix_hpg/cuda_resources/raygen.cu:122-142 (maxBounces == 0)
+ *   pixel resolve  plugins/optix_hpg/cuda_resources/raygen.cu:237-268
+ *   ingest         plugins/optix_hpg/src/optix/SphereGeometry.cpp:94-146 with the Accessor
+ *                  semantics of plugins/geometry_calls/include/geometry_calls/SimpleSphericalParticles.h:85-175
+ *   TF colouring   plugins/datatools/src/AddParticleColors.cpp:19-117
+ * Build-owned parts with no reference counterpart (AO estimator, LBVH) follow the
+ * published methods named at each function.
+ *
+ * Compile with -ffp-contract=off: the FP32 sequences below must not be fused.
+ */
+#define _GNU_SOURCE
+#include "oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdatomic.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#include <unistd.h>
+
+double orc_now(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+/* ------------------------------------------------------------------ RNG */
+/* TEA encryption (Wheeler & Needham 1994) with an arbitrary key and cycle count. */
+void orc_tea_encrypt(uint32_t v[2], const uint32_t k[4], int cycles) {
+    uint32_t v0 = v[0], v1 = v[1], s0 = 0;
+    for (int n = 0; n < cycles; ++n) {
+        s0 += 0x9e3779b9u;
+        v0 += ((v1 << 4) + k[0]) ^ (v1 + s0) ^ ((v1 >> 5) + k[1]);
+        v1 += ((v0 << 4) + k[2]) ^ (v0 + s0) ^ ((v0 >> 5) + k[3]);
+    }
+    v[0] = v0;
+    v[1] = v1;
+}
+/* random.h:31-44: tea<16> is TEA with the key {a341316c, c8013ea4, ad90777d, 7e95761e} and 16
+ * cycles, returning the first word (Zafar, Olano, Curtis 2010 "GPU random numbers via the TEA"). */
+uint32_t orc_tea16(uint32_t val0, uint32_t val1) {
+    static const uint32_t key[4] = {0xa341316cu, 0xc8013ea4u, 0xad90777du, 0x7e95761eu};
+    uint32_t v[2] = {val0, val1};
+    orc_tea_encrypt(v, key, 16);
+    return v[0];
+}
+/* LCG with the Numerical Recipes constants, 24-bit output, random.h:47-52. */
+uint32_t orc_lcg(uint32_t* prev) {
+    *prev = 1664525u * *prev + 1013904223u;
+    return *prev & 0x00FFFFFFu;
+}
+/* random.h:60-62: float in [0,1). */
+float orc_rnd(uint32_t* prev) { return (float)orc_lcg(prev) / (float)0x01000000; }
+
+/* ------------------------------------------------------------ camera */
+static void cross3(const float a[3], const float b[3], float o[3]) {
+    /* glm::cross component order */
+    o[0] = a[1] * b[2] - b[1] * a[2];
+    o[1] = a[2] * b[0] - b[2] * a[0];
+    o[2] = a[0] * b[1] - b[0] * a[1];
+}
+static float dot3(const float a[3], const float b[3]) {
+    float x = a[0] * b[0], y = a[1] * b[1], z = a[2] * b[2];
+    return (x + y) + z;
+}
+static void normalize3(float v[3]) {
+    /* glm::normalize: v * inversesqrt(dot(v,v)), inversesqrt = 1 / sqrt */
+    float inv = 1.0f / sqrtf(dot3(v, v));
+    v[0] = v[0] * inv;
+    v[1] = v[1] * inv;
+    v[2] = v[2] * inv;
+}
+
+void orc_frame_state_from_camera(const orc_camera* cam, orc_frame_state* fs) {
+    /* Renderer.cpp:163-171 */
+    for (int i = 0; i < 3; ++i) {
+        fs->center[i] = cam->position[i];
+        fs->front[i] = cam->direction[i];
+        fs->up[i] = cam->up[i];
+    }
+    cross3(cam->direction, cam->up, fs->right);
+    /* Renderer.cpp:148-157: near clip hard-coded to 100 */
+    float hfov = 0.5f * cam->fovy;
+    fs->th = tanf(hfov) * 100.0f;
+    fs->rw = fs->th * cam->aspect;
+    fs->nearp = 100.0f;
+    /* Renderer.cpp:138-146 + glm::frustum entries used by Camera.h:392-397 */
+    float n = cam->near_plane, f = cam->far_plane;
+    fs->depth_a = -(f + n) / (f - n);
+    fs->depth_b = -1.0f;
+    fs->depth_c = -(2.0f * f * n) / (f - n);
+}
+
+/* raygen.cu:241-245 and camera.h:11-16; rx, ry are the two rnd() draws of the sample. */
+void orc_primary_ray(const orc_frame_state* fs, uint32_t w, uint32_t h, uint32_t x, uint32_t y, float rx, float ry,
+                     float o[3], float d[3]) {
+    float u = -fs->rw + ((fs->rw + fs->rw) * ((float)x + rx)) / (float)w;
+    float v = -(fs->th + ((-fs->th - fs->th) * ((float)y + ry)) / (float)h);
+    for (int i = 0; i < 3; ++i) {
+        o[i] = fs->center[i];
+        d[i] = (fs->nearp * fs->front[i] + u * fs->right[i]) + v * fs->up[i];
+    }
+    normalize3(d);
+}
+
+/* ------------------------------------------------------ intersection */
+/* sphere.cuh:14-41 (Ray Tracing Gems ch. 7). Returns 1 and *t_out on a hit with tmin < t < tmax. */
+int orc_intersect_f32(const float o[3], const float d[3], float tmin, float tmax, const float sph[4], float* t_out) {
+    float oc[3] = {o[0] - sph[0], o[1] - sph[1], o[2] - sph[2]};
+    float sq = sph[3] * sph[3];
+    float noc[3] = {-oc[0], -oc[1], -oc[2]};
+    float b = dot3(noc, d);
+    float tmp[3] = {oc[0] + b * d[0], oc[1] + b * d[1], oc[2] + b * d[2]};
+    float delta = sq - dot3(tmp, tmp);
+    if (delta < 0.0f) return 0;
+    float c = dot3(oc, oc) - sq;
+    float q = b + copysignf(sqrtf(delta), b);
+    float ta = c / q, tb = q;
+    float t = fminf(ta, tb);
+    if (t < 0.0f) return 0;
+    if (t > tmin && t < tmax) {
+        *t_out = t;
+        return 1;
+    }
+    return 0;
+}
+
+/* FP64 ray-sphere with the same acceptance rules (nearest root < 0 -> miss). disc_out receives
+ * the discriminant normalised by r^2 (0 = tangent). */
+int orc_intersect_f64(const double o[3], const double d[3], double tmin, const double sph[4], double* t_out,
+                      double* disc_out) {
+    double oc[3] = {o[0] - sph[0], o[1] - sph[1], o[2] - sph[2]};
+    double a = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    double h = -(oc[0] * d[0] + oc[1] * d[1] + oc[2] * d[2]);
+    double p[3] = {oc[0] + (h / a) * d[0], oc[1] + (h / a) * d[1], oc[2] + (h / a) * d[2]};
+    double r2 = sph[3] * sph[3];
+    double disc = r2 - (p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+    if (disc_out) *disc_out = disc / r2;
+    if (disc < 0.0) return 0;
+    double cc = (oc[0] * oc[0] + oc[1] * oc[1] + oc[2] * oc[2]) - r2;
+    double q = h + copysign(sqrt(a * disc), h);
+    double t0 = cc / q, t1 = q / a;
+    double t = t0 < t1 ? t0 : t1;
+    if (!(t >= 0.0)) return 0;
+    if (t > tmin) {
+        *t_out = t;
+        return 1;
+    }
+    return 0;
+}
+
+/* ----------------------------------------------------------- ingest */
+static const uint32_t VSIZE[5] = {0, 12, 16, 6, 24};      /* SimpleSphericalParticles.cpp:7 */
+static const uint32_t CSIZE[8] = {0, 3, 4, 12, 16, 4, 8, 8}; /* SimpleSphericalParticles.cpp:9 */
+
+/* AddParticleColors.cpp:19-36 */
+static float tf_lerp(float a, float b, float inter) { return a * (1.0f - inter) + b * inter; }
+static void tf_sample_base(const float* tf, uint32_t n, int base, float rest, float out[4]) {
+    if (base < 0 || n == 0) {
+        out[0] = out[1] = out[2] = out[3] = 0.0f;
+        return;
+    }
+    uint32_t last = n - 1;
+    if ((uint32_t)base >= last) {
+        for (int c = 0; c < 4; ++c) out[c] = tf[last * 4 + c];
+        return;
+    }
+    for (int c = 0; c < 4; ++c) out[c] = tf_lerp(tf[base * 4 + c], tf[(base + 1) * 4 + c], rest);
+}
+/* AddParticleColors.cpp:72-94: fac = 1/(max-min+1e-8), val = (i-min)*fac*size, modf split. */
+void orc_tf_sample(const float* tf, uint32_t tf_n, float vmin, float vmax, float value, float out[4]) {
+    float fac = 1.0f / (vmax - vmin + 1e-8f);
+    float val = (value - vmin) * fac * (float)tf_n;
+    float main_part = truncf(val);
+    float rest = val - main_part; /* modf: exact for finite val */
+    /* the reference's static_cast<int> is undefined beyond int range and for NaN: values at or
+     * past the table end take the last entry, NaN and values below -1 take zero */
+    if (main_part > (float)tf_n) main_part = (float)tf_n;
+    if (!(main_part >= -1.0f)) main_part = -1.0f;
+    tf_sample_base(tf, tf_n, (int)main_part, rest, out);
+}
+
+static double rd_f64(const uint8_t* p) { double v; memcpy(&v, p, 8); return v; }
+static float rd_f32(const uint8_t* p) { float v; memcpy(&v, p, 4); return v; }
+static uint16_t rd_u16(const uint8_t* p) { uint16_t v; memcpy(&v, p, 2); return v; }
+
+/* SphereGeometry.cpp:94-146 with Accessor Get_f = static_cast<float> (Accessor.h:78-90):
+ * integer colours are NOT normalised; global colours are /255 (SphereGeometry.cpp:179-181). */
+int orc_convert_lists(const orc_list* lists, uint32_t n_lists, const float* tf, uint32_t tf_n, float tf_min,
+                      float tf_max, int tf_auto_range, float* xyzr_out, float* rgba_out, uint64_t* n_out) {
+    float vmin = tf_min, vmax = tf_max;
+    if (tf_auto_range) { /* AddParticleColors.cpp:59-64 */
+        vmin = FLT_MAX;
+        vmax = -FLT_MAX;
+        for (uint32_t l = 0; l < n_lists; ++l) {
+            vmin = fminf(vmin, lists[l].col_min);
+            vmax = fmaxf(vmax, lists[l].col_max);
+        }
+    }
+    uint64_t k = 0;
+    for (uint32_t l = 0; l < n_lists; ++l) {
+        const orc_list* L = &lists[l];
+        if (L->count == 0 || L->vert_type <= 0 || L->vert_type > 4) continue;
+        uint64_t vs = L->vert_stride ? L->vert_stride : VSIZE[L->vert_type];
+        uint64_t cs = L->col_stride ? L->col_stride : (L->col_type >= 0 && L->col_type < 8 ? CSIZE[L->col_type] : 0);
+        int is_int = (L->col_type == 5 || L->col_type == 7);
+        int has_color = L->col_type != 0 && !is_int;
+        int use_tf = is_int && tf && tf_n > 0;
+        for (uint64_t i = 0; i < L->count; ++i, ++k) {
+            const uint8_t* v = (const uint8_t*)L->vert + i * vs;
+            float* o = xyzr_out + 4 * k;
+            switch (L->vert_type) {
+            case 1: o[0] = rd_f32(v); o[1] = rd_f32(v + 4); o[2] = rd_f32(v + 8); o[3] = L->global_radius; break;
+            case 2: o[0] = rd_f32(v); o[1] = rd_f32(v + 4); o[2] = rd_f32(v + 8); o[3] = rd_f32(v + 12); break;
+            case 3: o[0] = (float)rd_u16(v); o[1] = (float)rd_u16(v + 2); o[2] = (float)rd_u16(v + 4); o[3] = L->global_radius; break;
+            case 4: o[0] = (float)rd_f64(v); o[1] = (float)rd_f64(v + 8); o[2] = (float)rd_f64(v + 16); o[3] = L->global_radius; break;
+            }
+            float* c = rgba_out + 4 * k;
+            const uint8_t* cp = (const uint8_t*)L->col + i * cs;
+            if (use_tf) {
+                float val = L->col_type == 5 ? rd_f32(cp) : (float)rd_f64(cp);
+                orc_tf_sample(tf, tf_n, vmin, vmax, val, c);
+            } else if (!has_color) {
+                for (int q = 0; q < 4; ++q) c[q] = (float)L->global_rgba[q] / 255.0f;
+            } else {
+                switch (L->col_type) {
+                case 1: c[0] = cp[0]; c[1] = cp[1]; c[2] = cp[2]; c[3] = 255.0f; break;
+                case 2: c[0] = cp[0]; c[1] = cp[1]; c[2] = cp[2]; c[3] = cp[3]; break;
+                case 3: c[0] = rd_f32(cp); c[1] = rd_f32(cp + 4); c[2] = rd_f32(cp + 8); c[3] = 1.0f; break;
+                case 4: c[0] = rd_f32(cp); c[1] = rd_f32(cp + 4); c[2] = rd_f32(cp + 8); c[3] = rd_f32(cp + 12); break;
+                case 6: c[0] = rd_u16(cp); c[1] = rd_u16(cp + 2); c[2] = rd_u16(cp + 4); c[3] = rd_u16(cp + 6); break;
+                }
+            }
+        }
+    }
+    *n_out = k;
+    return 0;
+}
+
+/* ------------------------------------------------------ pixel helpers */
+static uint32_t to_rgba8(const float c[4]) {
+    uint32_t r = 0;
+    for (int i = 0; i < 4; ++i) {
+        float v = fminf(fmaxf(c[i], 0.0f), 1.0f);
+        uint32_t b = (uint32_t)(v * 255.0f + 0.5f);
+        r |= b << (8 * i);
+    }
+    return r;
+}
+static uint16_t f2h(float f) { /* round-to-nearest-even float -> IEEE half */
+    uint32_t x;
+    memcpy(&x, &f, 4);
+    uint32_t sign = (x >> 16) & 0x8000u;
+    uint32_t mant = x & 0x7fffffu;
+    int32_t exp = (int32_t)((x >> 23) & 0xff);
+    if (exp == 0xff) return (uint16_t)(sign | 0x7c00u | (mant ? 0x200u : 0));
+    int32_t e = exp - 127 + 15;
+    if (e >= 31) return (uint16_t)(sign | 0x7c00u);
+    if (e <= 0) {
+        if (e < -10) return (uint16_t)sign;
+        mant |= 0x800000u;
+        uint32_t shift = (uint32_t)(14 - e);
+        uint32_t h = mant >> shift;
+        uint32_t rem = mant & ((1u << shift) - 1);
+        uint32_t half = 1u << (shift - 1);
+        if (rem > half || (rem == half && (h & 1))) h++;
+        return (uint16_t)(sign | h);
+    }
+    uint32_t h = ((uint32_t)e << 10) | (mant >> 13);
+    uint32_t rem = mant & 0x1fffu;
+    if (rem > 0x1000u || (rem == 0x1000u && (h & 1))) h++;
+    return (uint16_t)(sign | h);
+}
+
+/* Scene access abstraction: brute force or LBVH. */
+typedef struct scene_view {
+    const float* xyzr; /* brute force: original order */
+    uint64_t n;
+    const orc_bvh* bvh; /* NULL = brute force */
+    float pad;          /* slab padding for the LBVH traversal */
+    uint64_t node_visits, sphere_tests;
+} scene_view;
+
+static int closest_brute(scene_view* s, const float o[3], const float d[3], float tmin, float tmax, float* t_out,
+                         int32_t* id_out) {
+    float best = tmax;
+    int32_t bid = -1;
+    for (uint64_t i = 0; i < s->n; ++i) {
+        float t;
+        if (orc_intersect_f32(o, d, tmin, tmax, s->xyzr + 4 * i, &t)) {
+            if (t < best || (t == best && (int32_t)i < bid)) {
+                best = t;
+                bid = (int32_t)i;
+            }
+        }
+    }
+    s->sphere_tests += s->n;
+    *t_out = best;
+    *id_out = bid;
+    return bid >= 0;
+}
+static int any_brute(scene_view* s, const float o[3], const float d[3], float tmin, float tmax, int32_t skip) {
+    for (uint64_t i = 0; i < s->n; ++i) {
+        float t;
+        if ((int32_t)i == skip) continue;
+        s->sphere_tests++;
+        if (orc_intersect_f32(o, d, tmin, tmax, s->xyzr + 4 * i, &t)) return 1;
+    }
+    return 0;
+}
+
+/* Robust slab test (padding per Ize 2013, "Robust BVH ray traversal"): the box is widened by
+ * pad on every side, which covers the rounding of the FP32 sphere test and of this test. */
+static int slab(const float* box, const float o[3], const float inv[3], float pad, float tmin, float tmax, float* tn) {
+    float t0 = -FLT_MAX, t1 = FLT_MAX;
+    for (int a = 0; a < 3; ++a) {
+        float lo = (box[a] - pad - o[a]) * inv[a];
+        float hi = (box[3 + a] + pad - o[a]) * inv[a];
+        float n = fminf(lo, hi), f = fmaxf(lo, hi);
+        t0 = fmaxf(t0, n);
+        t1 = fminf(t1, f);
+    }
+    *tn = t0;
+    return t0 <= t1 && t1 >= tmin && t0 <= tmax;
+}
+
+static void safe_inv(const float d[3], float inv[3]) {
+    for (int a = 0; a < 3; ++a) {
+        float v = d[a];
+        if (fabsf(v) < 1e-30f) v = copysignf(1e-30f, v);
+        inv[a] = 1.0f / v;
+    }
+}
+
+static int closest_bvh(scene_view* s, const float o[3], const float d[3], float tmin, float tmax, float* t_out,
+                       int32_t* id_out) {
+    const orc_bvh* b = s->bvh;
+    float best = tmax;
+    int32_t bid = -1;
+    if (b->n == 1) {
+        float t;
+        s->sphere_tests++;
+        if (orc_intersect_f32(o, d, tmin, tmax, b->leaf_sph, &t)) {
+            best = t;
+            bid = (int32_t)b->leaf_id[0];
+        }
+        *t_out = best;
+        *id_out = bid;
+        return bid >= 0;
+    }
+    float inv[3];
+    safe_inv(d, inv);
+    int32_t stack[256];
+    int sp = 0;
+    int32_t node = 0;
+    for (;;) {
+        s->node_visits++;
+        const float* bx = b->node_box + 12 * (size_t)node;
+        const int32_t* ch = b->node_child + 2 * (size_t)node;
+        float tn[2];
+        int hit[2];
+        for (int c = 0; c < 2; ++c) hit[c] = slab(bx + 6 * c, o, inv, s->pad, tmin, best, &tn[c]);
+        int32_t next[2];
+        int nn = 0;
+        int order0 = (hit[0] && hit[1] && tn[1] < tn[0]) ? 1 : 0;
+        for (int k = 0; k < 2; ++k) {
+            int c = k ^ order0;
+            if (!hit[c]) continue;
+            if (tn[c] > best) continue;
+            int32_t r = ch[c];
+            if (r < 0) {
+                uint32_t leaf = (uint32_t)(~r);
+                float t;
+                s->sphere_tests++;
+                if (orc_intersect_f32(o, d, tmin, tmax, b->leaf_sph + 4 * (size_t)leaf, &t)) {
+                    int32_t id = (int32_t)b->leaf_id[leaf];
+                    if (t < best || (t == best && id < bid)) {
+                        best = t;
+                        bid = id;
+                    }
+                }
+            } else {
+                next[nn++] = r;
+            }
+        }
+        if (nn == 2) {
+            stack[sp++] = next[1];
+            node = next[0];
+        } else if (nn == 1) {
+            node = next[0];
+        } else {
+            /* pop, skipping entries that can no longer hold a closer hit is not needed for
+             * correctness; the slab test at the child rejects them. */
+            if (sp == 0) break;
+            node = stack[--sp];
+        }
+    }
+    *t_out = best;
+    *id_out = bid;
+    return bid >= 0;
+}
+
+static int any_bvh(scene_view* s, const float o[3], const float d[3], float tmin, float tmax, int32_t skip) {
+    const orc_bvh* b = s->bvh;
+    if (b->n == 1) {
+        float t;
+        if ((int32_t)b->leaf_id[0] == skip) return 0;
+        s->sphere_tests++;
+        return orc_intersect_f32(o, d, tmin, tmax, b->leaf_sph, &t);
+    }
+    float inv[3];
+    safe_inv(d, inv);
+    int32_t stack[256];
+    int sp = 0;
+    int32_t node = 0;
+    for (;;) {
+        s->node_visits++;
+        const float* bx = b->node_box + 12 * (size_t)node;
+        const int32_t* ch = b->node_child + 2 * (size_t)node;
+        int32_t next[2];
+        int nn = 0;
+        for (int c = 0; c < 2; ++c) {
+            float tn;
+            if (!slab(bx + 6 * c, o, inv, s->pad, tmin, tmax, &tn)) continue;
+            int32_t r = ch[c];
+            if (r < 0) {
+                uint32_t leaf = (uint32_t)(~r);
+                if ((int32_t)b->leaf_id[leaf] == skip) continue;
+                float t;
+                s->sphere_tests++;
+                if (orc_intersect_f32(o, d, tmin, tmax, b->leaf_sph + 4 * (size_t)leaf, &t)) return 1;
+            } else {
+                next[nn++] = r;
+            }
+        }
+        if (nn == 2) {
+            stack[sp++] = next[1];
+            node = next[0];
+        } else if (nn == 1) {
+            node = next[0];
+        } else {
+            if (sp == 0) return 0;
+            node = stack[--sp];
+        }
+    }
+}
+
+/* closest hit dispatcher; the tie rule "equal t -> lowest particle id" is applied by both paths */
+static int closest(scene_view* s, const float o[3], const float d[3], float tmin, float tmax, float* t, int32_t* id) {
+    return s->bvh ? closest_bvh(s, o, d, tmin, tmax, t, id) : closest_brute(s, o, d, tmin, tmax, t, id);
+}
+static int anyhit(scene_view* s, const float o[3], const float d[3], float tmin, float tmax, int32_t skip) {
+    return s->bvh ? any_bvh(s, o, d, tmin, tmax, skip) : any_brute(s, o, d, tmin, tmax, skip);
+}
+
+static const float* sphere_of(const scene_view* s, int32_t id, const float* xyzr_orig) {
+    (void)s;
+    return xyzr_orig + 4 * (size_t)id;
+}
+
+/* The AO sample direction: a point uniform on the unit sphere by Marsaglia (1972, "Choosing a
+ * point from the surface of a sphere"), added to the unit normal, gives a cosine-weighted
+ * direction about the normal. Only +,*,/,sqrt are used so that CPU and GPU agree bit for bit. */
+static void ao_direction(const float n[3], uint32_t* seed, float dir[3]) {
+    float x1, x2, s2;
+    do {
+        x1 = 2.0f * orc_rnd(seed) - 1.0f;
+        x2 = 2.0f * orc_rnd(seed) - 1.0f;
+        s2 = x1 * x1 + x2 * x2;
+    } while (s2 >= 1.0f || s2 == 0.0f);
+    float h = 2.0f * sqrtf(1.0f - s2);
+    dir[0] = n[0] + x1 * h;
+    dir[1] = n[1] + x2 * h;
+    dir[2] = n[2] + (1.0f - 2.0f * s2);
+    float l2 = dot3(dir, dir);
+    if (l2 < 1e-12f) {
+        dir[0] = n[0];
+        dir[1] = n[1];
+        dir[2] = n[2];
+    } else {
+        float inv = 1.0f / sqrtf(l2);
+        dir[0] = dir[0] * inv;
+        dir[1] = dir[1] * inv;
+        dir[2] = dir[2] * inv;
+    }
+}
+
+/* exported for the statistical check of the AO sampler */
+void orc_ao_direction(const float n[3], uint32_t seed, float dir[3]) { ao_direction(n, &seed, dir); }
+
+typedef struct pixel_out {
+    float col[4];
+    float depth, t;
+    int32_t id;
+    float aux[8];
+} pixel_out;
+
+/* One pixel: raygen.cu:182-268 for mode 0; the AO mode keeps the same primary-ray stream and
+ * adds one AO ray per sample from its own TEA-seeded stream. */
+static void render_pixel(scene_view* s, const float* xyzr, const float* rgba, const orc_frame_state* fs,
+                         const orc_params* p, uint32_t x, uint32_t y, pixel_out* po) {
+    uint32_t pix = y * p->width + x;
+    uint32_t seed = orc_tea16(pix, p->frame_idx);
+    float col[4] = {0, 0, 0, 0};
+    float depth = FLT_MAX;
+    float best_t = INFINITY;
+    int32_t best_id = -1;
+    float aux[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    float vis_sum = 0.0f;
+    for (uint32_t smp = 0; smp < p->spp; ++smp) {
+        float rx = orc_rnd(&seed);
+        float ry = orc_rnd(&seed);
+        float o[3], d[3];
+        orc_primary_ray(fs, p->width, p->height, x, y, rx, ry, o, d);
+        float t;
+        int32_t id;
+        float sc[4];
+        float ray_depth = FLT_MAX;
+        if (!closest(s, o, d, 1e-6f, 1e20f, &t, &id)) {
+            if (p->mode == 1) {
+                sc[0] = p->background[0]; sc[1] = p->background[1]; sc[2] = p->background[2]; sc[3] = p->background[3];
+            } else {
+                sc[0] = sc[1] = sc[2] = 1.0f * 0.8f; /* raygen.cu:131-133 */
+                sc[3] = 1.0f;
+            }
+        } else {
+            ray_depth = t; /* set_depth, utils_device.h:217-226 */
+            const float* sp = sphere_of(s, id, xyzr);
+            float P[3], N[3];
+            for (int a = 0; a < 3; ++a) {
+                P[a] = o[a] + t * d[a];
+                N[a] = P[a] - sp[a];
+            }
+            if (dot3(N, d) > 0.0f) {
+                N[0] = -N[0]; N[1] = -N[1]; N[2] = -N[2];
+            }
+            normalize3(N);
+            const float* alb = rgba + 4 * (size_t)id;
+            if (p->mode == 1) {
+                uint32_t aseed = orc_tea16(pix, 0xA0000000u + p->frame_idx * 4096u + smp);
+                float ad[3];
+                ao_direction(N, &aseed, ad);
+                int occ = anyhit(s, P, ad, 0.0f, p->ao_distance, id);
+                float V = occ ? 0.0f : 1.0f;
+                sc[0] = alb[0] * V; sc[1] = alb[1] * V; sc[2] = alb[2] * V; sc[3] = 1.0f;
+                vis_sum += V;
+            } else {
+                float sh = 0.2f + 0.6f * fabsf(dot3(N, d)); /* raygen.cu:141 */
+                sc[0] = alb[0] * sh; sc[1] = alb[1] * sh; sc[2] = alb[2] * sh; sc[3] = 1.0f;
+            }
+            if (t < best_t) {
+                best_t = t;
+                best_id = id;
+            }
+            if (smp == 0) {
+                aux[0] = alb[0]; aux[1] = alb[1]; aux[2] = alb[2];
+                aux[3] = N[0]; aux[4] = N[1]; aux[5] = N[2];
+                aux[6] = t;
+            }
+        }
+        for (int c = 0; c < 4; ++c) col[c] += sc[c];
+        depth = fminf(depth, ray_depth);
+    }
+    for (int c = 0; c < 4; ++c) col[c] /= (float)p->spp;
+    if (depth < FLT_MAX) {
+        depth = (fs->depth_c / depth) - fs->depth_a;
+        depth = 0.5f * (depth + 1.0f);
+    } else {
+        depth = 1.0f;
+        for (int c = 0; c < 4; ++c) col[c] = p->background[c];
+    }
+    aux[7] = vis_sum / (float)p->spp;
+    memcpy(po->col, col, sizeof col);
+    po->depth = depth;
+    po->t = best_t;
+    po->id = best_id;
+    memcpy(po->aux, aux, sizeof aux);
+}
+
+static void store_pixel(const pixel_out* po, size_t i, uint32_t* rgba8, float* depth, float* color, float* t_out,
+                        int32_t* id_out, uint16_t* aux) {
+    if (rgba8) rgba8[i] = to_rgba8(po->col);
+    if (depth) depth[i] = po->depth;
+    if (color) memcpy(color + 4 * i, po->col, 16);
+    if (t_out) t_out[i] = po->t;
+    if (id_out) id_out[i] = po->id;
+    if (aux)
+        for (int c = 0; c < 8; ++c) aux[8 * i + c] = f2h(po->aux[c]);
+}
+
+int orc_render_brute(const float* xyzr, const float* rgba, uint64_t n, const orc_camera* cam, const orc_params* p,
+                     uint32_t* rgba8, float* depth, float* color, float* t_out, int32_t* id_out, uint16_t* aux) {
+    orc_frame_state fs;
+    orc_frame_state_from_camera(cam, &fs);
+    scene_view s = {xyzr, n, NULL, 0.0f, 0, 0};
+    for (uint32_t y = 0; y < p->height; ++y)
+        for (uint32_t x = 0; x < p->width; ++x) {
+            pixel_out po;
+            render_pixel(&s, xyzr, rgba, &fs, p, x, y, &po);
+            store_pixel(&po, (size_t)y * p->width + x, rgba8, depth, color, t_out, id_out, aux);
+        }
+    return 0;
+}
+
+/* FP64 primary hit per pixel (first sample's jittered ray, generated in FP32 as the reference
+ * does): nearest id, its t, the normalised discriminant and the second-nearest t (ties). */
+int orc_primary_f64(const float* xyzr, uint64_t n, const orc_camera* cam, const orc_params* p, uint32_t x0,
+                    uint32_t y0, uint32_t x1, uint32_t y1, int32_t* id_out, double* t_out, double* disc_out,
+                    double* t2_out) {
+    orc_frame_state fs;
+    orc_frame_state_from_camera(cam, &fs);
+    size_t k = 0;
+    for (uint32_t y = y0; y < y1; ++y)
+        for (uint32_t x = x0; x < x1; ++x, ++k) {
+            uint32_t seed = orc_tea16(y * p->width + x, p->frame_idx);
+            float rx = orc_rnd(&seed), ry = orc_rnd(&seed);
+            float of[3], df[3];
+            orc_primary_ray(&fs, p->width, p->height, x, y, rx, ry, of, df);
+            double o[3] = {of[0], of[1], of[2]}, d[3] = {df[0], df[1], df[2]};
+            double best = INFINITY, second = INFINITY, bdisc = 0.0;
+            int32_t bid = -1;
+            for (uint64_t i = 0; i < n; ++i) {
+                double sp[4] = {xyzr[4 * i], xyzr[4 * i + 1], xyzr[4 * i + 2], xyzr[4 * i + 3]};
+                double t, disc;
+                if (orc_intersect_f64(o, d, 1e-6, sp, &t, &disc) && t < 1e20) {
+                    if (t < best) {
+                        second = best;
+                        best = t;
+                        bid = (int32_t)i;
+                        bdisc = disc;
+                    } else if (t < second) {
+                        second = t;
+                    }
+                }
+            }
+            id_out[k] = bid;
+            t_out[k] = best;
+            disc_out[k] = bdisc;
+            t2_out[k] = second;
+        }
+    return 0;
+}
+
+/* --------------------------------------------------------------- LBVH */
+/* Morton codes (Morton 1966) over the centre bounds with one uniform scale, then a stable sort
+ * by key, the Karras (2012, "Maximizing parallelism in the construction of BVHs, octrees and
+ * k-d trees") hierarchy with the index tie-break for equal keys, and bottom-up boxes. */
+static uint32_t expand10(uint32_t v) {
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+static uint64_t expand21(uint64_t v) {
+    v &= 0x1fffffull;
+    v = (v | (v << 32)) & 0x1f00000000ffffull;
+    v = (v | (v << 16)) & 0x1f0000ff0000ffull;
+    v = (v | (v << 8)) & 0x100f00f00f00f00full;
+    v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+    v = (v | (v << 2)) & 0x1249249249249249ull;
+    return v;
+}
+
+static int delta_k(const uint64_t* keys, uint64_t n, int64_t i, int64_t j) {
+    if (j < 0 || j >= (int64_t)n) return -1;
+    uint64_t a = keys[i], b = keys[j];
+    if (a == b) return 64 + __builtin_clz((uint32_t)i ^ (uint32_t)j);
+    return __builtin_clzll(a ^ b);
+}
+
+int orc_bvh_build(const float* xyzr, uint64_t n, int bits, orc_bvh* out) {
+    memset(out, 0, sizeof *out);
+    if (n == 0 || n > 0x7fffffffull) return -1;
+    out->n = n;
+    float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (uint64_t i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = fminf(lo[a], xyzr[4 * i + a]);
+            hi[a] = fmaxf(hi[a], xyzr[4 * i + a]);
+        }
+    float ext = fmaxf(fmaxf(hi[0] - lo[0], hi[1] - lo[1]), hi[2] - lo[2]);
+    float cells = bits == 63 ? 2097152.0f : 1024.0f;
+    uint32_t maxq = bits == 63 ? 2097151u : 1023u;
+    float scale = ext > 0.0f ? cells / ext : 0.0f;
+    uint64_t* key = malloc(n * 8), *key2 = malloc(n * 8);
+    uint32_t* idx = malloc(n * 4), *idx2 = malloc(n * 4);
+    for (uint64_t i = 0; i < n; ++i) {
+        uint32_t q[3];
+        for (int a = 0; a < 3; ++a) {
+            float f = (xyzr[4 * i + a] - lo[a]) * scale;
+            uint32_t u = (uint32_t)f;
+            q[a] = u > maxq ? maxq : u;
+        }
+        key[i] = bits == 63 ? ((expand21(q[0]) << 2) | (expand21(q[1]) << 1) | expand21(q[2]))
+                            : (uint64_t)((expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]));
+        idx[i] = (uint32_t)i;
+    }
+    /* stable LSD radix sort, 8-bit digits */
+    int passes = bits == 63 ? 8 : 4;
+    for (int ps = 0; ps < passes; ++ps) {
+        size_t cnt[257] = {0};
+        int sh = 8 * ps;
+        for (uint64_t i = 0; i < n; ++i) cnt[((key[i] >> sh) & 0xff) + 1]++;
+        for (int b = 0; b < 256; ++b) cnt[b + 1] += cnt[b];
+        for (uint64_t i = 0; i < n; ++i) {
+            size_t dst = cnt[(key[i] >> sh) & 0xff]++;
+            key2[dst] = key[i];
+            idx2[dst] = idx[i];
+        }
+        uint64_t* tk = key; key = key2; key2 = tk;
+        uint32_t* ti = idx; idx = idx2; idx2 = ti;
+    }
+    free(key2);
+    free(idx2);
+    out->keys = key;
+    out->leaf_id = idx;
+    out->leaf_sph = malloc(n * 16);
+    for (uint64_t k = 0; k < n; ++k) memcpy(out->leaf_sph + 4 * k, xyzr + 4 * (size_t)idx[k], 16);
+    if (n == 1) return 0;
+    uint64_t m = n - 1;
+    out->node_box = malloc(m * 12 * sizeof(float));
+    out->node_child = malloc(m * 2 * sizeof(int32_t));
+    int32_t* parent = malloc((2 * n - 1) * sizeof(int32_t)); /* [0,m) internal, [m, m+n) leaves */
+    parent[0] = -1;
+    for (int64_t i = 0; i < (int64_t)m; ++i) {
+        int dir = (delta_k(key, n, i, i + 1) - delta_k(key, n, i, i - 1)) >= 0 ? 1 : -1;
+        int dmin = delta_k(key, n, i, i - dir);
+        int64_t lmax = 2;
+        while (delta_k(key, n, i, i + lmax * dir) > dmin) lmax *= 2;
+        int64_t l = 0;
+        for (int64_t t = lmax / 2; t >= 1; t /= 2)
+            if (delta_k(key, n, i, i + (l + t) * dir) > dmin) l += t;
+        int64_t j = i + l * dir;
+        int dnode = delta_k(key, n, i, j);
+        int64_t s = 0, t = l;
+        do {
+            t = (t + 1) >> 1;
+            if (delta_k(key, n, i, i + (s + t) * dir) > dnode) s += t;
+        } while (t > 1);
+        int64_t gamma = i + s * dir + (dir < 0 ? -1 : 0);
+        int64_t first = i < j ? i : j, last = i < j ? j : i;
+        int32_t left = (first == gamma) ? ~(int32_t)gamma : (int32_t)gamma;
+        int32_t right = (last == gamma + 1) ? ~(int32_t)(gamma + 1) : (int32_t)(gamma + 1);
+        out->node_child[2 * i] = left;
+        out->node_child[2 * i + 1] = right;
+        parent[left < 0 ? m + (uint64_t)(~left) : (uint64_t)left] = (int32_t)i;
+        parent[right < 0 ? m + (uint64_t)(~right) : (uint64_t)right] = (int32_t)i;
+    }
+    /* bottom-up boxes: second arrival at a node computes its box */
+    uint8_t* seen = calloc(m, 1);
+    float* nbox = malloc(m * 6 * sizeof(float)); /* box of internal node i */
+    for (uint64_t k = 0; k < n; ++k) {
+        const float* sp = out->leaf_sph + 4 * k;
+        float b[6] = {sp[0] - sp[3], sp[1] - sp[3], sp[2] - sp[3], sp[0] + sp[3], sp[1] + sp[3], sp[2] + sp[3]};
+        int32_t child = ~(int32_t)k;
+        int32_t p = parent[m + k];
+        while (p >= 0) {
+            int slot = out->node_child[2 * p] == child ? 0 : 1;
+            memcpy(out->node_box + 12 * (size_t)p + 6 * slot, b, sizeof b);
+            if (!seen[p]) {
+                seen[p] = 1;
+                break;
+            }
+            const float* c0 = out->node_box + 12 * (size_t)p;
+            for (int a = 0; a < 3; ++a) {
+                b[a] = fminf(c0[a], c0[6 + a]);
+                b[3 + a] = fmaxf(c0[3 + a], c0[9 + a]);
+            }
+            memcpy(nbox + 6 * (size_t)p, b, sizeof b);
+            child = p;
+            p = parent[p];
+        }
+    }
+    free(seen);
+    free(nbox);
+    free(parent);
+    return 0;
+}
+
+void orc_bvh_free(orc_bvh* b) {
+    free(b->node_box);
+    free(b->node_child);
+    free(b->leaf_id);
+    free(b->keys);
+    free(b->leaf_sph);
+    memset(b, 0, sizeof *b);
+}
+
+int orc_bvh_copy_out(const orc_bvh* b, float* node_box, int32_t* node_child, uint32_t* leaf_id, uint64_t* keys) {
+    uint64_t m = b->n > 0 ? b->n - 1 : 0;
+    if (node_box && m) memcpy(node_box, b->node_box, m * 12 * sizeof(float));
+    if (node_child && m) memcpy(node_child, b->node_child, m * 2 * sizeof(int32_t));
+    if (leaf_id) memcpy(leaf_id, b->leaf_id, b->n * 4);
+    if (keys) memcpy(keys, b->keys, b->n * 8);
+    return 0;
+}
+
+/* Multi-threaded LBVH renderer: the "mmospray-equivalent" CPU path. Same camera, RNG, FP32
+ * intersection, shading and AO estimator as above; 16x16 pixel tiles handed out by an atomic
+ * counter to `threads` workers (OSPRay uses TBB tiles, AbstractOSPRayRenderer.cpp:126-132). */
+typedef struct mt_job {
+    const orc_bvh* b;
+    const float* xyzr;
+    const float* rgba;
+    const orc_frame_state* fs;
+    const orc_params* p;
+    float pad;
+    atomic_uint next;
+    uint32_t tiles_x, tiles, row0, row1;
+    uint32_t* rgba8;
+    float* depth;
+    float* color;
+    float* t_out;
+    int32_t* id_out;
+    uint16_t* aux;
+    atomic_ullong visits, tests;
+} mt_job;
+
+static void* mt_worker(void* arg) {
+    mt_job* J = arg;
+    scene_view s = {J->xyzr, J->b->n, J->b, J->pad, 0, 0};
+    for (;;) {
+        uint32_t tile = atomic_fetch_add(&J->next, 1u);
+        if (tile >= J->tiles) break;
+        uint32_t tx = tile % J->tiles_x, ty = tile / J->tiles_x;
+        for (uint32_t y = J->row0 + ty * 16; y < J->row0 + ty * 16 + 16 && y < J->row1; ++y)
+            for (uint32_t x = tx * 16; x < tx * 16 + 16 && x < J->p->width; ++x) {
+                pixel_out po;
+                render_pixel(&s, J->xyzr, J->rgba, J->fs, J->p, x, y, &po);
+                store_pixel(&po, (size_t)y * J->p->width + x, J->rgba8, J->depth, J->color, J->t_out, J->id_out,
+                            J->aux);
+            }
+    }
+    atomic_fetch_add(&J->visits, s.node_visits);
+    atomic_fetch_add(&J->tests, s.sphere_tests);
+    return NULL;
+}
+
+int orc_render_bvh(const orc_bvh* b, const float* rgba, const orc_camera* cam, const orc_params* p, int threads,
+                   uint32_t* rgba8, float* depth, float* color, float* t_out, int32_t* id_out, uint16_t* aux,
+                   uint64_t* counters) {
+    orc_frame_state fs;
+    orc_frame_state_from_camera(cam, &fs);
+    /* particle positions in original order, rebuilt from leaf order for shading lookups */
+    float* xyzr = malloc(b->n * 16);
+    for (uint64_t k = 0; k < b->n; ++k) memcpy(xyzr + 4 * (size_t)b->leaf_id[k], b->leaf_sph + 4 * k, 16);
+    float amax = 0.0f;
+    for (uint64_t k = 0; k < b->n; ++k)
+        for (int a = 0; a < 3; ++a) amax = fmaxf(amax, fabsf(b->leaf_sph[4 * k + a]) + fabsf(b->leaf_sph[4 * k + 3]));
+    float cmax = fmaxf(fmaxf(fabsf(cam->position[0]), fabsf(cam->position[1])), fabsf(cam->position[2]));
+    mt_job J;
+    memset(&J, 0, sizeof J);
+    J.b = b;
+    J.xyzr = xyzr;
+    J.rgba = rgba;
+    J.fs = &fs;
+    J.p = p;
+    J.pad = (fmaxf(cmax, amax) + amax) * (1.0f / 131072.0f);
+    J.row0 = p->row_end > p->row_begin ? p->row_begin : 0;
+    J.row1 = p->row_end > p->row_begin ? p->row_end : p->height;
+    J.tiles_x = (p->width + 15) / 16;
+    J.tiles = J.tiles_x * ((J.row1 - J.row0 + 15) / 16);
+    atomic_init(&J.next, 0u);
+    J.rgba8 = rgba8;
+    J.depth = depth;
+    J.color = color;
+    J.t_out = t_out;
+    J.id_out = id_out;
+    J.aux = aux;
+    atomic_init(&J.visits, 0ull);
+    atomic_init(&J.tests, 0ull);
+    if (threads <= 0) threads = (int)sysconf(_SC_NPROCESSORS_ONLN);
+    if (threads > 256) threads = 256;
+    pthread_t th[256];
+    for (int i = 0; i < threads; ++i) pthread_create(&th[i], NULL, mt_worker, &J);
+    for (int i = 0; i < threads; ++i) pthread_join(th[i], NULL);
+    if (counters) {
+        counters[0] = atomic_load(&J.visits);
+        counters[1] = atomic_load(&J.tests);
+    }
+    free(xyzr);
+    return 0;
+}

@@ -1,0 +1,205 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle, bit for bit.
+
+Hit IDs, t, pseudo depth, float colour, RGBA8 and the fp16 feature buffer must be identical to
+the oracle's FP32 restatement on the same inputs (integer and FP32 sequences are the same, so
+there is no tolerance). The FP64 oracle bounds the FP32 answer itself (see test_oracle_cpu).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import assert_frames_equal, case, oracle_params, small_camera
+from megamol_amd import ParticleList, _lib as L, look_at, mmpld, params, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def render_both(r, xyzr, rgba, cam, p, bits=30, aux=False):
+    r.set_spheres(xyzr, rgba)
+    r.build(L.BUILD_REBUILD, bits)
+    g = r.render(cam, p)
+    if aux:
+        fr = r.download(p.width, p.height, outputs=(), aux=True)
+        g.aux = fr.aux
+    o = O.render_brute(xyzr, rgba, cam, oracle_params(p), aux=aux)
+    return g, o
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+@pytest.mark.parametrize("mode", [L.MODE_LAMBERT, L.MODE_AO])
+@pytest.mark.parametrize("bits", [30, 63])
+def test_small_scene_bit_exact(gpu_renderer, seed, mode, bits):
+    case(f"small scene seed={seed} mode={mode} bits={bits}")
+    xyzr, rgba = scenes.random_spheres(3000, seed=seed)
+    xyzr[200:205] = xyzr[7]  # exact duplicates -> ties
+    cam = small_camera(96, 72)
+    p = params(96, 72, spp=3, mode=mode, ao_distance=0.3, background=(0.1, 0.2, 0.3, 1.0), flags=L.FLAG_AUX)
+    g, o = render_both(gpu_renderer, xyzr, rgba, cam, p, bits, aux=True)
+    assert (o["hit_id"] >= 0).mean() > 0.5
+    assert_frames_equal(g, o, aux=True, what=f"seed={seed} mode={mode} bits={bits}")
+
+
+def test_lbvh_equals_oracle_lbvh(gpu_renderer):
+    for bits in (30, 63):
+        for n, dup in [(1000, False), (50000, True), (2, False), (3, True)]:
+            case(f"lbvh n={n} dup={dup} bits={bits}")
+            g = np.random.default_rng(n + bits)
+            xyzr = np.zeros((n, 4), np.float32)
+            xyzr[:, :3] = g.random((n, 3)).astype(np.float32)
+            xyzr[:, 3] = g.uniform(0.001, 0.01, n).astype(np.float32)
+            if dup:
+                xyzr[n // 2:, :3] = xyzr[: n - n // 2, :3]
+            gpu_renderer.set_spheres(xyzr)
+            gpu_renderer.build(L.BUILD_REBUILD, bits)
+            box, child, ids, keys, height = gpu_renderer.bvh()
+            ob = O.Bvh(xyzr, bits)
+            obox, ochild, oids, okeys = ob.arrays()
+            np.testing.assert_array_equal(keys, okeys)
+            np.testing.assert_array_equal(ids, oids)
+            np.testing.assert_array_equal(child, ochild)
+            np.testing.assert_array_equal(box.view(np.uint32), obox.view(np.uint32))
+            assert height[0] >= int(np.ceil(np.log2(n)))
+
+
+def test_edge_cases(gpu_renderer):
+    cam = small_camera(40, 30)
+    p = params(40, 30, spp=2, mode=L.MODE_AO, ao_distance=1.0, flags=L.FLAG_AUX)
+    cases = {
+        "one sphere": np.array([[0.5, 0.5, 0.5, 0.2]], np.float32),
+        "two spheres": np.array([[0.4, 0.5, 0.5, 0.1], [0.6, 0.5, 0.5, 0.1]], np.float32),
+        "all same centre": np.tile(np.array([[0.5, 0.5, 0.5, 0.2]], np.float32), (64, 1)),
+        "camera inside a sphere": np.array([[0.5, 0.5, 2.2, 0.5], [0.5, 0.5, 0.5, 0.2]], np.float32),
+        "behind the camera": np.array([[0.5, 0.5, 4.0, 0.3]], np.float32),
+        "zero radius": np.array([[0.5, 0.5, 0.5, 0.0], [0.3, 0.5, 0.5, 0.1]], np.float32),
+    }
+    for name, xyzr in cases.items():
+        case(f"edge: {name}")
+        rgba = np.linspace(0.1, 1.0, len(xyzr) * 4, dtype=np.float32).reshape(-1, 4)
+        g, o = render_both(gpu_renderer, xyzr, rgba, cam, p, 30, aux=True)
+        assert_frames_equal(g, o, aux=True, what=name)
+        if name == "all same centre":
+            assert set(np.unique(g.hit_id)) <= {-1, 0}
+
+
+def test_mmpld_fixture_lists_through_the_abi(gpu_renderer):
+    import glob
+    from pathlib import Path
+    files = sorted(glob.glob(str(Path(__file__).parent / "golden" / "mmpld" / "*.mmpld")))
+    cam = look_at((0.5, 0.5, 6.0), (0, 0, 0), (0, 1, 0), 60.0, 1.0, 0.1, 20)
+    p = params(48, 48, spp=1, mode=L.MODE_LAMBERT)
+    tf = scenes.transfer_function_256()
+    for with_tf in (False, True):
+        gpu_renderer.set_transfer_function(tf if with_tf else None)
+        for f in files:
+            case(f"mmpld {Path(f).name} tf={with_tf}")
+            hdr, frames = mmpld.read(f)
+            gpu_renderer.set_lists(frames[0].lists)
+            gpu_renderer.build()
+            g = gpu_renderer.render(cam, p)
+            xyzr, rgba = O.convert_lists(frames[0].lists, tf=tf if with_tf else None)
+            o = O.render_brute(xyzr, rgba, cam, oracle_params(p))
+            assert_frames_equal(g, o, what=Path(f).name)
+    gpu_renderer.set_transfer_function(None)
+
+
+def test_transfer_function_intensity_list(gpu_renderer):
+    case("TF intensity list, auto range and explicit range")
+    g = np.random.default_rng(5)
+    n = 5000
+    xyzr = np.zeros((n, 4), np.float32)
+    xyzr[:, :3] = g.random((n, 3)).astype(np.float32)
+    xyzr[:, 3] = 0.03
+    inten = g.uniform(-0.5, 1.5, n).astype(np.float32)
+    tf = scenes.transfer_function_256()
+    cam = small_camera(64, 64)
+    p = params(64, 64)
+    for vmin, vmax in [(1.0, 0.0), (0.0, 1.0), (0.25, 0.3)]:
+        lst = ParticleList(L.VERTDATA_FLOAT_XYZR, xyzr, L.COLDATA_FLOAT_I, inten, col_min=-0.5, col_max=1.5,
+                           count=n)
+        gpu_renderer.set_transfer_function(tf, vmin, vmax)
+        gpu_renderer.set_lists([lst])
+        gpu_renderer.build()
+        gr = gpu_renderer.render(cam, p)
+        oxyzr, orgba = O.convert_lists([lst], tf=tf, tf_min=vmin, tf_max=vmax)
+        o = O.render_brute(oxyzr, orgba, cam, oracle_params(p))
+        assert_frames_equal(gr, o, what=f"tf range {vmin},{vmax}")
+    gpu_renderer.set_transfer_function(None)
+
+
+def test_refit_equals_rebuild(gpu_renderer):
+    xyzr, rgba = scenes.random_spheres(20000, seed=4, rmin=0.005, rmax=0.02)
+    cam = small_camera(80, 80)
+    p = params(80, 80, spp=2, mode=L.MODE_AO, ao_distance=0.2)
+    gpu_renderer.set_spheres(xyzr, rgba)
+    gpu_renderer.build(L.BUILD_REBUILD, 30)
+    g = np.random.default_rng(1)
+    moved = xyzr.copy()
+    for step in range(3):
+        case(f"refit step {step}")
+        moved[:, :3] += g.normal(0, 0.01, (len(moved), 3)).astype(np.float32)
+        gpu_renderer.set_spheres(moved, rgba)
+        gpu_renderer.build(L.BUILD_REFIT, 30)
+        a = gpu_renderer.render(cam, p)
+        o = O.render_brute(moved, rgba, cam, oracle_params(p))
+        assert_frames_equal(a, o, what=f"refit step {step}")
+    gpu_renderer.build(L.BUILD_REBUILD, 30)
+    b = gpu_renderer.render(cam, p)
+    np.testing.assert_array_equal(a.hit_id, b.hit_id)
+    np.testing.assert_array_equal(a.color, b.color)
+
+
+def test_progressive_accumulation(gpu_renderer):
+    case("progressive accumulation")
+    xyzr, rgba = scenes.random_spheres(2000, seed=6)
+    cam = small_camera(48, 48)
+    gpu_renderer.set_spheres(xyzr, rgba)
+    gpu_renderer.build()
+    frames = []
+    for k in range(3):
+        p = params(48, 48, spp=1, mode=L.MODE_AO, ao_distance=0.3, frame_idx=k, accumulate=1)
+        frames.append(gpu_renderer.render(cam, p).color)
+    exp = None
+    for k in range(3):
+        o = O.render_brute(xyzr, rgba, cam, O.make_params(48, 48, 1, 1, 0.3, k))
+        c = o["color"]
+        if exp is None:
+            exp = c
+        else:
+            # raygen.cu:254-260: old + a * (new - old) before the miss override
+            a = np.float32(1.0) / np.float32(k + 1)
+            exp = exp + a * (c - exp)
+            miss = o["hit_id"] < 0
+            exp[miss] = np.array([0, 0, 0, 1], np.float32)
+        np.testing.assert_array_equal(frames[k].view(np.uint32), exp.view(np.uint32))
+
+
+def test_error_paths(gpu_renderer):
+    from megamol_amd import SphereRenderer
+    with SphereRenderer(0) as r:
+        cam = small_camera(8, 8)
+        with pytest.raises(L.MMHRTError) as e:
+            r.render(cam, params(8, 8))
+        assert e.value.code == L.E_STATE
+        with pytest.raises(L.MMHRTError) as e:
+            r.build()
+        assert e.value.code == L.E_STATE
+        r.set_spheres(np.array([[0, 0, 0, 1]], np.float32))
+        with pytest.raises(L.MMHRTError) as e:
+            r.build(L.BUILD_REBUILD, 31)
+        assert e.value.code == L.E_INVALID
+        r.build()
+        with pytest.raises(L.MMHRTError) as e:
+            r.render(cam, params(0, 0))
+        assert e.value.code == L.E_INVALID
+        with pytest.raises(L.MMHRTError) as e:
+            r.render(cam, params(8, 8, spp=0))
+        assert e.value.code == L.E_INVALID
+        with pytest.raises(L.MMHRTError) as e:
+            r.set_lists([ParticleList(9, np.zeros(16, np.uint8), count=1)])
+        assert e.value.code == L.E_INVALID
+        # empty and ragged lists are skipped (SphereGeometry.cpp:98-100)
+        pts = np.array([[0.1, 0.2, 0.3], [0.4, 0.5, 0.6]], np.float32)
+        r.set_lists([ParticleList(L.VERTDATA_FLOAT_XYZ, pts, count=0),
+                     ParticleList(L.VERTDATA_NONE, pts, count=2),
+                     ParticleList(L.VERTDATA_FLOAT_XYZ, pts, count=2, global_radius=0.05)])
+        assert r.info()[0] == 2

@@ -1,0 +1,340 @@
+"""CPU tests: pin the oracle against published vectors, closed forms and independent methods.
+
+These tests never touch the GPU. They make the oracle trustworthy before it judges the HIP path.
+"""
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import small_camera
+from megamol_amd import look_at, mmpld, scenes
+
+GOLD = Path(__file__).resolve().parent / "golden"
+
+
+# ------------------------------------------------------------------ RNG
+def test_lcg_numerical_recipes_vector():
+    # "An even quicker generator", Numerical Recipes in C 2nd ed. sec. 7.1: idum from 0
+    want = [0x3C6EF35F, 0x47502932, 0xD1CCF6E9, 0xAAF95334, 0x6252E503, 0x9F2EC686, 0x57FE6C2D, 0xA3D95FA8,
+            0x81FDBEE7, 0x94F0AF1A, 0xCBF633B1]
+    assert O.lcg_raw(0, len(want)) == want
+    # rnd() = low 24 bits / 2^24 (random.h:60-62)
+    r = O.lcg_stream(0, 4)
+    assert r == [float(np.float32((w & 0xFFFFFF) / 16777216.0)) for w in want[:4]]
+
+
+def test_tea_published_vector():
+    # TEA (Wheeler & Needham 1994), 32 cycles, zero key, zero plaintext -> 41EA3A0A 94BAA940
+    assert O.tea_encrypt(0, 0, [0, 0, 0, 0], 32) == (0x41EA3A0A, 0x94BAA940)
+
+
+def test_tea16_is_tea_with_reference_key():
+    key = [0xA341316C, 0xC8013EA4, 0xAD90777D, 0x7E95761E]  # random.h:39-40
+    for a, b in [(0, 0), (1, 0), (12345, 7), (2 ** 32 - 1, 99)]:
+        assert O.tea16(a, b) == O.tea_encrypt(a, b, key, 16)[0]
+    # decorrelation: neighbouring pixel seeds differ in about half their bits
+    bits = [bin(O.tea16(i, 0) ^ O.tea16(i + 1, 0)).count("1") for i in range(256)]
+    assert 14 < np.mean(bits) < 18
+
+
+# --------------------------------------------------------------- camera
+def test_frame_state_closed_form():
+    cam = look_at((0, 0, 5), (0, 0, 0), (0, 1, 0), fovy_deg=50.0, aspect=1.5, near=0.5, far=20.0)
+    fs = O.frame_state(cam)
+    assert fs.nearp == 100.0
+    assert math.isclose(fs.th, math.tan(math.radians(25.0)) * 100.0, rel_tol=1e-6)
+    assert math.isclose(fs.rw, fs.th * 1.5, rel_tol=1e-6)
+    assert np.allclose(list(fs.right), [1, 0, 0], atol=1e-7)  # cross(dir, up) with dir=-z, up=y
+    n, f = 0.5, 20.0
+    assert math.isclose(fs.depth_a, -(f + n) / (f - n), rel_tol=1e-6)
+    assert math.isclose(fs.depth_c, -2 * f * n / (f - n), rel_tol=1e-6)
+
+
+def test_primary_ray_geometry():
+    W, H = 640, 480
+    cam = look_at((1, 2, 3), (1, 2, 0), (0, 1, 0), fovy_deg=60.0, aspect=W / H)
+    fs = O.frame_state(cam)
+    import ctypes as C
+    lib = O.load()
+    o = np.zeros(3, np.float32)
+    d = np.zeros(3, np.float32)
+    # centre of the image plane: u = v = 0 -> the ray is the view direction
+    lib.orc_primary_ray(C.byref(fs), W, H, W // 2, H // 2, 0.0, 0.0, o.ctypes.data, d.ctypes.data)
+    assert np.allclose(d, [0, 0, -1], atol=1e-6) and np.allclose(o, [1, 2, 3])
+    # bottom-left corner (y = 0 is the bottom row): vertical angle = fovy / 2, looking down
+    lib.orc_primary_ray(C.byref(fs), W, H, 0, 0, 0.0, 0.0, o.ctypes.data, d.ctypes.data)
+    assert d[1] < 0 and d[0] < 0
+    vert = math.degrees(math.atan2(-d[1], -d[2]))
+    horiz = math.degrees(math.atan2(-d[0], -d[2]))
+    assert math.isclose(vert, 30.0, abs_tol=1e-4)
+    assert math.isclose(math.tan(math.radians(horiz)), math.tan(math.radians(30.0)) * W / H, rel_tol=1e-5)
+    assert math.isclose(float(np.linalg.norm(d)), 1.0, abs_tol=1e-6)
+
+
+def test_pseudo_depth_maps_near_to_0_and_far_to_1():
+    # raygen.cu:262-264: 0.5 * ((C / t) - A) + 0.5 is the GL depth of distance t
+    n, f = 2.0, 11.0
+    for t_hit, want in [(n, 0.0), (f, 1.0)]:
+        # a sphere whose front is at distance t_hit along the optical axis
+        xyzr = np.array([[0, 0, -(t_hit + 1.0), 1.0]], np.float32)
+        cam = look_at((0, 0, 0), (0, 0, -1), (0, 1, 0), fovy_deg=1e-3, aspect=1.0, near=n, far=f)
+        o = O.render_brute(xyzr, np.ones((1, 4), np.float32), cam, O.make_params(1, 1))
+        assert o["hit_id"][0, 0] == 0
+        assert math.isclose(float(o["t"][0, 0]), t_hit, rel_tol=1e-6)
+        assert math.isclose(float(o["depth"][0, 0]), want, abs_tol=1e-5)
+
+
+# --------------------------------------------------------- intersection
+@pytest.mark.parametrize("h", [0.0, 0.3, 0.6, 0.9, 0.999])
+def test_intersection_closed_form(h):
+    # ray parallel to -z at lateral offset h from a unit sphere at the origin: t = 5 - sqrt(1 - h^2)
+    t = O.intersect_f32([h, 0, 5], [0, 0, -1], [0, 0, 0, 1])
+    assert t is not None and math.isclose(t, 5 - math.sqrt(1 - h * h), rel_tol=2e-6)
+    t64, disc = O.intersect_f64([h, 0, 5], [0, 0, -1], [0, 0, 0, 1])
+    assert math.isclose(t64, 5 - math.sqrt(1 - h * h), rel_tol=1e-12)
+    assert math.isclose(disc, 1 - h * h, abs_tol=1e-12)
+
+
+def test_intersection_special_cases():
+    assert O.intersect_f32([1, 0, 5], [0, 0, -1], [0, 0, 0, 1]) == 5.0  # tangent: delta == 0 is a hit
+    assert O.intersect_f32([1.0001, 0, 5], [0, 0, -1], [0, 0, 0, 1]) is None  # just outside
+    assert O.intersect_f32([0, 0, 0.5], [0, 0, -1], [0, 0, 0, 1]) is None  # origin inside (sphere.cuh:34-36)
+    assert O.intersect_f32([0, 0, -5], [0, 0, -1], [0, 0, 0, 1]) is None  # sphere behind
+    assert O.intersect_f32([0, 0, 5], [0, 0, -1], [0, 0, 0, 1], tmin=4.0) is None  # t == tmin excluded
+    assert O.intersect_f32([0, 0, 5], [0, 0, -1], [0, 0, 0, 1], tmax=4.0) is None  # t == tmax excluded
+    assert O.intersect_f32([0, 0, 5], [0, 0, -1], [0, 0, 0, 1], tmax=4.0001) == 4.0
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_fp32_matches_fp64_except_grazing_and_ties(seed):
+    xyzr, rgba = scenes.random_spheres(1500, seed=seed, rmin=0.01, rmax=0.06)
+    cam = small_camera(96, 96)
+    p = O.make_params(96, 96)
+    f32 = O.render_brute(xyzr, rgba, cam, p)
+    ids, t64, disc, t2 = O.primary_f64(xyzr, cam, p)
+    mism = ids != f32["hit_id"]
+    for y, x in zip(*np.nonzero(mism)):
+        grazing = abs(disc[y, x]) < 1e-4 or (f32["hit_id"][y, x] < 0 or ids[y, x] < 0)
+        tie = np.isfinite(t2[y, x]) and abs(t2[y, x] - t64[y, x]) <= 1e-5 * t64[y, x]
+        assert grazing or tie, (y, x)
+    assert mism.mean() <= 1e-3
+    hit = (ids >= 0) & ~mism
+    rel = np.abs(f32["t"][hit].astype(np.float64) - t64[hit]) / t64[hit]
+    assert rel.max() <= 1e-5
+
+
+# ----------------------------------------------------------------- LBVH
+def radix_tree_topdown(keys: np.ndarray):
+    """Independent construction: the binary radix tree over sorted (key, index) pairs, built
+    top-down by splitting each range at its highest differing bit (Karras 2012 sec. 3 defines the
+    LBVH as exactly this tree); internal node for range [i, j] is numbered by its split end
+    adjacent to the range boundary, as in the paper (node index = i if it is the left end of its
+    parent's split, else j)."""
+    n = len(keys)
+    ak = [(int(k) << 32) | i for i, k in enumerate(keys)]  # augmented keys, unique
+    child = {}
+
+    def split(first, last):
+        a, b = ak[first], ak[last]
+        common = 96 - (a ^ b).bit_length()
+        s = first
+        step = last - first
+        while True:
+            step = (step + 1) >> 1
+            new = s + step
+            if new < last:
+                pre = 96 - (a ^ ak[new]).bit_length()
+                if pre > common:
+                    s = new
+            if step <= 1:
+                break
+        return s
+
+    def ref(first, last, is_left_of_parent):
+        if first == last:
+            return ~first
+        return first if not is_left_of_parent else last
+
+    stack = [(0, n - 1, 0)]
+    while stack:
+        first, last, idx = stack.pop()
+        g = split(first, last)
+        left = ref(first, g, True)
+        right = ref(g + 1, last, False)
+        child[idx] = (left, right)
+        if first != g:
+            stack.append((first, g, left))
+        if g + 1 != last:
+            stack.append((g + 1, last, right))
+    return np.array([child[i] for i in range(n - 1)], np.int32)
+
+
+@pytest.mark.parametrize("n,dups,bits", [(2, False, 30), (7, False, 30), (300, False, 30), (300, True, 30),
+                                         (1000, True, 63), (513, False, 63)])
+def test_lbvh_topology_matches_radix_tree(n, dups, bits):
+    g = np.random.default_rng(n)
+    xyzr = np.zeros((n, 4), np.float32)
+    xyzr[:, :3] = g.random((n, 3)).astype(np.float32)
+    if dups:  # many identical centres -> equal Morton keys -> index tie-break
+        xyzr[n // 2:, :3] = xyzr[: n - n // 2, :3]
+        xyzr[::7, :3] = xyzr[0, :3]
+    xyzr[:, 3] = 0.01
+    b = O.Bvh(xyzr, bits)
+    box, child, ids, keys = b.arrays()
+    assert np.all(np.diff(keys.astype(np.float64)) >= 0)
+    same = keys[1:] == keys[:-1]
+    assert np.all(ids[1:][same] > ids[:-1][same]), "equal keys keep the original order (stable)"
+    assert sorted(ids.tolist()) == list(range(n))
+    np.testing.assert_array_equal(child, radix_tree_topdown(keys))
+
+
+def test_lbvh_boxes_contain_their_subtrees():
+    xyzr, _ = scenes.random_spheres(2000, seed=9)
+    b = O.Bvh(xyzr, 30)
+    box, child, ids, keys = b.arrays()
+    leaf = xyzr[ids]
+    lo = np.full((len(child), 3), np.inf, np.float32)
+    hi = np.full((len(child), 3), -np.inf, np.float32)
+
+    def sub(r):
+        if r < 0:
+            s = leaf[~r]
+            return s[:3] - s[3], s[:3] + s[3]
+        return lo[r], hi[r]
+
+    order = np.argsort([-1])  # placeholder to keep numpy import used
+    # post-order evaluation
+    post, stack = [], [0]
+    while stack:
+        i = stack.pop()
+        post.append(i)
+        for r in child[i]:
+            if r >= 0:
+                stack.append(r)
+    for i in reversed(post):
+        for c in range(2):
+            l, h = sub(child[i][c])
+            np.testing.assert_array_equal(box[i, 6 * c:6 * c + 3], l)
+            np.testing.assert_array_equal(box[i, 6 * c + 3:6 * c + 6], h)
+        lo[i] = np.minimum(box[i, 0:3], box[i, 6:9])
+        hi[i] = np.maximum(box[i, 3:6], box[i, 9:12])
+    del order
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("bits", [30, 63])
+def test_lbvh_traversal_equals_brute_force(mode, bits):
+    xyzr, rgba = scenes.random_spheres(2500, seed=11)
+    xyzr[100:110] = xyzr[0]  # exact duplicates: ties resolve to the lowest id
+    cam = small_camera(80, 60)
+    p = O.make_params(80, 60, spp=2, mode=mode, ao_distance=0.25, want_aux=1)
+    a = O.render_brute(xyzr, rgba, cam, p, aux=True)
+    b = O.Bvh(xyzr, bits).render(rgba, cam, p, threads=4, aux=True)
+    for k in ("hit_id", "t", "depth", "color", "rgba8"):
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+    np.testing.assert_array_equal(a["aux"].view(np.uint16), b["aux"].view(np.uint16))
+    assert not np.isin(np.arange(101, 110), a["hit_id"]).any(), "duplicates of sphere 0 never win"
+
+
+# ------------------------------------------------------------------- AO
+def test_ao_direction_is_cosine_weighted():
+    n = np.array([0.3, -0.5, 0.8], np.float32)
+    n /= np.linalg.norm(n)
+    cos = np.array([float(O.ao_direction(n, O.tea16(i, 77)) @ n) for i in range(20000)])
+    assert cos.min() >= -1e-6
+    assert abs(cos.mean() - 2 / 3) < 0.01  # E[cos] of a cosine-weighted hemisphere
+    # cos^2 is uniform on [0,1] for a cosine-weighted hemisphere
+    hist, _ = np.histogram(cos ** 2, bins=10, range=(0, 1))
+    assert np.all(np.abs(hist / len(cos) - 0.1) < 0.01)
+
+
+def test_ao_isolated_sphere_is_unoccluded():
+    xyzr = np.array([[0.5, 0.5, 0.5, 0.3]], np.float32)
+    rgba = np.array([[0.2, 0.4, 0.6, 1.0]], np.float32)
+    cam = small_camera(32, 32)
+    for frame in range(4):  # 1 spp: a hit pixel's only sample is the one that hit
+        o = O.render_brute(xyzr, rgba, cam, O.make_params(32, 32, spp=1, mode=1, ao_distance=10.0, frame_idx=frame,
+                                                          want_aux=1), aux=True)
+        hit = o["hit_id"] >= 0
+        assert hit.sum() > 50
+        assert np.all(o["aux"][hit][:, 7] == 1.0)
+        np.testing.assert_array_equal(o["color"][hit][:, :3], np.broadcast_to(rgba[0, :3], (hit.sum(), 3)))
+
+
+def test_ao_enclosed_point_is_occluded():
+    # a small sphere in the middle of a closed shell of large spheres; the camera sits inside
+    # the shell, so every AO ray from the small sphere hits the shell within ao_distance
+    shell = []
+    for d in [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)]:
+        shell.append([3 * d[0], 3 * d[1], 3 * d[2], 2.6])
+    xyzr = np.array([[0, 0, 0, 0.1]] + shell, np.float32)
+    rgba = np.ones((len(xyzr), 4), np.float32)
+    cam = look_at((0, 0, 0.25), (0, 0, 0), (0, 1, 0), 20.0, 1.0, 0.01, 10)
+    o = O.render_brute(xyzr, rgba, cam, O.make_params(16, 16, spp=8, mode=1, ao_distance=100.0, want_aux=1),
+                       aux=True)
+    small = o["hit_id"] == 0
+    assert small.sum() > 10
+    assert np.all(o["aux"][small][:, 7] == 0.0)
+
+
+# ------------------------------------------------------------ ingest / TF
+def test_tf_closed_form_linear_ramp():
+    n = 256
+    tf = np.repeat(np.linspace(0, 1, n, dtype=np.float32)[:, None], 4, 1)
+    out = np.zeros(4, np.float32)
+    import ctypes as C
+    lib = O.load()
+    for v in [0.0, 0.1, 0.37, 0.5, 0.9]:
+        lib.orc_tf_sample(tf.ctypes.data, n, C.c_float(0.0), C.c_float(1.0), C.c_float(v), out.ctypes.data)
+        assert math.isclose(float(out[0]), v * n / (n - 1), abs_tol=2e-6)
+    lib.orc_tf_sample(tf.ctypes.data, n, C.c_float(0.0), C.c_float(1.0), C.c_float(1.0), out.ctypes.data)
+    assert out[0] == 1.0  # at the top of the range: last entry (AddParticleColors.cpp:28-29)
+    lib.orc_tf_sample(tf.ctypes.data, n, C.c_float(0.0), C.c_float(1.0), C.c_float(-1.0), out.ctypes.data)
+    assert np.all(out == 0.0)  # base < 0 -> zero (AddParticleColors.cpp:25-26)
+    lib.orc_tf_sample(tf.ctypes.data, n, C.c_float(0.0), C.c_float(1.0), C.c_float(np.nan), out.ctypes.data)
+    assert np.all(out == 0.0)
+
+
+def test_mmpld_fixtures_ingest():
+    e = np.load(GOLD / "mmpld" / "expected.npz")
+    files = sorted((GOLD / "mmpld").glob("*.mmpld"))
+    assert len(files) == 17
+    for f in files:
+        hdr, frames = mmpld.read(str(f))
+        assert hdr.version == 102 and hdr.frame_count == 1 and hdr.bbox == (-2, -2, -2, 2, 2, 2)
+        assert len(frames[0].lists) == 4
+        xyzr, rgba = O.convert_lists(frames[0].lists)
+        np.testing.assert_array_equal(xyzr, e[f.stem + "_xyzr"], err_msg=f.stem)
+        np.testing.assert_array_equal(rgba, e[f.stem + "_rgba"], err_msg=f.stem)
+
+
+def test_mmpld_roundtrip_versions_and_errors(tmp_path):
+    g = np.random.default_rng(0)
+    xyz = g.random((50, 3))
+    for ver in (100, 102, 103):
+        p = tmp_path / f"v{ver}.mmpld"
+        ls = mmpld.ListSpec(2, 5, xyz, radius=g.random(50), colors=g.random((50, 4)))
+        mmpld.write(str(p), [[ls], [ls]], version=ver)
+        hdr, frames = mmpld.read(str(p))
+        assert hdr.version == ver and len(frames) == 2
+        xyzr, rgba = O.convert_lists(frames[1].lists)
+        np.testing.assert_array_equal(xyzr[:, :3], xyz.astype(np.float32))
+    bad = tmp_path / "bad.mmpld"
+    raw = bytearray(open(tmp_path / "v103.mmpld", "rb").read())
+    raw[0:5] = b"XXXXX"
+    bad.write_bytes(bytes(raw))
+    with pytest.raises(mmpld.MMPLDError, match="id wrong"):
+        mmpld.read(str(bad))
+    raw = bytearray(open(tmp_path / "v103.mmpld", "rb").read())
+    raw[6:8] = (99).to_bytes(2, "little")
+    bad.write_bytes(bytes(raw))
+    with pytest.raises(mmpld.MMPLDError, match="version wrong"):
+        mmpld.read(str(bad))
+    raw = bytearray(open(tmp_path / "v103.mmpld", "rb").read())
+    raw[8:12] = (0).to_bytes(4, "little")
+    bad.write_bytes(bytes(raw))
+    with pytest.raises(mmpld.MMPLDError, match="any frame"):
+        mmpld.read(str(bad))
