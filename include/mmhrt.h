@@ -122,6 +122,9 @@ typedef struct mmhrt_render_params {
     int32_t accumulate;    /* progressive accumulation (raygen.cu:254-260) */
     float background[4];   /* RGBA background (CallRender3D::BackgroundColor) */
     uint32_t flags;        /* MMHRT_FLAG_* */
+    uint32_t window[4];    /* x0, y0, w, h: render only this pixel window of the width x height frame
+                              (w == 0: the whole frame); outputs are w x h and equal that crop of the
+                              full frame bit for bit */
 } mmhrt_render_params;
 
 /* ---- framebuffer: host destinations (any pointer may be NULL = skip) ------ */
@@ -198,6 +201,34 @@ MMHRT_API int32_t mmhrt_get_timings(mmhrt_ctx ctx, float* times_ms, int32_t n);
  * "count_visits"): node visits, sphere tests, rays traced. Also refreshes the option
  * "stack_overflow" (1 if a traversal stack ever exceeded its capacity; 0 expected). */
 MMHRT_API int32_t mmhrt_get_counters(mmhrt_ctx ctx, uint64_t* node_visits, uint64_t* sphere_tests, uint64_t* rays);
+
+/* ---- learned denoiser ------------------------------------------------------
+ * No counterpart in the reference (optix_hpg has no denoiser). A 4-level U-Net (3x3 convs,
+ * 32/48/64/96 channels, ReLU, skip concatenation) run and trained with BF16 MFMA kernels that
+ * accumulate in FP32. It predicts per-pixel AO visibility v; the output colour is albedo * v on
+ * hit pixels and the background elsewhere.
+ * Parameters (fp32, canonical order): per layer W[tap][c_in][c_out] then bias[c_out]; layers
+ * enc0a(16->32) enc0b(32->32) enc1a(32->48) enc1b(48->48) enc2a(48->64) enc2b(64->64)
+ * bota(64->96) botb(96->96) dec2a(64+96->64) dec2b(64->64) dec1a(48+64->48) dec1b(48->48)
+ * dec0a(32+48->32) dec0b(32->32) out(32->16, channel 0 used, no ReLU).
+ * Feature record per pixel (12 fp16): colour rgb, albedo rgb, normal xyz, linear depth t
+ * (0 on miss), visibility, 0 — the colour channels of a render plus its MMHRT_FLAG_AUX buffer.
+ * depth_scale divides t before it enters the network. */
+MMHRT_API int32_t mmhrt_denoiser_init(mmhrt_ctx ctx, uint64_t seed);
+MMHRT_API int32_t mmhrt_denoiser_param_count(mmhrt_ctx ctx, uint64_t* n);
+MMHRT_API int32_t mmhrt_denoiser_get_params(mmhrt_ctx ctx, float* out, uint64_t n);
+MMHRT_API int32_t mmhrt_denoiser_set_params(mmhrt_ctx ctx, const float* in, uint64_t n);
+/* Denoise the last rendered frame in place on the device (it must carry MMHRT_FLAG_AUX). */
+MMHRT_API int32_t mmhrt_denoise(mmhrt_ctx ctx, float depth_scale);
+/* Denoise host features [b][h][w][12] fp16 into out_color [b][h][w][3] fp32. */
+MMHRT_API int32_t mmhrt_denoise_features(mmhrt_ctx ctx, const uint16_t* feats, uint32_t b, uint32_t h, uint32_t w,
+                                         float depth_scale, const float bg[4], float* out_color);
+/* One training step on a batch: loss = mean squared error of the composed colour against
+ * target [b][h][w][3]. grads_out (NULL to skip) receives d(loss)/d(params) before the update;
+ * apply = 0 computes the gradients only; otherwise Adam (0.9, 0.999, 1e-8) with step lr. */
+MMHRT_API int32_t mmhrt_denoiser_train_step(mmhrt_ctx ctx, const uint16_t* feats, const float* target, uint32_t b,
+                                            uint32_t h, uint32_t w, float depth_scale, const float bg[4], float lr,
+                                            int32_t apply, float* loss_out, float* grads_out);
 
 /* Tuning knobs (autotuner output): "stack_lds" (LDS stack entries per lane), "tile_w", "tile_h",
  * "persistent" (0/1), "refill" (ballot refill threshold in lanes), "count_visits" (0/1). */

@@ -46,6 +46,7 @@ class RenderParams(C.Structure):
         ("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32), ("mode", C.c_int32),
         ("ao_distance", C.c_float), ("max_bounces", C.c_int32), ("frame_idx", C.c_uint32),
         ("accumulate", C.c_int32), ("background", C.c_float * 4), ("flags", C.c_uint32),
+        ("window", C.c_uint32 * 4),
     ]
 
 
@@ -73,6 +74,15 @@ SIGNATURES = {
     "mmhrt_get_timings": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32]),
     "mmhrt_get_counters": (C.c_int32, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                        C.POINTER(C.c_uint64)]),
+    "mmhrt_denoiser_init": (C.c_int32, [C.c_void_p, C.c_uint64]),
+    "mmhrt_denoiser_param_count": (C.c_int32, [C.c_void_p, C.POINTER(C.c_uint64)]),
+    "mmhrt_denoiser_get_params": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint64]),
+    "mmhrt_denoiser_set_params": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint64]),
+    "mmhrt_denoise": (C.c_int32, [C.c_void_p, C.c_float]),
+    "mmhrt_denoise_features": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float,
+                                           C.c_void_p, C.c_void_p]),
+    "mmhrt_denoiser_train_step": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                              C.c_float, C.c_void_p, C.c_float, C.c_int32, C.c_void_p, C.c_void_p]),
     "mmhrt_set_option": (C.c_int32, [C.c_void_p, C.c_char_p, C.c_int64]),
     "mmhrt_get_option": (C.c_int32, [C.c_void_p, C.c_char_p, C.POINTER(C.c_int64)]),
 }

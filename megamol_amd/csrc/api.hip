@@ -8,6 +8,7 @@
 
 #include "../../include/mmhrt.h"
 #include "ctx.hpp"
+#include "denoise.hpp"
 
 namespace mmhrt {
 
@@ -68,6 +69,16 @@ void frame_consts(const mmhrt_camera* cam, const mmhrt_render_params* p, FrameCo
     f.accumulate = p->accumulate;
     f.want_aux = (p->flags & MMHRT_FLAG_AUX) ? 1 : 0;
     f.max_bounces = p->max_bounces;
+    if (p->window[2] > 0 && p->window[3] > 0) {
+        f.win_x0 = p->window[0];
+        f.win_y0 = p->window[1];
+        f.win_w = p->window[2];
+        f.win_h = p->window[3];
+    } else {
+        f.win_x0 = f.win_y0 = 0;
+        f.win_w = p->width;
+        f.win_h = p->height;
+    }
 }
 
 int check_ctx(mmhrt_ctx c) {
@@ -127,10 +138,15 @@ int32_t mmhrt_destroy(mmhrt_ctx c) {
     release(c->sph); release(c->col); release(c->staging);
     release(c->keys64); release(c->ka); release(c->kb); release(c->kla); release(c->klb);
     release(c->va); release(c->vb); release(c->hist); release(c->scan_tmp);
-    release(c->leaf_sph); release(c->leaf_id); release(c->nodes); release(c->parent); release(c->flags);
+    release(c->leaf_sph); release(c->leaf_id); release(c->nodes); release(c->parent); release(c->flags); release(c->box_cnt);
     release(c->partial); release(c->info); release(c->tf_buf);
     release(c->fb_col); release(c->fb_depth); release(c->fb_t); release(c->fb_id); release(c->fb_rgba8);
     release(c->fb_aux); release(c->spill); release(c->counters);
+    if (c->dn) {
+        c->dn->release_all();
+        delete c->dn;
+        c->dn = nullptr;
+    }
     if (c->tm.init)
         for (int i = 0; i < 12; ++i) hipEventDestroy(c->tm.ev[i]);
     hipStreamDestroy(c->stream);
@@ -306,11 +322,17 @@ int32_t mmhrt_render(mmhrt_ctx c, const mmhrt_camera* cam, const mmhrt_render_pa
     if (p->mode < 0 || p->mode > 2) return fail(c, MMHRT_E_INVALID, "render: unknown mode");
     if (!c->built || c->built_n != c->n) return fail(c, MMHRT_E_STATE, "render: build the LBVH first");
     if (!(cam->fovy > 0.0f) || !(cam->aspect > 0.0f)) return fail(c, MMHRT_E_INVALID, "render: bad fovy or aspect");
-    if (p->accumulate && p->frame_idx > 0 && (c->fb_w != p->width || c->fb_h != p->height))
+    if (p->window[2] > 0 && ((uint64_t)p->window[0] + p->window[2] > p->width ||
+                             (uint64_t)p->window[1] + p->window[3] > p->height || p->window[3] == 0))
+        return fail(c, MMHRT_E_INVALID, "render: window outside the frame");
+    uint32_t ow = p->window[2] > 0 ? p->window[2] : p->width, oh = p->window[2] > 0 ? p->window[3] : p->height;
+    if (p->accumulate && p->frame_idx > 0 && (c->fb_w != ow || c->fb_h != oh))
         return fail(c, MMHRT_E_STATE, "render: accumulation needs the previous frame at the same size");
     FrameConsts f;
     frame_consts(cam, p, f);
     if (int e = render_frame(c, f)) return e;
+    c->last_aux = (p->flags & MMHRT_FLAG_AUX) != 0;
+    for (int i = 0; i < 4; ++i) c->last_bg[i] = p->background[i];
     if (out && !(p->flags & MMHRT_FLAG_NO_DOWNLOAD)) return mmhrt_download(c, out);
     return MMHRT_OK;
 }

@@ -11,6 +11,9 @@
 #include "rt_math.hpp"
 
 namespace mmhrt {
+namespace dn {
+struct Denoiser;
+}
 
 // BVH2 node, 64 bytes: both child boxes in one node so that one visit tests two children
 // with four 16-byte loads. child ref >= 0: internal node index; < 0: leaf ~k (k-th sphere in
@@ -77,6 +80,7 @@ struct mmhrt_ctx_s {
     mmhrt::DevBuf<mmhrt::Node> nodes;
     mmhrt::DevBuf<int32_t> parent;
     mmhrt::DevBuf<uint32_t> flags;
+    mmhrt::DevBuf<uint32_t> box_cnt;  // worklist sizes of the bottom-up box rounds
     mmhrt::DevBuf<float> partial;
     mmhrt::DevBuf<mmhrt::SceneInfo> info;
     mmhrt::SceneInfo info_host{};
@@ -102,6 +106,11 @@ struct mmhrt_ctx_s {
     int64_t opt_refit_period = 4;  // MMHRT_BUILD_AUTO: refits between rebuilds
     int64_t refits_since_build = 0;
     int64_t stack_overflow = 0;  // set by mmhrt_get_counters from the trace kernel's flag
+
+    // learned denoiser (created on first use)
+    mmhrt::dn::Denoiser* dn = nullptr;
+    bool last_aux = false;
+    float last_bg[4] = {0, 0, 0, 1};
 
     // timing
     mmhrt::Timers tm;

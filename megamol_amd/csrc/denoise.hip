@@ -1,0 +1,934 @@
+// denoise.hip — learned denoiser for 1-spp AO frames: a 4-level U-Net (3x3 convolutions,
+// 32/48/64/96 channels, ReLU, skip concatenation, 2x2 average pooling, nearest upsampling),
+// trained and run with hand-written BF16 MFMA kernels that accumulate in FP32.
+//
+// No counterpart in the reference (MegaMol's optix_hpg path has no denoiser); the renderer
+// itself produces the training pairs (1-spp features against high-spp frames).
+//
+// Tensors are NHWC bf16 with channel counts padded to multiples of 16. A 3x3 convolution is an
+// implicit GEMM: M = pixels, N = output channels, K = 9 taps x input channels. One wavefront
+// computes 64 consecutive pixels of one row against all output channels with
+// v_mfma_f32_16x16x16_bf16 (4 pixel blocks x N/16 channel blocks of 16x16 accumulators).
+// The input of a decoder convolution is the concatenation [skip (full res), coarser (half res,
+// nearest-upsampled)]; the kernel reads both tensors directly, so neither the upsampled nor the
+// concatenated tensor is ever stored.
+//
+// Lane layout of v_mfma_f32_16x16x16_bf16 (64 lanes): A holds row l%16, k = 4*(l/16)..+3;
+// B holds column l%16, k = 4*(l/16)..+3; the 4 results of a lane are rows 4*(l/16)..+3 of
+// column l%16.
+#include <math.h>
+
+#include <algorithm>
+#include <random>
+#include <vector>
+
+#include "denoise.hpp"
+
+namespace mmhrt {
+namespace dn {
+namespace {
+
+typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+// round to nearest even
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);  // NaN stays NaN
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+union Frag {
+    bf16x4 v;
+    uint2 u;
+    uint16_t h[4];
+};
+
+// ------------------------------------------------------------------ forward convolution
+// X1: [B,H,W,C1], X2: [B,H/2,W/2,C2] (nearest upsampled) or null; Y: [B,H,W,CO] (bf16) or
+// Yf (fp32). Wf: weights in fragment order [tap][ci/16][co/16][co%16][ci%16] (bf16).
+template <int NB>
+__global__ __launch_bounds__(256) void k_conv_fwd(ConvArgs a) {
+    const int lane = __lane_id();
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int runs_per_row = (a.W + 63) / 64;
+    const int row = wave / runs_per_row;  // b*H + y
+    if (row >= a.B * a.H) return;
+    const int x0 = (wave % runs_per_row) * 64;
+    const int b = row / a.H, y = row % a.H;
+    const int nb0 = blockIdx.y * NB;  // output-channel block offset (grid.y splits CO)
+    const int CC = (a.C1 + a.C2) / 16;
+    const int CC1 = a.C1 / 16;
+    const int NBT = a.CO / 16;
+    f32x4 acc[4][NB];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int kq = lane >> 4;  // k quad
+    const int lr = lane & 15;  // row (A) / column (B)
+    for (int t = 0; t < 9; ++t) {
+        const int yy = y + t / 3 - 1;
+        const bool yok = yy >= 0 && yy < a.H;
+        for (int cc = 0; cc < CC; ++cc) {
+            Frag fa[4];
+#pragma unroll
+            for (int pb = 0; pb < 4; ++pb) {
+                int xx = x0 + pb * 16 + lr + t % 3 - 1;
+                fa[pb].u = make_uint2(0u, 0u);
+                if (yok && xx >= 0 && xx < a.W) {
+                    const uint16_t* src;
+                    if (cc < CC1)
+                        src = a.X1 + (((size_t)b * a.H + yy) * a.W + xx) * a.C1 + cc * 16 + kq * 4;
+                    else
+                        src = a.X2 + (((size_t)b * (a.H >> 1) + (yy >> 1)) * (a.W >> 1) + (xx >> 1)) * a.C2 +
+                              (cc - CC1) * 16 + kq * 4;
+                    fa[pb].u = *reinterpret_cast<const uint2*>(src);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                int nb = nb0 + j;
+                if (nb >= NBT) break;
+                Frag fb;
+                fb.u = *reinterpret_cast<const uint2*>(a.Wt + ((((size_t)t * CC + cc) * NBT + nb) * 16 + lr) * 16 + kq * 4);
+#pragma unroll
+                for (int pb = 0; pb < 4; ++pb)
+                    acc[pb][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fa[pb].v, fb.v, acc[pb][j], 0, 0, 0);
+            }
+        }
+    }
+    // epilogue: bias, ReLU, store
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        int nb = nb0 + j;
+        if (nb >= NBT) break;
+        int co = nb * 16 + lr;
+        float bias = a.bias[co];
+#pragma unroll
+        for (int pb = 0; pb < 4; ++pb)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                int x = x0 + pb * 16 + kq * 4 + e;
+                if (x >= a.W) continue;
+                float v = acc[pb][j][e] + bias;
+                if (a.relu) v = fmaxf(v, 0.0f);
+                size_t o = (((size_t)b * a.H + y) * a.W + x) * a.CO + co;
+                if (a.Yf)
+                    a.Yf[o] = v;
+                else
+                    a.Y[o] = f2bf(v);
+            }
+    }
+}
+
+// ------------------------------------------------------------------ backward data
+// dXc[p, ci] = sum_t sum_co G[p - o_t, co] * W[t][ci][co], G = dY * (Y > 0 if relu).
+// dXc is [B,H,W,CI] bf16 (full-resolution concatenated input gradient).
+// Wb: weights in fragment order [tap][co/16][ci/16][ci%16][co%16].
+template <int NB>
+__global__ __launch_bounds__(256) void k_conv_bwd_data(ConvBwdArgs a) {
+    const int lane = __lane_id();
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int runs_per_row = (a.W + 63) / 64;
+    const int row = wave / runs_per_row;
+    if (row >= a.B * a.H) return;
+    const int x0 = (wave % runs_per_row) * 64;
+    const int b = row / a.H, y = row % a.H;
+    const int nb0 = blockIdx.y * NB;
+    const int KC = a.CO / 16;  // k chunks per tap (output channels of the forward conv)
+    const int NBT = a.CI / 16;
+    f32x4 acc[4][NB];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int kq = lane >> 4, lr = lane & 15;
+    for (int t = 0; t < 9; ++t) {
+        const int yy = y - (t / 3 - 1);
+        const bool yok = yy >= 0 && yy < a.H;
+        for (int kc = 0; kc < KC; ++kc) {
+            Frag fa[4];
+#pragma unroll
+            for (int pb = 0; pb < 4; ++pb) {
+                int xx = x0 + pb * 16 + lr - (t % 3 - 1);
+                fa[pb].u = make_uint2(0u, 0u);
+                if (yok && xx >= 0 && xx < a.W) {
+                    size_t o = (((size_t)b * a.H + yy) * a.W + xx) * a.CO + kc * 16 + kq * 4;
+                    Frag g;
+                    g.u = *reinterpret_cast<const uint2*>(a.dY + o);
+                    if (a.relu) {
+                        Frag yv;
+                        yv.u = *reinterpret_cast<const uint2*>(a.Y + o);
+#pragma unroll
+                        for (int e = 0; e < 4; ++e)
+                            if (!(bf2f(yv.h[e]) > 0.0f)) g.h[e] = 0;
+                    }
+                    fa[pb] = g;
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                int nb = nb0 + j;
+                if (nb >= NBT) break;
+                Frag fb;
+                fb.u = *reinterpret_cast<const uint2*>(a.Wt + ((((size_t)t * KC + kc) * NBT + nb) * 16 + lr) * 16 + kq * 4);
+#pragma unroll
+                for (int pb = 0; pb < 4; ++pb)
+                    acc[pb][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fa[pb].v, fb.v, acc[pb][j], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        int nb = nb0 + j;
+        if (nb >= NBT) break;
+        int ci = nb * 16 + lr;
+#pragma unroll
+        for (int pb = 0; pb < 4; ++pb)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                int x = x0 + pb * 16 + kq * 4 + e;
+                if (x >= a.W) continue;
+                a.dX[(((size_t)b * a.H + y) * a.W + x) * a.CI + ci] = f2bf(acc[pb][j][e]);
+            }
+    }
+}
+
+// ------------------------------------------------------------------ backward weights
+// dW[t][ci][co] += sum_p X[p + o_t, ci] * G[p, co] over this block's pixel range; one wave per
+// (tap, 16 input channels) and pixel split; K of the MFMA runs over 16 pixels at a time.
+template <int NB>
+__global__ __launch_bounds__(64) void k_conv_bwd_weight(ConvWgtArgs a) {
+    const int lane = __lane_id();
+    const int CC = (a.C1 + a.C2) / 16;
+    const int CC1 = a.C1 / 16;
+    const int t = blockIdx.x / CC, cc = blockIdx.x % CC;
+    const int NBT = a.CO / 16;
+    const int nb0 = blockIdx.z * NB;
+    const int dy = t / 3 - 1, dx = t % 3 - 1;
+    const long P = (long)a.B * a.H * a.W;
+    const long per = (P + gridDim.y - 1) / gridDim.y;
+    const long p0 = (long)blockIdx.y * per;
+    const long p1 = p0 + per < P ? p0 + per : P;
+    f32x4 acc[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int kq = lane >> 4, lr = lane & 15;
+    for (long base = p0; base < p1; base += 16) {
+        Frag fa;
+        Frag fg[NB];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            long p = base + kq * 4 + e;
+            uint16_t xv = 0;
+            bool in = p < p1;
+            int b = 0, y = 0, x = 0;
+            if (in) {
+                x = (int)(p % a.W);
+                long r = p / a.W;
+                y = (int)(r % a.H);
+                b = (int)(r / a.H);
+                int yy = y + dy, xx = x + dx;
+                if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) {
+                    if (cc < CC1)
+                        xv = a.X1[(((size_t)b * a.H + yy) * a.W + xx) * a.C1 + cc * 16 + lr];
+                    else
+                        xv = a.X2[(((size_t)b * (a.H >> 1) + (yy >> 1)) * (a.W >> 1) + (xx >> 1)) * a.C2 +
+                                  (cc - CC1) * 16 + lr];
+                }
+            }
+            fa.h[e] = xv;
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                int nb = nb0 + j;
+                uint16_t gv = 0;
+                if (in && nb < NBT) {
+                    size_t o = (size_t)p * a.CO + nb * 16 + lr;
+                    gv = a.dY[o];
+                    if (a.relu && !(bf2f(a.Y[o]) > 0.0f)) gv = 0;
+                }
+                fg[j].h[e] = gv;
+            }
+        }
+        // A = X^T (rows: 16 input channels, k: 16 pixels), B = G (k: 16 pixels, columns: 16 outputs)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fa.v, fg[j].v, acc[j], 0, 0, 0);
+    }
+    // result rows = input channels 4*kq+e, column = output channel lr
+    const int CI = a.C1 + a.C2;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        int nb = nb0 + j;
+        if (nb >= NBT) break;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            int ci = cc * 16 + kq * 4 + e;
+            atomicAdd(a.dW + ((size_t)t * CI + ci) * a.CO + nb * 16 + lr, acc[j][e]);
+        }
+    }
+}
+
+// bias gradient: db[co] = sum_p G[p, co]
+__global__ __launch_bounds__(256) void k_bias_grad(const uint16_t* __restrict__ dY, const uint16_t* __restrict__ Y,
+                                                   int relu, long P, int CO, float* db) {
+    int co = blockIdx.y;
+    float s = 0.0f;
+    for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < P; p += (long)gridDim.x * blockDim.x) {
+        size_t o = (size_t)p * CO + co;
+        float g = bf2f(dY[o]);
+        if (relu && !(bf2f(Y[o]) > 0.0f)) g = 0.0f;
+        s += g;
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    __shared__ float ws[4];
+    if (__lane_id() == 0) ws[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(db + co, ws[0] + ws[1] + ws[2] + ws[3]);
+}
+
+// ------------------------------------------------------------------ pooling / concat split
+__global__ void k_avgpool(const uint16_t* __restrict__ X, uint16_t* __restrict__ Y, int B, int H, int W, int C) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int Ho = H / 2, Wo = W / 2;
+    long n = (long)B * Ho * Wo * C;
+    if (i >= n) return;
+    int c = (int)(i % C);
+    long r = i / C;
+    int xo = (int)(r % Wo);
+    r /= Wo;
+    int yo = (int)(r % Ho);
+    int b = (int)(r / Ho);
+    const uint16_t* s = X + (((size_t)b * H + 2 * yo) * W + 2 * xo) * C + c;
+    float v = bf2f(s[0]) + bf2f(s[C]) + bf2f(s[(size_t)W * C]) + bf2f(s[(size_t)W * C + C]);
+    Y[i] = f2bf(0.25f * v);
+}
+
+// dX(full) += dPooled / 4 spread over each 2x2 block (accumulate into an existing gradient)
+__global__ void k_avgpool_bwd(const uint16_t* __restrict__ dY, uint16_t* __restrict__ dX, int B, int H, int W, int C,
+                              int accumulate) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long n = (long)B * H * W * C;
+    if (i >= n) return;
+    int c = (int)(i % C);
+    long r = i / C;
+    int x = (int)(r % W);
+    r /= W;
+    int y = (int)(r % H);
+    int b = (int)(r / H);
+    float g = 0.25f * bf2f(dY[(((size_t)b * (H / 2) + y / 2) * (W / 2) + x / 2) * C + c]);
+    if (accumulate) g += bf2f(dX[i]);
+    dX[i] = f2bf(g);
+}
+
+// Split the gradient of a concatenated input: channels [0,C1) go to dX1 (full res),
+// channels [C1,C1+C2) are summed over 2x2 blocks into dX2 (half res, nearest-upsample adjoint).
+__global__ void k_concat_split(const uint16_t* __restrict__ dXc, int B, int H, int W, int C1, int C2,
+                               uint16_t* __restrict__ dX1, uint16_t* __restrict__ dX2) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int CI = C1 + C2;
+    long n1 = (long)B * H * W * C1;
+    long n2 = (long)B * (H / 2) * (W / 2) * C2;
+    if (i < n1) {
+        int c = (int)(i % C1);
+        long p = i / C1;
+        dX1[i] = dXc[(size_t)p * CI + c];
+    }
+    if (i < n2) {
+        int c = (int)(i % C2);
+        long r = i / C2;
+        int xo = (int)(r % (W / 2));
+        r /= (W / 2);
+        int yo = (int)(r % (H / 2));
+        int b = (int)(r / (H / 2));
+        const uint16_t* s = dXc + (((size_t)b * H + 2 * yo) * W + 2 * xo) * CI + C1 + c;
+        float v = bf2f(s[0]) + bf2f(s[CI]) + bf2f(s[(size_t)W * CI]) + bf2f(s[(size_t)W * CI + CI]);
+        dX2[i] = f2bf(v);
+    }
+}
+
+// ------------------------------------------------------------------ features / output
+// Raw feature record per pixel (12 halfs): colour rgb, albedo rgb, normal xyz, t, visibility, 0.
+// Network input (16 bf16): visibility, albedo rgb, normal xyz, depth t/depth_scale, colour rgb,
+// hit mask, 0...
+__global__ void k_pack_input(const uint16_t* __restrict__ raw, int B, int H, int W, int Hp, int Wp, float inv_depth,
+                             uint16_t* __restrict__ X0) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long n = (long)B * Hp * Wp;
+    if (i >= n) return;
+    int x = (int)(i % Wp);
+    long r = i / Wp;
+    int y = (int)(r % Hp);
+    int b = (int)(r / Hp);
+    float f[16];
+    for (int k = 0; k < 16; ++k) f[k] = 0.0f;
+    if (x < W && y < H) {
+        const uint16_t* s = raw + (((size_t)b * H + y) * W + x) * 12;
+        float v[12];
+        for (int k = 0; k < 12; ++k) v[k] = __half2float(__ushort_as_half(s[k]));
+        float t = v[9];
+        bool hit = t > 0.0f;
+        f[0] = v[10];
+        f[1] = v[3]; f[2] = v[4]; f[3] = v[5];
+        f[4] = v[6]; f[5] = v[7]; f[6] = v[8];
+        f[7] = hit ? t * inv_depth : 0.0f;
+        f[8] = v[0]; f[9] = v[1]; f[10] = v[2];
+        f[11] = hit ? 1.0f : 0.0f;
+    }
+    uint16_t* d = X0 + i * 16;
+    for (int k = 0; k < 16; ++k) d[k] = f2bf(f[k]);
+}
+
+// Frame -> raw features (device path of the renderer output).
+__global__ void k_frame_features(const float4* __restrict__ col, const __half* __restrict__ aux, long P,
+                                 uint16_t* __restrict__ raw) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    float4 c = col[i];
+    const uint16_t* a = reinterpret_cast<const uint16_t*>(aux) + i * 8;
+    uint16_t* d = raw + i * 12;
+    d[0] = __half_as_ushort(__float2half(c.x));
+    d[1] = __half_as_ushort(__float2half(c.y));
+    d[2] = __half_as_ushort(__float2half(c.z));
+    for (int k = 0; k < 8; ++k) d[3 + k] = a[k];
+    d[11] = 0;
+}
+
+// Network output (channel 0 of OUT, fp32 [B,Hp,Wp,16]) -> colour albedo * v on hit pixels,
+// background elsewhere. With a target, also the squared error and d(loss)/d(out).
+__global__ __launch_bounds__(256) void k_compose(const float* __restrict__ out, const uint16_t* __restrict__ raw, int B,
+                                                 int H, int W, int Hp, int Wp, float4 bg,
+                                                 const float* __restrict__ target, float inv_count,
+                                                 float* __restrict__ color, uint16_t* __restrict__ dout,
+                                                 float* __restrict__ loss) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long n = (long)B * Hp * Wp;
+    float se = 0.0f;
+    if (i < n) {
+        int x = (int)(i % Wp);
+        long r = i / Wp;
+        int y = (int)(r % Hp);
+        int b = (int)(r / Hp);
+        float g0 = 0.0f;
+        if (x < W && y < H) {
+            size_t q = ((size_t)b * H + y) * W + x;
+            const uint16_t* s = raw + q * 12;
+            float t = __half2float(__ushort_as_half(s[9]));
+            float al[3] = {__half2float(__ushort_as_half(s[3])), __half2float(__ushort_as_half(s[4])),
+                           __half2float(__ushort_as_half(s[5]))};
+            float v = out[i * 16];
+            float c[3];
+            bool hit = t > 0.0f;
+            for (int k = 0; k < 3; ++k) c[k] = hit ? al[k] * v : (k == 0 ? bg.x : (k == 1 ? bg.y : bg.z));
+            if (color)
+                for (int k = 0; k < 3; ++k) color[q * 3 + k] = c[k];
+            if (target) {
+                for (int k = 0; k < 3; ++k) {
+                    float d = c[k] - target[q * 3 + k];
+                    se += d * d;
+                    if (hit) g0 += 2.0f * d * al[k] * inv_count;
+                }
+            }
+        }
+        if (dout) {
+            uint16_t* d = dout + i * 16;
+            d[0] = f2bf(g0);
+            for (int k = 1; k < 16; ++k) d[k] = 0;
+        }
+    }
+    if (loss) {
+        for (int off = 32; off > 0; off >>= 1) se += __shfl_xor(se, off, 64);
+        __shared__ float ws[4];
+        if (__lane_id() == 0) ws[threadIdx.x >> 6] = se;
+        __syncthreads();
+        if (threadIdx.x == 0) atomicAdd(loss, (ws[0] + ws[1] + ws[2] + ws[3]) * inv_count);
+    }
+}
+
+// Denoised colour back into the frame buffers.
+__global__ void k_compose_frame(const float* __restrict__ color, long P, float4* __restrict__ fb_col,
+                                uint32_t* __restrict__ fb_rgba8) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    float4 c = fb_col[i];
+    c.x = color[i * 3];
+    c.y = color[i * 3 + 1];
+    c.z = color[i * 3 + 2];
+    fb_col[i] = c;
+    fb_rgba8[i] = pack_rgba8(c);
+}
+
+// ------------------------------------------------------------------ parameters
+// Canonical fp32 layout per layer: W[t][ci][co] then bias[co]. Fragment layouts for the MFMA
+// kernels are rebuilt from it after every update.
+__global__ void k_relayout(const float* __restrict__ Wc, int CI, int CO, uint16_t* __restrict__ Wf,
+                           uint16_t* __restrict__ Wb) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long n = 9L * CI * CO;
+    if (i >= n) return;
+    int co = (int)(i % CO);
+    long r = i / CO;
+    int ci = (int)(r % CI);
+    int t = (int)(r / CI);
+    uint16_t v = f2bf(Wc[i]);
+    // forward: [t][ci/16][co/16][co%16][ci%16]
+    int CC = CI / 16, NBT = CO / 16;
+    Wf[((((size_t)t * CC + ci / 16) * NBT + co / 16) * 16 + co % 16) * 16 + ci % 16] = v;
+    // backward data: [t][co/16][ci/16][ci%16][co%16]
+    int KC = CO / 16, NBI = CI / 16;
+    Wb[((((size_t)t * KC + co / 16) * NBI + ci / 16) * 16 + ci % 16) * 16 + co % 16] = v;
+}
+
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, long n, float lr, float b1, float b2, float eps, float c1, float c2) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float gi = g[i];
+    float mi = b1 * m[i] + (1.0f - b1) * gi;
+    float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+}
+
+__global__ void k_zero_f32(float* p, long n) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = 0.0f;
+}
+
+inline unsigned nblk(long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+// ------------------------------------------------------------------ host side
+const LayerSpec kLayers[kNumLayers] = {
+    // name, C1 (full-res input), C2 (half-res input), CO, relu
+    {"enc0a", 16, 0, 32, 1}, {"enc0b", 32, 0, 32, 1},  {"enc1a", 32, 0, 48, 1}, {"enc1b", 48, 0, 48, 1},
+    {"enc2a", 48, 0, 64, 1}, {"enc2b", 64, 0, 64, 1},  {"bota", 64, 0, 96, 1},  {"botb", 96, 0, 96, 1},
+    {"dec2a", 64, 96, 64, 1}, {"dec2b", 64, 0, 64, 1}, {"dec1a", 48, 64, 48, 1}, {"dec1b", 48, 0, 48, 1},
+    {"dec0a", 32, 48, 32, 1}, {"dec0b", 32, 0, 32, 1}, {"out", 32, 0, 16, 0},
+};
+
+size_t param_count() {
+    size_t n = 0;
+    for (const auto& L : kLayers) n += 9ull * (L.c1 + L.c2) * L.co + L.co;
+    return n;
+}
+
+static size_t param_offset(int l) {
+    size_t n = 0;
+    for (int i = 0; i < l; ++i) n += 9ull * (kLayers[i].c1 + kLayers[i].c2) * kLayers[i].co + kLayers[i].co;
+    return n;
+}
+
+static void conv_fwd(hipStream_t s, const ConvArgs& a) {
+    long waves = (long)a.B * a.H * ((a.W + 63) / 64);
+    int NBT = a.CO / 16;
+    int nb = NBT <= 6 ? NBT : 4;
+    dim3 g(nblk(waves * 64), (NBT + nb - 1) / nb);
+    switch (nb) {
+    case 1: hipLaunchKernelGGL(k_conv_fwd<1>, g, dim3(256), 0, s, a); break;
+    case 2: hipLaunchKernelGGL(k_conv_fwd<2>, g, dim3(256), 0, s, a); break;
+    case 3: hipLaunchKernelGGL(k_conv_fwd<3>, g, dim3(256), 0, s, a); break;
+    case 4: hipLaunchKernelGGL(k_conv_fwd<4>, g, dim3(256), 0, s, a); break;
+    case 5: hipLaunchKernelGGL(k_conv_fwd<5>, g, dim3(256), 0, s, a); break;
+    default: hipLaunchKernelGGL(k_conv_fwd<6>, g, dim3(256), 0, s, a); break;
+    }
+}
+
+static void conv_bwd_data(hipStream_t s, const ConvBwdArgs& a) {
+    long waves = (long)a.B * a.H * ((a.W + 63) / 64);
+    int NBT = a.CI / 16;
+    int nb = NBT <= 5 ? NBT : (NBT % 5 == 0 ? 5 : 4);
+    dim3 g(nblk(waves * 64), (NBT + nb - 1) / nb);
+    switch (nb) {
+    case 1: hipLaunchKernelGGL(k_conv_bwd_data<1>, g, dim3(256), 0, s, a); break;
+    case 2: hipLaunchKernelGGL(k_conv_bwd_data<2>, g, dim3(256), 0, s, a); break;
+    case 3: hipLaunchKernelGGL(k_conv_bwd_data<3>, g, dim3(256), 0, s, a); break;
+    case 4: hipLaunchKernelGGL(k_conv_bwd_data<4>, g, dim3(256), 0, s, a); break;
+    default: hipLaunchKernelGGL(k_conv_bwd_data<5>, g, dim3(256), 0, s, a); break;
+    }
+}
+
+static void conv_bwd_weight(hipStream_t s, const ConvWgtArgs& a, int cu) {
+    int CC = (a.C1 + a.C2) / 16;
+    int NBT = a.CO / 16;
+    int nb = NBT <= 6 ? NBT : 4;
+    long P = (long)a.B * a.H * a.W;
+    int splits = (int)std::max<long>(1, std::min<long>((long)cu * 8 / (9 * CC), (P + 1023) / 1024));
+    dim3 g(9 * CC, splits, (NBT + nb - 1) / nb);
+    switch (nb) {
+    case 1: hipLaunchKernelGGL(k_conv_bwd_weight<1>, g, dim3(64), 0, s, a); break;
+    case 2: hipLaunchKernelGGL(k_conv_bwd_weight<2>, g, dim3(64), 0, s, a); break;
+    case 3: hipLaunchKernelGGL(k_conv_bwd_weight<3>, g, dim3(64), 0, s, a); break;
+    case 4: hipLaunchKernelGGL(k_conv_bwd_weight<4>, g, dim3(64), 0, s, a); break;
+    case 5: hipLaunchKernelGGL(k_conv_bwd_weight<5>, g, dim3(64), 0, s, a); break;
+    default: hipLaunchKernelGGL(k_conv_bwd_weight<6>, g, dim3(64), 0, s, a); break;
+    }
+}
+
+
+void launch_compose(hipStream_t s, const float* out, const uint16_t* raw, int B, int H, int W, int Hp, int Wp,
+                    float4 bg, const float* target, float inv_count, float* color, uint16_t* dout, float* loss) {
+    long n = (long)B * Hp * Wp;
+    hipLaunchKernelGGL(k_compose, dim3(nblk(n)), dim3(256), 0, s, out, raw, B, H, W, Hp, Wp, bg, target, inv_count,
+                       color, dout, loss);
+}
+void zero(hipStream_t s, float* p, long n) {
+    hipLaunchKernelGGL(k_zero_f32, dim3((unsigned)std::min<long>(nblk(n), 4096)), dim3(256), 0, s, p, n);
+}
+void adam(hipStream_t s, float* p, const float* g, float* m, float* v, long n, float lr, float b1, float b2, float eps,
+          float c1, float c2) {
+    hipLaunchKernelGGL(k_adam, dim3(nblk(n)), dim3(256), 0, s, p, g, m, v, n, lr, b1, b2, eps, c1, c2);
+}
+
+int Denoiser::ensure_params(mmhrt_ctx_s* c) {
+    size_t n = param_count();
+    int e = 0;
+    if ((e = mmhrt::ensure(c, params, n, "denoiser params"))) return e;
+    if ((e = mmhrt::ensure(c, grads, n, "denoiser grads"))) return e;
+    if ((e = mmhrt::ensure(c, adam_m, n, "denoiser adam m"))) return e;
+    if ((e = mmhrt::ensure(c, adam_v, n, "denoiser adam v"))) return e;
+    size_t wn = 0;
+    for (const auto& L : kLayers) wn += 9ull * (L.c1 + L.c2) * L.co;
+    if ((e = mmhrt::ensure(c, wf, wn, "denoiser fwd weights"))) return e;
+    if ((e = mmhrt::ensure(c, wb, wn, "denoiser bwd weights"))) return e;
+    if ((e = mmhrt::ensure(c, scalar, 4, "denoiser scalars"))) return e;
+    return 0;
+}
+
+void Denoiser::relayout(hipStream_t s) {
+    size_t po = 0, wo = 0;
+    for (const auto& L : kLayers) {
+        int CI = L.c1 + L.c2;
+        long nw = 9L * CI * L.co;
+        hipLaunchKernelGGL(k_relayout, dim3(nblk(nw)), dim3(256), 0, s, params.p + po, CI, L.co, wf.p + wo, wb.p + wo);
+        po += nw + L.co;
+        wo += nw;
+    }
+}
+
+int Denoiser::ensure_acts(mmhrt_ctx_s* c, int B, int Hp, int Wp, bool train) {
+    size_t P0 = (size_t)B * Hp * Wp;
+    size_t need = 0;
+    // activation slots: per layer output + pooled tensors + input
+    size_t res[4] = {P0, P0 / 4, P0 / 16, P0 / 64};
+    int lvl[kNumLayers] = {0, 0, 1, 1, 2, 2, 3, 3, 2, 2, 1, 1, 0, 0, 0};
+    for (int l = 0; l < kNumLayers; ++l) need += res[lvl[l]] * kLayers[l].co;
+    need += P0 * 16 + res[1] * 32 + res[2] * 48 + res[3] * 64;  // input + pooled
+    int e = 0;
+    if ((e = mmhrt::ensure(c, acts, need, "denoiser activations"))) return e;
+    if ((e = mmhrt::ensure(c, out, P0 * 16, "denoiser output"))) return e;
+    if (train) {
+        size_t g = P0 * (16 + 288 + 32) + (P0 / 4) * 48 + (P0 / 16) * 64;
+        if ((e = mmhrt::ensure(c, gbuf, g, "denoiser gradient buffers"))) return e;
+    }
+    // carve activation slots
+    size_t o = 0;
+    x0 = acts.p + o;
+    o += P0 * 16;
+    for (int l = 0; l < kNumLayers; ++l) {
+        a[l] = acts.p + o;
+        o += res[lvl[l]] * kLayers[l].co;
+    }
+    pool[0] = acts.p + o;
+    o += res[1] * 32;
+    pool[1] = acts.p + o;
+    o += res[2] * 48;
+    pool[2] = acts.p + o;
+    o += res[3] * 64;
+    return 0;
+}
+
+// Forward pass over X0 [B,Hp,Wp,16] into `out` (fp32, channel 0 used).
+void Denoiser::forward(hipStream_t s, int B, int Hp, int Wp) {
+    int Hs[4] = {Hp, Hp / 2, Hp / 4, Hp / 8}, Ws[4] = {Wp, Wp / 2, Wp / 4, Wp / 8};
+    size_t wo = 0, po = 0;
+    auto conv = [&](int l, const uint16_t* x1, const uint16_t* x2, int lv, uint16_t* y, float* yf) {
+        const LayerSpec& L = kLayers[l];
+        ConvArgs A{};
+        A.X1 = x1; A.X2 = x2; A.C1 = L.c1; A.C2 = L.c2; A.CO = L.co;
+        A.B = B; A.H = Hs[lv]; A.W = Ws[lv];
+        A.Wt = wf.p + wo; A.bias = params.p + po + 9L * (L.c1 + L.c2) * L.co;
+        A.Y = y; A.Yf = yf; A.relu = L.relu;
+        conv_fwd(s, A);
+        wo += 9L * (L.c1 + L.c2) * L.co;
+        po += 9L * (L.c1 + L.c2) * L.co + L.co;
+    };
+    auto pl = [&](const uint16_t* x, uint16_t* y, int lv, int C) {
+        long n = (long)B * (Hs[lv] / 2) * (Ws[lv] / 2) * C;
+        hipLaunchKernelGGL(k_avgpool, dim3(nblk(n)), dim3(256), 0, s, x, y, B, Hs[lv], Ws[lv], C);
+    };
+    conv(0, x0, nullptr, 0, a[0], nullptr);
+    conv(1, a[0], nullptr, 0, a[1], nullptr);
+    pl(a[1], pool[0], 0, 32);
+    conv(2, pool[0], nullptr, 1, a[2], nullptr);
+    conv(3, a[2], nullptr, 1, a[3], nullptr);
+    pl(a[3], pool[1], 1, 48);
+    conv(4, pool[1], nullptr, 2, a[4], nullptr);
+    conv(5, a[4], nullptr, 2, a[5], nullptr);
+    pl(a[5], pool[2], 2, 64);
+    conv(6, pool[2], nullptr, 3, a[6], nullptr);
+    conv(7, a[6], nullptr, 3, a[7], nullptr);
+    conv(8, a[5], a[7], 2, a[8], nullptr);
+    conv(9, a[8], nullptr, 2, a[9], nullptr);
+    conv(10, a[3], a[9], 1, a[10], nullptr);
+    conv(11, a[10], nullptr, 1, a[11], nullptr);
+    conv(12, a[1], a[11], 0, a[12], nullptr);
+    conv(13, a[12], nullptr, 0, a[13], nullptr);
+    conv(14, a[13], nullptr, 0, nullptr, out.p);
+}
+
+// Backward pass. dOut (bf16 [B,Hp,Wp,16]) must be in the first P0*16 halfs of gbuf; weight and
+// bias gradients are accumulated into `grads` (zero it first).
+void Denoiser::backward(hipStream_t s, int B, int Hp, int Wp, int cu) {
+    int Hs[4] = {Hp, Hp / 2, Hp / 4, Hp / 8}, Ws[4] = {Wp, Wp / 2, Wp / 4, Wp / 8};
+    size_t P0 = (size_t)B * Hp * Wp;
+    const int lvl[kNumLayers] = {0, 0, 1, 1, 2, 2, 3, 3, 2, 2, 1, 1, 0, 0, 0};
+    // gbuf: dOut | three rotating temporaries (96 channels at full res) | skip gradients
+    uint16_t* dout = gbuf.p;
+    uint16_t* tmp[3] = {gbuf.p + P0 * 16, gbuf.p + P0 * (16 + 96), gbuf.p + P0 * (16 + 192)};
+    uint16_t* dskip[3];  // gradients of a[1] (lv0, 32), a[3] (lv1, 48), a[5] (lv2, 64)
+    dskip[0] = gbuf.p + P0 * (16 + 288);
+    dskip[1] = dskip[0] + P0 * 32;
+    dskip[2] = dskip[1] + (P0 / 4) * 48;
+    const uint16_t* in1[kNumLayers] = {x0, a[0], pool[0], a[2], pool[1], a[4], pool[2], a[6],
+                                       a[5], a[8], a[3], a[10], a[1], a[12], a[13]};
+    const uint16_t* in2[kNumLayers] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                       a[7], nullptr, a[9], nullptr, a[11], nullptr, nullptr};
+    size_t woff[kNumLayers], poff[kNumLayers];
+    {
+        size_t wo = 0, po = 0;
+        for (int l = 0; l < kNumLayers; ++l) {
+            woff[l] = wo;
+            poff[l] = po;
+            wo += 9ull * (kLayers[l].c1 + kLayers[l].c2) * kLayers[l].co;
+            po += 9ull * (kLayers[l].c1 + kLayers[l].c2) * kLayers[l].co + kLayers[l].co;
+        }
+    }
+    auto free_tmp = [&](const uint16_t* x, const uint16_t* y) -> uint16_t* {
+        for (int i = 0; i < 3; ++i)
+            if (tmp[i] != x && tmp[i] != y) return tmp[i];
+        return tmp[0];
+    };
+    const uint16_t* cur = dout;
+    for (int l = kNumLayers - 1; l >= 0; --l) {
+        const LayerSpec& L = kLayers[l];
+        int lv = lvl[l];
+        int H = Hs[lv], W = Ws[lv];
+        int CI = L.c1 + L.c2;
+        const uint16_t* Y = L.relu ? a[l] : nullptr;
+        ConvWgtArgs wa{};
+        wa.X1 = in1[l]; wa.X2 = in2[l]; wa.C1 = L.c1; wa.C2 = L.c2; wa.CO = L.co;
+        wa.B = B; wa.H = H; wa.W = W; wa.dY = cur; wa.Y = Y; wa.relu = L.relu; wa.dW = grads.p + poff[l];
+        conv_bwd_weight(s, wa, cu);
+        long P = (long)B * H * W;
+        hipLaunchKernelGGL(k_bias_grad, dim3((unsigned)std::min<long>(nblk(P), 256), L.co), dim3(256), 0, s, cur, Y,
+                           L.relu, P, L.co, grads.p + poff[l] + 9ull * CI * L.co);
+        if (l == 0) break;
+        uint16_t* dx = free_tmp(cur, nullptr);
+        ConvBwdArgs ba{};
+        ba.dY = cur; ba.Y = Y; ba.relu = L.relu; ba.CI = CI; ba.CO = L.co; ba.B = B; ba.H = H; ba.W = W;
+        ba.Wt = wb.p + woff[l]; ba.dX = dx;
+        conv_bwd_data(s, ba);
+        if (L.c2 > 0) {  // decoder: [skip | upsampled coarser]
+            int si = (l == 8) ? 2 : (l == 10 ? 1 : 0);
+            uint16_t* dco = free_tmp(cur, dx);
+            long n1 = (long)B * H * W * L.c1, n2 = (long)B * (H / 2) * (W / 2) * L.c2;
+            hipLaunchKernelGGL(k_concat_split, dim3(nblk(std::max(n1, n2))), dim3(256), 0, s, dx, B, H, W, L.c1,
+                               L.c2, dskip[si], dco);
+            cur = dco;
+        } else if (l == 2 || l == 4 || l == 6) {  // input was the pooled skip: add the pooling adjoint
+            int si = l / 2 - 1;
+            int lp = lv - 1;
+            long n = (long)B * Hs[lp] * Ws[lp] * L.c1;
+            hipLaunchKernelGGL(k_avgpool_bwd, dim3(nblk(n)), dim3(256), 0, s, dx, dskip[si], B, Hs[lp], Ws[lp], L.c1, 1);
+            cur = dskip[si];
+        } else {
+            cur = dx;
+        }
+    }
+}
+
+}  // namespace dn
+}  // namespace mmhrt
+
+// ====================================================================== C ABI
+using mmhrt::dn::Denoiser;
+using mmhrt::dn::kLayers;
+using mmhrt::dn::kNumLayers;
+
+namespace {
+int dn_check(mmhrt_ctx c, bool need_ready) {
+    if (!c) return MMHRT_E_INVALID;
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) return mmhrt::hip_fail(c, e, "hipSetDevice");
+    if (!c->dn) c->dn = new Denoiser();
+    if (need_ready && !c->dn->ready)
+        return mmhrt::fail(c, MMHRT_E_STATE, "denoiser: call mmhrt_denoiser_init or mmhrt_denoiser_set_params first");
+    return MMHRT_OK;
+}
+inline int ceil8(int v) { return (v + 7) & ~7; }
+
+// Features (already in dn->raw) -> network input -> forward -> composed colour in dn->color.
+int dn_run(mmhrt_ctx c, int B, int H, int W, float depth_scale, const float bg[4], const float* target, float* loss,
+           bool train) {
+    Denoiser& d = *c->dn;
+    int Hp = ceil8(H), Wp = ceil8(W);
+    if (int e = d.ensure_acts(c, B, Hp, Wp, train)) return e;
+    size_t P = (size_t)B * H * W, Pp = (size_t)B * Hp * Wp;
+    if (int e = mmhrt::ensure(c, d.color, P * 3, "denoiser colour")) return e;
+    hipStream_t s = c->stream;
+    float inv_depth = depth_scale > 0.0f ? 1.0f / depth_scale : 1.0f;
+    hipLaunchKernelGGL(mmhrt::dn::k_pack_input, dim3((unsigned)((Pp + 255) / 256)), dim3(256), 0, s, d.raw.p, B, H,
+                       W, Hp, Wp, inv_depth, d.x0);
+    d.forward(s, B, Hp, Wp);
+    float4 bgc = make_float4(bg ? bg[0] : 0.f, bg ? bg[1] : 0.f, bg ? bg[2] : 0.f, bg ? bg[3] : 1.f);
+    if (loss) mmhrt::launch_fill_u32(s, reinterpret_cast<uint32_t*>(d.scalar.p), 0u, 1);
+    mmhrt::dn::launch_compose(s, d.out.p, d.raw.p, B, H, W, Hp, Wp, bgc, target, 1.0f / (float)(P * 3), d.color.p,
+                              train ? d.gbuf.p : nullptr, loss ? d.scalar.p : nullptr);
+    return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int32_t mmhrt_denoiser_init(mmhrt_ctx c, uint64_t seed) {
+    if (int e = dn_check(c, false)) return e;
+    Denoiser& d = *c->dn;
+    if (int e = d.ensure_params(c)) return e;
+    std::vector<float> h(mmhrt::dn::param_count());
+    std::mt19937_64 rng(seed);
+    std::normal_distribution<float> nd(0.0f, 1.0f);
+    size_t o = 0;
+    for (int l = 0; l < kNumLayers; ++l) {
+        const auto& L = kLayers[l];
+        int CI = L.c1 + L.c2;
+        int fan_in = 9 * (l == 0 ? 12 : CI);
+        float sd = sqrtf(2.0f / (float)fan_in) * (l == kNumLayers - 1 ? 0.1f : 1.0f);
+        for (int t = 0; t < 9; ++t)
+            for (int ci = 0; ci < CI; ++ci)
+                for (int co = 0; co < L.co; ++co) {
+                    bool live = !(l == 0 && ci >= 12) && !(l == kNumLayers - 1 && co > 0);
+                    h[o++] = live ? sd * nd(rng) : 0.0f;
+                }
+        for (int co = 0; co < L.co; ++co) h[o++] = (l == kNumLayers - 1 && co == 0) ? 0.5f : 0.0f;
+    }
+    return mmhrt_denoiser_set_params(c, h.data(), h.size());
+}
+
+int32_t mmhrt_denoiser_param_count(mmhrt_ctx c, uint64_t* n) {
+    if (!n) return MMHRT_E_INVALID;
+    (void)c;
+    *n = mmhrt::dn::param_count();
+    return MMHRT_OK;
+}
+
+int32_t mmhrt_denoiser_get_params(mmhrt_ctx c, float* out, uint64_t n) {
+    if (int e = dn_check(c, true)) return e;
+    if (!out || n != mmhrt::dn::param_count()) return mmhrt::fail(c, MMHRT_E_INVALID, "denoiser_get_params: bad size");
+    hipError_t e = hipMemcpyAsync(out, c->dn->params.p, n * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    return e == hipSuccess ? MMHRT_OK : mmhrt::hip_fail(c, e, "denoiser_get_params");
+}
+
+int32_t mmhrt_denoiser_set_params(mmhrt_ctx c, const float* in, uint64_t n) {
+    if (int e = dn_check(c, false)) return e;
+    if (!in || n != mmhrt::dn::param_count()) return mmhrt::fail(c, MMHRT_E_INVALID, "denoiser_set_params: bad size");
+    Denoiser& d = *c->dn;
+    if (int e = d.ensure_params(c)) return e;
+    hipError_t e = hipMemcpyAsync(d.params.p, in, n * 4, hipMemcpyHostToDevice, c->stream);
+    if (e != hipSuccess) return mmhrt::hip_fail(c, e, "denoiser_set_params");
+    mmhrt::dn::zero(c->stream, d.adam_m.p, (long)n);
+    mmhrt::dn::zero(c->stream, d.adam_v.p, (long)n);
+    d.relayout(c->stream);
+    d.step = 0;
+    e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return mmhrt::hip_fail(c, e, "denoiser_set_params");
+    d.ready = true;
+    return MMHRT_OK;
+}
+
+int32_t mmhrt_denoise(mmhrt_ctx c, float depth_scale) {
+    if (int e = dn_check(c, true)) return e;
+    if (c->fb_w == 0 || !c->fb_aux.p || !c->last_aux)
+        return mmhrt::fail(c, MMHRT_E_STATE, "denoise: render a frame with MMHRT_FLAG_AUX first");
+    Denoiser& d = *c->dn;
+    int H = (int)c->fb_h, W = (int)c->fb_w;
+    size_t P = (size_t)H * W;
+    if (int e = mmhrt::ensure(c, d.raw, P * 12, "denoiser features")) return e;
+    hipLaunchKernelGGL(mmhrt::dn::k_frame_features, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, c->stream,
+                       c->fb_col.p, c->fb_aux.p, (long)P, d.raw.p);
+    if (int e = dn_run(c, 1, H, W, depth_scale, c->last_bg, nullptr, nullptr, false)) return e;
+    hipLaunchKernelGGL(mmhrt::dn::k_compose_frame, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, c->stream,
+                       d.color.p, (long)P, c->fb_col.p, c->fb_rgba8.p);
+    hipError_t he = hipGetLastError();
+    return he == hipSuccess ? MMHRT_OK : mmhrt::hip_fail(c, he, "denoise launch");
+}
+
+int32_t mmhrt_denoise_features(mmhrt_ctx c, const uint16_t* feats, uint32_t b, uint32_t h, uint32_t w,
+                               float depth_scale, const float bg[4], float* out_color) {
+    if (int e = dn_check(c, true)) return e;
+    if (!feats || !out_color || b == 0 || h == 0 || w == 0)
+        return mmhrt::fail(c, MMHRT_E_INVALID, "denoise_features: bad arguments");
+    Denoiser& d = *c->dn;
+    size_t P = (size_t)b * h * w;
+    if (int e = mmhrt::ensure(c, d.raw, P * 12, "denoiser features")) return e;
+    hipError_t he = hipMemcpyAsync(d.raw.p, feats, P * 24, hipMemcpyHostToDevice, c->stream);
+    if (he != hipSuccess) return mmhrt::hip_fail(c, he, "denoise_features upload");
+    if (int e = dn_run(c, (int)b, (int)h, (int)w, depth_scale, bg, nullptr, nullptr, false)) return e;
+    he = hipMemcpyAsync(out_color, d.color.p, P * 12, hipMemcpyDeviceToHost, c->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
+    return he == hipSuccess ? MMHRT_OK : mmhrt::hip_fail(c, he, "denoise_features");
+}
+
+int32_t mmhrt_denoiser_train_step(mmhrt_ctx c, const uint16_t* feats, const float* target, uint32_t b, uint32_t h,
+                                  uint32_t w, float depth_scale, const float bg[4], float lr, int32_t apply,
+                                  float* loss_out, float* grads_out) {
+    if (int e = dn_check(c, true)) return e;
+    if (!feats || !target || b == 0 || h == 0 || w == 0)
+        return mmhrt::fail(c, MMHRT_E_INVALID, "train_step: bad arguments");
+    Denoiser& d = *c->dn;
+    size_t P = (size_t)b * h * w;
+    if (int e = mmhrt::ensure(c, d.raw, P * 12, "denoiser features")) return e;
+    if (int e = mmhrt::ensure(c, d.target, P * 3, "denoiser target")) return e;
+    hipStream_t s = c->stream;
+    hipError_t he = hipMemcpyAsync(d.raw.p, feats, P * 24, hipMemcpyHostToDevice, s);
+    if (he == hipSuccess) he = hipMemcpyAsync(d.target.p, target, P * 12, hipMemcpyHostToDevice, s);
+    if (he != hipSuccess) return mmhrt::hip_fail(c, he, "train_step upload");
+    if (int e = dn_run(c, (int)b, (int)h, (int)w, depth_scale, bg, d.target.p, loss_out ? d.scalar.p : d.scalar.p, true))
+        return e;
+    size_t n = mmhrt::dn::param_count();
+    mmhrt::dn::zero(s, d.grads.p, (long)n);
+    d.backward(s, (int)b, ceil8((int)h), ceil8((int)w), c->cu_count);
+    if (grads_out) {
+        he = hipMemcpyAsync(grads_out, d.grads.p, n * 4, hipMemcpyDeviceToHost, s);
+        if (he != hipSuccess) return mmhrt::hip_fail(c, he, "train_step grads");
+    }
+    if (apply) {
+        d.step++;
+        const float b1 = 0.9f, b2 = 0.999f;
+        float c1 = 1.0f / (1.0f - powf(b1, (float)d.step)), c2 = 1.0f / (1.0f - powf(b2, (float)d.step));
+        mmhrt::dn::adam(s, d.params.p, d.grads.p, d.adam_m.p, d.adam_v.p, (long)n, lr, b1, b2, 1e-8f, c1, c2);
+        d.relayout(s);
+    }
+    if (loss_out) {
+        he = hipMemcpyAsync(loss_out, d.scalar.p, 4, hipMemcpyDeviceToHost, s);
+        if (he != hipSuccess) return mmhrt::hip_fail(c, he, "train_step loss");
+    }
+    he = hipStreamSynchronize(s);
+    return he == hipSuccess ? MMHRT_OK : mmhrt::hip_fail(c, he, "train_step");
+}
+
+}  // extern "C"
+
+namespace mmhrt {
+namespace dn {
+void Denoiser::release_all() {
+    mmhrt::release(params); mmhrt::release(grads); mmhrt::release(adam_m); mmhrt::release(adam_v);
+    mmhrt::release(scalar); mmhrt::release(wf); mmhrt::release(wb); mmhrt::release(acts); mmhrt::release(gbuf);
+    mmhrt::release(raw); mmhrt::release(out); mmhrt::release(target); mmhrt::release(color);
+}
+}  // namespace dn
+}  // namespace mmhrt

@@ -1,0 +1,80 @@
+// denoise.hpp — U-Net denoiser state and kernel argument blocks.
+#pragma once
+#include "ctx.hpp"
+
+namespace mmhrt {
+namespace dn {
+
+constexpr int kNumLayers = 15;
+
+struct LayerSpec {
+    const char* name;
+    int c1;    // input channels at the layer's resolution
+    int c2;    // input channels at half resolution (nearest-upsampled), 0 if none
+    int co;    // output channels
+    int relu;
+};
+extern const LayerSpec kLayers[kNumLayers];
+size_t param_count();
+void launch_compose(hipStream_t s, const float* out, const uint16_t* raw, int B, int H, int W, int Hp, int Wp,
+                    float4 bg, const float* target, float inv_count, float* color, uint16_t* dout, float* loss);
+void zero(hipStream_t s, float* p, long n);
+void adam(hipStream_t s, float* p, const float* g, float* m, float* v, long n, float lr, float b1, float b2, float eps,
+          float c1, float c2);
+
+struct ConvArgs {
+    const uint16_t* X1;
+    const uint16_t* X2;
+    int C1, C2, CO;
+    int B, H, W;
+    const uint16_t* Wt;
+    const float* bias;
+    uint16_t* Y;
+    float* Yf;
+    int relu;
+};
+struct ConvBwdArgs {
+    const uint16_t* dY;
+    const uint16_t* Y;
+    int relu;
+    int CI, CO;
+    int B, H, W;
+    const uint16_t* Wt;
+    uint16_t* dX;
+};
+struct ConvWgtArgs {
+    const uint16_t* X1;
+    const uint16_t* X2;
+    int C1, C2, CO;
+    int B, H, W;
+    const uint16_t* dY;
+    const uint16_t* Y;
+    int relu;
+    float* dW;
+};
+
+struct Denoiser {
+    bool ready = false;
+    int64_t step = 0;
+    DevBuf<float> params, grads, adam_m, adam_v, scalar;
+    DevBuf<uint16_t> wf, wb;   // bf16 weights in forward / backward fragment order
+    DevBuf<uint16_t> acts;     // activations
+    DevBuf<uint16_t> gbuf;     // backward temporaries
+    DevBuf<uint16_t> raw;      // raw per-pixel features (12 halfs)
+    DevBuf<float> out;         // network output, fp32 [B,Hp,Wp,16]
+    DevBuf<float> target;      // training target colour
+    DevBuf<float> color;       // composed colour [B,H,W,3]
+    uint16_t* x0 = nullptr;
+    uint16_t* a[kNumLayers] = {};
+    uint16_t* pool[3] = {};
+
+    int ensure_params(mmhrt_ctx_s* c);
+    int ensure_acts(mmhrt_ctx_s* c, int B, int Hp, int Wp, bool train);
+    void relayout(hipStream_t s);
+    void forward(hipStream_t s, int B, int Hp, int Wp);
+    void backward(hipStream_t s, int B, int Hp, int Wp, int cu);
+    void release_all();
+};
+
+}  // namespace dn
+}  // namespace mmhrt

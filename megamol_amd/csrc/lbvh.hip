@@ -11,6 +11,8 @@
 //   5. Karras (2012) hierarchy: one thread per internal node, index tie-break for equal keys
 //   6. bottom-up boxes: one thread per leaf climbs; the second arrival at a node (atomic
 //      counter) writes the node's box into its parent and the subtree height
+#include <algorithm>
+
 #include "ctx.hpp"
 
 namespace mmhrt {
@@ -75,16 +77,33 @@ __global__ __launch_bounds__(256) void k_reduce_partial(const float4* __restrict
     if (threadIdx.x < NRED) partial[(size_t)blockIdx.x * NRED + threadIdx.x] = red[threadIdx.x][0];
 }
 
-__global__ void k_reduce_final(const float* __restrict__ partial, int nblocks, SceneInfo* info, int bits) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ __launch_bounds__(256) void k_reduce_final(const float* __restrict__ partial, int nblocks,
+                                                      SceneInfo* info, int bits) {
+    __shared__ float red[NRED][256];
     float v[NRED];
-    for (int k = 0; k < NRED; ++k) v[k] = partial[k];
-    for (int b = 1; b < nblocks; ++b)
+    for (int k = 0; k < NRED; ++k) {
+        bool is_min = (k < 3) || (k >= 6 && k < 9);
+        v[k] = k == 12 ? 0.0f : (is_min ? INFINITY : -INFINITY);
+    }
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
         for (int k = 0; k < NRED; ++k) {
             float x = partial[(size_t)b * NRED + k];
             bool is_min = (k < 3) || (k >= 6 && k < 9);
             v[k] = is_min ? fminf(v[k], x) : fmaxf(v[k], x);
         }
+    for (int k = 0; k < NRED; ++k) red[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st)
+            for (int k = 0; k < NRED; ++k) {
+                float a = red[k][threadIdx.x], b = red[k][threadIdx.x + st];
+                bool is_min = (k < 3) || (k >= 6 && k < 9);
+                red[k][threadIdx.x] = is_min ? fminf(a, b) : fmaxf(a, b);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    for (int k = 0; k < NRED; ++k) v[k] = red[k][0];
     for (int a = 0; a < 3; ++a) {
         info->clo[a] = v[a];
         info->chi[a] = v[3 + a];
@@ -329,43 +348,193 @@ __device__ __forceinline__ void store_child_box(Node* nd, int slot, const float 
     for (int k = 0; k < 6; ++k) f[o + k] = b[k];
 }
 
-__global__ __launch_bounds__(256) void k_boxes(const float4* __restrict__ leaf_sph, int64_t n, Node* nodes,
-                                               const int32_t* __restrict__ parent, uint32_t* flags, SceneInfo* info) {
-    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    int64_t m = n - 1;
-    float4 s = leaf_sph[k];
-    float b[6] = {s.x - s.w, s.y - s.w, s.z - s.w, s.x + s.w, s.y + s.w, s.z + s.w};
-    int32_t child = ~(int32_t)k;
-    int32_t p = parent[m + k];
-    int h = 0;
-    while (p >= 0) {
-        Node* nd = nodes + p;
-        int4 dd = nd->d;
-        int slot = dd.x == child ? 0 : 1;
-        store_child_box(nd, slot, b);
-        __threadfence();
-        uint32_t prev = atomicAdd(flags + p, 1u);
-        if (prev == 0) return;
-        __threadfence();
-        // both child boxes are now visible; read them through L2
-        const volatile float* f = reinterpret_cast<const volatile float*>(nd);
-        float c[12];
+// Bottom-up boxes without cross-workgroup fences. A node's box is final once both of its child
+// slots are written; a device-scope fence per climb step (the classic single-kernel scheme)
+// would write back L2 on every step, so completion is propagated in rounds instead:
+//   init:   every internal node writes the boxes of its leaf children into its own slots and
+//           counts them in flags[i]; nodes with two leaf children form the first worklist
+//   rounds: each worklist node unions its two slots, writes the result into its parent's slot
+//           and bumps the parent's counter; the second arrival appends the parent to the next
+//           worklist. Kernel boundaries make the slots of earlier rounds visible.
+//   finish: once a worklist is small, one workgroup runs the remaining rounds itself with
+//           barriers between them (workgroup scope, no cache writeback).
+constexpr int BOX_ROUNDS = 48;      // kernel rounds before the single-workgroup finish
+constexpr uint32_t BOX_SMALL = 2048;  // worklist size handed to the single workgroup
+
+__device__ __forceinline__ void leaf_box(const float4 s, float b[6]) {
+    b[0] = s.x - s.w; b[1] = s.y - s.w; b[2] = s.z - s.w;
+    b[3] = s.x + s.w; b[4] = s.y + s.w; b[5] = s.z + s.w;
+}
+
+// Wave-aggregated append: one atomic per wavefront. All lanes of the wave must call it.
+__device__ __forceinline__ void wave_append(bool want, uint32_t v, uint32_t* list, uint32_t* cnt) {
+    uint64_t m = __ballot(want);
+    if (m == 0) return;
+    int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if ((int)__lane_id() == leader) base = atomicAdd(cnt, (uint32_t)__popcll(m));
+    base = __shfl(base, leader, 64);
+    if (want) list[base + (uint32_t)__popcll(m & lanemask_lt())] = v;
+}
+
+// Block-aggregated worklist append: a block gathers up to BOX_TILE entries in LDS, reserves
+// space with one global atomic and writes them out contiguously.
+constexpr int BOX_ITEMS = 16;
+constexpr int BOX_TILE = 256 * BOX_ITEMS;
+
+struct BoxAppend {
+    uint32_t* s_list;
+    uint32_t* s_n;
+    uint32_t* s_base;
+    __device__ __forceinline__ void begin() {
+        if (threadIdx.x == 0) *s_n = 0;
+        __syncthreads();
+    }
+    __device__ __forceinline__ void push(uint32_t v) { s_list[atomicAdd(s_n, 1u)] = v; }
+    __device__ __forceinline__ void flush(uint32_t* list, uint32_t* cnt) {
+        __syncthreads();
+        uint32_t n = *s_n;
+        if (threadIdx.x == 0 && n) *s_base = atomicAdd(cnt, n);
+        __syncthreads();
+        uint32_t base = *s_base;
+        for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) list[base + k] = s_list[k];
+        __syncthreads();
+    }
+};
+
+__global__ __launch_bounds__(256) void k_box_init(const float4* __restrict__ leaf_sph, int64_t m, Node* nodes,
+                                                  uint32_t* __restrict__ flags, uint32_t* __restrict__ list,
+                                                  uint32_t* cnt) {
+    __shared__ uint32_t s_list[BOX_TILE];
+    __shared__ uint32_t s_n, s_base;
+    BoxAppend ap{s_list, &s_n, &s_base};
+    for (int64_t tile = (int64_t)blockIdx.x * BOX_TILE; tile < m; tile += (int64_t)gridDim.x * BOX_TILE) {
+        ap.begin();
+        for (int j = 0; j < BOX_ITEMS; ++j) {
+            int64_t i = tile + (int64_t)j * 256 + threadIdx.x;
+            if (i >= m) break;
+            int4 d = nodes[i].d;
+            uint32_t nl = 0;
+            float b[6];
+            if (d.x < 0) {
+                leaf_box(leaf_sph[~d.x], b);
+                store_child_box(nodes + i, 0, b);
+                ++nl;
+            }
+            if (d.y < 0) {
+                leaf_box(leaf_sph[~d.y], b);
+                store_child_box(nodes + i, 1, b);
+                ++nl;
+            }
+            flags[i] = nl;
+            if (nl == 2) ap.push((uint32_t)i);
+        }
+        ap.flush(list, cnt + 1);
+    }
+}
+
+// Union the two slots of node p, record its height, hand the box to the parent.
+// Returns the parent when this arrival completes it, else -1.
+template <bool VOL>
+__device__ __forceinline__ int32_t box_step(Node* nodes, const int32_t* __restrict__ parent, uint32_t* flags,
+                                            int32_t p, SceneInfo* info) {
+    float c[12];
+    int4 d;
+    if constexpr (VOL) {
+        const volatile float* f = reinterpret_cast<const volatile float*>(nodes + p);
 #pragma unroll
         for (int q = 0; q < 12; ++q) c[q] = f[q];
-        for (int a = 0; a < 3; ++a) {
-            b[a] = fminf(c[a], c[6 + a]);
-            b[3 + a] = fmaxf(c[3 + a], c[9 + a]);
-        }
-        int hs = 0;
-        int32_t other = slot == 0 ? dd.y : dd.x;
-        if (other >= 0) hs = ((const volatile int*)(&nodes[other].d))[3];
-        h = 1 + (h > hs ? h : hs);
-        ((volatile int*)(&nd->d))[3] = h;
-        child = p;
-        p = parent[p];
+        const volatile int* di = reinterpret_cast<const volatile int*>(&nodes[p].d);
+        d = make_int4(di[0], di[1], di[2], di[3]);
+    } else {
+        const float* f = reinterpret_cast<const float*>(nodes + p);
+#pragma unroll
+        for (int q = 0; q < 12; ++q) c[q] = f[q];
+        d = nodes[p].d;
     }
-    info->height = (uint32_t)h;
+    float b[6];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        b[a] = fminf(c[a], c[6 + a]);
+        b[3 + a] = fmaxf(c[3 + a], c[9 + a]);
+    }
+    int h0 = 0, h1 = 0;
+    if (d.x >= 0) h0 = VOL ? ((const volatile int*)&nodes[d.x].d)[3] : nodes[d.x].d.w;
+    if (d.y >= 0) h1 = VOL ? ((const volatile int*)&nodes[d.y].d)[3] : nodes[d.y].d.w;
+    int h = 1 + (h0 > h1 ? h0 : h1);
+    reinterpret_cast<int*>(&nodes[p].d)[3] = h;
+    int32_t pp = parent[p];
+    if (pp < 0) {
+        info->height = (uint32_t)h;
+        return -1;
+    }
+    int first = VOL ? ((const volatile int*)&nodes[pp].d)[0] : nodes[pp].d.x;
+    store_child_box(nodes + pp, first == p ? 0 : 1, b);
+    uint32_t old = atomicAdd(flags + pp, 1u);
+    return old == 1u ? pp : -1;
+}
+
+__device__ void box_finish(Node* nodes, const int32_t* __restrict__ parent, uint32_t* flags, uint32_t* la,
+                           uint32_t* lb, uint32_t n_cur, uint32_t* cnt, SceneInfo* info) {
+    __shared__ uint32_t s_out;
+    uint32_t* lin = la;
+    uint32_t* lout = lb;
+    while (n_cur > 0) {
+        if (threadIdx.x == 0) s_out = 0;
+        __syncthreads();
+        for (uint32_t base = 0; base < n_cur; base += blockDim.x) {
+            uint32_t i = base + threadIdx.x;
+            if (i < n_cur) {
+                int32_t p = (int32_t)((const volatile uint32_t*)lin)[i];
+                int32_t pp = box_step<true>(nodes, parent, flags, p, info);
+                if (pp >= 0) lout[atomicAdd(&s_out, 1u)] = (uint32_t)pp;
+            }
+        }
+        __threadfence_block();
+        __syncthreads();
+        n_cur = s_out;
+        uint32_t* t = lin;
+        lin = lout;
+        lout = t;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) cnt[0] = 1u;
+}
+
+// Round r: worklist r lives in list[r & 1], its size in cnt[1 + r]; cnt[0] marks completion.
+__global__ __launch_bounds__(256) void k_box_round(Node* nodes, const int32_t* __restrict__ parent, uint32_t* flags,
+                                                   uint32_t* l0, uint32_t* l1, uint32_t* cnt, int r, SceneInfo* info) {
+    __shared__ uint32_t s_list[BOX_TILE];
+    __shared__ uint32_t s_n, s_base;
+    if (cnt[0]) return;
+    uint32_t n_in = cnt[1 + r];
+    uint32_t* lin = (r & 1) ? l1 : l0;
+    uint32_t* lout = (r & 1) ? l0 : l1;
+    if (n_in <= BOX_SMALL) {
+        if (blockIdx.x == 0) box_finish(nodes, parent, flags, lin, lout, n_in, cnt, info);
+        return;
+    }
+    BoxAppend ap{s_list, &s_n, &s_base};
+    for (uint32_t tile = blockIdx.x * BOX_TILE; tile < n_in; tile += gridDim.x * BOX_TILE) {
+        ap.begin();
+        for (int j = 0; j < BOX_ITEMS; ++j) {
+            uint32_t i = tile + j * 256 + threadIdx.x;
+            if (i >= n_in) break;
+            int32_t pp = box_step<false>(nodes, parent, flags, (int32_t)lin[i], info);
+            if (pp >= 0) ap.push((uint32_t)pp);
+        }
+        ap.flush(lout, cnt + 2 + r);
+    }
+}
+
+// Whatever is left after BOX_ROUNDS kernel rounds: one workgroup finishes it.
+__global__ __launch_bounds__(256) void k_box_tail(Node* nodes, const int32_t* __restrict__ parent, uint32_t* flags,
+                                                  uint32_t* l0, uint32_t* l1, uint32_t* cnt, SceneInfo* info) {
+    if (cnt[0]) return;
+    int r = BOX_ROUNDS;
+    uint32_t* lin = (r & 1) ? l1 : l0;
+    uint32_t* lout = (r & 1) ? l0 : l1;
+    box_finish(nodes, parent, flags, lin, lout, cnt[1 + r], cnt, info);
 }
 
 // Single-sphere scene: one node whose two children are the same leaf.
@@ -456,7 +625,7 @@ int reduce_scene(mmhrt_ctx_s* c) {
     if (int e = ensure(c, c->partial, (size_t)nb * NRED, "reduction partials")) return e;
     if (int e = ensure(c, c->info, 1, "scene info")) return e;
     hipLaunchKernelGGL(k_reduce_partial, dim3(nb), dim3(256), 0, c->stream, c->sph.p, c->n, c->partial.p);
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(64), 0, c->stream, c->partial.p, nb, c->info.p, c->bits);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, c->stream, c->partial.p, nb, c->info.p, c->bits);
     return 0;
 }
 
@@ -467,9 +636,17 @@ static int boxes(mmhrt_ctx_s* c) {
         hipLaunchKernelGGL(k_single, dim3(1), dim3(1), 0, s, c->leaf_sph.p, c->nodes.p, c->info.p);
         return 0;
     }
-    launch_fill_u32(s, c->flags.p, 0u, n - 1);
-    hipLaunchKernelGGL(k_boxes, dim3(blocks_for(n, 256)), dim3(256), 0, s, c->leaf_sph.p, (int64_t)n, c->nodes.p,
-                       c->parent.p, c->flags.p, c->info.p);
+    uint64_t m = n - 1;
+    if (int e = ensure(c, c->box_cnt, BOX_ROUNDS + 4, "box round counters")) return e;
+    launch_fill_u32(s, c->box_cnt.p, 0u, BOX_ROUNDS + 4);
+    unsigned ig = (unsigned)std::min<uint64_t>((m + BOX_TILE - 1) / BOX_TILE, 8192);
+    hipLaunchKernelGGL(k_box_init, dim3(ig), dim3(256), 0, s, c->leaf_sph.p, (int64_t)m, c->nodes.p,
+                       c->flags.p, c->va.p, c->box_cnt.p);
+    for (int r = 0; r < BOX_ROUNDS; ++r)
+        hipLaunchKernelGGL(k_box_round, dim3(1024), dim3(256), 0, s, c->nodes.p, c->parent.p, c->flags.p, c->va.p,
+                           c->vb.p, c->box_cnt.p, r, c->info.p);
+    hipLaunchKernelGGL(k_box_tail, dim3(1), dim3(256), 0, s, c->nodes.p, c->parent.p, c->flags.p, c->va.p, c->vb.p,
+                       c->box_cnt.p, c->info.p);
     return 0;
 }
 

@@ -57,6 +57,7 @@ struct FrameConsts {
     float bg[4];
     uint32_t width, height, spp, frame_idx;
     int32_t mode, accumulate, want_aux, max_bounces;
+    uint32_t win_x0, win_y0, win_w, win_h;  // rendered pixel window (the whole frame by default)
 };
 
 // Primary ray of one sample: raygen.cu:241-245 + camera.h:11-16.

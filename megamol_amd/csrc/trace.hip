@@ -222,7 +222,8 @@ __device__ __forceinline__ V3 random_in_unit_sphere(uint32_t& seed) {
 template <int S, int MODE, bool COUNT>
 __device__ void shade_pixel(const TraceArgs& A, uint32_t px, uint32_t py, float pad, LaneStack<S>& st, Count& cnt) {
     const FrameConsts& f = A.fc;
-    uint32_t pix = py * f.width + px;
+    uint32_t pix = py * f.width + px;  // full-frame index: the random stream of the pixel
+    uint32_t opix = (py - f.win_y0) * f.win_w + (px - f.win_x0);  // index in the output window
     uint32_t seed = tea16(pix, f.frame_idx);
     float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
     float depth = FLT_MAX;
@@ -323,7 +324,7 @@ __device__ void shade_pixel(const TraceArgs& A, uint32_t px, uint32_t py, float 
     col = make_float4(col.x / fs, col.y / fs, col.z / fs, col.w / fs);
     if (f.accumulate && f.frame_idx > 0) {
         // raygen.cu:254-260, lerp(a, b, t) = a + t * (b - a) (utils_device.h:152-154)
-        float4 old = A.out_col[pix];
+        float4 old = A.out_col[opix];
         float a = 1.0f / (float)(f.frame_idx + 1);
         col = make_float4(old.x + a * (col.x - old.x), old.y + a * (col.y - old.y), old.z + a * (col.z - old.z),
                           old.w + a * (col.w - old.w));
@@ -335,11 +336,11 @@ __device__ void shade_pixel(const TraceArgs& A, uint32_t px, uint32_t py, float 
         depth = 1.0f;
         col = make_float4(f.bg[0], f.bg[1], f.bg[2], f.bg[3]);
     }
-    A.out_col[pix] = col;
-    A.out_depth[pix] = depth;
-    A.out_t[pix] = best_t;
-    A.out_id[pix] = best_leaf >= 0 ? (int32_t)A.leaf_id[best_leaf] : -1;
-    A.out_rgba8[pix] = pack_rgba8(col);
+    A.out_col[opix] = col;
+    A.out_depth[opix] = depth;
+    A.out_t[opix] = best_t;
+    A.out_id[opix] = best_leaf >= 0 ? (int32_t)A.leaf_id[best_leaf] : -1;
+    A.out_rgba8[opix] = pack_rgba8(col);
     if (f.want_aux) {
         aux[7] = vis_sum / fs;
         uint32_t w[4];
@@ -347,7 +348,7 @@ __device__ void shade_pixel(const TraceArgs& A, uint32_t px, uint32_t py, float 
         for (int k = 0; k < 4; ++k)
             w[k] = (uint32_t)__half_as_ushort(__float2half(aux[2 * k])) |
                    ((uint32_t)__half_as_ushort(__float2half(aux[2 * k + 1])) << 16);
-        *reinterpret_cast<uint4*>(A.out_aux + (size_t)pix * 8) = make_uint4(w[0], w[1], w[2], w[3]);
+        *reinterpret_cast<uint4*>(A.out_aux + (size_t)opix * 8) = make_uint4(w[0], w[1], w[2], w[3]);
     }
 }
 
@@ -375,8 +376,9 @@ __global__ __launch_bounds__(TB) void k_trace(TraceArgs A) {
         }
         if (tile >= A.n_tiles) break;
         uint32_t tx = tile % A.tiles_x, ty = tile / A.tiles_x;
-        uint32_t px = tx * 8 + (lane & 7), py = ty * 8 + (lane >> 3);
-        if (px < A.fc.width && py < A.fc.height) shade_pixel<S, MODE, COUNT>(A, px, py, pad, st, cnt);
+        uint32_t px = A.fc.win_x0 + tx * 8 + (lane & 7), py = A.fc.win_y0 + ty * 8 + (lane >> 3);
+        if (px < A.fc.win_x0 + A.fc.win_w && py < A.fc.win_y0 + A.fc.win_h)
+            shade_pixel<S, MODE, COUNT>(A, px, py, pad, st, cnt);
     }
     if (COUNT) {
         atomicAdd(reinterpret_cast<unsigned long long*>(A.ctr + 2), (unsigned long long)cnt.visits);
@@ -412,7 +414,7 @@ void launch1(const TraceArgs& A, unsigned grid, bool steal, bool count, int mode
 }  // namespace
 
 int render_frame(mmhrt_ctx_s* c, const FrameConsts& fc) {
-    uint32_t W = fc.width, H = fc.height;
+    uint32_t W = fc.win_w, H = fc.win_h;  // output size = the rendered window
     size_t P = (size_t)W * H;
     int e = 0;
     if ((e = ensure(c, c->fb_col, P, "framebuffer colour"))) return e;

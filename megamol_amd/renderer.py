@@ -38,7 +38,9 @@ class ParticleList:
         s = L.SphereList()
         s.vert_type = self.vert_type
         s.col_type = self.col_type
-        vs = self.vert_stride or L.VERT_SIZE[self.vert_type]
+        # unknown types pass through unchanged: the library rejects them with E_INVALID
+        vsz = L.VERT_SIZE[self.vert_type] if 0 <= self.vert_type < len(L.VERT_SIZE) else 0
+        vs = self.vert_stride or vsz
         n = self.count if self.count is not None else (self.vert.nbytes - self.vert_offset) // max(vs, 1)
         s.vert = self.vert.ctypes.data + self.vert_offset if self.vert is not None else None
         s.col = (self.col.ctypes.data + self.col_offset) if self.col is not None else None

@@ -1,0 +1,83 @@
+"""GPU tests of the BF16 MFMA denoiser against the CPU reference network (oracle/unet_ref.py).
+
+Activations are bf16 on the GPU, so the comparisons carry a tolerance: relative L2 error of the
+output and of every layer's weight gradient below 4e-2 against a float32 reference that rounds
+its activations to bf16 at the same points, and cosine similarity of gradients above 0.995.
+"""
+import numpy as np
+import pytest
+
+from helpers import case
+from megamol_amd import denoiser as D
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+from oracle import unet_ref as R  # noqa: E402
+from test_denoiser_cpu import random_features  # noqa: E402
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+@pytest.fixture(scope="module")
+def dnz(gpu_renderer):
+    return D.Denoiser(gpu_renderer, seed=7)
+
+
+@pytest.mark.parametrize("shape", [(1, 16, 16), (2, 24, 40), (1, 37, 70)])
+def test_forward_matches_reference(dnz, shape):
+    case(f"denoiser forward {shape}")
+    b, h, w = shape
+    feats = random_features(b, h, w, seed=h)
+    p = dnz.get_params()
+    g = np.random.default_rng(1)
+    p = (p + g.normal(size=p.shape).astype(np.float32) * 0.02).astype(np.float32)  # non-trivial biases too
+    dnz.set_params(p)
+    out = dnz.denoise(feats, depth_scale=2.0)
+    ref = R.denoise(p, feats, 2.0, round_bf16=True)
+    assert out.shape == ref.shape
+    assert rel(out, ref) < 4e-2, rel(out, ref)
+
+
+def test_gradients_match_reference(dnz):
+    case("denoiser gradients")
+    b, h, w = 2, 32, 32
+    feats = random_features(b, h, w, seed=5)
+    target = np.random.default_rng(2).random((b, h, w, 3)).astype(np.float32)
+    p = dnz.get_params()
+    p = (p + np.random.default_rng(3).normal(size=p.shape).astype(np.float32) * 0.02).astype(np.float32)
+    dnz.set_params(p)
+    loss, g = dnz.train_step(feats, target, 2.0, lr=0.0, apply=False, want_grads=True)
+    rloss, rg, _ = R.loss_and_grads(p, feats, target, 2.0, "float32", round_bf16=True)
+    assert abs(loss - rloss) <= 2e-2 * rloss
+    o = 0
+    for name, c1, c2, co, _ in D.LAYERS:
+        n = 9 * (c1 + c2) * co + co
+        gg, rr = g[o:o + n], rg[o:o + n]
+        o += n
+        if np.linalg.norm(rr) == 0:
+            continue
+        cos = float(gg @ rr / (np.linalg.norm(gg) * np.linalg.norm(rr)))
+        assert rel(gg, rr) < 4e-2 and cos > 0.995, (name, rel(gg, rr), cos)
+    assert np.array_equal(dnz.get_params(), p), "apply=0 must not update the parameters"
+
+
+def test_training_reduces_loss_and_set_get_roundtrip(dnz, tmp_path):
+    case("denoiser training smoke")
+    dnz.set_params(D.Denoiser(dnz.r, seed=11).get_params())
+    feats = random_features(4, 32, 32, seed=9)
+    # a learnable target: colour = albedo * visibility of a smooth field
+    f = feats.astype(np.float32)
+    yy, xx = np.mgrid[0:32, 0:32] / 32.0
+    v = 0.5 + 0.4 * np.sin(6 * xx) * np.cos(5 * yy)
+    target = np.where(f[..., 9:10] > 0, f[..., 3:6] * v[None, ..., None], 0).astype(np.float32)
+    losses = [dnz.train_step(feats, target, 2.0, lr=2e-3) for _ in range(60)]
+    assert losses[-1] < 0.5 * losses[0], (losses[0], losses[-1])
+    path = tmp_path / "w.bin"
+    dnz.save(str(path), {"loss": losses[-1]})
+    p = dnz.get_params()
+    dnz.set_params(np.zeros_like(p))
+    meta = dnz.load(str(path))
+    assert meta["loss"] == losses[-1]
+    np.testing.assert_array_equal(dnz.get_params(), p)
