@@ -77,7 +77,21 @@ def build(verbose: bool = False, jobs: int | None = None) -> Path:
     tmp = LIB / "libmmhrt.so.tmp"
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)])
     os.replace(tmp, out)
+    build_module_test()
     return out
+
+
+def build_module_test() -> Path:
+    """The MegaMol renderer module compiled against the in-repo stand-ins of the MegaMol types,
+    with a driver that renders through it (host C++ only, links libmmhrt.so)."""
+    mod = PKG / "module"
+    outdir = mod / "_build"
+    outdir.mkdir(exist_ok=True)
+    exe = outdir / "module_test"
+    _run(["g++", "-std=c++17", "-O2", "-Wall", "-I", str(mod / "mmstub"), "-I", str(mod), "-I",
+          str(PKG.parent / "include"), str(mod / "module_test.cpp"), str(mod / "HIPSphereRenderer.cpp"), "-L",
+          str(LIB), "-lmmhrt", "-Wl,-rpath,$ORIGIN/../../lib", "-o", str(exe)])
+    return exe
 
 
 if __name__ == "__main__":

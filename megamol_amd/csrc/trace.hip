@@ -15,6 +15,8 @@
 // whatever the visiting order. AO rays stop at the first occluder (any hit).
 #include <float.h>
 
+#include <algorithm>
+
 #include "ctx.hpp"
 
 namespace mmhrt {
@@ -99,6 +101,106 @@ struct Count {
     uint64_t visits = 0, tests = 0, rays = 0;
 };
 
+// One node visit of a closest-hit ray (state: node, stack, t_best, best leaf). Returns true when
+// the ray is finished. A closest-hit trace is this step repeated from the root.
+template <int S, bool COUNT>
+__device__ __forceinline__ bool step_closest(const TraceArgs& A, const RayPre& r, float tmin, float tmax, float& t_best,
+                                             int& best, int32_t& node, LaneStack<S>& st, Count& cnt) {
+    const Node* nd = A.nodes + node;
+    float4 a = nd->a, b = nd->b, c = nd->c;
+    int4 dd = nd->d;
+    if (COUNT) cnt.visits++;
+    float tn0 = slab(r, a.x, a.y, a.z, a.w, b.x, b.y, tmin, t_best);
+    float tn1 = slab(r, b.z, b.w, c.x, c.y, c.z, c.w, tmin, t_best);
+    int32_t ch0 = dd.x, ch1 = dd.y;
+    if (tn0 != INFINITY && ch0 < 0) {
+        int leaf = ~ch0;
+        float t = sphere_t(r.o, r.d, A.leaf_sph[leaf], tmin, tmax);
+        if (COUNT) cnt.tests++;
+        if (t >= 0.0f) {
+            if (t < t_best) {
+                t_best = t;
+                best = leaf;
+            } else if (t == t_best && best >= 0 && A.leaf_id[leaf] < A.leaf_id[best]) {
+                best = leaf;
+            }
+        }
+        tn0 = INFINITY;
+    }
+    if (tn1 != INFINITY && ch1 < 0) {
+        int leaf = ~ch1;
+        float t = sphere_t(r.o, r.d, A.leaf_sph[leaf], tmin, tmax);
+        if (COUNT) cnt.tests++;
+        if (t >= 0.0f) {
+            if (t < t_best) {
+                t_best = t;
+                best = leaf;
+            } else if (t == t_best && best >= 0 && A.leaf_id[leaf] < A.leaf_id[best]) {
+                best = leaf;
+            }
+        }
+        tn1 = INFINITY;
+    }
+    bool h0 = tn0 <= t_best, h1 = tn1 <= t_best;
+    if (h0 && h1) {
+        bool first0 = tn0 <= tn1;
+        st.push(first0 ? ch1 : ch0);
+        node = first0 ? ch0 : ch1;
+    } else if (h0) {
+        node = ch0;
+    } else if (h1) {
+        node = ch1;
+    } else {
+        if (st.sp == 0) return true;
+        node = st.pop();
+    }
+    return false;
+}
+
+// One node visit of an any-hit ray ignoring leaf `skip`: 0 = continue, 1 = finished without a
+// hit, 2 = occluded.
+template <int S, bool COUNT>
+__device__ __forceinline__ int step_any(const TraceArgs& A, const RayPre& r, float tmin, float tmax, int skip,
+                                        int32_t& node, LaneStack<S>& st, Count& cnt) {
+    const Node* nd = A.nodes + node;
+    float4 a = nd->a, b = nd->b, c = nd->c;
+    int4 dd = nd->d;
+    if (COUNT) cnt.visits++;
+    float tn0 = slab(r, a.x, a.y, a.z, a.w, b.x, b.y, tmin, tmax);
+    float tn1 = slab(r, b.z, b.w, c.x, c.y, c.z, c.w, tmin, tmax);
+    int32_t ch0 = dd.x, ch1 = dd.y;
+    bool h0 = tn0 != INFINITY, h1 = tn1 != INFINITY;
+    if (h0 && ch0 < 0) {
+        int leaf = ~ch0;
+        if (leaf != skip) {
+            if (COUNT) cnt.tests++;
+            if (sphere_t(r.o, r.d, A.leaf_sph[leaf], tmin, tmax) >= 0.0f) return 2;
+        }
+        h0 = false;
+    }
+    if (h1 && ch1 < 0) {
+        int leaf = ~ch1;
+        if (leaf != skip) {
+            if (COUNT) cnt.tests++;
+            if (sphere_t(r.o, r.d, A.leaf_sph[leaf], tmin, tmax) >= 0.0f) return 2;
+        }
+        h1 = false;
+    }
+    if (h0 && h1) {
+        bool first0 = tn0 <= tn1;
+        st.push(first0 ? ch1 : ch0);
+        node = first0 ? ch0 : ch1;
+    } else if (h0) {
+        node = ch0;
+    } else if (h1) {
+        node = ch1;
+    } else {
+        if (st.sp == 0) return 1;
+        node = st.pop();
+    }
+    return 0;
+}
+
 // Closest hit: returns the leaf index of the nearest reported intersection (or -1) and its t.
 template <int S, bool COUNT>
 __device__ int trace_closest(const TraceArgs& A, const RayPre& r, float tmin, float tmax, float& t_best,
@@ -107,55 +209,7 @@ __device__ int trace_closest(const TraceArgs& A, const RayPre& r, float tmin, fl
     t_best = tmax;
     int32_t node = 0;
     st.sp = 0;
-    for (;;) {
-        const Node* nd = A.nodes + node;
-        float4 a = nd->a, b = nd->b, c = nd->c;
-        int4 dd = nd->d;
-        if (COUNT) cnt.visits++;
-        float tn0 = slab(r, a.x, a.y, a.z, a.w, b.x, b.y, tmin, t_best);
-        float tn1 = slab(r, b.z, b.w, c.x, c.y, c.z, c.w, tmin, t_best);
-        int32_t ch0 = dd.x, ch1 = dd.y;
-        if (tn0 != INFINITY && ch0 < 0) {
-            int leaf = ~ch0;
-            float t = sphere_t(r.o, r.d, A.leaf_sph[leaf], tmin, tmax);
-            if (COUNT) cnt.tests++;
-            if (t >= 0.0f) {
-                if (t < t_best) {
-                    t_best = t;
-                    best = leaf;
-                } else if (t == t_best && best >= 0 && A.leaf_id[leaf] < A.leaf_id[best]) {
-                    best = leaf;
-                }
-            }
-            tn0 = INFINITY;
-        }
-        if (tn1 != INFINITY && ch1 < 0) {
-            int leaf = ~ch1;
-            float t = sphere_t(r.o, r.d, A.leaf_sph[leaf], tmin, tmax);
-            if (COUNT) cnt.tests++;
-            if (t >= 0.0f) {
-                if (t < t_best) {
-                    t_best = t;
-                    best = leaf;
-                } else if (t == t_best && best >= 0 && A.leaf_id[leaf] < A.leaf_id[best]) {
-                    best = leaf;
-                }
-            }
-            tn1 = INFINITY;
-        }
-        bool h0 = tn0 <= t_best, h1 = tn1 <= t_best;
-        if (h0 && h1) {
-            bool first0 = tn0 <= tn1;
-            st.push(first0 ? ch1 : ch0);
-            node = first0 ? ch0 : ch1;
-        } else if (h0) {
-            node = ch0;
-        } else if (h1) {
-            node = ch1;
-        } else {
-            if (st.sp == 0) break;
-            node = st.pop();
-        }
+    while (!step_closest<S, COUNT>(A, r, tmin, tmax, t_best, best, node, st, cnt)) {
     }
     return best;
 }
@@ -166,44 +220,10 @@ __device__ bool trace_any(const TraceArgs& A, const RayPre& r, float tmin, float
                           Count& cnt) {
     int32_t node = 0;
     st.sp = 0;
-    for (;;) {
-        const Node* nd = A.nodes + node;
-        float4 a = nd->a, b = nd->b, c = nd->c;
-        int4 dd = nd->d;
-        if (COUNT) cnt.visits++;
-        float tn0 = slab(r, a.x, a.y, a.z, a.w, b.x, b.y, tmin, tmax);
-        float tn1 = slab(r, b.z, b.w, c.x, c.y, c.z, c.w, tmin, tmax);
-        int32_t ch0 = dd.x, ch1 = dd.y;
-        bool h0 = tn0 != INFINITY, h1 = tn1 != INFINITY;
-        if (h0 && ch0 < 0) {
-            int leaf = ~ch0;
-            if (leaf != skip) {
-                if (COUNT) cnt.tests++;
-                if (sphere_t(r.o, r.d, A.leaf_sph[leaf], tmin, tmax) >= 0.0f) return true;
-            }
-            h0 = false;
-        }
-        if (h1 && ch1 < 0) {
-            int leaf = ~ch1;
-            if (leaf != skip) {
-                if (COUNT) cnt.tests++;
-                if (sphere_t(r.o, r.d, A.leaf_sph[leaf], tmin, tmax) >= 0.0f) return true;
-            }
-            h1 = false;
-        }
-        if (h0 && h1) {
-            bool first0 = tn0 <= tn1;
-            st.push(first0 ? ch1 : ch0);
-            node = first0 ? ch0 : ch1;
-        } else if (h0) {
-            node = ch0;
-        } else if (h1) {
-            node = ch1;
-        } else {
-            if (st.sp == 0) return false;
-            node = st.pop();
-        }
+    int res;
+    while ((res = step_any<S, COUNT>(A, r, tmin, tmax, skip, node, st, cnt)) == 0) {
     }
+    return res == 2;
 }
 
 // Diffuse bounce direction of raygen.cu:71-77,149-152: rejection sample in the unit cube,
@@ -217,6 +237,43 @@ __device__ __forceinline__ V3 random_in_unit_sphere(uint32_t& seed) {
         p = V3{2.0f * rx - 1.0f, 2.0f * ry - 1.0f, 2.0f * rz - 1.0f};
     } while (dot(p, p) >= 1.0f);
     return p;
+}
+
+// Pixel epilogue of raygen.cu:250-272: average over samples, progressive blend, pseudo depth
+// (background on a pixel that no sample hit), then every output buffer.
+__device__ __forceinline__ void write_pixel(const TraceArgs& A, uint32_t opix, float4 col, float depth, float best_t,
+                                            int best_leaf, float aux[8], float vis_sum) {
+    const FrameConsts& f = A.fc;
+    float fs = (float)f.spp;
+    col = make_float4(col.x / fs, col.y / fs, col.z / fs, col.w / fs);
+    if (f.accumulate && f.frame_idx > 0) {
+        // raygen.cu:254-260, lerp(a, b, t) = a + t * (b - a) (utils_device.h:152-154)
+        float4 old = A.out_col[opix];
+        float a = 1.0f / (float)(f.frame_idx + 1);
+        col = make_float4(old.x + a * (col.x - old.x), old.y + a * (col.y - old.y), old.z + a * (col.z - old.z),
+                          old.w + a * (col.w - old.w));
+    }
+    if (depth < FLT_MAX) {
+        depth = (f.depth_c / depth) - f.depth_a;
+        depth = 0.5f * (depth + 1.0f);
+    } else {
+        depth = 1.0f;
+        col = make_float4(f.bg[0], f.bg[1], f.bg[2], f.bg[3]);
+    }
+    A.out_col[opix] = col;
+    A.out_depth[opix] = depth;
+    A.out_t[opix] = best_t;
+    A.out_id[opix] = best_leaf >= 0 ? (int32_t)A.leaf_id[best_leaf] : -1;
+    A.out_rgba8[opix] = pack_rgba8(col);
+    if (f.want_aux) {
+        aux[7] = vis_sum / fs;
+        uint32_t w[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            w[k] = (uint32_t)__half_as_ushort(__float2half(aux[2 * k])) |
+                   ((uint32_t)__half_as_ushort(__float2half(aux[2 * k + 1])) << 16);
+        *reinterpret_cast<uint4*>(A.out_aux + (size_t)opix * 8) = make_uint4(w[0], w[1], w[2], w[3]);
+    }
 }
 
 template <int S, int MODE, bool COUNT>
@@ -320,36 +377,7 @@ __device__ void shade_pixel(const TraceArgs& A, uint32_t px, uint32_t py, float 
         col.w += sc.w;
         depth = fminf(depth, ray_depth);
     }
-    float fs = (float)f.spp;
-    col = make_float4(col.x / fs, col.y / fs, col.z / fs, col.w / fs);
-    if (f.accumulate && f.frame_idx > 0) {
-        // raygen.cu:254-260, lerp(a, b, t) = a + t * (b - a) (utils_device.h:152-154)
-        float4 old = A.out_col[opix];
-        float a = 1.0f / (float)(f.frame_idx + 1);
-        col = make_float4(old.x + a * (col.x - old.x), old.y + a * (col.y - old.y), old.z + a * (col.z - old.z),
-                          old.w + a * (col.w - old.w));
-    }
-    if (depth < FLT_MAX) {
-        depth = (f.depth_c / depth) - f.depth_a;
-        depth = 0.5f * (depth + 1.0f);
-    } else {
-        depth = 1.0f;
-        col = make_float4(f.bg[0], f.bg[1], f.bg[2], f.bg[3]);
-    }
-    A.out_col[opix] = col;
-    A.out_depth[opix] = depth;
-    A.out_t[opix] = best_t;
-    A.out_id[opix] = best_leaf >= 0 ? (int32_t)A.leaf_id[best_leaf] : -1;
-    A.out_rgba8[opix] = pack_rgba8(col);
-    if (f.want_aux) {
-        aux[7] = vis_sum / fs;
-        uint32_t w[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-            w[k] = (uint32_t)__half_as_ushort(__float2half(aux[2 * k])) |
-                   ((uint32_t)__half_as_ushort(__float2half(aux[2 * k + 1])) << 16);
-        *reinterpret_cast<uint4*>(A.out_aux + (size_t)opix * 8) = make_uint4(w[0], w[1], w[2], w[3]);
-    }
+    write_pixel(A, opix, col, depth, best_t, best_leaf, aux, vis_sum);
 }
 
 template <int S, int MODE, bool COUNT, bool STEAL>
@@ -387,6 +415,182 @@ __global__ __launch_bounds__(TB) void k_trace(TraceArgs A) {
     }
 }
 
+// Persistent traversal with wavefront-ballot compaction and refill (MODE 0 Lambert, 1 AO).
+// Each lane owns one pixel at a time and walks its samples in order (primary ray, then for AO
+// one occlusion ray). The inner loop advances every active lane by one node visit and keeps
+// going while at least `refill` lanes of the wave still trace; below that the wave leaves the
+// loop, finished lanes shade and set up their next ray (next AO ray, next sample, or a new pixel
+// taken with one wave-aggregated atomic), and traversal resumes. Per-pixel arithmetic and
+// random streams are those of k_trace, so the results are bit-identical.
+enum : int { L_IDLE = 0, L_NEED = 1, L_PRIM = 2, L_AO = 3, L_PRIM_DONE = 4, L_AO_DONE = 5, L_OFF = 6 };
+
+template <int S, int MODE, bool COUNT>
+__global__ __launch_bounds__(TB) void k_trace_refill(TraceArgs A, uint32_t refill) {
+    __shared__ int32_t lds_stack[S * TB];
+    const uint32_t gtid = blockIdx.x * TB + threadIdx.x;
+    LaneStack<S> st{lds_stack + threadIdx.x, A.spill + gtid, A.spill_stride, 0, A.ctr + 1};
+    Count cnt;
+    const FrameConsts& f = A.fc;
+    const float amax = A.info->absmax;
+    const float pad = (fmaxf(f.pad, amax) + amax) * (1.0f / 131072.0f);
+    const uint32_t lane = __lane_id();
+    const uint32_t tiles_x = (f.win_w + 7) / 8;
+    const uint32_t total_work = ((f.win_h + 7) / 8) * tiles_x * 64u;  // work indices: whole 8x8 tiles
+    int state = L_IDLE;
+    bool exhausted = false;
+    // pixel state
+    uint32_t px = 0, py = 0, pix = 0, opix = 0, s = 0, seed = 0;
+    float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
+    float depth = FLT_MAX, best_t = INFINITY, vis_sum = 0.0f;
+    int best_leaf_px = -1;
+    float aux[8];
+    // ray state
+    RayPre r;
+    float tmin = 0.f, tmax = 0.f, t_best = 0.f;
+    int best = -1, skip = -1;
+    int32_t node = 0;
+    float4 alb = make_float4(0.f, 0.f, 0.f, 0.f);
+    float prim_t = 0.f;
+    int prim_leaf = -1;
+    for (;;) {
+        // 1. idle lanes take pixels (work index -> 8x8 tile order inside the window)
+        uint64_t idle = __ballot(state == L_IDLE);
+        if (idle && !exhausted) {
+            int leader = __ffsll((unsigned long long)idle) - 1;
+            uint32_t base = 0;
+            if ((int)lane == leader) base = atomicAdd(A.ctr, (uint32_t)__popcll(idle));
+            base = __shfl(base, leader, 64);
+            if (base + (uint32_t)__popcll(idle) >= total_work) exhausted = true;
+            if (state == L_IDLE) {
+                uint32_t w = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+                uint32_t tile = w >> 6, k = w & 63u;
+                uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
+                uint32_t lx = tx * 8 + (k & 7), ly = ty * 8 + (k >> 3);
+                // tiles past the window edge map to no pixel; the lane simply asks again
+                if (w < total_work && lx < f.win_w && ly < f.win_h) {
+                    px = f.win_x0 + lx;
+                    py = f.win_y0 + ly;
+                    pix = py * f.width + px;
+                    opix = ly * f.win_w + lx;
+                    s = 0;
+                    seed = tea16(pix, f.frame_idx);
+                    col = make_float4(0.f, 0.f, 0.f, 0.f);
+                    depth = FLT_MAX;
+                    best_t = INFINITY;
+                    best_leaf_px = -1;
+                    vis_sum = 0.0f;
+#pragma unroll
+                    for (int k2 = 0; k2 < 8; ++k2) aux[k2] = 0.f;
+                    state = L_NEED;
+                } else if (w >= total_work) {
+                    state = L_OFF;
+                }
+            }
+        }
+        if (exhausted && state == L_IDLE) state = L_OFF;
+        // 2. new primary rays
+        if (state == L_NEED) {
+            float jx = rnd(seed);
+            float jy = rnd(seed);
+            V3 o, d;
+            primary_ray(f, px, py, jx, jy, o, d);
+            r = prep(o, d, pad);
+            tmin = 1e-6f;
+            tmax = 1e20f;
+            t_best = tmax;
+            best = -1;
+            node = 0;
+            st.sp = 0;
+            if (COUNT) cnt.rays++;
+            state = L_PRIM;
+        }
+        uint64_t act = __ballot(state == L_PRIM || state == L_AO);
+        uint64_t live = __ballot(state != L_OFF);
+        if (!live) break;
+        // 3. traversal: one node visit per active lane per iteration
+        while (act) {
+            if (state == L_PRIM) {
+                if (step_closest<S, COUNT>(A, r, tmin, tmax, t_best, best, node, st, cnt)) state = L_PRIM_DONE;
+            } else if (state == L_AO) {
+                int res = step_any<S, COUNT>(A, r, tmin, tmax, skip, node, st, cnt);
+                if (res) {
+                    state = L_AO_DONE;
+                    best = res;  // 2 = occluded
+                }
+            }
+            act = __ballot(state == L_PRIM || state == L_AO);
+            if ((uint32_t)__popcll(act) < refill) break;
+        }
+        // 4. shading of finished rays
+        bool sample_done = false;
+        if (state == L_PRIM_DONE) {
+            if (best < 0) {
+                float4 sc = MODE == 1 ? make_float4(f.bg[0], f.bg[1], f.bg[2], f.bg[3])
+                                      : make_float4(1.0f * 0.8f, 1.0f * 0.8f, 1.0f * 0.8f, 1.0f);
+                col.x += sc.x; col.y += sc.y; col.z += sc.z; col.w += sc.w;
+                sample_done = true;
+            } else {
+                float t = t_best;
+                float4 sp = A.leaf_sph[best];
+                V3 o = r.o, d = r.d;
+                V3 Pp = V3{o.x + t * d.x, o.y + t * d.y, o.z + t * d.z};
+                V3 N = V3{Pp.x - sp.x, Pp.y - sp.y, Pp.z - sp.z};
+                if (dot(N, d) > 0.0f) N = neg(N);
+                N = normalize(N);
+                alb = A.col[A.leaf_id[best]];
+                depth = fminf(depth, t);
+                if (t < best_t) {
+                    best_t = t;
+                    best_leaf_px = best;
+                }
+                if (s == 0) {
+                    aux[0] = alb.x; aux[1] = alb.y; aux[2] = alb.z;
+                    aux[3] = N.x; aux[4] = N.y; aux[5] = N.z;
+                    aux[6] = t;
+                }
+                if (MODE == 1) {
+                    uint32_t aseed = tea16(pix, 0xA0000000u + f.frame_idx * 4096u + s);
+                    V3 ad = ao_direction(N, aseed);
+                    prim_t = t;
+                    prim_leaf = best;
+                    r = prep(Pp, ad, pad);
+                    tmin = 0.0f;
+                    tmax = f.ao_distance;
+                    skip = best;
+                    node = 0;
+                    st.sp = 0;
+                    if (COUNT) cnt.rays++;
+                    state = L_AO;
+                } else {
+                    float sh = 0.2f + 0.6f * fabsf(dot(N, d));
+                    col.x += alb.x * sh; col.y += alb.y * sh; col.z += alb.z * sh; col.w += 1.0f;
+                    sample_done = true;
+                }
+            }
+        } else if (state == L_AO_DONE) {
+            float V = best == 2 ? 0.0f : 1.0f;
+            col.x += alb.x * V; col.y += alb.y * V; col.z += alb.z * V; col.w += 1.0f;
+            vis_sum += V;
+            sample_done = true;
+            (void)prim_t;
+            (void)prim_leaf;
+        }
+        if (sample_done) {
+            if (++s < f.spp) {
+                state = L_NEED;
+            } else {
+                write_pixel(A, opix, col, depth, best_t, best_leaf_px, aux, vis_sum);
+                state = L_IDLE;
+            }
+        }
+    }
+    if (COUNT) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(A.ctr + 2), (unsigned long long)cnt.visits);
+        atomicAdd(reinterpret_cast<unsigned long long*>(A.ctr + 4), (unsigned long long)cnt.tests);
+        atomicAdd(reinterpret_cast<unsigned long long*>(A.ctr + 6), (unsigned long long)cnt.rays);
+    }
+}
+
 template <int S, int MODE, bool COUNT>
 void launch3(const TraceArgs& A, unsigned grid, bool steal, hipStream_t s) {
     if (steal)
@@ -409,6 +613,21 @@ void launch1(const TraceArgs& A, unsigned grid, bool steal, bool count, int mode
         launch2<S, 2>(A, grid, steal, count, s);
     else
         launch2<S, 0>(A, grid, steal, count, s);
+}
+
+template <int S>
+void launch_refill(const TraceArgs& A, unsigned grid, uint32_t refill, bool count, int mode, hipStream_t s) {
+    if (mode == 1) {
+        if (count)
+            hipLaunchKernelGGL((k_trace_refill<S, 1, true>), dim3(grid), dim3(TB), 0, s, A, refill);
+        else
+            hipLaunchKernelGGL((k_trace_refill<S, 1, false>), dim3(grid), dim3(TB), 0, s, A, refill);
+    } else {
+        if (count)
+            hipLaunchKernelGGL((k_trace_refill<S, 0, true>), dim3(grid), dim3(TB), 0, s, A, refill);
+        else
+            hipLaunchKernelGGL((k_trace_refill<S, 0, false>), dim3(grid), dim3(TB), 0, s, A, refill);
+    }
 }
 
 }  // namespace
@@ -454,11 +673,21 @@ int render_frame(mmhrt_ctx_s* c, const FrameConsts& fc) {
     hipEventRecord(c->tm.ev[6], c->stream);
     bool steal = c->opt_persistent != 0;
     bool count = c->opt_count != 0;
-    switch (S) {
-    case 8: launch1<8>(A, grid, steal, count, fc.mode, c->stream); break;
-    case 24: launch1<24>(A, grid, steal, count, fc.mode, c->stream); break;
-    case 32: launch1<32>(A, grid, steal, count, fc.mode, c->stream); break;
-    default: launch1<16>(A, grid, steal, count, fc.mode, c->stream); break;
+    uint32_t refill = (uint32_t)std::max<int64_t>(0, std::min<int64_t>(64, c->opt_refill));
+    if (refill > 0 && fc.mode != 2) {
+        switch (S) {
+        case 8: launch_refill<8>(A, grid, refill, count, fc.mode, c->stream); break;
+        case 24: launch_refill<24>(A, grid, refill, count, fc.mode, c->stream); break;
+        case 32: launch_refill<32>(A, grid, refill, count, fc.mode, c->stream); break;
+        default: launch_refill<16>(A, grid, refill, count, fc.mode, c->stream); break;
+        }
+    } else {
+        switch (S) {
+        case 8: launch1<8>(A, grid, steal, count, fc.mode, c->stream); break;
+        case 24: launch1<24>(A, grid, steal, count, fc.mode, c->stream); break;
+        case 32: launch1<32>(A, grid, steal, count, fc.mode, c->stream); break;
+        default: launch1<16>(A, grid, steal, count, fc.mode, c->stream); break;
+        }
     }
     hipEventRecord(c->tm.ev[7], c->stream);
     hipError_t he = hipGetLastError();

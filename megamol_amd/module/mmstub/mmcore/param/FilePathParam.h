@@ -1,0 +1,6 @@
+#pragma once
+#include <filesystem>
+#include "mmcore/param/ParamSlot.h"
+namespace megamol::core::param {
+class FilePathParam : public AbstractParam { public: explicit FilePathParam(const std::string& v) : v_(v) {} std::filesystem::path Value() const { return v_; } void SetValue(const std::string& v) { v_ = v; } private: std::filesystem::path v_; };
+}
