@@ -202,6 +202,29 @@ MMHRT_API int32_t mmhrt_get_timings(mmhrt_ctx ctx, float* times_ms, int32_t n);
  * "stack_overflow" (1 if a traversal stack ever exceeded its capacity; 0 expected). */
 MMHRT_API int32_t mmhrt_get_counters(mmhrt_ctx ctx, uint64_t* node_visits, uint64_t* sphere_tests, uint64_t* rays);
 
+/* ---- batched time series ---------------------------------------------------
+ * No single counterpart in the reference, which renders one frame per Render call and rebuilds
+ * its BVH on every frame change (SphereGeometry.cpp:252-257). All timesteps stay resident; per
+ * timestep the LBVH is rebuilt (t % refit_period == 0) or refitted, then all cameras of that
+ * timestep are traced by one persistent launch; use_graph captures the whole sequence into one
+ * hipGraph and launches it once. */
+typedef struct mmhrt_job {
+    uint32_t timestep;
+    uint32_t reserved;
+    mmhrt_camera camera;
+    mmhrt_render_params params; /* mode Lambert or AO; window honoured; no accumulation */
+    uint64_t out_offset;        /* first pixel of this job in the batch outputs */
+} mmhrt_job;
+/* Upload `timesteps` x `count` packed float4 xyzr spheres (timestep-major) and one colour per
+ * sphere shared by all timesteps (rgba NULL: global_rgba / 255). */
+MMHRT_API int32_t mmhrt_set_timesteps(mmhrt_ctx ctx, const float* xyzr, uint32_t timesteps, uint64_t count,
+                                      const float* rgba, const uint8_t global_rgba[4]);
+/* Render all jobs; rgba8_out / depth_out (NULL to skip) receive the concatenated job outputs;
+ * ms_out the device time of the whole sequence (graph launch or eager issue). */
+MMHRT_API int32_t mmhrt_render_batch(mmhrt_ctx ctx, const mmhrt_job* jobs, uint32_t n_jobs, int32_t refit_period,
+                                     int32_t morton_bits, int32_t use_graph, uint32_t* rgba8_out, float* depth_out,
+                                     float* ms_out);
+
 /* ---- learned denoiser ------------------------------------------------------
  * No counterpart in the reference (optix_hpg has no denoiser). A 4-level U-Net (3x3 convs,
  * 32/48/64/96 channels, ReLU, skip concatenation) run and trained with BF16 MFMA kernels that

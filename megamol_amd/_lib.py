@@ -57,6 +57,11 @@ class Framebuffer(C.Structure):
     ]
 
 
+class Job(C.Structure):
+    _fields_ = [("timestep", C.c_uint32), ("reserved", C.c_uint32), ("camera", Camera), ("params", RenderParams),
+                ("out_offset", C.c_uint64)]
+
+
 # name -> (restype, argtypes); must match include/mmhrt.h exactly
 SIGNATURES = {
     "mmhrt_create": (C.c_int32, [C.c_int32, C.POINTER(C.c_void_p)]),
@@ -74,6 +79,9 @@ SIGNATURES = {
     "mmhrt_get_timings": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32]),
     "mmhrt_get_counters": (C.c_int32, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                        C.POINTER(C.c_uint64)]),
+    "mmhrt_set_timesteps": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint64, C.c_void_p, C.c_void_p]),
+    "mmhrt_render_batch": (C.c_int32, [C.c_void_p, C.POINTER(Job), C.c_uint32, C.c_int32, C.c_int32, C.c_int32,
+                                       C.c_void_p, C.c_void_p, C.POINTER(C.c_float)]),
     "mmhrt_denoiser_init": (C.c_int32, [C.c_void_p, C.c_uint64]),
     "mmhrt_denoiser_param_count": (C.c_int32, [C.c_void_p, C.POINTER(C.c_uint64)]),
     "mmhrt_denoiser_get_params": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint64]),

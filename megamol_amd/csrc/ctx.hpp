@@ -107,6 +107,18 @@ struct mmhrt_ctx_s {
     int64_t refits_since_build = 0;
     int64_t stack_overflow = 0;  // set by mmhrt_get_counters from the trace kernel's flag
 
+    // batched time series: all timesteps resident, one hipGraph for the whole sequence
+    mmhrt::DevBuf<float4> ts_sph;      // timesteps x n spheres
+    uint32_t ts_count = 0;
+    uint64_t ts_n = 0;
+    const float4* sph_cur = nullptr;   // spheres the next build reads (a timestep slice), null = sph
+    bool capturing = false;            // stream capture in progress: no event records
+    mmhrt::DevBuf<float4> bt_col;
+    mmhrt::DevBuf<float> bt_depth, bt_t;
+    mmhrt::DevBuf<int32_t> bt_id;
+    mmhrt::DevBuf<uint32_t> bt_rgba8, batch_ctr, batch_prefix;
+    mmhrt::DevBuf<uint8_t> batch_jobs;
+
     // learned denoiser (created on first use)
     mmhrt::dn::Denoiser* dn = nullptr;
     bool last_aux = false;
@@ -172,4 +184,16 @@ int reduce_scene(mmhrt_ctx_s* c);
 
 // trace.hip
 int render_frame(mmhrt_ctx_s* c, const FrameConsts& fc);
+uint32_t tile_log2(int64_t tw);
+size_t batch_job_size();
+int batch_fill_job(mmhrt_ctx_s* c, const FrameConsts& fc, size_t out_offset, void* dst);
+void launch_trace_batch(mmhrt_ctx_s* c, const void* jobs, const uint32_t* prefix, uint32_t n_jobs, uint32_t* ctr,
+                        int mode);
+
+// event record on the context stream, skipped while the stream is being captured into a graph
+inline void rec(mmhrt_ctx_s* c, int i) {
+    if (!c->capturing) (void)hipEventRecord(c->tm.ev[i], c->stream);
+}
+// spheres the build reads: the current timestep slice, or the context's spheres
+inline const float4* cur_spheres(const mmhrt_ctx_s* c) { return c->sph_cur ? c->sph_cur : c->sph.p; }
 }  // namespace mmhrt

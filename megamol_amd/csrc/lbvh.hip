@@ -595,19 +595,19 @@ template <typename K>
 int build_typed(mmhrt_ctx_s* c, K* ka, K* kb) {
     uint64_t n = c->n;
     hipStream_t s = c->stream;
-    hipEventRecord(c->tm.ev[1], s);
-    hipLaunchKernelGGL(k_morton<K>, dim3(blocks_for(n, 256)), dim3(256), 0, s, c->sph.p, n, c->info.p, ka, c->va.p);
-    hipEventRecord(c->tm.ev[2], s);
+    rec(c, 1);
+    hipLaunchKernelGGL(k_morton<K>, dim3(blocks_for(n, 256)), dim3(256), 0, s, cur_spheres(c), n, c->info.p, ka, c->va.p);
+    rec(c, 2);
     int passes = sizeof(K) == 8 ? 8 : 4;
     if (int e = sort_pairs<K>(c, ka, kb, c->va.p, c->vb.p, n, passes)) return e;
-    hipEventRecord(c->tm.ev[3], s);
-    hipLaunchKernelGGL(k_gather<K>, dim3(blocks_for(n, 256)), dim3(256), 0, s, ka, c->va.p, c->sph.p, c->leaf_sph.p,
+    rec(c, 3);
+    hipLaunchKernelGGL(k_gather<K>, dim3(blocks_for(n, 256)), dim3(256), 0, s, ka, c->va.p, cur_spheres(c), c->leaf_sph.p,
                        c->leaf_id.p, c->keys64.p, n);
     if (n > 1) {
         hipLaunchKernelGGL(k_karras<K>, dim3(blocks_for(n - 1, 256)), dim3(256), 0, s, ka, (int64_t)n, c->nodes.p,
                            c->parent.p);
     }
-    hipEventRecord(c->tm.ev[4], s);
+    rec(c, 4);
     return 0;
 }
 
@@ -624,7 +624,7 @@ int reduce_scene(mmhrt_ctx_s* c) {
     const int nb = 1024;
     if (int e = ensure(c, c->partial, (size_t)nb * NRED, "reduction partials")) return e;
     if (int e = ensure(c, c->info, 1, "scene info")) return e;
-    hipLaunchKernelGGL(k_reduce_partial, dim3(nb), dim3(256), 0, c->stream, c->sph.p, c->n, c->partial.p);
+    hipLaunchKernelGGL(k_reduce_partial, dim3(nb), dim3(256), 0, c->stream, cur_spheres(c), c->n, c->partial.p);
     hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, c->stream, c->partial.p, nb, c->info.p, c->bits);
     return 0;
 }
@@ -672,7 +672,7 @@ int build_lbvh(mmhrt_ctx_s* c, int bits) {
         if ((e = ensure(c, c->ka, n, "keys"))) return e;
         if ((e = ensure(c, c->kb, n, "keys"))) return e;
     }
-    hipEventRecord(c->tm.ev[0], c->stream);
+    rec(c, 0);
     if ((e = reduce_scene(c))) return e;
     if (bits == 63)
         e = build_typed<uint64_t>(c, c->kla.p, c->klb.p);
@@ -680,7 +680,7 @@ int build_lbvh(mmhrt_ctx_s* c, int bits) {
         e = build_typed<uint32_t>(c, c->ka.p, c->kb.p);
     if (e) return e;
     if ((e = boxes(c))) return e;
-    hipEventRecord(c->tm.ev[5], c->stream);
+    rec(c, 5);
     hipError_t he = hipGetLastError();
     if (he != hipSuccess) return hip_fail(c, he, "LBVH build launch");
     c->built = true;
@@ -692,16 +692,16 @@ int build_lbvh(mmhrt_ctx_s* c, int bits) {
 int refit_lbvh(mmhrt_ctx_s* c) {
     if (!c->built || c->built_n != c->n) return fail(c, MMHRT_E_STATE, "refit: no matching LBVH to refit");
     uint64_t n = c->n;
-    hipEventRecord(c->tm.ev[0], c->stream);
+    rec(c, 0);
     if (int e = reduce_scene(c)) return e;
-    hipEventRecord(c->tm.ev[1], c->stream);
-    hipEventRecord(c->tm.ev[2], c->stream);
-    hipEventRecord(c->tm.ev[3], c->stream);
-    hipLaunchKernelGGL(k_regather, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, c->sph.p, c->leaf_id.p,
+    rec(c, 1);
+    rec(c, 2);
+    rec(c, 3);
+    hipLaunchKernelGGL(k_regather, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, cur_spheres(c), c->leaf_id.p,
                        c->leaf_sph.p, n);
-    hipEventRecord(c->tm.ev[4], c->stream);
+    rec(c, 4);
     if (int e = boxes(c)) return e;
-    hipEventRecord(c->tm.ev[5], c->stream);
+    rec(c, 5);
     hipError_t he = hipGetLastError();
     if (he != hipSuccess) return hip_fail(c, he, "LBVH refit launch");
     c->info_valid = false;

@@ -16,6 +16,7 @@
 #include <float.h>
 
 #include <algorithm>
+#include <cstring>
 
 #include "ctx.hpp"
 
@@ -35,6 +36,7 @@ struct TraceArgs {
     uint32_t spill_stride;  // lanes in the grid
     uint32_t* ctr;          // [0] tile counter, [1] overflow flag, [2..] u64 counters
     uint32_t n_tiles, tiles_x;
+    uint32_t tw_log2;  // pixel tile = (1 << tw_log2) x (64 >> tw_log2), one wavefront
     float4* out_col;
     float* out_depth;
     float* out_t;
@@ -404,7 +406,8 @@ __global__ __launch_bounds__(TB) void k_trace(TraceArgs A) {
         }
         if (tile >= A.n_tiles) break;
         uint32_t tx = tile % A.tiles_x, ty = tile / A.tiles_x;
-        uint32_t px = A.fc.win_x0 + tx * 8 + (lane & 7), py = A.fc.win_y0 + ty * 8 + (lane >> 3);
+        const uint32_t tw = 1u << A.tw_log2, th = 64u >> A.tw_log2;
+        uint32_t px = A.fc.win_x0 + tx * tw + (lane & (tw - 1)), py = A.fc.win_y0 + ty * th + (lane >> A.tw_log2);
         if (px < A.fc.win_x0 + A.fc.win_w && py < A.fc.win_y0 + A.fc.win_h)
             shade_pixel<S, MODE, COUNT>(A, px, py, pad, st, cnt);
     }
@@ -434,8 +437,9 @@ __global__ __launch_bounds__(TB) void k_trace_refill(TraceArgs A, uint32_t refil
     const float amax = A.info->absmax;
     const float pad = (fmaxf(f.pad, amax) + amax) * (1.0f / 131072.0f);
     const uint32_t lane = __lane_id();
-    const uint32_t tiles_x = (f.win_w + 7) / 8;
-    const uint32_t total_work = ((f.win_h + 7) / 8) * tiles_x * 64u;  // work indices: whole 8x8 tiles
+    const uint32_t tw = 1u << A.tw_log2, th = 64u >> A.tw_log2;
+    const uint32_t tiles_x = (f.win_w + tw - 1) / tw;
+    const uint32_t total_work = ((f.win_h + th - 1) / th) * tiles_x * 64u;  // work indices: whole tiles
     int state = L_IDLE;
     bool exhausted = false;
     // pixel state
@@ -465,7 +469,7 @@ __global__ __launch_bounds__(TB) void k_trace_refill(TraceArgs A, uint32_t refil
                 uint32_t w = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
                 uint32_t tile = w >> 6, k = w & 63u;
                 uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
-                uint32_t lx = tx * 8 + (k & 7), ly = ty * 8 + (k >> 3);
+                uint32_t lx = tx * tw + (k & (tw - 1)), ly = ty * th + (k >> A.tw_log2);
                 // tiles past the window edge map to no pixel; the lane simply asks again
                 if (w < total_work && lx < f.win_w && ly < f.win_h) {
                     px = f.win_x0 + lx;
@@ -630,7 +634,98 @@ void launch_refill(const TraceArgs& A, unsigned grid, uint32_t refill, bool coun
     }
 }
 
+// Batched trace: all jobs (cameras) of one timestep share one persistent launch. The global tile
+// index runs over the concatenated tiles of the jobs (prefix[j] = first tile of job j); each
+// wavefront steals tiles and shades them with its job's constants and output buffers.
+template <int S, int MODE>
+__global__ __launch_bounds__(TB) void k_trace_batch(const TraceArgs* __restrict__ jobs,
+                                                    const uint32_t* __restrict__ prefix, uint32_t n_jobs,
+                                                    uint32_t* ctr) {
+    __shared__ int32_t lds_stack[S * TB];
+    const uint32_t gtid = blockIdx.x * TB + threadIdx.x;
+    LaneStack<S> st{lds_stack + threadIdx.x, jobs[0].spill + gtid, jobs[0].spill_stride, 0, ctr + 1};
+    Count cnt;
+    const uint32_t lane = __lane_id();
+    const uint32_t total = prefix[n_jobs];
+    for (;;) {
+        uint32_t t0 = 0;
+        if (lane == 0) t0 = atomicAdd(ctr, 1u);
+        const uint32_t tile = __shfl(t0, 0, 64);
+        if (tile >= total) break;
+        uint32_t j = 0;
+        while (j + 1 < n_jobs && prefix[j + 1] <= tile) ++j;
+        const TraceArgs& A = jobs[j];
+        const float amax = A.info->absmax;
+        const float pad = (fmaxf(A.fc.pad, amax) + amax) * (1.0f / 131072.0f);
+        const uint32_t lt = tile - prefix[j];
+        const uint32_t tw = 1u << A.tw_log2, th = 64u >> A.tw_log2;
+        const uint32_t tx = lt % A.tiles_x, ty = lt / A.tiles_x;
+        const uint32_t px = A.fc.win_x0 + tx * tw + (lane & (tw - 1)), py = A.fc.win_y0 + ty * th + (lane >> A.tw_log2);
+        if (px < A.fc.win_x0 + A.fc.win_w && py < A.fc.win_y0 + A.fc.win_h)
+            shade_pixel<S, MODE, false>(A, px, py, pad, st, cnt);
+    }
+}
+
 }  // namespace
+
+// One batched launch over the jobs of a timestep (mode is the same for all of them).
+void launch_trace_batch(mmhrt_ctx_s* c, const void* jobs, const uint32_t* prefix, uint32_t n_jobs, uint32_t* ctr,
+                        int mode) {
+    unsigned grid = (unsigned)std::max<int64_t>(1, c->cu_count * c->opt_blocks_per_cu);
+    const TraceArgs* J = static_cast<const TraceArgs*>(jobs);
+    int S = (int)c->opt_stack_lds;
+    launch_fill_u32(c->stream, ctr, 0u, 16);
+#define MMHRT_BATCH(SV)                                                                                     \
+    if (mode == 1)                                                                                          \
+        hipLaunchKernelGGL((k_trace_batch<SV, 1>), dim3(grid), dim3(TB), 0, c->stream, J, prefix, n_jobs, ctr); \
+    else                                                                                                    \
+        hipLaunchKernelGGL((k_trace_batch<SV, 0>), dim3(grid), dim3(TB), 0, c->stream, J, prefix, n_jobs, ctr);
+    switch (S) {
+    case 8: MMHRT_BATCH(8) break;
+    case 24: MMHRT_BATCH(24) break;
+    case 32: MMHRT_BATCH(32) break;
+    default: MMHRT_BATCH(16) break;
+    }
+#undef MMHRT_BATCH
+}
+
+// Job constants for the batched kernel, written into a host buffer of batch_job_size() bytes.
+size_t batch_job_size() { return sizeof(TraceArgs); }
+int batch_fill_job(mmhrt_ctx_s* c, const FrameConsts& fc, size_t out_offset, void* dst) {
+    uint32_t W = fc.win_w, H = fc.win_h;
+    (void)H;
+    TraceArgs A;
+    A.nodes = c->nodes.p;
+    A.leaf_sph = c->leaf_sph.p;
+    A.leaf_id = c->leaf_id.p;
+    A.col = c->col.p;
+    A.info = c->info.p;
+    A.spill = c->spill.p;
+    A.spill_stride = (uint32_t)(std::max<int64_t>(1, c->cu_count * c->opt_blocks_per_cu) * TB);
+    A.ctr = nullptr;
+    A.tw_log2 = tile_log2(c->opt_tile_w);
+    A.tiles_x = (W + (1u << A.tw_log2) - 1) >> A.tw_log2;
+    A.n_tiles = A.tiles_x * ((fc.win_h + (64u >> A.tw_log2) - 1) / (64u >> A.tw_log2));
+    A.out_col = c->bt_col.p + out_offset;
+    A.out_depth = c->bt_depth.p + out_offset;
+    A.out_t = c->bt_t.p + out_offset;
+    A.out_id = c->bt_id.p + out_offset;
+    A.out_rgba8 = c->bt_rgba8.p + out_offset;
+    A.out_aux = nullptr;
+    A.fc = fc;
+    A.fc.want_aux = 0;
+    A.fc.accumulate = 0;
+    std::memcpy(dst, &A, sizeof A);
+    return (int)A.n_tiles;
+}
+
+// tile width option -> log2 (1, 2, 4, 8, 16, 32 or 64 pixels wide; 64 pixels per tile)
+uint32_t tile_log2(int64_t tw) {
+    uint32_t l = 3;
+    for (uint32_t k = 0; k <= 6; ++k)
+        if ((int64_t)(1u << k) == tw) l = k;
+    return l;
+}
 
 int render_frame(mmhrt_ctx_s* c, const FrameConsts& fc) {
     uint32_t W = fc.win_w, H = fc.win_h;  // output size = the rendered window
@@ -660,8 +755,9 @@ int render_frame(mmhrt_ctx_s* c, const FrameConsts& fc) {
     A.spill = c->spill.p;
     A.spill_stride = lanes;
     A.ctr = c->counters.p;
-    A.tiles_x = (W + 7) / 8;
-    A.n_tiles = A.tiles_x * ((H + 7) / 8);
+    A.tw_log2 = tile_log2(c->opt_tile_w);
+    A.tiles_x = (W + (1u << A.tw_log2) - 1) >> A.tw_log2;
+    A.n_tiles = A.tiles_x * ((H + (64u >> A.tw_log2) - 1) / (64u >> A.tw_log2));
     A.out_col = c->fb_col.p;
     A.out_depth = c->fb_depth.p;
     A.out_t = c->fb_t.p;
@@ -670,7 +766,7 @@ int render_frame(mmhrt_ctx_s* c, const FrameConsts& fc) {
     A.out_aux = fc.want_aux ? c->fb_aux.p : nullptr;
     A.fc = fc;
     launch_fill_u32(c->stream, c->counters.p, 0u, 16);
-    hipEventRecord(c->tm.ev[6], c->stream);
+    rec(c, 6);
     bool steal = c->opt_persistent != 0;
     bool count = c->opt_count != 0;
     uint32_t refill = (uint32_t)std::max<int64_t>(0, std::min<int64_t>(64, c->opt_refill));
@@ -689,7 +785,7 @@ int render_frame(mmhrt_ctx_s* c, const FrameConsts& fc) {
         default: launch1<16>(A, grid, steal, count, fc.mode, c->stream); break;
         }
     }
-    hipEventRecord(c->tm.ev[7], c->stream);
+    rec(c, 7);
     hipError_t he = hipGetLastError();
     if (he != hipSuccess) return hip_fail(c, he, "trace launch");
     return 0;

@@ -162,7 +162,8 @@ class SphereRenderer:
 
     # ---- frame out
     def render(self, cam: L.Camera, p: L.RenderParams, outputs=("rgba8", "depth", "color", "t", "hit_id")) -> Frame:
-        W, H = p.width, p.height
+        # outputs cover the render window when one is set
+        W, H = (p.window[2], p.window[3]) if p.window[2] > 0 else (p.width, p.height)
         fr = Frame(W, H)
         fb = L.Framebuffer()
         if p.flags & L.FLAG_NO_DOWNLOAD:
@@ -200,6 +201,40 @@ class SphereRenderer:
 
     def sync(self) -> None:
         self._ck(self.lib.mmhrt_sync(self.ctx))
+
+    # ---- batched time series
+    def set_timesteps(self, xyzr_steps: np.ndarray, rgba: np.ndarray | None = None,
+                      global_rgba=(255, 255, 255, 255)) -> None:
+        """xyzr_steps: (T, N, 4) float32, all resident on the device."""
+        xs = np.ascontiguousarray(xyzr_steps, np.float32)
+        T, N = xs.shape[0], xs.shape[1]
+        g = (C.c_uint8 * 4)(*global_rgba)
+        rp = None
+        if rgba is not None:
+            rgba = np.ascontiguousarray(rgba, np.float32)
+            rp = rgba.ctypes.data
+        self._ck(self.lib.mmhrt_set_timesteps(self.ctx, xs.ctypes.data, T, N, rp, g))
+
+    def render_batch(self, jobs: list[tuple], refit_period: int = 1, bits: int = 30, use_graph: bool = True):
+        """jobs: (timestep, camera, params). Returns (list of (rgba8, depth) per job, device ms)."""
+        arr = (L.Job * len(jobs))()
+        sizes = []
+        off = 0
+        for i, (t, cam, p) in enumerate(jobs):
+            arr[i].timestep = t
+            arr[i].camera = cam
+            arr[i].params = p
+            arr[i].out_offset = off
+            w, h = (p.window[2], p.window[3]) if p.window[2] > 0 else (p.width, p.height)
+            sizes.append((w, h, off))
+            off += w * h
+        rgba = np.empty(off, np.uint32)
+        depth = np.empty(off, np.float32)
+        ms = C.c_float()
+        self._ck(self.lib.mmhrt_render_batch(self.ctx, arr, len(jobs), refit_period, bits, 1 if use_graph else 0,
+                                             rgba.ctypes.data, depth.ctypes.data, C.byref(ms)))
+        out = [(rgba[o:o + w * h].reshape(h, w), depth[o:o + w * h].reshape(h, w)) for (w, h, o) in sizes]
+        return out, float(ms.value)
 
     # ---- introspection
     def info(self):

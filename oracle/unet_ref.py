@@ -18,10 +18,39 @@ LAYERS = [("enc0a", 16, 0, 32, 1), ("enc0b", 32, 0, 32, 1), ("enc1a", 32, 0, 48,
           ("dec0a", 32, 48, 32, 1), ("dec0b", 32, 0, 32, 1), ("out", 32, 0, 16, 0)]
 
 
-def bf16_round(x):
-    """Round a float32 torch tensor to bfloat16 precision (nearest even), keep float32."""
+def bf16_round(x, grad=True):
+    """Round a float32 tensor to bfloat16 precision (nearest even), keeping float32. With grad,
+    the gradient flowing back through this point is rounded the same way: the GPU stores both
+    activations and activation gradients in bf16 between layers."""
     import torch
-    return x.to(torch.bfloat16).to(x.dtype)
+
+    class _Round(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, v):
+            return v.to(torch.bfloat16).to(v.dtype)
+
+        @staticmethod
+        def backward(ctx, g):
+            return g.to(torch.bfloat16).to(g.dtype) if grad else g
+
+    return _Round.apply(x)
+
+
+def bf16_round_grad_only(x):
+    """Identity forward; bf16-rounded gradient backward (the network output stays fp32 on the GPU
+    but its gradient is stored in bf16)."""
+    import torch
+
+    class _G(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, v):
+            return v.clone()
+
+        @staticmethod
+        def backward(ctx, g):
+            return g.to(torch.bfloat16).to(g.dtype)
+
+    return _G.apply(x)
 
 
 def pack(feats: np.ndarray, depth_scale: float, dtype="float32", round_bf16=False):
@@ -106,7 +135,8 @@ def forward(tp, x0, round_bf16=False):
     a11 = conv(11, a10)
     a12 = conv(12, a11, skip=a1)
     a13 = conv(13, a12)
-    return conv(14, a13)
+    out = conv(14, a13)
+    return bf16_round_grad_only(out) if round_bf16 else out
 
 
 def compose(out, feats: np.ndarray, bg=(0, 0, 0, 1)):
