@@ -178,18 +178,42 @@ __global__ __launch_bounds__(RB) void k_radix_hist(const K* __restrict__ keys, u
     hist[(size_t)threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
 }
 
+// Stable scatter of one tile of RT keys for one 8-bit digit. The tile is first ranked into
+// digit-sorted order in LDS (tile-local digit starts from the tile's histogram, wave-ballot ranks
+// inside each round), then written out in digit-contiguous runs so that the global writes of
+// consecutive threads go to consecutive addresses.
 template <typename K>
 __global__ __launch_bounds__(RB) void k_radix_scatter(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                       K* __restrict__ kout, uint32_t* __restrict__ vout, uint64_t n,
-                                                      int shift, const uint32_t* __restrict__ offs, uint32_t tiles) {
-    __shared__ uint32_t running[256];     // next free slot per digit for this tile
+                                                      int shift, const uint32_t* __restrict__ hist,
+                                                      const uint32_t* __restrict__ offs, uint32_t tiles) {
+    __shared__ K s_key[RT];
+    __shared__ uint32_t s_val[RT];
+    __shared__ uint32_t running[256];        // next tile-local slot per digit
+    __shared__ uint32_t goff[256];           // global offset of the digit's run minus its local start
     __shared__ uint32_t wcnt[RB / 64][256];  // per-wave digit counts of the current round
-    __shared__ uint32_t wpre[RB / 64][256];  // exclusive prefix over waves + running base
+    __shared__ uint32_t wpre[RB / 64][256];  // slot of the first key of (wave, digit) in this round
+    __shared__ uint32_t wsum[RB / 64];
     const int wave = threadIdx.x >> 6;
-    running[threadIdx.x] = offs[(size_t)threadIdx.x * tiles + blockIdx.x];
+    const uint32_t lane = __lane_id();
+    // tile-local digit starts: exclusive scan of this tile's digit counts (thread = digit)
+    uint32_t cnt = hist[(size_t)threadIdx.x * tiles + blockIdx.x];
+    uint32_t inc = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(inc, o, 64);
+        if (lane >= (uint32_t)o) inc += y;
+    }
+    if (lane == 63) wsum[wave] = inc;
     for (int w = 0; w < RB / 64; ++w) wcnt[w][threadIdx.x] = 0;
     __syncthreads();
-    uint64_t base = (uint64_t)blockIdx.x * RT;
+    uint32_t wb = 0;
+    for (int w = 0; w < wave; ++w) wb += wsum[w];
+    const uint32_t lstart = wb + inc - cnt;
+    running[threadIdx.x] = lstart;
+    goff[threadIdx.x] = offs[(size_t)threadIdx.x * tiles + blockIdx.x] - lstart;
+    __syncthreads();
+    const uint64_t base = (uint64_t)blockIdx.x * RT;
     for (int r = 0; r < RI; ++r) {
         uint64_t i = base + (uint64_t)r * RB + threadIdx.x;
         bool valid = i < n;
@@ -198,8 +222,7 @@ __global__ __launch_bounds__(RB) void k_radix_scatter(const K* __restrict__ kin,
         uint32_t d = (uint32_t)((key >> shift) & 255u);
         uint64_t peers = match_digit(d, valid);
         uint32_t rank = (uint32_t)__popcll(peers & lanemask_lt());
-        if (valid && (__ffsll((unsigned long long)peers) - 1) == (int)__lane_id())
-            wcnt[wave][d] = (uint32_t)__popcll(peers);
+        if (valid && (__ffsll((unsigned long long)peers) - 1) == (int)lane) wcnt[wave][d] = (uint32_t)__popcll(peers);
         __syncthreads();
         {
             uint32_t acc = running[threadIdx.x];
@@ -215,10 +238,18 @@ __global__ __launch_bounds__(RB) void k_radix_scatter(const K* __restrict__ kin,
         __syncthreads();
         if (valid) {
             uint32_t pos = wpre[wave][d] + rank;
-            kout[pos] = key;
-            vout[pos] = val;
+            s_key[pos] = key;
+            s_val[pos] = val;
         }
         __syncthreads();
+    }
+    const uint32_t nvalid = (uint32_t)((n - base) < (uint64_t)RT ? (n - base) : (uint64_t)RT);
+    for (uint32_t i = threadIdx.x; i < nvalid; i += RB) {
+        K key = s_key[i];
+        uint32_t d = (uint32_t)((key >> shift) & 255u);
+        uint32_t g = goff[d] + i;
+        kout[g] = key;
+        vout[g] = s_val[i];
     }
 }
 
@@ -579,8 +610,8 @@ int sort_pairs(mmhrt_ctx_s* c, K* ka, K* kb, uint32_t* va, uint32_t* vb, uint64_
         hipLaunchKernelGGL(k_scan_sums, dim3(nchunks), dim3(SB), 0, c->stream, hist, hlen, c->scan_tmp.p);
         hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SB), 0, c->stream, c->scan_tmp.p, nchunks);
         hipLaunchKernelGGL(k_scan_chunks, dim3(nchunks), dim3(SB), 0, c->stream, hist, offs, hlen, c->scan_tmp.p);
-        hipLaunchKernelGGL(k_radix_scatter<K>, dim3(tiles), dim3(RB), 0, c->stream, src, vs, dst, vd, n, shift, offs,
-                           tiles);
+        hipLaunchKernelGGL(k_radix_scatter<K>, dim3(tiles), dim3(RB), 0, c->stream, src, vs, dst, vd, n, shift, hist,
+                           offs, tiles);
         K* tk = src;
         src = dst;
         dst = tk;
