@@ -33,7 +33,8 @@ class OrcFrameState(C.Structure):
 class OrcParams(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32), ("mode", C.c_int32),
                 ("ao_distance", C.c_float), ("frame_idx", C.c_uint32), ("background", C.c_float * 4),
-                ("want_aux", C.c_int32), ("row_begin", C.c_uint32), ("row_end", C.c_uint32)]
+                ("want_aux", C.c_int32), ("row_begin", C.c_uint32), ("row_end", C.c_uint32),
+                ("max_bounces", C.c_int32)]
 
 
 class OrcList(C.Structure):
@@ -107,10 +108,11 @@ def camera_from(cam) -> OrcCamera:
 
 
 def make_params(width, height, spp=1, mode=0, ao_distance=1e20, frame_idx=0, background=(0, 0, 0, 1),
-                want_aux=0) -> OrcParams:
+                want_aux=0, max_bounces=0) -> OrcParams:
     p = OrcParams()
     p.width, p.height, p.spp, p.mode = width, height, spp, mode
     p.ao_distance, p.frame_idx, p.want_aux = ao_distance, frame_idx, want_aux
+    p.max_bounces = max_bounces
     for i in range(4):
         p.background[i] = background[i]
     return p

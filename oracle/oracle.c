@@ -557,6 +557,54 @@ static void render_pixel(scene_view* s, const float* xyzr, const float* rgba, co
                 float V = occ ? 0.0f : 1.0f;
                 sc[0] = alb[0] * V; sc[1] = alb[1] * V; sc[2] = alb[2] * V; sc[3] = 1.0f;
                 vis_sum += V;
+            } else if (p->mode == 2 && p->max_bounces > 0) {
+                /* raygen.cu:122-155: col *= albedo per hit; miss -> col * 0.8; depth cap -> 0.1 */
+                float col3[3] = {alb[0], alb[1], alb[2]};
+                float ro[3] = {o[0], o[1], o[2]}, rd[3] = {d[0], d[1], d[2]}, rt = t, n[3] = {N[0], N[1], N[2]};
+                for (int depth_i = 0;; ++depth_i) {
+                    if (depth_i >= p->max_bounces) {
+                        sc[0] = sc[1] = sc[2] = 0.1f;
+                        break;
+                    }
+                    float so[3], u[3], sd[3];
+                    for (int a = 0; a < 3; ++a) so[a] = ro[a] + rt * rd[a];
+                    /* random_in_unit_sphere (raygen.cu:71-77), draws taken x, y, z in order */
+                    do {
+                        float rx = orc_rnd(&seed);
+                        float ry = orc_rnd(&seed);
+                        float rz = orc_rnd(&seed);
+                        u[0] = 2.0f * rx - 1.0f;
+                        u[1] = 2.0f * ry - 1.0f;
+                        u[2] = 2.0f * rz - 1.0f;
+                    } while (dot3(u, u) >= 1.0f);
+                    for (int a = 0; a < 3; ++a) sd[a] = n[a] + u[a];
+                    normalize3(sd);
+                    float tb;
+                    int32_t ib;
+                    if (!closest(s, so, sd, 1e-3f, 1e8f, &tb, &ib)) {
+                        for (int a = 0; a < 3; ++a) sc[a] = col3[a] * 0.8f;
+                        break;
+                    }
+                    const float* sb = sphere_of(s, ib, xyzr);
+                    const float* ab = rgba + 4 * (size_t)ib;
+                    float Pb[3], Nb[3];
+                    for (int a = 0; a < 3; ++a) {
+                        Pb[a] = so[a] + tb * sd[a];
+                        Nb[a] = Pb[a] - sb[a];
+                    }
+                    if (dot3(Nb, sd) > 0.0f) {
+                        Nb[0] = -Nb[0]; Nb[1] = -Nb[1]; Nb[2] = -Nb[2];
+                    }
+                    normalize3(Nb);
+                    for (int a = 0; a < 3; ++a) {
+                        col3[a] = col3[a] * ab[a];
+                        ro[a] = so[a];
+                        rd[a] = sd[a];
+                        n[a] = Nb[a];
+                    }
+                    rt = tb;
+                }
+                sc[3] = 1.0f;
             } else {
                 float sh = 0.2f + 0.6f * fabsf(dot3(N, d)); /* raygen.cu:141 */
                 sc[0] = alb[0] * sh; sc[1] = alb[1] * sh; sc[2] = alb[2] * sh; sc[3] = 1.0f;

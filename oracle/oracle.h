@@ -33,6 +33,7 @@ typedef struct orc_params {
     float background[4];
     int32_t want_aux;
     uint32_t row_begin, row_end; /* LBVH renderer only: rows [begin, end) (end 0 = all rows) */
+    int32_t max_bounces;         /* mode 2: diffuse bounces of raygen.cu:144-154 */
 } orc_params;
 
 typedef struct orc_list {

@@ -20,7 +20,7 @@ def small_camera(w=64, h=48, pos=(0.5, 0.5, 2.2), target=(0.5, 0.5, 0.5), fovy=4
 def oracle_params(p):
     """Oracle params from a megamol_amd RenderParams."""
     return O.make_params(p.width, p.height, p.spp, p.mode, p.ao_distance, p.frame_idx, tuple(p.background),
-                         1 if (p.flags & 1) else 0)
+                         1 if (p.flags & 1) else 0, p.max_bounces)
 
 
 def assert_frames_equal(g, o, aux=False, what=""):

@@ -203,3 +203,13 @@ def test_error_paths(gpu_renderer):
                      ParticleList(L.VERTDATA_NONE, pts, count=2),
                      ParticleList(L.VERTDATA_FLOAT_XYZ, pts, count=2, global_radius=0.05)])
         assert r.info()[0] == 2
+
+
+@pytest.mark.parametrize("bounces", [1, 3])
+def test_bounce_mode_bit_exact(gpu_renderer, bounces):
+    case(f"diffuse bounces {bounces}")
+    xyzr, rgba = scenes.random_spheres(3000, seed=21)
+    cam = small_camera(64, 48)
+    p = params(64, 48, spp=2, mode=L.MODE_BOUNCE, max_bounces=bounces)
+    g, o = render_both(gpu_renderer, xyzr, rgba, cam, p, 30)
+    assert_frames_equal(g, o, what=f"bounces={bounces}")

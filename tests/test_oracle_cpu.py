@@ -338,3 +338,27 @@ def test_mmpld_roundtrip_versions_and_errors(tmp_path):
     bad.write_bytes(bytes(raw))
     with pytest.raises(mmpld.MMPLDError, match="any frame"):
         mmpld.read(str(bad))
+
+
+def test_bounce_closed_forms():
+    # raygen.cu:122-155. An isolated sphere: a bounce direction N + u (|u| < 1) leaves the convex
+    # sphere, so one bounce always misses: colour = albedo * 0.8; with a zero-length bounce budget
+    # the depth cap gives 0.1 -- here max_bounces = 1 reaches the cap only after a second hit.
+    xyzr = np.array([[0.5, 0.5, 0.5, 0.3]], np.float32)
+    rgba = np.array([[0.2, 0.4, 0.6, 1.0]], np.float32)
+    cam = small_camera(32, 32)
+    for mb in (1, 2, 5):
+        o = O.render_brute(xyzr, rgba, cam, O.make_params(32, 32, spp=1, mode=2, max_bounces=mb))
+        hit = o["hit_id"] >= 0
+        assert hit.sum() > 50
+        np.testing.assert_allclose(o["color"][hit][:, :3], np.broadcast_to(rgba[0, :3] * np.float32(0.8),
+                                                                             (hit.sum(), 3)), rtol=0, atol=0)
+    # two touching spheres facing each other: some bounces hit the neighbour; a budget of 1
+    # then ends at the 0.1 cap, a budget of 2 continues
+    xyzr = np.array([[0.3, 0.5, 0.5, 0.2], [0.7, 0.5, 0.5, 0.2]], np.float32)
+    rgba = np.ones((2, 4), np.float32)
+    o1 = O.render_brute(xyzr, rgba, cam, O.make_params(32, 32, spp=4, mode=2, max_bounces=1))
+    vals = {round(float(v), 5) for v in o1["color"][o1["hit_id"] >= 0][:, 0]}
+    # per-sample values are 0.8 (escaped or primary miss) or 0.1 (capped); 4-sample averages lie on that grid
+    grid = {round((k * 0.1 + (4 - k) * 0.8) / 4, 5) for k in range(5)}
+    assert vals <= grid and len(vals) > 1
