@@ -317,7 +317,7 @@ __device__ void shade_pixel(const TraceArgs& A, uint32_t px, uint32_t py, float 
             float4 alb = A.col[id];
             if (MODE == 1) {
                 uint32_t aseed = tea16(pix, 0xA0000000u + f.frame_idx * 4096u + s);
-                V3 ad = ao_direction(N, aseed);
+                V3 ad = ao_direction(N, aseed, s, f.spp);
                 RayPre ra = prep(P, ad, pad);
                 if (COUNT) cnt.rays++;
                 bool occ = trace_any<S, COUNT>(A, ra, 0.0f, f.ao_distance, leaf, st, cnt);
@@ -554,7 +554,7 @@ __global__ __launch_bounds__(TB) void k_trace_refill(TraceArgs A, uint32_t refil
                 }
                 if (MODE == 1) {
                     uint32_t aseed = tea16(pix, 0xA0000000u + f.frame_idx * 4096u + s);
-                    V3 ad = ao_direction(N, aseed);
+                    V3 ad = ao_direction(N, aseed, s, f.spp);
                     prim_t = t;
                     prim_leaf = best;
                     r = prep(Pp, ad, pad);

@@ -87,6 +87,8 @@ def make_pairs(r, n, cells, seed, crop=128, spp_target=256, min_hit=0.25, width=
                 continue
             feats[k], tg[k] = f, t
             k += 1
+            if k % 250 == 0:
+                log(f"{k}/{n} pairs")
     return feats, tg
 
 

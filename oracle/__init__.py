@@ -67,7 +67,7 @@ def load():
         f.orc_tea16.restype = C.c_uint32
         f.orc_tea16.argtypes = [C.c_uint32, C.c_uint32]
         f.orc_tea_encrypt.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
-        f.orc_ao_direction.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p]
+        f.orc_ao_direction.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p]
         f.orc_lcg.restype = C.c_uint32
         f.orc_lcg.argtypes = [C.POINTER(C.c_uint32)]
         f.orc_rnd.restype = C.c_float
@@ -235,10 +235,11 @@ def tea_encrypt(v0, v1, key, cycles):
     return int(v[0]), int(v[1])
 
 
-def ao_direction(normal, seed):
+def ao_direction(normal, seed, smp=0, spp=1):
+    """AO direction of sample smp of spp from the sample's stream seed."""
     n = np.asarray(normal, np.float32)
     d = np.zeros(3, np.float32)
-    load().orc_ao_direction(n.ctypes.data, seed, d.ctypes.data)
+    load().orc_ao_direction(n.ctypes.data, seed, smp, spp, d.ctypes.data)
     return d
 
 

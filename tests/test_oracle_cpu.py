@@ -251,6 +251,61 @@ def test_ao_direction_is_cosine_weighted():
     assert np.all(np.abs(hist / len(cos) - 0.1) < 0.01)
 
 
+def _concentric(u1, u2):
+    """Shirley-Chiu concentric map in float64: disk radius and azimuth of a unit-square point."""
+    a, b = 2 * u1 - 1, 2 * u2 - 1
+    if abs(a) > abs(b):
+        return abs(a), (np.pi / 4) * (b / a) + (np.pi if a < 0 else 0.0)
+    return abs(b), np.pi / 2 - (np.pi / 4) * (a / b) + (np.pi if b < 0 else 0.0)
+
+
+@pytest.mark.parametrize("normal", [(0.3, -0.5, 0.8), (0, 0, 1), (0, 0, -1), (1, 0, 0), (-0.6, 0.0, -0.8)])
+def test_ao_direction_closed_forms(normal):
+    """Per sample, with the jitter taken from the LCG stream: the polar angle satisfies
+    cos = sqrt(1 - r^2) (Malley over the concentric map, r = max(|a|, |b|)) and the azimuth
+    difference of two samples equals that of their concentric-map angles, whatever the tangent
+    basis; stratum s covers [s % g, s % g + 1) x [s / g, s / g + 1) / g."""
+    n = np.array(normal, np.float32)
+    n /= np.linalg.norm(n)
+    spp, g = 18, 4  # 16 stratified samples and two unstratified ones
+    # any unit vector in the tangent plane serves as the azimuth origin
+    e = np.cross(n.astype(np.float64), [1.0, 0, 0] if abs(n[0]) < 0.9 else [0, 1.0, 0])
+    e /= np.linalg.norm(e)
+    f = np.cross(n.astype(np.float64), e)
+    az, phi = [], []
+    for seed in (3, 99):
+        for s in range(spp):
+            d = O.ao_direction(n, seed + 1000 * s, s, spp).astype(np.float64)
+            r1, r2 = O.lcg_stream(seed + 1000 * s, 2)
+            u1, u2 = (((s % g) + r1) / g, ((s // g) + r2) / g) if s < g * g else (r1, r2)
+            r, ph = _concentric(u1, u2)
+            assert abs(np.linalg.norm(d) - 1) < 1e-6
+            assert abs(d @ n - np.sqrt(max(0.0, 1 - r * r))) < 2e-6, (s, d @ n, r)
+            if r > 1e-2:  # the azimuth of a direction next to the pole is ill-conditioned
+                az.append(np.arctan2(d @ f, d @ e))
+                phi.append(ph)
+    az, phi = np.array(az), np.array(phi)
+    assert len(az) > 30
+    diff = np.angle(np.exp(1j * ((az - az[0]) - (phi - phi[0]))))
+    assert np.abs(diff).max() < 1e-4, np.abs(diff).max()
+
+
+def test_ao_strata_reduce_variance():
+    """Stratified cosine-weighted samples: E[cos] = 2/3 per pixel estimate with far lower spread
+    than independent samples (the point of the strata)."""
+    n = np.array([0.0, 0.6, 0.8], np.float32)
+    spp = 64
+    strat, indep = [], []
+    for pix in range(100):
+        c1 = [O.ao_direction(n, O.tea16(pix, s), s, spp) @ n for s in range(spp)]
+        c2 = [O.ao_direction(n, O.tea16(pix, s), 0, 1) @ n for s in range(spp)]
+        strat.append(np.mean(c1))
+        indep.append(np.mean(c2))
+    # per-pixel spread is about 0.013 (stratified) against 0.029 (independent): 4 sigma of the mean
+    assert abs(np.mean(strat) - 2 / 3) < 5e-3
+    assert np.std(strat) < 0.6 * np.std(indep), (np.std(strat), np.std(indep))
+
+
 def test_ao_isolated_sphere_is_unoccluded():
     xyzr = np.array([[0.5, 0.5, 0.5, 0.3]], np.float32)
     rgba = np.array([[0.2, 0.4, 0.6, 1.0]], np.float32)
