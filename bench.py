@@ -274,15 +274,19 @@ def main():
 
     # per-kernel durations (HIP events on the library stream), one frame at a time
     tr, bu, so = [], [], []
+    stages = {k: [] for k in ("morton", "sort", "hierarchy", "boxes")}
     for _ in range(args.steps):
         frame()
         t = r.timings()
         tr.append(t["trace"])
         bu.append(t["build"])
         so.append(t["sort"])
+        for k in stages:
+            stages[k].append(t[k])
     trace_ms = float(np.mean(tr))
     build_ms = float(np.mean(bu))
-    log(f"kernel times: trace {trace_ms:.3f} ms, build {build_ms:.3f} ms (sort {np.mean(so):.3f})")
+    build_stages = {k: round(float(np.mean(v)), 3) for k, v in stages.items()}
+    log(f"kernel times: trace {trace_ms:.3f} ms, build {build_ms:.3f} ms {build_stages}")
 
     # instrumented frame: node visits, sphere tests, rays (algorithmic bytes of the trace kernel)
     r.set_option("count_visits", 1)
@@ -340,6 +344,7 @@ def main():
         "mrays_per_s": round(mrays, 1),
         "rays_per_frame": int(rays),
         "build_ms": round(build_ms, 3),
+        "build_stages_ms": build_stages,
         "trace_ms": round(trace_ms, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,

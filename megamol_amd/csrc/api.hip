@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "../../include/mmhrt.h"
 #include "ctx.hpp"
@@ -136,9 +137,9 @@ int32_t mmhrt_destroy(mmhrt_ctx c) {
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
     release(c->sph); release(c->col); release(c->staging);
-    release(c->keys64); release(c->ka); release(c->kb); release(c->kla); release(c->klb);
+    release(c->meet); release(c->meet_far); release(c->ka); release(c->kb); release(c->kla); release(c->klb);
     release(c->va); release(c->vb); release(c->hist); release(c->scan_tmp);
-    release(c->leaf_sph); release(c->leaf_id); release(c->nodes); release(c->parent); release(c->flags); release(c->box_cnt);
+    release(c->leaf_sph); release(c->leaf_id); release(c->nodes); release(c->flags); release(c->box_cnt);
     release(c->partial); release(c->info); release(c->tf_buf);
     release(c->fb_col); release(c->fb_depth); release(c->fb_t); release(c->fb_id); release(c->fb_rgba8);
     release(c->fb_aux); release(c->spill); release(c->counters);
@@ -391,7 +392,16 @@ int32_t mmhrt_get_bvh(mmhrt_ctx c, float* nodes, uint32_t* leaf_ids, uint64_t* k
     hipError_t e = hipStreamSynchronize(c->stream);
     if (e == hipSuccess && nodes) e = hipMemcpy(nodes, c->nodes.p, m * sizeof(Node), hipMemcpyDeviceToHost);
     if (e == hipSuccess && leaf_ids) e = hipMemcpy(leaf_ids, c->leaf_id.p, n * 4, hipMemcpyDeviceToHost);
-    if (e == hipSuccess && keys) e = hipMemcpy(keys, c->keys64.p, n * 8, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && keys) {
+        // the sorted keys stay in the first ping-pong buffer until the next build
+        if (c->bits == 63) {
+            e = hipMemcpy(keys, c->kla.p, n * 8, hipMemcpyDeviceToHost);
+        } else {
+            std::vector<uint32_t> k32(n);
+            e = hipMemcpy(k32.data(), c->ka.p, n * 4, hipMemcpyDeviceToHost);
+            for (uint64_t i = 0; i < n; ++i) keys[i] = k32[i];
+        }
+    }
     if (e != hipSuccess) return hip_fail(c, e, "get_bvh");
     return MMHRT_OK;
 }

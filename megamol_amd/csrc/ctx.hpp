@@ -22,7 +22,7 @@ struct __align__(16) Node {
     float4 a;  // c0.lo.x c0.lo.y c0.lo.z c0.hi.x
     float4 b;  // c0.hi.y c0.hi.z c1.lo.x c1.lo.y
     float4 c;  // c1.lo.z c1.hi.x c1.hi.y c1.hi.z
-    int4 d;    // child0, child1, parent, height
+    int4 d;    // child0, child1, -1, height
 };
 
 // Scene facts computed on the device by the bounds reduction.
@@ -70,7 +70,6 @@ struct mmhrt_ctx_s {
     int bits = 30;
     bool built = false;
     uint64_t built_n = 0;
-    mmhrt::DevBuf<uint64_t> keys64;  // Morton keys in leaf order
     mmhrt::DevBuf<uint32_t> ka, kb;  // 32-bit key ping-pong
     mmhrt::DevBuf<uint64_t> kla, klb;  // 64-bit key ping-pong
     mmhrt::DevBuf<uint32_t> va, vb;  // values ping-pong
@@ -78,8 +77,9 @@ struct mmhrt_ctx_s {
     mmhrt::DevBuf<float4> leaf_sph;
     mmhrt::DevBuf<uint32_t> leaf_id;
     mmhrt::DevBuf<mmhrt::Node> nodes;
-    mmhrt::DevBuf<int32_t> parent;
     mmhrt::DevBuf<uint32_t> flags;
+    mmhrt::DevBuf<mmhrt::Node> meet;     // per split position: the records of its two children (build scratch)
+    mmhrt::DevBuf<int2> meet_far;  // per split position: the children's far range ends
     mmhrt::DevBuf<uint32_t> box_cnt;  // worklist sizes of the bottom-up box rounds
     mmhrt::DevBuf<float> partial;
     mmhrt::DevBuf<mmhrt::SceneInfo> info;

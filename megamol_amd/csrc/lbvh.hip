@@ -8,9 +8,9 @@
 //   3. LSD radix sort of (key, particle index), 8-bit digits, stable: a tile's keys are ranked
 //      with wavefront ballots (8 ballots give each lane the mask of same-digit lanes)
 //   4. gather spheres into Morton order (leaf k), keep leaf -> particle index
-//   5. Karras (2012) hierarchy: one thread per internal node, index tie-break for equal keys
-//   6. bottom-up boxes: one thread per leaf climbs; the second arrival at a node (atomic
-//      counter) writes the node's box into its parent and the subtree height
+//   5. hierarchy and boxes together, bottom-up: one thread per leaf climbs, the second arrival
+//      at a parent merges and writes the whole node (Karras 2012 numbering and index tie-break
+//      for equal keys, Apetrei 2014 parent rule; LDS meeting places inside 512-leaf chunks)
 #include <algorithm>
 
 #include "ctx.hpp"
@@ -317,23 +317,21 @@ __global__ __launch_bounds__(SB) void k_scan_chunks(const uint32_t* __restrict__
     }
 }
 
-// ---------------------------------------------------------------- gather / hierarchy / boxes
-template <typename K>
-__global__ __launch_bounds__(256) void k_gather(const K* __restrict__ keys, const uint32_t* __restrict__ vals,
-                                                const float4* __restrict__ sph, float4* __restrict__ leaf_sph,
-                                                uint32_t* __restrict__ leaf_id, uint64_t* __restrict__ keys64,
+// ---------------------------------------------------------------- gather / hierarchy + boxes
+__global__ __launch_bounds__(256) void k_gather(const uint32_t* __restrict__ vals, const float4* __restrict__ sph,
+                                                float4* __restrict__ leaf_sph, uint32_t* __restrict__ leaf_id,
                                                 uint64_t n) {
     uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     uint32_t v = vals[k];
     leaf_sph[k] = sph[v];
     leaf_id[k] = v;
-    keys64[k] = (uint64_t)keys[k];
 }
 
+// Common-prefix length of sorted leaves i and j: of the keys, or key bits + that of the indices
+// when the keys are equal (Karras 2012, sec. 4: duplicates ordered by index).
 template <typename K>
-__device__ __forceinline__ int kdelta(const K* __restrict__ keys, int64_t n, int64_t i, int64_t j) {
-    if (j < 0 || j >= n) return -1;
+__device__ __forceinline__ int kdelta(const K* __restrict__ keys, int64_t i, int64_t j) {
     K a = keys[i], b = keys[j];
     if (a == b) return (int)(8 * sizeof(K)) + __clz((uint32_t)i ^ (uint32_t)j);
     if constexpr (sizeof(K) == 8)
@@ -342,171 +340,244 @@ __device__ __forceinline__ int kdelta(const K* __restrict__ keys, int64_t n, int
         return __clz((int)(a ^ b));
 }
 
+// delta(k) of leaves k and k + 1; -1 past either end of [0, n - 2].
 template <typename K>
-__global__ __launch_bounds__(256) void k_karras(const K* __restrict__ keys, int64_t n, Node* __restrict__ nodes,
-                                                int32_t* __restrict__ parent) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t m = n - 1;
-    if (i >= m) return;
-    if (i == 0) parent[0] = -1;  // the root is nobody's child
-    int dir = (kdelta(keys, n, i, i + 1) - kdelta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
-    int dmin = kdelta(keys, n, i, i - dir);
-    int64_t lmax = 2;
-    while (kdelta(keys, n, i, i + lmax * dir) > dmin) lmax <<= 1;
-    int64_t l = 0;
-    for (int64_t t = lmax >> 1; t >= 1; t >>= 1)
-        if (kdelta(keys, n, i, i + (l + t) * dir) > dmin) l += t;
-    int64_t j = i + l * dir;
-    int dnode = kdelta(keys, n, i, j);
-    int64_t s = 0, t = l;
-    do {
-        t = (t + 1) >> 1;
-        if (kdelta(keys, n, i, i + (s + t) * dir) > dnode) s += t;
-    } while (t > 1);
-    int64_t gamma = i + s * dir + (dir < 0 ? -1 : 0);
-    int64_t first = i < j ? i : j, last = i < j ? j : i;
-    int32_t left = (first == gamma) ? ~(int32_t)gamma : (int32_t)gamma;
-    int32_t right = (last == gamma + 1) ? ~(int32_t)(gamma + 1) : (int32_t)(gamma + 1);
-    nodes[i].d = make_int4(left, right, -1, 0);
-    parent[left < 0 ? m + (int64_t)(~left) : (int64_t)left] = (int32_t)i;
-    parent[right < 0 ? m + (int64_t)(~right) : (int64_t)right] = (int32_t)i;
+__device__ __forceinline__ int split_delta(const K* __restrict__ keys, int64_t n, int64_t k) {
+    if (k < 0 || k >= n - 1) return -1;
+    return kdelta(keys, k, k + 1);
 }
 
-__device__ __forceinline__ void store_child_box(Node* nd, int slot, const float b[6]) {
-    float* f = reinterpret_cast<float*>(nd);
-    int o = slot * 6;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) f[o + k] = b[k];
-}
-
-// Bottom-up boxes without cross-workgroup fences. A node's box is final once both of its child
-// slots are written; a device-scope fence per climb step (the classic single-kernel scheme)
-// would write back L2 on every step, so completion is propagated in rounds instead:
-//   init:   every internal node writes the boxes of its leaf children into its own slots and
-//           counts them in flags[i]; nodes with two leaf children form the first worklist
-//   rounds: each worklist node unions its two slots, writes the result into its parent's slot
-//           and bumps the parent's counter; the second arrival appends the parent to the next
-//           worklist. Kernel boundaries make the slots of earlier rounds visible.
+// The hierarchy and its boxes in one bottom-up pass. A node covering leaves [l, r] is the left
+// child of the node split between r and r + 1 when delta(r) > delta(l - 1), otherwise the right
+// child of the node split between l - 1 and l (the parent rule of Apetrei 2014, "Fast and simply
+// scalable bounding volume hierarchy construction"; the two deltas never tie for distinct
+// augmented keys). Nodes carry the numbering of Karras 2012 — the root is 0, a left child is
+// numbered by its last leaf and a right child by its first — so the tree and its numbering are
+// those of the top-down construction.
+//
+// Each thread climbs from a leaf. Two children meet at their parent's split position s; the
+// first to arrive parks its box, reference, height and far range end there and stops, the
+// second merges, writes nothing until the merged node knows its own side (which fixes its
+// number), then writes the whole 64-byte node and climbs on. A device-scope fence per climb step
+// would write back L2 on every step, so the meeting places are split by where they lie:
+//   chunk:  workgroup g owns leaves [g*BC, (g+1)*BC); splits between two of its leaves meet in
+//           LDS (a workgroup-scope atomic decides the order). A split at a chunk boundary, or an
+//           in-chunk split whose other child never arrives (it straddles the boundary), meets in
+//           global memory, meet[s], where a device-scope counter flags[s] counts the arrivals;
+//           the second arrival appends s to the first worklist.
+//   rounds: each worklist split merges its two records, writes its node, delivers to its own
+//           parent's meet record and bumps that counter; the second arrival appends the parent's
+//           split to the next worklist. Kernel boundaries make the records of earlier rounds
+//           visible.
 //   finish: once a worklist is small, one workgroup runs the remaining rounds itself with
 //           barriers between them (workgroup scope, no cache writeback).
-constexpr int BOX_ROUNDS = 48;      // kernel rounds before the single-workgroup finish
+// Straddling nodes are a small fraction: the ancestors of one node per chunk boundary.
+constexpr int BOX_ROUNDS = 48;        // kernel rounds before the single-workgroup finish
 constexpr uint32_t BOX_SMALL = 2048;  // worklist size handed to the single workgroup
+constexpr int BC = 512;               // leaves per chunk workgroup (34 KB of LDS: four per CU)
+constexpr int BOX_ITEMS = 16;
+constexpr int BOX_TILE = 256 * BOX_ITEMS;
 
 __device__ __forceinline__ void leaf_box(const float4 s, float b[6]) {
     b[0] = s.x - s.w; b[1] = s.y - s.w; b[2] = s.z - s.w;
     b[3] = s.x + s.w; b[4] = s.y + s.w; b[5] = s.z + s.w;
 }
 
-// Wave-aggregated append: one atomic per wavefront. All lanes of the wave must call it.
-__device__ __forceinline__ void wave_append(bool want, uint32_t v, uint32_t* list, uint32_t* cnt) {
-    uint64_t m = __ballot(want);
-    if (m == 0) return;
-    int leader = __ffsll((unsigned long long)m) - 1;
-    uint32_t base = 0;
-    if ((int)__lane_id() == leader) base = atomicAdd(cnt, (uint32_t)__popcll(m));
-    base = __shfl(base, leader, 64);
-    if (want) list[base + (uint32_t)__popcll(m & lanemask_lt())] = v;
+// Whole node: the two child boxes, the child references and the height, one 64-byte line.
+__device__ __forceinline__ void store_node(Node* nd, const float f[12], int2 k, int h) {
+    nd->a = make_float4(f[0], f[1], f[2], f[3]);
+    nd->b = make_float4(f[4], f[5], f[6], f[7]);
+    nd->c = make_float4(f[8], f[9], f[10], f[11]);
+    nd->d = make_int4(k.x, k.y, -1, h);
 }
 
-// Block-aggregated worklist append: a block gathers up to BOX_TILE entries in LDS, reserves
-// space with one global atomic and writes them out contiguously.
-constexpr int BOX_ITEMS = 16;
-constexpr int BOX_TILE = 256 * BOX_ITEMS;
+// One child's record at a global meeting place: box in slot `side` of a,b,c; d = (ref0, ref1,
+// height0, height1); far[side] = the child's range end away from the split.
+__device__ __forceinline__ void meet_put(Node* mt, int2* mf, int side, const float b[6], int32_t ref, int h,
+                                         int32_t far) {
+    float* f = reinterpret_cast<float*>(mt) + side * 6;
+#pragma unroll
+    for (int a = 0; a < 6; ++a) f[a] = b[a];
+    int* d = reinterpret_cast<int*>(&mt->d);
+    d[side] = ref;
+    d[2 + side] = h;
+    reinterpret_cast<int*>(mf)[side] = far;
+}
 
-struct BoxAppend {
+// Block-aggregated worklist append: up to 256 entries gathered in LDS, one global atomic per
+// block; entries past that go straight to the list.
+struct SplitList {
     uint32_t* s_list;
     uint32_t* s_n;
     uint32_t* s_base;
-    __device__ __forceinline__ void begin() {
-        if (threadIdx.x == 0) *s_n = 0;
-        __syncthreads();
+    __device__ __forceinline__ void push(uint32_t v, uint32_t* list, uint32_t* cnt) {
+        uint32_t q = atomicAdd(s_n, 1u);
+        if (q < 256)
+            s_list[q] = v;
+        else
+            list[atomicAdd(cnt, 1u)] = v;
     }
-    __device__ __forceinline__ void push(uint32_t v) { s_list[atomicAdd(s_n, 1u)] = v; }
     __device__ __forceinline__ void flush(uint32_t* list, uint32_t* cnt) {
         __syncthreads();
-        uint32_t n = *s_n;
-        if (threadIdx.x == 0 && n) *s_base = atomicAdd(cnt, n);
+        uint32_t nl = *s_n < 256u ? *s_n : 256u;
+        if (threadIdx.x == 0 && nl) *s_base = atomicAdd(cnt, nl);
         __syncthreads();
-        uint32_t base = *s_base;
-        for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) list[base + k] = s_list[k];
+        for (uint32_t k = threadIdx.x; k < nl; k += blockDim.x) list[*s_base + k] = s_list[k];
+        __syncthreads();
+        if (threadIdx.x == 0) *s_n = 0;
         __syncthreads();
     }
 };
 
-__global__ __launch_bounds__(256) void k_box_init(const float4* __restrict__ leaf_sph, int64_t m, Node* nodes,
-                                                  uint32_t* __restrict__ flags, uint32_t* __restrict__ list,
-                                                  uint32_t* cnt) {
-    __shared__ uint32_t s_list[BOX_TILE];
+// The chunk pass. flags[] must be zero on entry; n >= 2.
+template <typename K>
+__global__ __launch_bounds__(256) void k_bu_chunk(const K* __restrict__ keys, const float4* __restrict__ leaf_sph,
+                                                  int64_t n, Node* __restrict__ nodes, Node* meet, int2* meet_far,
+                                                  uint32_t* flags, uint32_t* list, uint32_t* cnt, SceneInfo* info) {
+    __shared__ float s_box[BC][12];  // the two child boxes parked at each in-chunk split
+    __shared__ int32_t s_ref[BC][2];
+    __shared__ int32_t s_far[BC][2];
+    __shared__ uint8_t s_h[BC][2];
+    __shared__ uint8_t s_side[BC];   // side of the parked child (meaningful with one arrival)
+    __shared__ uint32_t s_cnt[BC];   // arrivals per in-chunk split
+    __shared__ uint32_t s_list[256];
     __shared__ uint32_t s_n, s_base;
-    BoxAppend ap{s_list, &s_n, &s_base};
-    for (int64_t tile = (int64_t)blockIdx.x * BOX_TILE; tile < m; tile += (int64_t)gridDim.x * BOX_TILE) {
-        ap.begin();
-        for (int j = 0; j < BOX_ITEMS; ++j) {
-            int64_t i = tile + (int64_t)j * 256 + threadIdx.x;
-            if (i >= m) break;
-            int4 d = nodes[i].d;
-            uint32_t nl = 0;
-            float b[6];
-            if (d.x < 0) {
-                leaf_box(leaf_sph[~d.x], b);
-                store_child_box(nodes + i, 0, b);
-                ++nl;
+    SplitList ap{s_list, &s_n, &s_base};
+    const int64_t cs = (int64_t)blockIdx.x * BC;
+    const int64_t ce = cs + BC < n ? cs + BC : n;
+    for (int k = threadIdx.x; k < BC; k += 256) s_cnt[k] = 0;
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    for (int64_t k = cs + threadIdx.x; k < ce; k += 256) {
+        float b[6];
+        leaf_box(leaf_sph[k], b);
+        float f[12];
+        int2 kd = make_int2(0, 0);
+        int h = 0;
+        int64_t l = k, r = k;
+        int32_t ref = ~(int32_t)k;
+        bool internal = false;
+        while (true) {
+            if (l == 0 && r == n - 1) {  // the root
+                store_node(nodes, f, kd, h);
+                info->height = (uint32_t)h;
+                break;
             }
-            if (d.y < 0) {
-                leaf_box(leaf_sph[~d.y], b);
-                store_child_box(nodes + i, 1, b);
-                ++nl;
+            const int dl = split_delta(keys, n, l - 1), dr = split_delta(keys, n, r);
+            const bool left = dr > dl;
+            const int64_t s = left ? r : l - 1;
+            if (internal) {
+                const int64_t idx = left ? r : l;
+                ref = (int32_t)idx;
+                store_node(nodes + idx, f, kd, h);
             }
-            flags[i] = nl;
-            if (nl == 2) ap.push((uint32_t)i);
+            const int side = left ? 0 : 1;
+            const int32_t far = (int32_t)(left ? l : r);
+            if (s < cs || s + 1 >= ce) {  // the split lies on a chunk boundary
+                meet_put(meet + s, meet_far + s, side, b, ref, h, far);
+                if (atomicAdd(flags + s, 1u) == 1u) ap.push((uint32_t)s, list, cnt + 1);
+                break;
+            }
+            const int slot = (int)(s - cs);
+#pragma unroll
+            for (int a = 0; a < 6; ++a) s_box[slot][side * 6 + a] = b[a];
+            s_ref[slot][side] = ref;
+            s_far[slot][side] = far;
+            s_h[slot][side] = (uint8_t)h;
+            s_side[slot] = (uint8_t)side;
+            uint32_t old = __hip_atomic_fetch_add(&s_cnt[slot], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (old == 0u) break;  // the sibling's thread merges
+#pragma unroll
+            for (int a = 0; a < 12; ++a) f[a] = s_box[slot][a];
+#pragma unroll
+            for (int a = 0; a < 6; ++a) f[side * 6 + a] = b[a];
+            kd = side == 0 ? make_int2(ref, s_ref[slot][1]) : make_int2(s_ref[slot][0], ref);
+            const int hs = s_h[slot][1 - side];
+            const int64_t fo = s_far[slot][1 - side];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                b[a] = fminf(f[a], f[6 + a]);
+                b[3 + a] = fmaxf(f[3 + a], f[9 + a]);
+            }
+            h = 1 + (h > hs ? h : hs);
+            if (side == 0)
+                r = fo;
+            else
+                l = fo;
+            internal = true;
         }
-        ap.flush(list, cnt + 1);
     }
+    __syncthreads();
+    // in-chunk splits with one arrival: the other child straddles the chunk, so the parked
+    // child moves to the global meeting place
+    const int nsplit = (int)(ce - cs) - 1;
+    for (int slot = threadIdx.x; slot < nsplit; slot += 256) {
+        if (s_cnt[slot] != 1u) continue;
+        const int side = s_side[slot];
+        const int64_t s = cs + slot;
+        float b[6];
+#pragma unroll
+        for (int a = 0; a < 6; ++a) b[a] = s_box[slot][side * 6 + a];
+        meet_put(meet + s, meet_far + s, side, b, s_ref[slot][side], s_h[slot][side], s_far[slot][side]);
+        if (atomicAdd(flags + s, 1u) == 1u) ap.push((uint32_t)s, list, cnt + 1);
+    }
+    ap.flush(list, cnt + 1);
 }
 
-// Union the two slots of node p, record its height, hand the box to the parent.
-// Returns the parent when this arrival completes it, else -1.
-template <bool VOL>
-__device__ __forceinline__ int32_t box_step(Node* nodes, const int32_t* __restrict__ parent, uint32_t* flags,
-                                            int32_t p, SceneInfo* info) {
-    float c[12];
-    int4 d;
+// Merge the two records at split s, write the node, deliver it to its parent's meeting place.
+// Returns the parent's split when this delivery completes it, else -1.
+template <typename K, bool VOL>
+__device__ __forceinline__ int64_t bu_step(const K* __restrict__ keys, int64_t n, Node* nodes, Node* meet,
+                                           int2* meet_far, uint32_t* flags, int64_t s, SceneInfo* info) {
+    float f[12];
+    int dv[4];
+    int fr[2];
     if constexpr (VOL) {
-        const volatile float* f = reinterpret_cast<const volatile float*>(nodes + p);
+        const volatile float* src = reinterpret_cast<const volatile float*>(meet + s);
 #pragma unroll
-        for (int q = 0; q < 12; ++q) c[q] = f[q];
-        const volatile int* di = reinterpret_cast<const volatile int*>(&nodes[p].d);
-        d = make_int4(di[0], di[1], di[2], di[3]);
+        for (int q = 0; q < 12; ++q) f[q] = src[q];
+        const volatile int* di = reinterpret_cast<const volatile int*>(&meet[s].d);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) dv[q] = di[q];
+        const volatile int* fi = reinterpret_cast<const volatile int*>(meet_far + s);
+        fr[0] = fi[0];
+        fr[1] = fi[1];
     } else {
-        const float* f = reinterpret_cast<const float*>(nodes + p);
+        const float* src = reinterpret_cast<const float*>(meet + s);
 #pragma unroll
-        for (int q = 0; q < 12; ++q) c[q] = f[q];
-        d = nodes[p].d;
+        for (int q = 0; q < 12; ++q) f[q] = src[q];
+        int4 d = meet[s].d;
+        dv[0] = d.x; dv[1] = d.y; dv[2] = d.z; dv[3] = d.w;
+        int2 ff = meet_far[s];
+        fr[0] = ff.x;
+        fr[1] = ff.y;
+    }
+    const int2 kd = make_int2(dv[0], dv[1]);
+    const int h = 1 + (dv[2] > dv[3] ? dv[2] : dv[3]);
+    const int64_t l = fr[0], r = fr[1];
+    if (l == 0 && r == n - 1) {
+        store_node(nodes, f, kd, h);
+        info->height = (uint32_t)h;
+        return -1;
     }
     float b[6];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        b[a] = fminf(c[a], c[6 + a]);
-        b[3 + a] = fmaxf(c[3 + a], c[9 + a]);
+        b[a] = fminf(f[a], f[6 + a]);
+        b[3 + a] = fmaxf(f[3 + a], f[9 + a]);
     }
-    int h0 = 0, h1 = 0;
-    if (d.x >= 0) h0 = VOL ? ((const volatile int*)&nodes[d.x].d)[3] : nodes[d.x].d.w;
-    if (d.y >= 0) h1 = VOL ? ((const volatile int*)&nodes[d.y].d)[3] : nodes[d.y].d.w;
-    int h = 1 + (h0 > h1 ? h0 : h1);
-    reinterpret_cast<int*>(&nodes[p].d)[3] = h;
-    int32_t pp = parent[p];
-    if (pp < 0) {
-        info->height = (uint32_t)h;
-        return -1;
-    }
-    int first = VOL ? ((const volatile int*)&nodes[pp].d)[0] : nodes[pp].d.x;
-    store_child_box(nodes + pp, first == p ? 0 : 1, b);
-    uint32_t old = atomicAdd(flags + pp, 1u);
-    return old == 1u ? pp : -1;
+    const int dl = split_delta(keys, n, l - 1), dr = split_delta(keys, n, r);
+    const bool left = dr > dl;
+    const int64_t idx = left ? r : l;
+    store_node(nodes + idx, f, kd, h);
+    const int64_t s2 = left ? r : l - 1;
+    meet_put(meet + s2, meet_far + s2, left ? 0 : 1, b, (int32_t)idx, h, (int32_t)(left ? l : r));
+    return atomicAdd(flags + s2, 1u) == 1u ? s2 : -1;
 }
 
-__device__ void box_finish(Node* nodes, const int32_t* __restrict__ parent, uint32_t* flags, uint32_t* la,
-                           uint32_t* lb, uint32_t n_cur, uint32_t* cnt, SceneInfo* info) {
+template <typename K>
+__device__ void bu_finish(const K* __restrict__ keys, int64_t n, Node* nodes, Node* meet, int2* meet_far,
+                          uint32_t* flags, uint32_t* la, uint32_t* lb, uint32_t n_cur, uint32_t* cnt, SceneInfo* info) {
     __shared__ uint32_t s_out;
     uint32_t* lin = la;
     uint32_t* lout = lb;
@@ -516,9 +587,9 @@ __device__ void box_finish(Node* nodes, const int32_t* __restrict__ parent, uint
         for (uint32_t base = 0; base < n_cur; base += blockDim.x) {
             uint32_t i = base + threadIdx.x;
             if (i < n_cur) {
-                int32_t p = (int32_t)((const volatile uint32_t*)lin)[i];
-                int32_t pp = box_step<true>(nodes, parent, flags, p, info);
-                if (pp >= 0) lout[atomicAdd(&s_out, 1u)] = (uint32_t)pp;
+                int64_t s = (int64_t)((const volatile uint32_t*)lin)[i];
+                int64_t s2 = bu_step<K, true>(keys, n, nodes, meet, meet_far, flags, s, info);
+                if (s2 >= 0) lout[atomicAdd(&s_out, 1u)] = (uint32_t)s2;
             }
         }
         __threadfence_block();
@@ -533,48 +604,52 @@ __device__ void box_finish(Node* nodes, const int32_t* __restrict__ parent, uint
 }
 
 // Round r: worklist r lives in list[r & 1], its size in cnt[1 + r]; cnt[0] marks completion.
-__global__ __launch_bounds__(256) void k_box_round(Node* nodes, const int32_t* __restrict__ parent, uint32_t* flags,
-                                                   uint32_t* l0, uint32_t* l1, uint32_t* cnt, int r, SceneInfo* info) {
-    __shared__ uint32_t s_list[BOX_TILE];
+template <typename K>
+__global__ __launch_bounds__(256) void k_bu_round(const K* __restrict__ keys, int64_t n, Node* nodes, Node* meet,
+                                                  int2* meet_far, uint32_t* flags, uint32_t* l0, uint32_t* l1,
+                                                  uint32_t* cnt, int r, SceneInfo* info) {
+    __shared__ uint32_t s_list[256];
     __shared__ uint32_t s_n, s_base;
     if (cnt[0]) return;
     uint32_t n_in = cnt[1 + r];
     uint32_t* lin = (r & 1) ? l1 : l0;
     uint32_t* lout = (r & 1) ? l0 : l1;
     if (n_in <= BOX_SMALL) {
-        if (blockIdx.x == 0) box_finish(nodes, parent, flags, lin, lout, n_in, cnt, info);
+        if (blockIdx.x == 0) bu_finish<K>(keys, n, nodes, meet, meet_far, flags, lin, lout, n_in, cnt, info);
         return;
     }
-    BoxAppend ap{s_list, &s_n, &s_base};
+    SplitList ap{s_list, &s_n, &s_base};
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
     for (uint32_t tile = blockIdx.x * BOX_TILE; tile < n_in; tile += gridDim.x * BOX_TILE) {
-        ap.begin();
         for (int j = 0; j < BOX_ITEMS; ++j) {
             uint32_t i = tile + j * 256 + threadIdx.x;
             if (i >= n_in) break;
-            int32_t pp = box_step<false>(nodes, parent, flags, (int32_t)lin[i], info);
-            if (pp >= 0) ap.push((uint32_t)pp);
+            int64_t s2 = bu_step<K, false>(keys, n, nodes, meet, meet_far, flags, (int64_t)lin[i], info);
+            if (s2 >= 0) ap.push((uint32_t)s2, lout, cnt + 2 + r);
         }
         ap.flush(lout, cnt + 2 + r);
     }
 }
 
 // Whatever is left after BOX_ROUNDS kernel rounds: one workgroup finishes it.
-__global__ __launch_bounds__(256) void k_box_tail(Node* nodes, const int32_t* __restrict__ parent, uint32_t* flags,
-                                                  uint32_t* l0, uint32_t* l1, uint32_t* cnt, SceneInfo* info) {
+template <typename K>
+__global__ __launch_bounds__(256) void k_bu_tail(const K* __restrict__ keys, int64_t n, Node* nodes, Node* meet,
+                                                 int2* meet_far, uint32_t* flags, uint32_t* l0, uint32_t* l1,
+                                                 uint32_t* cnt, SceneInfo* info) {
     if (cnt[0]) return;
     int r = BOX_ROUNDS;
     uint32_t* lin = (r & 1) ? l1 : l0;
     uint32_t* lout = (r & 1) ? l0 : l1;
-    box_finish(nodes, parent, flags, lin, lout, cnt[1 + r], cnt, info);
+    bu_finish<K>(keys, n, nodes, meet, meet_far, flags, lin, lout, cnt[1 + r], cnt, info);
 }
 
 // Single-sphere scene: one node whose two children are the same leaf.
 __global__ void k_single(const float4* __restrict__ leaf_sph, Node* nodes, SceneInfo* info) {
     float4 s = leaf_sph[0];
-    float b[6] = {s.x - s.w, s.y - s.w, s.z - s.w, s.x + s.w, s.y + s.w, s.z + s.w};
-    store_child_box(nodes, 0, b);
-    store_child_box(nodes, 1, b);
-    nodes[0].d = make_int4(~0, ~0, -1, 1);
+    float f[12] = {s.x - s.w, s.y - s.w, s.z - s.w, s.x + s.w, s.y + s.w, s.z + s.w,
+                   s.x - s.w, s.y - s.w, s.z - s.w, s.x + s.w, s.y + s.w, s.z + s.w};
+    store_node(nodes, f, make_int2(~0, ~0), 1);
     info->height = 1;
 }
 
@@ -622,6 +697,33 @@ int sort_pairs(mmhrt_ctx_s* c, K* ka, K* kb, uint32_t* va, uint32_t* vb, uint64_
     return 0;  // passes is even: result is back in (ka, va)
 }
 
+// Hierarchy and boxes over the sorted keys and gathered leaf spheres (also the refit: the
+// same keys give the same tree, the boxes come from the current spheres).
+template <typename K>
+int hierarchy(mmhrt_ctx_s* c, const K* keys) {
+    uint64_t n = c->n;
+    hipStream_t s = c->stream;
+    if (n == 1) {
+        hipLaunchKernelGGL(k_single, dim3(1), dim3(1), 0, s, c->leaf_sph.p, c->nodes.p, c->info.p);
+        return 0;
+    }
+    uint64_t m = n - 1;
+    if (int e = ensure(c, c->box_cnt, BOX_ROUNDS + 4, "box round counters")) return e;
+    if (int e = ensure(c, c->meet, m, "meeting records")) return e;
+    if (int e = ensure(c, c->meet_far, m, "meeting ranges")) return e;
+    if (int e = ensure(c, c->flags, m, "meeting counters")) return e;
+    launch_fill_u32(s, c->box_cnt.p, 0u, BOX_ROUNDS + 4);
+    launch_fill_u32(s, c->flags.p, 0u, m);
+    hipLaunchKernelGGL(k_bu_chunk<K>, dim3(blocks_for(n, BC)), dim3(256), 0, s, keys, c->leaf_sph.p, (int64_t)n,
+                       c->nodes.p, c->meet.p, c->meet_far.p, c->flags.p, c->va.p, c->box_cnt.p, c->info.p);
+    for (int r = 0; r < BOX_ROUNDS; ++r)
+        hipLaunchKernelGGL(k_bu_round<K>, dim3(1024), dim3(256), 0, s, keys, (int64_t)n, c->nodes.p, c->meet.p,
+                           c->meet_far.p, c->flags.p, c->va.p, c->vb.p, c->box_cnt.p, r, c->info.p);
+    hipLaunchKernelGGL(k_bu_tail<K>, dim3(1), dim3(256), 0, s, keys, (int64_t)n, c->nodes.p, c->meet.p, c->meet_far.p,
+                       c->flags.p, c->va.p, c->vb.p, c->box_cnt.p, c->info.p);
+    return 0;
+}
+
 template <typename K>
 int build_typed(mmhrt_ctx_s* c, K* ka, K* kb) {
     uint64_t n = c->n;
@@ -632,14 +734,10 @@ int build_typed(mmhrt_ctx_s* c, K* ka, K* kb) {
     int passes = sizeof(K) == 8 ? 8 : 4;
     if (int e = sort_pairs<K>(c, ka, kb, c->va.p, c->vb.p, n, passes)) return e;
     rec(c, 3);
-    hipLaunchKernelGGL(k_gather<K>, dim3(blocks_for(n, 256)), dim3(256), 0, s, ka, c->va.p, cur_spheres(c), c->leaf_sph.p,
-                       c->leaf_id.p, c->keys64.p, n);
-    if (n > 1) {
-        hipLaunchKernelGGL(k_karras<K>, dim3(blocks_for(n - 1, 256)), dim3(256), 0, s, ka, (int64_t)n, c->nodes.p,
-                           c->parent.p);
-    }
+    hipLaunchKernelGGL(k_gather, dim3(blocks_for(n, 256)), dim3(256), 0, s, c->va.p, cur_spheres(c), c->leaf_sph.p,
+                       c->leaf_id.p, n);
     rec(c, 4);
-    return 0;
+    return hierarchy<K>(c, ka);
 }
 
 }  // namespace
@@ -660,27 +758,6 @@ int reduce_scene(mmhrt_ctx_s* c) {
     return 0;
 }
 
-static int boxes(mmhrt_ctx_s* c) {
-    uint64_t n = c->n;
-    hipStream_t s = c->stream;
-    if (n == 1) {
-        hipLaunchKernelGGL(k_single, dim3(1), dim3(1), 0, s, c->leaf_sph.p, c->nodes.p, c->info.p);
-        return 0;
-    }
-    uint64_t m = n - 1;
-    if (int e = ensure(c, c->box_cnt, BOX_ROUNDS + 4, "box round counters")) return e;
-    launch_fill_u32(s, c->box_cnt.p, 0u, BOX_ROUNDS + 4);
-    unsigned ig = (unsigned)std::min<uint64_t>((m + BOX_TILE - 1) / BOX_TILE, 8192);
-    hipLaunchKernelGGL(k_box_init, dim3(ig), dim3(256), 0, s, c->leaf_sph.p, (int64_t)m, c->nodes.p,
-                       c->flags.p, c->va.p, c->box_cnt.p);
-    for (int r = 0; r < BOX_ROUNDS; ++r)
-        hipLaunchKernelGGL(k_box_round, dim3(1024), dim3(256), 0, s, c->nodes.p, c->parent.p, c->flags.p, c->va.p,
-                           c->vb.p, c->box_cnt.p, r, c->info.p);
-    hipLaunchKernelGGL(k_box_tail, dim3(1), dim3(256), 0, s, c->nodes.p, c->parent.p, c->flags.p, c->va.p, c->vb.p,
-                       c->box_cnt.p, c->info.p);
-    return 0;
-}
-
 int build_lbvh(mmhrt_ctx_s* c, int bits) {
     uint64_t n = c->n;
     if (n == 0) return fail(c, MMHRT_E_STATE, "build: no spheres set");
@@ -688,14 +765,11 @@ int build_lbvh(mmhrt_ctx_s* c, int bits) {
     c->bits = bits;
     uint64_t m = n > 1 ? n - 1 : 1;
     int e = 0;
-    if ((e = ensure(c, c->keys64, n, "keys"))) return e;
     if ((e = ensure(c, c->va, n, "sort values"))) return e;
     if ((e = ensure(c, c->vb, n, "sort values"))) return e;
     if ((e = ensure(c, c->leaf_sph, n, "leaf spheres"))) return e;
     if ((e = ensure(c, c->leaf_id, n, "leaf ids"))) return e;
     if ((e = ensure(c, c->nodes, m, "nodes"))) return e;
-    if ((e = ensure(c, c->parent, 2 * n, "parents"))) return e;
-    if ((e = ensure(c, c->flags, m, "flags"))) return e;
     if (bits == 63) {
         if ((e = ensure(c, c->kla, n, "keys"))) return e;
         if ((e = ensure(c, c->klb, n, "keys"))) return e;
@@ -710,7 +784,6 @@ int build_lbvh(mmhrt_ctx_s* c, int bits) {
     else
         e = build_typed<uint32_t>(c, c->ka.p, c->kb.p);
     if (e) return e;
-    if ((e = boxes(c))) return e;
     rec(c, 5);
     hipError_t he = hipGetLastError();
     if (he != hipSuccess) return hip_fail(c, he, "LBVH build launch");
@@ -731,7 +804,8 @@ int refit_lbvh(mmhrt_ctx_s* c) {
     hipLaunchKernelGGL(k_regather, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, cur_spheres(c), c->leaf_id.p,
                        c->leaf_sph.p, n);
     rec(c, 4);
-    if (int e = boxes(c)) return e;
+    int e = c->bits == 63 ? hierarchy<uint64_t>(c, c->kla.p) : hierarchy<uint32_t>(c, c->ka.p);
+    if (e) return e;
     rec(c, 5);
     hipError_t he = hipGetLastError();
     if (he != hipSuccess) return hip_fail(c, he, "LBVH refit launch");
