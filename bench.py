@@ -302,6 +302,21 @@ def main():
     achieved = algo_bytes / (trace_ms * 1e-3) / 1e9
     mrays = rays / (ms * 1e-3) / 1e6
     log(f"visits {visits} tests {tests} rays {rays} overflow {overflow} hit px {hit_px}")
+    split = None
+    if mode == L.MODE_AO:
+        # the same primary rays alone (Lambert shading): the AO rays' share by difference
+        pl = L.RenderParams.from_buffer_copy(p)
+        pl.mode = L.MODE_LAMBERT
+        r.set_option("count_visits", 1)
+        r.render_async(cam, pl)
+        pv, pt, pr = r.counters()
+        r.set_option("count_visits", 0)
+        ao_rays = max(rays - pr, 1)
+        split = {"primary": {"rays": int(pr), "visits_per_ray": round(pv / max(pr, 1), 2),
+                             "tests_per_ray": round(pt / max(pr, 1), 2)},
+                 "ao": {"rays": int(ao_rays), "visits_per_ray": round((visits - pv) / ao_rays, 2),
+                        "tests_per_ray": round((tests - pt) / ao_rays, 2)}}
+        log(f"per-ray work: {split}")
 
     dn = None
     if mode == L.MODE_AO and args.config == "fcc" and not args.no_denoise:
@@ -346,6 +361,7 @@ def main():
         "build_ms": round(build_ms, 3),
         "build_stages_ms": build_stages,
         "trace_ms": round(trace_ms, 3),
+        "trace_work": split,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "k_trace (AO, persistent)", "algorithmic_bytes": int(algo_bytes)},

@@ -445,6 +445,7 @@ static int64_t* option_slot(mmhrt_ctx c, const char* key) {
     if (k == "tile_w") return &c->opt_tile_w;
     if (k == "tile_h") return &c->opt_tile_h;
     if (k == "refit_period") return &c->opt_refit_period;
+    if (k == "dn_k32") return &c->opt_dn_k32;
     if (k == "stack_overflow") return &c->stack_overflow;
     return nullptr;
 }

@@ -15,7 +15,9 @@
 //
 // Lane layout of v_mfma_f32_16x16x16_bf16 (64 lanes): A holds row l%16, k = 4*(l/16)..+3;
 // B holds column l%16, k = 4*(l/16)..+3; the 4 results of a lane are rows 4*(l/16)..+3 of
-// column l%16.
+// column l%16. v_mfma_f32_16x16x32_bf16 is the same with 8 k values per lane (k = 8*(l/16)..+7).
+// The forward pass uses the latter over K = 9 taps x CI input channels taken 32 at a time (a
+// lane's 8 values are 8 consecutive channels of one tap, one 16-byte load).
 #include <math.h>
 
 #include <algorithm>
@@ -44,6 +46,11 @@ union Frag {
     bf16x4 v;
     uint2 u;
     uint16_t h[4];
+};
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+union Frag8 {
+    bf16x8 v;
+    uint4 u;
 };
 
 // ------------------------------------------------------------------ forward convolution
@@ -116,6 +123,88 @@ __global__ __launch_bounds__(256) void k_conv_fwd(ConvArgs a) {
                 float v = acc[pb][j][e] + bias;
                 if (a.relu) v = fmaxf(v, 0.0f);
                 size_t o = (((size_t)b * a.H + y) * a.W + x) * a.CO + co;
+                if (a.Yf)
+                    a.Yf[o] = v;
+                else
+                    a.Y[o] = f2bf(v);
+            }
+    }
+}
+
+// The same convolution with v_mfma_f32_16x16x32_bf16 over 32-wide chunks of k = tap*CI + ci.
+template <int NB>
+__global__ __launch_bounds__(256) void k_conv_fwd32(ConvArgs a) {
+    const int lane = __lane_id();
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int runs_per_row = (a.W + 63) / 64;
+    const int row = wave / runs_per_row;  // b*H + y
+    if (row >= a.B * a.H) return;
+    const int x0 = (wave % runs_per_row) * 64;
+    const int b = row / a.H, y = row % a.H;
+    const int nb0 = blockIdx.y * NB;
+    const int CI = a.C1 + a.C2;
+    const int K9 = 9 * CI;
+    const int nkc = (K9 + 31) / 32;
+    const int NBT = a.CO / 16;
+    f32x4 acc[4][NB];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int q = lane >> 4;   // which 8 of the chunk's 32 k values
+    const int lr = lane & 15;  // row (A) / column (B)
+    for (int kc = 0; kc < nkc; ++kc) {
+        Frag8 fa[4];
+#pragma unroll
+        for (int pb = 0; pb < 4; ++pb) fa[pb].u = make_uint4(0u, 0u, 0u, 0u);
+        const int k0 = kc * 32 + q * 8;
+        if (k0 < K9) {
+            const int t = k0 / CI;
+            const int ci0 = k0 - t * CI;
+            const int yy = y + t / 3 - 1;
+            const int dx = t % 3 - 1;
+            if (yy >= 0 && yy < a.H) {
+#pragma unroll
+                for (int pb = 0; pb < 4; ++pb) {
+                    const int xx = x0 + pb * 16 + lr + dx;
+                    if (xx >= 0 && xx < a.W) {
+                        const uint16_t* src;
+                        if (ci0 < a.C1)
+                            src = a.X1 + (((size_t)b * a.H + yy) * a.W + xx) * a.C1 + ci0;
+                        else
+                            src = a.X2 + (((size_t)b * (a.H >> 1) + (yy >> 1)) * (a.W >> 1) + (xx >> 1)) * a.C2 +
+                                  (ci0 - a.C1);
+                        fa[pb].u = *reinterpret_cast<const uint4*>(src);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            const int nb = nb0 + j;
+            if (nb >= NBT) break;
+            Frag8 fb;
+            fb.u = *reinterpret_cast<const uint4*>(a.Wt32 + (((size_t)kc * NBT + nb) * 64 + lane) * 8);
+#pragma unroll
+            for (int pb = 0; pb < 4; ++pb)
+                acc[pb][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[pb].v, fb.v, acc[pb][j], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        const int nb = nb0 + j;
+        if (nb >= NBT) break;
+        const int co = nb * 16 + lr;
+        const float bias = a.bias[co];
+#pragma unroll
+        for (int pb = 0; pb < 4; ++pb)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int x = x0 + pb * 16 + q * 4 + e;
+                if (x >= a.W) continue;
+                float v = acc[pb][j][e] + bias;
+                if (a.relu) v = fmaxf(v, 0.0f);
+                const size_t o = (((size_t)b * a.H + y) * a.W + x) * a.CO + co;
                 if (a.Yf)
                     a.Yf[o] = v;
                 else
@@ -481,6 +570,24 @@ __global__ void k_relayout(const float* __restrict__ Wc, int CI, int CO, uint16_
     Wb[((((size_t)t * KC + co / 16) * NBI + ci / 16) * 16 + ci % 16) * 16 + co % 16] = v;
 }
 
+// Canonical W[t][ci][co] -> [k/32][co/16][lane][8] with k = t*CI + ci, zero past 9*CI.
+__global__ void k_relayout32(const float* __restrict__ Wc, int CI, int CO, uint16_t* __restrict__ W32) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nkc = (9 * CI + 31) / 32;
+    long n = (long)nkc * 32 * CO;
+    if (i >= n) return;
+    const int e = (int)(i % 8);
+    long r = i / 8;
+    const int lane = (int)(r % 64);
+    r /= 64;
+    const int NBT = CO / 16;
+    const int nb = (int)(r % NBT);
+    const int kc = (int)(r / NBT);
+    const int k = kc * 32 + 8 * (lane >> 4) + e;
+    const int co = nb * 16 + (lane & 15);
+    W32[i] = k < 9 * CI ? f2bf(Wc[(size_t)k * CO + co]) : (uint16_t)0;
+}
+
 __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
                        float* __restrict__ v, long n, float lr, float b1, float b2, float eps, float c1, float c2) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -522,11 +629,22 @@ static size_t param_offset(int l) {
     return n;
 }
 
-static void conv_fwd(hipStream_t s, const ConvArgs& a) {
+static void conv_fwd(hipStream_t s, const ConvArgs& a, bool k32) {
     long waves = (long)a.B * a.H * ((a.W + 63) / 64);
     int NBT = a.CO / 16;
     int nb = NBT <= 6 ? NBT : 4;
     dim3 g(nblk(waves * 64), (NBT + nb - 1) / nb);
+    if (k32) {
+        switch (nb) {
+        case 1: hipLaunchKernelGGL(k_conv_fwd32<1>, g, dim3(256), 0, s, a); break;
+        case 2: hipLaunchKernelGGL(k_conv_fwd32<2>, g, dim3(256), 0, s, a); break;
+        case 3: hipLaunchKernelGGL(k_conv_fwd32<3>, g, dim3(256), 0, s, a); break;
+        case 4: hipLaunchKernelGGL(k_conv_fwd32<4>, g, dim3(256), 0, s, a); break;
+        case 5: hipLaunchKernelGGL(k_conv_fwd32<5>, g, dim3(256), 0, s, a); break;
+        default: hipLaunchKernelGGL(k_conv_fwd32<6>, g, dim3(256), 0, s, a); break;
+        }
+        return;
+    }
     switch (nb) {
     case 1: hipLaunchKernelGGL(k_conv_fwd<1>, g, dim3(256), 0, s, a); break;
     case 2: hipLaunchKernelGGL(k_conv_fwd<2>, g, dim3(256), 0, s, a); break;
@@ -594,18 +712,24 @@ int Denoiser::ensure_params(mmhrt_ctx_s* c) {
     for (const auto& L : kLayers) wn += 9ull * (L.c1 + L.c2) * L.co;
     if ((e = mmhrt::ensure(c, wf, wn, "denoiser fwd weights"))) return e;
     if ((e = mmhrt::ensure(c, wb, wn, "denoiser bwd weights"))) return e;
+    size_t wn32 = 0;
+    for (const auto& L : kLayers) wn32 += (size_t)((9 * (L.c1 + L.c2) + 31) / 32) * 32 * L.co;
+    if ((e = mmhrt::ensure(c, wf32, wn32, "denoiser fwd weights (k32)"))) return e;
     if ((e = mmhrt::ensure(c, scalar, 4, "denoiser scalars"))) return e;
     return 0;
 }
 
 void Denoiser::relayout(hipStream_t s) {
-    size_t po = 0, wo = 0;
+    size_t po = 0, wo = 0, w32 = 0;
     for (const auto& L : kLayers) {
         int CI = L.c1 + L.c2;
         long nw = 9L * CI * L.co;
+        long n32 = (long)((9 * CI + 31) / 32) * 32 * L.co;
         hipLaunchKernelGGL(k_relayout, dim3(nblk(nw)), dim3(256), 0, s, params.p + po, CI, L.co, wf.p + wo, wb.p + wo);
+        hipLaunchKernelGGL(k_relayout32, dim3(nblk(n32)), dim3(256), 0, s, params.p + po, CI, L.co, wf32.p + w32);
         po += nw + L.co;
         wo += nw;
+        w32 += n32;
     }
 }
 
@@ -644,16 +768,18 @@ int Denoiser::ensure_acts(mmhrt_ctx_s* c, int B, int Hp, int Wp, bool train) {
 // Forward pass over X0 [B,Hp,Wp,16] into `out` (fp32, channel 0 used).
 void Denoiser::forward(hipStream_t s, int B, int Hp, int Wp) {
     int Hs[4] = {Hp, Hp / 2, Hp / 4, Hp / 8}, Ws[4] = {Wp, Wp / 2, Wp / 4, Wp / 8};
-    size_t wo = 0, po = 0;
+    size_t wo = 0, po = 0, w32 = 0;
     auto conv = [&](int l, const uint16_t* x1, const uint16_t* x2, int lv, uint16_t* y, float* yf) {
         const LayerSpec& L = kLayers[l];
         ConvArgs A{};
         A.X1 = x1; A.X2 = x2; A.C1 = L.c1; A.C2 = L.c2; A.CO = L.co;
         A.B = B; A.H = Hs[lv]; A.W = Ws[lv];
-        A.Wt = wf.p + wo; A.bias = params.p + po + 9L * (L.c1 + L.c2) * L.co;
+        A.Wt = wf.p + wo; A.Wt32 = wf32.p + w32;
+        A.bias = params.p + po + 9L * (L.c1 + L.c2) * L.co;
         A.Y = y; A.Yf = yf; A.relu = L.relu;
-        conv_fwd(s, A);
+        conv_fwd(s, A, k32);
         wo += 9L * (L.c1 + L.c2) * L.co;
+        w32 += (size_t)((9 * (L.c1 + L.c2) + 31) / 32) * 32 * L.co;
         po += 9L * (L.c1 + L.c2) * L.co + L.co;
     };
     auto pl = [&](const uint16_t* x, uint16_t* y, int lv, int C) {
@@ -783,6 +909,7 @@ int dn_run(mmhrt_ctx c, int B, int H, int W, float depth_scale, const float bg[4
     float inv_depth = depth_scale > 0.0f ? 1.0f / depth_scale : 1.0f;
     hipLaunchKernelGGL(mmhrt::dn::k_pack_input, dim3((unsigned)((Pp + 255) / 256)), dim3(256), 0, s, d.raw.p, B, H,
                        W, Hp, Wp, inv_depth, d.x0);
+    d.k32 = c->opt_dn_k32 != 0;
     d.forward(s, B, Hp, Wp);
     float4 bgc = make_float4(bg ? bg[0] : 0.f, bg ? bg[1] : 0.f, bg ? bg[2] : 0.f, bg ? bg[3] : 1.f);
     if (loss) mmhrt::launch_fill_u32(s, reinterpret_cast<uint32_t*>(d.scalar.p), 0u, 1);
@@ -927,7 +1054,7 @@ namespace mmhrt {
 namespace dn {
 void Denoiser::release_all() {
     mmhrt::release(params); mmhrt::release(grads); mmhrt::release(adam_m); mmhrt::release(adam_v);
-    mmhrt::release(scalar); mmhrt::release(wf); mmhrt::release(wb); mmhrt::release(acts); mmhrt::release(gbuf);
+    mmhrt::release(scalar); mmhrt::release(wf); mmhrt::release(wb); mmhrt::release(wf32); mmhrt::release(acts); mmhrt::release(gbuf);
     mmhrt::release(raw); mmhrt::release(out); mmhrt::release(target); mmhrt::release(color);
 }
 }  // namespace dn

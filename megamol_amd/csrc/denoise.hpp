@@ -27,7 +27,8 @@ struct ConvArgs {
     const uint16_t* X2;
     int C1, C2, CO;
     int B, H, W;
-    const uint16_t* Wt;
+    const uint16_t* Wt;    // [tap][ci/16][co/16][co%16][ci%16] (16x16x16 kernel)
+    const uint16_t* Wt32;  // [k/32][co/16][lane][8], k = tap*CI + ci zero-padded (16x16x32 kernel)
     const float* bias;
     uint16_t* Y;
     float* Yf;
@@ -58,6 +59,8 @@ struct Denoiser {
     int64_t step = 0;
     DevBuf<float> params, grads, adam_m, adam_v, scalar;
     DevBuf<uint16_t> wf, wb;   // bf16 weights in forward / backward fragment order
+    DevBuf<uint16_t> wf32;     // bf16 weights in the 16x16x32 forward fragment order
+    bool k32 = true;           // forward with v_mfma_f32_16x16x32_bf16
     DevBuf<uint16_t> acts;     // activations
     DevBuf<uint16_t> gbuf;     // backward temporaries
     DevBuf<uint16_t> raw;      // raw per-pixel features (12 halfs)

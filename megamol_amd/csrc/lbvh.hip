@@ -159,29 +159,41 @@ __global__ __launch_bounds__(256) void k_morton(const float4* __restrict__ sph, 
 }
 
 // ---------------------------------------------------------------- radix sort
+// A tile is RT consecutive keys; wave w of the block owns the contiguous segment
+// [w * RT/4, (w+1) * RT/4) and walks it in RI rounds of 64, so the wave order, the round order
+// and the lane order together are the input order (what makes the ranking stable). Every
+// thread issues all of its RI key (and value) loads before it ranks anything.
+constexpr int RW = RT / (RB / 64);  // keys per wave segment
+
 template <typename K>
 __global__ __launch_bounds__(RB) void k_radix_hist(const K* __restrict__ keys, uint64_t n, int shift,
                                                    uint32_t* __restrict__ hist, uint32_t tiles) {
     __shared__ uint32_t h[256];
     h[threadIdx.x] = 0;
-    __syncthreads();
-    uint64_t base = (uint64_t)blockIdx.x * RT;
+    const uint64_t base = (uint64_t)blockIdx.x * RT + (uint64_t)(threadIdx.x >> 6) * RW + __lane_id();
+    uint32_t d[RI];
+#pragma unroll
     for (int r = 0; r < RI; ++r) {
-        uint64_t i = base + (uint64_t)r * RB + threadIdx.x;
-        bool valid = i < n;
-        uint32_t d = valid ? (uint32_t)((keys[i] >> shift) & 255u) : 0u;
-        uint64_t peers = match_digit(d, valid);
+        uint64_t i = base + (uint64_t)r * 64;
+        d[r] = i < n ? (uint32_t)((keys[i] >> shift) & 255u) : 256u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RI; ++r) {
+        bool valid = d[r] < 256u;
+        uint64_t peers = match_digit(d[r], valid);
         if (valid && (__ffsll((unsigned long long)peers) - 1) == (int)__lane_id())
-            atomicAdd(&h[d], (uint32_t)__popcll(peers));
+            atomicAdd(&h[d[r]], (uint32_t)__popcll(peers));
     }
     __syncthreads();
     hist[(size_t)threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
 }
 
-// Stable scatter of one tile of RT keys for one 8-bit digit. The tile is first ranked into
-// digit-sorted order in LDS (tile-local digit starts from the tile's histogram, wave-ballot ranks
-// inside each round), then written out in digit-contiguous runs so that the global writes of
-// consecutive threads go to consecutive addresses.
+// Stable scatter of one tile for one 8-bit digit. Each wave ranks its segment with ballots and
+// a running per-(wave, digit) count in LDS (no barrier between rounds), one pass over the 256
+// digits turns the per-wave counts into offsets, the keys are placed in digit-sorted order in
+// LDS and finally written out in digit-contiguous runs, so that consecutive threads write
+// consecutive addresses.
 template <typename K>
 __global__ __launch_bounds__(RB) void k_radix_scatter(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                       K* __restrict__ kout, uint32_t* __restrict__ vout, uint64_t n,
@@ -189,15 +201,24 @@ __global__ __launch_bounds__(RB) void k_radix_scatter(const K* __restrict__ kin,
                                                       const uint32_t* __restrict__ offs, uint32_t tiles) {
     __shared__ K s_key[RT];
     __shared__ uint32_t s_val[RT];
-    __shared__ uint32_t running[256];        // next tile-local slot per digit
-    __shared__ uint32_t goff[256];           // global offset of the digit's run minus its local start
-    __shared__ uint32_t wcnt[RB / 64][256];  // per-wave digit counts of the current round
-    __shared__ uint32_t wpre[RB / 64][256];  // slot of the first key of (wave, digit) in this round
+    __shared__ uint32_t wc[RB / 64][256];  // per-wave running digit counts, then per-wave offsets
+    __shared__ uint32_t goff[256];         // global offset of the digit's run minus its local start
     __shared__ uint32_t wsum[RB / 64];
     const int wave = threadIdx.x >> 6;
     const uint32_t lane = __lane_id();
+    const uint64_t tbase = (uint64_t)blockIdx.x * RT;
+    const uint64_t base = tbase + (uint64_t)wave * RW + lane;
+    K key[RI];
+    uint32_t val[RI];
+#pragma unroll
+    for (int r = 0; r < RI; ++r) {
+        uint64_t i = base + (uint64_t)r * 64;
+        bool valid = i < n;
+        key[r] = valid ? kin[i] : (K)0;
+        val[r] = valid ? vin[i] : 0u;
+    }
     // tile-local digit starts: exclusive scan of this tile's digit counts (thread = digit)
-    uint32_t cnt = hist[(size_t)threadIdx.x * tiles + blockIdx.x];
+    const uint32_t cnt = hist[(size_t)threadIdx.x * tiles + blockIdx.x];
     uint32_t inc = cnt;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
@@ -205,50 +226,51 @@ __global__ __launch_bounds__(RB) void k_radix_scatter(const K* __restrict__ kin,
         if (lane >= (uint32_t)o) inc += y;
     }
     if (lane == 63) wsum[wave] = inc;
-    for (int w = 0; w < RB / 64; ++w) wcnt[w][threadIdx.x] = 0;
+#pragma unroll
+    for (int w = 0; w < RB / 64; ++w) wc[w][threadIdx.x] = 0;
     __syncthreads();
     uint32_t wb = 0;
     for (int w = 0; w < wave; ++w) wb += wsum[w];
     const uint32_t lstart = wb + inc - cnt;
-    running[threadIdx.x] = lstart;
     goff[threadIdx.x] = offs[(size_t)threadIdx.x * tiles + blockIdx.x] - lstart;
-    __syncthreads();
-    const uint64_t base = (uint64_t)blockIdx.x * RT;
-    for (int r = 0; r < RI; ++r) {
-        uint64_t i = base + (uint64_t)r * RB + threadIdx.x;
-        bool valid = i < n;
-        K key = valid ? kin[i] : (K)0;
-        uint32_t val = valid ? vin[i] : 0u;
-        uint32_t d = (uint32_t)((key >> shift) & 255u);
-        uint64_t peers = match_digit(d, valid);
-        uint32_t rank = (uint32_t)__popcll(peers & lanemask_lt());
-        if (valid && (__ffsll((unsigned long long)peers) - 1) == (int)lane) wcnt[wave][d] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        {
-            uint32_t acc = running[threadIdx.x];
+    // ranks inside the wave's segment
+    uint32_t rank[RI];
 #pragma unroll
-            for (int w = 0; w < RB / 64; ++w) {
-                uint32_t c = wcnt[w][threadIdx.x];
-                wpre[w][threadIdx.x] = acc;
-                acc += c;
-                wcnt[w][threadIdx.x] = 0;
-            }
-            running[threadIdx.x] = acc;
-        }
-        __syncthreads();
-        if (valid) {
-            uint32_t pos = wpre[wave][d] + rank;
-            s_key[pos] = key;
-            s_val[pos] = val;
-        }
-        __syncthreads();
+    for (int r = 0; r < RI; ++r) {
+        bool valid = base + (uint64_t)r * 64 < n;
+        uint32_t d = (uint32_t)((key[r] >> shift) & 255u);
+        uint64_t peers = match_digit(d, valid);
+        uint32_t before = valid ? wc[wave][d] : 0u;
+        rank[r] = before + (uint32_t)__popcll(peers & lanemask_lt());
+        if (valid && (__ffsll((unsigned long long)peers) - 1) == (int)lane) wc[wave][d] = before + (uint32_t)__popcll(peers);
     }
-    const uint32_t nvalid = (uint32_t)((n - base) < (uint64_t)RT ? (n - base) : (uint64_t)RT);
+    __syncthreads();
+    {  // thread = digit: per-wave counts -> tile-local offsets of each wave's run
+        uint32_t acc = lstart;
+#pragma unroll
+        for (int w = 0; w < RB / 64; ++w) {
+            uint32_t c = wc[w][threadIdx.x];
+            wc[w][threadIdx.x] = acc;
+            acc += c;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RI; ++r) {
+        if (base + (uint64_t)r * 64 < n) {
+            uint32_t d = (uint32_t)((key[r] >> shift) & 255u);
+            uint32_t pos = wc[wave][d] + rank[r];
+            s_key[pos] = key[r];
+            s_val[pos] = val[r];
+        }
+    }
+    __syncthreads();
+    const uint32_t nvalid = (uint32_t)((n - tbase) < (uint64_t)RT ? (n - tbase) : (uint64_t)RT);
     for (uint32_t i = threadIdx.x; i < nvalid; i += RB) {
-        K key = s_key[i];
-        uint32_t d = (uint32_t)((key >> shift) & 255u);
+        K k = s_key[i];
+        uint32_t d = (uint32_t)((k >> shift) & 255u);
         uint32_t g = goff[d] + i;
-        kout[g] = key;
+        kout[g] = k;
         vout[g] = s_val[i];
     }
 }

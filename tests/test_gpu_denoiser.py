@@ -26,9 +26,11 @@ def dnz(gpu_renderer):
     return D.Denoiser(gpu_renderer, seed=7)
 
 
+@pytest.mark.parametrize("k32", [0, 1])
 @pytest.mark.parametrize("shape", [(1, 16, 16), (2, 24, 40), (1, 37, 70)])
-def test_forward_matches_reference(dnz, shape):
-    case(f"denoiser forward {shape}")
+def test_forward_matches_reference(dnz, shape, k32):
+    case(f"denoiser forward {shape} k32={k32}")
+    dnz.r.set_option("dn_k32", k32)
     b, h, w = shape
     feats = random_features(b, h, w, seed=h)
     p = dnz.get_params()
@@ -37,6 +39,7 @@ def test_forward_matches_reference(dnz, shape):
     dnz.set_params(p)
     out = dnz.denoise(feats, depth_scale=2.0)
     ref = R.denoise(p, feats, 2.0, round_bf16=True)
+    dnz.r.set_option("dn_k32", 0)
     assert out.shape == ref.shape
     assert rel(out, ref) < 4e-2, rel(out, ref)
 
