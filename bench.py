@@ -136,20 +136,32 @@ def denoised_leg(r, scene, args, cam, p_plain):
     ref = colour(params(W, H, spp=256, mode=L.MODE_AO, ao_distance=args.ao_distance, frame_idx=7777))
     plain = colour(params(W, H, spp=args.spp, mode=L.MODE_AO, ao_distance=args.ao_distance))
     raw = colour(params(W, H, spp=1, mode=L.MODE_AO, ao_distance=args.ao_distance, flags=L.FLAG_AUX))
+    raw_aux = r.download(W, H, outputs=(), aux=True).aux
+    from megamol_amd.denoiser import box_filter_baseline
+    box = box_filter_baseline(raw_aux, (0.0, 0.0, 0.0))
     dnz.denoise_frame(depth_scale)
     den = r.download(W, H, outputs=("color",)).color[..., :3].copy()
     res = {"ms": round(ms, 3), "ms_without_denoiser": round(ms_nodn, 3),
            "denoiser_ms": round(ms - ms_nodn, 3),
            "plain_over_denoised": None,
            "psnr_db_vs_256spp": {"denoised_1spp": round(psnr(den, ref), 2), "plain": round(psnr(plain, ref), 2),
-                                 "raw_1spp": round(psnr(raw, ref), 2)},
+                                 "raw_1spp": round(psnr(raw, ref), 2), "box5_filter_1spp": round(psnr(box, ref), 2)},
            "ssim_vs_256spp": {"denoised_1spp": round(ssim(den, ref), 4), "plain": round(ssim(plain, ref), 4),
-                              "raw_1spp": round(ssim(raw, ref), 4)},
+                              "raw_1spp": round(ssim(raw, ref), 4), "box5_filter_1spp": round(ssim(box, ref), 4)},
            "weights": trained, "weights_meta": meta}
     log(f"denoised frame {ms:.2f} ms (no denoiser {ms_nodn:.2f}); PSNR {res['psnr_db_vs_256spp']}")
     if args.save_images:
-        np.savez_compressed(args.save_images, ref=ref.astype(np.float16), plain=plain.astype(np.float16),
-                            raw=raw.astype(np.float16), den=den.astype(np.float16))
+        # centre crops for offline error analysis: the 256-spp mean albedo (AO distance ~0: every
+        # sample unoccluded) and the 1-spp feature records next to the colour images
+        alb = colour(params(W, H, spp=256, mode=L.MODE_AO, ao_distance=1e-7, frame_idx=7777))
+        from megamol_amd.denoiser import features_from_frame
+        feats = features_from_frame(np.concatenate([raw, np.ones_like(raw[..., :1])], -1), raw_aux)
+        c = args.crop
+        y0, x0 = (H - c) // 2, (W - c) // 2
+        sl = (slice(y0, y0 + c), slice(x0, x0 + c))
+        np.savez_compressed(args.save_images, ref=ref[sl].astype(np.float16), plain=plain[sl].astype(np.float16),
+                            raw=raw[sl].astype(np.float16), den=den[sl].astype(np.float16),
+                            alb256=alb[sl].astype(np.float16), feats=feats[sl])
     return res
 
 
@@ -213,7 +225,8 @@ def main():
                     default=os.path.join(ROOT, "megamol_amd", "weights", "denoiser_fcc.bin"))
     ap.add_argument("--train-pairs", type=int, default=256)
     ap.add_argument("--train-steps", type=int, default=600)
-    ap.add_argument("--save-images", type=str, default="")
+    ap.add_argument("--save-images", type=str, default="", help="save centre crops of the denoised leg's images")
+    ap.add_argument("--crop", type=int, default=512)
     ap.add_argument("--cells", type=int, default=232)
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--width", type=int, default=1920)
@@ -225,7 +238,20 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=8)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--option", action="append", default=[], help="key=value library option")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB five times (calibrates the HBM "
+                         "byte counters of a profiler against a kernel of known traffic)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        n = args.copy_probe * (1 << 20) // 4
+        src = torch.arange(n, dtype=torch.int32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n, "launches": 5}))
+        return
     if args.config == "cube" and args.width == 1920:
         args.width = args.height = 1024
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:

@@ -228,12 +228,13 @@ MMHRT_API int32_t mmhrt_render_batch(mmhrt_ctx ctx, const mmhrt_job* jobs, uint3
 /* ---- learned denoiser ------------------------------------------------------
  * No counterpart in the reference (optix_hpg has no denoiser). A 4-level U-Net (3x3 convs,
  * 32/48/64/96 channels, ReLU, skip concatenation) run and trained with BF16 MFMA kernels that
- * accumulate in FP32. It predicts per-pixel AO visibility v; the output colour is albedo * v on
- * hit pixels and the background elsewhere.
+ * accumulate in FP32. It predicts per-pixel AO visibility v (output channel 0) and a colour
+ * residual r (channels 1-3); the output colour is albedo * v + r on hit pixels and the
+ * background elsewhere (the residual corrects the one-sample albedo towards the pixel's mean).
  * Parameters (fp32, canonical order): per layer W[tap][c_in][c_out] then bias[c_out]; layers
  * enc0a(16->32) enc0b(32->32) enc1a(32->48) enc1b(48->48) enc2a(48->64) enc2b(64->64)
  * bota(64->96) botb(96->96) dec2a(64+96->64) dec2b(64->64) dec1a(48+64->48) dec1b(48->48)
- * dec0a(32+48->32) dec0b(32->32) out(32->16, channel 0 used, no ReLU).
+ * dec0a(32+48->32) dec0b(32->32) out(32->16, channels 0-3 used, no ReLU).
  * Feature record per pixel (12 fp16): colour rgb, albedo rgb, normal xyz, linear depth t
  * (0 on miss), visibility, 0 — the colour channels of a render plus its MMHRT_FLAG_AUX buffer.
  * depth_scale divides t before it enters the network. */

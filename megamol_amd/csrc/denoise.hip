@@ -485,8 +485,9 @@ __global__ void k_frame_features(const float4* __restrict__ col, const __half* _
     d[11] = 0;
 }
 
-// Network output (channel 0 of OUT, fp32 [B,Hp,Wp,16]) -> colour albedo * v on hit pixels,
-// background elsewhere. With a target, also the squared error and d(loss)/d(out).
+// Network output (fp32 [B,Hp,Wp,16]: v in channel 0, colour residual in 1-3) -> colour
+// albedo * v + residual on hit pixels, background elsewhere. With a target, also the squared
+// error and d(loss)/d(out).
 __global__ __launch_bounds__(256) void k_compose(const float* __restrict__ out, const uint16_t* __restrict__ raw, int B,
                                                  int H, int W, int Hp, int Wp, float4 bg,
                                                  const float* __restrict__ target, float inv_count,
@@ -500,7 +501,7 @@ __global__ __launch_bounds__(256) void k_compose(const float* __restrict__ out, 
         long r = i / Wp;
         int y = (int)(r % Hp);
         int b = (int)(r / Hp);
-        float g0 = 0.0f;
+        float g0 = 0.0f, gr[3] = {0.0f, 0.0f, 0.0f};
         if (x < W && y < H) {
             size_t q = ((size_t)b * H + y) * W + x;
             const uint16_t* s = raw + q * 12;
@@ -510,21 +511,26 @@ __global__ __launch_bounds__(256) void k_compose(const float* __restrict__ out, 
             float v = out[i * 16];
             float c[3];
             bool hit = t > 0.0f;
-            for (int k = 0; k < 3; ++k) c[k] = hit ? al[k] * v : (k == 0 ? bg.x : (k == 1 ? bg.y : bg.z));
+            for (int k = 0; k < 3; ++k)
+                c[k] = hit ? al[k] * v + out[i * 16 + 1 + k] : (k == 0 ? bg.x : (k == 1 ? bg.y : bg.z));
             if (color)
                 for (int k = 0; k < 3; ++k) color[q * 3 + k] = c[k];
             if (target) {
                 for (int k = 0; k < 3; ++k) {
                     float d = c[k] - target[q * 3 + k];
                     se += d * d;
-                    if (hit) g0 += 2.0f * d * al[k] * inv_count;
+                    if (hit) {
+                        g0 += 2.0f * d * al[k] * inv_count;
+                        gr[k] = 2.0f * d * inv_count;
+                    }
                 }
             }
         }
         if (dout) {
             uint16_t* d = dout + i * 16;
             d[0] = f2bf(g0);
-            for (int k = 1; k < 16; ++k) d[k] = 0;
+            for (int k = 1; k < 4; ++k) d[k] = f2bf(gr[k - 1]);
+            for (int k = 4; k < 16; ++k) d[k] = 0;
         }
     }
     if (loss) {
@@ -765,7 +771,7 @@ int Denoiser::ensure_acts(mmhrt_ctx_s* c, int B, int Hp, int Wp, bool train) {
     return 0;
 }
 
-// Forward pass over X0 [B,Hp,Wp,16] into `out` (fp32, channel 0 used).
+// Forward pass over X0 [B,Hp,Wp,16] into `out` (fp32, channels 0-3 used).
 void Denoiser::forward(hipStream_t s, int B, int Hp, int Wp) {
     int Hs[4] = {Hp, Hp / 2, Hp / 4, Hp / 8}, Ws[4] = {Wp, Wp / 2, Wp / 4, Wp / 8};
     size_t wo = 0, po = 0, w32 = 0;

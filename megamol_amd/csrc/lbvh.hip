@@ -10,7 +10,7 @@
 //   4. gather spheres into Morton order (leaf k), keep leaf -> particle index
 //   5. hierarchy and boxes together, bottom-up: one thread per leaf climbs, the second arrival
 //      at a parent merges and writes the whole node (Karras 2012 numbering and index tie-break
-//      for equal keys, Apetrei 2014 parent rule; LDS meeting places inside 512-leaf chunks)
+//      for equal keys, Apetrei 2014 parent rule; LDS meeting places inside 1024-leaf chunks)
 #include <algorithm>
 
 #include "ctx.hpp"
@@ -396,7 +396,7 @@ __device__ __forceinline__ int split_delta(const K* __restrict__ keys, int64_t n
 // Straddling nodes are a small fraction: the ancestors of one node per chunk boundary.
 constexpr int BOX_ROUNDS = 48;        // kernel rounds before the single-workgroup finish
 constexpr uint32_t BOX_SMALL = 2048;  // worklist size handed to the single workgroup
-constexpr int BC = 512;               // leaves per chunk workgroup (34 KB of LDS: four per CU)
+constexpr int BC = 1024;              // leaves (and threads) per chunk workgroup (74 KB of LDS: two per CU)
 constexpr int BOX_ITEMS = 16;
 constexpr int BOX_TILE = 256 * BOX_ITEMS;
 
@@ -451,14 +451,15 @@ struct SplitList {
     }
 };
 
-// The chunk pass. flags[] must be zero on entry; n >= 2.
+// The chunk pass, one thread per leaf. flags[] must be zero on entry; n >= 2.
 template <typename K>
-__global__ __launch_bounds__(256) void k_bu_chunk(const K* __restrict__ keys, const float4* __restrict__ leaf_sph,
-                                                  int64_t n, Node* __restrict__ nodes, Node* meet, int2* meet_far,
-                                                  uint32_t* flags, uint32_t* list, uint32_t* cnt, SceneInfo* info) {
+__global__ __launch_bounds__(BC) void k_bu_chunk(const K* __restrict__ keys, const float4* __restrict__ leaf_sph,
+                                                 int64_t n, Node* __restrict__ nodes, Node* meet, int2* meet_far,
+                                                 uint32_t* flags, uint32_t* list, uint32_t* cnt, SceneInfo* info) {
     __shared__ float s_box[BC][12];  // the two child boxes parked at each in-chunk split
     __shared__ int32_t s_ref[BC][2];
     __shared__ int32_t s_far[BC][2];
+    __shared__ int s_delta[BC + 1];  // delta(cs - 1 + j): every delta a climb inside the chunk reads
     __shared__ uint8_t s_h[BC][2];
     __shared__ uint8_t s_side[BC];   // side of the parked child (meaningful with one arrival)
     __shared__ uint32_t s_cnt[BC];   // arrivals per in-chunk split
@@ -467,10 +468,11 @@ __global__ __launch_bounds__(256) void k_bu_chunk(const K* __restrict__ keys, co
     SplitList ap{s_list, &s_n, &s_base};
     const int64_t cs = (int64_t)blockIdx.x * BC;
     const int64_t ce = cs + BC < n ? cs + BC : n;
-    for (int k = threadIdx.x; k < BC; k += 256) s_cnt[k] = 0;
+    s_cnt[threadIdx.x] = 0;
+    for (int j = threadIdx.x; j <= BC; j += BC) s_delta[j] = split_delta(keys, n, cs - 1 + j);
     if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
-    for (int64_t k = cs + threadIdx.x; k < ce; k += 256) {
+    for (int64_t k = cs + threadIdx.x; k < ce; k += BC) {
         float b[6];
         leaf_box(leaf_sph[k], b);
         float f[12];
@@ -485,7 +487,7 @@ __global__ __launch_bounds__(256) void k_bu_chunk(const K* __restrict__ keys, co
                 info->height = (uint32_t)h;
                 break;
             }
-            const int dl = split_delta(keys, n, l - 1), dr = split_delta(keys, n, r);
+            const int dl = s_delta[l - cs], dr = s_delta[r + 1 - cs];  // l - 1 >= cs - 1, r <= ce - 1
             const bool left = dr > dl;
             const int64_t s = left ? r : l - 1;
             if (internal) {
@@ -533,7 +535,7 @@ __global__ __launch_bounds__(256) void k_bu_chunk(const K* __restrict__ keys, co
     // in-chunk splits with one arrival: the other child straddles the chunk, so the parked
     // child moves to the global meeting place
     const int nsplit = (int)(ce - cs) - 1;
-    for (int slot = threadIdx.x; slot < nsplit; slot += 256) {
+    for (int slot = threadIdx.x; slot < nsplit; slot += BC) {
         if (s_cnt[slot] != 1u) continue;
         const int side = s_side[slot];
         const int64_t s = cs + slot;
@@ -736,7 +738,7 @@ int hierarchy(mmhrt_ctx_s* c, const K* keys) {
     if (int e = ensure(c, c->flags, m, "meeting counters")) return e;
     launch_fill_u32(s, c->box_cnt.p, 0u, BOX_ROUNDS + 4);
     launch_fill_u32(s, c->flags.p, 0u, m);
-    hipLaunchKernelGGL(k_bu_chunk<K>, dim3(blocks_for(n, BC)), dim3(256), 0, s, keys, c->leaf_sph.p, (int64_t)n,
+    hipLaunchKernelGGL(k_bu_chunk<K>, dim3(blocks_for(n, BC)), dim3(BC), 0, s, keys, c->leaf_sph.p, (int64_t)n,
                        c->nodes.p, c->meet.p, c->meet_far.p, c->flags.p, c->va.p, c->box_cnt.p, c->info.p);
     for (int r = 0; r < BOX_ROUNDS; ++r)
         hipLaunchKernelGGL(k_bu_round<K>, dim3(1024), dim3(256), 0, s, keys, (int64_t)n, c->nodes.p, c->meet.p,

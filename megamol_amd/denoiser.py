@@ -20,7 +20,7 @@ LAYERS = [("enc0a", 16, 0, 32, 1), ("enc0b", 32, 0, 32, 1), ("enc1a", 32, 0, 48,
           ("dec0a", 32, 48, 32, 1), ("dec0b", 32, 0, 32, 1), ("out", 32, 0, 16, 0)]
 
 MAGIC = b"MMHRTDN1"
-FORMAT_VERSION = 1
+FORMAT_VERSION = 2  # 2: output channels 1-3 are a colour residual added to albedo * v
 
 
 def param_count() -> int:
@@ -85,6 +85,22 @@ def features_from_frame(color: np.ndarray, aux: np.ndarray) -> np.ndarray:
     f[..., 0:3] = color[..., 0:3].astype(np.float16)
     f[..., 3:11] = aux
     return f
+
+
+def box_filter_baseline(aux: np.ndarray, bg=(0.0, 0.0, 0.0), k: int = 5) -> np.ndarray:
+    """The non-learned comparison for the network: the 1-spp visibility averaged over the hit
+    pixels of a k x k window, times the pixel's albedo; background on misses. aux is a frame's
+    MMHRT_FLAG_AUX buffer (H,W,8: albedo rgb, normal xyz, t, visibility)."""
+    from scipy.ndimage import uniform_filter
+    a = aux.astype(np.float32)
+    hit = (a[..., 6] > 0).astype(np.float32)
+    v = a[..., 7] * hit
+    num = uniform_filter(v, k, mode="constant")
+    den = uniform_filter(hit, k, mode="constant")
+    vf = np.where(den > 0, num / np.maximum(den, 1e-6), 0.0)
+    out = a[..., 0:3] * vf[..., None]
+    out[hit == 0] = np.asarray(bg, np.float32)
+    return out
 
 
 class Denoiser:

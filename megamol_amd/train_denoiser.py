@@ -123,6 +123,8 @@ def main():
     ap.add_argument("--pairs", type=int, default=2000)
     ap.add_argument("--eval-pairs", type=int, default=64)
     ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--spp-target", type=int, default=256)
     ap.add_argument("--crop", type=int, default=128)
     ap.add_argument("--seeds", type=str, default="21,22")
@@ -145,11 +147,12 @@ def main():
     load_fcc(r, args.cells, args.eval_seed)
     ef, et = make_pairs(r, args.eval_pairs, args.cells, args.eval_seed, args.crop, args.spp_target)
     before = evaluate(dn, ef, et)
-    losses = train(dn, feats, targets, args.steps)
+    losses = train(dn, feats, targets, args.steps, batch=args.batch, lr=args.lr)
     after = evaluate(dn, ef, et)
     log(f"held-out PSNR: raw 1spp {after[0]:.2f} dB, untrained net {before[1]:.2f} dB, trained {after[1]:.2f} dB")
     if args.out:
-        save_weights(args.out, dn.get_params(), {"pairs": len(feats), "steps": args.steps, "spp_target": args.spp_target,
+        save_weights(args.out, dn.get_params(), {"pairs": len(feats), "steps": args.steps, "batch": args.batch,
+                                                 "spp_target": args.spp_target,
                                                  "crop": args.crop, "seeds": seeds, "final_loss": losses[-1],
                                                  "heldout_psnr_raw": after[0], "heldout_psnr_denoised": after[1]})
         log(f"saved {args.out}")

@@ -4,7 +4,7 @@ MFMA kernels in megamol_amd/csrc/denoise.hip; the product never imports it).
 Restates, with PyTorch on the CPU (fp32 or fp64), the network that include/mmhrt.h documents:
 input packing of the 12-value feature records, the 15 convolutions of the 4-level U-Net
 (3x3, padding 1, ReLU except the last), 2x2 average pooling, nearest 2x upsampling with
-[skip, upsampled] concatenation, the albedo * v composition and the mean squared error.
+[skip, upsampled] concatenation, the albedo * v + residual composition and the mean squared error.
 There is no reference counterpart (MegaMol has no denoiser); finite differences in float64 pin
 the autograd gradients of this restatement.
 """
@@ -109,6 +109,10 @@ def forward(tp, x0, round_bf16=False):
     def conv(i, x, skip=None):
         if skip is not None:
             x = torch.cat([skip, F.interpolate(x, scale_factor=2, mode="nearest")], 1)
+            if round_bf16:
+                # the GPU keeps the gradient of the concatenated input in bf16 before splitting it
+                # into the skip part and the 2x2 sums of the upsampled part
+                x = bf16_round_grad_only(x)
         w, b = tp[i]
         y = F.conv2d(x, w, b, padding=1)
         if LAYERS[i][4]:
@@ -140,15 +144,17 @@ def forward(tp, x0, round_bf16=False):
 
 
 def compose(out, feats: np.ndarray, bg=(0, 0, 0, 1)):
-    """out (B,16,Hp,Wp) -> colour (B,H,W,3): albedo * v on hit pixels, background elsewhere."""
+    """out (B,16,Hp,Wp) -> colour (B,H,W,3): albedo * v + residual (channels 1-3) on hit pixels,
+    background elsewhere."""
     import torch
     B, H, W, _ = feats.shape
     f = torch.from_numpy(feats.astype(np.float32)).to(out.dtype)
     v = out[:, 0, :H, :W]
+    res = out[:, 1:4, :H, :W].permute(0, 2, 3, 1)
     hit = (f[..., 9] > 0)
     alb = f[..., 3:6]
     bgt = torch.tensor(bg[:3], dtype=out.dtype).expand(B, H, W, 3)
-    return torch.where(hit[..., None], alb * v[..., None], bgt)
+    return torch.where(hit[..., None], alb * v[..., None] + res, bgt)
 
 
 def loss_and_grads(params, feats, target, depth_scale, dtype="float64", round_bf16=False):

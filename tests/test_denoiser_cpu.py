@@ -89,3 +89,21 @@ def test_psnr_and_ssim_closed_forms():
     b = a + 0.1
     assert abs(D.psnr(a, b) - 20.0) < 1e-9  # mse 0.01 -> 20 dB
     assert D.ssim(a + 0.5, a + 0.5) == pytest.approx(1.0)
+
+
+def test_box_filter_baseline_closed_forms():
+    from megamol_amd.denoiser import box_filter_baseline
+    aux = np.zeros((9, 9, 8), np.float16)
+    aux[..., 0:3] = np.array([0.2, 0.4, 0.6], np.float16)
+    aux[..., 6] = 1.0  # every pixel hit
+    aux[..., 7] = 1.0
+    aux[4, 4, 7] = 0.0  # one occluded sample in the middle
+    out = box_filter_baseline(aux, (0.1, 0.1, 0.1), k=3)
+    # the middle 3x3 window holds 8 visible samples of 9; windows that miss it keep v = 1
+    np.testing.assert_allclose(out[4, 4], np.array([0.2, 0.4, 0.6]) * (8 / 9), rtol=2e-3)
+    np.testing.assert_allclose(out[0, 0], np.array([0.2, 0.4, 0.6]), rtol=2e-3)
+    # misses take the background and do not count in their neighbours' averages
+    aux[0, :, 6] = 0.0
+    out = box_filter_baseline(aux, (0.1, 0.1, 0.1), k=3)
+    np.testing.assert_allclose(out[0, 3], [0.1, 0.1, 0.1])
+    np.testing.assert_allclose(out[1, 0], np.array([0.2, 0.4, 0.6]), rtol=2e-3)

@@ -57,6 +57,7 @@ def test_gradients_match_reference(dnz):
     _, eg, _ = R.loss_and_grads(p, feats, target, 2.0, "float64", round_bf16=False)
     assert abs(loss - rloss) <= 2e-2 * rloss
     o = 0
+    rows = []
     for name, c1, c2, co, _ in D.LAYERS:
         n = 9 * (c1 + c2) * co + co
         gg, rr, ee = g[o:o + n], rg[o:o + n], eg[o:o + n]
@@ -64,13 +65,17 @@ def test_gradients_match_reference(dnz):
         if np.linalg.norm(rr) == 0:
             continue
         cos = float(gg @ rr / (np.linalg.norm(gg) * np.linalg.norm(rr)))
+        nb = 9 * (c1 + c2) * co  # bias gradients follow the weights
+        rows.append((name, rel(gg, ee), rel(rr, ee), rel(gg, rr), cos, rel(gg[nb:], ee[nb:]), rel(rr[nb:], ee[nb:])))
+    for r in rows:
+        print("%-6s gpu-exact %.4f bf16ref-exact %.4f gpu-bf16ref %.4f cos %.5f | bias gpu %.4f ref %.4f" % r)
+    for name, err_gpu, err_bf16, err_pair, cos, _, _ in rows:
         # bf16 storage of activations and their gradients alone moves the deepest layers' weight
         # gradients by about 3% from the exact float64 ones; the GPU must be as close to exact as
         # the bf16-rounding reference is, up to a factor 2 (two independent roundings), and close
         # to that reference itself
-        err_gpu, err_bf16 = rel(gg, ee), rel(rr, ee)
         assert err_gpu < max(2.0 * err_bf16, 1e-2), (name, err_gpu, err_bf16)
-        assert rel(gg, rr) < 6e-2 and cos > 0.995, (name, rel(gg, rr), cos)
+        assert err_pair < 6e-2 and cos > 0.995, (name, err_pair, cos)
     assert np.array_equal(dnz.get_params(), p), "apply=0 must not update the parameters"
 
 
@@ -92,3 +97,26 @@ def test_training_reduces_loss_and_set_get_roundtrip(dnz, tmp_path):
     meta = dnz.load(str(path))
     assert meta["loss"] == losses[-1]
     np.testing.assert_array_equal(dnz.get_params(), p)
+
+
+def test_trained_network_gains_over_the_method_without_it(gpu_renderer):
+    """The committed weights on held-out crops of the 50M-sphere FCC scene (a scene seed and
+    cameras never used in training): the network against the raw 1-spp frame and against the
+    non-learned alternative, a 5x5 box filter of the 1-spp visibility times the albedo, all
+    compared with 256-spp frames of the same crops."""
+    import os
+    from megamol_amd import train_denoiser as T
+    from megamol_amd.denoiser import psnr
+    case("trained network vs box filter on held-out FCC crops")
+    path = os.path.join(os.path.dirname(D.__file__), "weights", "denoiser_fcc.bin")
+    dn = D.Denoiser(gpu_renderer, seed=None)
+    dn.load(path)
+    T.load_fcc(gpu_renderer, 232, seed=31)
+    feats, targets = T.make_pairs(gpu_renderer, 12, 232, seed=31, crop=128, spp_target=256)
+    den = dn.denoise(feats, 1.0)
+    raw = feats[..., 0:3].astype(np.float32)
+    box = np.stack([D.box_filter_baseline(f[..., 3:11], (0.0, 0.0, 0.0)) for f in feats])
+    p_den, p_raw, p_box = psnr(den, targets), psnr(raw, targets), psnr(box, targets)
+    print(f"held-out PSNR vs 256 spp: network {p_den:.2f} dB, box filter {p_box:.2f} dB, raw 1 spp {p_raw:.2f} dB")
+    assert p_den > p_box + 2.0
+    assert p_den > p_raw + 6.0
