@@ -90,9 +90,13 @@ def denoised_leg(r, scene, args, cam, p_plain):
     dnz = Denoiser(r, seed=1234)
     wpath = args.weights
     trained = "loaded"
+    meta = None
     if wpath and os.path.exists(wpath):
-        meta = dnz.load(wpath)
-    else:
+        try:
+            meta = dnz.load(wpath)
+        except ValueError as e:
+            log(f"weights file {wpath} unusable ({e})")
+    if meta is None:
         from megamol_amd import train_denoiser as T
         log("no denoiser weights: training a small model on this scene's pairs (other cameras)")
         f, t = T.make_pairs(r, args.train_pairs, args.cells, 99, crop=128, spp_target=256)

@@ -167,7 +167,7 @@ constexpr int RW = RT / (RB / 64);  // keys per wave segment
 
 template <typename K>
 __global__ __launch_bounds__(RB) void k_radix_hist(const K* __restrict__ keys, uint64_t n, int shift,
-                                                   uint32_t* __restrict__ hist, uint32_t tiles) {
+                                                   uint32_t* __restrict__ hist) {
     __shared__ uint32_t h[256];
     h[threadIdx.x] = 0;
     const uint64_t base = (uint64_t)blockIdx.x * RT + (uint64_t)(threadIdx.x >> 6) * RW + __lane_id();
@@ -186,7 +186,7 @@ __global__ __launch_bounds__(RB) void k_radix_hist(const K* __restrict__ keys, u
             atomicAdd(&h[d[r]], (uint32_t)__popcll(peers));
     }
     __syncthreads();
-    hist[(size_t)threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
+    hist[(size_t)blockIdx.x * 256 + threadIdx.x] = h[threadIdx.x];
 }
 
 // Stable scatter of one tile for one 8-bit digit. Each wave ranks its segment with ballots and
@@ -198,7 +198,7 @@ template <typename K>
 __global__ __launch_bounds__(RB) void k_radix_scatter(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                       K* __restrict__ kout, uint32_t* __restrict__ vout, uint64_t n,
                                                       int shift, const uint32_t* __restrict__ hist,
-                                                      const uint32_t* __restrict__ offs, uint32_t tiles) {
+                                                      const uint32_t* __restrict__ offs) {
     __shared__ K s_key[RT];
     __shared__ uint32_t s_val[RT];
     __shared__ uint32_t wc[RB / 64][256];  // per-wave running digit counts, then per-wave offsets
@@ -218,7 +218,7 @@ __global__ __launch_bounds__(RB) void k_radix_scatter(const K* __restrict__ kin,
         val[r] = valid ? vin[i] : 0u;
     }
     // tile-local digit starts: exclusive scan of this tile's digit counts (thread = digit)
-    const uint32_t cnt = hist[(size_t)threadIdx.x * tiles + blockIdx.x];
+    const uint32_t cnt = hist[(size_t)blockIdx.x * 256 + threadIdx.x];
     uint32_t inc = cnt;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
@@ -232,7 +232,7 @@ __global__ __launch_bounds__(RB) void k_radix_scatter(const K* __restrict__ kin,
     uint32_t wb = 0;
     for (int w = 0; w < wave; ++w) wb += wsum[w];
     const uint32_t lstart = wb + inc - cnt;
-    goff[threadIdx.x] = offs[(size_t)threadIdx.x * tiles + blockIdx.x] - lstart;
+    goff[threadIdx.x] = offs[(size_t)blockIdx.x * 256 + threadIdx.x] - lstart;
     // ranks inside the wave's segment
     uint32_t rank[RI];
 #pragma unroll
@@ -275,13 +275,41 @@ __global__ __launch_bounds__(RB) void k_radix_scatter(const K* __restrict__ kin,
     }
 }
 
-// Exclusive scan of a u32 array: per-chunk sums, a one-block scan of the sums, then chunks.
-constexpr int SB = 256, SI = 16, ST = SB * SI;
+// Global digit offsets of every tile: offs[t][d] = (keys with a smaller digit) + (keys with
+// digit d in earlier tiles). hist and offs are tile-major ([tile][256]), so every pass below
+// reads or writes whole 1 KB rows. Column sums per group of CG tiles, one block that scans the
+// groups and the digits, then each group's running prefix down its tiles.
+constexpr int CG = 64;  // tiles per column group
 
-__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t x, uint32_t* total) {
-    __shared__ uint32_t wsum[SB / 64];
-    uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
-    uint32_t inc = x;
+__global__ __launch_bounds__(256) void k_col_sums(const uint32_t* __restrict__ hist, uint32_t tiles,
+                                                  uint32_t* __restrict__ gsum) {
+    const uint32_t t0 = blockIdx.x * CG;
+    uint32_t v[CG];
+#pragma unroll
+    for (int k = 0; k < CG; ++k) v[k] = t0 + k < tiles ? hist[(size_t)(t0 + k) * 256 + threadIdx.x] : 0u;
+    uint32_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < CG; ++k) acc += v[k];
+    gsum[(size_t)blockIdx.x * 256 + threadIdx.x] = acc;
+}
+
+// One block: exclusive prefix of each digit over the groups (16 loads in flight per thread),
+// and the exclusive scan of the digit totals into row `groups`.
+__global__ __launch_bounds__(256) void k_col_top(uint32_t* __restrict__ gsum, uint32_t groups) {
+    __shared__ uint32_t wsum[4];
+    const uint32_t d = threadIdx.x, lane = __lane_id(), wave = threadIdx.x >> 6;
+    uint32_t run = 0;
+    for (uint32_t g0 = 0; g0 < groups; g0 += 16) {
+        uint32_t v[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) v[k] = g0 + k < groups ? gsum[(size_t)(g0 + k) * 256 + d] : 0u;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            if (g0 + k < groups) gsum[(size_t)(g0 + k) * 256 + d] = run;
+            run += v[k];
+        }
+    }
+    uint32_t inc = run;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
         uint32_t y = __shfl_up(inc, o, 64);
@@ -289,53 +317,23 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t x, uint32_t* t
     }
     if (lane == 63) wsum[wave] = inc;
     __syncthreads();
-    uint32_t wbase = 0, all = 0;
-    for (int w = 0; w < SB / 64; ++w) {
-        uint32_t s = wsum[w];
-        if (w < (int)wave) wbase += s;
-        all += s;
-    }
-    __syncthreads();
-    *total = all;
-    return wbase + inc - x;
+    uint32_t base = 0;
+    for (uint32_t w = 0; w < wave; ++w) base += wsum[w];
+    gsum[(size_t)groups * 256 + d] = base + inc - run;  // keys with a smaller digit
 }
 
-__global__ __launch_bounds__(SB) void k_scan_sums(const uint32_t* __restrict__ in, uint64_t len,
-                                                  uint32_t* __restrict__ sums) {
-    uint64_t base = (uint64_t)blockIdx.x * ST;
-    uint32_t s = 0;
-    for (int r = 0; r < SI; ++r) {
-        uint64_t i = base + (uint64_t)r * SB + threadIdx.x;
-        if (i < len) s += in[i];
-    }
-    uint32_t tot;
-    (void)block_exclusive_scan(s, &tot);
-    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
-}
-
-__global__ __launch_bounds__(SB) void k_scan_top(uint32_t* __restrict__ sums, uint32_t nsums) {
-    uint32_t carry = 0;
-    for (uint32_t base = 0; base < nsums; base += SB) {
-        uint32_t i = base + threadIdx.x;
-        uint32_t x = i < nsums ? sums[i] : 0u;
-        uint32_t tot;
-        uint32_t ex = block_exclusive_scan(x, &tot);
-        if (i < nsums) sums[i] = carry + ex;
-        carry += tot;
-    }
-}
-
-__global__ __launch_bounds__(SB) void k_scan_chunks(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
-                                                    uint64_t len, const uint32_t* __restrict__ sums) {
-    uint64_t base = (uint64_t)blockIdx.x * ST;
-    uint32_t carry = sums[blockIdx.x];
-    for (int r = 0; r < SI; ++r) {
-        uint64_t i = base + (uint64_t)r * SB + threadIdx.x;
-        uint32_t x = i < len ? in[i] : 0u;
-        uint32_t tot;
-        uint32_t ex = block_exclusive_scan(x, &tot);
-        if (i < len) out[i] = carry + ex;
-        carry += tot;
+__global__ __launch_bounds__(256) void k_col_offsets(const uint32_t* __restrict__ hist, uint32_t tiles,
+                                                     uint32_t groups, const uint32_t* __restrict__ gsum,
+                                                     uint32_t* __restrict__ offs) {
+    const uint32_t t0 = blockIdx.x * CG;
+    uint32_t v[CG];
+#pragma unroll
+    for (int k = 0; k < CG; ++k) v[k] = t0 + k < tiles ? hist[(size_t)(t0 + k) * 256 + threadIdx.x] : 0u;
+    uint32_t run = gsum[(size_t)blockIdx.x * 256 + threadIdx.x] + gsum[(size_t)groups * 256 + threadIdx.x];
+#pragma unroll
+    for (int k = 0; k < CG; ++k) {
+        if (t0 + k < tiles) offs[(size_t)(t0 + k) * 256 + threadIdx.x] = run;
+        run += v[k];
     }
 }
 
@@ -694,9 +692,9 @@ template <typename K>
 int sort_pairs(mmhrt_ctx_s* c, K* ka, K* kb, uint32_t* va, uint32_t* vb, uint64_t n, int passes) {
     uint32_t tiles = (uint32_t)((n + RT - 1) / RT);
     uint64_t hlen = (uint64_t)tiles * 256;
-    uint32_t nchunks = (uint32_t)((hlen + ST - 1) / ST);
+    uint32_t groups = (tiles + CG - 1) / CG;
     if (int e = ensure(c, c->hist, hlen * 2, "radix histogram")) return e;
-    if (int e = ensure(c, c->scan_tmp, nchunks, "scan sums")) return e;
+    if (int e = ensure(c, c->scan_tmp, (size_t)(groups + 1) * 256, "radix column sums")) return e;
     uint32_t* hist = c->hist.p;
     uint32_t* offs = c->hist.p + hlen;
     K* src = ka;
@@ -705,12 +703,13 @@ int sort_pairs(mmhrt_ctx_s* c, K* ka, K* kb, uint32_t* va, uint32_t* vb, uint64_
     uint32_t* vd = vb;
     for (int p = 0; p < passes; ++p) {
         int shift = 8 * p;
-        hipLaunchKernelGGL(k_radix_hist<K>, dim3(tiles), dim3(RB), 0, c->stream, src, n, shift, hist, tiles);
-        hipLaunchKernelGGL(k_scan_sums, dim3(nchunks), dim3(SB), 0, c->stream, hist, hlen, c->scan_tmp.p);
-        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SB), 0, c->stream, c->scan_tmp.p, nchunks);
-        hipLaunchKernelGGL(k_scan_chunks, dim3(nchunks), dim3(SB), 0, c->stream, hist, offs, hlen, c->scan_tmp.p);
+        hipLaunchKernelGGL(k_radix_hist<K>, dim3(tiles), dim3(RB), 0, c->stream, src, n, shift, hist);
+        hipLaunchKernelGGL(k_col_sums, dim3(groups), dim3(256), 0, c->stream, hist, tiles, c->scan_tmp.p);
+        hipLaunchKernelGGL(k_col_top, dim3(1), dim3(256), 0, c->stream, c->scan_tmp.p, groups);
+        hipLaunchKernelGGL(k_col_offsets, dim3(groups), dim3(256), 0, c->stream, hist, tiles, groups, c->scan_tmp.p,
+                           offs);
         hipLaunchKernelGGL(k_radix_scatter<K>, dim3(tiles), dim3(RB), 0, c->stream, src, vs, dst, vd, n, shift, hist,
-                           offs, tiles);
+                           offs);
         K* tk = src;
         src = dst;
         dst = tk;
