@@ -223,7 +223,9 @@ def main():
     ap.add_argument("--config", choices=["fcc", "cube", "series"], default="fcc")
     ap.add_argument("--series-n", type=int, default=4_000_000)
     ap.add_argument("--series-t", type=int, default=64)
-    ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--autotune", action="store_true", help="series config: tune refit period and launch knobs")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="fcc/cube configs: keep the library's default launch knobs instead of tuning them")
     ap.add_argument("--no-denoise", action="store_true")
     ap.add_argument("--weights", type=str,
                     default=os.path.join(ROOT, "megamol_amd", "weights", "denoiser_fcc.bin"))
@@ -288,6 +290,15 @@ def main():
     def frame():
         r.build(L.BUILD_REBUILD, args.bits)
         r.render_async(cam, p)
+
+    tuned = None
+    if not args.no_autotune and not args.option:
+        from megamol_amd import autotune
+        r.build(L.BUILD_REBUILD, args.bits)
+        cfg, best, cache = autotune.tune_frame(r, cam, p)
+        tuned = {"config": cfg, "trace_ms": round(best, 3),
+                 "candidates_trace_ms": {k: round(v, 3) for k, v in cache.items()}}
+        log(f"autotuned launch: {cfg} ({best:.2f} ms trace)")
 
     for i in range(args.warmup):
         frame()
@@ -392,6 +403,7 @@ def main():
         "build_stages_ms": build_stages,
         "trace_ms": round(trace_ms, 3),
         "trace_work": split,
+        "autotune": tuned,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "k_trace (AO, persistent)", "algorithmic_bytes": int(algo_bytes)},

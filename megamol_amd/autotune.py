@@ -23,7 +23,7 @@ SPACE = {
     "tile_w": [4, 8, 16, 32],
     "stack_lds": [8, 16, 24, 32],
     "blocks_per_cu": [2, 4, 8],
-    "refill": [0, 16, 32, 48],
+    "refill": [0, 16, 32, 48, 56],
 }
 DEFAULTS = {"tile_w": 8, "stack_lds": 16, "blocks_per_cu": 8, "refill": 0}
 

@@ -392,11 +392,9 @@ __device__ __forceinline__ int split_delta(const K* __restrict__ keys, int64_t n
 //   finish: once a worklist is small, one workgroup runs the remaining rounds itself with
 //           barriers between them (workgroup scope, no cache writeback).
 // Straddling nodes are a small fraction: the ancestors of one node per chunk boundary.
-constexpr int BOX_ROUNDS = 48;        // kernel rounds before the single-workgroup finish
-constexpr uint32_t BOX_SMALL = 2048;  // worklist size handed to the single workgroup
+constexpr int BOX_ROUNDS = 32;        // kernel rounds before the single-workgroup finish
+constexpr uint32_t BOX_SMALL = 512;   // worklist size handed to the single workgroup
 constexpr int BC = 1024;              // leaves (and threads) per chunk workgroup (74 KB of LDS: two per CU)
-constexpr int BOX_ITEMS = 16;
-constexpr int BOX_TILE = 256 * BOX_ITEMS;
 
 __device__ __forceinline__ void leaf_box(const float4 s, float b[6]) {
     b[0] = s.x - s.w; b[1] = s.y - s.w; b[2] = s.z - s.w;
@@ -643,10 +641,12 @@ __global__ __launch_bounds__(256) void k_bu_round(const K* __restrict__ keys, in
     SplitList ap{s_list, &s_n, &s_base};
     if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
-    for (uint32_t tile = blockIdx.x * BOX_TILE; tile < n_in; tile += gridDim.x * BOX_TILE) {
-        for (int j = 0; j < BOX_ITEMS; ++j) {
-            uint32_t i = tile + j * 256 + threadIdx.x;
-            if (i >= n_in) break;
+    // one item per thread while the list is shorter than the grid: every step is a chain of
+    // dependent memory accesses, so the items are spread as widely as possible
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n_in; base += stride) {
+        const uint32_t i = base + threadIdx.x;
+        if (i < n_in) {
             int64_t s2 = bu_step<K, false>(keys, n, nodes, meet, meet_far, flags, (int64_t)lin[i], info);
             if (s2 >= 0) ap.push((uint32_t)s2, lout, cnt + 2 + r);
         }
