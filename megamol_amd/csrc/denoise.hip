@@ -132,31 +132,33 @@ __global__ __launch_bounds__(256) void k_conv_fwd(ConvArgs a) {
 }
 
 // The same convolution with v_mfma_f32_16x16x32_bf16 over 32-wide chunks of k = tap*CI + ci.
-template <int NB>
+// A wavefront computes PB blocks of 16 pixels (PB = 4, 2 or 1: the coarse levels use fewer
+// pixels and fewer output-channel blocks per wavefront so that the launch still fills the chip).
+template <int NB, int PB>
 __global__ __launch_bounds__(256) void k_conv_fwd32(ConvArgs a) {
     const int lane = __lane_id();
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int runs_per_row = (a.W + 63) / 64;
+    const int runs_per_row = (a.W + 16 * PB - 1) / (16 * PB);
     const int row = wave / runs_per_row;  // b*H + y
     if (row >= a.B * a.H) return;
-    const int x0 = (wave % runs_per_row) * 64;
+    const int x0 = (wave % runs_per_row) * 16 * PB;
     const int b = row / a.H, y = row % a.H;
     const int nb0 = blockIdx.y * NB;
     const int CI = a.C1 + a.C2;
     const int K9 = 9 * CI;
     const int nkc = (K9 + 31) / 32;
     const int NBT = a.CO / 16;
-    f32x4 acc[4][NB];
+    f32x4 acc[PB][NB];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < PB; ++i)
 #pragma unroll
         for (int j = 0; j < NB; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
     const int q = lane >> 4;   // which 8 of the chunk's 32 k values
     const int lr = lane & 15;  // row (A) / column (B)
     for (int kc = 0; kc < nkc; ++kc) {
-        Frag8 fa[4];
+        Frag8 fa[PB];
 #pragma unroll
-        for (int pb = 0; pb < 4; ++pb) fa[pb].u = make_uint4(0u, 0u, 0u, 0u);
+        for (int pb = 0; pb < PB; ++pb) fa[pb].u = make_uint4(0u, 0u, 0u, 0u);
         const int k0 = kc * 32 + q * 8;
         if (k0 < K9) {
             const int t = k0 / CI;
@@ -165,7 +167,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd32(ConvArgs a) {
             const int dx = t % 3 - 1;
             if (yy >= 0 && yy < a.H) {
 #pragma unroll
-                for (int pb = 0; pb < 4; ++pb) {
+                for (int pb = 0; pb < PB; ++pb) {
                     const int xx = x0 + pb * 16 + lr + dx;
                     if (xx >= 0 && xx < a.W) {
                         const uint16_t* src;
@@ -186,7 +188,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd32(ConvArgs a) {
             Frag8 fb;
             fb.u = *reinterpret_cast<const uint4*>(a.Wt32 + (((size_t)kc * NBT + nb) * 64 + lane) * 8);
 #pragma unroll
-            for (int pb = 0; pb < 4; ++pb)
+            for (int pb = 0; pb < PB; ++pb)
                 acc[pb][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[pb].v, fb.v, acc[pb][j], 0, 0, 0);
         }
     }
@@ -197,7 +199,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd32(ConvArgs a) {
         const int co = nb * 16 + lr;
         const float bias = a.bias[co];
 #pragma unroll
-        for (int pb = 0; pb < 4; ++pb)
+        for (int pb = 0; pb < PB; ++pb)
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const int x = x0 + pb * 16 + q * 4 + e;
@@ -635,19 +637,45 @@ static size_t param_offset(int l) {
     return n;
 }
 
-static void conv_fwd(hipStream_t s, const ConvArgs& a, bool k32) {
+// shape = 0 picks the launch shape of the 16x16x32 kernel from the launch size; otherwise it
+// forces pixel blocks per wave (shape % 10: 4, 2 or 1) and, for 4, channel blocks per wave
+// (shape / 10, 0 = the automatic choice) — a test hook that reaches every instantiation.
+static void conv_fwd(hipStream_t s, const ConvArgs& a, bool k32, int shape) {
     long waves = (long)a.B * a.H * ((a.W + 63) / 64);
     int NBT = a.CO / 16;
     int nb = NBT <= 6 ? NBT : 4;
     dim3 g(nblk(waves * 64), (NBT + nb - 1) / nb);
     if (k32) {
-        switch (nb) {
-        case 1: hipLaunchKernelGGL(k_conv_fwd32<1>, g, dim3(256), 0, s, a); break;
-        case 2: hipLaunchKernelGGL(k_conv_fwd32<2>, g, dim3(256), 0, s, a); break;
-        case 3: hipLaunchKernelGGL(k_conv_fwd32<3>, g, dim3(256), 0, s, a); break;
-        case 4: hipLaunchKernelGGL(k_conv_fwd32<4>, g, dim3(256), 0, s, a); break;
-        case 5: hipLaunchKernelGGL(k_conv_fwd32<5>, g, dim3(256), 0, s, a); break;
-        default: hipLaunchKernelGGL(k_conv_fwd32<6>, g, dim3(256), 0, s, a); break;
+        // enough wavefronts to fill 256 CUs: fewer channel blocks, then fewer pixels per wave
+        const long want = 6144;
+        int pbk = 4;
+        if (shape != 0) {
+            pbk = shape % 10;
+            if (pbk != 1 && pbk != 2) pbk = 4;
+            if (pbk != 4)
+                nb = 1;
+            else if (shape / 10 > 0)
+                nb = std::min(shape / 10, std::min(NBT, 6));
+        } else if (waves * (NBT / nb) < want) {
+            nb = 1;
+            if (waves * NBT < want) pbk = 2;
+            if (waves * 2 * NBT < want) pbk = 1;
+        }
+        long wv = (long)a.B * a.H * ((a.W + 16 * pbk - 1) / (16 * pbk));
+        dim3 g32(nblk(wv * 64), (NBT + nb - 1) / nb);
+        if (pbk == 1) {
+            hipLaunchKernelGGL((k_conv_fwd32<1, 1>), g32, dim3(256), 0, s, a);
+        } else if (pbk == 2) {
+            hipLaunchKernelGGL((k_conv_fwd32<1, 2>), g32, dim3(256), 0, s, a);
+        } else {
+            switch (nb) {
+            case 1: hipLaunchKernelGGL((k_conv_fwd32<1, 4>), g32, dim3(256), 0, s, a); break;
+            case 2: hipLaunchKernelGGL((k_conv_fwd32<2, 4>), g32, dim3(256), 0, s, a); break;
+            case 3: hipLaunchKernelGGL((k_conv_fwd32<3, 4>), g32, dim3(256), 0, s, a); break;
+            case 4: hipLaunchKernelGGL((k_conv_fwd32<4, 4>), g32, dim3(256), 0, s, a); break;
+            case 5: hipLaunchKernelGGL((k_conv_fwd32<5, 4>), g32, dim3(256), 0, s, a); break;
+            default: hipLaunchKernelGGL((k_conv_fwd32<6, 4>), g32, dim3(256), 0, s, a); break;
+            }
         }
         return;
     }
@@ -783,7 +811,7 @@ void Denoiser::forward(hipStream_t s, int B, int Hp, int Wp) {
         A.Wt = wf.p + wo; A.Wt32 = wf32.p + w32;
         A.bias = params.p + po + 9L * (L.c1 + L.c2) * L.co;
         A.Y = y; A.Yf = yf; A.relu = L.relu;
-        conv_fwd(s, A, k32);
+        conv_fwd(s, A, k32, conv_shape);
         wo += 9L * (L.c1 + L.c2) * L.co;
         w32 += (size_t)((9 * (L.c1 + L.c2) + 31) / 32) * 32 * L.co;
         po += 9L * (L.c1 + L.c2) * L.co + L.co;
@@ -916,6 +944,7 @@ int dn_run(mmhrt_ctx c, int B, int H, int W, float depth_scale, const float bg[4
     hipLaunchKernelGGL(mmhrt::dn::k_pack_input, dim3((unsigned)((Pp + 255) / 256)), dim3(256), 0, s, d.raw.p, B, H,
                        W, Hp, Wp, inv_depth, d.x0);
     d.k32 = c->opt_dn_k32 != 0;
+    d.conv_shape = (int)c->opt_dn_conv_shape;
     d.forward(s, B, Hp, Wp);
     float4 bgc = make_float4(bg ? bg[0] : 0.f, bg ? bg[1] : 0.f, bg ? bg[2] : 0.f, bg ? bg[3] : 1.f);
     if (loss) mmhrt::launch_fill_u32(s, reinterpret_cast<uint32_t*>(d.scalar.p), 0u, 1);

@@ -61,6 +61,7 @@ struct Denoiser {
     DevBuf<uint16_t> wf, wb;   // bf16 weights in forward / backward fragment order
     DevBuf<uint16_t> wf32;     // bf16 weights in the 16x16x32 forward fragment order
     bool k32 = true;           // forward with v_mfma_f32_16x16x32_bf16
+    int conv_shape = 0;        // 0: automatic launch shape of the k32 kernel (see conv_fwd)
     DevBuf<uint16_t> acts;     // activations
     DevBuf<uint16_t> gbuf;     // backward temporaries
     DevBuf<uint16_t> raw;      // raw per-pixel features (12 halfs)

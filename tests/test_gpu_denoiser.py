@@ -26,11 +26,14 @@ def dnz(gpu_renderer):
     return D.Denoiser(gpu_renderer, seed=7)
 
 
-@pytest.mark.parametrize("k32", [0, 1])
+@pytest.mark.parametrize("k32", [0, 1, 14, 24, 34, 2, 4])
 @pytest.mark.parametrize("shape", [(1, 16, 16), (2, 24, 40), (1, 37, 70)])
 def test_forward_matches_reference(dnz, shape, k32):
+    """k32: 0 = 16x16x16 kernels, 1 = 16x16x32 with the automatic launch shape, other values force
+    the 16x16x32 launch shape (channel blocks * 10 + pixel blocks per wavefront)."""
     case(f"denoiser forward {shape} k32={k32}")
-    dnz.r.set_option("dn_k32", k32)
+    dnz.r.set_option("dn_k32", 1 if k32 else 0)
+    dnz.r.set_option("dn_conv_shape", k32 if k32 > 1 else 0)
     b, h, w = shape
     feats = random_features(b, h, w, seed=h)
     p = dnz.get_params()
@@ -39,7 +42,8 @@ def test_forward_matches_reference(dnz, shape, k32):
     dnz.set_params(p)
     out = dnz.denoise(feats, depth_scale=2.0)
     ref = R.denoise(p, feats, 2.0, round_bf16=True)
-    dnz.r.set_option("dn_k32", 0)
+    dnz.r.set_option("dn_k32", 1)
+    dnz.r.set_option("dn_conv_shape", 0)
     assert out.shape == ref.shape
     assert rel(out, ref) < 4e-2, rel(out, ref)
 
