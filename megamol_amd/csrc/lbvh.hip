@@ -178,13 +178,11 @@ __global__ __launch_bounds__(RB) void k_radix_hist(const K* __restrict__ keys, u
         d[r] = i < n ? (uint32_t)((keys[i] >> shift) & 255u) : 256u;
     }
     __syncthreads();
+    // plain LDS atomics: the LDS unit serialises same-address lanes itself, which is cheaper
+    // here than matching digits across the wave with ballots
 #pragma unroll
-    for (int r = 0; r < RI; ++r) {
-        bool valid = d[r] < 256u;
-        uint64_t peers = match_digit(d[r], valid);
-        if (valid && (__ffsll((unsigned long long)peers) - 1) == (int)__lane_id())
-            atomicAdd(&h[d[r]], (uint32_t)__popcll(peers));
-    }
+    for (int r = 0; r < RI; ++r)
+        if (d[r] < 256u) atomicAdd(&h[d[r]], 1u);
     __syncthreads();
     hist[(size_t)blockIdx.x * 256 + threadIdx.x] = h[threadIdx.x];
 }
@@ -380,7 +378,8 @@ __device__ __forceinline__ int split_delta(const K* __restrict__ keys, int64_t n
 // second merges, writes nothing until the merged node knows its own side (which fixes its
 // number), then writes the whole 64-byte node and climbs on. A device-scope fence per climb step
 // would write back L2 on every step, so the meeting places are split by where they lie:
-//   chunk:  workgroup g owns leaves [g*BC, (g+1)*BC); splits between two of its leaves meet in
+//   chunk:  workgroup g owns leaves [g*BC, (g+1)*BC) (BC = 1024, 512 or 256, a launch option);
+//           splits between two of its leaves meet in
 //           LDS (a workgroup-scope atomic decides the order). A split at a chunk boundary, or an
 //           in-chunk split whose other child never arrives (it straddles the boundary), meets in
 //           global memory, meet[s], where a device-scope counter flags[s] counts the arrivals;
@@ -394,7 +393,6 @@ __device__ __forceinline__ int split_delta(const K* __restrict__ keys, int64_t n
 // Straddling nodes are a small fraction: the ancestors of one node per chunk boundary.
 constexpr int BOX_ROUNDS = 32;        // kernel rounds before the single-workgroup finish
 constexpr uint32_t BOX_SMALL = 512;   // worklist size handed to the single workgroup
-constexpr int BC = 1024;              // leaves (and threads) per chunk workgroup (74 KB of LDS: two per CU)
 
 __device__ __forceinline__ void leaf_box(const float4 s, float b[6]) {
     b[0] = s.x - s.w; b[1] = s.y - s.w; b[2] = s.z - s.w;
@@ -448,7 +446,7 @@ struct SplitList {
 };
 
 // The chunk pass, one thread per leaf. flags[] must be zero on entry; n >= 2.
-template <typename K>
+template <typename K, int BC>
 __global__ __launch_bounds__(BC) void k_bu_chunk(const K* __restrict__ keys, const float4* __restrict__ leaf_sph,
                                                  int64_t n, Node* __restrict__ nodes, Node* meet, int2* meet_far,
                                                  uint32_t* flags, uint32_t* list, uint32_t* cnt, SceneInfo* info) {
@@ -464,13 +462,15 @@ __global__ __launch_bounds__(BC) void k_bu_chunk(const K* __restrict__ keys, con
     SplitList ap{s_list, &s_n, &s_base};
     const int64_t cs = (int64_t)blockIdx.x * BC;
     const int64_t ce = cs + BC < n ? cs + BC : n;
+    const int64_t k = cs + threadIdx.x;
+    const float4 sph = k < ce ? leaf_sph[k] : make_float4(0.f, 0.f, 0.f, 0.f);  // in flight over the barrier
     s_cnt[threadIdx.x] = 0;
     for (int j = threadIdx.x; j <= BC; j += BC) s_delta[j] = split_delta(keys, n, cs - 1 + j);
     if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
-    for (int64_t k = cs + threadIdx.x; k < ce; k += BC) {
+    if (k < ce) {
         float b[6];
-        leaf_box(leaf_sph[k], b);
+        leaf_box(sph, b);
         float f[12];
         int2 kd = make_int2(0, 0);
         int h = 0;
@@ -737,8 +737,20 @@ int hierarchy(mmhrt_ctx_s* c, const K* keys) {
     if (int e = ensure(c, c->flags, m, "meeting counters")) return e;
     launch_fill_u32(s, c->box_cnt.p, 0u, BOX_ROUNDS + 4);
     launch_fill_u32(s, c->flags.p, 0u, m);
-    hipLaunchKernelGGL(k_bu_chunk<K>, dim3(blocks_for(n, BC)), dim3(BC), 0, s, keys, c->leaf_sph.p, (int64_t)n,
-                       c->nodes.p, c->meet.p, c->meet_far.p, c->flags.p, c->va.p, c->box_cnt.p, c->info.p);
+    // leaves per chunk workgroup: 1024 (77 KB of LDS, two per CU), 512 or 256
+    const int bc = c->opt_build_chunk == 256 ? 256 : (c->opt_build_chunk == 512 ? 512 : 1024);
+    if (bc == 256)
+        hipLaunchKernelGGL((k_bu_chunk<K, 256>), dim3(blocks_for(n, 256)), dim3(256), 0, s, keys, c->leaf_sph.p,
+                           (int64_t)n, c->nodes.p, c->meet.p, c->meet_far.p, c->flags.p, c->va.p, c->box_cnt.p,
+                           c->info.p);
+    else if (bc == 512)
+        hipLaunchKernelGGL((k_bu_chunk<K, 512>), dim3(blocks_for(n, 512)), dim3(512), 0, s, keys, c->leaf_sph.p,
+                           (int64_t)n, c->nodes.p, c->meet.p, c->meet_far.p, c->flags.p, c->va.p, c->box_cnt.p,
+                           c->info.p);
+    else
+        hipLaunchKernelGGL((k_bu_chunk<K, 1024>), dim3(blocks_for(n, 1024)), dim3(1024), 0, s, keys, c->leaf_sph.p,
+                           (int64_t)n, c->nodes.p, c->meet.p, c->meet_far.p, c->flags.p, c->va.p, c->box_cnt.p,
+                           c->info.p);
     for (int r = 0; r < BOX_ROUNDS; ++r)
         hipLaunchKernelGGL(k_bu_round<K>, dim3(1024), dim3(256), 0, s, keys, (int64_t)n, c->nodes.p, c->meet.p,
                            c->meet_far.p, c->flags.p, c->va.p, c->vb.p, c->box_cnt.p, r, c->info.p);
