@@ -46,6 +46,23 @@ __device__ __forceinline__ float rnd(uint32_t& st) {
     st = st * 1664525u + 1013904223u;
     return (float)(st & 0x00FFFFFFu) / 16777216.0f;
 }
+// The LCG state after k steps from st, by binary powers of the affine map x -> a x + c
+// (F. B. Brown, "Random number generation with arbitrary strides", 1994): the same 32-bit
+// state that k calls of rnd() reach.
+__device__ __forceinline__ uint32_t lcg_skip(uint32_t st, uint32_t k) {
+    uint32_t A = 1u, C = 0u;                 // accumulated map x -> A x + C
+    uint32_t a = 1664525u, c = 1013904223u;  // the map raised to the current power of two
+    while (k) {
+        if (k & 1u) {
+            A = A * a;
+            C = C * a + c;
+        }
+        c = c * (a + 1u);
+        a = a * a;
+        k >>= 1;
+    }
+    return A * st + C;
+}
 
 // Per-frame constants derived on the host from the camera (Renderer.cpp:136-172).
 struct FrameConsts {

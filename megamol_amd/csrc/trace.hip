@@ -418,6 +418,154 @@ __global__ __launch_bounds__(TB) void k_trace(TraceArgs A) {
     }
 }
 
+// One sample of a Lambert (MODE 0) or AO (MODE 1) pixel, the arithmetic of shade_pixel: the
+// sample colour, its depth contribution, the hit (t, leaf), the visibility and, for the
+// features, albedo and normal.
+template <int S, int MODE, bool COUNT>
+__device__ __forceinline__ void shade_sample(const TraceArgs& A, uint32_t px, uint32_t py, uint32_t pix, uint32_t s,
+                                             float jx, float jy, float pad, LaneStack<S>& st, Count& cnt,
+                                             float4& sc, float& ray_depth, float& t, int& leaf, float& V,
+                                             float4& alb_out, V3& n_out) {
+    const FrameConsts& f = A.fc;
+    V3 o, d;
+    primary_ray(f, px, py, jx, jy, o, d);
+    RayPre r = prep(o, d, pad);
+    if (COUNT) cnt.rays++;
+    leaf = trace_closest<S, COUNT>(A, r, 1e-6f, 1e20f, t, st, cnt);
+    ray_depth = FLT_MAX;
+    V = 0.0f;
+    if (leaf < 0) {
+        if (MODE == 1)
+            sc = make_float4(f.bg[0], f.bg[1], f.bg[2], f.bg[3]);
+        else
+            sc = make_float4(1.0f * 0.8f, 1.0f * 0.8f, 1.0f * 0.8f, 1.0f);
+        return;
+    }
+    ray_depth = t;
+    float4 sp = A.leaf_sph[leaf];
+    uint32_t id = A.leaf_id[leaf];
+    V3 P = V3{o.x + t * d.x, o.y + t * d.y, o.z + t * d.z};
+    V3 N = V3{P.x - sp.x, P.y - sp.y, P.z - sp.z};
+    if (dot(N, d) > 0.0f) N = neg(N);
+    N = normalize(N);
+    float4 alb = A.col[id];
+    alb_out = alb;
+    n_out = N;
+    if (MODE == 1) {
+        uint32_t aseed = tea16(pix, 0xA0000000u + f.frame_idx * 4096u + s);
+        V3 ad = ao_direction(N, aseed, s, f.spp);
+        RayPre ra = prep(P, ad, pad);
+        if (COUNT) cnt.rays++;
+        bool occ = trace_any<S, COUNT>(A, ra, 0.0f, f.ao_distance, leaf, st, cnt);
+        V = occ ? 0.0f : 1.0f;
+        sc = make_float4(alb.x * V, alb.y * V, alb.z * V, 1.0f);
+    } else {
+        float sh = 0.2f + 0.6f * fabsf(dot(N, d));
+        sc = make_float4(alb.x * sh, alb.y * sh, alb.z * sh, 1.0f);
+    }
+}
+
+// Sample-parallel persistent traversal (MODE 0 Lambert, 1 AO; spp a power of two or a multiple
+// of 64). A wavefront takes an 8x8 (tile_w x 64/tile_w) pixel tile and walks it in passes; in a
+// pass, the lanes hold L = min(spp, 64) consecutive samples of each of 64/L pixels, so the
+// primary rays of a lane group start in one pixel and a background pixel costs one quick pass.
+// Sample s of a pixel draws its jitter from the pixel's stream advanced by 2s steps
+// (lcg_skip), and the group folds its samples into the pixel in sample order (colour sums,
+// first strictly nearer hit, visibility, sample-0 features), so every output is bit-identical
+// to the lane-per-pixel kernels.
+template <int S, int MODE, bool COUNT>
+__global__ __launch_bounds__(TB) void k_trace_sl(TraceArgs A) {
+    __shared__ int32_t lds_stack[S * TB];
+    const uint32_t gtid = blockIdx.x * TB + threadIdx.x;
+    LaneStack<S> st{lds_stack + threadIdx.x, A.spill + gtid, A.spill_stride, 0, A.ctr + 1};
+    Count cnt;
+    const FrameConsts& f = A.fc;
+    const float amax = A.info->absmax;
+    const float pad = (fmaxf(f.pad, amax) + amax) * (1.0f / 131072.0f);
+    const uint32_t lane = __lane_id();
+    const uint32_t L = f.spp >= 64u ? 64u : f.spp;  // lanes per pixel
+    const uint32_t G = 64u / L;                      // pixels per pass
+    const uint32_t rounds = f.spp / L;
+    const uint32_t g = lane / L, sl = lane % L;
+    const uint32_t base = g * L;  // first lane of this lane's pixel group
+    const uint32_t tw = 1u << A.tw_log2, th = 64u >> A.tw_log2;
+    for (;;) {
+        uint32_t t0 = 0;
+        if (lane == 0) t0 = atomicAdd(A.ctr, 1u);
+        const uint32_t tile = __shfl(t0, 0, 64);
+        if (tile >= A.n_tiles) break;
+        const uint32_t tx = tile % A.tiles_x, ty = tile / A.tiles_x;
+        for (uint32_t q = 0; q < 64u / G; ++q) {
+            const uint32_t tp = q * G + g;  // pixel of this group inside the tile
+            const uint32_t px = f.win_x0 + tx * tw + (tp & (tw - 1)), py = f.win_y0 + ty * th + (tp >> A.tw_log2);
+            const bool valid = px < f.win_x0 + f.win_w && py < f.win_y0 + f.win_h;
+            const uint32_t pix = py * f.width + px;
+            const uint32_t seed0 = tea16(pix, f.frame_idx);
+            float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
+            float depth = FLT_MAX, best_t = INFINITY, vis_sum = 0.0f;
+            int best_leaf = -1;
+            float aux[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            for (uint32_t rd = 0; rd < rounds; ++rd) {
+                const uint32_t s = rd * L + sl;
+                float4 sc = make_float4(0.f, 0.f, 0.f, 0.f);
+                float ray_depth = FLT_MAX, t = INFINITY, V = 0.0f;
+                int leaf = -1;
+                float4 alb = make_float4(0.f, 0.f, 0.f, 0.f);
+                V3 N = V3{0.f, 0.f, 0.f};
+                if (valid) {
+                    uint32_t seed = lcg_skip(seed0, 2u * s);
+                    float jx = rnd(seed);
+                    float jy = rnd(seed);
+                    shade_sample<S, MODE, COUNT>(A, px, py, pix, s, jx, jy, pad, st, cnt, sc, ray_depth, t, leaf, V,
+                                                 alb, N);
+                }
+                // fold the group's samples into the pixel, in sample order
+                for (uint32_t i = 0; i < L; ++i) {
+                    const int src = (int)(base + i);
+                    const float cx = __shfl(sc.x, src, 64), cy = __shfl(sc.y, src, 64);
+                    const float cz = __shfl(sc.z, src, 64), cw = __shfl(sc.w, src, 64);
+                    const float ti = __shfl(t, src, 64), di = __shfl(ray_depth, src, 64);
+                    const float vi = __shfl(V, src, 64);
+                    const int li = __shfl(leaf, src, 64);
+                    col.x += cx;
+                    col.y += cy;
+                    col.z += cz;
+                    col.w += cw;
+                    depth = fminf(depth, di);
+                    if (li >= 0) {
+                        vis_sum += vi;
+                        if (ti < best_t) {
+                            best_t = ti;
+                            best_leaf = li;
+                        }
+                    }
+                }
+                if (rd == 0) {  // features of sample 0
+                    const int src = (int)base;
+                    const int l0 = __shfl(leaf, src, 64);
+                    const float a0 = __shfl(alb.x, src, 64), a1 = __shfl(alb.y, src, 64), a2 = __shfl(alb.z, src, 64);
+                    const float n0 = __shfl(N.x, src, 64), n1 = __shfl(N.y, src, 64), n2 = __shfl(N.z, src, 64);
+                    const float t0s = __shfl(t, src, 64);
+                    if (l0 >= 0) {
+                        aux[0] = a0; aux[1] = a1; aux[2] = a2;
+                        aux[3] = n0; aux[4] = n1; aux[5] = n2;
+                        aux[6] = t0s;
+                    }
+                }
+            }
+            if (valid && sl == 0) {
+                const uint32_t opix = (py - f.win_y0) * f.win_w + (px - f.win_x0);
+                write_pixel(A, opix, col, depth, best_t, best_leaf, aux, vis_sum);
+            }
+        }
+    }
+    if (COUNT) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(A.ctr + 2), (unsigned long long)cnt.visits);
+        atomicAdd(reinterpret_cast<unsigned long long*>(A.ctr + 4), (unsigned long long)cnt.tests);
+        atomicAdd(reinterpret_cast<unsigned long long*>(A.ctr + 6), (unsigned long long)cnt.rays);
+    }
+}
+
 // Persistent traversal with wavefront-ballot compaction and refill (MODE 0 Lambert, 1 AO).
 // Each lane owns one pixel at a time and walks its samples in order (primary ray, then for AO
 // one occlusion ray). The inner loop advances every active lane by one node visit and keeps
@@ -620,6 +768,21 @@ void launch1(const TraceArgs& A, unsigned grid, bool steal, bool count, int mode
 }
 
 template <int S>
+void launch_sl(const TraceArgs& A, unsigned grid, bool count, int mode, hipStream_t s) {
+    if (mode == 1) {
+        if (count)
+            hipLaunchKernelGGL((k_trace_sl<S, 1, true>), dim3(grid), dim3(TB), 0, s, A);
+        else
+            hipLaunchKernelGGL((k_trace_sl<S, 1, false>), dim3(grid), dim3(TB), 0, s, A);
+    } else {
+        if (count)
+            hipLaunchKernelGGL((k_trace_sl<S, 0, true>), dim3(grid), dim3(TB), 0, s, A);
+        else
+            hipLaunchKernelGGL((k_trace_sl<S, 0, false>), dim3(grid), dim3(TB), 0, s, A);
+    }
+}
+
+template <int S>
 void launch_refill(const TraceArgs& A, unsigned grid, uint32_t refill, bool count, int mode, hipStream_t s) {
     if (mode == 1) {
         if (count)
@@ -770,7 +933,16 @@ int render_frame(mmhrt_ctx_s* c, const FrameConsts& fc) {
     bool steal = c->opt_persistent != 0;
     bool count = c->opt_count != 0;
     uint32_t refill = (uint32_t)std::max<int64_t>(0, std::min<int64_t>(64, c->opt_refill));
-    if (refill > 0 && fc.mode != 2) {
+    // sample-parallel lanes: Lambert/AO with spp a power of two up to 64 or a multiple of 64
+    const bool sl_shape = fc.spp > 1 && ((fc.spp >= 64 && fc.spp % 64 == 0) || (fc.spp < 64 && 64 % fc.spp == 0));
+    if (c->opt_sample_lanes != 0 && refill == 0 && fc.mode != 2 && sl_shape) {
+        switch (S) {
+        case 8: launch_sl<8>(A, grid, count, fc.mode, c->stream); break;
+        case 24: launch_sl<24>(A, grid, count, fc.mode, c->stream); break;
+        case 32: launch_sl<32>(A, grid, count, fc.mode, c->stream); break;
+        default: launch_sl<16>(A, grid, count, fc.mode, c->stream); break;
+        }
+    } else if (refill > 0 && fc.mode != 2) {
         switch (S) {
         case 8: launch_refill<8>(A, grid, refill, count, fc.mode, c->stream); break;
         case 24: launch_refill<24>(A, grid, refill, count, fc.mode, c->stream); break;

@@ -39,6 +39,35 @@ def test_variants_bit_identical(gpu_renderer, mode):
         set_opts(gpu_renderer, VARIANTS[0])
 
 
+@pytest.mark.parametrize("spp", [2, 4, 64, 128])
+@pytest.mark.parametrize("mode", [L.MODE_LAMBERT, L.MODE_AO])
+def test_sample_lanes_bit_identical(gpu_renderer, mode, spp):
+    """The sample-parallel kernel (lanes = samples of a pixel, jitter stream advanced by LCG
+    skip-ahead, ordered fold) against the lane-per-pixel kernel and the oracle."""
+    xyzr, rgba = scenes.random_spheres(5000, seed=14, rmin=0.005, rmax=0.03)
+    w, h = (37, 29) if spp >= 64 else (101, 67)
+    cam = small_camera(w, h)
+    p = params(w, h, spp=spp, mode=mode, ao_distance=0.2, flags=L.FLAG_AUX)
+    gpu_renderer.set_spheres(xyzr, rgba)
+    gpu_renderer.build()
+    frames = {}
+    try:
+        for sl in (1, 0):
+            case(f"sample lanes {sl} mode {mode} spp {spp}")
+            gpu_renderer.set_option("sample_lanes", sl)
+            g = gpu_renderer.render(cam, p)
+            g.aux = gpu_renderer.download(w, h, outputs=(), aux=True).aux
+            frames[sl] = g
+    finally:
+        gpu_renderer.set_option("sample_lanes", 1)
+    a, b = frames[1], frames[0]
+    for k in ("hit_id", "t", "depth", "color", "rgba8", "aux"):
+        np.testing.assert_array_equal(getattr(a, k).view(np.uint8), getattr(b, k).view(np.uint8), err_msg=k)
+    if spp <= 4:
+        o = O.render_brute(xyzr, rgba, cam, oracle_params(p), aux=True)
+        assert_frames_equal(a, o, aux=True, what=f"sample lanes spp {spp}")
+
+
 def test_window_is_a_crop_of_the_full_frame(gpu_renderer):
     xyzr, rgba = scenes.random_spheres(4000, seed=13)
     cam = small_camera(120, 90)
