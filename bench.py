@@ -26,6 +26,24 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE summary of the dominant kernel on the 50M FCC 64-spp
+# AO frame (committed with the command that made it); its DRAM bytes per launch fill
+# roofline.traffic for that workload
+DRAM_PROFILE = os.path.join(ROOT, "profiles", "k_trace_sl_fcc50m_ao64_dram.md")
+
+
+def dram_traffic(workload_key):
+    """DRAM bytes per launch recorded in the committed counter summary for this workload, or None."""
+    try:
+        txt = open(DRAM_PROFILE).read()
+    except OSError:
+        return None
+    if workload_key not in txt:
+        return None
+    for line in txt.splitlines():
+        if line.startswith("dram_bytes_per_launch:"):
+            return int(line.split(":")[1].strip())
+    return None
 
 
 def log(msg):
@@ -341,6 +359,8 @@ def main():
     hit_samples = rays - P * args.spp if mode == L.MODE_AO else hit_px * args.spp
     algo_bytes = 64 * visits + 16 * tests + 16 * max(hit_samples, 0) + 32 * P
     achieved = algo_bytes / (trace_ms * 1e-3) / 1e9
+    traffic = (dram_traffic(f"cells {args.cells}, {args.width}x{args.height}, spp {args.spp}, mode {args.mode}")
+               if args.config == "fcc" else None)
     mrays = rays / (ms * 1e-3) / 1e6
     log(f"visits {visits} tests {tests} rays {rays} overflow {overflow} hit px {hit_px}")
     split = None
@@ -405,7 +425,8 @@ def main():
         "trace_work": split,
         "autotune": tuned,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "dram_frac": (round(traffic / (trace_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None),
                      "kernel": "k_trace (AO, persistent)", "algorithmic_bytes": int(algo_bytes)},
         "cpu_baseline": cpu,
     }
