@@ -544,6 +544,137 @@ __global__ __launch_bounds__(BC) void k_bu_chunk(const K* __restrict__ keys, con
     ap.flush(list, cnt + 1);
 }
 
+// The chunk pass, level-synchronous: the same tree, numbering and meeting rules as
+// k_bu_chunk, but the climb runs in rounds over a compact LDS list of the splits whose two
+// children have arrived, so a wavefront's lanes stay busy instead of following the few long
+// climbs. Round 0 delivers the leaves; round r merges the splits completed in round r - 1,
+// writes their nodes and delivers them to their parents. LDS records are chunk-relative 16-bit
+// (a reference is the leaf or node index minus the chunk start, bit 15 marking a leaf).
+// flags[] must be zero on entry; n >= 2.
+template <typename K, int BC>
+__global__ __launch_bounds__(BC) void k_bu_chunk_ls(const K* __restrict__ keys, const float4* __restrict__ leaf_sph,
+                                                    int64_t n64, Node* __restrict__ nodes, Node* meet,
+                                                    int2* meet_far, uint32_t* flags, uint32_t* list, uint32_t* cnt,
+                                                    SceneInfo* info) {
+    __shared__ float s_box[BC][12];     // the two child boxes parked at each in-chunk split
+    __shared__ uint16_t s_ref[BC][2];   // chunk-relative child references
+    __shared__ uint16_t s_far[BC][2];   // chunk-relative far range ends
+    __shared__ int8_t s_delta[BC + 1];  // delta(cs - 1 + j)
+    __shared__ uint8_t s_h[BC][2];
+    __shared__ uint8_t s_side[BC];
+    __shared__ uint32_t s_cnt[BC];
+    __shared__ uint16_t s_lst[2][BC];   // splits completed in the previous / current round
+    __shared__ uint32_t s_nl[2];
+    __shared__ uint32_t s_list[256];
+    __shared__ uint32_t s_n, s_base;
+    SplitList ap{s_list, &s_n, &s_base};
+    const int32_t n = (int32_t)n64;
+    const int32_t cs = (int32_t)blockIdx.x * BC;
+    const int32_t ce = cs + BC < n ? cs + BC : n;
+    const int32_t tid = (int32_t)threadIdx.x;
+    const int32_t k = cs + tid;
+    const float4 sph = k < ce ? leaf_sph[k] : make_float4(0.f, 0.f, 0.f, 0.f);
+    s_cnt[tid] = 0;
+    for (int j = tid; j <= BC; j += BC) s_delta[j] = (int8_t)split_delta(keys, n64, (int64_t)cs - 1 + j);
+    if (tid == 0) {
+        s_n = 0;
+        s_nl[0] = 0;
+        s_nl[1] = 0;
+    }
+    __syncthreads();
+    // deliver node X = [l, r] (box b, height h, reference ref, its children's boxes f and
+    // references kd when internal) to its parent; returns nothing, may push the parent split
+    auto deliver = [&](int32_t l, int32_t r, const float b[6], int h, bool internal, const float f[12], int2 kd,
+                       int32_t leaf, uint32_t* nl_next, uint16_t* lst_next) {
+        if (l == 0 && r == n - 1) {  // the root
+            store_node(nodes, f, kd, h);
+            info->height = (uint32_t)h;
+            return;
+        }
+        const int dl = s_delta[l - cs], dr = s_delta[r + 1 - cs];  // l - 1 >= cs - 1, r <= ce - 1
+        const bool left = dr > dl;
+        const int32_t sp = left ? r : l - 1;
+        int32_t ref;
+        uint16_t rrel;
+        if (internal) {
+            const int32_t idx = left ? r : l;
+            store_node(nodes + idx, f, kd, h);
+            ref = idx;
+            rrel = (uint16_t)(idx - cs);
+        } else {
+            ref = ~leaf;
+            rrel = (uint16_t)(0x8000u | (uint32_t)(leaf - cs));
+        }
+        const int side = left ? 0 : 1;
+        const int32_t far = left ? l : r;
+        if (sp < cs || sp + 1 >= ce) {  // the split lies on a chunk boundary
+            meet_put(meet + sp, meet_far + sp, side, b, ref, h, far);
+            if (atomicAdd(flags + sp, 1u) == 1u) ap.push((uint32_t)sp, list, cnt + 1);
+            return;
+        }
+        const int slot = sp - cs;
+#pragma unroll
+        for (int a = 0; a < 6; ++a) s_box[slot][side * 6 + a] = b[a];
+        s_ref[slot][side] = rrel;
+        s_far[slot][side] = (uint16_t)(far - cs);
+        s_h[slot][side] = (uint8_t)h;
+        s_side[slot] = (uint8_t)side;
+        uint32_t old = __hip_atomic_fetch_add(&s_cnt[slot], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (old == 1u) lst_next[atomicAdd(nl_next, 1u)] = (uint16_t)slot;
+    };
+    auto gref = [&](uint16_t rr) -> int32_t {
+        return (rr & 0x8000u) ? ~(cs + (int32_t)(rr & 0x7fffu)) : cs + (int32_t)rr;
+    };
+    // round 0: the leaves
+    if (k < ce) {
+        float b[6];
+        leaf_box(sph, b);
+        float f0[12];
+        deliver(k, k, b, 0, false, f0, make_int2(0, 0), k, &s_nl[0], s_lst[0]);
+    }
+    __syncthreads();
+    for (int rd = 0;; ++rd) {
+        const int cur = rd & 1, nxt = cur ^ 1;
+        const uint32_t count = s_nl[cur];
+        if (count == 0) break;
+        for (uint32_t i = (uint32_t)tid; i < count; i += BC) {
+            const int slot = s_lst[cur][i];
+            float f[12];
+#pragma unroll
+            for (int a = 0; a < 12; ++a) f[a] = s_box[slot][a];
+            const int2 kd = make_int2(gref(s_ref[slot][0]), gref(s_ref[slot][1]));
+            const int h0 = s_h[slot][0], h1 = s_h[slot][1];
+            const int h = 1 + (h0 > h1 ? h0 : h1);
+            const int32_t l = cs + (int32_t)s_far[slot][0], r = cs + (int32_t)s_far[slot][1];
+            float b[6];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                b[a] = fminf(f[a], f[6 + a]);
+                b[3 + a] = fmaxf(f[3 + a], f[9 + a]);
+            }
+            deliver(l, r, b, h, true, f, kd, 0, &s_nl[nxt], s_lst[nxt]);
+        }
+        __syncthreads();
+        if (tid == 0) s_nl[cur] = 0;
+        __syncthreads();
+    }
+    // in-chunk splits with one arrival: the other child straddles the chunk, so the parked
+    // child moves to the global meeting place
+    const int nsplit = ce - cs - 1;
+    for (int slot = tid; slot < nsplit; slot += BC) {
+        if (s_cnt[slot] != 1u) continue;
+        const int side = s_side[slot];
+        const int32_t sp = cs + slot;
+        float b[6];
+#pragma unroll
+        for (int a = 0; a < 6; ++a) b[a] = s_box[slot][side * 6 + a];
+        meet_put(meet + sp, meet_far + sp, side, b, gref(s_ref[slot][side]), s_h[slot][side],
+                 cs + (int32_t)s_far[slot][side]);
+        if (atomicAdd(flags + sp, 1u) == 1u) ap.push((uint32_t)sp, list, cnt + 1);
+    }
+    ap.flush(list, cnt + 1);
+}
+
 // Merge the two records at split s, write the node, deliver it to its parent's meeting place.
 // Returns the parent's split when this delivery completes it, else -1.
 template <typename K, bool VOL>
@@ -737,9 +868,23 @@ int hierarchy(mmhrt_ctx_s* c, const K* keys) {
     if (int e = ensure(c, c->flags, m, "meeting counters")) return e;
     launch_fill_u32(s, c->box_cnt.p, 0u, BOX_ROUNDS + 4);
     launch_fill_u32(s, c->flags.p, 0u, m);
-    // leaves per chunk workgroup: 1024 (77 KB of LDS, two per CU), 512 or 256
+    // leaves per chunk workgroup: 1024 (two per CU), 512 or 256; level-synchronous rounds unless
+    // the option build_ls is 0
     const int bc = c->opt_build_chunk == 256 ? 256 : (c->opt_build_chunk == 512 ? 512 : 1024);
-    if (bc == 256)
+    if (c->opt_build_ls != 0) {
+        if (bc == 256)
+            hipLaunchKernelGGL((k_bu_chunk_ls<K, 256>), dim3(blocks_for(n, 256)), dim3(256), 0, s, keys, c->leaf_sph.p,
+                               (int64_t)n, c->nodes.p, c->meet.p, c->meet_far.p, c->flags.p, c->va.p, c->box_cnt.p,
+                               c->info.p);
+        else if (bc == 512)
+            hipLaunchKernelGGL((k_bu_chunk_ls<K, 512>), dim3(blocks_for(n, 512)), dim3(512), 0, s, keys, c->leaf_sph.p,
+                               (int64_t)n, c->nodes.p, c->meet.p, c->meet_far.p, c->flags.p, c->va.p, c->box_cnt.p,
+                               c->info.p);
+        else
+            hipLaunchKernelGGL((k_bu_chunk_ls<K, 1024>), dim3(blocks_for(n, 1024)), dim3(1024), 0, s, keys,
+                               c->leaf_sph.p, (int64_t)n, c->nodes.p, c->meet.p, c->meet_far.p, c->flags.p, c->va.p,
+                               c->box_cnt.p, c->info.p);
+    } else if (bc == 256)
         hipLaunchKernelGGL((k_bu_chunk<K, 256>), dim3(blocks_for(n, 256)), dim3(256), 0, s, keys, c->leaf_sph.p,
                            (int64_t)n, c->nodes.p, c->meet.p, c->meet_far.p, c->flags.p, c->va.p, c->box_cnt.p,
                            c->info.p);

@@ -39,12 +39,15 @@ def test_small_scene_bit_exact(gpu_renderer, seed, mode, bits):
     assert_frames_equal(g, o, aux=True, what=f"seed={seed} mode={mode} bits={bits}")
 
 
-@pytest.mark.parametrize("chunk", [1024, 512, 256])
-def test_lbvh_equals_oracle_lbvh(gpu_renderer, chunk):
+@pytest.mark.parametrize("chunk,ls", [(1024, 1), (512, 1), (256, 1), (1024, 0), (256, 0)])
+def test_lbvh_equals_oracle_lbvh(gpu_renderer, chunk, ls):
+    """Both chunk passes (level-synchronous rounds, or one climbing thread per leaf) at every
+    chunk size give the oracle's tree node for node."""
     gpu_renderer.set_option("build_chunk", chunk)
+    gpu_renderer.set_option("build_ls", ls)
     for bits in (30, 63):
         for n, dup in [(1000, False), (50000, True), (2, False), (3, True), (4097, False)]:
-            case(f"lbvh n={n} dup={dup} bits={bits} chunk={chunk}")
+            case(f"lbvh n={n} dup={dup} bits={bits} chunk={chunk} ls={ls}")
             g = np.random.default_rng(n + bits)
             xyzr = np.zeros((n, 4), np.float32)
             xyzr[:, :3] = g.random((n, 3)).astype(np.float32)
@@ -62,6 +65,7 @@ def test_lbvh_equals_oracle_lbvh(gpu_renderer, chunk):
             np.testing.assert_array_equal(box.view(np.uint32), obox.view(np.uint32))
             assert height[0] >= int(np.ceil(np.log2(n)))
     gpu_renderer.set_option("build_chunk", 1024)
+    gpu_renderer.set_option("build_ls", 1)
 
 
 def test_edge_cases(gpu_renderer):
