@@ -215,6 +215,98 @@ __global__ __launch_bounds__(256) void k_conv_fwd32(ConvArgs a) {
     }
 }
 
+// The same convolution staged through LDS, for the full-resolution layers. A workgroup (4
+// wavefronts) computes 4 rows x 64 pixels x all output channels. For each 32-channel slice of
+// the input it stages rows y0-1..y0+4, pixels x0-1..x0+64 in LDS (the decoder's [skip,
+// upsampled] concatenation is resolved here, once per element), then every wavefront runs the 9
+// taps of its row out of LDS. Pixels are padded to 40 channels in LDS so that the 16 lanes of an
+// A-fragment read land on distinct banks. Weights: [tap][slice][co/16][lane][8], zero past CI.
+constexpr int LT_PX = 66, LT_ROWS = 6, LT_CP = 40;
+
+template <int NB>
+__global__ __launch_bounds__(256) void k_conv_fwd_lds(ConvArgs a) {
+    __shared__ __attribute__((aligned(16))) uint16_t tile[LT_ROWS * LT_PX * LT_CP];
+    const int CI = a.C1 + a.C2;
+    const int nslice = (CI + 31) / 32;
+    const int runs = (a.W + 63) / 64;
+    const int rblocks = (a.H + 3) / 4;
+    int blk = blockIdx.x;
+    const int run = blk % runs;
+    blk /= runs;
+    const int rb = blk % rblocks;
+    const int b = blk / rblocks;
+    const int x0 = run * 64, y0 = rb * 4;
+    const int NBT = a.CO / 16;
+    const int wave = threadIdx.x >> 6;
+    const int lane = __lane_id();
+    const int q = lane >> 4, lr = lane & 15;
+    f32x4 acc[4][NB];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int sl = 0; sl < nslice; ++sl) {
+        // stage the slice: 6 rows x 66 pixels x 4 groups of 8 channels, 16 bytes per thread step
+        for (int i = threadIdx.x; i < LT_ROWS * LT_PX * 4; i += 256) {
+            const int g8 = i & 3, pxl = (i >> 2) % LT_PX, ry = (i >> 2) / LT_PX;
+            const int yy = y0 - 1 + ry, xx = x0 - 1 + pxl, ci0 = sl * 32 + g8 * 8;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (ci0 < CI && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) {
+                const uint16_t* src;
+                if (ci0 < a.C1)
+                    src = a.X1 + (((size_t)b * a.H + yy) * a.W + xx) * a.C1 + ci0;
+                else
+                    src = a.X2 + (((size_t)b * (a.H >> 1) + (yy >> 1)) * (a.W >> 1) + (xx >> 1)) * a.C2 + (ci0 - a.C1);
+                v = *reinterpret_cast<const uint4*>(src);
+            }
+            *reinterpret_cast<uint4*>(tile + (ry * LT_PX + pxl) * LT_CP + g8 * 8) = v;
+        }
+        __syncthreads();
+#pragma unroll 1
+        for (int t = 0; t < 9; ++t) {
+            const int ry = wave + t / 3;  // tile row of input row y + t/3 - 1
+            const int dx = t % 3;         // tile pixel offset = output pixel - x0 + dx
+            Frag8 fa[4];
+#pragma unroll
+            for (int pb = 0; pb < 4; ++pb)
+                fa[pb].u = *reinterpret_cast<const uint4*>(tile + (ry * LT_PX + pb * 16 + lr + dx) * LT_CP + q * 8);
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                const int nb = j;
+                if (nb >= NBT) break;
+                Frag8 fb;
+                fb.u = *reinterpret_cast<const uint4*>(a.Wl + ((((size_t)t * nslice + sl) * NBT + nb) * 64 + lane) * 8);
+#pragma unroll
+                for (int pb = 0; pb < 4; ++pb)
+                    acc[pb][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[pb].v, fb.v, acc[pb][j], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    const int y = y0 + wave;
+    if (y >= a.H) return;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        if (j >= NBT) break;
+        const int co = j * 16 + lr;
+        const float bias = a.bias[co];
+#pragma unroll
+        for (int pb = 0; pb < 4; ++pb)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int x = x0 + pb * 16 + q * 4 + e;
+                if (x >= a.W) continue;
+                float v = acc[pb][j][e] + bias;
+                if (a.relu) v = fmaxf(v, 0.0f);
+                const size_t o = (((size_t)b * a.H + y) * a.W + x) * a.CO + co;
+                if (a.Yf)
+                    a.Yf[o] = v;
+                else
+                    a.Y[o] = f2bf(v);
+            }
+    }
+}
+
 // ------------------------------------------------------------------ backward data
 // dXc[p, ci] = sum_t sum_co G[p - o_t, co] * W[t][ci][co], G = dY * (Y > 0 if relu).
 // dXc is [B,H,W,CI] bf16 (full-resolution concatenated input gradient).
@@ -578,6 +670,25 @@ __global__ void k_relayout(const float* __restrict__ Wc, int CI, int CO, uint16_
     Wb[((((size_t)t * KC + co / 16) * NBI + ci / 16) * 16 + ci % 16) * 16 + co % 16] = v;
 }
 
+// Canonical W[t][ci][co] -> [t][ci/32][co/16][lane][8], zero past CI (the LDS-staged kernel).
+__global__ void k_relayout_lds(const float* __restrict__ Wc, int CI, int CO, uint16_t* __restrict__ Wl) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nslice = (CI + 31) / 32, NBT = CO / 16;
+    long n = 9L * nslice * NBT * 64 * 8;
+    if (i >= n) return;
+    const int e = (int)(i % 8);
+    long r = i / 8;
+    const int lane = (int)(r % 64);
+    r /= 64;
+    const int nb = (int)(r % NBT);
+    r /= NBT;
+    const int sl = (int)(r % nslice);
+    const int t = (int)(r / nslice);
+    const int ci = sl * 32 + 8 * (lane >> 4) + e;
+    const int co = nb * 16 + (lane & 15);
+    Wl[i] = ci < CI ? f2bf(Wc[((size_t)t * CI + ci) * CO + co]) : (uint16_t)0;
+}
+
 // Canonical W[t][ci][co] -> [k/32][co/16][lane][8] with k = t*CI + ci, zero past 9*CI.
 __global__ void k_relayout32(const float* __restrict__ Wc, int CI, int CO, uint16_t* __restrict__ W32) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -637,9 +748,10 @@ static size_t param_offset(int l) {
     return n;
 }
 
-// shape = 0 picks the launch shape of the 16x16x32 kernel from the launch size; otherwise it
-// forces pixel blocks per wave (shape % 10: 4, 2 or 1) and, for 4, channel blocks per wave
-// (shape / 10, 0 = the automatic choice) — a test hook that reaches every instantiation.
+// shape = 0 picks the kernel and launch shape from the launch size (the LDS-staged kernel for
+// large launches, otherwise the 16x16x32 kernel with fewer channel and pixel blocks per wave
+// as the launch shrinks); a test hook forces one: 5 = LDS-staged, else pixel blocks per wave
+// (shape % 10: 4, 2 or 1) and, for 4, channel blocks per wave (shape / 10, 0 = automatic).
 static void conv_fwd(hipStream_t s, const ConvArgs& a, bool k32, int shape) {
     long waves = (long)a.B * a.H * ((a.W + 63) / 64);
     int NBT = a.CO / 16;
@@ -649,6 +761,20 @@ static void conv_fwd(hipStream_t s, const ConvArgs& a, bool k32, int shape) {
         // enough wavefronts to fill 256 CUs: fewer channel blocks, then fewer pixels per wave
         const long want = 6144;
         int pbk = 4;
+        // the LDS-staged kernel where the launch is large (the full-resolution layers)
+        const bool lds = shape == 5 || (shape == 0 && waves >= want && NBT <= 6);
+        if (lds) {
+            long blocks = (long)a.B * ((a.H + 3) / 4) * ((a.W + 63) / 64);
+            switch (NBT) {
+            case 1: hipLaunchKernelGGL((k_conv_fwd_lds<1>), dim3(blocks), dim3(256), 0, s, a); break;
+            case 2: hipLaunchKernelGGL((k_conv_fwd_lds<2>), dim3(blocks), dim3(256), 0, s, a); break;
+            case 3: hipLaunchKernelGGL((k_conv_fwd_lds<3>), dim3(blocks), dim3(256), 0, s, a); break;
+            case 4: hipLaunchKernelGGL((k_conv_fwd_lds<4>), dim3(blocks), dim3(256), 0, s, a); break;
+            case 5: hipLaunchKernelGGL((k_conv_fwd_lds<5>), dim3(blocks), dim3(256), 0, s, a); break;
+            default: hipLaunchKernelGGL((k_conv_fwd_lds<6>), dim3(blocks), dim3(256), 0, s, a); break;
+            }
+            return;
+        }
         if (shape != 0) {
             pbk = shape % 10;
             if (pbk != 1 && pbk != 2) pbk = 4;
@@ -749,21 +875,27 @@ int Denoiser::ensure_params(mmhrt_ctx_s* c) {
     size_t wn32 = 0;
     for (const auto& L : kLayers) wn32 += (size_t)((9 * (L.c1 + L.c2) + 31) / 32) * 32 * L.co;
     if ((e = mmhrt::ensure(c, wf32, wn32, "denoiser fwd weights (k32)"))) return e;
+    size_t wnl = 0;
+    for (const auto& L : kLayers) wnl += (size_t)9 * ((L.c1 + L.c2 + 31) / 32) * 32 * L.co;
+    if ((e = mmhrt::ensure(c, wl, wnl, "denoiser fwd weights (LDS)"))) return e;
     if ((e = mmhrt::ensure(c, scalar, 4, "denoiser scalars"))) return e;
     return 0;
 }
 
 void Denoiser::relayout(hipStream_t s) {
-    size_t po = 0, wo = 0, w32 = 0;
+    size_t po = 0, wo = 0, w32 = 0, wlo = 0;
     for (const auto& L : kLayers) {
         int CI = L.c1 + L.c2;
         long nw = 9L * CI * L.co;
         long n32 = (long)((9 * CI + 31) / 32) * 32 * L.co;
         hipLaunchKernelGGL(k_relayout, dim3(nblk(nw)), dim3(256), 0, s, params.p + po, CI, L.co, wf.p + wo, wb.p + wo);
         hipLaunchKernelGGL(k_relayout32, dim3(nblk(n32)), dim3(256), 0, s, params.p + po, CI, L.co, wf32.p + w32);
+        long nl = 9L * ((CI + 31) / 32) * 32 * L.co;
+        hipLaunchKernelGGL(k_relayout_lds, dim3(nblk(nl)), dim3(256), 0, s, params.p + po, CI, L.co, wl.p + wlo);
         po += nw + L.co;
         wo += nw;
         w32 += n32;
+        wlo += nl;
     }
 }
 
@@ -802,18 +934,19 @@ int Denoiser::ensure_acts(mmhrt_ctx_s* c, int B, int Hp, int Wp, bool train) {
 // Forward pass over X0 [B,Hp,Wp,16] into `out` (fp32, channels 0-3 used).
 void Denoiser::forward(hipStream_t s, int B, int Hp, int Wp) {
     int Hs[4] = {Hp, Hp / 2, Hp / 4, Hp / 8}, Ws[4] = {Wp, Wp / 2, Wp / 4, Wp / 8};
-    size_t wo = 0, po = 0, w32 = 0;
+    size_t wo = 0, po = 0, w32 = 0, wlo = 0;
     auto conv = [&](int l, const uint16_t* x1, const uint16_t* x2, int lv, uint16_t* y, float* yf) {
         const LayerSpec& L = kLayers[l];
         ConvArgs A{};
         A.X1 = x1; A.X2 = x2; A.C1 = L.c1; A.C2 = L.c2; A.CO = L.co;
         A.B = B; A.H = Hs[lv]; A.W = Ws[lv];
-        A.Wt = wf.p + wo; A.Wt32 = wf32.p + w32;
+        A.Wt = wf.p + wo; A.Wt32 = wf32.p + w32; A.Wl = wl.p + wlo;
         A.bias = params.p + po + 9L * (L.c1 + L.c2) * L.co;
         A.Y = y; A.Yf = yf; A.relu = L.relu;
         conv_fwd(s, A, k32, conv_shape);
         wo += 9L * (L.c1 + L.c2) * L.co;
         w32 += (size_t)((9 * (L.c1 + L.c2) + 31) / 32) * 32 * L.co;
+        wlo += (size_t)9 * ((L.c1 + L.c2 + 31) / 32) * 32 * L.co;
         po += 9L * (L.c1 + L.c2) * L.co + L.co;
     };
     auto pl = [&](const uint16_t* x, uint16_t* y, int lv, int C) {
@@ -1089,7 +1222,7 @@ namespace mmhrt {
 namespace dn {
 void Denoiser::release_all() {
     mmhrt::release(params); mmhrt::release(grads); mmhrt::release(adam_m); mmhrt::release(adam_v);
-    mmhrt::release(scalar); mmhrt::release(wf); mmhrt::release(wb); mmhrt::release(wf32); mmhrt::release(acts); mmhrt::release(gbuf);
+    mmhrt::release(scalar); mmhrt::release(wf); mmhrt::release(wb); mmhrt::release(wf32); mmhrt::release(wl); mmhrt::release(acts); mmhrt::release(gbuf);
     mmhrt::release(raw); mmhrt::release(out); mmhrt::release(target); mmhrt::release(color);
 }
 }  // namespace dn

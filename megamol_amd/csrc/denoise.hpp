@@ -29,6 +29,7 @@ struct ConvArgs {
     int B, H, W;
     const uint16_t* Wt;    // [tap][ci/16][co/16][co%16][ci%16] (16x16x16 kernel)
     const uint16_t* Wt32;  // [k/32][co/16][lane][8], k = tap*CI + ci zero-padded (16x16x32 kernel)
+    const uint16_t* Wl;    // [tap][ci/32][co/16][lane][8], ci zero-padded (LDS-staged kernel)
     const float* bias;
     uint16_t* Y;
     float* Yf;
@@ -60,6 +61,7 @@ struct Denoiser {
     DevBuf<float> params, grads, adam_m, adam_v, scalar;
     DevBuf<uint16_t> wf, wb;   // bf16 weights in forward / backward fragment order
     DevBuf<uint16_t> wf32;     // bf16 weights in the 16x16x32 forward fragment order
+    DevBuf<uint16_t> wl;       // bf16 weights in the LDS-staged forward fragment order
     bool k32 = true;           // forward with v_mfma_f32_16x16x32_bf16
     int conv_shape = 0;        // 0: automatic launch shape of the k32 kernel (see conv_fwd)
     DevBuf<uint16_t> acts;     // activations
