@@ -450,6 +450,7 @@ static int64_t* option_slot(mmhrt_ctx c, const char* key) {
     if (k == "build_chunk") return &c->opt_build_chunk;
     if (k == "sample_lanes") return &c->opt_sample_lanes;
     if (k == "build_ls") return &c->opt_build_ls;
+    if (k == "sort_ri") return &c->opt_sort_ri;
     if (k == "stack_overflow") return &c->stack_overflow;
     return nullptr;
 }

@@ -20,8 +20,7 @@ namespace mmhrt {
 namespace {
 
 constexpr int RB = 256;      // threads per block
-constexpr int RI = 16;       // keys per thread per tile
-constexpr int RT = RB * RI;  // keys per tile
+// keys per thread per tile: RI = 8, 16 or 32 (a launch option); a tile is RB * RI keys
 
 __device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << __lane_id()) - 1ull; }
 
@@ -163,11 +162,11 @@ __global__ __launch_bounds__(256) void k_morton(const float4* __restrict__ sph, 
 // [w * RT/4, (w+1) * RT/4) and walks it in RI rounds of 64, so the wave order, the round order
 // and the lane order together are the input order (what makes the ranking stable). Every
 // thread issues all of its RI key (and value) loads before it ranks anything.
-constexpr int RW = RT / (RB / 64);  // keys per wave segment
 
-template <typename K>
+template <typename K, int RI>
 __global__ __launch_bounds__(RB) void k_radix_hist(const K* __restrict__ keys, uint64_t n, int shift,
                                                    uint32_t* __restrict__ hist) {
+    constexpr int RT = RB * RI, RW = RT / (RB / 64);
     __shared__ uint32_t h[256];
     h[threadIdx.x] = 0;
     const uint64_t base = (uint64_t)blockIdx.x * RT + (uint64_t)(threadIdx.x >> 6) * RW + __lane_id();
@@ -192,11 +191,12 @@ __global__ __launch_bounds__(RB) void k_radix_hist(const K* __restrict__ keys, u
 // digits turns the per-wave counts into offsets, the keys are placed in digit-sorted order in
 // LDS and finally written out in digit-contiguous runs, so that consecutive threads write
 // consecutive addresses.
-template <typename K>
+template <typename K, int RI>
 __global__ __launch_bounds__(RB) void k_radix_scatter(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                       K* __restrict__ kout, uint32_t* __restrict__ vout, uint64_t n,
                                                       int shift, const uint32_t* __restrict__ hist,
                                                       const uint32_t* __restrict__ offs) {
+    constexpr int RT = RB * RI, RW = RT / (RB / 64);
     __shared__ K s_key[RT];
     __shared__ uint32_t s_val[RT];
     __shared__ uint32_t wc[RB / 64][256];  // per-wave running digit counts, then per-wave offsets
@@ -819,8 +819,9 @@ __global__ void k_fill(uint32_t* p, uint32_t v, uint64_t n) {
 
 inline unsigned blocks_for(uint64_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
-template <typename K>
-int sort_pairs(mmhrt_ctx_s* c, K* ka, K* kb, uint32_t* va, uint32_t* vb, uint64_t n, int passes) {
+template <typename K, int RI>
+int sort_pairs_t(mmhrt_ctx_s* c, K* ka, K* kb, uint32_t* va, uint32_t* vb, uint64_t n, int passes) {
+    constexpr int RT = RB * RI;
     uint32_t tiles = (uint32_t)((n + RT - 1) / RT);
     uint64_t hlen = (uint64_t)tiles * 256;
     uint32_t groups = (tiles + CG - 1) / CG;
@@ -834,13 +835,13 @@ int sort_pairs(mmhrt_ctx_s* c, K* ka, K* kb, uint32_t* va, uint32_t* vb, uint64_
     uint32_t* vd = vb;
     for (int p = 0; p < passes; ++p) {
         int shift = 8 * p;
-        hipLaunchKernelGGL(k_radix_hist<K>, dim3(tiles), dim3(RB), 0, c->stream, src, n, shift, hist);
+        hipLaunchKernelGGL((k_radix_hist<K, RI>), dim3(tiles), dim3(RB), 0, c->stream, src, n, shift, hist);
         hipLaunchKernelGGL(k_col_sums, dim3(groups), dim3(256), 0, c->stream, hist, tiles, c->scan_tmp.p);
         hipLaunchKernelGGL(k_col_top, dim3(1), dim3(256), 0, c->stream, c->scan_tmp.p, groups);
         hipLaunchKernelGGL(k_col_offsets, dim3(groups), dim3(256), 0, c->stream, hist, tiles, groups, c->scan_tmp.p,
                            offs);
-        hipLaunchKernelGGL(k_radix_scatter<K>, dim3(tiles), dim3(RB), 0, c->stream, src, vs, dst, vd, n, shift, hist,
-                           offs);
+        hipLaunchKernelGGL((k_radix_scatter<K, RI>), dim3(tiles), dim3(RB), 0, c->stream, src, vs, dst, vd, n, shift,
+                           hist, offs);
         K* tk = src;
         src = dst;
         dst = tk;
@@ -849,6 +850,13 @@ int sort_pairs(mmhrt_ctx_s* c, K* ka, K* kb, uint32_t* va, uint32_t* vb, uint64_
         vd = tv;
     }
     return 0;  // passes is even: result is back in (ka, va)
+}
+
+template <typename K>
+int sort_pairs(mmhrt_ctx_s* c, K* ka, K* kb, uint32_t* va, uint32_t* vb, uint64_t n, int passes) {
+    if (c->opt_sort_ri == 8) return sort_pairs_t<K, 8>(c, ka, kb, va, vb, n, passes);
+    if (c->opt_sort_ri == 32 && sizeof(K) == 4) return sort_pairs_t<K, 32>(c, ka, kb, va, vb, n, passes);
+    return sort_pairs_t<K, 16>(c, ka, kb, va, vb, n, passes);
 }
 
 // Hierarchy and boxes over the sorted keys and gathered leaf spheres (also the refit: the

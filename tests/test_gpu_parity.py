@@ -39,15 +39,17 @@ def test_small_scene_bit_exact(gpu_renderer, seed, mode, bits):
     assert_frames_equal(g, o, aux=True, what=f"seed={seed} mode={mode} bits={bits}")
 
 
-@pytest.mark.parametrize("chunk,ls", [(1024, 1), (512, 1), (256, 1), (1024, 0), (256, 0)])
-def test_lbvh_equals_oracle_lbvh(gpu_renderer, chunk, ls):
+@pytest.mark.parametrize("chunk,ls,ri", [(1024, 1, 16), (512, 1, 8), (256, 1, 32), (1024, 0, 16), (256, 0, 8)])
+def test_lbvh_equals_oracle_lbvh(gpu_renderer, chunk, ls, ri):
     """Both chunk passes (level-synchronous rounds, or one climbing thread per leaf) at every
-    chunk size give the oracle's tree node for node."""
+    chunk size, over radix tiles of 8, 16 or 32 keys per thread, give the oracle's tree node
+    for node (sorted keys and leaf order included)."""
     gpu_renderer.set_option("build_chunk", chunk)
     gpu_renderer.set_option("build_ls", ls)
+    gpu_renderer.set_option("sort_ri", ri)
     for bits in (30, 63):
         for n, dup in [(1000, False), (50000, True), (2, False), (3, True), (4097, False)]:
-            case(f"lbvh n={n} dup={dup} bits={bits} chunk={chunk} ls={ls}")
+            case(f"lbvh n={n} dup={dup} bits={bits} chunk={chunk} ls={ls} ri={ri}")
             g = np.random.default_rng(n + bits)
             xyzr = np.zeros((n, 4), np.float32)
             xyzr[:, :3] = g.random((n, 3)).astype(np.float32)
@@ -66,6 +68,7 @@ def test_lbvh_equals_oracle_lbvh(gpu_renderer, chunk, ls):
             assert height[0] >= int(np.ceil(np.log2(n)))
     gpu_renderer.set_option("build_chunk", 1024)
     gpu_renderer.set_option("build_ls", 1)
+    gpu_renderer.set_option("sort_ri", 16)
 
 
 def test_edge_cases(gpu_renderer):
