@@ -114,6 +114,10 @@ def forward(tp, x0, round_bf16=False):
                 # into the skip part and the 2x2 sums of the upsampled part
                 x = bf16_round_grad_only(x)
         w, b = tp[i]
+        if round_bf16:
+            # the GPU multiplies bf16 copies of the fp32 master weights; their gradient is taken
+            # with respect to the master weights (straight through the rounding)
+            w = bf16_round(w, grad=False)
         y = F.conv2d(x, w, b, padding=1)
         if LAYERS[i][4]:
             y = F.relu(y)
