@@ -109,7 +109,7 @@ struct mmhrt_ctx_s {
     int64_t opt_build_chunk = 1024; // leaves per LBVH chunk workgroup: 1024, 512 or 256
     int64_t opt_sample_lanes = 1;   // sample-parallel traversal for power-of-two spp (Lambert/AO)
     int64_t opt_build_ls = 1;       // level-synchronous LBVH chunk pass (0: one climbing thread per leaf)
-    int64_t opt_sort_ri = 16;       // radix sort keys per thread per tile: 8, 16 or 32 (32-bit keys)
+    int64_t opt_sort_ri = 32;       // radix sort keys per thread per tile: 8, 16 or 32 (32 for 32-bit keys only)
     int64_t refits_since_build = 0;
     int64_t stack_overflow = 0;  // set by mmhrt_get_counters from the trace kernel's flag
 

@@ -1,11 +1,10 @@
 """GPU tests of the BF16 MFMA denoiser against the CPU reference network (oracle/unet_ref.py).
 
-Activations are bf16 on the GPU, so the comparisons carry a tolerance: relative L2 error of the
-output below 4e-2 against a float32 reference that rounds its activations to bf16 at the same
-points; per layer, weight gradients within 0.1 (cosine above 0.995) of that reference and no
-further from the exact float64 gradients than twice the bf16 reference's own distance to them
-(two pipelines that round to bf16 at the same points but accumulate in different orders land
-on different bf16 values; each is about as far from exact as the other).
+Weights and activations are bf16 on the GPU, so the comparisons are against a float32 reference
+that rounds weights, activations and activation gradients to bf16 at the same points: relative
+L2 error of the output below 1e-2; per layer, weight gradients within 5e-3 (cosine above
+0.9999) of that reference and no further (x1.2) from the exact float64 gradients than it is.
+Only the fp32 accumulation order differs, which moves results by a few 1e-4.
 """
 import numpy as np
 import pytest
@@ -50,7 +49,8 @@ def test_forward_matches_reference(dnz, shape, k32):
     dnz.r.set_option("dn_k32", 1)
     dnz.r.set_option("dn_conv_shape", 0)
     assert out.shape == ref.shape
-    assert rel(out, ref) < 4e-2, rel(out, ref)
+    print(f"forward rel. L2 error {rel(out, ref):.2e}")
+    assert rel(out, ref) < 1e-2, rel(out, ref)
 
 
 def test_gradients_match_reference(dnz):
@@ -80,12 +80,12 @@ def test_gradients_match_reference(dnz):
     for r in rows:
         print("%-6s gpu-exact %.4f bf16ref-exact %.4f gpu-bf16ref %.4f cos %.5f | bias gpu %.4f ref %.4f" % r)
     for name, err_gpu, err_bf16, err_pair, cos, _, _ in rows:
-        # bf16 storage of activations and their gradients alone moves the deepest layers' weight
-        # gradients by about 3% from the exact float64 ones; the GPU must be as close to exact as
-        # the bf16-rounding reference is, up to a factor 2 (two independent roundings), and close
-        # to that reference itself
-        assert err_gpu < max(2.0 * err_bf16, 1e-2), (name, err_gpu, err_bf16)
-        assert err_pair < 0.1 and cos > 0.995, (name, err_pair, cos)
+        # bf16 weights and bf16 storage of activations and their gradients move the deepest
+        # layers' weight gradients by about 3% from the exact float64 ones; the reference rounds
+        # at the same points, so the GPU must match it to a few 1e-4 (accumulation order) and be
+        # no further from exact than it
+        assert err_gpu < max(1.2 * err_bf16, 1e-3), (name, err_gpu, err_bf16)
+        assert err_pair < 5e-3 and cos > 0.9999, (name, err_pair, cos)
     assert np.array_equal(dnz.get_params(), p), "apply=0 must not update the parameters"
 
 
