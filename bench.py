@@ -50,6 +50,11 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def scenes_mod():
+    from megamol_amd import scenes
+    return scenes
+
+
 def make_scene(args):
     from megamol_amd import ParticleList, _lib as L, scenes
 
@@ -64,6 +69,20 @@ def make_scene(args):
         return dict(lists=lists, tf=tf, xyzr=xyzr, inten=inten, cam=cam, workload=(
             f"FCC crystal {len(xyzr)} spheres (cells {args.cells}), {args.width}x{args.height}, AO {args.spp} spp, "
             f"AO distance {args.ao_distance}, LBVH rebuild per frame"))
+    if args.config == "clusters":
+        t0 = time.time()
+        n = args.n if args.n != 1_000_000 else 500_000_000
+        xyzr, cl = scenes.clusters(n, with_cluster=True)
+        inten = (cl.astype(np.float32) * np.float32(1.0 / 10_000))
+        del cl
+        log(f"generated {len(xyzr)} clustered spheres in {time.time() - t0:.1f}s")
+        tf = scenes.transfer_function_256()
+        lists = [ParticleList(L.VERTDATA_FLOAT_XYZR, xyzr, L.COLDATA_FLOAT_I, inten, count=len(xyzr),
+                              col_min=0.0, col_max=1.0)]
+        cam = scenes.clusters_camera(width=args.width, height=args.height)
+        return dict(lists=lists, tf=tf, xyzr=xyzr, inten=inten, cam=cam, workload=(
+            f"clustered scene {len(xyzr)} spheres (10k Gaussian clusters in a 1e4 box), {args.width}x{args.height}, "
+            f"AO {args.spp} spp, AO distance {args.ao_distance}, LBVH rebuild per frame"))
     xyzr, rgba8 = scenes.uniform_cube(args.n)
     lists = [ParticleList(L.VERTDATA_FLOAT_XYZR, xyzr, L.COLDATA_UINT8_RGBA, rgba8, count=len(xyzr))]
     cam = scenes.uniform_cube_camera(args.width, args.height)
@@ -238,7 +257,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", choices=["fcc", "cube", "series"], default="fcc")
+    ap.add_argument("--config", choices=["fcc", "cube", "series", "clusters"], default="fcc")
+    ap.add_argument("--orbit", type=int, default=0,
+                    help="clusters config: also render an orbit of this many cameras at 1 spp in one batched launch")
     ap.add_argument("--series-n", type=int, default=4_000_000)
     ap.add_argument("--series-t", type=int, default=64)
     ap.add_argument("--autotune", action="store_true", help="series config: tune refit period and launch knobs")
@@ -278,6 +299,10 @@ def main():
         return
     if args.config == "cube" and args.width == 1920:
         args.width = args.height = 1024
+    if args.config == "clusters" and args.width == 1920:
+        args.width, args.height = 3840, 2160
+    if args.config == "clusters" and args.bits == 30:
+        args.bits = 63  # 30-bit codes put thousands of clustered spheres on one key (10-unit cells)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the sphere path runs on a single MI355X in this build "
               "(multi-GPU image-tile sharding is designed, not built). Run with --gpus 1.", file=sys.stderr)
@@ -380,11 +405,26 @@ def main():
         log(f"per-ray work: {split}")
 
     dn = None
-    if mode == L.MODE_AO and args.config == "fcc" and not args.no_denoise:
+    if mode == L.MODE_AO and args.config in ("fcc", "clusters") and not args.no_denoise:
         dn = denoised_leg(r, scene, args, cam, p)
         dn["plain_over_denoised"] = round(ms / dn["ms"], 2)
         dn["target_8x_at_38db"] = bool(dn["plain_over_denoised"] >= 8.0 and
                                        dn["psnr_db_vs_256spp"]["denoised_1spp"] >= 38.0)
+
+    orbit = None
+    if args.config == "clusters" and args.orbit > 0:
+        # the orbit path: every camera at 1 spp in one batched persistent launch captured in a
+        # hipGraph (one LBVH build for the static scene, then all traversal jobs)
+        cams = scenes_mod().orbit_cameras(args.orbit, width=args.width, height=args.height)
+        r.set_timesteps(scene["xyzr"][None], None, (200, 200, 220, 255))
+        po = params(args.width, args.height, spp=1, mode=L.MODE_AO, ao_distance=args.ao_distance)
+        jobs = [(0, c, po) for c in cams]
+        r.render_batch(jobs, refit_period=1, use_graph=True)  # warm-up: allocations, graph
+        _, oms = r.render_batch(jobs, refit_period=1, use_graph=True)
+        orbit = {"frames": args.orbit, "spp": 1, "ms_per_launch": round(oms, 3),
+                 "ms_per_frame": round(oms / args.orbit, 3),
+                 "note": "one launch: LBVH build of the timestep + all orbit frames, hipGraph"}
+        log(f"orbit: {orbit}")
 
     cpu = None
     if not args.skip_cpu:
@@ -429,6 +469,7 @@ def main():
                      "dram_frac": (round(traffic / (trace_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None),
                      "kernel": "k_trace (AO, persistent)", "algorithmic_bytes": int(algo_bytes)},
         "cpu_baseline": cpu,
+        "orbit": orbit,
     }
     print(json.dumps(out), flush=True)
     r.close()

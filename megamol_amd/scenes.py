@@ -127,19 +127,50 @@ def time_series_cameras(step: int, box: float = 200.0, seed: int = 3):
 
 
 def clusters(n: int, n_clusters: int = 10_000, box: float = 1e4, mu_sigma: float = 3.0, s_sigma: float = 0.5,
-             mu_r: float = -0.5, s_r: float = 0.3, seed: int = 5):
+             mu_r: float = -0.5, s_r: float = 0.3, seed: int = 5, with_cluster=False):
+    """n spheres drawn from n_clusters Gaussian clusters: centres uniform in a box^3 cube,
+    sigma lognormal(mu_sigma, s_sigma), radius lognormal(mu_r, s_r); xyzr float32 (N,4), sorted by
+    cluster. with_cluster also returns each sphere's cluster index (int32)."""
     g_c, g_s, g_a, g_p, g_r = _rngs(seed, 5)
     centres = g_c.random((n_clusters, 3)) * box
     sig = g_s.lognormal(mu_sigma, s_sigma, n_clusters)
-    assign = np.sort(g_a.integers(0, n_clusters, n))
+    counts = g_a.multinomial(n, np.full(n_clusters, 1.0 / n_clusters))
     xyzr = np.empty((n, 4), np.float32)
+    cl = np.empty(n, np.int32) if with_cluster else None
+    o = 0
     step = 1 << 24
+    a_all = np.repeat(np.arange(n_clusters, dtype=np.int32), counts)
     for o in range(0, n, step):
-        a = assign[o:o + step]
+        a = a_all[o:o + step]
         m = len(a)
         xyzr[o:o + m, :3] = (centres[a] + g_p.normal(size=(m, 3)) * sig[a, None]).astype(np.float32)
         xyzr[o:o + m, 3] = g_r.lognormal(mu_r, s_r, m).astype(np.float32)
-    return xyzr
+        if with_cluster:
+            cl[o:o + m] = a
+    return (xyzr, cl) if with_cluster else xyzr
+
+
+def clusters_camera(box: float = 1e4, width=3840, height=2160, fovy_deg=40.0):
+    """Looking at the centre of the cluster box from outside one corner region."""
+    c = np.full(3, 0.5 * box)
+    d = np.array([0.35, 0.25, 1.0])
+    d /= np.linalg.norm(d)
+    dist = 0.9 * box
+    pos = c + d * dist
+    return look_at(tuple(pos), tuple(c), (0, 1, 0), fovy_deg=fovy_deg, aspect=width / height, near=1.0,
+                   far=4.0 * box)
+
+
+def orbit_cameras(k: int, box: float = 1e4, width=3840, height=2160, fovy_deg=40.0):
+    """k cameras on a circle around the box centre (the orbit path of the clustered scene)."""
+    c = np.full(3, 0.5 * box)
+    out = []
+    for i in range(k):
+        a = 2.0 * np.pi * i / k
+        pos = c + 0.9 * box * np.array([np.sin(a) * 0.95, 0.25, np.cos(a) * 0.95])
+        out.append(look_at(tuple(pos), tuple(c), (0, 1, 0), fovy_deg=fovy_deg, aspect=width / height, near=1.0,
+                           far=4.0 * box))
+    return out
 
 
 def random_spheres(n: int, seed: int = 0, lo=0.0, hi=1.0, rmin=0.01, rmax=0.05):
