@@ -115,9 +115,14 @@ __device__ __forceinline__ bool step_closest(const TraceArgs& A, const RayPre& r
     float tn0 = slab(r, a.x, a.y, a.z, a.w, b.x, b.y, tmin, t_best);
     float tn1 = slab(r, b.z, b.w, c.x, c.y, c.z, c.w, tmin, t_best);
     int32_t ch0 = dd.x, ch1 = dd.y;
-    if (tn0 != INFINITY && ch0 < 0) {
+    // both leaf spheres are fetched before either is tested: one memory round trip, not two
+    const bool lf0 = tn0 != INFINITY && ch0 < 0, lf1 = tn1 != INFINITY && ch1 < 0;
+    float4 sp0 = make_float4(0.f, 0.f, 0.f, 0.f), sp1 = sp0;
+    if (lf0) sp0 = A.leaf_sph[~ch0];
+    if (lf1) sp1 = A.leaf_sph[~ch1];
+    if (lf0) {
         int leaf = ~ch0;
-        float t = sphere_t(r.o, r.d, A.leaf_sph[leaf], tmin, tmax);
+        float t = sphere_t(r.o, r.d, sp0, tmin, tmax);
         if (COUNT) cnt.tests++;
         if (t >= 0.0f) {
             if (t < t_best) {
@@ -129,9 +134,9 @@ __device__ __forceinline__ bool step_closest(const TraceArgs& A, const RayPre& r
         }
         tn0 = INFINITY;
     }
-    if (tn1 != INFINITY && ch1 < 0) {
+    if (lf1) {
         int leaf = ~ch1;
-        float t = sphere_t(r.o, r.d, A.leaf_sph[leaf], tmin, tmax);
+        float t = sphere_t(r.o, r.d, sp1, tmin, tmax);
         if (COUNT) cnt.tests++;
         if (t >= 0.0f) {
             if (t < t_best) {
@@ -172,19 +177,21 @@ __device__ __forceinline__ int step_any(const TraceArgs& A, const RayPre& r, flo
     float tn1 = slab(r, b.z, b.w, c.x, c.y, c.z, c.w, tmin, tmax);
     int32_t ch0 = dd.x, ch1 = dd.y;
     bool h0 = tn0 != INFINITY, h1 = tn1 != INFINITY;
+    const bool lf0 = h0 && ch0 < 0 && ~ch0 != skip, lf1 = h1 && ch1 < 0 && ~ch1 != skip;
+    float4 sp0 = make_float4(0.f, 0.f, 0.f, 0.f), sp1 = sp0;
+    if (lf0) sp0 = A.leaf_sph[~ch0];
+    if (lf1) sp1 = A.leaf_sph[~ch1];
     if (h0 && ch0 < 0) {
-        int leaf = ~ch0;
-        if (leaf != skip) {
+        if (lf0) {
             if (COUNT) cnt.tests++;
-            if (sphere_t(r.o, r.d, A.leaf_sph[leaf], tmin, tmax) >= 0.0f) return 2;
+            if (sphere_t(r.o, r.d, sp0, tmin, tmax) >= 0.0f) return 2;
         }
         h0 = false;
     }
     if (h1 && ch1 < 0) {
-        int leaf = ~ch1;
-        if (leaf != skip) {
+        if (lf1) {
             if (COUNT) cnt.tests++;
-            if (sphere_t(r.o, r.d, A.leaf_sph[leaf], tmin, tmax) >= 0.0f) return 2;
+            if (sphere_t(r.o, r.d, sp1, tmin, tmax) >= 0.0f) return 2;
         }
         h1 = false;
     }
