@@ -151,6 +151,15 @@ def denoised_leg(r, scene, args, cam, p_plain):
         r.render_async(cam, p1)
         dnz.denoise_frame(depth_scale)
 
+    tuned1 = None
+    if not args.no_autotune and not args.option:
+        # the 1-spp frame has its own best launch (lane-per-pixel kernel, refill threshold)
+        from megamol_amd import autotune
+        r.build(L.BUILD_REBUILD, args.bits)
+        cfg1, best1, _ = autotune.tune_frame(r, cam, p1)
+        tuned1 = {"config": cfg1, "trace_ms": round(best1, 3)}
+        log(f"autotuned 1-spp launch: {cfg1} ({best1:.3f} ms trace)")
+
     for _ in range(args.warmup):
         frame()
     r.sync()
@@ -189,7 +198,7 @@ def denoised_leg(r, scene, args, cam, p_plain):
                                  "raw_1spp": round(psnr(raw, ref), 2), "box5_filter_1spp": round(psnr(box, ref), 2)},
            "ssim_vs_256spp": {"denoised_1spp": round(ssim(den, ref), 4), "plain": round(ssim(plain, ref), 4),
                               "raw_1spp": round(ssim(raw, ref), 4), "box5_filter_1spp": round(ssim(box, ref), 4)},
-           "weights": trained, "weights_meta": meta}
+           "weights": trained, "weights_meta": meta, "autotune_1spp": tuned1}
     log(f"denoised frame {ms:.2f} ms (no denoiser {ms_nodn:.2f}); PSNR {res['psnr_db_vs_256spp']}")
     if args.save_images:
         # centre crops for offline error analysis: the 256-spp mean albedo (AO distance ~0: every
