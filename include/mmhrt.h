@@ -137,6 +137,18 @@ typedef struct mmhrt_framebuffer {
     uint16_t* aux;     /* width*height*8 fp16: albedo rgb, normal xyz, linear depth, visibility */
 } mmhrt_framebuffer;
 
+/* ---- framebuffer: device views of the last rendered frame ----------------- */
+typedef struct mmhrt_device_framebuffer {
+    void* color;     /* width*height float4 RGBA (RGBA32F, the reference's colour surface) */
+    void* depth;     /* width*height float pseudo depth (R32F, the reference's depth surface) */
+    void* t;         /* width*height float linear distance */
+    void* hit_id;    /* width*height int32 global particle index, -1 on miss */
+    void* rgba8;     /* width*height RGBA8 */
+    void* aux;       /* width*height*8 fp16 features, NULL unless the frame had MMHRT_FLAG_AUX */
+    void* stream;    /* hipStream_t the frame was rendered on: consumers order their work after it */
+    uint32_t width, height;
+} mmhrt_device_framebuffer;
+
 /* ---- build policy --------------------------------------------------------- */
 enum {
     MMHRT_BUILD_REBUILD = 0, /* full LBVH build (the reference always builds: SphereGeometry.cpp:197-214) */
@@ -179,6 +191,12 @@ MMHRT_API int32_t mmhrt_render(mmhrt_ctx ctx, const mmhrt_camera* cam, const mmh
                                mmhrt_framebuffer* out);
 /* Copy the last rendered frame to host buffers (any pointer NULL = skip). */
 MMHRT_API int32_t mmhrt_download(mmhrt_ctx ctx, mmhrt_framebuffer* out);
+
+/* Device pointers of the last rendered frame for zero-copy chaining on the GPU: the analogue of
+ * the CUDA surfaces and exec stream of CallRender3DCUDA (plugins/optix_hpg/src/CallRender3DCUDA.h:10-26).
+ * The buffers belong to the context and stay valid until the next render, a resize or
+ * mmhrt_destroy; the frame may still be in flight on `stream`. */
+MMHRT_API int32_t mmhrt_get_device_framebuffer(mmhrt_ctx ctx, mmhrt_device_framebuffer* out);
 
 /* Wait for all queued work on the context's stream. */
 MMHRT_API int32_t mmhrt_sync(mmhrt_ctx ctx);

@@ -57,6 +57,14 @@ class Framebuffer(C.Structure):
     ]
 
 
+class DeviceFramebuffer(C.Structure):
+    _fields_ = [
+        ("color", C.c_void_p), ("depth", C.c_void_p), ("t", C.c_void_p), ("hit_id", C.c_void_p),
+        ("rgba8", C.c_void_p), ("aux", C.c_void_p), ("stream", C.c_void_p), ("width", C.c_uint32),
+        ("height", C.c_uint32),
+    ]
+
+
 class Job(C.Structure):
     _fields_ = [("timestep", C.c_uint32), ("reserved", C.c_uint32), ("camera", Camera), ("params", RenderParams),
                 ("out_offset", C.c_uint64)]
@@ -73,6 +81,7 @@ SIGNATURES = {
     "mmhrt_build": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32]),
     "mmhrt_render": (C.c_int32, [C.c_void_p, C.POINTER(Camera), C.POINTER(RenderParams), C.POINTER(Framebuffer)]),
     "mmhrt_download": (C.c_int32, [C.c_void_p, C.POINTER(Framebuffer)]),
+    "mmhrt_get_device_framebuffer": (C.c_int32, [C.c_void_p, C.POINTER(DeviceFramebuffer)]),
     "mmhrt_sync": (C.c_int32, [C.c_void_p]),
     "mmhrt_get_info": (C.c_int32, [C.c_void_p, C.POINTER(C.c_uint64), C.c_void_p, C.POINTER(C.c_uint64)]),
     "mmhrt_get_bvh": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),

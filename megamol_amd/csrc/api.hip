@@ -358,6 +358,23 @@ int32_t mmhrt_download(mmhrt_ctx c, mmhrt_framebuffer* out) {
     return MMHRT_OK;
 }
 
+int32_t mmhrt_get_device_framebuffer(mmhrt_ctx c, mmhrt_device_framebuffer* out) {
+    if (int e = check_ctx(c)) return e;
+    if (!out) return fail(c, MMHRT_E_INVALID, "get_device_framebuffer: out is NULL");
+    if (c->fb_w == 0) return fail(c, MMHRT_E_STATE, "get_device_framebuffer: nothing rendered yet");
+    size_t P = (size_t)c->fb_w * c->fb_h;
+    out->color = c->fb_col.p;
+    out->depth = c->fb_depth.p;
+    out->t = c->fb_t.p;
+    out->hit_id = c->fb_id.p;
+    out->rgba8 = c->fb_rgba8.p;
+    out->aux = (c->last_aux && c->fb_aux.p && c->fb_aux.cap >= P * 8) ? c->fb_aux.p : nullptr;
+    out->stream = c->stream;
+    out->width = c->fb_w;
+    out->height = c->fb_h;
+    return MMHRT_OK;
+}
+
 int32_t mmhrt_sync(mmhrt_ctx c) {
     if (int e = check_ctx(c)) return e;
     hipError_t e = hipStreamSynchronize(c->stream);

@@ -185,6 +185,12 @@ class SphereRenderer:
         self._ck(self.lib.mmhrt_download(self.ctx, C.byref(fb)))
         return fr
 
+    def device_framebuffer(self) -> L.DeviceFramebuffer:
+        """Device pointers (and the stream) of the last rendered frame, for zero-copy consumers."""
+        d = L.DeviceFramebuffer()
+        self._ck(self.lib.mmhrt_get_device_framebuffer(self.ctx, C.byref(d)))
+        return d
+
     @staticmethod
     def _alloc(fr: Frame, fb: L.Framebuffer, outputs, aux) -> None:
         W, H = fr.width, fr.height

@@ -223,3 +223,34 @@ def test_bounce_mode_bit_exact(gpu_renderer, bounces):
     p = params(64, 48, spp=2, mode=L.MODE_BOUNCE, max_bounces=bounces)
     g, o = render_both(gpu_renderer, xyzr, rgba, cam, p, 30)
     assert_frames_equal(g, o, what=f"bounces={bounces}")
+
+
+def test_device_framebuffer_is_the_downloaded_frame(gpu_renderer):
+    """The zero-copy device views (CallRender3DCUDA analogue) hold exactly what download copies."""
+    import ctypes as C
+    case("device framebuffer")
+    xyzr, rgba = scenes.random_spheres(3000, seed=8)
+    cam = small_camera(70, 50)
+    p = params(70, 50, spp=2, mode=L.MODE_AO, ao_distance=0.3, flags=L.FLAG_AUX)
+    gpu_renderer.set_spheres(xyzr, rgba)
+    gpu_renderer.build()
+    fr = gpu_renderer.render(cam, p)
+    aux = gpu_renderer.download(70, 50, outputs=(), aux=True).aux
+    d = gpu_renderer.device_framebuffer()
+    assert (d.width, d.height) == (70, 50) and d.stream and d.aux
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    n = 70 * 50
+
+    def get(ptr, dtype, count):
+        out = np.empty(count, dtype)
+        assert hip.hipMemcpy(out.ctypes.data, ptr, out.nbytes, 2) == 0  # hipMemcpyDeviceToHost
+        return out
+
+    np.testing.assert_array_equal(get(d.hit_id, np.int32, n).reshape(50, 70), fr.hit_id)
+    np.testing.assert_array_equal(get(d.rgba8, np.uint32, n).reshape(50, 70), fr.rgba8)
+    np.testing.assert_array_equal(get(d.color, np.float32, 4 * n).reshape(50, 70, 4).view(np.uint32),
+                                  fr.color.view(np.uint32))
+    np.testing.assert_array_equal(get(d.depth, np.float32, n).reshape(50, 70).view(np.uint32),
+                                  fr.depth.view(np.uint32))
+    np.testing.assert_array_equal(get(d.aux, np.uint16, 8 * n).reshape(50, 70, 8), aux.view(np.uint16))
