@@ -46,6 +46,14 @@ def dram_traffic(workload_key):
     return None
 
 
+def trace_kernel_label(r, spp, mode):
+    """Name of the traversal kernel the library dispatches for this frame (mirrors render_frame)."""
+    sl = (r.get_option("sample_lanes") and r.get_option("refill") == 0 and spp > 1
+          and ((spp <= 64 and spp & (spp - 1) == 0) or spp % 64 == 0))
+    name = "k_trace_sl" if sl else "k_trace"
+    return f"{name} ({mode}, {spp} spp)"
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -393,6 +401,7 @@ def main():
     hit_samples = rays - P * args.spp if mode == L.MODE_AO else hit_px * args.spp
     algo_bytes = 64 * visits + 16 * tests + 16 * max(hit_samples, 0) + 32 * P
     achieved = algo_bytes / (trace_ms * 1e-3) / 1e9
+    kernel_name = trace_kernel_label(r, args.spp, args.mode)
     traffic = (dram_traffic(f"cells {args.cells}, {args.width}x{args.height}, spp {args.spp}, mode {args.mode}")
                if args.config == "fcc" else None)
     mrays = rays / (ms * 1e-3) / 1e6
@@ -476,7 +485,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "dram_frac": (round(traffic / (trace_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None),
-                     "kernel": "k_trace (AO, persistent)", "algorithmic_bytes": int(algo_bytes)},
+                     "kernel": kernel_name, "algorithmic_bytes": int(algo_bytes)},
         "cpu_baseline": cpu,
         "orbit": orbit,
     }
