@@ -275,10 +275,13 @@ MMHRT_API int32_t mmhrt_denoiser_train_step(mmhrt_ctx ctx, const uint16_t* feats
 /* Tuning knobs (autotuner output): "stack_lds" (LDS stack entries per lane), "tile_w", "tile_h",
  * "persistent" (0/1), "refill" (ballot refill threshold in lanes), "blocks_per_cu" (persistent
  * workgroups per CU), "refit_period" (MMHRT_BUILD_AUTO: refits between rebuilds), "build_chunk"
- * (leaves per LBVH chunk workgroup: 1024, 512 or 256), "dn_k32" (denoiser forward on the
- * 16x16x32 MFMA kernels, default 1), "dn_conv_shape" (test hook: forced launch shape of those
- * kernels, 0 = automatic). Diagnostics: "count_visits" (0/1) and, read-only in effect,
- * "stack_overflow". */
+ * (leaves per LBVH chunk workgroup: 1024, 512 or 256), "build_ls" (level-synchronous chunk
+ * pass, default 1; 0: one climbing thread per leaf), "build_grid" (test hook: at most this many
+ * persistent chunk workgroups, 0 = as many as fit), "sort_ri" (radix sort keys per thread per
+ * tile: 8, 16 or 32), "sample_lanes" (samples of a pixel on the lanes for power-of-two spp,
+ * default 1), "dn_k32" (denoiser forward on the 16x16x32 MFMA kernels, default 1),
+ * "dn_conv_shape" (test hook: forced launch shape of those kernels, 0 = automatic).
+ * Diagnostics: "count_visits" (0/1) and, read-only in effect, "stack_overflow". */
 MMHRT_API int32_t mmhrt_set_option(mmhrt_ctx ctx, const char* key, int64_t value);
 MMHRT_API int32_t mmhrt_get_option(mmhrt_ctx ctx, const char* key, int64_t* value);
 

@@ -138,7 +138,7 @@ int32_t mmhrt_destroy(mmhrt_ctx c) {
     hipStreamSynchronize(c->stream);
     release(c->sph); release(c->col); release(c->staging);
     release(c->meet); release(c->meet_far); release(c->ka); release(c->kb); release(c->kla); release(c->klb);
-    release(c->va); release(c->vb); release(c->hist); release(c->scan_tmp);
+    release(c->va); release(c->vb); release(c->hist); release(c->scan_tmp); release(c->sort_fault);
     release(c->leaf_sph); release(c->leaf_id); release(c->nodes); release(c->flags); release(c->box_cnt);
     release(c->partial); release(c->info); release(c->tf_buf);
     release(c->fb_col); release(c->fb_depth); release(c->fb_t); release(c->fb_id); release(c->fb_rgba8);
@@ -468,6 +468,8 @@ static int64_t* option_slot(mmhrt_ctx c, const char* key) {
     if (k == "sample_lanes") return &c->opt_sample_lanes;
     if (k == "build_ls") return &c->opt_build_ls;
     if (k == "sort_ri") return &c->opt_sort_ri;
+    if (k == "build_grid") return &c->opt_build_grid;
+    if (k == "sort_onesweep") return &c->opt_sort_onesweep;
     if (k == "stack_overflow") return &c->stack_overflow;
     return nullptr;
 }
@@ -482,6 +484,16 @@ int32_t mmhrt_set_option(mmhrt_ctx c, const char* key, int64_t value) {
 
 int32_t mmhrt_get_option(mmhrt_ctx c, const char* key, int64_t* value) {
     if (!c || !value) return MMHRT_E_INVALID;
+    if (key && std::string(key) == "sort_fault") {  // read from the device after the queued work
+        uint32_t f = 0;
+        if (c->sort_fault.p) {
+            hipError_t e = hipMemcpyAsync(&f, c->sort_fault.p, 4, hipMemcpyDeviceToHost, c->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+            if (e != hipSuccess) return hip_fail(c, e, "get_option sort_fault");
+        }
+        *value = f;
+        return MMHRT_OK;
+    }
     int64_t* s = option_slot(c, key);
     if (!s) return fail(c, MMHRT_E_INVALID, std::string("get_option: unknown key ") + (key ? key : "(null)"));
     *value = *s;

@@ -25,6 +25,17 @@ struct __align__(16) Node {
     int4 d;    // child0, child1, -1, height
 };
 
+// 4-wide record of node X (same index as X): the boxes and references of X's grandchildren
+// (a leaf child takes one slot itself, its pair slot is empty), X's own children and height.
+// Written top-down for the nodes at even depth; the traversal of high-spp frames starts at the
+// root's record and descends two levels per visit.
+constexpr int32_t REC4_EMPTY = (int32_t)0x80000000;
+struct __align__(16) Rec4 {
+    float4 b[6];  // slot k: lo = f[6k..6k+2], hi = f[6k+3..6k+5]
+    int4 ref;     // slot references: node index, ~leaf, or REC4_EMPTY
+    int4 meta;    // X's children (node index or ~leaf), height, 0
+};
+
 // Scene facts computed on the device by the bounds reduction.
 struct SceneInfo {
     float clo[3], chi[3];  // centre bounds
@@ -74,10 +85,15 @@ struct mmhrt_ctx_s {
     mmhrt::DevBuf<uint64_t> kla, klb;  // 64-bit key ping-pong
     mmhrt::DevBuf<uint32_t> va, vb;  // values ping-pong
     mmhrt::DevBuf<uint32_t> hist, scan_tmp;
+    mmhrt::DevBuf<uint32_t> sort_fault;  // set by a single-pass radix pass whose look-back wait ran out
     mmhrt::DevBuf<float4> leaf_sph;
     mmhrt::DevBuf<uint32_t> leaf_id;
     mmhrt::DevBuf<mmhrt::Node> nodes;
     mmhrt::DevBuf<uint32_t> flags;
+    mmhrt::DevBuf<mmhrt::Rec4> rec4;  // 4-wide records (valid while rec4_valid)
+    bool rec4_valid = false;
+    mmhrt::DevBuf<uint32_t> rec4_list;  // collapse frontiers (two halves)
+    mmhrt::DevBuf<uint32_t> rec4_cnt;
     mmhrt::DevBuf<mmhrt::Node> meet;     // per split position: the records of its two children (build scratch)
     mmhrt::DevBuf<int2> meet_far;  // per split position: the children's far range ends
     mmhrt::DevBuf<uint32_t> box_cnt;  // worklist sizes of the bottom-up box rounds
@@ -110,6 +126,9 @@ struct mmhrt_ctx_s {
     int64_t opt_sample_lanes = 1;   // sample-parallel traversal for power-of-two spp (Lambert/AO)
     int64_t opt_build_ls = 1;       // level-synchronous LBVH chunk pass (0: one climbing thread per leaf)
     int64_t opt_sort_ri = 32;       // radix sort keys per thread per tile: 8, 16 or 32 (32 for 32-bit keys only)
+    int64_t opt_sort_onesweep = 0;  // single-pass radix passes with decoupled look-back (0: histogram + scans + scatter)
+    int64_t opt_build_grid = 0;     // test hook: cap on the persistent chunk workgroups (0 = as many as fit)
+    int64_t opt_bvh4_spp = 8;       // 4-wide traversal for frames of at least this many spp (0: never)
     int64_t refits_since_build = 0;
     int64_t stack_overflow = 0;  // set by mmhrt_get_counters from the trace kernel's flag
 

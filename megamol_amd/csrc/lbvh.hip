@@ -7,10 +7,10 @@
 //   2. Morton codes, 30-bit (10 per axis) or 63-bit (21 per axis), over one uniform scale
 //   3. LSD radix sort of (key, particle index), 8-bit digits, stable: a tile's keys are ranked
 //      with wavefront ballots (8 ballots give each lane the mask of same-digit lanes)
-//   4. gather spheres into Morton order (leaf k), keep leaf -> particle index
-//   5. hierarchy and boxes together, bottom-up: one thread per leaf climbs, the second arrival
-//      at a parent merges and writes the whole node (Karras 2012 numbering and index tie-break
-//      for equal keys, Apetrei 2014 parent rule; LDS meeting places inside 1024-leaf chunks)
+//   4. hierarchy and boxes together, bottom-up, with the gather of the spheres into Morton order
+//      (leaf k) on the way: the second arrival at a parent merges and writes the whole node
+//      (Karras 2012 numbering and index tie-break for equal keys, Apetrei 2014 parent rule; LDS
+//      meeting places inside 1024-leaf chunks)
 #include <algorithm>
 
 #include "ctx.hpp"
@@ -177,11 +177,16 @@ __global__ __launch_bounds__(RB) void k_radix_hist(const K* __restrict__ keys, u
         d[r] = i < n ? (uint32_t)((keys[i] >> shift) & 255u) : 256u;
     }
     __syncthreads();
-    // plain LDS atomics: the LDS unit serialises same-address lanes itself, which is cheaper
-    // here than matching digits across the wave with ballots
+    // LDS atomics; the lanes whose digit equals lane 0's add as one (Morton digits of
+    // neighbouring keys are often equal, and the LDS unit serialises same-address lanes)
 #pragma unroll
-    for (int r = 0; r < RI; ++r)
-        if (d[r] < 256u) atomicAdd(&h[d[r]], 1u);
+    for (int r = 0; r < RI; ++r) {
+        const uint32_t d0 = __builtin_amdgcn_readfirstlane(d[r]);
+        const bool eq = d[r] == d0;
+        const uint64_t m = __ballot(eq);
+        if (__lane_id() == 0 && d0 < 256u) atomicAdd(&h[d0], (uint32_t)__popcll(m));
+        if (!eq && d[r] < 256u) atomicAdd(&h[d[r]], 1u);
+    }
     __syncthreads();
     hist[(size_t)blockIdx.x * 256 + threadIdx.x] = h[threadIdx.x];
 }
@@ -332,6 +337,194 @@ __global__ __launch_bounds__(256) void k_col_offsets(const uint32_t* __restrict_
     for (int k = 0; k < CG; ++k) {
         if (t0 + k < tiles) offs[(size_t)(t0 + k) * 256 + threadIdx.x] = run;
         run += v[k];
+    }
+}
+
+// ---------------------------------------------------------------- single-pass radix passes
+// The alternative to histogram + column scans + scatter per pass (option sort_onesweep): the
+// digit counts of every pass over all keys come from one read (they do not depend on the key
+// order), and each pass is then one kernel. Tiles take a ticket in launch order, rank their keys
+// as the scatter above does, publish their per-digit counts, and find their global offsets by
+// looking back over the earlier tiles' published counts until one that already holds its
+// inclusive prefix (decoupled look-back, Merrill & Garland 2016; one pass per digit as in
+// Adinets & Merrill 2022, "Onesweep"). A status word is flag (2 bits) | count (30 bits), written
+// and read with device-scope atomics, so no fence is needed. Every tile that a tile waits for
+// took its ticket earlier and is running; the wait is still bounded, and on exhaustion the
+// kernel records a fault and goes on (a wrong order, never a hang).
+constexpr uint32_t OS_AGG = 1u << 30, OS_INC = 2u << 30, OS_MASK = (1u << 30) - 1u;
+constexpr int OS_SPIN = 1 << 20;
+
+// Adds digit d of every active lane to the LDS histogram h; lanes with the first active lane's
+// digit add as one.
+__device__ __forceinline__ void hist_add(uint32_t* h, uint32_t d) {
+    const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+    const bool eq = d == d0;
+    const uint64_t m = __ballot(eq);
+    const uint64_t act = __ballot(true);
+    if (__lane_id() == (uint32_t)(__ffsll((unsigned long long)act) - 1)) atomicAdd(&h[d0], (uint32_t)__popcll(m));
+    if (!eq) atomicAdd(&h[d], 1u);
+}
+
+// ghist[p][d]: keys whose digit p is d, for p < passes (ghist zero on entry).
+template <typename K>
+__global__ __launch_bounds__(256) void k_radix_ghist(const K* __restrict__ keys, uint64_t n, int passes,
+                                                     uint32_t* __restrict__ ghist) {
+    __shared__ uint32_t h[8][256];
+    for (int p = 0; p < 8; ++p) h[p][threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const K k = keys[i];
+        for (int p = 0; p < passes; ++p) hist_add(h[p], (uint32_t)((k >> (8 * p)) & 255u));
+    }
+    __syncthreads();
+    for (int p = 0; p < passes; ++p) {
+        const uint32_t v = h[p][threadIdx.x];
+        if (v) atomicAdd(&ghist[p * 256 + threadIdx.x], v);
+    }
+}
+
+// dstart[p][d]: exclusive scan of ghist[p] over the digits (one block, thread = digit).
+__global__ __launch_bounds__(256) void k_radix_gscan(const uint32_t* __restrict__ ghist, int passes,
+                                                     uint32_t* __restrict__ dstart) {
+    __shared__ uint32_t wsum[4];
+    const uint32_t d = threadIdx.x, lane = __lane_id(), wave = threadIdx.x >> 6;
+    for (int p = 0; p < passes; ++p) {
+        const uint32_t c = ghist[p * 256 + d];
+        uint32_t inc = c;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            uint32_t y = __shfl_up(inc, o, 64);
+            if (lane >= (uint32_t)o) inc += y;
+        }
+        if (lane == 63) wsum[wave] = inc;
+        __syncthreads();
+        uint32_t base = 0;
+        for (uint32_t w = 0; w < wave; ++w) base += wsum[w];
+        dstart[p * 256 + d] = base + inc - c;
+        __syncthreads();
+    }
+}
+
+template <typename K, int RI>
+__global__ __launch_bounds__(RB) void k_radix_onesweep(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                       K* __restrict__ kout, uint32_t* __restrict__ vout, uint64_t n,
+                                                       int shift, const uint32_t* __restrict__ dstart,
+                                                       uint32_t* status, uint32_t* ticket, uint32_t* fault) {
+    constexpr int RT = RB * RI, RW = RT / (RB / 64);
+    __shared__ K s_key[RT];
+    __shared__ uint32_t s_val[RT];
+    __shared__ uint32_t wc[RB / 64][256];  // per-wave running digit counts, then per-wave offsets
+    __shared__ uint32_t goff[256];         // global offset of the digit's run minus its local start
+    __shared__ uint32_t wsum[RB / 64];
+    __shared__ uint32_t s_tile, s_bad;
+    const int wave = threadIdx.x >> 6;
+    const uint32_t lane = __lane_id();
+    if (threadIdx.x == 0) {
+        s_tile = atomicAdd(ticket, 1u);
+        s_bad = 0;
+    }
+#pragma unroll
+    for (int w = 0; w < RB / 64; ++w) wc[w][threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint64_t tbase = (uint64_t)tile * RT;
+    const uint64_t base = tbase + (uint64_t)wave * RW + lane;
+    K key[RI];
+    uint32_t val[RI];
+#pragma unroll
+    for (int r = 0; r < RI; ++r) {
+        uint64_t i = base + (uint64_t)r * 64;
+        bool valid = i < n;
+        key[r] = valid ? kin[i] : (K)0;
+        val[r] = valid ? vin[i] : 0u;
+    }
+    uint32_t rank[RI];
+#pragma unroll
+    for (int r = 0; r < RI; ++r) {
+        bool valid = base + (uint64_t)r * 64 < n;
+        uint32_t d = (uint32_t)((key[r] >> shift) & 255u);
+        uint64_t peers = match_digit(d, valid);
+        uint32_t before = valid ? wc[wave][d] : 0u;
+        rank[r] = before + (uint32_t)__popcll(peers & lanemask_lt());
+        if (valid && (__ffsll((unsigned long long)peers) - 1) == (int)lane) wc[wave][d] = before + (uint32_t)__popcll(peers);
+    }
+    __syncthreads();
+    // thread = digit: the tile's count, published at once; tile-local start of the digit; each
+    // wave's offset within the digit's run
+    const uint32_t dg = threadIdx.x;
+    uint32_t cw[RB / 64];
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int w = 0; w < RB / 64; ++w) {
+        cw[w] = wc[w][dg];
+        cnt += cw[w];
+    }
+    uint32_t* st = status + (size_t)tile * 256 + dg;
+    __hip_atomic_store(st, (tile == 0 ? OS_INC : OS_AGG) | cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t inc = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(inc, o, 64);
+        if (lane >= (uint32_t)o) inc += y;
+    }
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    uint32_t wb = 0;
+    for (int w = 0; w < wave; ++w) wb += wsum[w];
+    const uint32_t lstart = wb + inc - cnt;
+    {
+        uint32_t acc = lstart;
+#pragma unroll
+        for (int w = 0; w < RB / 64; ++w) {
+            wc[w][dg] = acc;
+            acc += cw[w];
+        }
+    }
+    // look back: keys with digit dg in all earlier tiles
+    uint32_t excl = 0;
+    if (tile > 0) {
+        for (int64_t t = (int64_t)tile - 1; t >= 0; --t) {
+            const uint32_t* pt = status + (size_t)t * 256 + dg;
+            uint32_t v = __hip_atomic_load(pt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int spin = 0;
+            while ((v >> 30) == 0u) {
+                if (++spin > OS_SPIN) {
+                    *fault = 1u;
+                    s_bad = 1u;
+                    v = OS_INC;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                v = __hip_atomic_load(pt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            excl += v & OS_MASK;
+            if ((v >> 30) == 2u) break;
+        }
+        __hip_atomic_store(st, OS_INC | (excl + cnt), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    goff[dg] = dstart[dg] + excl - lstart;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RI; ++r) {
+        if (base + (uint64_t)r * 64 < n) {
+            uint32_t d = (uint32_t)((key[r] >> shift) & 255u);
+            uint32_t pos = wc[wave][d] + rank[r];
+            s_key[pos] = key[r];
+            s_val[pos] = val[r];
+        }
+    }
+    __syncthreads();
+    if (s_bad) return;  // offsets unknown: write nothing (the fault flag reports it)
+    const uint32_t nvalid = (uint32_t)((n - tbase) < (uint64_t)RT ? (n - tbase) : (uint64_t)RT);
+    for (uint32_t i = threadIdx.x; i < nvalid; i += RB) {
+        K k = s_key[i];
+        uint32_t d = (uint32_t)((k >> shift) & 255u);
+        uint32_t g = goff[d] + i;
+        if (g < n) {
+            kout[g] = k;
+            vout[g] = s_val[i];
+        }
     }
 }
 
@@ -551,11 +744,22 @@ __global__ __launch_bounds__(BC) void k_bu_chunk(const K* __restrict__ keys, con
 // writes their nodes and delivers them to their parents. LDS records are chunk-relative 16-bit
 // (a reference is the leaf or node index minus the chunk start, bit 15 marking a leaf).
 // flags[] must be zero on entry; n >= 2.
+// The chunk pass, level-synchronous and persistent: workgroup g takes chunks g, g + G, g + 2G, ...
+// (G = the grid) and also gathers the leaf spheres into Morton order (leaf_sph[k] = sph[ids[k]],
+// and leaf_id[k] = ids[k] when write_ids). Round 0 delivers the leaves, each later round merges
+// the splits that the previous round completed; completed splits are kept in compact LDS lists,
+// so the active items stay packed in the first waves. Rounds touch LDS only: deliveries to the
+// two boundary splits of a chunk write their global meeting record at once but take their
+// arrival count after the rounds, together with the in-chunk splits left with one arrival. The
+// loads of the next chunk's spheres and keys (and of the ids one chunk further) are issued just
+// before those arrival counts, so that the one wait for global memory per chunk covers both.
+// flags[] must be zero on entry; n >= 2.
 template <typename K, int BC>
-__global__ __launch_bounds__(BC) void k_bu_chunk_ls(const K* __restrict__ keys, const float4* __restrict__ leaf_sph,
-                                                    int64_t n64, Node* __restrict__ nodes, Node* meet,
-                                                    int2* meet_far, uint32_t* flags, uint32_t* list, uint32_t* cnt,
-                                                    SceneInfo* info) {
+__global__ __launch_bounds__(BC) void k_bu_chunk_ls(const K* __restrict__ keys, const uint32_t* __restrict__ ids,
+                                                    const float4* __restrict__ sph, float4* __restrict__ leaf_sph,
+                                                    uint32_t* __restrict__ leaf_id, int write_ids, int64_t n64,
+                                                    Node* __restrict__ nodes, Node* meet, int2* meet_far,
+                                                    uint32_t* flags, uint32_t* list, uint32_t* cnt, SceneInfo* info) {
     __shared__ float s_box[BC][12];     // the two child boxes parked at each in-chunk split
     __shared__ uint16_t s_ref[BC][2];   // chunk-relative child references
     __shared__ uint16_t s_far[BC][2];   // chunk-relative far range ends
@@ -564,113 +768,180 @@ __global__ __launch_bounds__(BC) void k_bu_chunk_ls(const K* __restrict__ keys, 
     __shared__ uint8_t s_side[BC];
     __shared__ uint32_t s_cnt[BC];
     __shared__ uint16_t s_lst[2][BC];   // splits completed in the previous / current round
-    __shared__ uint32_t s_nl[2];
+    __shared__ uint32_t s_nl[3];        // list sizes: read in round r, appended in r + 1, cleared in r + 2
+    __shared__ int32_t s_pend[2];       // boundary splits delivered to in this chunk (left, right)
     __shared__ uint32_t s_list[256];
     __shared__ uint32_t s_n, s_base;
     SplitList ap{s_list, &s_n, &s_base};
     const int32_t n = (int32_t)n64;
-    const int32_t cs = (int32_t)blockIdx.x * BC;
-    const int32_t ce = cs + BC < n ? cs + BC : n;
+    const int32_t nchunks = (n + BC - 1) / BC;
     const int32_t tid = (int32_t)threadIdx.x;
-    const int32_t k = cs + tid;
-    const float4 sph = k < ce ? leaf_sph[k] : make_float4(0.f, 0.f, 0.f, 0.f);
-    s_cnt[tid] = 0;
-    for (int j = tid; j <= BC; j += BC) s_delta[j] = (int8_t)split_delta(keys, n64, (int64_t)cs - 1 + j);
-    if (tid == 0) {
-        s_n = 0;
-        s_nl[0] = 0;
-        s_nl[1] = 0;
-    }
-    __syncthreads();
-    // deliver node X = [l, r] (box b, height h, reference ref, its children's boxes f and
-    // references kd when internal) to its parent; returns nothing, may push the parent split
-    auto deliver = [&](int32_t l, int32_t r, const float b[6], int h, bool internal, const float f[12], int2 kd,
-                       int32_t leaf, uint32_t* nl_next, uint16_t* lst_next) {
-        if (l == 0 && r == n - 1) {  // the root
-            store_node(nodes, f, kd, h);
-            info->height = (uint32_t)h;
-            return;
+    const int32_t G = (int32_t)gridDim.x;
+    if (tid == 0) s_n = 0;
+    // prefetch pipeline: ids of chunk + 2G, spheres and keys of chunk + G. Every load is
+    // unconditional, at an index clamped into [0, n - 1]; validity is applied where the values
+    // are used, so that no load result merges with a constant at a branch join.
+    auto clampi = [&](int64_t i) -> int64_t { return i < 0 ? 0 : (i > n64 - 1 ? n64 - 1 : i); };
+    auto load_id = [&](int32_t ch) -> uint32_t { return ids[clampi((int64_t)ch * BC + tid)]; };
+    // keys i, i + 1 and i + 2 for i = cs - 1 + tid: delta(i), and delta(i + 1) for the last
+    // thread (lane-dependent addresses keep these vector loads)
+    auto load_keys = [&](int32_t ch, K& ka, K& kb, K& kc) {
+        const int64_t i = (int64_t)ch * BC - 1 + tid;
+        ka = keys[clampi(i)];
+        kb = keys[clampi(i + 1)];
+        kc = keys[clampi(i + 2)];
+    };
+    int32_t ch = (int32_t)blockIdx.x;
+    uint32_t id_cur = load_id(ch);
+    float4 sp_cur = sph[id_cur];
+    K ka_cur, kb_cur, kc_cur;
+    load_keys(ch, ka_cur, kb_cur, kc_cur);
+    uint32_t id_next = load_id(ch + G);
+    for (; ch < nchunks; ch += G) {
+        const int32_t cs = ch * BC;
+        const int32_t ce = cs + BC < n ? cs + BC : n;
+        const int32_t k = cs + tid;
+        if (k < ce) {
+            leaf_sph[k] = sp_cur;
+            if (write_ids) leaf_id[k] = id_cur;
         }
-        const int dl = s_delta[l - cs], dr = s_delta[r + 1 - cs];  // l - 1 >= cs - 1, r <= ce - 1
-        const bool left = dr > dl;
-        const int32_t sp = left ? r : l - 1;
-        int32_t ref;
-        uint16_t rrel;
-        if (internal) {
-            const int32_t idx = left ? r : l;
-            store_node(nodes + idx, f, kd, h);
-            ref = idx;
-            rrel = (uint16_t)(idx - cs);
-        } else {
-            ref = ~leaf;
-            rrel = (uint16_t)(0x8000u | (uint32_t)(leaf - cs));
+        // chunk set-up: delta(cs - 1 + j) for j in [0, BC] from the keys in registers
+        {
+            // delta(i) of keys a = key i, b = key i + 1 (-1 past either end), as split_delta
+            auto dl = [&](int32_t i, K a, K b) -> int {
+                if (i < 0 || i + 1 >= n) return -1;
+                if (a == b) return (int)(8 * sizeof(K)) + __clz((uint32_t)i ^ (uint32_t)(i + 1));
+                if constexpr (sizeof(K) == 8)
+                    return __clzll((long long)(a ^ b));
+                else
+                    return __clz((int)(a ^ b));
+            };
+            const int32_t i = cs - 1 + tid;
+            s_delta[tid] = (int8_t)dl(i, ka_cur, kb_cur);
+            if (tid == BC - 1) s_delta[BC] = (int8_t)dl(i + 1, kb_cur, kc_cur);
+            if (tid == 0) {
+                s_nl[0] = 0;
+                s_nl[1] = 0;
+                s_nl[2] = 0;
+                s_pend[0] = -1;
+                s_pend[1] = -1;
+            }
         }
-        const int side = left ? 0 : 1;
-        const int32_t far = left ? l : r;
-        if (sp < cs || sp + 1 >= ce) {  // the split lies on a chunk boundary
-            meet_put(meet + sp, meet_far + sp, side, b, ref, h, far);
+        s_cnt[tid] = 0;
+        __syncthreads();
+        // deliver node X = [l, r] (box b, height h; its children's boxes f and references kd
+        // when internal) to its parent
+        auto deliver = [&](int32_t l, int32_t r, const float b[6], int h, bool internal, const float f[12], int2 kd,
+                           int32_t leaf, uint32_t* nl_next, uint16_t* lst_next) {
+            if (l == 0 && r == n - 1) {  // the root
+                store_node(nodes, f, kd, h);
+                info->height = (uint32_t)h;
+                return;
+            }
+            const int dl = s_delta[l - cs], dr = s_delta[r + 1 - cs];  // l - 1 >= cs - 1, r <= ce - 1
+            const bool left = dr > dl;
+            const int32_t sp = left ? r : l - 1;
+            int32_t ref;
+            uint16_t rrel;
+            if (internal) {
+                const int32_t idx = left ? r : l;
+                store_node(nodes + idx, f, kd, h);
+                ref = idx;
+                rrel = (uint16_t)(idx - cs);
+            } else {
+                ref = ~leaf;
+                rrel = (uint16_t)(0x8000u | (uint32_t)(leaf - cs));
+            }
+            const int side = left ? 0 : 1;
+            const int32_t far = left ? l : r;
+            if (sp < cs || sp + 1 >= ce) {  // a boundary split: sp = cs - 1 or ce - 1
+                meet_put(meet + sp, meet_far + sp, side, b, ref, h, far);
+                s_pend[sp < cs ? 0 : 1] = sp;
+                return;
+            }
+            const int slot = sp - cs;
+#pragma unroll
+            for (int a = 0; a < 6; ++a) s_box[slot][side * 6 + a] = b[a];
+            s_ref[slot][side] = rrel;
+            s_far[slot][side] = (uint16_t)(far - cs);
+            s_h[slot][side] = (uint8_t)h;
+            s_side[slot] = (uint8_t)side;
+            // the merge happens in the next round, after a barrier: a relaxed count suffices
+            uint32_t old = __hip_atomic_fetch_add(&s_cnt[slot], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (old == 1u) lst_next[atomicAdd(nl_next, 1u)] = (uint16_t)slot;
+        };
+        auto gref = [&](uint16_t rr) -> int32_t {
+            return (rr & 0x8000u) ? ~(cs + (int32_t)(rr & 0x7fffu)) : cs + (int32_t)rr;
+        };
+        // the leaves
+        if (k < ce) {
+            float b[6];
+            leaf_box(sp_cur, b);
+            float f0[12];
+            deliver(k, k, b, 0, false, f0, make_int2(0, 0), k, &s_nl[0], s_lst[0]);
+        }
+        __syncthreads();
+        for (int rd = 0;; ++rd) {
+            const int c0 = rd % 3;
+            const uint32_t count = s_nl[c0];
+            if (count == 0) break;
+            if (tid == 0) s_nl[(rd + 2) % 3] = 0;
+            uint32_t* nl_next = &s_nl[(rd + 1) % 3];
+            const uint16_t* lst = s_lst[rd & 1];
+            uint16_t* lst_next = s_lst[(rd + 1) & 1];
+            for (uint32_t i = (uint32_t)tid; i < count; i += BC) {
+                const int slot = lst[i];
+                float f[12];
+#pragma unroll
+                for (int a = 0; a < 12; ++a) f[a] = s_box[slot][a];
+                const int2 kd = make_int2(gref(s_ref[slot][0]), gref(s_ref[slot][1]));
+                const int h0 = s_h[slot][0], h1 = s_h[slot][1];
+                const int h = 1 + (h0 > h1 ? h0 : h1);
+                const int32_t l = cs + (int32_t)s_far[slot][0], r = cs + (int32_t)s_far[slot][1];
+                float b[6];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    b[a] = fminf(f[a], f[6 + a]);
+                    b[3 + a] = fmaxf(f[3 + a], f[9 + a]);
+                }
+                deliver(l, r, b, h, true, f, kd, 0, nl_next, lst_next);
+            }
+            __syncthreads();
+        }
+        // next chunk's loads (its ids are already here; the ids after it start now), issued before
+        // the arrival counts below so that one wait covers both
+        const float4 sp_next = sph[id_next];
+        K ka_next, kb_next, kc_next;
+        load_keys(ch + G, ka_next, kb_next, kc_next);
+        const uint32_t id_next2 = load_id(ch + 2 * G);
+        // arrival counts at the global meeting places: the boundary splits delivered to, and the
+        // in-chunk splits with one arrival (their other child straddles the chunk), whose parked
+        // child moves to the global record first
+        if (tid < 2 && s_pend[tid] >= 0) {
+            const int32_t sp = s_pend[tid];
             if (atomicAdd(flags + sp, 1u) == 1u) ap.push((uint32_t)sp, list, cnt + 1);
-            return;
         }
-        const int slot = sp - cs;
-#pragma unroll
-        for (int a = 0; a < 6; ++a) s_box[slot][side * 6 + a] = b[a];
-        s_ref[slot][side] = rrel;
-        s_far[slot][side] = (uint16_t)(far - cs);
-        s_h[slot][side] = (uint8_t)h;
-        s_side[slot] = (uint8_t)side;
-        uint32_t old = __hip_atomic_fetch_add(&s_cnt[slot], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-        if (old == 1u) lst_next[atomicAdd(nl_next, 1u)] = (uint16_t)slot;
-    };
-    auto gref = [&](uint16_t rr) -> int32_t {
-        return (rr & 0x8000u) ? ~(cs + (int32_t)(rr & 0x7fffu)) : cs + (int32_t)rr;
-    };
-    // round 0: the leaves
-    if (k < ce) {
-        float b[6];
-        leaf_box(sph, b);
-        float f0[12];
-        deliver(k, k, b, 0, false, f0, make_int2(0, 0), k, &s_nl[0], s_lst[0]);
-    }
-    __syncthreads();
-    for (int rd = 0;; ++rd) {
-        const int cur = rd & 1, nxt = cur ^ 1;
-        const uint32_t count = s_nl[cur];
-        if (count == 0) break;
-        for (uint32_t i = (uint32_t)tid; i < count; i += BC) {
-            const int slot = s_lst[cur][i];
-            float f[12];
-#pragma unroll
-            for (int a = 0; a < 12; ++a) f[a] = s_box[slot][a];
-            const int2 kd = make_int2(gref(s_ref[slot][0]), gref(s_ref[slot][1]));
-            const int h0 = s_h[slot][0], h1 = s_h[slot][1];
-            const int h = 1 + (h0 > h1 ? h0 : h1);
-            const int32_t l = cs + (int32_t)s_far[slot][0], r = cs + (int32_t)s_far[slot][1];
+        const int nsplit = ce - cs - 1;
+        for (int slot = tid; slot < nsplit; slot += BC) {
+            if (s_cnt[slot] != 1u) continue;
+            const int side = s_side[slot];
+            const int32_t sp = cs + slot;
             float b[6];
 #pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                b[a] = fminf(f[a], f[6 + a]);
-                b[3 + a] = fmaxf(f[3 + a], f[9 + a]);
-            }
-            deliver(l, r, b, h, true, f, kd, 0, &s_nl[nxt], s_lst[nxt]);
+            for (int a = 0; a < 6; ++a) b[a] = s_box[slot][side * 6 + a];
+            meet_put(meet + sp, meet_far + sp, side, b, gref(s_ref[slot][side]), s_h[slot][side],
+                     cs + (int32_t)s_far[slot][side]);
+            if (atomicAdd(flags + sp, 1u) == 1u) ap.push((uint32_t)sp, list, cnt + 1);
         }
         __syncthreads();
-        if (tid == 0) s_nl[cur] = 0;
-        __syncthreads();
-    }
-    // in-chunk splits with one arrival: the other child straddles the chunk, so the parked
-    // child moves to the global meeting place
-    const int nsplit = ce - cs - 1;
-    for (int slot = tid; slot < nsplit; slot += BC) {
-        if (s_cnt[slot] != 1u) continue;
-        const int side = s_side[slot];
-        const int32_t sp = cs + slot;
-        float b[6];
-#pragma unroll
-        for (int a = 0; a < 6; ++a) b[a] = s_box[slot][side * 6 + a];
-        meet_put(meet + sp, meet_far + sp, side, b, gref(s_ref[slot][side]), s_h[slot][side],
-                 cs + (int32_t)s_far[slot][side]);
-        if (atomicAdd(flags + sp, 1u) == 1u) ap.push((uint32_t)sp, list, cnt + 1);
+        if (s_n > 192u) ap.flush(list, cnt + 1);  // s_n is read by every thread after the barrier
+        // rotate the pipeline
+        id_cur = id_next;
+        id_next = id_next2;
+        sp_cur = sp_next;
+        ka_cur = ka_next;
+        kb_cur = kb_next;
+        kc_cur = kc_next;
     }
     ap.flush(list, cnt + 1);
 }
@@ -833,6 +1104,32 @@ int sort_pairs_t(mmhrt_ctx_s* c, K* ka, K* kb, uint32_t* va, uint32_t* vb, uint6
     K* dst = kb;
     uint32_t* vs = va;
     uint32_t* vd = vb;
+    if (c->opt_sort_onesweep != 0 && n < (1ull << 30)) {
+        // single-pass passes: status words per pass and tile, digit counts of all passes,
+        // tickets and the fault flag
+        if (int e = ensure(c, c->hist, hlen * (uint64_t)passes, "radix tile status")) return e;
+        if (int e = ensure(c, c->scan_tmp, (size_t)passes * 512 + 16, "radix digit counts")) return e;
+        if (!c->sort_fault.p) {
+            if (int e = ensure(c, c->sort_fault, 1, "radix fault flag")) return e;
+            launch_fill_u32(c->stream, c->sort_fault.p, 0u, 1);
+        }
+        uint32_t* ghist = c->scan_tmp.p;
+        uint32_t* dstart = ghist + passes * 256;
+        uint32_t* tickets = dstart + passes * 256;
+        launch_fill_u32(c->stream, c->hist.p, 0u, hlen * (uint64_t)passes);
+        launch_fill_u32(c->stream, ghist, 0u, (uint64_t)passes * 256);
+        launch_fill_u32(c->stream, tickets, 0u, (uint64_t)passes);
+        unsigned gb = (unsigned)std::min<uint64_t>(1024, (n + 255) / 256);
+        hipLaunchKernelGGL(k_radix_ghist<K>, dim3(gb), dim3(256), 0, c->stream, src, n, passes, ghist);
+        hipLaunchKernelGGL(k_radix_gscan, dim3(1), dim3(256), 0, c->stream, ghist, passes, dstart);
+        for (int p = 0; p < passes; ++p) {
+            hipLaunchKernelGGL((k_radix_onesweep<K, RI>), dim3(tiles), dim3(RB), 0, c->stream, src, vs, dst, vd, n, 8 * p,
+                               dstart + p * 256, c->hist.p + hlen * p, tickets + p, c->sort_fault.p);
+            std::swap(src, dst);
+            std::swap(vs, vd);
+        }
+        return 0;
+    }
     for (int p = 0; p < passes; ++p) {
         int shift = 8 * p;
         hipLaunchKernelGGL((k_radix_hist<K, RI>), dim3(tiles), dim3(RB), 0, c->stream, src, n, shift, hist);
@@ -859,12 +1156,41 @@ int sort_pairs(mmhrt_ctx_s* c, K* ka, K* kb, uint32_t* va, uint32_t* vb, uint64_
     return sort_pairs_t<K, 16>(c, ka, kb, va, vb, n, passes);
 }
 
-// Hierarchy and boxes over the sorted keys and gathered leaf spheres (also the refit: the
-// same keys give the same tree, the boxes come from the current spheres).
+// Persistent grid of the chunk pass: as many workgroups as fit on the device at once.
+template <typename K, int BC>
+unsigned chunk_grid(mmhrt_ctx_s* c, uint64_t n) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_bu_chunk_ls<K, BC>, BC, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    const uint64_t chunks = (n + BC - 1) / BC;
+    uint64_t g = (uint64_t)per_cu * (uint64_t)(c->cu_count > 0 ? c->cu_count : 256);
+    if (c->opt_build_grid > 0 && (uint64_t)c->opt_build_grid < g) g = (uint64_t)c->opt_build_grid;
+    return (unsigned)(chunks < g ? chunks : g);
+}
+
+template <typename K, int BC>
+void launch_chunk_ls(mmhrt_ctx_s* c, const K* keys, const uint32_t* ids, bool write_ids, uint32_t* list) {
+    const uint64_t n = c->n;
+    hipLaunchKernelGGL((k_bu_chunk_ls<K, BC>), dim3(chunk_grid<K, BC>(c, n)), dim3(BC), 0, c->stream, keys, ids,
+                       cur_spheres(c), c->leaf_sph.p, c->leaf_id.p, write_ids ? 1 : 0, (int64_t)n, c->nodes.p,
+                       c->meet.p, c->meet_far.p, c->flags.p, list, c->box_cnt.p, c->info.p);
+}
+
+// Hierarchy and boxes over the sorted keys, gathering the leaf spheres on the way: ids[k] is
+// the particle of leaf k (the sorted values after a build, leaf_id on a refit, which keeps the
+// tree of the same keys and takes the boxes of the current spheres).
 template <typename K>
-int hierarchy(mmhrt_ctx_s* c, const K* keys) {
+int hierarchy(mmhrt_ctx_s* c, const K* keys, const uint32_t* ids, bool write_ids) {
     uint64_t n = c->n;
     hipStream_t s = c->stream;
+    if (n == 1 || c->opt_build_ls == 0) {
+        if (write_ids)
+            hipLaunchKernelGGL(k_gather, dim3(blocks_for(n, 256)), dim3(256), 0, s, ids, cur_spheres(c),
+                               c->leaf_sph.p, c->leaf_id.p, n);
+        else
+            hipLaunchKernelGGL(k_regather, dim3(blocks_for(n, 256)), dim3(256), 0, s, cur_spheres(c), ids,
+                               c->leaf_sph.p, n);
+    }
     if (n == 1) {
         hipLaunchKernelGGL(k_single, dim3(1), dim3(1), 0, s, c->leaf_sph.p, c->nodes.p, c->info.p);
         return 0;
@@ -876,39 +1202,34 @@ int hierarchy(mmhrt_ctx_s* c, const K* keys) {
     if (int e = ensure(c, c->flags, m, "meeting counters")) return e;
     launch_fill_u32(s, c->box_cnt.p, 0u, BOX_ROUNDS + 4);
     launch_fill_u32(s, c->flags.p, 0u, m);
+    // worklists of the straddler rounds: the spare key buffer (free once the sort is done) and
+    // the spare value buffer
+    uint32_t* l0 = sizeof(K) == 8 ? reinterpret_cast<uint32_t*>(c->klb.p) : reinterpret_cast<uint32_t*>(c->kb.p);
+    uint32_t* l1 = c->vb.p;
     // leaves per chunk workgroup: 1024 (two per CU), 512 or 256; level-synchronous rounds unless
     // the option build_ls is 0
     const int bc = c->opt_build_chunk == 256 ? 256 : (c->opt_build_chunk == 512 ? 512 : 1024);
     if (c->opt_build_ls != 0) {
         if (bc == 256)
-            hipLaunchKernelGGL((k_bu_chunk_ls<K, 256>), dim3(blocks_for(n, 256)), dim3(256), 0, s, keys, c->leaf_sph.p,
-                               (int64_t)n, c->nodes.p, c->meet.p, c->meet_far.p, c->flags.p, c->va.p, c->box_cnt.p,
-                               c->info.p);
+            launch_chunk_ls<K, 256>(c, keys, ids, write_ids, l0);
         else if (bc == 512)
-            hipLaunchKernelGGL((k_bu_chunk_ls<K, 512>), dim3(blocks_for(n, 512)), dim3(512), 0, s, keys, c->leaf_sph.p,
-                               (int64_t)n, c->nodes.p, c->meet.p, c->meet_far.p, c->flags.p, c->va.p, c->box_cnt.p,
-                               c->info.p);
+            launch_chunk_ls<K, 512>(c, keys, ids, write_ids, l0);
         else
-            hipLaunchKernelGGL((k_bu_chunk_ls<K, 1024>), dim3(blocks_for(n, 1024)), dim3(1024), 0, s, keys,
-                               c->leaf_sph.p, (int64_t)n, c->nodes.p, c->meet.p, c->meet_far.p, c->flags.p, c->va.p,
-                               c->box_cnt.p, c->info.p);
+            launch_chunk_ls<K, 1024>(c, keys, ids, write_ids, l0);
     } else if (bc == 256)
         hipLaunchKernelGGL((k_bu_chunk<K, 256>), dim3(blocks_for(n, 256)), dim3(256), 0, s, keys, c->leaf_sph.p,
-                           (int64_t)n, c->nodes.p, c->meet.p, c->meet_far.p, c->flags.p, c->va.p, c->box_cnt.p,
-                           c->info.p);
+                           (int64_t)n, c->nodes.p, c->meet.p, c->meet_far.p, c->flags.p, l0, c->box_cnt.p, c->info.p);
     else if (bc == 512)
         hipLaunchKernelGGL((k_bu_chunk<K, 512>), dim3(blocks_for(n, 512)), dim3(512), 0, s, keys, c->leaf_sph.p,
-                           (int64_t)n, c->nodes.p, c->meet.p, c->meet_far.p, c->flags.p, c->va.p, c->box_cnt.p,
-                           c->info.p);
+                           (int64_t)n, c->nodes.p, c->meet.p, c->meet_far.p, c->flags.p, l0, c->box_cnt.p, c->info.p);
     else
         hipLaunchKernelGGL((k_bu_chunk<K, 1024>), dim3(blocks_for(n, 1024)), dim3(1024), 0, s, keys, c->leaf_sph.p,
-                           (int64_t)n, c->nodes.p, c->meet.p, c->meet_far.p, c->flags.p, c->va.p, c->box_cnt.p,
-                           c->info.p);
+                           (int64_t)n, c->nodes.p, c->meet.p, c->meet_far.p, c->flags.p, l0, c->box_cnt.p, c->info.p);
     for (int r = 0; r < BOX_ROUNDS; ++r)
         hipLaunchKernelGGL(k_bu_round<K>, dim3(1024), dim3(256), 0, s, keys, (int64_t)n, c->nodes.p, c->meet.p,
-                           c->meet_far.p, c->flags.p, c->va.p, c->vb.p, c->box_cnt.p, r, c->info.p);
+                           c->meet_far.p, c->flags.p, l0, l1, c->box_cnt.p, r, c->info.p);
     hipLaunchKernelGGL(k_bu_tail<K>, dim3(1), dim3(256), 0, s, keys, (int64_t)n, c->nodes.p, c->meet.p, c->meet_far.p,
-                       c->flags.p, c->va.p, c->vb.p, c->box_cnt.p, c->info.p);
+                       c->flags.p, l0, l1, c->box_cnt.p, c->info.p);
     return 0;
 }
 
@@ -922,10 +1243,8 @@ int build_typed(mmhrt_ctx_s* c, K* ka, K* kb) {
     int passes = sizeof(K) == 8 ? 8 : 4;
     if (int e = sort_pairs<K>(c, ka, kb, c->va.p, c->vb.p, n, passes)) return e;
     rec(c, 3);
-    hipLaunchKernelGGL(k_gather, dim3(blocks_for(n, 256)), dim3(256), 0, s, c->va.p, cur_spheres(c), c->leaf_sph.p,
-                       c->leaf_id.p, n);
-    rec(c, 4);
-    return hierarchy<K>(c, ka);
+    rec(c, 4);  // the leaf gather runs inside the hierarchy pass
+    return hierarchy<K>(c, ka, c->va.p, true);
 }
 
 }  // namespace
@@ -983,16 +1302,14 @@ int build_lbvh(mmhrt_ctx_s* c, int bits) {
 
 int refit_lbvh(mmhrt_ctx_s* c) {
     if (!c->built || c->built_n != c->n) return fail(c, MMHRT_E_STATE, "refit: no matching LBVH to refit");
-    uint64_t n = c->n;
     rec(c, 0);
     if (int e = reduce_scene(c)) return e;
     rec(c, 1);
     rec(c, 2);
     rec(c, 3);
-    hipLaunchKernelGGL(k_regather, dim3(blocks_for(n, 256)), dim3(256), 0, c->stream, cur_spheres(c), c->leaf_id.p,
-                       c->leaf_sph.p, n);
     rec(c, 4);
-    int e = c->bits == 63 ? hierarchy<uint64_t>(c, c->kla.p) : hierarchy<uint32_t>(c, c->ka.p);
+    int e = c->bits == 63 ? hierarchy<uint64_t>(c, c->kla.p, c->leaf_id.p, false)
+                          : hierarchy<uint32_t>(c, c->ka.p, c->leaf_id.p, false);
     if (e) return e;
     rec(c, 5);
     hipError_t he = hipGetLastError();

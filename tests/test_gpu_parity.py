@@ -39,17 +39,24 @@ def test_small_scene_bit_exact(gpu_renderer, seed, mode, bits):
     assert_frames_equal(g, o, aux=True, what=f"seed={seed} mode={mode} bits={bits}")
 
 
-@pytest.mark.parametrize("chunk,ls,ri", [(1024, 1, 16), (512, 1, 8), (256, 1, 32), (1024, 0, 16), (256, 0, 8)])
-def test_lbvh_equals_oracle_lbvh(gpu_renderer, chunk, ls, ri):
+@pytest.mark.parametrize("chunk,ls,ri,grid,os1", [(1024, 1, 16, 0, 0), (512, 1, 8, 0, 0), (256, 1, 32, 0, 0),
+                                                  (1024, 0, 16, 0, 0), (256, 0, 8, 0, 0), (1024, 1, 32, 3, 0),
+                                                  (256, 1, 16, 1, 0), (512, 1, 32, 7, 0), (1024, 1, 32, 0, 1),
+                                                  (1024, 1, 16, 0, 1), (512, 1, 8, 5, 1)])
+def test_lbvh_equals_oracle_lbvh(gpu_renderer, chunk, ls, ri, grid, os1):
     """Both chunk passes (level-synchronous rounds, or one climbing thread per leaf) at every
-    chunk size, over radix tiles of 8, 16 or 32 keys per thread, give the oracle's tree node
-    for node (sorted keys and leaf order included)."""
+    chunk size, over radix tiles of 8, 16 or 32 keys per thread, with either radix pass
+    (histogram + scans + scatter, or single-pass with look-back: os1), give the oracle's tree
+    node for node (sorted keys and leaf order included). grid > 0 caps the persistent chunk
+    workgroups, so that each one runs many chunks through its load pipeline."""
     gpu_renderer.set_option("build_chunk", chunk)
     gpu_renderer.set_option("build_ls", ls)
     gpu_renderer.set_option("sort_ri", ri)
+    gpu_renderer.set_option("build_grid", grid)
+    gpu_renderer.set_option("sort_onesweep", os1)
     for bits in (30, 63):
-        for n, dup in [(1000, False), (50000, True), (2, False), (3, True), (4097, False)]:
-            case(f"lbvh n={n} dup={dup} bits={bits} chunk={chunk} ls={ls} ri={ri}")
+        for n, dup in [(1000, False), (50000, True), (2, False), (3, True), (4097, False), (1_200_000, False)]:
+            case(f"lbvh n={n} dup={dup} bits={bits} chunk={chunk} ls={ls} ri={ri} grid={grid} onesweep={os1}")
             g = np.random.default_rng(n + bits)
             xyzr = np.zeros((n, 4), np.float32)
             xyzr[:, :3] = g.random((n, 3)).astype(np.float32)
@@ -66,9 +73,12 @@ def test_lbvh_equals_oracle_lbvh(gpu_renderer, chunk, ls, ri):
             np.testing.assert_array_equal(child, ochild)
             np.testing.assert_array_equal(box.view(np.uint32), obox.view(np.uint32))
             assert height[0] >= int(np.ceil(np.log2(n)))
+            assert gpu_renderer.get_option("sort_fault") == 0
+    gpu_renderer.set_option("sort_onesweep", 0)
     gpu_renderer.set_option("build_chunk", 1024)
     gpu_renderer.set_option("build_ls", 1)
-    gpu_renderer.set_option("sort_ri", 16)
+    gpu_renderer.set_option("sort_ri", 32)
+    gpu_renderer.set_option("build_grid", 0)
 
 
 def test_edge_cases(gpu_renderer):
@@ -136,7 +146,12 @@ def test_transfer_function_intensity_list(gpu_renderer):
     gpu_renderer.set_transfer_function(None)
 
 
-def test_refit_equals_rebuild(gpu_renderer):
+@pytest.mark.parametrize("grid", [0, 2])
+def test_refit_equals_rebuild(gpu_renderer, grid):
+    """A refit (same keys and tree, boxes from the moved spheres) renders what the oracle renders
+    from the moved spheres, and what a rebuild renders; grid 2 runs the refit's chunk pass
+    through two persistent workgroups."""
+    gpu_renderer.set_option("build_grid", grid)
     xyzr, rgba = scenes.random_spheres(20000, seed=4, rmin=0.005, rmax=0.02)
     cam = small_camera(80, 80)
     p = params(80, 80, spp=2, mode=L.MODE_AO, ao_distance=0.2)
@@ -145,7 +160,7 @@ def test_refit_equals_rebuild(gpu_renderer):
     g = np.random.default_rng(1)
     moved = xyzr.copy()
     for step in range(3):
-        case(f"refit step {step}")
+        case(f"refit step {step} grid {grid}")
         moved[:, :3] += g.normal(0, 0.01, (len(moved), 3)).astype(np.float32)
         gpu_renderer.set_spheres(moved, rgba)
         gpu_renderer.build(L.BUILD_REFIT, 30)
@@ -154,6 +169,7 @@ def test_refit_equals_rebuild(gpu_renderer):
         assert_frames_equal(a, o, what=f"refit step {step}")
     gpu_renderer.build(L.BUILD_REBUILD, 30)
     b = gpu_renderer.render(cam, p)
+    gpu_renderer.set_option("build_grid", 0)
     np.testing.assert_array_equal(a.hit_id, b.hit_id)
     np.testing.assert_array_equal(a.color, b.color)
 
