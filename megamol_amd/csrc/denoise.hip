@@ -36,10 +36,49 @@ typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 // round to nearest even
 __device__ __forceinline__ uint16_t f2bf(float f) {
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);  // NaN stays NaN
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
+    const uint32_t u = __float_as_uint(f);
+    const uint32_t r = (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+    const uint32_t nan = (u >> 16) | 0x40u;  // NaN stays NaN
+    return (uint16_t)(((u & 0x7fffffffu) > 0x7f800000u) ? nan : r);  // a select, no branch
+}
+
+// Epilogue of 16x16 MFMA output blocks: lane (q, lr) holds channel co = 16 nb + lr of pixels
+// x0 + 16 pb + 4q + e, e < 4; bias added, optional ReLU, bf16 or fp32 out. The values are final
+// before the first store, so no store waits for anything. (Transposing within lane quads to
+// store 4 channels at once measured slower: the shuffles and registers cost more than the
+// narrow stores.)
+template <int PB, int NB>
+__device__ __forceinline__ void store_blocks(const ConvArgs& a, const f32x4 (&acc)[PB][NB], const float (&bias)[NB],
+                                             int nb0, int NBT, int b, int y, int x0, int q, int lr) {
+    const size_t rowo = ((size_t)b * a.H + y) * a.W;
+    const bool f32out = a.Yf != nullptr;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        if (nb0 + j >= NBT) break;
+        const int co = (nb0 + j) * 16 + lr;
+#pragma unroll
+        for (int pb = 0; pb < PB; ++pb) {
+            float v[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                v[e] = acc[pb][j][e] + bias[j];
+                if (a.relu) v[e] = fmaxf(v[e], 0.0f);
+            }
+            if (f32out) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int x = x0 + pb * 16 + q * 4 + e;
+                    if (x < a.W) a.Yf[(rowo + x) * a.CO + co] = v[e];
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int x = x0 + pb * 16 + q * 4 + e;
+                    if (x < a.W) a.Y[(rowo + x) * a.CO + co] = f2bf(v[e]);
+                }
+            }
+        }
+    }
 }
 
 union Frag {
@@ -155,64 +194,64 @@ __global__ __launch_bounds__(256) void k_conv_fwd32(ConvArgs a) {
         for (int j = 0; j < NB; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
     const int q = lane >> 4;   // which 8 of the chunk's 32 k values
     const int lr = lane & 15;  // row (A) / column (B)
-    for (int kc = 0; kc < nkc; ++kc) {
-        Frag8 fa[PB];
+    float bias[NB];
 #pragma unroll
-        for (int pb = 0; pb < PB; ++pb) fa[pb].u = make_uint4(0u, 0u, 0u, 0u);
+    for (int j = 0; j < NB; ++j) bias[j] = a.bias[(nb0 + j < NBT ? nb0 + j : 0) * 16 + lr];
+    // fragments of chunk kc: A loads at clamped, always valid addresses, zeroed where the tap
+    // falls outside the image or past CI (a guarded load would be a branch and a wait per chunk)
+    auto load_a = [&](int kc, Frag8 (&fa)[PB]) {
         const int k0 = kc * 32 + q * 8;
-        if (k0 < K9) {
-            const int t = k0 / CI;
-            const int ci0 = k0 - t * CI;
-            const int yy = y + t / 3 - 1;
-            const int dx = t % 3 - 1;
-            if (yy >= 0 && yy < a.H) {
+        const bool kok = k0 < K9;
+        const int kk = kok ? k0 : 0;
+        const int t = kk / CI;
+        const int ci0 = kk - t * CI;
+        const int yy = y + t / 3 - 1;
+        const int dx = t % 3 - 1;
+        const bool yok = kok && yy >= 0 && yy < a.H;
+        const int yc = yy < 0 ? 0 : (yy >= a.H ? a.H - 1 : yy);
+        const bool two = ci0 >= a.C1;
 #pragma unroll
-                for (int pb = 0; pb < PB; ++pb) {
-                    const int xx = x0 + pb * 16 + lr + dx;
-                    if (xx >= 0 && xx < a.W) {
-                        const uint16_t* src;
-                        if (ci0 < a.C1)
-                            src = a.X1 + (((size_t)b * a.H + yy) * a.W + xx) * a.C1 + ci0;
-                        else
-                            src = a.X2 + (((size_t)b * (a.H >> 1) + (yy >> 1)) * (a.W >> 1) + (xx >> 1)) * a.C2 +
-                                  (ci0 - a.C1);
-                        fa[pb].u = *reinterpret_cast<const uint4*>(src);
-                    }
-                }
-            }
+        for (int pb = 0; pb < PB; ++pb) {
+            const int xx = x0 + pb * 16 + lr + dx;
+            const bool ok = yok && xx >= 0 && xx < a.W;
+            const int xc = xx < 0 ? 0 : (xx >= a.W ? a.W - 1 : xx);
+            const uint16_t* src =
+                two ? a.X2 + (((size_t)b * (a.H >> 1) + (yc >> 1)) * (a.W >> 1) + (xc >> 1)) * a.C2 + (ci0 - a.C1)
+                    : a.X1 + (((size_t)b * a.H + yc) * a.W + xc) * a.C1 + ci0;
+            const uint4 v = *reinterpret_cast<const uint4*>(src);
+            fa[pb].u = ok ? v : make_uint4(0u, 0u, 0u, 0u);
+        }
+    };
+    auto load_b = [&](int kc, Frag8 (&fb)[NB]) {
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            const int nb = nb0 + j < NBT ? nb0 + j : NBT - 1;
+            fb[j].u = *reinterpret_cast<const uint4*>(a.Wt32 + (((size_t)kc * NBT + nb) * 64 + lane) * 8);
+        }
+    };
+    // software pipeline: chunk kc + 1's fragments load while chunk kc's products run
+    Frag8 fa[PB], fb[NB];
+    load_a(0, fa);
+    load_b(0, fb);
+    for (int kc = 0; kc < nkc; ++kc) {
+        Frag8 ca[PB], cb[NB];
+#pragma unroll
+        for (int pb = 0; pb < PB; ++pb) ca[pb] = fa[pb];
+#pragma unroll
+        for (int j = 0; j < NB; ++j) cb[j] = fb[j];
+        if (kc + 1 < nkc) {
+            load_a(kc + 1, fa);
+            load_b(kc + 1, fb);
         }
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
-            const int nb = nb0 + j;
-            if (nb >= NBT) break;
-            Frag8 fb;
-            fb.u = *reinterpret_cast<const uint4*>(a.Wt32 + (((size_t)kc * NBT + nb) * 64 + lane) * 8);
+            if (nb0 + j >= NBT) break;
 #pragma unroll
             for (int pb = 0; pb < PB; ++pb)
-                acc[pb][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[pb].v, fb.v, acc[pb][j], 0, 0, 0);
+                acc[pb][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ca[pb].v, cb[j].v, acc[pb][j], 0, 0, 0);
         }
     }
-#pragma unroll
-    for (int j = 0; j < NB; ++j) {
-        const int nb = nb0 + j;
-        if (nb >= NBT) break;
-        const int co = nb * 16 + lr;
-        const float bias = a.bias[co];
-#pragma unroll
-        for (int pb = 0; pb < PB; ++pb)
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int x = x0 + pb * 16 + q * 4 + e;
-                if (x >= a.W) continue;
-                float v = acc[pb][j][e] + bias;
-                if (a.relu) v = fmaxf(v, 0.0f);
-                const size_t o = (((size_t)b * a.H + y) * a.W + x) * a.CO + co;
-                if (a.Yf)
-                    a.Yf[o] = v;
-                else
-                    a.Y[o] = f2bf(v);
-            }
-    }
+    store_blocks<PB, NB>(a, acc, bias, nb0, NBT, b, y, x0, q, lr);
 }
 
 // The same convolution staged through LDS, for the full-resolution layers. A workgroup (4
@@ -225,7 +264,17 @@ constexpr int LT_PX = 66, LT_ROWS = 6, LT_CP = 40;
 
 template <int NB>
 __global__ __launch_bounds__(256) void k_conv_fwd_lds(ConvArgs a) {
-    __shared__ __attribute__((aligned(16))) uint16_t tile[LT_ROWS * LT_PX * LT_CP];
+    // up to 3 output-channel blocks the slice's weights (9 * NB KB) sit in LDS beside the tile,
+    // so that the tap loop reads LDS only; wider layers read them from global memory, one tap ahead
+    constexpr bool WLDS = NB <= 3;
+    constexpr int NT = LT_ROWS * LT_PX * 4;  // 16-byte pieces of a slice
+    constexpr int NI = (NT + 255) / 256;
+    constexpr int NWT = 9 * NB * 64;         // 16-byte weight fragments of a slice
+    constexpr int NW = WLDS ? (NWT + 255) / 256 : 0;
+    // both arrays are padded to whole rounds of 256 pieces, so that every staging store is
+    // unconditional (a guarded store lets the compiler sink its load into the branch)
+    __shared__ __attribute__((aligned(16))) uint16_t tile[(NI * 256 / (LT_PX * 4) + 1) * LT_PX * LT_CP];
+    __shared__ uint4 s_w[WLDS ? NW * 256 : 1];
     const int CI = a.C1 + a.C2;
     const int nslice = (CI + 31) / 32;
     const int runs = (a.W + 63) / 64;
@@ -236,7 +285,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd_lds(ConvArgs a) {
     const int rb = blk % rblocks;
     const int b = blk / rblocks;
     const int x0 = run * 64, y0 = rb * 4;
-    const int NBT = a.CO / 16;
+    const int NBT = a.CO / 16;  // == NB: the launch picks the instance
     const int wave = threadIdx.x >> 6;
     const int lane = __lane_id();
     const int q = lane >> 4, lr = lane & 15;
@@ -245,25 +294,68 @@ __global__ __launch_bounds__(256) void k_conv_fwd_lds(ConvArgs a) {
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < NB; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    float bias[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) bias[j] = a.bias[(j < NBT ? j : 0) * 16 + lr];
+    const uint4* wl4 = reinterpret_cast<const uint4*>(a.Wl);
     for (int sl = 0; sl < nslice; ++sl) {
-        // stage the slice: 6 rows x 66 pixels x 4 groups of 8 channels, 16 bytes per thread step
-        for (int i = threadIdx.x; i < LT_ROWS * LT_PX * 4; i += 256) {
-            const int g8 = i & 3, pxl = (i >> 2) % LT_PX, ry = (i >> 2) / LT_PX;
+        // stage the slice: 6 rows x 66 pixels x 4 groups of 8 channels (and its weights). Every
+        // load is issued before any is stored, at a clamped (valid) address; pieces outside the
+        // image or past CI are zeroed by a mask when stored.
+        uint4 v[NI];
+        uint32_t okm = 0;
+#pragma unroll
+        for (int k = 0; k < NI; ++k) {
+            const int i = threadIdx.x + k * 256;
+            const int ii = i < NT ? i : 0;
+            const int g8 = ii & 3, pxl = (ii >> 2) % LT_PX, ry = (ii >> 2) / LT_PX;
             const int yy = y0 - 1 + ry, xx = x0 - 1 + pxl, ci0 = sl * 32 + g8 * 8;
-            uint4 v = make_uint4(0u, 0u, 0u, 0u);
-            if (ci0 < CI && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) {
-                const uint16_t* src;
-                if (ci0 < a.C1)
-                    src = a.X1 + (((size_t)b * a.H + yy) * a.W + xx) * a.C1 + ci0;
-                else
-                    src = a.X2 + (((size_t)b * (a.H >> 1) + (yy >> 1)) * (a.W >> 1) + (xx >> 1)) * a.C2 + (ci0 - a.C1);
-                v = *reinterpret_cast<const uint4*>(src);
-            }
-            *reinterpret_cast<uint4*>(tile + (ry * LT_PX + pxl) * LT_CP + g8 * 8) = v;
+            const bool ok = ci0 < CI && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
+            okm |= (ok ? 1u : 0u) << k;
+            const int yc = yy < 0 ? 0 : (yy >= a.H ? a.H - 1 : yy);
+            const int xc = xx < 0 ? 0 : (xx >= a.W ? a.W - 1 : xx);
+            const int cc = ci0 < CI ? ci0 : 0;
+            const uint16_t* src =
+                cc >= a.C1 ? a.X2 + (((size_t)b * (a.H >> 1) + (yc >> 1)) * (a.W >> 1) + (xc >> 1)) * a.C2 + (cc - a.C1)
+                           : a.X1 + (((size_t)b * a.H + yc) * a.W + xc) * a.C1 + cc;
+            v[k] = *reinterpret_cast<const uint4*>(src);
         }
+        uint4 wv[NW > 0 ? NW : 1];
+#pragma unroll
+        for (int k = 0; k < NW; ++k) {
+            const int i = threadIdx.x + k * 256;
+            const int ii = i < NWT ? i : 0;
+            const int t = ii / (NB * 64), r = ii - t * (NB * 64);  // r = nb * 64 + lane
+            wv[k] = wl4[((size_t)t * nslice + sl) * (NB * 64) + r];
+        }
+#pragma unroll
+        for (int k = 0; k < NI; ++k) {
+            const int i = threadIdx.x + k * 256;  // pieces past NT land in the padding rows
+            const int g8 = i & 3, pxl = (i >> 2) % LT_PX, ry = (i >> 2) / LT_PX;
+            const uint32_t m = 0u - ((okm >> k) & 1u);
+            *reinterpret_cast<uint4*>(tile + (ry * LT_PX + pxl) * LT_CP + g8 * 8) =
+                make_uint4(v[k].x & m, v[k].y & m, v[k].z & m, v[k].w & m);
+        }
+#pragma unroll
+        for (int k = 0; k < NW; ++k) s_w[threadIdx.x + k * 256] = wv[k];
         __syncthreads();
+        auto load_w = [&](int t, Frag8 (&fb)[NB]) {
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                if constexpr (WLDS)
+                    fb[j].u = s_w[(t * NB + j) * 64 + lane];
+                else
+                    fb[j].u = wl4[(((size_t)t * nslice + sl) * NB + j) * 64 + lane];
+            }
+        };
+        Frag8 fw[NB];
+        load_w(0, fw);
 #pragma unroll 1
         for (int t = 0; t < 9; ++t) {
+            Frag8 cw[NB];
+#pragma unroll
+            for (int j = 0; j < NB; ++j) cw[j] = fw[j];
+            if (t < 8) load_w(t + 1, fw);
             const int ry = wave + t / 3;  // tile row of input row y + t/3 - 1
             const int dx = t % 3;         // tile pixel offset = output pixel - x0 + dx
             Frag8 fa[4];
@@ -271,40 +363,16 @@ __global__ __launch_bounds__(256) void k_conv_fwd_lds(ConvArgs a) {
             for (int pb = 0; pb < 4; ++pb)
                 fa[pb].u = *reinterpret_cast<const uint4*>(tile + (ry * LT_PX + pb * 16 + lr + dx) * LT_CP + q * 8);
 #pragma unroll
-            for (int j = 0; j < NB; ++j) {
-                const int nb = j;
-                if (nb >= NBT) break;
-                Frag8 fb;
-                fb.u = *reinterpret_cast<const uint4*>(a.Wl + ((((size_t)t * nslice + sl) * NBT + nb) * 64 + lane) * 8);
+            for (int j = 0; j < NB; ++j)
 #pragma unroll
                 for (int pb = 0; pb < 4; ++pb)
-                    acc[pb][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[pb].v, fb.v, acc[pb][j], 0, 0, 0);
-            }
+                    acc[pb][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[pb].v, cw[j].v, acc[pb][j], 0, 0, 0);
         }
         __syncthreads();
     }
     const int y = y0 + wave;
     if (y >= a.H) return;
-#pragma unroll
-    for (int j = 0; j < NB; ++j) {
-        if (j >= NBT) break;
-        const int co = j * 16 + lr;
-        const float bias = a.bias[co];
-#pragma unroll
-        for (int pb = 0; pb < 4; ++pb)
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int x = x0 + pb * 16 + q * 4 + e;
-                if (x >= a.W) continue;
-                float v = acc[pb][j][e] + bias;
-                if (a.relu) v = fmaxf(v, 0.0f);
-                const size_t o = (((size_t)b * a.H + y) * a.W + x) * a.CO + co;
-                if (a.Yf)
-                    a.Yf[o] = v;
-                else
-                    a.Y[o] = f2bf(v);
-            }
-    }
+    store_blocks<4, NB>(a, acc, bias, 0, NBT, b, y, x0, q, lr);
 }
 
 // ------------------------------------------------------------------ backward data
@@ -761,10 +829,11 @@ static void conv_fwd(hipStream_t s, const ConvArgs& a, bool k32, int shape) {
         // enough wavefronts to fill 256 CUs: fewer channel blocks, then fewer pixels per wave
         const long want = 6144;
         int pbk = 4;
-        // the LDS-staged kernel where the launch is large (the full-resolution layers)
-        const bool lds = shape == 5 || (shape == 0 && waves >= want && NBT <= 6);
+        // the LDS-staged kernel for every layer (it beats the per-wavefront kernel at all four
+        // resolutions); shape 5 forces it, other nonzero shapes force the per-wavefront kernel
+        const bool lds = shape == 5 || (shape == 0 && NBT <= 6);
         if (lds) {
-            long blocks = (long)a.B * ((a.H + 3) / 4) * ((a.W + 63) / 64);
+            const long blocks = (long)a.B * ((a.H + 3) / 4) * ((a.W + 63) / 64);
             switch (NBT) {
             case 1: hipLaunchKernelGGL((k_conv_fwd_lds<1>), dim3(blocks), dim3(256), 0, s, a); break;
             case 2: hipLaunchKernelGGL((k_conv_fwd_lds<2>), dim3(blocks), dim3(256), 0, s, a); break;

@@ -48,16 +48,29 @@ __global__ __launch_bounds__(256) void k_reduce_partial(const float4* __restrict
         v[9 + i] = -INFINITY;  // sphere max
     }
     v[12] = 0.0f;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-        float4 s = sph[i];
-        float c[3] = {s.x, s.y, s.z};
-        float r = s.w;
-        for (int a = 0; a < 3; ++a) {
-            v[a] = fminf(v[a], c[a]);
-            v[3 + a] = fmaxf(v[3 + a], c[a]);
-            v[6 + a] = fminf(v[6 + a], c[a] - r);
-            v[9 + a] = fmaxf(v[9 + a], c[a] + r);
-            v[12] = fmaxf(v[12], fabsf(c[a]) + fabsf(r));
+    // 8 spheres per thread per step, loads issued first at clamped indices (min and max do not
+    // depend on the order, and a repeated sphere changes nothing)
+    constexpr int SB = 8;
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x * SB;
+    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x * SB; b < n; b += step) {
+        float4 sb[SB];
+#pragma unroll
+        for (int j = 0; j < SB; ++j) {
+            const uint64_t i = b + (uint64_t)j * blockDim.x + threadIdx.x;
+            sb[j] = sph[i < n ? i : n - 1];
+        }
+#pragma unroll
+        for (int j = 0; j < SB; ++j) {
+            const float4 s = sb[j];
+            float c[3] = {s.x, s.y, s.z};
+            float r = s.w;
+            for (int a = 0; a < 3; ++a) {
+                v[a] = fminf(v[a], c[a]);
+                v[3 + a] = fmaxf(v[3 + a], c[a]);
+                v[6 + a] = fminf(v[6 + a], c[a] - r);
+                v[9 + a] = fmaxf(v[9 + a], c[a] + r);
+                v[12] = fmaxf(v[12], fabsf(c[a]) + fabsf(r));
+            }
         }
     }
     __shared__ float red[NRED][256];
@@ -170,12 +183,17 @@ __global__ __launch_bounds__(RB) void k_radix_hist(const K* __restrict__ keys, u
     __shared__ uint32_t h[256];
     h[threadIdx.x] = 0;
     const uint64_t base = (uint64_t)blockIdx.x * RT + (uint64_t)(threadIdx.x >> 6) * RW + __lane_id();
+    // unconditional loads at clamped indices (a guarded load becomes a branch and a wait per
+    // key); keys past n count as digit 256, which is not counted
     uint32_t d[RI];
 #pragma unroll
     for (int r = 0; r < RI; ++r) {
         uint64_t i = base + (uint64_t)r * 64;
-        d[r] = i < n ? (uint32_t)((keys[i] >> shift) & 255u) : 256u;
+        d[r] = (uint32_t)((keys[i < n ? i : n - 1] >> shift) & 255u);
     }
+#pragma unroll
+    for (int r = 0; r < RI; ++r)
+        if (base + (uint64_t)r * 64 >= n) d[r] = 256u;
     __syncthreads();
     // LDS atomics; the lanes whose digit equals lane 0's add as one (Morton digits of
     // neighbouring keys are often equal, and the LDS unit serialises same-address lanes)
@@ -372,10 +390,24 @@ __global__ __launch_bounds__(256) void k_radix_ghist(const K* __restrict__ keys,
     __shared__ uint32_t h[8][256];
     for (int p = 0; p < 8; ++p) h[p][threadIdx.x] = 0;
     __syncthreads();
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const K k = keys[i];
-        for (int p = 0; p < passes; ++p) hist_add(h[p], (uint32_t)((k >> (8 * p)) & 255u));
+    // 16 keys per thread per step, all loads issued first (coalesced: key j of a step is
+    // 256 * j keys after key 0 of the workgroup's step)
+    constexpr int KB = 16;
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x * KB;
+    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x * KB; b < n; b += step) {
+        K k[KB];
+#pragma unroll
+        for (int j = 0; j < KB; ++j) {
+            const uint64_t i = b + (uint64_t)j * blockDim.x + threadIdx.x;
+            k[j] = keys[i < n ? i : n - 1];
+        }
+#pragma unroll
+        for (int j = 0; j < KB; ++j) {
+            const bool valid = b + (uint64_t)j * blockDim.x + threadIdx.x < n;
+            if (__ballot(valid) == 0) break;
+            if (valid)
+                for (int p = 0; p < passes; ++p) hist_add(h[p], (uint32_t)((k[j] >> (8 * p)) & 255u));
+        }
     }
     __syncthreads();
     for (int p = 0; p < passes; ++p) {
@@ -754,8 +786,10 @@ __global__ __launch_bounds__(BC) void k_bu_chunk(const K* __restrict__ keys, con
 // loads of the next chunk's spheres and keys (and of the ids one chunk further) are issued just
 // before those arrival counts, so that the one wait for global memory per chunk covers both.
 // flags[] must be zero on entry; n >= 2.
+// Two 1024-thread workgroups per CU (their LDS allows two) are 8 waves per SIMD: at most 64 VGPRs.
 template <typename K, int BC>
-__global__ __launch_bounds__(BC) void k_bu_chunk_ls(const K* __restrict__ keys, const uint32_t* __restrict__ ids,
+__global__ __launch_bounds__(BC) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_bu_chunk_ls(
+    const K* __restrict__ keys, const uint32_t* __restrict__ ids,
                                                     const float4* __restrict__ sph, float4* __restrict__ leaf_sph,
                                                     uint32_t* __restrict__ leaf_id, int write_ids, int64_t n64,
                                                     Node* __restrict__ nodes, Node* meet, int2* meet_far,
@@ -765,6 +799,7 @@ __global__ __launch_bounds__(BC) void k_bu_chunk_ls(const K* __restrict__ keys, 
     __shared__ uint16_t s_far[BC][2];   // chunk-relative far range ends
     __shared__ int8_t s_delta[BC + 1];  // delta(cs - 1 + j)
     __shared__ uint8_t s_h[BC][2];
+    __shared__ int8_t s_dx[BC][2];      // per side: delta(l - 1) of a left child, delta(r) of a right child
     __shared__ uint8_t s_side[BC];
     __shared__ uint32_t s_cnt[BC];
     __shared__ uint16_t s_lst[2][BC];   // splits completed in the previous / current round
@@ -829,16 +864,15 @@ __global__ __launch_bounds__(BC) void k_bu_chunk_ls(const K* __restrict__ keys, 
         }
         s_cnt[tid] = 0;
         __syncthreads();
-        // deliver node X = [l, r] (box b, height h; its children's boxes f and references kd
-        // when internal) to its parent
-        auto deliver = [&](int32_t l, int32_t r, const float b[6], int h, bool internal, const float f[12], int2 kd,
-                           int32_t leaf, uint32_t* nl_next, uint16_t* lst_next) {
+        // deliver node X = [l, r] (box b, height h, dl = delta(l - 1), dr = delta(r); its
+        // children's boxes f and references kd when internal) to its parent
+        auto deliver = [&](int32_t l, int32_t r, int dl, int dr, const float b[6], int h, bool internal,
+                           const float f[12], int2 kd, int32_t leaf, uint32_t* nl_next, uint16_t* lst_next) {
             if (l == 0 && r == n - 1) {  // the root
                 store_node(nodes, f, kd, h);
                 info->height = (uint32_t)h;
                 return;
             }
-            const int dl = s_delta[l - cs], dr = s_delta[r + 1 - cs];  // l - 1 >= cs - 1, r <= ce - 1
             const bool left = dr > dl;
             const int32_t sp = left ? r : l - 1;
             int32_t ref;
@@ -865,6 +899,7 @@ __global__ __launch_bounds__(BC) void k_bu_chunk_ls(const K* __restrict__ keys, 
             s_ref[slot][side] = rrel;
             s_far[slot][side] = (uint16_t)(far - cs);
             s_h[slot][side] = (uint8_t)h;
+            s_dx[slot][side] = (int8_t)(left ? dl : dr);  // the parent's own dl (left) or dr (right)
             s_side[slot] = (uint8_t)side;
             // the merge happens in the next round, after a barrier: a relaxed count suffices
             uint32_t old = __hip_atomic_fetch_add(&s_cnt[slot], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -878,7 +913,7 @@ __global__ __launch_bounds__(BC) void k_bu_chunk_ls(const K* __restrict__ keys, 
             float b[6];
             leaf_box(sp_cur, b);
             float f0[12];
-            deliver(k, k, b, 0, false, f0, make_int2(0, 0), k, &s_nl[0], s_lst[0]);
+            deliver(k, k, s_delta[tid], s_delta[tid + 1], b, 0, false, f0, make_int2(0, 0), k, &s_nl[0], s_lst[0]);
         }
         __syncthreads();
         for (int rd = 0;; ++rd) {
@@ -898,13 +933,14 @@ __global__ __launch_bounds__(BC) void k_bu_chunk_ls(const K* __restrict__ keys, 
                 const int h0 = s_h[slot][0], h1 = s_h[slot][1];
                 const int h = 1 + (h0 > h1 ? h0 : h1);
                 const int32_t l = cs + (int32_t)s_far[slot][0], r = cs + (int32_t)s_far[slot][1];
+                const int dl = s_dx[slot][0], dr = s_dx[slot][1];
                 float b[6];
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
                     b[a] = fminf(f[a], f[6 + a]);
                     b[3 + a] = fmaxf(f[3 + a], f[9 + a]);
                 }
-                deliver(l, r, b, h, true, f, kd, 0, nl_next, lst_next);
+                deliver(l, r, dl, dr, b, h, true, f, kd, 0, nl_next, lst_next);
             }
             __syncthreads();
         }
