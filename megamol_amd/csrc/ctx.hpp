@@ -89,7 +89,8 @@ struct mmhrt_ctx_s {
     mmhrt::DevBuf<float4> leaf_sph;
     mmhrt::DevBuf<uint32_t> leaf_id;
     mmhrt::DevBuf<mmhrt::Node> nodes;
-    mmhrt::DevBuf<uint32_t> flags;
+    mmhrt::DevBuf<uint32_t> flags;   // arrivals at the global meeting places (generation numbers)
+    uint32_t build_gen = 0;          // generation number of the last hierarchy pass
     mmhrt::DevBuf<mmhrt::Rec4> rec4;  // 4-wide records (valid while rec4_valid)
     bool rec4_valid = false;
     mmhrt::DevBuf<uint32_t> rec4_list;  // collapse frontiers (two halves)

@@ -147,34 +147,82 @@ __device__ __forceinline__ uint64_t spread21(uint64_t v) {
     return v;
 }
 
-template <typename K>
-__global__ __launch_bounds__(256) void k_morton(const float4* __restrict__ sph, uint64_t n,
-                                                const SceneInfo* __restrict__ info, K* __restrict__ keys,
-                                                uint32_t* __restrict__ vals) {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float4 s = sph[i];
-    float sc = info->scale;
-    constexpr uint32_t maxq = sizeof(K) == 8 ? 2097151u : 1023u;
-    float f[3] = {(s.x - info->clo[0]) * sc, (s.y - info->clo[1]) * sc, (s.z - info->clo[2]) * sc};
-    uint32_t q[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        uint32_t u = (uint32_t)f[a];
-        q[a] = u > maxq ? maxq : u;
-    }
-    if constexpr (sizeof(K) == 8)
-        keys[i] = (spread21(q[0]) << 2) | (spread21(q[1]) << 1) | spread21(q[2]);
-    else
-        keys[i] = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
-    vals[i] = (uint32_t)i;
-}
-
 // ---------------------------------------------------------------- radix sort
 // A tile is RT consecutive keys; wave w of the block owns the contiguous segment
 // [w * RT/4, (w+1) * RT/4) and walks it in RI rounds of 64, so the wave order, the round order
 // and the lane order together are the input order (what makes the ranking stable). Every
 // thread issues all of its RI key (and value) loads before it ranks anything.
+
+// Adds digit d of every active lane to the LDS histogram h; lanes with the first active lane's
+// digit add as one (Morton digits of neighbouring keys are often equal, and the LDS unit
+// serialises same-address lanes).
+__device__ __forceinline__ void hist_add(uint32_t* h, uint32_t d) {
+    const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+    const bool eq = d == d0;
+    const uint64_t m = __ballot(eq);
+    const uint64_t act = __ballot(true);
+    if (__lane_id() == (uint32_t)(__ffsll((unsigned long long)act) - 1)) atomicAdd(&h[d0], (uint32_t)__popcll(m));
+    if (!eq) atomicAdd(&h[d], 1u);
+}
+
+__device__ __forceinline__ uint32_t morton30(const float4 s, const SceneInfo* info, float sc) {
+    const float f[3] = {(s.x - info->clo[0]) * sc, (s.y - info->clo[1]) * sc, (s.z - info->clo[2]) * sc};
+    uint32_t q[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const uint32_t u = (uint32_t)f[a];
+        q[a] = u > 1023u ? 1023u : u;
+    }
+    return (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
+}
+__device__ __forceinline__ uint64_t morton63(const float4 s, const SceneInfo* info, float sc) {
+    const float f[3] = {(s.x - info->clo[0]) * sc, (s.y - info->clo[1]) * sc, (s.z - info->clo[2]) * sc};
+    uint32_t q[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const uint32_t u = (uint32_t)f[a];
+        q[a] = u > 2097151u ? 2097151u : u;
+    }
+    return (spread21(q[0]) << 2) | (spread21(q[1]) << 1) | spread21(q[2]);
+}
+
+// The Morton codes of one radix tile (RT spheres, laid out as the sort's tiles are) and the
+// tile's histogram of the first 8-bit digit: the first radix pass then needs no histogram pass,
+// and no value array either, since its values are the particle indices themselves.
+template <typename K, int RI>
+__global__ __launch_bounds__(RB) void k_morton_tile(const float4* __restrict__ sph, uint64_t n,
+                                                    const SceneInfo* __restrict__ info, K* __restrict__ keys,
+                                                    uint32_t* __restrict__ hist) {
+    constexpr int RT = RB * RI, RW = RT / (RB / 64), SB = 8;
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const float sc = info->scale;
+    const uint64_t base = (uint64_t)blockIdx.x * RT + (uint64_t)(threadIdx.x >> 6) * RW + __lane_id();
+    for (int r0 = 0; r0 < RI; r0 += SB) {
+        float4 sb[SB];  // SB sphere loads in flight, at clamped indices
+#pragma unroll
+        for (int j = 0; j < SB; ++j) {
+            const uint64_t i = base + (uint64_t)(r0 + j) * 64;
+            sb[j] = sph[i < n ? i : n - 1];
+        }
+#pragma unroll
+        for (int j = 0; j < SB; ++j) {
+            const uint64_t i = base + (uint64_t)(r0 + j) * 64;
+            K key;
+            if constexpr (sizeof(K) == 8)
+                key = morton63(sb[j], info, sc);
+            else
+                key = morton30(sb[j], info, sc);
+            if (i < n) {
+                keys[i] = key;
+                hist_add(h, (uint32_t)(key & 255u));
+            }
+        }
+    }
+    __syncthreads();
+    hist[(size_t)blockIdx.x * 256 + threadIdx.x] = h[threadIdx.x];
+}
 
 template <typename K, int RI>
 __global__ __launch_bounds__(RB) void k_radix_hist(const K* __restrict__ keys, uint64_t n, int shift,
@@ -209,7 +257,7 @@ __global__ __launch_bounds__(RB) void k_radix_hist(const K* __restrict__ keys, u
     hist[(size_t)blockIdx.x * 256 + threadIdx.x] = h[threadIdx.x];
 }
 
-// Stable scatter of one tile for one 8-bit digit. Each wave ranks its segment with ballots and
+// Stable scatter of one tile for one 8-bit digit (vin null: the values are the key indices). Each wave ranks its segment with ballots and
 // a running per-(wave, digit) count in LDS (no barrier between rounds), one pass over the 256
 // digits turns the per-wave counts into offsets, the keys are placed in digit-sorted order in
 // LDS and finally written out in digit-contiguous runs, so that consecutive threads write
@@ -236,7 +284,16 @@ __global__ __launch_bounds__(RB) void k_radix_scatter(const K* __restrict__ kin,
         uint64_t i = base + (uint64_t)r * 64;
         bool valid = i < n;
         key[r] = valid ? kin[i] : (K)0;
-        val[r] = valid ? vin[i] : 0u;
+    }
+    if (vin) {
+#pragma unroll
+        for (int r = 0; r < RI; ++r) {
+            uint64_t i = base + (uint64_t)r * 64;
+            val[r] = i < n ? vin[i] : 0u;
+        }
+    } else {  // the first pass: the values are the indices
+#pragma unroll
+        for (int r = 0; r < RI; ++r) val[r] = (uint32_t)(base + (uint64_t)r * 64);
     }
     // tile-local digit starts: exclusive scan of this tile's digit counts (thread = digit)
     const uint32_t cnt = hist[(size_t)blockIdx.x * 256 + threadIdx.x];
@@ -372,17 +429,6 @@ __global__ __launch_bounds__(256) void k_col_offsets(const uint32_t* __restrict_
 constexpr uint32_t OS_AGG = 1u << 30, OS_INC = 2u << 30, OS_MASK = (1u << 30) - 1u;
 constexpr int OS_SPIN = 1 << 20;
 
-// Adds digit d of every active lane to the LDS histogram h; lanes with the first active lane's
-// digit add as one.
-__device__ __forceinline__ void hist_add(uint32_t* h, uint32_t d) {
-    const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
-    const bool eq = d == d0;
-    const uint64_t m = __ballot(eq);
-    const uint64_t act = __ballot(true);
-    if (__lane_id() == (uint32_t)(__ffsll((unsigned long long)act) - 1)) atomicAdd(&h[d0], (uint32_t)__popcll(m));
-    if (!eq) atomicAdd(&h[d], 1u);
-}
-
 // ghist[p][d]: keys whose digit p is d, for p < passes (ghist zero on entry).
 template <typename K>
 __global__ __launch_bounds__(256) void k_radix_ghist(const K* __restrict__ keys, uint64_t n, int passes,
@@ -469,7 +515,16 @@ __global__ __launch_bounds__(RB) void k_radix_onesweep(const K* __restrict__ kin
         uint64_t i = base + (uint64_t)r * 64;
         bool valid = i < n;
         key[r] = valid ? kin[i] : (K)0;
-        val[r] = valid ? vin[i] : 0u;
+    }
+    if (vin) {
+#pragma unroll
+        for (int r = 0; r < RI; ++r) {
+            uint64_t i = base + (uint64_t)r * 64;
+            val[r] = i < n ? vin[i] : 0u;
+        }
+    } else {  // the first pass: the values are the indices
+#pragma unroll
+        for (int r = 0; r < RI; ++r) val[r] = (uint32_t)(base + (uint64_t)r * 64);
     }
     uint32_t rank[RI];
 #pragma unroll
@@ -616,7 +671,11 @@ __device__ __forceinline__ int split_delta(const K* __restrict__ keys, int64_t n
 //   finish: once a worklist is small, one workgroup runs the remaining rounds itself with
 //           barriers between them (workgroup scope, no cache writeback).
 // Straddling nodes are a small fraction: the ancestors of one node per chunk boundary.
+// Arrival at a global meeting place: every split receives exactly two arrivals per build, so an
+// exchange of the build's generation number gen into flags[s] tells them apart (the second one
+// reads gen back) without clearing the 4n-byte array between builds; it starts as all ones.
 constexpr int BOX_ROUNDS = 32;        // kernel rounds before the single-workgroup finish
+constexpr int GEN_SLOT = BOX_ROUNDS + 3;  // box_cnt slot holding the generation number
 constexpr uint32_t BOX_SMALL = 512;   // worklist size handed to the single workgroup
 
 __device__ __forceinline__ void leaf_box(const float4 s, float b[6]) {
@@ -670,7 +729,7 @@ struct SplitList {
     }
 };
 
-// The chunk pass, one thread per leaf. flags[] must be zero on entry; n >= 2.
+// The chunk pass, one thread per leaf; n >= 2.
 template <typename K, int BC>
 __global__ __launch_bounds__(BC) void k_bu_chunk(const K* __restrict__ keys, const float4* __restrict__ leaf_sph,
                                                  int64_t n, Node* __restrict__ nodes, Node* meet, int2* meet_far,
@@ -685,6 +744,7 @@ __global__ __launch_bounds__(BC) void k_bu_chunk(const K* __restrict__ keys, con
     __shared__ uint32_t s_list[256];
     __shared__ uint32_t s_n, s_base;
     SplitList ap{s_list, &s_n, &s_base};
+    const uint32_t gen = cnt[GEN_SLOT];
     const int64_t cs = (int64_t)blockIdx.x * BC;
     const int64_t ce = cs + BC < n ? cs + BC : n;
     const int64_t k = cs + threadIdx.x;
@@ -720,7 +780,7 @@ __global__ __launch_bounds__(BC) void k_bu_chunk(const K* __restrict__ keys, con
             const int32_t far = (int32_t)(left ? l : r);
             if (s < cs || s + 1 >= ce) {  // the split lies on a chunk boundary
                 meet_put(meet + s, meet_far + s, side, b, ref, h, far);
-                if (atomicAdd(flags + s, 1u) == 1u) ap.push((uint32_t)s, list, cnt + 1);
+                if (atomicExch(flags + s, gen) == gen) ap.push((uint32_t)s, list, cnt + 1);
                 break;
             }
             const int slot = (int)(s - cs);
@@ -764,7 +824,7 @@ __global__ __launch_bounds__(BC) void k_bu_chunk(const K* __restrict__ keys, con
 #pragma unroll
         for (int a = 0; a < 6; ++a) b[a] = s_box[slot][side * 6 + a];
         meet_put(meet + s, meet_far + s, side, b, s_ref[slot][side], s_h[slot][side], s_far[slot][side]);
-        if (atomicAdd(flags + s, 1u) == 1u) ap.push((uint32_t)s, list, cnt + 1);
+        if (atomicExch(flags + s, gen) == gen) ap.push((uint32_t)s, list, cnt + 1);
     }
     ap.flush(list, cnt + 1);
 }
@@ -775,7 +835,7 @@ __global__ __launch_bounds__(BC) void k_bu_chunk(const K* __restrict__ keys, con
 // climbs. Round 0 delivers the leaves; round r merges the splits completed in round r - 1,
 // writes their nodes and delivers them to their parents. LDS records are chunk-relative 16-bit
 // (a reference is the leaf or node index minus the chunk start, bit 15 marking a leaf).
-// flags[] must be zero on entry; n >= 2.
+// n >= 2.
 // The chunk pass, level-synchronous and persistent: workgroup g takes chunks g, g + G, g + 2G, ...
 // (G = the grid) and also gathers the leaf spheres into Morton order (leaf_sph[k] = sph[ids[k]],
 // and leaf_id[k] = ids[k] when write_ids). Round 0 delivers the leaves, each later round merges
@@ -785,7 +845,7 @@ __global__ __launch_bounds__(BC) void k_bu_chunk(const K* __restrict__ keys, con
 // arrival count after the rounds, together with the in-chunk splits left with one arrival. The
 // loads of the next chunk's spheres and keys (and of the ids one chunk further) are issued just
 // before those arrival counts, so that the one wait for global memory per chunk covers both.
-// flags[] must be zero on entry; n >= 2.
+// n >= 2.
 // Two 1024-thread workgroups per CU (their LDS allows two) are 8 waves per SIMD: at most 64 VGPRs.
 template <typename K, int BC>
 __global__ __launch_bounds__(BC) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_bu_chunk_ls(
@@ -808,6 +868,7 @@ __global__ __launch_bounds__(BC) __attribute__((amdgpu_waves_per_eu(8, 8))) void
     __shared__ uint32_t s_list[256];
     __shared__ uint32_t s_n, s_base;
     SplitList ap{s_list, &s_n, &s_base};
+    const uint32_t gen = cnt[GEN_SLOT];
     const int32_t n = (int32_t)n64;
     const int32_t nchunks = (n + BC - 1) / BC;
     const int32_t tid = (int32_t)threadIdx.x;
@@ -955,7 +1016,7 @@ __global__ __launch_bounds__(BC) __attribute__((amdgpu_waves_per_eu(8, 8))) void
         // child moves to the global record first
         if (tid < 2 && s_pend[tid] >= 0) {
             const int32_t sp = s_pend[tid];
-            if (atomicAdd(flags + sp, 1u) == 1u) ap.push((uint32_t)sp, list, cnt + 1);
+            if (atomicExch(flags + sp, gen) == gen) ap.push((uint32_t)sp, list, cnt + 1);
         }
         const int nsplit = ce - cs - 1;
         for (int slot = tid; slot < nsplit; slot += BC) {
@@ -967,7 +1028,7 @@ __global__ __launch_bounds__(BC) __attribute__((amdgpu_waves_per_eu(8, 8))) void
             for (int a = 0; a < 6; ++a) b[a] = s_box[slot][side * 6 + a];
             meet_put(meet + sp, meet_far + sp, side, b, gref(s_ref[slot][side]), s_h[slot][side],
                      cs + (int32_t)s_far[slot][side]);
-            if (atomicAdd(flags + sp, 1u) == 1u) ap.push((uint32_t)sp, list, cnt + 1);
+            if (atomicExch(flags + sp, gen) == gen) ap.push((uint32_t)sp, list, cnt + 1);
         }
         __syncthreads();
         if (s_n > 192u) ap.flush(list, cnt + 1);  // s_n is read by every thread after the barrier
@@ -986,7 +1047,8 @@ __global__ __launch_bounds__(BC) __attribute__((amdgpu_waves_per_eu(8, 8))) void
 // Returns the parent's split when this delivery completes it, else -1.
 template <typename K, bool VOL>
 __device__ __forceinline__ int64_t bu_step(const K* __restrict__ keys, int64_t n, Node* nodes, Node* meet,
-                                           int2* meet_far, uint32_t* flags, int64_t s, SceneInfo* info) {
+                                           int2* meet_far, uint32_t* flags, int64_t s, SceneInfo* info,
+                                           uint32_t gen) {
     float f[12];
     int dv[4];
     int fr[2];
@@ -1030,13 +1092,14 @@ __device__ __forceinline__ int64_t bu_step(const K* __restrict__ keys, int64_t n
     store_node(nodes + idx, f, kd, h);
     const int64_t s2 = left ? r : l - 1;
     meet_put(meet + s2, meet_far + s2, left ? 0 : 1, b, (int32_t)idx, h, (int32_t)(left ? l : r));
-    return atomicAdd(flags + s2, 1u) == 1u ? s2 : -1;
+    return atomicExch(flags + s2, gen) == gen ? s2 : -1;
 }
 
 template <typename K>
 __device__ void bu_finish(const K* __restrict__ keys, int64_t n, Node* nodes, Node* meet, int2* meet_far,
                           uint32_t* flags, uint32_t* la, uint32_t* lb, uint32_t n_cur, uint32_t* cnt, SceneInfo* info) {
     __shared__ uint32_t s_out;
+    const uint32_t gen = cnt[GEN_SLOT];
     uint32_t* lin = la;
     uint32_t* lout = lb;
     while (n_cur > 0) {
@@ -1046,7 +1109,7 @@ __device__ void bu_finish(const K* __restrict__ keys, int64_t n, Node* nodes, No
             uint32_t i = base + threadIdx.x;
             if (i < n_cur) {
                 int64_t s = (int64_t)((const volatile uint32_t*)lin)[i];
-                int64_t s2 = bu_step<K, true>(keys, n, nodes, meet, meet_far, flags, s, info);
+                int64_t s2 = bu_step<K, true>(keys, n, nodes, meet, meet_far, flags, s, info, gen);
                 if (s2 >= 0) lout[atomicAdd(&s_out, 1u)] = (uint32_t)s2;
             }
         }
@@ -1069,6 +1132,7 @@ __global__ __launch_bounds__(256) void k_bu_round(const K* __restrict__ keys, in
     __shared__ uint32_t s_list[256];
     __shared__ uint32_t s_n, s_base;
     if (cnt[0]) return;
+    const uint32_t gen = cnt[GEN_SLOT];
     uint32_t n_in = cnt[1 + r];
     uint32_t* lin = (r & 1) ? l1 : l0;
     uint32_t* lout = (r & 1) ? l0 : l1;
@@ -1085,7 +1149,7 @@ __global__ __launch_bounds__(256) void k_bu_round(const K* __restrict__ keys, in
     for (uint32_t base = blockIdx.x * blockDim.x; base < n_in; base += stride) {
         const uint32_t i = base + threadIdx.x;
         if (i < n_in) {
-            int64_t s2 = bu_step<K, false>(keys, n, nodes, meet, meet_far, flags, (int64_t)lin[i], info);
+            int64_t s2 = bu_step<K, false>(keys, n, nodes, meet, meet_far, flags, (int64_t)lin[i], info, gen);
             if (s2 >= 0) ap.push((uint32_t)s2, lout, cnt + 2 + r);
         }
         ap.flush(lout, cnt + 2 + r);
@@ -1126,25 +1190,33 @@ __global__ void k_fill(uint32_t* p, uint32_t v, uint64_t n) {
 
 inline unsigned blocks_for(uint64_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
+// Morton codes of the current spheres into ka, then the LSD radix sort of (key, particle index)
+// over `passes` 8-bit digits; the result is in (ka, va). The Morton pass also produces the first
+// digit's tile histograms, and the first pass takes the particle indices as its values.
 template <typename K, int RI>
-int sort_pairs_t(mmhrt_ctx_s* c, K* ka, K* kb, uint32_t* va, uint32_t* vb, uint64_t n, int passes) {
+int morton_sort_t(mmhrt_ctx_s* c, K* ka, K* kb, uint32_t* va, uint32_t* vb, uint64_t n, int passes) {
     constexpr int RT = RB * RI;
     uint32_t tiles = (uint32_t)((n + RT - 1) / RT);
     uint64_t hlen = (uint64_t)tiles * 256;
     uint32_t groups = (tiles + CG - 1) / CG;
-    if (int e = ensure(c, c->hist, hlen * 2, "radix histogram")) return e;
-    if (int e = ensure(c, c->scan_tmp, (size_t)(groups + 1) * 256, "radix column sums")) return e;
+    const bool os = c->opt_sort_onesweep != 0 && n < (1ull << 30);
+    if (int e = ensure(c, c->hist, hlen * (os ? std::max(2, passes) : 2), "radix histogram")) return e;
+    if (int e = ensure(c, c->scan_tmp, std::max((size_t)(groups + 1) * 256, (size_t)passes * 512 + 16),
+                       "radix column sums"))
+        return e;
     uint32_t* hist = c->hist.p;
     uint32_t* offs = c->hist.p + hlen;
+    rec(c, 1);
+    hipLaunchKernelGGL((k_morton_tile<K, RI>), dim3(tiles), dim3(RB), 0, c->stream, cur_spheres(c), n, c->info.p, ka,
+                       hist);
+    rec(c, 2);
     K* src = ka;
     K* dst = kb;
-    uint32_t* vs = va;
+    uint32_t* vs = nullptr;  // first pass: values are the indices
     uint32_t* vd = vb;
-    if (c->opt_sort_onesweep != 0 && n < (1ull << 30)) {
+    if (os) {
         // single-pass passes: status words per pass and tile, digit counts of all passes,
         // tickets and the fault flag
-        if (int e = ensure(c, c->hist, hlen * (uint64_t)passes, "radix tile status")) return e;
-        if (int e = ensure(c, c->scan_tmp, (size_t)passes * 512 + 16, "radix digit counts")) return e;
         if (!c->sort_fault.p) {
             if (int e = ensure(c, c->sort_fault, 1, "radix fault flag")) return e;
             launch_fill_u32(c->stream, c->sort_fault.p, 0u, 1);
@@ -1161,35 +1233,37 @@ int sort_pairs_t(mmhrt_ctx_s* c, K* ka, K* kb, uint32_t* va, uint32_t* vb, uint6
         for (int p = 0; p < passes; ++p) {
             hipLaunchKernelGGL((k_radix_onesweep<K, RI>), dim3(tiles), dim3(RB), 0, c->stream, src, vs, dst, vd, n, 8 * p,
                                dstart + p * 256, c->hist.p + hlen * p, tickets + p, c->sort_fault.p);
-            std::swap(src, dst);
-            std::swap(vs, vd);
+            src = dst;
+            dst = (p & 1) ? kb : ka;
+            vs = vd;
+            vd = (p & 1) ? vb : va;
         }
         return 0;
     }
     for (int p = 0; p < passes; ++p) {
         int shift = 8 * p;
-        hipLaunchKernelGGL((k_radix_hist<K, RI>), dim3(tiles), dim3(RB), 0, c->stream, src, n, shift, hist);
+        if (p > 0)
+            hipLaunchKernelGGL((k_radix_hist<K, RI>), dim3(tiles), dim3(RB), 0, c->stream, src, n, shift, hist);
         hipLaunchKernelGGL(k_col_sums, dim3(groups), dim3(256), 0, c->stream, hist, tiles, c->scan_tmp.p);
         hipLaunchKernelGGL(k_col_top, dim3(1), dim3(256), 0, c->stream, c->scan_tmp.p, groups);
         hipLaunchKernelGGL(k_col_offsets, dim3(groups), dim3(256), 0, c->stream, hist, tiles, groups, c->scan_tmp.p,
                            offs);
         hipLaunchKernelGGL((k_radix_scatter<K, RI>), dim3(tiles), dim3(RB), 0, c->stream, src, vs, dst, vd, n, shift,
                            hist, offs);
-        K* tk = src;
+        // ping-pong: pass p writes (kb, vb) when p is even, (ka, va) when odd
         src = dst;
-        dst = tk;
-        uint32_t* tv = vs;
+        dst = (p & 1) ? kb : ka;
         vs = vd;
-        vd = tv;
+        vd = (p & 1) ? vb : va;
     }
     return 0;  // passes is even: result is back in (ka, va)
 }
 
 template <typename K>
-int sort_pairs(mmhrt_ctx_s* c, K* ka, K* kb, uint32_t* va, uint32_t* vb, uint64_t n, int passes) {
-    if (c->opt_sort_ri == 8) return sort_pairs_t<K, 8>(c, ka, kb, va, vb, n, passes);
-    if (c->opt_sort_ri == 32 && sizeof(K) == 4) return sort_pairs_t<K, 32>(c, ka, kb, va, vb, n, passes);
-    return sort_pairs_t<K, 16>(c, ka, kb, va, vb, n, passes);
+int morton_sort(mmhrt_ctx_s* c, K* ka, K* kb, uint32_t* va, uint32_t* vb, uint64_t n, int passes) {
+    if (c->opt_sort_ri == 8) return morton_sort_t<K, 8>(c, ka, kb, va, vb, n, passes);
+    if (c->opt_sort_ri == 32 && sizeof(K) == 4) return morton_sort_t<K, 32>(c, ka, kb, va, vb, n, passes);
+    return morton_sort_t<K, 16>(c, ka, kb, va, vb, n, passes);
 }
 
 // Persistent grid of the chunk pass: as many workgroups as fit on the device at once.
@@ -1235,9 +1309,14 @@ int hierarchy(mmhrt_ctx_s* c, const K* keys, const uint32_t* ids, bool write_ids
     if (int e = ensure(c, c->box_cnt, BOX_ROUNDS + 4, "box round counters")) return e;
     if (int e = ensure(c, c->meet, m, "meeting records")) return e;
     if (int e = ensure(c, c->meet_far, m, "meeting ranges")) return e;
+    const bool fresh = !c->flags.p || c->flags.cap < m;
     if (int e = ensure(c, c->flags, m, "meeting counters")) return e;
-    launch_fill_u32(s, c->box_cnt.p, 0u, BOX_ROUNDS + 4);
-    launch_fill_u32(s, c->flags.p, 0u, m);
+    if (fresh) launch_fill_u32(s, c->flags.p, 0xffffffffu, c->flags.cap);  // no generation yet
+    // each pass takes a new generation; builds captured into a graph keep theirs, so such a
+    // graph is launched once (the batched time series), never replayed
+    if (++c->build_gen == 0xffffffffu) c->build_gen = 1;
+    launch_fill_u32(s, c->box_cnt.p, 0u, BOX_ROUNDS + 3);
+    launch_fill_u32(s, c->box_cnt.p + GEN_SLOT, c->build_gen, 1);
     // worklists of the straddler rounds: the spare key buffer (free once the sort is done) and
     // the spare value buffer
     uint32_t* l0 = sizeof(K) == 8 ? reinterpret_cast<uint32_t*>(c->klb.p) : reinterpret_cast<uint32_t*>(c->kb.p);
@@ -1273,11 +1352,8 @@ template <typename K>
 int build_typed(mmhrt_ctx_s* c, K* ka, K* kb) {
     uint64_t n = c->n;
     hipStream_t s = c->stream;
-    rec(c, 1);
-    hipLaunchKernelGGL(k_morton<K>, dim3(blocks_for(n, 256)), dim3(256), 0, s, cur_spheres(c), n, c->info.p, ka, c->va.p);
-    rec(c, 2);
     int passes = sizeof(K) == 8 ? 8 : 4;
-    if (int e = sort_pairs<K>(c, ka, kb, c->va.p, c->vb.p, n, passes)) return e;
+    if (int e = morton_sort<K>(c, ka, kb, c->va.p, c->vb.p, n, passes)) return e;
     rec(c, 3);
     rec(c, 4);  // the leaf gather runs inside the hierarchy pass
     return hierarchy<K>(c, ka, c->va.p, true);
