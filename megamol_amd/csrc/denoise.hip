@@ -52,6 +52,7 @@ __device__ __forceinline__ void store_blocks(const ConvArgs& a, const f32x4 (&ac
                                              int nb0, int NBT, int b, int y, int x0, int q, int lr) {
     const size_t rowo = ((size_t)b * a.H + y) * a.W;
     const bool f32out = a.Yf != nullptr;
+    const int yc = a.YfC ? a.YfC : a.CO;
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
         if (nb0 + j >= NBT) break;
@@ -68,7 +69,7 @@ __device__ __forceinline__ void store_blocks(const ConvArgs& a, const f32x4 (&ac
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     const int x = x0 + pb * 16 + q * 4 + e;
-                    if (x < a.W) a.Yf[(rowo + x) * a.CO + co] = v[e];
+                    if (x < a.W && co < yc) a.Yf[(rowo + x) * yc + co] = v[e];
                 }
             } else {
 #pragma unroll
@@ -162,8 +163,10 @@ __global__ __launch_bounds__(256) void k_conv_fwd(ConvArgs a) {
                 float v = acc[pb][j][e] + bias;
                 if (a.relu) v = fmaxf(v, 0.0f);
                 size_t o = (((size_t)b * a.H + y) * a.W + x) * a.CO + co;
-                if (a.Yf)
-                    a.Yf[o] = v;
+                if (a.Yf) {
+                    const int yc = a.YfC ? a.YfC : a.CO;
+                    if (co < yc) a.Yf[(((size_t)b * a.H + y) * a.W + x) * yc + co] = v;
+                }
                 else
                     a.Y[o] = f2bf(v);
             }
@@ -373,6 +376,139 @@ __global__ __launch_bounds__(256) void k_conv_fwd_lds(ConvArgs a) {
     const int y = y0 + wave;
     if (y >= a.H) return;
     store_blocks<4, NB>(a, acc, bias, 0, NBT, b, y, x0, q, lr);
+}
+
+// The same convolution for single-slice layers (CI <= 32) of up to 32 output channels, with the
+// input brought into LDS by direct global->LDS loads (global_load_lds_dwordx4: 16 bytes per lane
+// into consecutive LDS slots), double buffered: a persistent workgroup walks its tiles and issues
+// the next tile's loads before it computes the current one, without holding the data in
+// registers. The layer's weights are loaded into LDS once. The tile is unpadded, and a pixel's
+// four 8-channel groups are stored XOR-swizzled (slot 4P + (g ^ ((P >> 1) & 3)) for tile pixel
+// P), so that the 8 consecutive pixels of an A-fragment read phase cover all 32 banks. Pieces
+// outside the image or past CI are zeroed with ordinary LDS stores before the loads are issued
+// and left out of them (exec mask).
+constexpr int DT_SLOTS = LT_ROWS * LT_PX * 4;    // 1584 16-byte slots of a tile
+constexpr int DT_INST = (DT_SLOTS + 63) / 64;    // 25 wave-wide loads
+constexpr int DT_ALLOC = DT_INST * 64;           // slots allocated per buffer
+
+template <int NB>
+__global__ __launch_bounds__(256) void k_conv_fwd_dma(ConvArgs a) {
+    __shared__ uint4 tA[DT_ALLOC];
+    __shared__ uint4 tB[DT_ALLOC];
+    constexpr int NWI = (9 * NB * 64 + 255) / 256;
+    __shared__ uint4 s_w[NWI * 256];  // padded: the weight stores are unconditional
+    const int CI = a.C1 + a.C2;  // <= 32: one slice
+    const int runs = (a.W + 63) / 64;
+    const int rblocks = (a.H + 3) / 4;
+    const int ntiles = a.B * rblocks * runs;
+    const int NBT = a.CO / 16;  // == NB
+    const int wave = threadIdx.x >> 6;
+    const int lane = __lane_id();
+    const int q = lane >> 4, lr = lane & 15;
+    int tl = (int)blockIdx.x;
+    if (tl >= ntiles) return;
+    float bias[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) bias[j] = a.bias[j * 16 + lr];
+    {  // the weights, once, with ordinary loads and stores (slot = (t * NB + nb) * 64 + lane)
+        const uint4* wl4 = reinterpret_cast<const uint4*>(a.Wl);
+        uint4 wv[NWI];
+#pragma unroll
+        for (int k = 0; k < NWI; ++k) {
+            const int i = threadIdx.x + k * 256;
+            wv[k] = wl4[i < 9 * NB * 64 ? i : 0];
+        }
+#pragma unroll
+        for (int k = 0; k < NWI; ++k) s_w[threadIdx.x + k * 256] = wv[k];
+    }
+    auto tile_origin = [&](int t, int& b, int& x0, int& y0) {
+        const int run = t % runs;
+        const int r2 = t / runs;
+        y0 = (r2 % rblocks) * 4;
+        b = r2 / rblocks;
+        x0 = run * 64;
+    };
+    // loads of tile t into buf: wave w issues wave-wide loads w, w + 4, ...; the zeros first, so
+    // that no LDS store follows a load into the same buffer
+    auto issue = [&](uint4* buf, int t) {
+        int b, x0, y0;
+        tile_origin(t, b, x0, y0);
+        uint32_t okm = 0;
+#pragma unroll
+        for (int k = 0; k < (DT_INST + 3) / 4; ++k) {
+            const int qi = wave + 4 * k;
+            const int S = qi * 64 + lane;
+            const int P = S >> 2, g = (S & 3) ^ ((P >> 1) & 3);
+            const int ry = P / LT_PX, pxl = P - ry * LT_PX;
+            const int yy = y0 - 1 + ry, xx = x0 - 1 + pxl, ci0 = g * 8;
+            const bool ok = qi < DT_INST && P < LT_ROWS * LT_PX && ci0 < CI && yy >= 0 && yy < a.H && xx >= 0 &&
+                            xx < a.W;
+            okm |= (ok ? 1u : 0u) << k;
+            if (qi < DT_INST && !ok) buf[S] = make_uint4(0u, 0u, 0u, 0u);
+        }
+#pragma unroll
+        for (int k = 0; k < (DT_INST + 3) / 4; ++k) {
+            const int qi = wave + 4 * k;
+            if (!((okm >> k) & 1u)) continue;
+            const int S = qi * 64 + lane;
+            const int P = S >> 2, g = (S & 3) ^ ((P >> 1) & 3);
+            const int ry = P / LT_PX, pxl = P - ry * LT_PX;
+            const int yy = y0 - 1 + ry, xx = x0 - 1 + pxl, ci0 = g * 8;
+            const uint16_t* src =
+                ci0 >= a.C1 ? a.X2 + (((size_t)b * (a.H >> 1) + (yy >> 1)) * (a.W >> 1) + (xx >> 1)) * a.C2 + (ci0 - a.C1)
+                            : a.X1 + (((size_t)b * a.H + yy) * a.W + xx) * a.C1 + ci0;
+            __builtin_amdgcn_global_load_lds((const void*)src, (__attribute__((address_space(3))) void*)(buf + qi * 64),
+                                             16, 0, 0);
+        }
+    };
+    auto compute = [&](const uint4* buf, f32x4 (&acc)[4][NB]) {
+#pragma unroll 1
+        for (int t = 0; t < 9; ++t) {
+            Frag8 cw[NB];
+#pragma unroll
+            for (int j = 0; j < NB; ++j) cw[j].u = s_w[(t * NB + j) * 64 + lane];
+            const int ry = wave + t / 3;
+            const int dx = t % 3;
+            Frag8 fa[4];
+#pragma unroll
+            for (int pb = 0; pb < 4; ++pb) {
+                const int P = ry * LT_PX + pb * 16 + lr + dx;
+                fa[pb].u = buf[P * 4 + (q ^ ((P >> 1) & 3))];
+            }
+#pragma unroll
+            for (int j = 0; j < NB; ++j)
+#pragma unroll
+                for (int pb = 0; pb < 4; ++pb)
+                    acc[pb][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[pb].v, cw[j].v, acc[pb][j], 0, 0, 0);
+        }
+    };
+    issue(tA, tl);
+    // one step: wait for this tile (the barrier's wait), start the next into the other buffer,
+    // compute and store this one; the buffers are distinct arrays, so the compiler knows that
+    // reads of one need not wait for loads into the other
+    auto step = [&](const uint4* cur, uint4* nxt) -> bool {
+        __syncthreads();
+        const int ntl = tl + (int)gridDim.x;
+        const bool more = ntl < ntiles;
+        if (more) issue(nxt, ntl);
+        f32x4 acc[4][NB];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < NB; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+        compute(cur, acc);
+        int b, x0, y0;
+        tile_origin(tl, b, x0, y0);
+        const int y = y0 + wave;
+        if (y < a.H) store_blocks<4, NB>(a, acc, bias, 0, NBT, b, y, x0, q, lr);
+        __syncthreads();  // every wave is done reading cur before it is refilled
+        tl = ntl;
+        return more;
+    };
+    while (true) {
+        if (!step(tA, tB)) break;
+        if (!step(tB, tA)) break;
+    }
 }
 
 // ------------------------------------------------------------------ backward data
@@ -654,7 +790,8 @@ __global__ __launch_bounds__(256) void k_compose(const float* __restrict__ out, 
                                                  int H, int W, int Hp, int Wp, float4 bg,
                                                  const float* __restrict__ target, float inv_count,
                                                  float* __restrict__ color, uint16_t* __restrict__ dout,
-                                                 float* __restrict__ loss) {
+                                                 float* __restrict__ loss, float4* __restrict__ fb_col,
+                                                 uint32_t* __restrict__ fb_rgba8) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long n = (long)B * Hp * Wp;
     float se = 0.0f;
@@ -670,13 +807,22 @@ __global__ __launch_bounds__(256) void k_compose(const float* __restrict__ out, 
             float t = __half2float(__ushort_as_half(s[9]));
             float al[3] = {__half2float(__ushort_as_half(s[3])), __half2float(__ushort_as_half(s[4])),
                            __half2float(__ushort_as_half(s[5]))};
-            float v = out[i * 16];
+            const float4 o4 = reinterpret_cast<const float4*>(out)[i];  // visibility, colour residual
+            const float v = o4.x, res[3] = {o4.y, o4.z, o4.w};
             float c[3];
             bool hit = t > 0.0f;
             for (int k = 0; k < 3; ++k)
-                c[k] = hit ? al[k] * v + out[i * 16 + 1 + k] : (k == 0 ? bg.x : (k == 1 ? bg.y : bg.z));
+                c[k] = hit ? al[k] * v + res[k] : (k == 0 ? bg.x : (k == 1 ? bg.y : bg.z));
             if (color)
                 for (int k = 0; k < 3; ++k) color[q * 3 + k] = c[k];
+            if (fb_col) {  // a rendered frame: colour into the framebuffer (alpha kept), RGBA8 beside it
+                float4 f = fb_col[q];
+                f.x = c[0];
+                f.y = c[1];
+                f.z = c[2];
+                fb_col[q] = f;
+                fb_rgba8[q] = pack_rgba8(f);
+            }
             if (target) {
                 for (int k = 0; k < 3; ++k) {
                     float d = c[k] - target[q * 3 + k];
@@ -705,18 +851,6 @@ __global__ __launch_bounds__(256) void k_compose(const float* __restrict__ out, 
 }
 
 // Denoised colour back into the frame buffers.
-__global__ void k_compose_frame(const float* __restrict__ color, long P, float4* __restrict__ fb_col,
-                                uint32_t* __restrict__ fb_rgba8) {
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P) return;
-    float4 c = fb_col[i];
-    c.x = color[i * 3];
-    c.y = color[i * 3 + 1];
-    c.z = color[i * 3 + 2];
-    fb_col[i] = c;
-    fb_rgba8[i] = pack_rgba8(c);
-}
-
 // ------------------------------------------------------------------ parameters
 // Canonical fp32 layout per layer: W[t][ci][co] then bias[co]. Fragment layouts for the MFMA
 // kernels are rebuilt from it after every update.
@@ -820,6 +954,25 @@ static size_t param_offset(int l) {
 // large launches, otherwise the 16x16x32 kernel with fewer channel and pixel blocks per wave
 // as the launch shrinks); a test hook forces one: 5 = LDS-staged, else pixel blocks per wave
 // (shape % 10: 4, 2 or 1) and, for 4, channel blocks per wave (shape / 10, 0 = automatic).
+// Persistent grid of the LDS-DMA kernel: the workgroups that fit on the device at once (queried
+// once per instance), at most one per tile.
+template <int NB>
+void launch_dma(hipStream_t s, const ConvArgs& a, long tiles) {
+    static int fit = 0;
+    if (fit == 0) {
+        int per_cu = 0, dev = 0, cus = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_conv_fwd_dma<NB>, 256, 0) != hipSuccess ||
+            per_cu < 1)
+            per_cu = 1;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+            cus = 256;
+        fit = per_cu * cus;
+    }
+    const long g = tiles < fit ? tiles : fit;
+    hipLaunchKernelGGL((k_conv_fwd_dma<NB>), dim3((unsigned)g), dim3(256), 0, s, a);
+}
+
 static void conv_fwd(hipStream_t s, const ConvArgs& a, bool k32, int shape) {
     long waves = (long)a.B * a.H * ((a.W + 63) / 64);
     int NBT = a.CO / 16;
@@ -829,9 +982,19 @@ static void conv_fwd(hipStream_t s, const ConvArgs& a, bool k32, int shape) {
         // enough wavefronts to fill 256 CUs: fewer channel blocks, then fewer pixels per wave
         const long want = 6144;
         int pbk = 4;
+        // the LDS-DMA double-buffered kernel (persistent) for single-slice layers of up to 32
+        // output channels: the default for them (shape 0) and shape 6
+        if ((shape == 0 || shape == 6) && a.C1 + a.C2 <= 32 && NBT <= 2) {
+            const long tiles = (long)a.B * ((a.H + 3) / 4) * ((a.W + 63) / 64);
+            if (NBT == 1)
+                launch_dma<1>(s, a, tiles);
+            else
+                launch_dma<2>(s, a, tiles);
+            return;
+        }
         // the LDS-staged kernel for every layer (it beats the per-wavefront kernel at all four
         // resolutions); shape 5 forces it, other nonzero shapes force the per-wavefront kernel
-        const bool lds = shape == 5 || (shape == 0 && NBT <= 6);
+        const bool lds = shape == 5 || shape == 6 || (shape == 0 && NBT <= 6);
         if (lds) {
             const long blocks = (long)a.B * ((a.H + 3) / 4) * ((a.W + 63) / 64);
             switch (NBT) {
@@ -917,10 +1080,11 @@ static void conv_bwd_weight(hipStream_t s, const ConvWgtArgs& a, int cu) {
 
 
 void launch_compose(hipStream_t s, const float* out, const uint16_t* raw, int B, int H, int W, int Hp, int Wp,
-                    float4 bg, const float* target, float inv_count, float* color, uint16_t* dout, float* loss) {
+                    float4 bg, const float* target, float inv_count, float* color, uint16_t* dout, float* loss,
+                    float4* fb_col, uint32_t* fb_rgba8) {
     long n = (long)B * Hp * Wp;
     hipLaunchKernelGGL(k_compose, dim3(nblk(n)), dim3(256), 0, s, out, raw, B, H, W, Hp, Wp, bg, target, inv_count,
-                       color, dout, loss);
+                       color, dout, loss, fb_col, fb_rgba8);
 }
 void zero(hipStream_t s, float* p, long n) {
     hipLaunchKernelGGL(k_zero_f32, dim3((unsigned)std::min<long>(nblk(n), 4096)), dim3(256), 0, s, p, n);
@@ -978,7 +1142,7 @@ int Denoiser::ensure_acts(mmhrt_ctx_s* c, int B, int Hp, int Wp, bool train) {
     need += P0 * 16 + res[1] * 32 + res[2] * 48 + res[3] * 64;  // input + pooled
     int e = 0;
     if ((e = mmhrt::ensure(c, acts, need, "denoiser activations"))) return e;
-    if ((e = mmhrt::ensure(c, out, P0 * 16, "denoiser output"))) return e;
+    if ((e = mmhrt::ensure(c, out, P0 * 4, "denoiser output"))) return e;
     if (train) {
         size_t g = P0 * (16 + 288 + 32) + (P0 / 4) * 48 + (P0 / 16) * 64;
         if ((e = mmhrt::ensure(c, gbuf, g, "denoiser gradient buffers"))) return e;
@@ -1012,6 +1176,7 @@ void Denoiser::forward(hipStream_t s, int B, int Hp, int Wp) {
         A.Wt = wf.p + wo; A.Wt32 = wf32.p + w32; A.Wl = wl.p + wlo;
         A.bias = params.p + po + 9L * (L.c1 + L.c2) * L.co;
         A.Y = y; A.Yf = yf; A.relu = L.relu;
+        A.YfC = yf ? 4 : 0;  // the output layer: only channels 0-3 (visibility, colour residual) are used
         conv_fwd(s, A, k32, conv_shape);
         wo += 9L * (L.c1 + L.c2) * L.co;
         w32 += (size_t)((9 * (L.c1 + L.c2) + 31) / 32) * 32 * L.co;
@@ -1134,13 +1299,16 @@ int dn_check(mmhrt_ctx c, bool need_ready) {
 inline int ceil8(int v) { return (v + 7) & ~7; }
 
 // Features (already in dn->raw) -> network input -> forward -> composed colour in dn->color.
+// fb_col / fb_rgba8: compose straight into a rendered frame's framebuffer (B = 1) instead of the
+// colour buffer
 int dn_run(mmhrt_ctx c, int B, int H, int W, float depth_scale, const float bg[4], const float* target, float* loss,
-           bool train) {
+           bool train, float4* fb_col = nullptr, uint32_t* fb_rgba8 = nullptr) {
     Denoiser& d = *c->dn;
     int Hp = ceil8(H), Wp = ceil8(W);
     if (int e = d.ensure_acts(c, B, Hp, Wp, train)) return e;
     size_t P = (size_t)B * H * W, Pp = (size_t)B * Hp * Wp;
-    if (int e = mmhrt::ensure(c, d.color, P * 3, "denoiser colour")) return e;
+    if (!fb_col)
+        if (int e = mmhrt::ensure(c, d.color, P * 3, "denoiser colour")) return e;
     hipStream_t s = c->stream;
     float inv_depth = depth_scale > 0.0f ? 1.0f / depth_scale : 1.0f;
     hipLaunchKernelGGL(mmhrt::dn::k_pack_input, dim3((unsigned)((Pp + 255) / 256)), dim3(256), 0, s, d.raw.p, B, H,
@@ -1150,8 +1318,9 @@ int dn_run(mmhrt_ctx c, int B, int H, int W, float depth_scale, const float bg[4
     d.forward(s, B, Hp, Wp);
     float4 bgc = make_float4(bg ? bg[0] : 0.f, bg ? bg[1] : 0.f, bg ? bg[2] : 0.f, bg ? bg[3] : 1.f);
     if (loss) mmhrt::launch_fill_u32(s, reinterpret_cast<uint32_t*>(d.scalar.p), 0u, 1);
-    mmhrt::dn::launch_compose(s, d.out.p, d.raw.p, B, H, W, Hp, Wp, bgc, target, 1.0f / (float)(P * 3), d.color.p,
-                              train ? d.gbuf.p : nullptr, loss ? d.scalar.p : nullptr);
+    mmhrt::dn::launch_compose(s, d.out.p, d.raw.p, B, H, W, Hp, Wp, bgc, target, 1.0f / (float)(P * 3),
+                              fb_col ? nullptr : d.color.p, train ? d.gbuf.p : nullptr, loss ? d.scalar.p : nullptr,
+                              fb_col, fb_rgba8);
     return 0;
 }
 }  // namespace
@@ -1224,9 +1393,8 @@ int32_t mmhrt_denoise(mmhrt_ctx c, float depth_scale) {
     if (int e = mmhrt::ensure(c, d.raw, P * 12, "denoiser features")) return e;
     hipLaunchKernelGGL(mmhrt::dn::k_frame_features, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, c->stream,
                        c->fb_col.p, c->fb_aux.p, (long)P, d.raw.p);
-    if (int e = dn_run(c, 1, H, W, depth_scale, c->last_bg, nullptr, nullptr, false)) return e;
-    hipLaunchKernelGGL(mmhrt::dn::k_compose_frame, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, c->stream,
-                       d.color.p, (long)P, c->fb_col.p, c->fb_rgba8.p);
+    if (int e = dn_run(c, 1, H, W, depth_scale, c->last_bg, nullptr, nullptr, false, c->fb_col.p, c->fb_rgba8.p))
+        return e;
     hipError_t he = hipGetLastError();
     return he == hipSuccess ? MMHRT_OK : mmhrt::hip_fail(c, he, "denoise launch");
 }

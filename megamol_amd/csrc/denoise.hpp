@@ -17,7 +17,8 @@ struct LayerSpec {
 extern const LayerSpec kLayers[kNumLayers];
 size_t param_count();
 void launch_compose(hipStream_t s, const float* out, const uint16_t* raw, int B, int H, int W, int Hp, int Wp,
-                    float4 bg, const float* target, float inv_count, float* color, uint16_t* dout, float* loss);
+                    float4 bg, const float* target, float inv_count, float* color, uint16_t* dout, float* loss,
+                    float4* fb_col, uint32_t* fb_rgba8);
 void zero(hipStream_t s, float* p, long n);
 void adam(hipStream_t s, float* p, const float* g, float* m, float* v, long n, float lr, float b1, float b2, float eps,
           float c1, float c2);
@@ -34,6 +35,7 @@ struct ConvArgs {
     uint16_t* Y;
     float* Yf;
     int relu;
+    int YfC;  // channels written to Yf, which is also its pixel stride (0: all CO)
 };
 struct ConvBwdArgs {
     const uint16_t* dY;
