@@ -54,6 +54,77 @@ def trace_kernel_label(r, spp, mode):
     return f"{name} ({mode}, {spp} spp)"
 
 
+# Traversal ablation: one choice changed at a time from the first row of each frame (every
+# variant renders the same image bit for bit). Frames: the 64-spp AO frame and the denoised frame's 1-spp
+# AO frame with the feature buffer.
+ABLATION = {
+    "64spp": [
+        ("sample lanes, work stealing, LDS stack 16, 8x8 tiles", {"stack_lds": 16, "refill": 0}),
+        ("LDS stack 8 entries", {"stack_lds": 8, "refill": 0}),
+        ("LDS stack 24 entries (library default)", {"stack_lds": 24, "refill": 0}),
+        ("LDS stack 32 entries", {"stack_lds": 32, "refill": 0}),
+        ("4x16 tiles", {"stack_lds": 16, "refill": 0, "tile_w": 4}),
+        ("lane per pixel (samples in a loop)", {"stack_lds": 16, "refill": 0, "sample_lanes": 0}),
+        ("lane per pixel, static tile order (no work stealing)",
+         {"stack_lds": 16, "refill": 0, "sample_lanes": 0, "persistent": 0}),
+        ("lane per pixel + ballot compaction (refill at 32 idle lanes)",
+         {"stack_lds": 16, "sample_lanes": 0, "refill": 32}),
+        ("sample lanes, 2 workgroups per CU", {"stack_lds": 16, "refill": 0, "blocks_per_cu": 2}),
+    ],
+    "1spp": [
+        ("lane per pixel, work stealing, LDS stack 16, 8x8 tiles, no compaction", {"stack_lds": 16, "refill": 0}),
+        ("static tile order (no work stealing)", {"stack_lds": 16, "refill": 0, "persistent": 0}),
+        ("ballot compaction (refill at 16 idle lanes; library default)", {"stack_lds": 16, "refill": 16}),
+        ("ballot compaction (refill at 32 idle lanes)", {"stack_lds": 16, "refill": 32}),
+        ("LDS stack 8 entries", {"stack_lds": 8, "refill": 0}),
+        ("LDS stack 32 entries", {"stack_lds": 32, "refill": 0}),
+        ("4 workgroups per CU", {"stack_lds": 16, "refill": 0, "blocks_per_cu": 4}),
+    ],
+}
+
+
+def run_ablation(r, cam, args):
+    """Trace time (HIP events) and Mrays/s of each ABLATION variant on one LBVH of the scene."""
+    from megamol_amd import _lib as L, params
+    keys = sorted({k for fr in ABLATION.values() for _, o in fr for k in o})
+    defaults = {k: r.get_option(k) for k in keys}
+    r.build(L.BUILD_REBUILD, args.bits)
+    frames = {
+        "64spp": params(args.width, args.height, spp=args.spp, mode=L.MODE_AO, ao_distance=args.ao_distance,
+                        background=(0.0, 0.0, 0.0, 1.0), flags=L.FLAG_NO_DOWNLOAD),
+        "1spp": params(args.width, args.height, spp=1, mode=L.MODE_AO, ao_distance=args.ao_distance,
+                       background=(0.0, 0.0, 0.0, 1.0), flags=L.FLAG_AUX | L.FLAG_NO_DOWNLOAD),
+    }
+    rows = []
+    for fname, variants in ABLATION.items():
+        p = frames[fname]
+        r.set_option("count_visits", 1)
+        r.render_async(cam, p)
+        _, _, rays = r.counters()
+        r.set_option("count_visits", 0)
+        ref_id = r.download(args.width, args.height, outputs=("hit_id",)).hit_id.copy()
+        for name, opts in variants:
+            for k in keys:
+                r.set_option(k, defaults[k])
+            for k, v in opts.items():
+                r.set_option(k, v)
+            r.render_async(cam, p)
+            r.sync()
+            same = bool(np.array_equal(r.download(args.width, args.height, outputs=("hit_id",)).hit_id, ref_id))
+            tr = []
+            for _ in range(args.steps):
+                r.render_async(cam, p)
+                tr.append(r.timings()["trace"])
+            ms = float(np.median(tr))
+            rows.append({"frame": fname, "variant": name, "options": opts, "trace_ms": round(ms, 3),
+                         "mrays_per_s": round(rays / (ms * 1e-3) / 1e6, 1), "same_hit_ids": same})
+            log(f"{fname} | {name}: {ms:.3f} ms, {rays / (ms * 1e-3) / 1e6:.0f} Mrays/s")
+    for k in keys:
+        r.set_option(k, defaults[k])
+    print(json.dumps({"ablation": rows, "workload": f"{args.config}, {args.width}x{args.height}, AO distance "
+                                                 f"{args.ao_distance}"}), flush=True)
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -300,6 +371,9 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=8)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--option", action="append", default=[], help="key=value library option")
+    ap.add_argument("--ablation", action="store_true",
+                    help="only trace the 64-spp and 1-spp AO frames once per traversal variant (one choice "
+                         "changed at a time) and print their Mrays/s")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB five times (calibrates the HBM "
                          "byte counters of a profiler against a kernel of known traffic)")
@@ -346,6 +420,10 @@ def main():
     p = params(args.width, args.height, spp=args.spp, mode=mode, ao_distance=args.ao_distance,
                background=(0.0, 0.0, 0.0, 1.0), flags=L.FLAG_NO_DOWNLOAD)
     cam = scene["cam"]
+    if args.ablation:
+        run_ablation(r, cam, args)
+        r.close()
+        return
 
     def frame():
         r.build(L.BUILD_REBUILD, args.bits)

@@ -939,7 +939,8 @@ int render_frame(mmhrt_ctx_s* c, const FrameConsts& fc) {
     rec(c, 6);
     bool steal = c->opt_persistent != 0;
     bool count = c->opt_count != 0;
-    uint32_t refill = (uint32_t)std::max<int64_t>(0, std::min<int64_t>(64, c->opt_refill));
+    const int64_t rq = c->opt_refill < 0 ? (fc.spp == 1 ? 16 : 0) : c->opt_refill;
+    uint32_t refill = (uint32_t)std::max<int64_t>(0, std::min<int64_t>(64, rq));
     // sample-parallel lanes: Lambert/AO with spp a power of two up to 64 or a multiple of 64
     const bool sl_shape = fc.spp > 1 && ((fc.spp >= 64 && fc.spp % 64 == 0) || (fc.spp < 64 && 64 % fc.spp == 0));
     if (c->opt_sample_lanes != 0 && refill == 0 && fc.mode != 2 && sl_shape) {

@@ -15,6 +15,9 @@ VARIANTS = [dict(stack_lds=16, persistent=1, refill=0), dict(stack_lds=8, persis
             dict(stack_lds=24, persistent=1, refill=64)]
 
 
+DEFAULTS = dict(stack_lds=24, persistent=1, refill=-1)  # the library's (refill -1: 16 at 1 spp, else 0)
+
+
 def set_opts(r, opts):
     for k, v in opts.items():
         r.set_option(k, v)
@@ -36,7 +39,7 @@ def test_variants_bit_identical(gpu_renderer, mode):
             g.aux = gpu_renderer.download(101, 67, outputs=(), aux=True).aux
             assert_frames_equal(g, o, aux=True, what=str(v))
     finally:
-        set_opts(gpu_renderer, VARIANTS[0])
+        set_opts(gpu_renderer, DEFAULTS)
 
 
 @pytest.mark.parametrize("spp", [2, 4, 64, 128])
@@ -84,7 +87,7 @@ def test_window_is_a_crop_of_the_full_frame(gpu_renderer):
             np.testing.assert_array_equal(win.hit_id, full.hit_id[y0:y0 + h, x0:x0 + w])
             np.testing.assert_array_equal(win.color, full.color[y0:y0 + h, x0:x0 + w])
             np.testing.assert_array_equal(win.depth, full.depth[y0:y0 + h, x0:x0 + w])
-    gpu_renderer.set_option("refill", 0)
+    gpu_renderer.set_option("refill", -1)
     p = params(120, 90)
     p.window[:] = [100, 0, 40, 10]
     with pytest.raises(L.MMHRTError) as e:
