@@ -128,6 +128,13 @@ int32_t mmhrt_create(int32_t device, mmhrt_ctx* out) {
     if (c->cu_count <= 0) c->cu_count = 1;
     for (int i = 0; i < 12; ++i) hipEventCreate(&c->tm.ev[i]);
     c->tm.init = true;
+    // pinned word through which a build tells the next one how many straddler rounds it needed
+    if (hipHostMalloc((void**)&c->rounds_host, sizeof(uint32_t), hipHostMallocDefault) == hipSuccess)
+        *c->rounds_host = 0;
+    else {
+        (void)hipGetLastError();
+        c->rounds_host = nullptr;
+    }
     *out = c;
     return MMHRT_OK;
 }
@@ -143,6 +150,8 @@ int32_t mmhrt_destroy(mmhrt_ctx c) {
     release(c->partial); release(c->info); release(c->tf_buf);
     release(c->fb_col); release(c->fb_depth); release(c->fb_t); release(c->fb_id); release(c->fb_rgba8);
     release(c->fb_aux); release(c->spill); release(c->counters);
+    if (c->rounds_host) (void)hipHostFree(c->rounds_host);
+    c->rounds_host = nullptr;
     if (c->dn) {
         c->dn->release_all();
         delete c->dn;

@@ -91,12 +91,13 @@ struct mmhrt_ctx_s {
     mmhrt::DevBuf<mmhrt::Node> nodes;
     mmhrt::DevBuf<uint32_t> flags;   // arrivals at the global meeting places (generation numbers)
     uint32_t build_gen = 0;          // generation number of the last hierarchy pass
+    uint32_t* rounds_host = nullptr; // pinned: straddler kernel rounds the last build needed
     mmhrt::DevBuf<mmhrt::Rec4> rec4;  // 4-wide records (valid while rec4_valid)
     bool rec4_valid = false;
     mmhrt::DevBuf<uint32_t> rec4_list;  // collapse frontiers (two halves)
     mmhrt::DevBuf<uint32_t> rec4_cnt;
     mmhrt::DevBuf<mmhrt::Node> meet;     // per split position: the records of its two children (build scratch)
-    mmhrt::DevBuf<int2> meet_far;  // per split position: the children's far range ends
+    mmhrt::DevBuf<int4> meet_far;  // per split position: the children's far range ends and parent deltas
     mmhrt::DevBuf<uint32_t> box_cnt;  // worklist sizes of the bottom-up box rounds
     mmhrt::DevBuf<float> partial;
     mmhrt::DevBuf<mmhrt::SceneInfo> info;

@@ -677,20 +677,40 @@ __global__ __launch_bounds__(256) void k_bias_grad(const uint16_t* __restrict__ 
 }
 
 // ------------------------------------------------------------------ pooling / concat split
+// 2x2 average pooling, 8 channels per thread (C is a multiple of 8): four 16-byte loads, one
+// 16-byte store; each channel is ((x00 + x01) + x10) + x11, times 0.25.
 __global__ void k_avgpool(const uint16_t* __restrict__ X, uint16_t* __restrict__ Y, int B, int H, int W, int C) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    int Ho = H / 2, Wo = W / 2;
-    long n = (long)B * Ho * Wo * C;
+    int Ho = H / 2, Wo = W / 2, C8 = C / 8;
+    long n = (long)B * Ho * Wo * C8;
     if (i >= n) return;
-    int c = (int)(i % C);
-    long r = i / C;
+    int c = (int)(i % C8) * 8;
+    long r = i / C8;
     int xo = (int)(r % Wo);
     r /= Wo;
     int yo = (int)(r % Ho);
     int b = (int)(r / Ho);
     const uint16_t* s = X + (((size_t)b * H + 2 * yo) * W + 2 * xo) * C + c;
-    float v = bf2f(s[0]) + bf2f(s[C]) + bf2f(s[(size_t)W * C]) + bf2f(s[(size_t)W * C + C]);
-    Y[i] = f2bf(0.25f * v);
+    const uint4 q00 = *reinterpret_cast<const uint4*>(s);
+    const uint4 q01 = *reinterpret_cast<const uint4*>(s + C);
+    const uint4 q10 = *reinterpret_cast<const uint4*>(s + (size_t)W * C);
+    const uint4 q11 = *reinterpret_cast<const uint4*>(s + (size_t)W * C + C);
+    const uint32_t a00[4] = {q00.x, q00.y, q00.z, q00.w}, a01[4] = {q01.x, q01.y, q01.z, q01.w};
+    const uint32_t a10[4] = {q10.x, q10.y, q10.z, q10.w}, a11[4] = {q11.x, q11.y, q11.z, q11.w};
+    uint32_t o[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        uint32_t w = 0;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int sh = 16 * h;
+            const float v = bf2f((uint16_t)(a00[k] >> sh)) + bf2f((uint16_t)(a01[k] >> sh)) +
+                            bf2f((uint16_t)(a10[k] >> sh)) + bf2f((uint16_t)(a11[k] >> sh));
+            w |= (uint32_t)f2bf(0.25f * v) << sh;
+        }
+        o[k] = w;
+    }
+    *reinterpret_cast<uint4*>(Y + (((size_t)b * Ho + yo) * Wo + xo) * C + c) = make_uint4(o[0], o[1], o[2], o[3]);
 }
 
 // dX(full) += dPooled / 4 spread over each 2x2 block (accumulate into an existing gradient)
@@ -1184,7 +1204,7 @@ void Denoiser::forward(hipStream_t s, int B, int Hp, int Wp) {
         po += 9L * (L.c1 + L.c2) * L.co + L.co;
     };
     auto pl = [&](const uint16_t* x, uint16_t* y, int lv, int C) {
-        long n = (long)B * (Hs[lv] / 2) * (Ws[lv] / 2) * C;
+        long n = (long)B * (Hs[lv] / 2) * (Ws[lv] / 2) * (C / 8);
         hipLaunchKernelGGL(k_avgpool, dim3(nblk(n)), dim3(256), 0, s, x, y, B, Hs[lv], Ws[lv], C);
     };
     conv(0, x0, nullptr, 0, a[0], nullptr);

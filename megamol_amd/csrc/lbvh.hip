@@ -676,6 +676,7 @@ __device__ __forceinline__ int split_delta(const K* __restrict__ keys, int64_t n
 // reads gen back) without clearing the 4n-byte array between builds; it starts as all ones.
 constexpr int BOX_ROUNDS = 32;        // kernel rounds before the single-workgroup finish
 constexpr int GEN_SLOT = BOX_ROUNDS + 3;  // box_cnt slot holding the generation number
+constexpr int ROUNDS_SLOT = BOX_ROUNDS + 2;  // box_cnt slot: kernel rounds this build needed
 constexpr uint32_t BOX_SMALL = 512;   // worklist size handed to the single workgroup
 
 __device__ __forceinline__ void leaf_box(const float4 s, float b[6]) {
@@ -692,16 +693,20 @@ __device__ __forceinline__ void store_node(Node* nd, const float f[12], int2 k, 
 }
 
 // One child's record at a global meeting place: box in slot `side` of a,b,c; d = (ref0, ref1,
-// height0, height1); far[side] = the child's range end away from the split.
-__device__ __forceinline__ void meet_put(Node* mt, int2* mf, int side, const float b[6], int32_t ref, int h,
-                                         int32_t far) {
+// height0, height1); mf = (far0, far1, dx0, dx1) with far[side] the child's range end away from
+// the split and dx[side] the one delta the merged parent needs from this child (delta(l - 1) of a
+// left child, delta(r) of a right child), so that merging reads no keys.
+__device__ __forceinline__ void meet_put(Node* mt, int4* mf, int side, const float b[6], int32_t ref, int h,
+                                         int32_t far, int dx) {
     float* f = reinterpret_cast<float*>(mt) + side * 6;
 #pragma unroll
     for (int a = 0; a < 6; ++a) f[a] = b[a];
     int* d = reinterpret_cast<int*>(&mt->d);
     d[side] = ref;
     d[2 + side] = h;
-    reinterpret_cast<int*>(mf)[side] = far;
+    int* m = reinterpret_cast<int*>(mf);
+    m[side] = far;
+    m[2 + side] = dx;
 }
 
 // Block-aggregated worklist append: up to 256 entries gathered in LDS, one global atomic per
@@ -732,7 +737,7 @@ struct SplitList {
 // The chunk pass, one thread per leaf; n >= 2.
 template <typename K, int BC>
 __global__ __launch_bounds__(BC) void k_bu_chunk(const K* __restrict__ keys, const float4* __restrict__ leaf_sph,
-                                                 int64_t n, Node* __restrict__ nodes, Node* meet, int2* meet_far,
+                                                 int64_t n, Node* __restrict__ nodes, Node* meet, int4* meet_far,
                                                  uint32_t* flags, uint32_t* list, uint32_t* cnt, SceneInfo* info) {
     __shared__ float s_box[BC][12];  // the two child boxes parked at each in-chunk split
     __shared__ int32_t s_ref[BC][2];
@@ -779,7 +784,7 @@ __global__ __launch_bounds__(BC) void k_bu_chunk(const K* __restrict__ keys, con
             const int side = left ? 0 : 1;
             const int32_t far = (int32_t)(left ? l : r);
             if (s < cs || s + 1 >= ce) {  // the split lies on a chunk boundary
-                meet_put(meet + s, meet_far + s, side, b, ref, h, far);
+                meet_put(meet + s, meet_far + s, side, b, ref, h, far, left ? dl : dr);
                 if (atomicExch(flags + s, gen) == gen) ap.push((uint32_t)s, list, cnt + 1);
                 break;
             }
@@ -823,7 +828,9 @@ __global__ __launch_bounds__(BC) void k_bu_chunk(const K* __restrict__ keys, con
         float b[6];
 #pragma unroll
         for (int a = 0; a < 6; ++a) b[a] = s_box[slot][side * 6 + a];
-        meet_put(meet + s, meet_far + s, side, b, s_ref[slot][side], s_h[slot][side], s_far[slot][side]);
+        const int64_t fr = s_far[slot][side];  // delta(l - 1) of a left child, delta(r) of a right one
+        meet_put(meet + s, meet_far + s, side, b, s_ref[slot][side], s_h[slot][side], (int32_t)fr,
+                 s_delta[side == 0 ? fr - cs : fr + 1 - cs]);
         if (atomicExch(flags + s, gen) == gen) ap.push((uint32_t)s, list, cnt + 1);
     }
     ap.flush(list, cnt + 1);
@@ -852,7 +859,7 @@ __global__ __launch_bounds__(BC) __attribute__((amdgpu_waves_per_eu(8, 8))) void
     const K* __restrict__ keys, const uint32_t* __restrict__ ids,
                                                     const float4* __restrict__ sph, float4* __restrict__ leaf_sph,
                                                     uint32_t* __restrict__ leaf_id, int write_ids, int64_t n64,
-                                                    Node* __restrict__ nodes, Node* meet, int2* meet_far,
+                                                    Node* __restrict__ nodes, Node* meet, int4* meet_far,
                                                     uint32_t* flags, uint32_t* list, uint32_t* cnt, SceneInfo* info) {
     __shared__ float s_box[BC][12];     // the two child boxes parked at each in-chunk split
     __shared__ uint16_t s_ref[BC][2];   // chunk-relative child references
@@ -950,7 +957,7 @@ __global__ __launch_bounds__(BC) __attribute__((amdgpu_waves_per_eu(8, 8))) void
             const int side = left ? 0 : 1;
             const int32_t far = left ? l : r;
             if (sp < cs || sp + 1 >= ce) {  // a boundary split: sp = cs - 1 or ce - 1
-                meet_put(meet + sp, meet_far + sp, side, b, ref, h, far);
+                meet_put(meet + sp, meet_far + sp, side, b, ref, h, far, left ? dl : dr);
                 s_pend[sp < cs ? 0 : 1] = sp;
                 return;
             }
@@ -1027,7 +1034,7 @@ __global__ __launch_bounds__(BC) __attribute__((amdgpu_waves_per_eu(8, 8))) void
 #pragma unroll
             for (int a = 0; a < 6; ++a) b[a] = s_box[slot][side * 6 + a];
             meet_put(meet + sp, meet_far + sp, side, b, gref(s_ref[slot][side]), s_h[slot][side],
-                     cs + (int32_t)s_far[slot][side]);
+                     cs + (int32_t)s_far[slot][side], s_dx[slot][side]);
             if (atomicExch(flags + sp, gen) == gen) ap.push((uint32_t)sp, list, cnt + 1);
         }
         __syncthreads();
@@ -1047,11 +1054,11 @@ __global__ __launch_bounds__(BC) __attribute__((amdgpu_waves_per_eu(8, 8))) void
 // Returns the parent's split when this delivery completes it, else -1.
 template <typename K, bool VOL>
 __device__ __forceinline__ int64_t bu_step(const K* __restrict__ keys, int64_t n, Node* nodes, Node* meet,
-                                           int2* meet_far, uint32_t* flags, int64_t s, SceneInfo* info,
+                                           int4* meet_far, uint32_t* flags, int64_t s, SceneInfo* info,
                                            uint32_t gen) {
     float f[12];
     int dv[4];
-    int fr[2];
+    int fr[4];  // far0, far1, dx0, dx1
     if constexpr (VOL) {
         const volatile float* src = reinterpret_cast<const volatile float*>(meet + s);
 #pragma unroll
@@ -1060,17 +1067,19 @@ __device__ __forceinline__ int64_t bu_step(const K* __restrict__ keys, int64_t n
 #pragma unroll
         for (int q = 0; q < 4; ++q) dv[q] = di[q];
         const volatile int* fi = reinterpret_cast<const volatile int*>(meet_far + s);
-        fr[0] = fi[0];
-        fr[1] = fi[1];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) fr[q] = fi[q];
     } else {
         const float* src = reinterpret_cast<const float*>(meet + s);
 #pragma unroll
         for (int q = 0; q < 12; ++q) f[q] = src[q];
         int4 d = meet[s].d;
         dv[0] = d.x; dv[1] = d.y; dv[2] = d.z; dv[3] = d.w;
-        int2 ff = meet_far[s];
+        int4 ff = meet_far[s];
         fr[0] = ff.x;
         fr[1] = ff.y;
+        fr[2] = ff.z;
+        fr[3] = ff.w;
     }
     const int2 kd = make_int2(dv[0], dv[1]);
     const int h = 1 + (dv[2] > dv[3] ? dv[2] : dv[3]);
@@ -1086,17 +1095,17 @@ __device__ __forceinline__ int64_t bu_step(const K* __restrict__ keys, int64_t n
         b[a] = fminf(f[a], f[6 + a]);
         b[3 + a] = fmaxf(f[3 + a], f[9 + a]);
     }
-    const int dl = split_delta(keys, n, l - 1), dr = split_delta(keys, n, r);
+    const int dl = fr[2], dr = fr[3];  // delta(l - 1), delta(r), carried by the children
     const bool left = dr > dl;
     const int64_t idx = left ? r : l;
     store_node(nodes + idx, f, kd, h);
     const int64_t s2 = left ? r : l - 1;
-    meet_put(meet + s2, meet_far + s2, left ? 0 : 1, b, (int32_t)idx, h, (int32_t)(left ? l : r));
+    meet_put(meet + s2, meet_far + s2, left ? 0 : 1, b, (int32_t)idx, h, (int32_t)(left ? l : r), left ? dl : dr);
     return atomicExch(flags + s2, gen) == gen ? s2 : -1;
 }
 
 template <typename K>
-__device__ void bu_finish(const K* __restrict__ keys, int64_t n, Node* nodes, Node* meet, int2* meet_far,
+__device__ void bu_finish(const K* __restrict__ keys, int64_t n, Node* nodes, Node* meet, int4* meet_far,
                           uint32_t* flags, uint32_t* la, uint32_t* lb, uint32_t n_cur, uint32_t* cnt, SceneInfo* info) {
     __shared__ uint32_t s_out;
     const uint32_t gen = cnt[GEN_SLOT];
@@ -1127,7 +1136,7 @@ __device__ void bu_finish(const K* __restrict__ keys, int64_t n, Node* nodes, No
 // Round r: worklist r lives in list[r & 1], its size in cnt[1 + r]; cnt[0] marks completion.
 template <typename K>
 __global__ __launch_bounds__(256) void k_bu_round(const K* __restrict__ keys, int64_t n, Node* nodes, Node* meet,
-                                                  int2* meet_far, uint32_t* flags, uint32_t* l0, uint32_t* l1,
+                                                  int4* meet_far, uint32_t* flags, uint32_t* l0, uint32_t* l1,
                                                   uint32_t* cnt, int r, SceneInfo* info) {
     __shared__ uint32_t s_list[256];
     __shared__ uint32_t s_n, s_base;
@@ -1137,7 +1146,10 @@ __global__ __launch_bounds__(256) void k_bu_round(const K* __restrict__ keys, in
     uint32_t* lin = (r & 1) ? l1 : l0;
     uint32_t* lout = (r & 1) ? l0 : l1;
     if (n_in <= BOX_SMALL) {
-        if (blockIdx.x == 0) bu_finish<K>(keys, n, nodes, meet, meet_far, flags, lin, lout, n_in, cnt, info);
+        if (blockIdx.x == 0) {
+            if (threadIdx.x == 0) cnt[ROUNDS_SLOT] = (uint32_t)r;
+            bu_finish<K>(keys, n, nodes, meet, meet_far, flags, lin, lout, n_in, cnt, info);
+        }
         return;
     }
     SplitList ap{s_list, &s_n, &s_base};
@@ -1156,13 +1168,13 @@ __global__ __launch_bounds__(256) void k_bu_round(const K* __restrict__ keys, in
     }
 }
 
-// Whatever is left after BOX_ROUNDS kernel rounds: one workgroup finishes it.
+// Whatever is left after the r kernel rounds launched: one workgroup finishes it.
 template <typename K>
 __global__ __launch_bounds__(256) void k_bu_tail(const K* __restrict__ keys, int64_t n, Node* nodes, Node* meet,
-                                                 int2* meet_far, uint32_t* flags, uint32_t* l0, uint32_t* l1,
-                                                 uint32_t* cnt, SceneInfo* info) {
+                                                 int4* meet_far, uint32_t* flags, uint32_t* l0, uint32_t* l1,
+                                                 uint32_t* cnt, int r, SceneInfo* info) {
     if (cnt[0]) return;
-    int r = BOX_ROUNDS;
+    if (threadIdx.x == 0) cnt[ROUNDS_SLOT] = (uint32_t)r;
     uint32_t* lin = (r & 1) ? l1 : l0;
     uint32_t* lout = (r & 1) ? l0 : l1;
     bu_finish<K>(keys, n, nodes, meet, meet_far, flags, lin, lout, cnt[1 + r], cnt, info);
@@ -1340,11 +1352,20 @@ int hierarchy(mmhrt_ctx_s* c, const K* keys, const uint32_t* ids, bool write_ids
     else
         hipLaunchKernelGGL((k_bu_chunk<K, 1024>), dim3(blocks_for(n, 1024)), dim3(1024), 0, s, keys, c->leaf_sph.p,
                            (int64_t)n, c->nodes.p, c->meet.p, c->meet_far.p, c->flags.p, l0, c->box_cnt.p, c->info.p);
-    for (int r = 0; r < BOX_ROUNDS; ++r)
+    // kernel rounds: as many as the last build needed plus two (the count comes back through
+    // pinned host memory one build late; the tail finishes whatever is left, so it is only a hint)
+    int rounds = BOX_ROUNDS;
+    if (c->rounds_host) {
+        const int last = (int)*(volatile uint32_t*)c->rounds_host;
+        if (last > 0 && last + 2 < BOX_ROUNDS) rounds = last + 2;
+    }
+    for (int r = 0; r < rounds; ++r)
         hipLaunchKernelGGL(k_bu_round<K>, dim3(1024), dim3(256), 0, s, keys, (int64_t)n, c->nodes.p, c->meet.p,
                            c->meet_far.p, c->flags.p, l0, l1, c->box_cnt.p, r, c->info.p);
     hipLaunchKernelGGL(k_bu_tail<K>, dim3(1), dim3(256), 0, s, keys, (int64_t)n, c->nodes.p, c->meet.p, c->meet_far.p,
-                       c->flags.p, l0, l1, c->box_cnt.p, c->info.p);
+                       c->flags.p, l0, l1, c->box_cnt.p, rounds, c->info.p);
+    if (c->rounds_host)
+        (void)hipMemcpyAsync(c->rounds_host, c->box_cnt.p + ROUNDS_SLOT, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
     return 0;
 }
 
