@@ -788,20 +788,47 @@ __global__ void k_pack_input(const uint16_t* __restrict__ raw, int B, int H, int
     for (int k = 0; k < 16; ++k) d[k] = f2bf(f[k]);
 }
 
-// Frame -> raw features (device path of the renderer output).
-__global__ void k_frame_features(const float4* __restrict__ col, const __half* __restrict__ aux, long P,
-                                 uint16_t* __restrict__ raw) {
+// Frame -> raw features and network input in one pass (device path of the renderer output): raw
+// = colour (as half), the 8 feature halfs, 0; the input channels are those k_pack_input makes of
+// raw (colour rounded to half first, then every channel to bf16). 16-byte loads, 8/16-byte stores.
+__global__ void k_frame_input(const float4* __restrict__ col, const __half* __restrict__ aux, int H, int W, int Hp,
+                              int Wp, float inv_depth, uint16_t* __restrict__ raw, uint16_t* __restrict__ X0) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P) return;
-    float4 c = col[i];
-    const uint16_t* a = reinterpret_cast<const uint16_t*>(aux) + i * 8;
-    uint16_t* d = raw + i * 12;
-    d[0] = __half_as_ushort(__float2half(c.x));
-    d[1] = __half_as_ushort(__float2half(c.y));
-    d[2] = __half_as_ushort(__float2half(c.z));
-    for (int k = 0; k < 8; ++k) d[3 + k] = a[k];
-    d[11] = 0;
+    long n = (long)Hp * Wp;
+    if (i >= n) return;
+    const int x = (int)(i % Wp), y = (int)(i / Wp);
+    uint32_t o[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+    if (x < W && y < H) {
+        const long p = (long)y * W + x;
+        const float4 c = col[p];
+        const uint4 q = reinterpret_cast<const uint4*>(aux)[p];  // 8 halfs: albedo 3, normal 3, t, coverage
+        const uint16_t h[8] = {(uint16_t)q.x, (uint16_t)(q.x >> 16), (uint16_t)q.y, (uint16_t)(q.y >> 16),
+                               (uint16_t)q.z, (uint16_t)(q.z >> 16), (uint16_t)q.w, (uint16_t)(q.w >> 16)};
+        const uint16_t c0 = __half_as_ushort(__float2half(c.x)), c1 = __half_as_ushort(__float2half(c.y)),
+                       c2 = __half_as_ushort(__float2half(c.z));
+        uint2* d = reinterpret_cast<uint2*>(raw + p * 12);  // raw: colour 3, aux 8, 0
+        d[0] = make_uint2((uint32_t)c0 | ((uint32_t)c1 << 16), (uint32_t)c2 | ((uint32_t)h[0] << 16));
+        d[1] = make_uint2((uint32_t)h[1] | ((uint32_t)h[2] << 16), (uint32_t)h[3] | ((uint32_t)h[4] << 16));
+        d[2] = make_uint2((uint32_t)h[5] | ((uint32_t)h[6] << 16), (uint32_t)h[7]);
+        auto hf = [](uint16_t v) { return __half2float(__ushort_as_half(v)); };
+        const float t = hf(h[6]);
+        const bool hit = t > 0.0f;
+        float f[16];
+        f[0] = hf(h[7]);
+        f[1] = hf(h[0]); f[2] = hf(h[1]); f[3] = hf(h[2]);
+        f[4] = hf(h[3]); f[5] = hf(h[4]); f[6] = hf(h[5]);
+        f[7] = hit ? t * inv_depth : 0.0f;
+        f[8] = hf(c0); f[9] = hf(c1); f[10] = hf(c2);
+        f[11] = hit ? 1.0f : 0.0f;
+        f[12] = f[13] = f[14] = f[15] = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) o[k] = (uint32_t)f2bf(f[2 * k]) | ((uint32_t)f2bf(f[2 * k + 1]) << 16);
+    }
+    uint4* dx = reinterpret_cast<uint4*>(X0 + i * 16);
+    dx[0] = make_uint4(o[0], o[1], o[2], o[3]);
+    dx[1] = make_uint4(o[4], o[5], o[6], o[7]);
 }
+
 
 // Network output (fp32 [B,Hp,Wp,16]: v in channel 0, colour residual in 1-3) -> colour
 // albedo * v + residual on hit pixels, background elsewhere. With a target, also the squared
@@ -1322,7 +1349,7 @@ inline int ceil8(int v) { return (v + 7) & ~7; }
 // fb_col / fb_rgba8: compose straight into a rendered frame's framebuffer (B = 1) instead of the
 // colour buffer
 int dn_run(mmhrt_ctx c, int B, int H, int W, float depth_scale, const float bg[4], const float* target, float* loss,
-           bool train, float4* fb_col = nullptr, uint32_t* fb_rgba8 = nullptr) {
+           bool train, float4* fb_col = nullptr, uint32_t* fb_rgba8 = nullptr, bool packed = false) {
     Denoiser& d = *c->dn;
     int Hp = ceil8(H), Wp = ceil8(W);
     if (int e = d.ensure_acts(c, B, Hp, Wp, train)) return e;
@@ -1331,8 +1358,9 @@ int dn_run(mmhrt_ctx c, int B, int H, int W, float depth_scale, const float bg[4
         if (int e = mmhrt::ensure(c, d.color, P * 3, "denoiser colour")) return e;
     hipStream_t s = c->stream;
     float inv_depth = depth_scale > 0.0f ? 1.0f / depth_scale : 1.0f;
-    hipLaunchKernelGGL(mmhrt::dn::k_pack_input, dim3((unsigned)((Pp + 255) / 256)), dim3(256), 0, s, d.raw.p, B, H,
-                       W, Hp, Wp, inv_depth, d.x0);
+    if (!packed)  // a rendered frame arrives already packed (k_frame_input)
+        hipLaunchKernelGGL(mmhrt::dn::k_pack_input, dim3((unsigned)((Pp + 255) / 256)), dim3(256), 0, s, d.raw.p, B,
+                           H, W, Hp, Wp, inv_depth, d.x0);
     d.k32 = c->opt_dn_k32 != 0;
     d.conv_shape = (int)c->opt_dn_conv_shape;
     d.forward(s, B, Hp, Wp);
@@ -1411,9 +1439,12 @@ int32_t mmhrt_denoise(mmhrt_ctx c, float depth_scale) {
     int H = (int)c->fb_h, W = (int)c->fb_w;
     size_t P = (size_t)H * W;
     if (int e = mmhrt::ensure(c, d.raw, P * 12, "denoiser features")) return e;
-    hipLaunchKernelGGL(mmhrt::dn::k_frame_features, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, c->stream,
-                       c->fb_col.p, c->fb_aux.p, (long)P, d.raw.p);
-    if (int e = dn_run(c, 1, H, W, depth_scale, c->last_bg, nullptr, nullptr, false, c->fb_col.p, c->fb_rgba8.p))
+    const int Hp = ceil8(H), Wp = ceil8(W);
+    if (int e = d.ensure_acts(c, 1, Hp, Wp, false)) return e;
+    const float inv_depth = depth_scale > 0.0f ? 1.0f / depth_scale : 1.0f;
+    hipLaunchKernelGGL(mmhrt::dn::k_frame_input, dim3((unsigned)(((size_t)Hp * Wp + 255) / 256)), dim3(256), 0,
+                       c->stream, c->fb_col.p, c->fb_aux.p, H, W, Hp, Wp, inv_depth, d.raw.p, d.x0);
+    if (int e = dn_run(c, 1, H, W, depth_scale, c->last_bg, nullptr, nullptr, false, c->fb_col.p, c->fb_rgba8.p, true))
         return e;
     hipError_t he = hipGetLastError();
     return he == hipSuccess ? MMHRT_OK : mmhrt::hip_fail(c, he, "denoise launch");

@@ -616,15 +616,6 @@ __global__ __launch_bounds__(RB) void k_radix_onesweep(const K* __restrict__ kin
 }
 
 // ---------------------------------------------------------------- gather / hierarchy + boxes
-__global__ __launch_bounds__(256) void k_gather(const uint32_t* __restrict__ vals, const float4* __restrict__ sph,
-                                                float4* __restrict__ leaf_sph, uint32_t* __restrict__ leaf_id,
-                                                uint64_t n) {
-    uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    uint32_t v = vals[k];
-    leaf_sph[k] = sph[v];
-    leaf_id[k] = v;
-}
 
 // Common-prefix length of sorted leaves i and j: of the keys, or key bits + that of the indices
 // when the keys are equal (Karras 2012, sec. 4: duplicates ordered by index).
@@ -844,8 +835,7 @@ __global__ __launch_bounds__(BC) void k_bu_chunk(const K* __restrict__ keys, con
 // (a reference is the leaf or node index minus the chunk start, bit 15 marking a leaf).
 // n >= 2.
 // The chunk pass, level-synchronous and persistent: workgroup g takes chunks g, g + G, g + 2G, ...
-// (G = the grid) and also gathers the leaf spheres into Morton order (leaf_sph[k] = sph[ids[k]],
-// and leaf_id[k] = ids[k] when write_ids). Round 0 delivers the leaves, each later round merges
+// (G = the grid) and also gathers the leaf spheres into Morton order (leaf_sph[k] = sph[ids[k]]). Round 0 delivers the leaves, each later round merges
 // the splits that the previous round completed; completed splits are kept in compact LDS lists,
 // so the active items stay packed in the first waves. Rounds touch LDS only: deliveries to the
 // two boundary splits of a chunk write their global meeting record at once but take their
@@ -858,7 +848,7 @@ template <typename K, int BC>
 __global__ __launch_bounds__(BC) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_bu_chunk_ls(
     const K* __restrict__ keys, const uint32_t* __restrict__ ids,
                                                     const float4* __restrict__ sph, float4* __restrict__ leaf_sph,
-                                                    uint32_t* __restrict__ leaf_id, int write_ids, int64_t n64,
+                                                    int64_t n64,
                                                     Node* __restrict__ nodes, Node* meet, int4* meet_far,
                                                     uint32_t* flags, uint32_t* list, uint32_t* cnt, SceneInfo* info) {
     __shared__ float s_box[BC][12];     // the two child boxes parked at each in-chunk split
@@ -906,7 +896,6 @@ __global__ __launch_bounds__(BC) __attribute__((amdgpu_waves_per_eu(8, 8))) void
         const int32_t k = cs + tid;
         if (k < ce) {
             leaf_sph[k] = sp_cur;
-            if (write_ids) leaf_id[k] = id_cur;
         }
         // chunk set-up: delta(cs - 1 + j) for j in [0, BC] from the keys in registers
         {
@@ -1203,10 +1192,12 @@ __global__ void k_fill(uint32_t* p, uint32_t v, uint64_t n) {
 inline unsigned blocks_for(uint64_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
 // Morton codes of the current spheres into ka, then the LSD radix sort of (key, particle index)
-// over `passes` 8-bit digits; the result is in (ka, va). The Morton pass also produces the first
-// digit's tile histograms, and the first pass takes the particle indices as its values.
+// over `passes` 8-bit digits; the sorted keys end in ka, the sorted values in vfinal (the last
+// pass writes them there). The Morton pass also produces the first digit's tile histograms, and
+// the first pass takes the particle indices as its values.
 template <typename K, int RI>
-int morton_sort_t(mmhrt_ctx_s* c, K* ka, K* kb, uint32_t* va, uint32_t* vb, uint64_t n, int passes) {
+int morton_sort_t(mmhrt_ctx_s* c, K* ka, K* kb, uint32_t* va, uint32_t* vb, uint32_t* vfinal, uint64_t n,
+                  int passes) {
     constexpr int RT = RB * RI;
     uint32_t tiles = (uint32_t)((n + RT - 1) / RT);
     uint64_t hlen = (uint64_t)tiles * 256;
@@ -1243,8 +1234,9 @@ int morton_sort_t(mmhrt_ctx_s* c, K* ka, K* kb, uint32_t* va, uint32_t* vb, uint
         hipLaunchKernelGGL(k_radix_ghist<K>, dim3(gb), dim3(256), 0, c->stream, src, n, passes, ghist);
         hipLaunchKernelGGL(k_radix_gscan, dim3(1), dim3(256), 0, c->stream, ghist, passes, dstart);
         for (int p = 0; p < passes; ++p) {
-            hipLaunchKernelGGL((k_radix_onesweep<K, RI>), dim3(tiles), dim3(RB), 0, c->stream, src, vs, dst, vd, n, 8 * p,
-                               dstart + p * 256, c->hist.p + hlen * p, tickets + p, c->sort_fault.p);
+            hipLaunchKernelGGL((k_radix_onesweep<K, RI>), dim3(tiles), dim3(RB), 0, c->stream, src, vs, dst,
+                               p == passes - 1 ? vfinal : vd, n, 8 * p, dstart + p * 256, c->hist.p + hlen * p,
+                               tickets + p, c->sort_fault.p);
             src = dst;
             dst = (p & 1) ? kb : ka;
             vs = vd;
@@ -1260,22 +1252,23 @@ int morton_sort_t(mmhrt_ctx_s* c, K* ka, K* kb, uint32_t* va, uint32_t* vb, uint
         hipLaunchKernelGGL(k_col_top, dim3(1), dim3(256), 0, c->stream, c->scan_tmp.p, groups);
         hipLaunchKernelGGL(k_col_offsets, dim3(groups), dim3(256), 0, c->stream, hist, tiles, groups, c->scan_tmp.p,
                            offs);
-        hipLaunchKernelGGL((k_radix_scatter<K, RI>), dim3(tiles), dim3(RB), 0, c->stream, src, vs, dst, vd, n, shift,
-                           hist, offs);
+        hipLaunchKernelGGL((k_radix_scatter<K, RI>), dim3(tiles), dim3(RB), 0, c->stream, src, vs, dst,
+                           p == passes - 1 ? vfinal : vd, n, shift, hist, offs);
         // ping-pong: pass p writes (kb, vb) when p is even, (ka, va) when odd
         src = dst;
         dst = (p & 1) ? kb : ka;
         vs = vd;
         vd = (p & 1) ? vb : va;
     }
-    return 0;  // passes is even: result is back in (ka, va)
+    return 0;  // passes is even: the keys are back in ka
 }
 
 template <typename K>
-int morton_sort(mmhrt_ctx_s* c, K* ka, K* kb, uint32_t* va, uint32_t* vb, uint64_t n, int passes) {
-    if (c->opt_sort_ri == 8) return morton_sort_t<K, 8>(c, ka, kb, va, vb, n, passes);
-    if (c->opt_sort_ri == 32 && sizeof(K) == 4) return morton_sort_t<K, 32>(c, ka, kb, va, vb, n, passes);
-    return morton_sort_t<K, 16>(c, ka, kb, va, vb, n, passes);
+int morton_sort(mmhrt_ctx_s* c, K* ka, K* kb, uint32_t* va, uint32_t* vb, uint32_t* vfinal, uint64_t n,
+                int passes) {
+    if (c->opt_sort_ri == 8) return morton_sort_t<K, 8>(c, ka, kb, va, vb, vfinal, n, passes);
+    if (c->opt_sort_ri == 32 && sizeof(K) == 4) return morton_sort_t<K, 32>(c, ka, kb, va, vb, vfinal, n, passes);
+    return morton_sort_t<K, 16>(c, ka, kb, va, vb, vfinal, n, passes);
 }
 
 // Persistent grid of the chunk pass: as many workgroups as fit on the device at once.
@@ -1291,10 +1284,10 @@ unsigned chunk_grid(mmhrt_ctx_s* c, uint64_t n) {
 }
 
 template <typename K, int BC>
-void launch_chunk_ls(mmhrt_ctx_s* c, const K* keys, const uint32_t* ids, bool write_ids, uint32_t* list) {
+void launch_chunk_ls(mmhrt_ctx_s* c, const K* keys, const uint32_t* ids, uint32_t* list) {
     const uint64_t n = c->n;
     hipLaunchKernelGGL((k_bu_chunk_ls<K, BC>), dim3(chunk_grid<K, BC>(c, n)), dim3(BC), 0, c->stream, keys, ids,
-                       cur_spheres(c), c->leaf_sph.p, c->leaf_id.p, write_ids ? 1 : 0, (int64_t)n, c->nodes.p,
+                       cur_spheres(c), c->leaf_sph.p, (int64_t)n, c->nodes.p,
                        c->meet.p, c->meet_far.p, c->flags.p, list, c->box_cnt.p, c->info.p);
 }
 
@@ -1302,17 +1295,12 @@ void launch_chunk_ls(mmhrt_ctx_s* c, const K* keys, const uint32_t* ids, bool wr
 // the particle of leaf k (the sorted values after a build, leaf_id on a refit, which keeps the
 // tree of the same keys and takes the boxes of the current spheres).
 template <typename K>
-int hierarchy(mmhrt_ctx_s* c, const K* keys, const uint32_t* ids, bool write_ids) {
+int hierarchy(mmhrt_ctx_s* c, const K* keys, const uint32_t* ids) {
     uint64_t n = c->n;
     hipStream_t s = c->stream;
-    if (n == 1 || c->opt_build_ls == 0) {
-        if (write_ids)
-            hipLaunchKernelGGL(k_gather, dim3(blocks_for(n, 256)), dim3(256), 0, s, ids, cur_spheres(c),
-                               c->leaf_sph.p, c->leaf_id.p, n);
-        else
-            hipLaunchKernelGGL(k_regather, dim3(blocks_for(n, 256)), dim3(256), 0, s, cur_spheres(c), ids,
-                               c->leaf_sph.p, n);
-    }
+    if (n == 1 || c->opt_build_ls == 0)
+        hipLaunchKernelGGL(k_regather, dim3(blocks_for(n, 256)), dim3(256), 0, s, cur_spheres(c), ids, c->leaf_sph.p,
+                           n);
     if (n == 1) {
         hipLaunchKernelGGL(k_single, dim3(1), dim3(1), 0, s, c->leaf_sph.p, c->nodes.p, c->info.p);
         return 0;
@@ -1338,11 +1326,11 @@ int hierarchy(mmhrt_ctx_s* c, const K* keys, const uint32_t* ids, bool write_ids
     const int bc = c->opt_build_chunk == 256 ? 256 : (c->opt_build_chunk == 512 ? 512 : 1024);
     if (c->opt_build_ls != 0) {
         if (bc == 256)
-            launch_chunk_ls<K, 256>(c, keys, ids, write_ids, l0);
+            launch_chunk_ls<K, 256>(c, keys, ids, l0);
         else if (bc == 512)
-            launch_chunk_ls<K, 512>(c, keys, ids, write_ids, l0);
+            launch_chunk_ls<K, 512>(c, keys, ids, l0);
         else
-            launch_chunk_ls<K, 1024>(c, keys, ids, write_ids, l0);
+            launch_chunk_ls<K, 1024>(c, keys, ids, l0);
     } else if (bc == 256)
         hipLaunchKernelGGL((k_bu_chunk<K, 256>), dim3(blocks_for(n, 256)), dim3(256), 0, s, keys, c->leaf_sph.p,
                            (int64_t)n, c->nodes.p, c->meet.p, c->meet_far.p, c->flags.p, l0, c->box_cnt.p, c->info.p);
@@ -1374,10 +1362,11 @@ int build_typed(mmhrt_ctx_s* c, K* ka, K* kb) {
     uint64_t n = c->n;
     hipStream_t s = c->stream;
     int passes = sizeof(K) == 8 ? 8 : 4;
-    if (int e = morton_sort<K>(c, ka, kb, c->va.p, c->vb.p, n, passes)) return e;
+    if (int e = morton_sort<K>(c, ka, kb, c->va.p, c->vb.p, c->leaf_id.p, n, passes)) return e;
     rec(c, 3);
     rec(c, 4);  // the leaf gather runs inside the hierarchy pass
-    return hierarchy<K>(c, ka, c->va.p, true);
+    // the sort's last pass wrote its values (the leaf -> particle map) straight into leaf_id
+    return hierarchy<K>(c, ka, c->leaf_id.p);
 }
 
 }  // namespace
@@ -1441,8 +1430,7 @@ int refit_lbvh(mmhrt_ctx_s* c) {
     rec(c, 2);
     rec(c, 3);
     rec(c, 4);
-    int e = c->bits == 63 ? hierarchy<uint64_t>(c, c->kla.p, c->leaf_id.p, false)
-                          : hierarchy<uint32_t>(c, c->ka.p, c->leaf_id.p, false);
+    int e = c->bits == 63 ? hierarchy<uint64_t>(c, c->kla.p, c->leaf_id.p) : hierarchy<uint32_t>(c, c->ka.p, c->leaf_id.p);
     if (e) return e;
     rec(c, 5);
     hipError_t he = hipGetLastError();

@@ -92,6 +92,32 @@ def test_gradients_match_reference(dnz):
     assert np.array_equal(dnz.get_params(), p), "apply=0 must not update the parameters"
 
 
+def test_frame_path_equals_feature_path(gpu_renderer):
+    """mmhrt_denoise on a rendered frame (its features packed into the network input on the device
+    in one pass, colours composed into the framebuffer) gives the same colours, bit for bit, as
+    mmhrt_denoise_features on that frame's downloaded colour and features."""
+    from helpers import small_camera
+    from megamol_amd import _lib as L, params, scenes
+    case("denoiser frame path vs feature path")
+    dn = D.Denoiser(gpu_renderer, seed=3)
+    xyzr, rgba = scenes.random_spheres(3000, seed=5, rmin=0.01, rmax=0.04)
+    w, h = 77, 45  # not multiples of 8: the padded input border is exercised
+    cam = small_camera(w, h)
+    bg = (0.1, 0.2, 0.3, 1.0)
+    p = params(w, h, spp=1, mode=L.MODE_AO, ao_distance=0.2, flags=L.FLAG_AUX, background=bg)
+    gpu_renderer.set_spheres(xyzr, rgba)
+    gpu_renderer.build()
+    fr = gpu_renderer.render(cam, p)
+    aux = gpu_renderer.download(w, h, outputs=(), aux=True).aux
+    feats = np.zeros((h, w, 12), np.float16)
+    feats[..., 0:3] = fr.color[..., :3].astype(np.float16)
+    feats[..., 3:11] = aux
+    ref = dn.denoise(feats, 1.5, bg=bg)[0]
+    dn.denoise_frame(1.5)
+    got = gpu_renderer.download(w, h, outputs=("color",)).color[..., :3]
+    np.testing.assert_array_equal(got, ref)
+
+
 def test_training_reduces_loss_and_set_get_roundtrip(dnz, tmp_path):
     case("denoiser training smoke")
     dnz.set_params(D.Denoiser(dnz.r, seed=11).get_params())
