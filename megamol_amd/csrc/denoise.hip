@@ -378,17 +378,21 @@ __global__ __launch_bounds__(256) void k_conv_fwd_lds(ConvArgs a) {
     store_blocks<4, NB>(a, acc, bias, 0, NBT, b, y, x0, q, lr);
 }
 
-// The same convolution for single-slice layers (CI <= 32) of up to 32 output channels, with the
-// input brought into LDS by direct global->LDS loads (global_load_lds_dwordx4: 16 bytes per lane
-// into consecutive LDS slots), double buffered: a persistent workgroup walks its tiles and issues
-// the next tile's loads before it computes the current one, without holding the data in
-// registers. The layer's weights are loaded into LDS once. The tile is unpadded, and a pixel's
-// four 8-channel groups are stored XOR-swizzled (slot 4P + (g ^ ((P >> 1) & 3)) for tile pixel
-// P), so that the 8 consecutive pixels of an A-fragment read phase cover all 32 banks. Pieces
-// outside the image or past CI are zeroed with ordinary LDS stores before the loads are issued
-// and left out of them (exec mask).
-constexpr int DT_SLOTS = LT_ROWS * LT_PX * 4;    // 1584 16-byte slots of a tile
-constexpr int DT_INST = (DT_SLOTS + 63) / 64;    // 25 wave-wide loads
+// The same convolution with the input brought into LDS by direct global->LDS loads
+// (global_load_lds_dwordx4: 16 bytes per lane into consecutive LDS slots), double buffered: a
+// persistent workgroup walks its (tile, slice) items and issues the next item's loads before it
+// computes the current one, without holding the data in registers. The tile is unpadded, and a
+// pixel's four 8-channel groups are stored XOR-swizzled (slot 4P + (g ^ ((P >> 1) & 3)) for tile
+// pixel P), so that the 8 consecutive pixels of an A-fragment read phase cover all 32 banks.
+// Pieces outside the image or past CI load 16 zero bytes from dn_zero16 instead, so every wave
+// issues the same fixed number of loads per item, and the compiler can count them when it waits
+// for anything else. The kernel serves single-slice layers (CI <= 32) of up to 32 output
+// channels, whose weights sit in LDS for the whole launch. (A multi-slice variant that loaded
+// each slice's weights into registers per item measured 2.4x slower than the LDS-staged kernel
+// on the 80-channel decoder layer: the weight latency it exposes per item is not hidden.)
+__device__ uint4 dn_zero16[1];                   // 16 zero bytes (static storage starts zeroed)
+constexpr int DT_SLOTS = LT_ROWS * LT_PX * 4;    // 1584 16-byte slots of a tile slice
+constexpr int DT_INST = (DT_SLOTS + 255) / 256 * 4;  // 28 wave-wide loads: 7 per wave
 constexpr int DT_ALLOC = DT_INST * 64;           // slots allocated per buffer
 
 template <int NB>
@@ -397,7 +401,8 @@ __global__ __launch_bounds__(256) void k_conv_fwd_dma(ConvArgs a) {
     __shared__ uint4 tB[DT_ALLOC];
     constexpr int NWI = (9 * NB * 64 + 255) / 256;
     __shared__ uint4 s_w[NWI * 256];  // padded: the weight stores are unconditional
-    const int CI = a.C1 + a.C2;  // <= 32: one slice
+    const int CI = a.C1 + a.C2;
+    const int nslice = 1;  // CI <= 32
     const int runs = (a.W + 63) / 64;
     const int rblocks = (a.H + 3) / 4;
     const int ntiles = a.B * rblocks * runs;
@@ -405,13 +410,13 @@ __global__ __launch_bounds__(256) void k_conv_fwd_dma(ConvArgs a) {
     const int wave = threadIdx.x >> 6;
     const int lane = __lane_id();
     const int q = lane >> 4, lr = lane & 15;
-    int tl = (int)blockIdx.x;
+    const uint4* wl4 = reinterpret_cast<const uint4*>(a.Wl);
+    int tl = (int)blockIdx.x, sl = 0;
     if (tl >= ntiles) return;
     float bias[NB];
 #pragma unroll
     for (int j = 0; j < NB; ++j) bias[j] = a.bias[j * 16 + lr];
-    {  // the weights, once, with ordinary loads and stores (slot = (t * NB + nb) * 64 + lane)
-        const uint4* wl4 = reinterpret_cast<const uint4*>(a.Wl);
+    {  // the layer's weights, once (slot = (t * NB + nb) * 64 + lane)
         uint4 wv[NWI];
 #pragma unroll
         for (int k = 0; k < NWI; ++k) {
@@ -428,37 +433,24 @@ __global__ __launch_bounds__(256) void k_conv_fwd_dma(ConvArgs a) {
         b = r2 / rblocks;
         x0 = run * 64;
     };
-    // loads of tile t into buf: wave w issues wave-wide loads w, w + 4, ...; the zeros first, so
-    // that no LDS store follows a load into the same buffer
-    auto issue = [&](uint4* buf, int t) {
+    // loads of item (t, slice) into buf: wave w issues wave-wide loads w, w + 4, ..., 7 of them
+    auto issue = [&](uint4* buf, int t, int slice) {
         int b, x0, y0;
         tile_origin(t, b, x0, y0);
-        uint32_t okm = 0;
 #pragma unroll
-        for (int k = 0; k < (DT_INST + 3) / 4; ++k) {
+        for (int k = 0; k < DT_INST / 4; ++k) {
             const int qi = wave + 4 * k;
             const int S = qi * 64 + lane;
             const int P = S >> 2, g = (S & 3) ^ ((P >> 1) & 3);
             const int ry = P / LT_PX, pxl = P - ry * LT_PX;
-            const int yy = y0 - 1 + ry, xx = x0 - 1 + pxl, ci0 = g * 8;
-            const bool ok = qi < DT_INST && P < LT_ROWS * LT_PX && ci0 < CI && yy >= 0 && yy < a.H && xx >= 0 &&
-                            xx < a.W;
-            okm |= (ok ? 1u : 0u) << k;
-            if (qi < DT_INST && !ok) buf[S] = make_uint4(0u, 0u, 0u, 0u);
-        }
-#pragma unroll
-        for (int k = 0; k < (DT_INST + 3) / 4; ++k) {
-            const int qi = wave + 4 * k;
-            if (!((okm >> k) & 1u)) continue;
-            const int S = qi * 64 + lane;
-            const int P = S >> 2, g = (S & 3) ^ ((P >> 1) & 3);
-            const int ry = P / LT_PX, pxl = P - ry * LT_PX;
-            const int yy = y0 - 1 + ry, xx = x0 - 1 + pxl, ci0 = g * 8;
+            const int yy = y0 - 1 + ry, xx = x0 - 1 + pxl, ci0 = slice * 32 + g * 8;
+            const bool ok = P < LT_ROWS * LT_PX && ci0 < CI && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
+            const int yc = ok ? yy : 0, xc = ok ? xx : 0, cc = ok ? ci0 : 0;
             const uint16_t* src =
-                ci0 >= a.C1 ? a.X2 + (((size_t)b * (a.H >> 1) + (yy >> 1)) * (a.W >> 1) + (xx >> 1)) * a.C2 + (ci0 - a.C1)
-                            : a.X1 + (((size_t)b * a.H + yy) * a.W + xx) * a.C1 + ci0;
-            __builtin_amdgcn_global_load_lds((const void*)src, (__attribute__((address_space(3))) void*)(buf + qi * 64),
-                                             16, 0, 0);
+                cc >= a.C1 ? a.X2 + (((size_t)b * (a.H >> 1) + (yc >> 1)) * (a.W >> 1) + (xc >> 1)) * a.C2 + (cc - a.C1)
+                           : a.X1 + (((size_t)b * a.H + yc) * a.W + xc) * a.C1 + cc;
+            const void* gsrc = ok ? (const void*)src : (const void*)dn_zero16;
+            __builtin_amdgcn_global_load_lds(gsrc, (__attribute__((address_space(3))) void*)(buf + qi * 64), 16, 0, 0);
         }
     };
     auto compute = [&](const uint4* buf, f32x4 (&acc)[4][NB]) {
@@ -482,27 +474,40 @@ __global__ __launch_bounds__(256) void k_conv_fwd_dma(ConvArgs a) {
                     acc[pb][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[pb].v, cw[j].v, acc[pb][j], 0, 0, 0);
         }
     };
-    issue(tA, tl);
-    // one step: wait for this tile (the barrier's wait), start the next into the other buffer,
-    // compute and store this one; the buffers are distinct arrays, so the compiler knows that
-    // reads of one need not wait for loads into the other
+    issue(tA, tl, sl);
+    f32x4 acc[4][NB];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // one step: wait for this item (the barrier's wait), start the next item into the other
+    // buffer, compute this one; the buffers are distinct arrays, so the compiler knows that reads
+    // of one need not wait for loads into the other
     auto step = [&](const uint4* cur, uint4* nxt) -> bool {
         __syncthreads();
-        const int ntl = tl + (int)gridDim.x;
+        int ntl = tl, nsl = sl + 1;
+        if (nsl == nslice) {
+            nsl = 0;
+            ntl = tl + (int)gridDim.x;
+        }
         const bool more = ntl < ntiles;
-        if (more) issue(nxt, ntl);
-        f32x4 acc[4][NB];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < NB; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+        // issued unconditionally (the last item reloads itself) so that every path has the same
+        // number of loads in flight and the compiler's waits can count them
+        issue(nxt, more ? ntl : tl, more ? nsl : sl);
         compute(cur, acc);
-        int b, x0, y0;
-        tile_origin(tl, b, x0, y0);
-        const int y = y0 + wave;
-        if (y < a.H) store_blocks<4, NB>(a, acc, bias, 0, NBT, b, y, x0, q, lr);
+        if (sl == nslice - 1) {  // the tile is complete
+            int b, x0, y0;
+            tile_origin(tl, b, x0, y0);
+            const int y = y0 + wave;
+            if (y < a.H) store_blocks<4, NB>(a, acc, bias, 0, NBT, b, y, x0, q, lr);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < NB; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+        }
         __syncthreads();  // every wave is done reading cur before it is refilled
         tl = ntl;
+        sl = nsl;
         return more;
     };
     while (true) {
