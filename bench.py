@@ -232,11 +232,15 @@ def denoised_leg(r, scene, args, cam, p_plain):
 
     tuned1 = None
     if not args.no_autotune and not args.option:
-        # the 1-spp frame has its own best launch (lane-per-pixel kernel, refill threshold)
+        # the 1-spp frame has its own best tree (leaf pairs) and launch (lane-per-pixel kernel,
+        # refill threshold)
         from megamol_amd import autotune
+        pairs1, pairs1_ms = autotune.tune_tree(r, cam, p1, args.bits)
         r.build(L.BUILD_REBUILD, args.bits)
         cfg1, best1, _ = autotune.tune_frame(r, cam, p1)
-        tuned1 = {"config": cfg1, "trace_ms": round(best1, 3)}
+        cfg1["leaf_pairs"] = pairs1
+        tuned1 = {"config": cfg1, "trace_ms": round(best1, 3),
+                  "leaf_pairs_build_plus_trace_ms": {str(k): round(v, 3) for k, v in pairs1_ms.items()}}
         log(f"autotuned 1-spp launch: {cfg1} ({best1:.3f} ms trace)")
 
     for _ in range(args.warmup):
@@ -432,9 +436,12 @@ def main():
     tuned = None
     if not args.no_autotune and not args.option:
         from megamol_amd import autotune
+        pairs, pairs_ms = autotune.tune_tree(r, cam, p, args.bits)
         r.build(L.BUILD_REBUILD, args.bits)
         cfg, best, cache = autotune.tune_frame(r, cam, p)
+        cfg["leaf_pairs"] = pairs
         tuned = {"config": cfg, "trace_ms": round(best, 3),
+                 "leaf_pairs_build_plus_trace_ms": {str(k): round(v, 3) for k, v in pairs_ms.items()},
                  "candidates_trace_ms": {k: round(v, 3) for k, v in cache.items()}}
         log(f"autotuned launch: {cfg} ({best:.2f} ms trace)")
 
@@ -472,18 +479,23 @@ def main():
     frame()
     visits, tests, rays = r.counters()
     overflow = r.get_option("stack_overflow")
+    pair_visits, leaf_fetches = r.get_option("pair_visits"), r.get_option("leaf_fetches")
     r.set_option("count_visits", 0)
     fr = r.download(args.width, args.height, outputs=("hit_id",))
     hit_px = int((fr.hit_id >= 0).sum())
     P = args.width * args.height
     hit_samples = rays - P * args.spp if mode == L.MODE_AO else hit_px * args.spp
-    algo_bytes = 64 * visits + 16 * tests + 16 * max(hit_samples, 0) + 32 * P
+    # a stored node is 64 bytes, a leaf pair's two spheres 32, a leaf child's sphere 16; shading
+    # reads each hit sphere (16) and the frame writes 32 bytes per pixel
+    algo_bytes = (64 * (visits - pair_visits) + 32 * pair_visits + 16 * leaf_fetches
+                  + 16 * max(hit_samples, 0) + 32 * P)
     achieved = algo_bytes / (trace_ms * 1e-3) / 1e9
     kernel_name = trace_kernel_label(r, args.spp, args.mode)
     traffic = (dram_traffic(f"cells {args.cells}, {args.width}x{args.height}, spp {args.spp}, mode {args.mode}")
                if args.config == "fcc" else None)
     mrays = rays / (ms * 1e-3) / 1e6
-    log(f"visits {visits} tests {tests} rays {rays} overflow {overflow} hit px {hit_px}")
+    log(f"visits {visits} (leaf pairs {pair_visits}) tests {tests} leaf fetches {leaf_fetches} rays {rays} "
+        f"overflow {overflow} hit px {hit_px}")
     split = None
     if mode == L.MODE_AO:
         # the same primary rays alone (Lambert shading): the AO rays' share by difference

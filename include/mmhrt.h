@@ -205,9 +205,12 @@ MMHRT_API int32_t mmhrt_sync(mmhrt_ctx ctx);
 MMHRT_API int32_t mmhrt_get_info(mmhrt_ctx ctx, uint64_t* n_spheres, float* bounds6, uint64_t* n_nodes);
 
 /* Copy the LBVH to host for structural checks: nodes (n_nodes*16 floats: child0 lo.xyz hi.xyz,
- * child1 lo.xyz hi.xyz, then the two child references as int32 bit patterns, 2 pad words),
- * leaf_ids (n_spheres uint32, particle index of leaf k), keys (n_spheres uint64 Morton keys in
- * leaf order). Any pointer may be NULL. */
+ * child1 lo.xyz hi.xyz, then as int32 bit patterns the two child references (>= 0: node index,
+ * < 0: ~k for leaf k), the pair flags and the subtree height), leaf_ids (n_spheres uint32,
+ * particle index of leaf k), keys (n_spheres uint64 Morton keys in leaf order). A node whose
+ * two children are leaves k, k+1 (a leaf pair) is not stored, except the root: its parent holds
+ * ~k with pair flag bit 0 (child0) or bit 1 (child1) set, and its slot is left unwritten. Any
+ * pointer may be NULL. */
 MMHRT_API int32_t mmhrt_get_bvh(mmhrt_ctx ctx, float* nodes, uint32_t* leaf_ids, uint64_t* keys);
 
 /* Per-kernel timing of the last mmhrt_build / mmhrt_render on the context's stream, measured
@@ -216,8 +219,10 @@ MMHRT_API int32_t mmhrt_get_bvh(mmhrt_ctx ctx, float* nodes, uint32_t* leaf_ids,
 MMHRT_API int32_t mmhrt_get_timings(mmhrt_ctx ctx, float* times_ms, int32_t n);
 
 /* Instrumented trace counters of the last render (when enabled with mmhrt_set_option
- * "count_visits"): node visits, sphere tests, rays traced. Also refreshes the option
- * "stack_overflow" (1 if a traversal stack ever exceeded its capacity; 0 expected). */
+ * "count_visits"): node visits, sphere tests, rays traced. Also refreshes the options
+ * "stack_overflow" (1 if a traversal stack ever exceeded its capacity; 0 expected),
+ * "pair_visits" (the node visits that were leaf pairs: two 16-byte spheres read instead of a
+ * 64-byte node) and "leaf_fetches" (spheres read for the leaf children of stored nodes). */
 MMHRT_API int32_t mmhrt_get_counters(mmhrt_ctx ctx, uint64_t* node_visits, uint64_t* sphere_tests, uint64_t* rays);
 
 /* ---- batched time series ---------------------------------------------------
@@ -279,10 +284,13 @@ MMHRT_API int32_t mmhrt_denoiser_train_step(mmhrt_ctx ctx, const uint16_t* feats
  * (leaves per LBVH chunk workgroup: 1024, 512 or 256), "build_ls" (level-synchronous chunk
  * pass, default 1; 0: one climbing thread per leaf), "build_grid" (test hook: at most this many
  * persistent chunk workgroups, 0 = as many as fit), "sort_ri" (radix sort keys per thread per
- * tile: 8, 16 or 32), "sample_lanes" (samples of a pixel on the lanes for power-of-two spp,
+ * tile: 8, 16 or 32), "sort_onesweep" (single-pass radix passes with look-back, default 0),
+ * "leaf_pairs" (default 1: nodes over two leaves are not stored, their parents reference the
+ * pair; 0: every node is stored), "sample_lanes" (samples of a pixel on the lanes for power-of-two spp,
  * default 1), "dn_k32" (denoiser forward on the 16x16x32 MFMA kernels, default 1),
  * "dn_conv_shape" (test hook: forced launch shape of those kernels, 0 = automatic).
- * Diagnostics: "count_visits" (0/1) and, read-only in effect, "stack_overflow". */
+ * Diagnostics: "count_visits" (0/1) and, read-only in effect, "stack_overflow", "pair_visits",
+ * "leaf_fetches" (refreshed by mmhrt_get_counters). */
 MMHRT_API int32_t mmhrt_set_option(mmhrt_ctx ctx, const char* key, int64_t value);
 MMHRT_API int32_t mmhrt_get_option(mmhrt_ctx ctx, const char* key, int64_t* value);
 

@@ -6,6 +6,7 @@ Knobs (library options, see include/mmhrt.h):
   blocks_per_cu  persistent workgroups per compute unit
   refill         ballot-refill threshold in active lanes (0 = plain persistent kernel)
   refit_period   batched series: rebuild every k-th timestep, refit in between
+  leaf_pairs     nodes over two leaves stored or not (tune_tree: build + trace of a frame)
 
 Every candidate is timed on the device (HIP events around the work, median of `reps` runs)
 and the search is coordinate descent from the library defaults: each knob in turn takes its
@@ -79,6 +80,29 @@ def tune_frame(r, cam, p, rounds=2, reps=3):
 
     measure(r)  # warm-up allocations
     return tune(measure, r, rounds=rounds, reps=reps)
+
+
+def tune_tree(r, cam, p, bits=30, reps=3):
+    """leaf_pairs by the time of the frame's build plus its trace: a tree without leaf pairs
+    takes longer to build (more nodes written), a tree with them costs the traversal a branch
+    per visit, which shows in the coherent sample-lanes kernel. Returns (value, {value: ms})."""
+    from . import _lib as L
+
+    res = {}
+    for v in (1, 0):
+        r.set_option("leaf_pairs", v)
+        ms = []
+        for _ in range(reps + 1):
+            r.build(L.BUILD_REBUILD, bits)
+            r.render_async(cam, p)
+            r.sync()
+            t = r.timings()
+            ms.append(t["build"] + t["trace"])
+        res[v] = float(np.median(ms[1:]))
+        log(f"leaf_pairs {v}: build + trace {res[v]:.3f} ms")
+    best = min(res, key=res.get)
+    r.set_option("leaf_pairs", best)
+    return best, res
 
 
 def tune_batch(r, jobs, periods=(1, 2, 4, 8, 16, 64), rounds=2, reps=2, use_graph=True):

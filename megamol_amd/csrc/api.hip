@@ -457,6 +457,8 @@ int32_t mmhrt_get_counters(mmhrt_ctx c, uint64_t* visits, uint64_t* tests, uint6
     if (tests) *tests = s;
     if (rays) *rays = r;
     c->stack_overflow = h[1];
+    memcpy(&c->pair_visits, h + 8, 8);
+    memcpy(&c->leaf_fetches, h + 10, 8);
     return MMHRT_OK;
 }
 
@@ -479,7 +481,10 @@ static int64_t* option_slot(mmhrt_ctx c, const char* key) {
     if (k == "sort_ri") return &c->opt_sort_ri;
     if (k == "build_grid") return &c->opt_build_grid;
     if (k == "sort_onesweep") return &c->opt_sort_onesweep;
+    if (k == "leaf_pairs") return &c->opt_leaf_pairs;
     if (k == "stack_overflow") return &c->stack_overflow;
+    if (k == "pair_visits") return &c->pair_visits;
+    if (k == "leaf_fetches") return &c->leaf_fetches;
     return nullptr;
 }
 

@@ -17,12 +17,13 @@ struct Denoiser;
 
 // BVH2 node, 64 bytes: both child boxes in one node so that one visit tests two children
 // with four 16-byte loads. child ref >= 0: internal node index; < 0: leaf ~k (k-th sphere in
-// Morton order). w of d holds the subtree height (leaves have height 0).
+// Morton order), or with the side's pair flag in d.z (bit 0: child 0, bit 1: child 1) the leaf
+// pair k, k + 1, whose node is not stored. w of d holds the subtree height (leaves have height 0).
 struct __align__(16) Node {
     float4 a;  // c0.lo.x c0.lo.y c0.lo.z c0.hi.x
     float4 b;  // c0.hi.y c0.hi.z c1.lo.x c1.lo.y
     float4 c;  // c1.lo.z c1.hi.x c1.hi.y c1.hi.z
-    int4 d;    // child0, child1, -1, height
+    int4 d;    // child0, child1, pair flags, height
 };
 
 // 4-wide record of node X (same index as X): the boxes and references of X's grandchildren
@@ -43,7 +44,7 @@ struct SceneInfo {
     float absmax;          // max over spheres of max(|x|,|y|,|z|) + r
     float scale;           // Morton cells per world unit
     uint32_t height;       // root height after a build
-    uint32_t pad_;
+    uint32_t pair_h;       // set before a hierarchy pass: 1 = leaf pairs are not stored, 0 = every node is
 };
 
 template <typename T>
@@ -91,6 +92,7 @@ struct mmhrt_ctx_s {
     mmhrt::DevBuf<mmhrt::Node> nodes;
     mmhrt::DevBuf<uint32_t> flags;   // arrivals at the global meeting places (generation numbers)
     uint32_t build_gen = 0;          // generation number of the last hierarchy pass
+    bool bvh_pairs = true;           // the current tree may reference leaf pairs (built with leaf_pairs on)
     uint32_t* rounds_host = nullptr; // pinned: straddler kernel rounds the last build needed
     mmhrt::DevBuf<mmhrt::Rec4> rec4;  // 4-wide records (valid while rec4_valid)
     bool rec4_valid = false;
@@ -131,8 +133,11 @@ struct mmhrt_ctx_s {
     int64_t opt_sort_onesweep = 0;  // single-pass radix passes with decoupled look-back (0: histogram + scans + scatter)
     int64_t opt_build_grid = 0;     // test hook: cap on the persistent chunk workgroups (0 = as many as fit)
     int64_t opt_bvh4_spp = 8;       // 4-wide traversal for frames of at least this many spp (0: never)
+    int64_t opt_leaf_pairs = 1;     // nodes over two leaves are not stored (their parents reference the pair)
     int64_t refits_since_build = 0;
     int64_t stack_overflow = 0;  // set by mmhrt_get_counters from the trace kernel's flag
+    int64_t pair_visits = 0;     // set by mmhrt_get_counters: visits of leaf pairs (of the node visits)
+    int64_t leaf_fetches = 0;    // set by mmhrt_get_counters: sphere loads of leaf children of stored nodes
 
     // batched time series: all timesteps resident, one hipGraph for the whole sequence
     mmhrt::DevBuf<float4> ts_sph;      // timesteps x n spheres

@@ -675,12 +675,20 @@ __device__ __forceinline__ void leaf_box(const float4 s, float b[6]) {
     b[3] = s.x + s.w; b[4] = s.y + s.w; b[5] = s.z + s.w;
 }
 
-// Whole node: the two child boxes, the child references and the height, one 64-byte line.
-__device__ __forceinline__ void store_node(Node* nd, const float f[12], int2 k, int h) {
+// Whole node: the two child boxes, the child references, the pair flags and the height, one
+// 64-byte line. A node whose two children are leaves (a leaf pair: leaves k and k + 1, height 1)
+// is not stored: its parent references it as leaf k and sets the pair flag of that side
+// (bit 0: child 0, bit 1: child 1), and a traversal tests both spheres there. The root is always
+// stored. pf(): the flags from the children's references and heights (a pair is the only leaf
+// reference of height 1).
+__device__ __forceinline__ int pair_flags(int2 k, int h0, int h1) {
+    return ((k.x < 0 && h0 == 1) ? 1 : 0) | ((k.y < 0 && h1 == 1) ? 2 : 0);
+}
+__device__ __forceinline__ void store_node(Node* nd, const float f[12], int2 k, int pf, int h) {
     nd->a = make_float4(f[0], f[1], f[2], f[3]);
     nd->b = make_float4(f[4], f[5], f[6], f[7]);
     nd->c = make_float4(f[8], f[9], f[10], f[11]);
-    nd->d = make_int4(k.x, k.y, -1, h);
+    nd->d = make_int4(k.x, k.y, pf, h);
 }
 
 // One child's record at a global meeting place: box in slot `side` of a,b,c; d = (ref0, ref1,
@@ -741,6 +749,7 @@ __global__ __launch_bounds__(BC) void k_bu_chunk(const K* __restrict__ keys, con
     __shared__ uint32_t s_n, s_base;
     SplitList ap{s_list, &s_n, &s_base};
     const uint32_t gen = cnt[GEN_SLOT];
+    const int pair_h = (int)info->pair_h;
     const int64_t cs = (int64_t)blockIdx.x * BC;
     const int64_t ce = cs + BC < n ? cs + BC : n;
     const int64_t k = cs + threadIdx.x;
@@ -754,13 +763,14 @@ __global__ __launch_bounds__(BC) void k_bu_chunk(const K* __restrict__ keys, con
         leaf_box(sph, b);
         float f[12];
         int2 kd = make_int2(0, 0);
+        int pf = 0;
         int h = 0;
         int64_t l = k, r = k;
         int32_t ref = ~(int32_t)k;
         bool internal = false;
         while (true) {
             if (l == 0 && r == n - 1) {  // the root
-                store_node(nodes, f, kd, h);
+                store_node(nodes, f, kd, pf, h);
                 info->height = (uint32_t)h;
                 break;
             }
@@ -768,9 +778,13 @@ __global__ __launch_bounds__(BC) void k_bu_chunk(const K* __restrict__ keys, con
             const bool left = dr > dl;
             const int64_t s = left ? r : l - 1;
             if (internal) {
-                const int64_t idx = left ? r : l;
-                ref = (int32_t)idx;
-                store_node(nodes + idx, f, kd, h);
+                if (h == pair_h) {  // a leaf pair: referenced as its first leaf, not stored
+                    ref = ~(int32_t)l;
+                } else {
+                    const int64_t idx = left ? r : l;
+                    ref = (int32_t)idx;
+                    store_node(nodes + idx, f, kd, pf, h);
+                }
             }
             const int side = left ? 0 : 1;
             const int32_t far = (int32_t)(left ? l : r);
@@ -794,6 +808,7 @@ __global__ __launch_bounds__(BC) void k_bu_chunk(const K* __restrict__ keys, con
             for (int a = 0; a < 6; ++a) f[side * 6 + a] = b[a];
             kd = side == 0 ? make_int2(ref, s_ref[slot][1]) : make_int2(s_ref[slot][0], ref);
             const int hs = s_h[slot][1 - side];
+            pf = side == 0 ? pair_flags(kd, h, hs) : pair_flags(kd, hs, h);
             const int64_t fo = s_far[slot][1 - side];
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
@@ -866,6 +881,7 @@ __global__ __launch_bounds__(BC) __attribute__((amdgpu_waves_per_eu(8, 8))) void
     __shared__ uint32_t s_n, s_base;
     SplitList ap{s_list, &s_n, &s_base};
     const uint32_t gen = cnt[GEN_SLOT];
+    const int pair_h = (int)info->pair_h;
     const int32_t n = (int32_t)n64;
     const int32_t nchunks = (n + BC - 1) / BC;
     const int32_t tid = (int32_t)threadIdx.x;
@@ -924,11 +940,15 @@ __global__ __launch_bounds__(BC) __attribute__((amdgpu_waves_per_eu(8, 8))) void
         // deliver node X = [l, r] (box b, height h, dl = delta(l - 1), dr = delta(r); its
         // children's boxes f and references kd when internal) to its parent
         auto deliver = [&](int32_t l, int32_t r, int dl, int dr, const float b[6], int h, bool internal,
-                           const float f[12], int2 kd, int32_t leaf, uint32_t* nl_next, uint16_t* lst_next) {
+                           const float f[12], int2 kd, int pf, int32_t leaf, uint32_t* nl_next, uint16_t* lst_next) {
             if (l == 0 && r == n - 1) {  // the root
-                store_node(nodes, f, kd, h);
+                store_node(nodes, f, kd, pf, h);
                 info->height = (uint32_t)h;
                 return;
+            }
+            if (internal && h == pair_h) {  // a leaf pair: delivered as its first leaf (height 1), not stored
+                internal = false;
+                leaf = l;
             }
             const bool left = dr > dl;
             const int32_t sp = left ? r : l - 1;
@@ -936,7 +956,7 @@ __global__ __launch_bounds__(BC) __attribute__((amdgpu_waves_per_eu(8, 8))) void
             uint16_t rrel;
             if (internal) {
                 const int32_t idx = left ? r : l;
-                store_node(nodes + idx, f, kd, h);
+                store_node(nodes + idx, f, kd, pf, h);
                 ref = idx;
                 rrel = (uint16_t)(idx - cs);
             } else {
@@ -970,7 +990,8 @@ __global__ __launch_bounds__(BC) __attribute__((amdgpu_waves_per_eu(8, 8))) void
             float b[6];
             leaf_box(sp_cur, b);
             float f0[12];
-            deliver(k, k, s_delta[tid], s_delta[tid + 1], b, 0, false, f0, make_int2(0, 0), k, &s_nl[0], s_lst[0]);
+            deliver(k, k, s_delta[tid], s_delta[tid + 1], b, 0, false, f0, make_int2(0, 0), 0, k, &s_nl[0],
+                    s_lst[0]);
         }
         __syncthreads();
         for (int rd = 0;; ++rd) {
@@ -997,7 +1018,7 @@ __global__ __launch_bounds__(BC) __attribute__((amdgpu_waves_per_eu(8, 8))) void
                     b[a] = fminf(f[a], f[6 + a]);
                     b[3 + a] = fmaxf(f[3 + a], f[9 + a]);
                 }
-                deliver(l, r, dl, dr, b, h, true, f, kd, 0, nl_next, lst_next);
+                deliver(l, r, dl, dr, b, h, true, f, kd, pair_flags(kd, h0, h1), 0, nl_next, lst_next);
             }
             __syncthreads();
         }
@@ -1072,9 +1093,10 @@ __device__ __forceinline__ int64_t bu_step(const K* __restrict__ keys, int64_t n
     }
     const int2 kd = make_int2(dv[0], dv[1]);
     const int h = 1 + (dv[2] > dv[3] ? dv[2] : dv[3]);
+    const int pf = pair_flags(kd, dv[2], dv[3]);
     const int64_t l = fr[0], r = fr[1];
     if (l == 0 && r == n - 1) {
-        store_node(nodes, f, kd, h);
+        store_node(nodes, f, kd, pf, h);
         info->height = (uint32_t)h;
         return -1;
     }
@@ -1086,10 +1108,16 @@ __device__ __forceinline__ int64_t bu_step(const K* __restrict__ keys, int64_t n
     }
     const int dl = fr[2], dr = fr[3];  // delta(l - 1), delta(r), carried by the children
     const bool left = dr > dl;
-    const int64_t idx = left ? r : l;
-    store_node(nodes + idx, f, kd, h);
+    int32_t ref;
+    if (h == (int)info->pair_h) {  // a leaf pair: referenced as its first leaf, not stored
+        ref = ~(int32_t)l;
+    } else {
+        const int64_t idx = left ? r : l;
+        store_node(nodes + idx, f, kd, pf, h);
+        ref = (int32_t)idx;
+    }
     const int64_t s2 = left ? r : l - 1;
-    meet_put(meet + s2, meet_far + s2, left ? 0 : 1, b, (int32_t)idx, h, (int32_t)(left ? l : r), left ? dl : dr);
+    meet_put(meet + s2, meet_far + s2, left ? 0 : 1, b, ref, h, (int32_t)(left ? l : r), left ? dl : dr);
     return atomicExch(flags + s2, gen) == gen ? s2 : -1;
 }
 
@@ -1174,7 +1202,7 @@ __global__ void k_single(const float4* __restrict__ leaf_sph, Node* nodes, Scene
     float4 s = leaf_sph[0];
     float f[12] = {s.x - s.w, s.y - s.w, s.z - s.w, s.x + s.w, s.y + s.w, s.z + s.w,
                    s.x - s.w, s.y - s.w, s.z - s.w, s.x + s.w, s.y + s.w, s.z + s.w};
-    store_node(nodes, f, make_int2(~0, ~0), 1);
+    store_node(nodes, f, make_int2(~0, ~0), 0, 1);
     info->height = 1;
 }
 
@@ -1317,6 +1345,8 @@ int hierarchy(mmhrt_ctx_s* c, const K* keys, const uint32_t* ids) {
     if (++c->build_gen == 0xffffffffu) c->build_gen = 1;
     launch_fill_u32(s, c->box_cnt.p, 0u, BOX_ROUNDS + 3);
     launch_fill_u32(s, c->box_cnt.p + GEN_SLOT, c->build_gen, 1);
+    launch_fill_u32(s, &c->info.p->pair_h, c->opt_leaf_pairs ? 1u : 0u, 1);
+    c->bvh_pairs = c->opt_leaf_pairs != 0;
     // worklists of the straddler rounds: the spare key buffer (free once the sort is done) and
     // the spare value buffer
     uint32_t* l0 = sizeof(K) == 8 ? reinterpret_cast<uint32_t*>(c->klb.p) : reinterpret_cast<uint32_t*>(c->kb.p);

@@ -8,7 +8,8 @@
 // reference's one (TEA seed, two LCG draws per sample for the jitter).
 //
 // Traversal: BVH2 with both child boxes in the 64-byte node (four 16-byte loads per visit),
-// ordered front-to-back, leaves tested inline with one 16-byte sphere load. The traversal stack
+// ordered front-to-back, leaves tested inline with one 16-byte sphere load; a node over two
+// leaves is not stored and its visit reads the two spheres instead (leaf pairs). The traversal stack
 // keeps its top S entries in LDS (lane-strided, conflict free) and spills deeper entries to a
 // per-lane global region. Closest hit keeps the lowest particle ID among equal t; boxes whose
 // entry distance equals the current best are still visited so that ties resolve the same way
@@ -34,7 +35,7 @@ struct TraceArgs {
     const SceneInfo* info;
     int32_t* spill;
     uint32_t spill_stride;  // lanes in the grid
-    uint32_t* ctr;          // [0] tile counter, [1] overflow flag, [2..] u64 counters
+    uint32_t* ctr;          // [0] tile counter, [1] overflow flag, [2..11] u64 counters
     uint32_t n_tiles, tiles_x;
     uint32_t tw_log2;  // pixel tile = (1 << tw_log2) x (64 >> tw_log2), one wavefront
     float4* out_col;
@@ -100,29 +101,66 @@ __device__ __forceinline__ float slab(const RayPre& r, float lx, float ly, float
 }
 
 struct Count {
-    uint64_t visits = 0, tests = 0, rays = 0;
+    uint64_t visits = 0, tests = 0, rays = 0, pair_visits = 0, leaf_fetches = 0;
 };
+
+// The fetch of one visit. node >= 0: an internal node (four 16-byte loads). node < 0: the leaf
+// pair ~node, ~node + 1, which the build does not store: its two spheres are read in the same
+// four loads (each twice; the second reads hit the same cache lines) and turned in place into
+// the node they stand for: child boxes the leaves' boxes (centre -+ radius, as the build
+// computes them), children the two leaves. Waves without a pair skip that branch. PAIRS = false:
+// the tree has none (built with leaf pairs off) and a visit is the node fetch alone.
+struct Visit {
+    float4 a, b, c;
+    int4 d;
+};
+template <bool COUNT, bool PAIRS>
+__device__ __forceinline__ Visit fetch(const TraceArgs& A, int32_t node, Count& cnt) {
+    const bool pair = PAIRS && node < 0;
+    const float4* q = pair ? A.leaf_sph + ~node : reinterpret_cast<const float4*>(A.nodes + node);
+    const float4* q23 = pair ? q : q + 2;
+    Visit v;
+    v.a = q[0];
+    v.b = q[1];
+    v.c = q23[0];
+    v.d = reinterpret_cast<const int4*>(q23)[1];
+    if (COUNT) {
+        cnt.visits++;
+        if (pair) cnt.pair_visits++;
+    }
+    if (pair) {
+        const float4 s0 = v.c, s1 = make_float4(__int_as_float(v.d.x), __int_as_float(v.d.y),
+                                                __int_as_float(v.d.z), __int_as_float(v.d.w));
+        v.a = make_float4(s0.x - s0.w, s0.y - s0.w, s0.z - s0.w, s0.x + s0.w);
+        v.b = make_float4(s0.y + s0.w, s0.z + s0.w, s1.x - s1.w, s1.y - s1.w);
+        v.c = make_float4(s1.z - s1.w, s1.x + s1.w, s1.y + s1.w, s1.z + s1.w);
+        v.d = make_int4(node, node - 1, 0, 1);  // ~l, ~(l + 1)
+    }
+    return v;
+}
 
 // One node visit of a closest-hit ray (state: node, stack, t_best, best leaf). Returns true when
 // the ray is finished. A closest-hit trace is this step repeated from the root.
-template <int S, bool COUNT>
+template <int S, bool COUNT, bool PAIRS = true>
 __device__ __forceinline__ bool step_closest(const TraceArgs& A, const RayPre& r, float tmin, float tmax, float& t_best,
                                              int& best, int32_t& node, LaneStack<S>& st, Count& cnt) {
-    const Node* nd = A.nodes + node;
-    float4 a = nd->a, b = nd->b, c = nd->c;
-    int4 dd = nd->d;
-    if (COUNT) cnt.visits++;
-    float tn0 = slab(r, a.x, a.y, a.z, a.w, b.x, b.y, tmin, t_best);
-    float tn1 = slab(r, b.z, b.w, c.x, c.y, c.z, c.w, tmin, t_best);
-    int32_t ch0 = dd.x, ch1 = dd.y;
-    // both leaf spheres are fetched before either is tested: one memory round trip, not two
-    const bool lf0 = tn0 != INFINITY && ch0 < 0, lf1 = tn1 != INFINITY && ch1 < 0;
+    const bool pair_visit = PAIRS && node < 0;
+    const Visit v = fetch<COUNT, PAIRS>(A, node, cnt);
+    float tn0 = slab(r, v.a.x, v.a.y, v.a.z, v.a.w, v.b.x, v.b.y, tmin, t_best);
+    float tn1 = slab(r, v.b.z, v.b.w, v.c.x, v.c.y, v.c.z, v.c.w, tmin, t_best);
+    const int32_t ch0 = v.d.x, ch1 = v.d.y;
+    // a leaf child (ref < 0 without its pair flag: bit 0 / bit 1 of d.z) is tested here; a leaf
+    // pair child is descended into like an internal node. Both leaf spheres are fetched before
+    // either is tested: one memory round trip (for a pair's own two, cache hits).
+    const bool lf0 = tn0 != INFINITY && ch0 < 0 && !(PAIRS && (v.d.z & 1));
+    const bool lf1 = tn1 != INFINITY && ch1 < 0 && !(PAIRS && (v.d.z & 2));
     float4 sp0 = make_float4(0.f, 0.f, 0.f, 0.f), sp1 = sp0;
     if (lf0) sp0 = A.leaf_sph[~ch0];
     if (lf1) sp1 = A.leaf_sph[~ch1];
-    if (lf0) {
-        int leaf = ~ch0;
-        float t = sphere_t(r.o, r.d, sp0, tmin, tmax);
+    if (COUNT && !pair_visit) cnt.leaf_fetches += (lf0 ? 1 : 0) + (lf1 ? 1 : 0);
+    // the hit is the least t, ties going to the least particle ID, whatever the test order
+    auto test = [&](int leaf, float4 sp) {
+        float t = sphere_t(r.o, r.d, sp, tmin, tmax);
         if (COUNT) cnt.tests++;
         if (t >= 0.0f) {
             if (t < t_best) {
@@ -132,20 +170,13 @@ __device__ __forceinline__ bool step_closest(const TraceArgs& A, const RayPre& r
                 best = leaf;
             }
         }
+    };
+    if (lf0) {
+        test(~ch0, sp0);
         tn0 = INFINITY;
     }
     if (lf1) {
-        int leaf = ~ch1;
-        float t = sphere_t(r.o, r.d, sp1, tmin, tmax);
-        if (COUNT) cnt.tests++;
-        if (t >= 0.0f) {
-            if (t < t_best) {
-                t_best = t;
-                best = leaf;
-            } else if (t == t_best && best >= 0 && A.leaf_id[leaf] < A.leaf_id[best]) {
-                best = leaf;
-            }
-        }
+        test(~ch1, sp1);
         tn1 = INFINITY;
     }
     bool h0 = tn0 <= t_best, h1 = tn1 <= t_best;
@@ -165,30 +196,30 @@ __device__ __forceinline__ bool step_closest(const TraceArgs& A, const RayPre& r
 }
 
 // One node visit of an any-hit ray ignoring leaf `skip`: 0 = continue, 1 = finished without a
-// hit, 2 = occluded.
-template <int S, bool COUNT>
+// hit, 2 = occluded. Nodes, leaves and leaf pairs as in step_closest.
+template <int S, bool COUNT, bool PAIRS = true>
 __device__ __forceinline__ int step_any(const TraceArgs& A, const RayPre& r, float tmin, float tmax, int skip,
                                         int32_t& node, LaneStack<S>& st, Count& cnt) {
-    const Node* nd = A.nodes + node;
-    float4 a = nd->a, b = nd->b, c = nd->c;
-    int4 dd = nd->d;
-    if (COUNT) cnt.visits++;
-    float tn0 = slab(r, a.x, a.y, a.z, a.w, b.x, b.y, tmin, tmax);
-    float tn1 = slab(r, b.z, b.w, c.x, c.y, c.z, c.w, tmin, tmax);
-    int32_t ch0 = dd.x, ch1 = dd.y;
+    const bool pair_visit = PAIRS && node < 0;
+    const Visit v = fetch<COUNT, PAIRS>(A, node, cnt);
+    float tn0 = slab(r, v.a.x, v.a.y, v.a.z, v.a.w, v.b.x, v.b.y, tmin, tmax);
+    float tn1 = slab(r, v.b.z, v.b.w, v.c.x, v.c.y, v.c.z, v.c.w, tmin, tmax);
+    const int32_t ch0 = v.d.x, ch1 = v.d.y;
     bool h0 = tn0 != INFINITY, h1 = tn1 != INFINITY;
-    const bool lf0 = h0 && ch0 < 0 && ~ch0 != skip, lf1 = h1 && ch1 < 0 && ~ch1 != skip;
+    const bool leaf0 = ch0 < 0 && !(PAIRS && (v.d.z & 1)), leaf1 = ch1 < 0 && !(PAIRS && (v.d.z & 2));
+    const bool lf0 = h0 && leaf0 && ~ch0 != skip, lf1 = h1 && leaf1 && ~ch1 != skip;
     float4 sp0 = make_float4(0.f, 0.f, 0.f, 0.f), sp1 = sp0;
     if (lf0) sp0 = A.leaf_sph[~ch0];
     if (lf1) sp1 = A.leaf_sph[~ch1];
-    if (h0 && ch0 < 0) {
+    if (COUNT && !pair_visit) cnt.leaf_fetches += (lf0 ? 1 : 0) + (lf1 ? 1 : 0);
+    if (h0 && leaf0) {
         if (lf0) {
             if (COUNT) cnt.tests++;
             if (sphere_t(r.o, r.d, sp0, tmin, tmax) >= 0.0f) return 2;
         }
         h0 = false;
     }
-    if (h1 && ch1 < 0) {
+    if (h1 && leaf1) {
         if (lf1) {
             if (COUNT) cnt.tests++;
             if (sphere_t(r.o, r.d, sp1, tmin, tmax) >= 0.0f) return 2;
@@ -211,26 +242,26 @@ __device__ __forceinline__ int step_any(const TraceArgs& A, const RayPre& r, flo
 }
 
 // Closest hit: returns the leaf index of the nearest reported intersection (or -1) and its t.
-template <int S, bool COUNT>
+template <int S, bool COUNT, bool PAIRS = true>
 __device__ int trace_closest(const TraceArgs& A, const RayPre& r, float tmin, float tmax, float& t_best,
                              LaneStack<S>& st, Count& cnt) {
     int best = -1;
     t_best = tmax;
     int32_t node = 0;
     st.sp = 0;
-    while (!step_closest<S, COUNT>(A, r, tmin, tmax, t_best, best, node, st, cnt)) {
+    while (!step_closest<S, COUNT, PAIRS>(A, r, tmin, tmax, t_best, best, node, st, cnt)) {
     }
     return best;
 }
 
 // Any hit in (tmin, tmax) ignoring leaf `skip`.
-template <int S, bool COUNT>
+template <int S, bool COUNT, bool PAIRS = true>
 __device__ bool trace_any(const TraceArgs& A, const RayPre& r, float tmin, float tmax, int skip, LaneStack<S>& st,
                           Count& cnt) {
     int32_t node = 0;
     st.sp = 0;
     int res;
-    while ((res = step_any<S, COUNT>(A, r, tmin, tmax, skip, node, st, cnt)) == 0) {
+    while ((res = step_any<S, COUNT, PAIRS>(A, r, tmin, tmax, skip, node, st, cnt)) == 0) {
     }
     return res == 2;
 }
@@ -422,13 +453,15 @@ __global__ __launch_bounds__(TB) void k_trace(TraceArgs A) {
         atomicAdd(reinterpret_cast<unsigned long long*>(A.ctr + 2), (unsigned long long)cnt.visits);
         atomicAdd(reinterpret_cast<unsigned long long*>(A.ctr + 4), (unsigned long long)cnt.tests);
         atomicAdd(reinterpret_cast<unsigned long long*>(A.ctr + 6), (unsigned long long)cnt.rays);
+        atomicAdd(reinterpret_cast<unsigned long long*>(A.ctr + 8), (unsigned long long)cnt.pair_visits);
+        atomicAdd(reinterpret_cast<unsigned long long*>(A.ctr + 10), (unsigned long long)cnt.leaf_fetches);
     }
 }
 
 // One sample of a Lambert (MODE 0) or AO (MODE 1) pixel, the arithmetic of shade_pixel: the
 // sample colour, its depth contribution, the hit (t, leaf), the visibility and, for the
 // features, albedo and normal.
-template <int S, int MODE, bool COUNT>
+template <int S, int MODE, bool COUNT, bool PAIRS>
 __device__ __forceinline__ void shade_sample(const TraceArgs& A, uint32_t px, uint32_t py, uint32_t pix, uint32_t s,
                                              float jx, float jy, float pad, LaneStack<S>& st, Count& cnt,
                                              float4& sc, float& ray_depth, float& t, int& leaf, float& V,
@@ -438,7 +471,7 @@ __device__ __forceinline__ void shade_sample(const TraceArgs& A, uint32_t px, ui
     primary_ray(f, px, py, jx, jy, o, d);
     RayPre r = prep(o, d, pad);
     if (COUNT) cnt.rays++;
-    leaf = trace_closest<S, COUNT>(A, r, 1e-6f, 1e20f, t, st, cnt);
+    leaf = trace_closest<S, COUNT, PAIRS>(A, r, 1e-6f, 1e20f, t, st, cnt);
     ray_depth = FLT_MAX;
     V = 0.0f;
     if (leaf < 0) {
@@ -463,7 +496,7 @@ __device__ __forceinline__ void shade_sample(const TraceArgs& A, uint32_t px, ui
         V3 ad = ao_direction(N, aseed, s, f.spp);
         RayPre ra = prep(P, ad, pad);
         if (COUNT) cnt.rays++;
-        bool occ = trace_any<S, COUNT>(A, ra, 0.0f, f.ao_distance, leaf, st, cnt);
+        bool occ = trace_any<S, COUNT, PAIRS>(A, ra, 0.0f, f.ao_distance, leaf, st, cnt);
         V = occ ? 0.0f : 1.0f;
         sc = make_float4(alb.x * V, alb.y * V, alb.z * V, 1.0f);
     } else {
@@ -480,7 +513,7 @@ __device__ __forceinline__ void shade_sample(const TraceArgs& A, uint32_t px, ui
 // (lcg_skip), and the group folds its samples into the pixel in sample order (colour sums,
 // first strictly nearer hit, visibility, sample-0 features), so every output is bit-identical
 // to the lane-per-pixel kernels.
-template <int S, int MODE, bool COUNT>
+template <int S, int MODE, bool COUNT, bool PAIRS>
 __global__ __launch_bounds__(TB) void k_trace_sl(TraceArgs A) {
     __shared__ int32_t lds_stack[S * TB];
     const uint32_t gtid = blockIdx.x * TB + threadIdx.x;
@@ -523,8 +556,8 @@ __global__ __launch_bounds__(TB) void k_trace_sl(TraceArgs A) {
                     uint32_t seed = lcg_skip(seed0, 2u * s);
                     float jx = rnd(seed);
                     float jy = rnd(seed);
-                    shade_sample<S, MODE, COUNT>(A, px, py, pix, s, jx, jy, pad, st, cnt, sc, ray_depth, t, leaf, V,
-                                                 alb, N);
+                    shade_sample<S, MODE, COUNT, PAIRS>(A, px, py, pix, s, jx, jy, pad, st, cnt, sc, ray_depth, t,
+                                                        leaf, V, alb, N);
                 }
                 // fold the group's samples into the pixel, in sample order
                 for (uint32_t i = 0; i < L; ++i) {
@@ -570,6 +603,8 @@ __global__ __launch_bounds__(TB) void k_trace_sl(TraceArgs A) {
         atomicAdd(reinterpret_cast<unsigned long long*>(A.ctr + 2), (unsigned long long)cnt.visits);
         atomicAdd(reinterpret_cast<unsigned long long*>(A.ctr + 4), (unsigned long long)cnt.tests);
         atomicAdd(reinterpret_cast<unsigned long long*>(A.ctr + 6), (unsigned long long)cnt.rays);
+        atomicAdd(reinterpret_cast<unsigned long long*>(A.ctr + 8), (unsigned long long)cnt.pair_visits);
+        atomicAdd(reinterpret_cast<unsigned long long*>(A.ctr + 10), (unsigned long long)cnt.leaf_fetches);
     }
 }
 
@@ -747,6 +782,8 @@ __global__ __launch_bounds__(TB) void k_trace_refill(TraceArgs A, uint32_t refil
         atomicAdd(reinterpret_cast<unsigned long long*>(A.ctr + 2), (unsigned long long)cnt.visits);
         atomicAdd(reinterpret_cast<unsigned long long*>(A.ctr + 4), (unsigned long long)cnt.tests);
         atomicAdd(reinterpret_cast<unsigned long long*>(A.ctr + 6), (unsigned long long)cnt.rays);
+        atomicAdd(reinterpret_cast<unsigned long long*>(A.ctr + 8), (unsigned long long)cnt.pair_visits);
+        atomicAdd(reinterpret_cast<unsigned long long*>(A.ctr + 10), (unsigned long long)cnt.leaf_fetches);
     }
 }
 
@@ -774,19 +811,28 @@ void launch1(const TraceArgs& A, unsigned grid, bool steal, bool count, int mode
         launch2<S, 0>(A, grid, steal, count, s);
 }
 
-template <int S>
-void launch_sl(const TraceArgs& A, unsigned grid, bool count, int mode, hipStream_t s) {
+template <int S, bool PAIRS>
+void launch_sl_p(const TraceArgs& A, unsigned grid, bool count, int mode, hipStream_t s) {
     if (mode == 1) {
         if (count)
-            hipLaunchKernelGGL((k_trace_sl<S, 1, true>), dim3(grid), dim3(TB), 0, s, A);
+            hipLaunchKernelGGL((k_trace_sl<S, 1, true, PAIRS>), dim3(grid), dim3(TB), 0, s, A);
         else
-            hipLaunchKernelGGL((k_trace_sl<S, 1, false>), dim3(grid), dim3(TB), 0, s, A);
+            hipLaunchKernelGGL((k_trace_sl<S, 1, false, PAIRS>), dim3(grid), dim3(TB), 0, s, A);
     } else {
         if (count)
-            hipLaunchKernelGGL((k_trace_sl<S, 0, true>), dim3(grid), dim3(TB), 0, s, A);
+            hipLaunchKernelGGL((k_trace_sl<S, 0, true, PAIRS>), dim3(grid), dim3(TB), 0, s, A);
         else
-            hipLaunchKernelGGL((k_trace_sl<S, 0, false>), dim3(grid), dim3(TB), 0, s, A);
+            hipLaunchKernelGGL((k_trace_sl<S, 0, false, PAIRS>), dim3(grid), dim3(TB), 0, s, A);
     }
+}
+// The sample-lanes kernel is the one whose rays are coherent enough for the cost of the
+// leaf-pair branch to show: a tree built without pairs gets the variant without it.
+template <int S>
+void launch_sl(const TraceArgs& A, unsigned grid, bool count, int mode, bool pairs, hipStream_t s) {
+    if (pairs)
+        launch_sl_p<S, true>(A, grid, count, mode, s);
+    else
+        launch_sl_p<S, false>(A, grid, count, mode, s);
 }
 
 template <int S>
@@ -945,10 +991,10 @@ int render_frame(mmhrt_ctx_s* c, const FrameConsts& fc) {
     const bool sl_shape = fc.spp > 1 && ((fc.spp >= 64 && fc.spp % 64 == 0) || (fc.spp < 64 && 64 % fc.spp == 0));
     if (c->opt_sample_lanes != 0 && refill == 0 && fc.mode != 2 && sl_shape) {
         switch (S) {
-        case 8: launch_sl<8>(A, grid, count, fc.mode, c->stream); break;
-        case 24: launch_sl<24>(A, grid, count, fc.mode, c->stream); break;
-        case 32: launch_sl<32>(A, grid, count, fc.mode, c->stream); break;
-        default: launch_sl<16>(A, grid, count, fc.mode, c->stream); break;
+        case 8: launch_sl<8>(A, grid, count, fc.mode, c->bvh_pairs, c->stream); break;
+        case 24: launch_sl<24>(A, grid, count, fc.mode, c->bvh_pairs, c->stream); break;
+        case 32: launch_sl<32>(A, grid, count, fc.mode, c->bvh_pairs, c->stream); break;
+        default: launch_sl<16>(A, grid, count, fc.mode, c->bvh_pairs, c->stream); break;
         }
     } else if (refill > 0 && fc.mode != 2) {
         switch (S) {

@@ -257,8 +257,9 @@ class SphereRenderer:
         keys = np.empty(n, np.uint64)
         self._ck(self.lib.mmhrt_get_bvh(self.ctx, nodes.ctypes.data, ids.ctypes.data, keys.ctypes.data))
         child = nodes[:, 12:14].copy().view(np.int32)
+        pairs = nodes[:, 14].copy().view(np.int32)
         height = nodes[:, 15].copy().view(np.int32)
-        return nodes[:, :12].copy(), child, ids, keys, height
+        return nodes[:, :12].copy(), child, ids, keys, height, pairs
 
     def timings(self) -> dict:
         t = (C.c_float * 8)()

@@ -39,6 +39,26 @@ def test_small_scene_bit_exact(gpu_renderer, seed, mode, bits):
     assert_frames_equal(g, o, aux=True, what=f"seed={seed} mode={mode} bits={bits}")
 
 
+def leaf_pair_form(ochild):
+    """The oracle's tree in the stored form: a non-root node whose two children are leaves (a leaf
+    pair) is not stored; its parent references it as its first leaf and sets the pair flag of
+    that side (bit 0 / bit 1). Returns the expected child references, the flags and the mask of
+    stored nodes."""
+    m = len(ochild)
+    is_pair = (ochild[:, 0] < 0) & (ochild[:, 1] < 0)
+    is_pair[0] = False
+    child = ochild.copy()
+    pairs = np.zeros(m, np.int32)
+    for side in (0, 1):
+        c = ochild[:, side]
+        hit = c >= 0
+        hit[hit] = is_pair[c[hit]]
+        child[hit, side] = ochild[c[hit], 0]
+        pairs[hit] |= 1 << side
+    # a pair's first child is its first leaf: ~l with l the lower end of its two-leaf range
+    return child, pairs, ~is_pair
+
+
 @pytest.mark.parametrize("chunk,ls,ri,grid,os1", [(1024, 1, 16, 0, 0), (512, 1, 8, 0, 0), (256, 1, 32, 0, 0),
                                                   (1024, 0, 16, 0, 0), (256, 0, 8, 0, 0), (1024, 1, 32, 3, 0),
                                                   (256, 1, 16, 1, 0), (512, 1, 32, 7, 0), (1024, 1, 32, 0, 1),
@@ -65,13 +85,15 @@ def test_lbvh_equals_oracle_lbvh(gpu_renderer, chunk, ls, ri, grid, os1):
                 xyzr[n // 2:, :3] = xyzr[: n - n // 2, :3]
             gpu_renderer.set_spheres(xyzr)
             gpu_renderer.build(L.BUILD_REBUILD, bits)
-            box, child, ids, keys, height = gpu_renderer.bvh()
+            box, child, ids, keys, height, pairs = gpu_renderer.bvh()
             ob = O.Bvh(xyzr, bits)
             obox, ochild, oids, okeys = ob.arrays()
             np.testing.assert_array_equal(keys, okeys)
             np.testing.assert_array_equal(ids, oids)
-            np.testing.assert_array_equal(child, ochild)
-            np.testing.assert_array_equal(box.view(np.uint32), obox.view(np.uint32))
+            echild, epairs, stored = leaf_pair_form(ochild)
+            np.testing.assert_array_equal(child[stored], echild[stored])
+            np.testing.assert_array_equal(pairs[stored], epairs[stored])
+            np.testing.assert_array_equal(box[stored].view(np.uint32), obox[stored].view(np.uint32))
             assert height[0] >= int(np.ceil(np.log2(n)))
             assert gpu_renderer.get_option("sort_fault") == 0
     gpu_renderer.set_option("sort_onesweep", 0)
@@ -79,6 +101,31 @@ def test_lbvh_equals_oracle_lbvh(gpu_renderer, chunk, ls, ri, grid, os1):
     gpu_renderer.set_option("build_ls", 1)
     gpu_renderer.set_option("sort_ri", 32)
     gpu_renderer.set_option("build_grid", 0)
+
+
+@pytest.mark.parametrize("ls", [1, 0])
+def test_lbvh_without_leaf_pairs_is_the_oracle_tree(gpu_renderer, ls):
+    """With leaf_pairs off every node is stored: the oracle's arrays exactly, no pair flags."""
+    gpu_renderer.set_option("leaf_pairs", 0)
+    gpu_renderer.set_option("build_ls", ls)
+    try:
+        for n in (3, 4097, 300_000):
+            case(f"lbvh without leaf pairs n={n} ls={ls}")
+            g = np.random.default_rng(n)
+            xyzr = np.zeros((n, 4), np.float32)
+            xyzr[:, :3] = g.random((n, 3)).astype(np.float32)
+            xyzr[:, 3] = g.uniform(0.001, 0.01, n).astype(np.float32)
+            gpu_renderer.set_spheres(xyzr)
+            gpu_renderer.build(L.BUILD_REBUILD, 30)
+            box, child, ids, keys, height, pairs = gpu_renderer.bvh()
+            obox, ochild, oids, okeys = O.Bvh(xyzr, 30).arrays()
+            np.testing.assert_array_equal(ids, oids)
+            np.testing.assert_array_equal(child, ochild)
+            np.testing.assert_array_equal(box.view(np.uint32), obox.view(np.uint32))
+            assert not pairs.any()
+    finally:
+        gpu_renderer.set_option("leaf_pairs", 1)
+        gpu_renderer.set_option("build_ls", 1)
 
 
 def test_edge_cases(gpu_renderer):

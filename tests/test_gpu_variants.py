@@ -71,6 +71,37 @@ def test_sample_lanes_bit_identical(gpu_renderer, mode, spp):
         assert_frames_equal(a, o, aux=True, what=f"sample lanes spp {spp}")
 
 
+@pytest.mark.parametrize("spp", [1, 3, 64])
+@pytest.mark.parametrize("mode", [L.MODE_LAMBERT, L.MODE_AO])
+def test_trees_with_and_without_leaf_pairs_render_the_same(gpu_renderer, mode, spp):
+    """A tree built with leaf pairs (nodes over two leaves not stored, their visits read the two
+    spheres) and one built without: the same frames, bit for bit, through every kernel that the
+    frame takes (the sample-lanes kernel has a variant for each kind of tree), equal to the
+    oracle."""
+    xyzr, rgba = scenes.random_spheres(5000, seed=15, rmin=0.005, rmax=0.03)
+    w, h = (37, 29) if spp >= 64 else (101, 67)
+    cam = small_camera(w, h)
+    p = params(w, h, spp=spp, mode=mode, ao_distance=0.2, flags=L.FLAG_AUX)
+    gpu_renderer.set_spheres(xyzr, rgba)
+    frames = {}
+    try:
+        for pairs in (1, 0):
+            case(f"leaf pairs {pairs} mode {mode} spp {spp}")
+            gpu_renderer.set_option("leaf_pairs", pairs)
+            gpu_renderer.build()
+            g = gpu_renderer.render(cam, p)
+            g.aux = gpu_renderer.download(w, h, outputs=(), aux=True).aux
+            frames[pairs] = g
+    finally:
+        gpu_renderer.set_option("leaf_pairs", 1)
+    a, b = frames[1], frames[0]
+    for k in ("hit_id", "t", "depth", "color", "rgba8", "aux"):
+        np.testing.assert_array_equal(getattr(a, k).view(np.uint8), getattr(b, k).view(np.uint8), err_msg=k)
+    if spp <= 3:
+        o = O.render_brute(xyzr, rgba, cam, oracle_params(p), aux=True)
+        assert_frames_equal(a, o, aux=True, what=f"leaf pairs spp {spp}")
+
+
 def test_window_is_a_crop_of_the_full_frame(gpu_renderer):
     xyzr, rgba = scenes.random_spheres(4000, seed=13)
     cam = small_camera(120, 90)
