@@ -6,11 +6,15 @@ U[0.35,0.5], intensity through a 256-entry transfer function), 1920x1080, ambien
 64 samples per pixel (1 AO ray per sample, AO distance 4.0), LBVH rebuilt every frame. A "step"
 is one frame: rebuild + render, with the spheres already resident in HBM.
 
-Prints one JSON line. `roofline` is measured live for the dominant kernel (the trace kernel):
-algorithmic bytes per launch (64 B per node visit + 16 B per sphere test + 16 B per colour
-fetch + 32 B per pixel written) divided by its average duration from HIP events on the
-library's stream. `cpu_baseline` times the CPU oracle (multi-threaded LBVH renderer with the
-same rules) on a bounded sample of the same frame.
+Prints one JSON line. `value` is the denoised 1-spp frame (LBVH rebuild + 1-spp AO trace with
+features + U-Net denoise). `roofline` is measured live for that frame's dominant kernel, the
+hierarchy chunk pass of the build: algorithmic bytes per launch (per leaf a 4-byte particle id,
+a 4-byte key and the 16-byte gathered sphere in, the 16-byte leaf sphere out; 64 bytes per
+stored node) divided by its average duration from HIP events on the library's stream.
+`roofline_plain_trace` does the same for the plain 64-spp frame's trace kernel (64 B per node
+visit, 32 B per leaf-pair visit, 16 B per leaf sphere fetched or hit sphere shaded, 32 B per
+pixel written). `cpu_baseline` times the CPU oracle (multi-threaded LBVH renderer with the same
+rules) on a bounded sample of the same frame.
 """
 from __future__ import annotations
 
@@ -26,16 +30,17 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE summary of the dominant kernel on the 50M FCC 64-spp
-# AO frame (committed with the command that made it); its DRAM bytes per launch fill
-# roofline.traffic for that workload
+# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE summaries (committed with the commands that made them):
+# the DRAM bytes per launch of the chunk pass in the denoised frame and of the trace kernel of the
+# plain 64-spp frame, for the 50M FCC workload
+DRAM_PROFILE_CHUNK = os.path.join(ROOT, "profiles", "k_bu_chunk_ls_fcc50m_dram.md")
 DRAM_PROFILE = os.path.join(ROOT, "profiles", "k_trace_sl_fcc50m_ao64_dram.md")
 
 
-def dram_traffic(workload_key):
-    """DRAM bytes per launch recorded in the committed counter summary for this workload, or None."""
+def dram_traffic(workload_key, path=DRAM_PROFILE):
+    """DRAM bytes per launch recorded in a committed counter summary for this workload, or None."""
     try:
-        txt = open(DRAM_PROFILE).read()
+        txt = open(path).read()
     except OSError:
         return None
     if workload_key not in txt:
@@ -258,6 +263,25 @@ def denoised_leg(r, scene, args, cam, p_plain):
         r.render_async(cam, p1)
     r.sync()
     ms_nodn = 1e3 * (time.perf_counter() - t0) / args.steps
+    # per-kernel durations of the frame's build and trace (HIP events), one frame at a time
+    kt = {"build": [], "chunk": [], "trace": []}
+    for _ in range(args.steps):
+        r.build(L.BUILD_REBUILD, args.bits)
+        r.render_async(cam, p1)
+        t = r.timings()
+        for k in kt:
+            kt[k].append(t[k])
+    kms = {k: float(np.mean(v)) for k, v in kt.items()}
+    n = len(scene["xyzr"])
+    pairs = r.get_option("leaf_pair_count")
+    stored = n - 1 - pairs
+    # the chunk pass reads per leaf its particle id, its key and its sphere (gathered), and writes
+    # the leaf sphere and every stored node (64 bytes; leaf pairs are not stored)
+    chunk_bytes = (4 + 4 + 16) * n + 16 * n + 64 * stored
+    chunk = {"kernel": "k_bu_chunk_ls (hierarchy chunk pass)", "ms": round(kms["chunk"], 3),
+             "algorithmic_bytes": int(chunk_bytes), "leaf_pairs": int(pairs), "stored_nodes": int(stored),
+             "build_ms": round(kms["build"], 3), "trace_ms": round(kms["trace"], 3)}
+    log(f"denoised frame kernels: {chunk}")
     # image quality against 256 spp (frame index 7777: independent of both frames' streams)
     W, H = args.width, args.height
 
@@ -281,7 +305,7 @@ def denoised_leg(r, scene, args, cam, p_plain):
                                  "raw_1spp": round(psnr(raw, ref), 2), "box5_filter_1spp": round(psnr(box, ref), 2)},
            "ssim_vs_256spp": {"denoised_1spp": round(ssim(den, ref), 4), "plain": round(ssim(plain, ref), 4),
                               "raw_1spp": round(ssim(raw, ref), 4), "box5_filter_1spp": round(ssim(box, ref), 4)},
-           "weights": trained, "weights_meta": meta, "autotune_1spp": tuned1}
+           "weights": trained, "weights_meta": meta, "autotune_1spp": tuned1, "chunk_pass": chunk}
     log(f"denoised frame {ms:.2f} ms (no denoiser {ms_nodn:.2f}); PSNR {res['psnr_db_vs_256spp']}")
     if args.save_images:
         # centre crops for offline error analysis: the 256-spp mean albedo (AO distance ~0: every
@@ -545,6 +569,21 @@ def main():
         log(f"CPU baseline: {cpu}")
 
     value = dn["ms"] if dn else ms
+    # roofline of the dominant kernel of the frame that `value` times: the chunk pass of the
+    # denoised frame's build (the plain frame's trace kernel when there is no denoised frame)
+    if dn:
+        cp = dn["chunk_pass"]
+        ach = cp["algorithmic_bytes"] / (cp["ms"] * 1e-3) / 1e9
+        ctraffic = (dram_traffic(f"cells {args.cells}, leaf pairs on", DRAM_PROFILE_CHUNK)
+                    if args.config == "fcc" else None)
+        roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": ctraffic,
+                    "dram_frac": (round(ctraffic / (cp["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if ctraffic else None),
+                    "kernel": cp["kernel"], "algorithmic_bytes": cp["algorithmic_bytes"]}
+    else:
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kernel_name,
+                    "algorithmic_bytes": int(algo_bytes)}
     out = {
         "metric": "ms/frame 50M spheres 1080p AO: denoised 1spp vs plain 64spp @>=38dB; Mrays/s",
         "value": round(value, 3),
@@ -572,10 +611,12 @@ def main():
         "trace_ms": round(trace_ms, 3),
         "trace_work": split,
         "autotune": tuned,
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "dram_frac": (round(traffic / (trace_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None),
-                     "kernel": kernel_name, "algorithmic_bytes": int(algo_bytes)},
+        "roofline": roofline,
+        "roofline_plain_trace": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                                 "dram_frac": (round(traffic / (trace_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                                               if traffic else None),
+                                 "kernel": kernel_name, "algorithmic_bytes": int(algo_bytes)},
         "cpu_baseline": cpu,
         "orbit": orbit,
     }

@@ -214,8 +214,8 @@ MMHRT_API int32_t mmhrt_get_info(mmhrt_ctx ctx, uint64_t* n_spheres, float* boun
 MMHRT_API int32_t mmhrt_get_bvh(mmhrt_ctx ctx, float* nodes, uint32_t* leaf_ids, uint64_t* keys);
 
 /* Per-kernel timing of the last mmhrt_build / mmhrt_render on the context's stream, measured
- * with HIP events: times_ms[0..7] = morton, sort, hierarchy, bounds, trace, resolve, total build,
- * total render. */
+ * with HIP events: times_ms[0..8] = morton, sort, hierarchy, bounds, trace, resolve, total build,
+ * total render, and the hierarchy's chunk pass alone (n up to 9). */
 MMHRT_API int32_t mmhrt_get_timings(mmhrt_ctx ctx, float* times_ms, int32_t n);
 
 /* Instrumented trace counters of the last render (when enabled with mmhrt_set_option
@@ -290,7 +290,8 @@ MMHRT_API int32_t mmhrt_denoiser_train_step(mmhrt_ctx ctx, const uint16_t* feats
  * default 1), "dn_k32" (denoiser forward on the 16x16x32 MFMA kernels, default 1),
  * "dn_conv_shape" (test hook: forced launch shape of those kernels, 0 = automatic).
  * Diagnostics: "count_visits" (0/1) and, read-only in effect, "stack_overflow", "pair_visits",
- * "leaf_fetches" (refreshed by mmhrt_get_counters). */
+ * "leaf_fetches" (refreshed by mmhrt_get_counters), "sort_fault", and "leaf_pair_count" (get
+ * only: the leaf pairs of the current tree, counted on the device from its sorted keys). */
 MMHRT_API int32_t mmhrt_set_option(mmhrt_ctx ctx, const char* key, int64_t value);
 MMHRT_API int32_t mmhrt_get_option(mmhrt_ctx ctx, const char* key, int64_t* value);
 

@@ -146,6 +146,7 @@ int32_t mmhrt_destroy(mmhrt_ctx c) {
     release(c->sph); release(c->col); release(c->staging);
     release(c->meet); release(c->meet_far); release(c->ka); release(c->kb); release(c->kla); release(c->klb);
     release(c->va); release(c->vb); release(c->hist); release(c->scan_tmp); release(c->sort_fault);
+    release(c->pair_count);
     release(c->leaf_sph); release(c->leaf_id); release(c->nodes); release(c->flags); release(c->box_cnt);
     release(c->partial); release(c->info); release(c->tf_buf);
     release(c->fb_col); release(c->fb_depth); release(c->fb_t); release(c->fb_id); release(c->fb_rgba8);
@@ -436,9 +437,10 @@ int32_t mmhrt_get_timings(mmhrt_ctx c, float* t, int32_t n) {
     if (int e = check_ctx(c)) return e;
     if (!t || n <= 0) return fail(c, MMHRT_E_INVALID, "get_timings: bad output");
     hipStreamSynchronize(c->stream);
-    float v[8] = {elapsed(c, 0, 2), elapsed(c, 2, 3), elapsed(c, 3, 4), elapsed(c, 4, 5),
-                  elapsed(c, 6, 7), 0.0f,           elapsed(c, 0, 5), elapsed(c, 6, 7)};
-    for (int i = 0; i < n && i < 8; ++i) t[i] = v[i];
+    float v[9] = {elapsed(c, 0, 2), elapsed(c, 2, 3), elapsed(c, 3, 4), elapsed(c, 4, 5),
+                  elapsed(c, 6, 7), 0.0f,           elapsed(c, 0, 5), elapsed(c, 6, 7),
+                  elapsed(c, 8, 9)};
+    for (int i = 0; i < n && i < 9; ++i) t[i] = v[i];
     return MMHRT_OK;
 }
 
@@ -506,6 +508,10 @@ int32_t mmhrt_get_option(mmhrt_ctx c, const char* key, int64_t* value) {
             if (e != hipSuccess) return hip_fail(c, e, "get_option sort_fault");
         }
         *value = f;
+        return MMHRT_OK;
+    }
+    if (key && std::string(key) == "leaf_pair_count") {  // counted on the device over the current tree
+        if (int e = count_leaf_pairs(c, value)) return e;
         return MMHRT_OK;
     }
     int64_t* s = option_slot(c, key);

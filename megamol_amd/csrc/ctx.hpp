@@ -87,6 +87,7 @@ struct mmhrt_ctx_s {
     mmhrt::DevBuf<uint32_t> va, vb;  // values ping-pong
     mmhrt::DevBuf<uint32_t> hist, scan_tmp;
     mmhrt::DevBuf<uint32_t> sort_fault;  // set by a single-pass radix pass whose look-back wait ran out
+    mmhrt::DevBuf<uint64_t> pair_count;  // instrumentation: leaf pairs of the current tree
     mmhrt::DevBuf<float4> leaf_sph;
     mmhrt::DevBuf<uint32_t> leaf_id;
     mmhrt::DevBuf<mmhrt::Node> nodes;
@@ -213,6 +214,7 @@ void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t v, uint64_t n);
 int build_lbvh(mmhrt_ctx_s* c, int bits);
 int refit_lbvh(mmhrt_ctx_s* c);
 int reduce_scene(mmhrt_ctx_s* c);
+int count_leaf_pairs(mmhrt_ctx_s* c, int64_t* out);
 
 // trace.hip
 int render_frame(mmhrt_ctx_s* c, const FrameConsts& fc);

@@ -1197,6 +1197,25 @@ __global__ __launch_bounds__(256) void k_bu_tail(const K* __restrict__ keys, int
     bu_finish<K>(keys, n, nodes, meet, meet_far, flags, lin, lout, cnt[1 + r], cnt, info);
 }
 
+// Leaf pairs of the tree over sorted keys: leaves l and l + 1 both attach to split l exactly
+// when delta(l) is above delta(l - 1) and delta(l + 1). (Instrumentation: the build does not
+// run this.)
+template <typename K>
+__global__ __launch_bounds__(256) void k_count_pairs(const K* __restrict__ keys, int64_t n,
+                                                     unsigned long long* out) {
+    __shared__ unsigned long long s_sum;
+    if (threadIdx.x == 0) s_sum = 0;
+    __syncthreads();
+    unsigned long long own = 0;
+    for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n - 1; l += (int64_t)gridDim.x * blockDim.x) {
+        const int d = split_delta(keys, n, l);
+        if (d > split_delta(keys, n, l - 1) && d > split_delta(keys, n, l + 1)) ++own;
+    }
+    if (own) atomicAdd(&s_sum, own);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_sum) atomicAdd(out, s_sum);
+}
+
 // Single-sphere scene: one node whose two children are the same leaf.
 __global__ void k_single(const float4* __restrict__ leaf_sph, Node* nodes, SceneInfo* info) {
     float4 s = leaf_sph[0];
@@ -1330,7 +1349,9 @@ int hierarchy(mmhrt_ctx_s* c, const K* keys, const uint32_t* ids) {
         hipLaunchKernelGGL(k_regather, dim3(blocks_for(n, 256)), dim3(256), 0, s, cur_spheres(c), ids, c->leaf_sph.p,
                            n);
     if (n == 1) {
+        rec(c, 8);
         hipLaunchKernelGGL(k_single, dim3(1), dim3(1), 0, s, c->leaf_sph.p, c->nodes.p, c->info.p);
+        rec(c, 9);
         return 0;
     }
     uint64_t m = n - 1;
@@ -1354,6 +1375,7 @@ int hierarchy(mmhrt_ctx_s* c, const K* keys, const uint32_t* ids) {
     // leaves per chunk workgroup: 1024 (two per CU), 512 or 256; level-synchronous rounds unless
     // the option build_ls is 0
     const int bc = c->opt_build_chunk == 256 ? 256 : (c->opt_build_chunk == 512 ? 512 : 1024);
+    rec(c, 8);
     if (c->opt_build_ls != 0) {
         if (bc == 256)
             launch_chunk_ls<K, 256>(c, keys, ids, l0);
@@ -1370,6 +1392,7 @@ int hierarchy(mmhrt_ctx_s* c, const K* keys, const uint32_t* ids) {
     else
         hipLaunchKernelGGL((k_bu_chunk<K, 1024>), dim3(blocks_for(n, 1024)), dim3(1024), 0, s, keys, c->leaf_sph.p,
                            (int64_t)n, c->nodes.p, c->meet.p, c->meet_far.p, c->flags.p, l0, c->box_cnt.p, c->info.p);
+    rec(c, 9);
     // kernel rounds: as many as the last build needed plus two (the count comes back through
     // pinned host memory one build late; the tail finishes whatever is left, so it is only a hint)
     int rounds = BOX_ROUNDS;
@@ -1400,6 +1423,25 @@ int build_typed(mmhrt_ctx_s* c, K* ka, K* kb) {
 }
 
 }  // namespace
+
+int count_leaf_pairs(mmhrt_ctx_s* c, int64_t* out) {
+    *out = 0;
+    if (!c->built || !c->bvh_pairs || c->n < 3) return 0;  // two leaves: the root, always stored
+    if (int e = ensure(c, c->pair_count, 1, "pair count")) return e;
+    launch_fill_u32(c->stream, reinterpret_cast<uint32_t*>(c->pair_count.p), 0u, 2);
+    unsigned long long* o = reinterpret_cast<unsigned long long*>(c->pair_count.p);
+    const unsigned g = 4096;
+    if (c->bits == 63)
+        hipLaunchKernelGGL(k_count_pairs<uint64_t>, dim3(g), dim3(256), 0, c->stream, c->kla.p, (int64_t)c->n, o);
+    else
+        hipLaunchKernelGGL(k_count_pairs<uint32_t>, dim3(g), dim3(256), 0, c->stream, c->ka.p, (int64_t)c->n, o);
+    uint64_t h = 0;
+    hipError_t e = hipMemcpyAsync(&h, c->pair_count.p, 8, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "count_leaf_pairs");
+    *out = (int64_t)h;
+    return 0;
+}
 
 void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t v, uint64_t n) {
     if (n == 0) return;

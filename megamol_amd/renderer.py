@@ -262,9 +262,9 @@ class SphereRenderer:
         return nodes[:, :12].copy(), child, ids, keys, height, pairs
 
     def timings(self) -> dict:
-        t = (C.c_float * 8)()
-        self._ck(self.lib.mmhrt_get_timings(self.ctx, t, 8))
-        names = ["morton", "sort", "hierarchy", "boxes", "trace", "resolve", "build", "render"]
+        t = (C.c_float * 9)()
+        self._ck(self.lib.mmhrt_get_timings(self.ctx, t, 9))
+        names = ["morton", "sort", "hierarchy", "boxes", "trace", "resolve", "build", "render", "chunk"]
         return dict(zip(names, [float(x) for x in t]))
 
     def counters(self):

@@ -67,8 +67,9 @@ def test_lbvh_equals_oracle_lbvh(gpu_renderer, chunk, ls, ri, grid, os1):
     """Both chunk passes (level-synchronous rounds, or one climbing thread per leaf) at every
     chunk size, over radix tiles of 8, 16 or 32 keys per thread, with either radix pass
     (histogram + scans + scatter, or single-pass with look-back: os1), give the oracle's tree
-    node for node (sorted keys and leaf order included). grid > 0 caps the persistent chunk
-    workgroups, so that each one runs many chunks through its load pipeline."""
+    node for node (sorted keys and leaf order included; leaf pairs in their stored form). grid > 0
+    caps the persistent chunk workgroups, so that each one runs many chunks through its load
+    pipeline."""
     gpu_renderer.set_option("build_chunk", chunk)
     gpu_renderer.set_option("build_ls", ls)
     gpu_renderer.set_option("sort_ri", ri)
