@@ -128,9 +128,9 @@ int32_t mmhrt_create(int32_t device, mmhrt_ctx* out) {
     if (c->cu_count <= 0) c->cu_count = 1;
     for (int i = 0; i < 12; ++i) hipEventCreate(&c->tm.ev[i]);
     c->tm.init = true;
-    // pinned word through which a build tells the next one how many straddler rounds it needed
-    if (hipHostMalloc((void**)&c->rounds_host, sizeof(uint32_t), hipHostMallocDefault) == hipSuccess)
-        *c->rounds_host = 0;
+    // pinned words through which a build tells the next one its straddler rounds (all ones: none yet)
+    if (hipHostMalloc((void**)&c->rounds_host, BOX_SLOTS * sizeof(uint32_t), hipHostMallocDefault) == hipSuccess)
+        memset(c->rounds_host, 0xff, BOX_SLOTS * sizeof(uint32_t));
     else {
         (void)hipGetLastError();
         c->rounds_host = nullptr;
@@ -484,6 +484,7 @@ static int64_t* option_slot(mmhrt_ctx c, const char* key) {
     if (k == "build_grid") return &c->opt_build_grid;
     if (k == "sort_onesweep") return &c->opt_sort_onesweep;
     if (k == "leaf_pairs") return &c->opt_leaf_pairs;
+    if (k == "build_tail") return &c->opt_build_tail;
     if (k == "stack_overflow") return &c->stack_overflow;
     if (k == "pair_visits") return &c->pair_visits;
     if (k == "leaf_fetches") return &c->leaf_fetches;

@@ -15,6 +15,11 @@ namespace dn {
 struct Denoiser;
 }
 
+// Counters of the straddler rounds of an LBVH build (lbvh.hip): completion state, the worklist
+// size of each of the 32 kernel rounds and their successor, the rounds run, the generation.
+constexpr int BOX_ROUNDS = 32;
+constexpr int BOX_SLOTS = BOX_ROUNDS + 4;
+
 // BVH2 node, 64 bytes: both child boxes in one node so that one visit tests two children
 // with four 16-byte loads. child ref >= 0: internal node index; < 0: leaf ~k (k-th sphere in
 // Morton order), or with the side's pair flag in d.z (bit 0: child 0, bit 1: child 1) the leaf
@@ -94,7 +99,7 @@ struct mmhrt_ctx_s {
     mmhrt::DevBuf<uint32_t> flags;   // arrivals at the global meeting places (generation numbers)
     uint32_t build_gen = 0;          // generation number of the last hierarchy pass
     bool bvh_pairs = true;           // the current tree may reference leaf pairs (built with leaf_pairs on)
-    uint32_t* rounds_host = nullptr; // pinned: straddler kernel rounds the last build needed
+    uint32_t* rounds_host = nullptr; // pinned: the last build's straddler counters (BOX_SLOTS words)
     mmhrt::DevBuf<mmhrt::Rec4> rec4;  // 4-wide records (valid while rec4_valid)
     bool rec4_valid = false;
     mmhrt::DevBuf<uint32_t> rec4_list;  // collapse frontiers (two halves)
@@ -135,6 +140,7 @@ struct mmhrt_ctx_s {
     int64_t opt_build_grid = 0;     // test hook: cap on the persistent chunk workgroups (0 = as many as fit)
     int64_t opt_bvh4_spp = 8;       // 4-wide traversal for frames of at least this many spp (0: never)
     int64_t opt_leaf_pairs = 1;     // nodes over two leaves are not stored (their parents reference the pair)
+    int64_t opt_build_tail = 512;   // straddler worklist size from which one workgroup finishes the tree
     int64_t refits_since_build = 0;
     int64_t stack_overflow = 0;  // set by mmhrt_get_counters from the trace kernel's flag
     int64_t pair_visits = 0;     // set by mmhrt_get_counters: visits of leaf pairs (of the node visits)

@@ -665,10 +665,8 @@ __device__ __forceinline__ int split_delta(const K* __restrict__ keys, int64_t n
 // Arrival at a global meeting place: every split receives exactly two arrivals per build, so an
 // exchange of the build's generation number gen into flags[s] tells them apart (the second one
 // reads gen back) without clearing the 4n-byte array between builds; it starts as all ones.
-constexpr int BOX_ROUNDS = 32;        // kernel rounds before the single-workgroup finish
 constexpr int GEN_SLOT = BOX_ROUNDS + 3;  // box_cnt slot holding the generation number
 constexpr int ROUNDS_SLOT = BOX_ROUNDS + 2;  // box_cnt slot: kernel rounds this build needed
-constexpr uint32_t BOX_SMALL = 512;   // worklist size handed to the single workgroup
 
 __device__ __forceinline__ void leaf_box(const float4 s, float b[6]) {
     b[0] = s.x - s.w; b[1] = s.y - s.w; b[2] = s.z - s.w;
@@ -1150,11 +1148,13 @@ __device__ void bu_finish(const K* __restrict__ keys, int64_t n, Node* nodes, No
     if (threadIdx.x == 0) cnt[0] = 1u;
 }
 
-// Round r: worklist r lives in list[r & 1], its size in cnt[1 + r]; cnt[0] marks completion.
+// Round r: worklist r lives in list[r & 1], its size in cnt[1 + r]. cnt[0]: 0 while rounds run,
+// 2 once a round found its list no longer than tail_items (that round's number in
+// cnt[ROUNDS_SLOT]) and left it to the tail, 1 when the tree is complete.
 template <typename K>
 __global__ __launch_bounds__(256) void k_bu_round(const K* __restrict__ keys, int64_t n, Node* nodes, Node* meet,
                                                   int4* meet_far, uint32_t* flags, uint32_t* l0, uint32_t* l1,
-                                                  uint32_t* cnt, int r, SceneInfo* info) {
+                                                  uint32_t* cnt, int r, uint32_t tail_items, SceneInfo* info) {
     __shared__ uint32_t s_list[256];
     __shared__ uint32_t s_n, s_base;
     if (cnt[0]) return;
@@ -1162,10 +1162,10 @@ __global__ __launch_bounds__(256) void k_bu_round(const K* __restrict__ keys, in
     uint32_t n_in = cnt[1 + r];
     uint32_t* lin = (r & 1) ? l1 : l0;
     uint32_t* lout = (r & 1) ? l0 : l1;
-    if (n_in <= BOX_SMALL) {
-        if (blockIdx.x == 0) {
-            if (threadIdx.x == 0) cnt[ROUNDS_SLOT] = (uint32_t)r;
-            bu_finish<K>(keys, n, nodes, meet, meet_far, flags, lin, lout, n_in, cnt, info);
+    if (n_in <= tail_items) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            cnt[ROUNDS_SLOT] = (uint32_t)r;
+            cnt[0] = 2u;
         }
         return;
     }
@@ -1185,16 +1185,20 @@ __global__ __launch_bounds__(256) void k_bu_round(const K* __restrict__ keys, in
     }
 }
 
-// Whatever is left after the r kernel rounds launched: one workgroup finishes it.
+// The rest in one 1024-thread workgroup: the list a round left to it, or, when all r launched
+// rounds ran, list r.
 template <typename K>
-__global__ __launch_bounds__(256) void k_bu_tail(const K* __restrict__ keys, int64_t n, Node* nodes, Node* meet,
-                                                 int4* meet_far, uint32_t* flags, uint32_t* l0, uint32_t* l1,
-                                                 uint32_t* cnt, int r, SceneInfo* info) {
-    if (cnt[0]) return;
-    if (threadIdx.x == 0) cnt[ROUNDS_SLOT] = (uint32_t)r;
-    uint32_t* lin = (r & 1) ? l1 : l0;
-    uint32_t* lout = (r & 1) ? l0 : l1;
-    bu_finish<K>(keys, n, nodes, meet, meet_far, flags, lin, lout, cnt[1 + r], cnt, info);
+__global__ __launch_bounds__(1024) void k_bu_tail(const K* __restrict__ keys, int64_t n, Node* nodes, Node* meet,
+                                                  int4* meet_far, uint32_t* flags, uint32_t* l0, uint32_t* l1,
+                                                  uint32_t* cnt, int r, SceneInfo* info) {
+    const uint32_t st = cnt[0];
+    if (st == 1u) return;
+    const int rr = st == 2u ? (int)cnt[ROUNDS_SLOT] : r;
+    __syncthreads();  // every thread has read cnt before thread 0 writes it
+    if (threadIdx.x == 0) cnt[ROUNDS_SLOT] = (uint32_t)rr;
+    uint32_t* lin = (rr & 1) ? l1 : l0;
+    uint32_t* lout = (rr & 1) ? l0 : l1;
+    bu_finish<K>(keys, n, nodes, meet, meet_far, flags, lin, lout, cnt[1 + rr], cnt, info);
 }
 
 // Leaf pairs of the tree over sorted keys: leaves l and l + 1 both attach to split l exactly
@@ -1355,7 +1359,7 @@ int hierarchy(mmhrt_ctx_s* c, const K* keys, const uint32_t* ids) {
         return 0;
     }
     uint64_t m = n - 1;
-    if (int e = ensure(c, c->box_cnt, BOX_ROUNDS + 4, "box round counters")) return e;
+    if (int e = ensure(c, c->box_cnt, BOX_SLOTS, "box round counters")) return e;
     if (int e = ensure(c, c->meet, m, "meeting records")) return e;
     if (int e = ensure(c, c->meet_far, m, "meeting ranges")) return e;
     const bool fresh = !c->flags.p || c->flags.cap < m;
@@ -1393,20 +1397,26 @@ int hierarchy(mmhrt_ctx_s* c, const K* keys, const uint32_t* ids) {
         hipLaunchKernelGGL((k_bu_chunk<K, 1024>), dim3(blocks_for(n, 1024)), dim3(1024), 0, s, keys, c->leaf_sph.p,
                            (int64_t)n, c->nodes.p, c->meet.p, c->meet_far.p, c->flags.p, l0, c->box_cnt.p, c->info.p);
     rec(c, 9);
-    // kernel rounds: as many as the last build needed plus two (the count comes back through
-    // pinned host memory one build late; the tail finishes whatever is left, so it is only a hint)
+    // kernel rounds: as many as the last build ran before its tail, each with a grid for the list
+    // that round had (the counts come back through pinned host memory one build late; they are
+    // hints only: the rounds loop over any list size and the tail finishes whatever is left)
+    const uint32_t tail_items = (uint32_t)(c->opt_build_tail > 0 ? c->opt_build_tail : 1);
     int rounds = BOX_ROUNDS;
-    if (c->rounds_host) {
-        const int last = (int)*(volatile uint32_t*)c->rounds_host;
-        if (last > 0 && last + 2 < BOX_ROUNDS) rounds = last + 2;
+    const volatile uint32_t* last = c->rounds_host;
+    if (last && last[ROUNDS_SLOT] < (uint32_t)BOX_ROUNDS) rounds = (int)last[ROUNDS_SLOT];
+    for (int r = 0; r < rounds; ++r) {
+        unsigned grid = 1024;
+        if (last && last[ROUNDS_SLOT] < (uint32_t)BOX_ROUNDS) {
+            const uint64_t items = last[1 + r] + last[1 + r] / 4 + 256;
+            grid = (unsigned)std::min<uint64_t>(1024, (items + 255) / 256);
+        }
+        hipLaunchKernelGGL(k_bu_round<K>, dim3(grid), dim3(256), 0, s, keys, (int64_t)n, c->nodes.p, c->meet.p,
+                           c->meet_far.p, c->flags.p, l0, l1, c->box_cnt.p, r, tail_items, c->info.p);
     }
-    for (int r = 0; r < rounds; ++r)
-        hipLaunchKernelGGL(k_bu_round<K>, dim3(1024), dim3(256), 0, s, keys, (int64_t)n, c->nodes.p, c->meet.p,
-                           c->meet_far.p, c->flags.p, l0, l1, c->box_cnt.p, r, c->info.p);
-    hipLaunchKernelGGL(k_bu_tail<K>, dim3(1), dim3(256), 0, s, keys, (int64_t)n, c->nodes.p, c->meet.p, c->meet_far.p,
-                       c->flags.p, l0, l1, c->box_cnt.p, rounds, c->info.p);
+    hipLaunchKernelGGL(k_bu_tail<K>, dim3(1), dim3(1024), 0, s, keys, (int64_t)n, c->nodes.p, c->meet.p,
+                       c->meet_far.p, c->flags.p, l0, l1, c->box_cnt.p, rounds, c->info.p);
     if (c->rounds_host)
-        (void)hipMemcpyAsync(c->rounds_host, c->box_cnt.p + ROUNDS_SLOT, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+        (void)hipMemcpyAsync(c->rounds_host, c->box_cnt.p, BOX_SLOTS * sizeof(uint32_t), hipMemcpyDeviceToHost, s);
     return 0;
 }
 
