@@ -863,7 +863,8 @@ __global__ __launch_bounds__(BC) __attribute__((amdgpu_waves_per_eu(8, 8))) void
                                                     const float4* __restrict__ sph, float4* __restrict__ leaf_sph,
                                                     int64_t n64,
                                                     Node* __restrict__ nodes, Node* meet, int4* meet_far,
-                                                    uint32_t* flags, uint32_t* list, uint32_t* cnt, SceneInfo* info) {
+                                                    uint32_t* flags, uint32_t* list, uint32_t* cnt, SceneInfo* info,
+                                                    int wave_items) {
     __shared__ float s_box[BC][12];     // the two child boxes parked at each in-chunk split
     __shared__ uint16_t s_ref[BC][2];   // chunk-relative child references
     __shared__ uint16_t s_far[BC][2];   // chunk-relative far range ends
@@ -991,33 +992,52 @@ __global__ __launch_bounds__(BC) __attribute__((amdgpu_waves_per_eu(8, 8))) void
             deliver(k, k, s_delta[tid], s_delta[tid + 1], b, 0, false, f0, make_int2(0, 0), 0, k, &s_nl[0],
                     s_lst[0]);
         }
+        // merge the two children parked at a completed in-chunk split and deliver the node
+        auto merge = [&](int slot, uint32_t* nl_next, uint16_t* lst_next) {
+            float f[12];
+#pragma unroll
+            for (int a = 0; a < 12; ++a) f[a] = s_box[slot][a];
+            const int2 kd = make_int2(gref(s_ref[slot][0]), gref(s_ref[slot][1]));
+            const int h0 = s_h[slot][0], h1 = s_h[slot][1];
+            const int h = 1 + (h0 > h1 ? h0 : h1);
+            const int32_t l = cs + (int32_t)s_far[slot][0], r = cs + (int32_t)s_far[slot][1];
+            const int dl = s_dx[slot][0], dr = s_dx[slot][1];
+            float b[6];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                b[a] = fminf(f[a], f[6 + a]);
+                b[3 + a] = fmaxf(f[3 + a], f[9 + a]);
+            }
+            deliver(l, r, dl, dr, b, h, true, f, kd, pair_flags(kd, h0, h1), 0, nl_next, lst_next);
+        };
         __syncthreads();
         for (int rd = 0;; ++rd) {
             const int c0 = rd % 3;
             const uint32_t count = s_nl[c0];
             if (count == 0) break;
+            if (count <= (uint32_t)wave_items) {
+                // the last levels on wavefront 0 alone: one wavefront's LDS accesses complete in issue
+                // order, so its rounds need no workgroup barrier, only a fence that keeps the compiler
+                // from moving LDS accesses across a round
+                if (tid < 64) {
+                    uint32_t cw = count;
+                    for (int r2 = rd; cw != 0; ++r2) {
+                        if (tid == 0) s_nl[(r2 + 2) % 3] = 0;
+                        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                        for (uint32_t i = (uint32_t)tid; i < cw; i += 64)
+                            merge(s_lst[r2 & 1][i], &s_nl[(r2 + 1) % 3], s_lst[(r2 + 1) & 1]);
+                        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                        cw = s_nl[(r2 + 1) % 3];
+                    }
+                }
+                __syncthreads();
+                break;
+            }
             if (tid == 0) s_nl[(rd + 2) % 3] = 0;
             uint32_t* nl_next = &s_nl[(rd + 1) % 3];
             const uint16_t* lst = s_lst[rd & 1];
             uint16_t* lst_next = s_lst[(rd + 1) & 1];
-            for (uint32_t i = (uint32_t)tid; i < count; i += BC) {
-                const int slot = lst[i];
-                float f[12];
-#pragma unroll
-                for (int a = 0; a < 12; ++a) f[a] = s_box[slot][a];
-                const int2 kd = make_int2(gref(s_ref[slot][0]), gref(s_ref[slot][1]));
-                const int h0 = s_h[slot][0], h1 = s_h[slot][1];
-                const int h = 1 + (h0 > h1 ? h0 : h1);
-                const int32_t l = cs + (int32_t)s_far[slot][0], r = cs + (int32_t)s_far[slot][1];
-                const int dl = s_dx[slot][0], dr = s_dx[slot][1];
-                float b[6];
-#pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    b[a] = fminf(f[a], f[6 + a]);
-                    b[3 + a] = fmaxf(f[3 + a], f[9 + a]);
-                }
-                deliver(l, r, dl, dr, b, h, true, f, kd, pair_flags(kd, h0, h1), 0, nl_next, lst_next);
-            }
+            for (uint32_t i = (uint32_t)tid; i < count; i += BC) merge(lst[i], nl_next, lst_next);
             __syncthreads();
         }
         // next chunk's loads (its ids are already here; the ids after it start now), issued before
@@ -1339,7 +1359,8 @@ void launch_chunk_ls(mmhrt_ctx_s* c, const K* keys, const uint32_t* ids, uint32_
     const uint64_t n = c->n;
     hipLaunchKernelGGL((k_bu_chunk_ls<K, BC>), dim3(chunk_grid<K, BC>(c, n)), dim3(BC), 0, c->stream, keys, ids,
                        cur_spheres(c), c->leaf_sph.p, (int64_t)n, c->nodes.p,
-                       c->meet.p, c->meet_far.p, c->flags.p, list, c->box_cnt.p, c->info.p);
+                       c->meet.p, c->meet_far.p, c->flags.p, list, c->box_cnt.p, c->info.p,
+                       (int)std::max<int64_t>(0, std::min<int64_t>(c->opt_build_wave, BC)));
 }
 
 // Hierarchy and boxes over the sorted keys, gathering the leaf spheres on the way: ids[k] is

@@ -59,25 +59,29 @@ def leaf_pair_form(ochild):
     return child, pairs, ~is_pair
 
 
-@pytest.mark.parametrize("chunk,ls,ri,grid,os1", [(1024, 1, 16, 0, 0), (512, 1, 8, 0, 0), (256, 1, 32, 0, 0),
-                                                  (1024, 0, 16, 0, 0), (256, 0, 8, 0, 0), (1024, 1, 32, 3, 0),
-                                                  (256, 1, 16, 1, 0), (512, 1, 32, 7, 0), (1024, 1, 32, 0, 1),
-                                                  (1024, 1, 16, 0, 1), (512, 1, 8, 5, 1)])
-def test_lbvh_equals_oracle_lbvh(gpu_renderer, chunk, ls, ri, grid, os1):
+@pytest.mark.parametrize("chunk,ls,ri,grid,os1,wave", [
+    (1024, 1, 16, 0, 0, 64), (512, 1, 8, 0, 0, 64), (256, 1, 32, 0, 0, 64), (1024, 0, 16, 0, 0, 64),
+    (256, 0, 8, 0, 0, 64), (1024, 1, 32, 3, 0, 64), (256, 1, 16, 1, 0, 64), (512, 1, 32, 7, 0, 64),
+    (1024, 1, 32, 0, 1, 64), (1024, 1, 16, 0, 1, 64), (512, 1, 8, 5, 1, 64), (1024, 1, 32, 0, 0, 0),
+    (1024, 1, 32, 2, 0, 512), (256, 1, 16, 0, 0, 256)])
+def test_lbvh_equals_oracle_lbvh(gpu_renderer, chunk, ls, ri, grid, os1, wave):
     """Both chunk passes (level-synchronous rounds, or one climbing thread per leaf) at every
     chunk size, over radix tiles of 8, 16 or 32 keys per thread, with either radix pass
     (histogram + scans + scatter, or single-pass with look-back: os1), give the oracle's tree
     node for node (sorted keys and leaf order included; leaf pairs in their stored form). grid > 0
     caps the persistent chunk workgroups, so that each one runs many chunks through its load
-    pipeline."""
+    pipeline; wave is the round size up to which one wavefront runs the rounds alone (0: never;
+    the chunk size: from the first merge round on)."""
     gpu_renderer.set_option("build_chunk", chunk)
     gpu_renderer.set_option("build_ls", ls)
     gpu_renderer.set_option("sort_ri", ri)
     gpu_renderer.set_option("build_grid", grid)
     gpu_renderer.set_option("sort_onesweep", os1)
+    gpu_renderer.set_option("build_wave", wave)
     for bits in (30, 63):
         for n, dup in [(1000, False), (50000, True), (2, False), (3, True), (4097, False), (1_200_000, False)]:
-            case(f"lbvh n={n} dup={dup} bits={bits} chunk={chunk} ls={ls} ri={ri} grid={grid} onesweep={os1}")
+            case(f"lbvh n={n} dup={dup} bits={bits} chunk={chunk} ls={ls} ri={ri} grid={grid} onesweep={os1} "
+                 f"wave={wave}")
             g = np.random.default_rng(n + bits)
             xyzr = np.zeros((n, 4), np.float32)
             xyzr[:, :3] = g.random((n, 3)).astype(np.float32)
@@ -102,6 +106,7 @@ def test_lbvh_equals_oracle_lbvh(gpu_renderer, chunk, ls, ri, grid, os1):
     gpu_renderer.set_option("build_ls", 1)
     gpu_renderer.set_option("sort_ri", 32)
     gpu_renderer.set_option("build_grid", 0)
+    gpu_renderer.set_option("build_wave", 64)
 
 
 @pytest.mark.parametrize("ls", [1, 0])
