@@ -486,6 +486,7 @@ static int64_t* option_slot(mmhrt_ctx c, const char* key) {
     if (k == "leaf_pairs") return &c->opt_leaf_pairs;
     if (k == "build_tail") return &c->opt_build_tail;
     if (k == "build_wave") return &c->opt_build_wave;
+    if (k == "sort_xcd") return &c->opt_sort_xcd;
     if (k == "stack_overflow") return &c->stack_overflow;
     if (k == "pair_visits") return &c->pair_visits;
     if (k == "leaf_fetches") return &c->leaf_fetches;

@@ -141,7 +141,8 @@ struct mmhrt_ctx_s {
     int64_t opt_bvh4_spp = 8;       // 4-wide traversal for frames of at least this many spp (0: never)
     int64_t opt_leaf_pairs = 1;     // nodes over two leaves are not stored (their parents reference the pair)
     int64_t opt_build_tail = 512;   // straddler worklist size from which one workgroup finishes the tree
-    int64_t opt_build_wave = 64;    // chunk pass: rounds of at most this many merges run on one wavefront (0: none)
+    int64_t opt_build_wave = 64;    // chunk pass: rounds of at most this many (<= 64) merges run on one wavefront (0: none)
+    int64_t opt_sort_xcd = 1;       // radix scatter: each XCD takes one contiguous range of tiles (0: round robin)
     int64_t refits_since_build = 0;
     int64_t stack_overflow = 0;  // set by mmhrt_get_counters from the trace kernel's flag
     int64_t pair_visits = 0;     // set by mmhrt_get_counters: visits of leaf pairs (of the node visits)

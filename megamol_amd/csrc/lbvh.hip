@@ -266,16 +266,25 @@ template <typename K, int RI>
 __global__ __launch_bounds__(RB) void k_radix_scatter(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                       K* __restrict__ kout, uint32_t* __restrict__ vout, uint64_t n,
                                                       int shift, const uint32_t* __restrict__ hist,
-                                                      const uint32_t* __restrict__ offs) {
+                                                      const uint32_t* __restrict__ offs, int xcd) {
     constexpr int RT = RB * RI, RW = RT / (RB / 64);
     __shared__ K s_key[RT];
     __shared__ uint32_t s_val[RT];
     __shared__ uint32_t wc[RB / 64][256];  // per-wave running digit counts, then per-wave offsets
     __shared__ uint32_t goff[256];         // global offset of the digit's run minus its local start
     __shared__ uint32_t wsum[RB / 64];
+    // xcd: workgroups are dealt to the 8 XCDs in turn, so workgroup w takes tile (w mod 8) * per +
+    // w / 8 and each XCD scatters one contiguous range of tiles: the runs that consecutive tiles
+    // write for a digit are adjacent, and their shared cache lines meet in the same L2
+    uint32_t tile = blockIdx.x;
+    if (xcd) {
+        const uint32_t per = gridDim.x / 8;
+        tile = (blockIdx.x % 8) * per + blockIdx.x / 8;
+        if ((uint64_t)tile * RT >= n) return;
+    }
     const int wave = threadIdx.x >> 6;
     const uint32_t lane = __lane_id();
-    const uint64_t tbase = (uint64_t)blockIdx.x * RT;
+    const uint64_t tbase = (uint64_t)tile * RT;
     const uint64_t base = tbase + (uint64_t)wave * RW + lane;
     K key[RI];
     uint32_t val[RI];
@@ -296,7 +305,7 @@ __global__ __launch_bounds__(RB) void k_radix_scatter(const K* __restrict__ kin,
         for (int r = 0; r < RI; ++r) val[r] = (uint32_t)(base + (uint64_t)r * 64);
     }
     // tile-local digit starts: exclusive scan of this tile's digit counts (thread = digit)
-    const uint32_t cnt = hist[(size_t)blockIdx.x * 256 + threadIdx.x];
+    const uint32_t cnt = hist[(size_t)tile * 256 + threadIdx.x];
     uint32_t inc = cnt;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
@@ -310,7 +319,7 @@ __global__ __launch_bounds__(RB) void k_radix_scatter(const K* __restrict__ kin,
     uint32_t wb = 0;
     for (int w = 0; w < wave; ++w) wb += wsum[w];
     const uint32_t lstart = wb + inc - cnt;
-    goff[threadIdx.x] = offs[(size_t)blockIdx.x * 256 + threadIdx.x] - lstart;
+    goff[threadIdx.x] = offs[(size_t)tile * 256 + threadIdx.x] - lstart;
     // ranks inside the wave's segment
     uint32_t rank[RI];
 #pragma unroll
@@ -1024,8 +1033,7 @@ __global__ __launch_bounds__(BC) __attribute__((amdgpu_waves_per_eu(8, 8))) void
                     for (int r2 = rd; cw != 0; ++r2) {
                         if (tid == 0) s_nl[(r2 + 2) % 3] = 0;
                         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                        for (uint32_t i = (uint32_t)tid; i < cw; i += 64)
-                            merge(s_lst[r2 & 1][i], &s_nl[(r2 + 1) % 3], s_lst[(r2 + 1) & 1]);
+                        if ((uint32_t)tid < cw) merge(s_lst[r2 & 1][tid], &s_nl[(r2 + 1) % 3], s_lst[(r2 + 1) & 1]);
                         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
                         cw = s_nl[(r2 + 1) % 3];
                     }
@@ -1323,8 +1331,9 @@ int morton_sort_t(mmhrt_ctx_s* c, K* ka, K* kb, uint32_t* va, uint32_t* vb, uint
         hipLaunchKernelGGL(k_col_top, dim3(1), dim3(256), 0, c->stream, c->scan_tmp.p, groups);
         hipLaunchKernelGGL(k_col_offsets, dim3(groups), dim3(256), 0, c->stream, hist, tiles, groups, c->scan_tmp.p,
                            offs);
-        hipLaunchKernelGGL((k_radix_scatter<K, RI>), dim3(tiles), dim3(RB), 0, c->stream, src, vs, dst,
-                           p == passes - 1 ? vfinal : vd, n, shift, hist, offs);
+        const bool xcd = c->opt_sort_xcd != 0;
+        hipLaunchKernelGGL((k_radix_scatter<K, RI>), dim3(xcd ? 8 * ((tiles + 7) / 8) : tiles), dim3(RB), 0,
+                           c->stream, src, vs, dst, p == passes - 1 ? vfinal : vd, n, shift, hist, offs, (int)xcd);
         // ping-pong: pass p writes (kb, vb) when p is even, (ka, va) when odd
         src = dst;
         dst = (p & 1) ? kb : ka;
@@ -1360,7 +1369,7 @@ void launch_chunk_ls(mmhrt_ctx_s* c, const K* keys, const uint32_t* ids, uint32_
     hipLaunchKernelGGL((k_bu_chunk_ls<K, BC>), dim3(chunk_grid<K, BC>(c, n)), dim3(BC), 0, c->stream, keys, ids,
                        cur_spheres(c), c->leaf_sph.p, (int64_t)n, c->nodes.p,
                        c->meet.p, c->meet_far.p, c->flags.p, list, c->box_cnt.p, c->info.p,
-                       (int)std::max<int64_t>(0, std::min<int64_t>(c->opt_build_wave, BC)));
+                       (int)std::max<int64_t>(0, std::min<int64_t>(c->opt_build_wave, 64)));
 }
 
 // Hierarchy and boxes over the sorted keys, gathering the leaf spheres on the way: ids[k] is
@@ -1422,12 +1431,15 @@ int hierarchy(mmhrt_ctx_s* c, const K* keys, const uint32_t* ids) {
     // that round had (the counts come back through pinned host memory one build late; they are
     // hints only: the rounds loop over any list size and the tail finishes whatever is left)
     const uint32_t tail_items = (uint32_t)(c->opt_build_tail > 0 ? c->opt_build_tail : 1);
+    // (two rounds more than the last build needed, all of them after a build that needed none or
+    // when no build has run: the tail is fastest on short lists)
     int rounds = BOX_ROUNDS;
     const volatile uint32_t* last = c->rounds_host;
-    if (last && last[ROUNDS_SLOT] < (uint32_t)BOX_ROUNDS) rounds = (int)last[ROUNDS_SLOT];
+    const bool hint = last && last[ROUNDS_SLOT] > 0 && last[ROUNDS_SLOT] + 2 < (uint32_t)BOX_ROUNDS;
+    if (hint) rounds = (int)last[ROUNDS_SLOT] + 2;
     for (int r = 0; r < rounds; ++r) {
         unsigned grid = 1024;
-        if (last && last[ROUNDS_SLOT] < (uint32_t)BOX_ROUNDS) {
+        if (hint && r < (int)last[ROUNDS_SLOT]) {
             const uint64_t items = last[1 + r] + last[1 + r] / 4 + 256;
             grid = (unsigned)std::min<uint64_t>(1024, (items + 255) / 256);
         }

@@ -59,29 +59,32 @@ def leaf_pair_form(ochild):
     return child, pairs, ~is_pair
 
 
-@pytest.mark.parametrize("chunk,ls,ri,grid,os1,wave", [
-    (1024, 1, 16, 0, 0, 64), (512, 1, 8, 0, 0, 64), (256, 1, 32, 0, 0, 64), (1024, 0, 16, 0, 0, 64),
-    (256, 0, 8, 0, 0, 64), (1024, 1, 32, 3, 0, 64), (256, 1, 16, 1, 0, 64), (512, 1, 32, 7, 0, 64),
-    (1024, 1, 32, 0, 1, 64), (1024, 1, 16, 0, 1, 64), (512, 1, 8, 5, 1, 64), (1024, 1, 32, 0, 0, 0),
-    (1024, 1, 32, 2, 0, 512), (256, 1, 16, 0, 0, 256)])
-def test_lbvh_equals_oracle_lbvh(gpu_renderer, chunk, ls, ri, grid, os1, wave):
+@pytest.mark.parametrize("chunk,ls,ri,grid,os1,wave,tail", [
+    (1024, 1, 16, 0, 0, 64, 512), (512, 1, 8, 0, 0, 64, 512), (256, 1, 32, 0, 0, 64, 512),
+    (1024, 0, 16, 0, 0, 64, 512), (256, 0, 8, 0, 0, 64, 512), (1024, 1, 32, 3, 0, 64, 512),
+    (256, 1, 16, 1, 0, 64, 512), (512, 1, 32, 7, 0, 64, 512), (1024, 1, 32, 0, 1, 64, 512),
+    (1024, 1, 16, 0, 1, 64, 512), (512, 1, 8, 5, 1, 64, 512), (1024, 1, 32, 0, 0, 0, 512),
+    (256, 1, 16, 0, 0, 32, 1 << 30), (1024, 1, 32, 0, 0, 64, 4096)])
+def test_lbvh_equals_oracle_lbvh(gpu_renderer, chunk, ls, ri, grid, os1, wave, tail):
     """Both chunk passes (level-synchronous rounds, or one climbing thread per leaf) at every
     chunk size, over radix tiles of 8, 16 or 32 keys per thread, with either radix pass
     (histogram + scans + scatter, or single-pass with look-back: os1), give the oracle's tree
     node for node (sorted keys and leaf order included; leaf pairs in their stored form). grid > 0
     caps the persistent chunk workgroups, so that each one runs many chunks through its load
-    pipeline; wave is the round size up to which one wavefront runs the rounds alone (0: never;
-    the chunk size: from the first merge round on)."""
+    pipeline; wave is the round size up to which one wavefront runs the rounds alone (0: never);
+    tail is the straddler list size from which one workgroup finishes the tree (1 << 30: it takes
+    every straddler from the chunk pass on)."""
     gpu_renderer.set_option("build_chunk", chunk)
     gpu_renderer.set_option("build_ls", ls)
     gpu_renderer.set_option("sort_ri", ri)
     gpu_renderer.set_option("build_grid", grid)
     gpu_renderer.set_option("sort_onesweep", os1)
     gpu_renderer.set_option("build_wave", wave)
+    gpu_renderer.set_option("build_tail", tail)
     for bits in (30, 63):
         for n, dup in [(1000, False), (50000, True), (2, False), (3, True), (4097, False), (1_200_000, False)]:
             case(f"lbvh n={n} dup={dup} bits={bits} chunk={chunk} ls={ls} ri={ri} grid={grid} onesweep={os1} "
-                 f"wave={wave}")
+                 f"wave={wave} tail={tail}")
             g = np.random.default_rng(n + bits)
             xyzr = np.zeros((n, 4), np.float32)
             xyzr[:, :3] = g.random((n, 3)).astype(np.float32)
@@ -107,6 +110,7 @@ def test_lbvh_equals_oracle_lbvh(gpu_renderer, chunk, ls, ri, grid, os1, wave):
     gpu_renderer.set_option("sort_ri", 32)
     gpu_renderer.set_option("build_grid", 0)
     gpu_renderer.set_option("build_wave", 64)
+    gpu_renderer.set_option("build_tail", 512)
 
 
 @pytest.mark.parametrize("ls", [1, 0])
