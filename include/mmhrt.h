@@ -287,7 +287,9 @@ MMHRT_API int32_t mmhrt_denoiser_train_step(mmhrt_ctx ctx, const uint16_t* feats
  * tile: 8, 16 or 32), "sort_onesweep" (single-pass radix passes with look-back, default 0),
  * "leaf_pairs" (default 1: nodes over two leaves are not stored, their parents reference the
  * pair; 0: every node is stored), "build_tail" (straddler worklist size from which one
- * workgroup finishes the tree, default 512), "build_wave" (chunk-pass rounds of at most this
+ * workgroup finishes the tree, default 512; used when "build_span" is 0), "build_span" (straddlers
+ * merged by one workgroup per span of this many chunks before the tail, default 256; 0: kernel
+ * rounds), "build_wave" (chunk-pass rounds of at most this
  * many merges, up to 64, run on one wavefront without workgroup barriers, default 64; 0: never),
  * "sample_lanes" (samples of a pixel on the lanes for power-of-two spp,
  * default 1), "dn_k32" (denoiser forward on the 16x16x32 MFMA kernels, default 1),

@@ -146,7 +146,7 @@ int32_t mmhrt_destroy(mmhrt_ctx c) {
     release(c->sph); release(c->col); release(c->staging);
     release(c->meet); release(c->meet_far); release(c->ka); release(c->kb); release(c->kla); release(c->klb);
     release(c->va); release(c->vb); release(c->hist); release(c->scan_tmp); release(c->sort_fault);
-    release(c->pair_count);
+    release(c->pair_count); release(c->xlist);
     release(c->leaf_sph); release(c->leaf_id); release(c->nodes); release(c->flags); release(c->box_cnt);
     release(c->partial); release(c->info); release(c->tf_buf);
     release(c->fb_col); release(c->fb_depth); release(c->fb_t); release(c->fb_id); release(c->fb_rgba8);
@@ -487,6 +487,7 @@ static int64_t* option_slot(mmhrt_ctx c, const char* key) {
     if (k == "build_tail") return &c->opt_build_tail;
     if (k == "build_wave") return &c->opt_build_wave;
     if (k == "sort_xcd") return &c->opt_sort_xcd;
+    if (k == "build_span") return &c->opt_build_span;
     if (k == "stack_overflow") return &c->stack_overflow;
     if (k == "pair_visits") return &c->pair_visits;
     if (k == "leaf_fetches") return &c->leaf_fetches;

@@ -106,6 +106,7 @@ struct mmhrt_ctx_s {
     mmhrt::DevBuf<uint32_t> rec4_cnt;
     mmhrt::DevBuf<mmhrt::Node> meet;     // per split position: the records of its two children (build scratch)
     mmhrt::DevBuf<int4> meet_far;  // per split position: the children's far range ends and parent deltas
+    mmhrt::DevBuf<uint32_t> xlist;  // span pass: splits exported to the tail (two halves)
     mmhrt::DevBuf<uint32_t> box_cnt;  // worklist sizes of the bottom-up box rounds
     mmhrt::DevBuf<float> partial;
     mmhrt::DevBuf<mmhrt::SceneInfo> info;
@@ -143,6 +144,7 @@ struct mmhrt_ctx_s {
     int64_t opt_build_tail = 512;   // straddler worklist size from which one workgroup finishes the tree
     int64_t opt_build_wave = 64;    // chunk pass: rounds of at most this many (<= 64) merges run on one wavefront (0: none)
     int64_t opt_sort_xcd = 1;       // radix scatter: each XCD takes one contiguous range of tiles (0: round robin)
+    int64_t opt_build_span = 256;   // straddlers: one workgroup per span of this many chunks, then the tail (0: rounds)
     int64_t refits_since_build = 0;
     int64_t stack_overflow = 0;  // set by mmhrt_get_counters from the trace kernel's flag
     int64_t pair_visits = 0;     // set by mmhrt_get_counters: visits of leaf pairs (of the node visits)

@@ -1229,6 +1229,70 @@ __global__ __launch_bounds__(1024) void k_bu_tail(const K* __restrict__ keys, in
     bu_finish<K>(keys, n, nodes, meet, meet_far, flags, lin, lout, cnt[1 + rr], cnt, info);
 }
 
+// The straddlers of one span of leaves [lo, hi), one 1024-thread workgroup per span. A node whose
+// split lies inside the span (lo <= s <= hi - 2) has both children inside it, so the span's
+// workgroup merges it once both have arrived, level by level with barriers between (records are
+// read with volatile loads, as in the tail). A delivery that completes one of the span's two
+// boundary splits is exported instead, and the tail finishes the exported splits. Level 0 is
+// the chunk pass's list, which every span scans for its own splits. la and lb (n entries each)
+// hold the span's level lists at offset lo; xl is the export list (count in cnt[2]).
+template <typename K>
+__global__ __launch_bounds__(1024) void k_bu_span(const K* __restrict__ keys, int64_t n, Node* nodes, Node* meet,
+                                                  int4* meet_far, uint32_t* flags, const uint32_t* __restrict__ list0,
+                                                  uint32_t* la, uint32_t* lb, uint32_t* xl, uint32_t* cnt,
+                                                  int64_t span, SceneInfo* info) {
+    __shared__ uint32_t s_out;
+    const uint32_t gen = cnt[GEN_SLOT];
+    const int64_t lo = (int64_t)blockIdx.x * span;
+    const int64_t hi = lo + span < n ? lo + span : n;
+    auto inside = [&](int64_t sp) { return sp >= lo && sp <= hi - 2; };
+    auto bound = [&](int64_t sp) { return sp == hi - 1 || sp == lo - 1; };
+    uint32_t* lin = la + lo;
+    uint32_t* lout = lb + lo;
+    if (threadIdx.x == 0) s_out = 0;
+    __syncthreads();
+    const uint32_t n0 = cnt[1];
+    for (uint32_t i = threadIdx.x; i < n0; i += blockDim.x) {
+        const int64_t sp = (int64_t)list0[i];
+        if (inside(sp))
+            lin[atomicAdd(&s_out, 1u)] = (uint32_t)sp;
+        else if (sp == hi - 1)  // completed on the span's right boundary by the chunk pass
+            xl[atomicAdd(cnt + 2, 1u)] = (uint32_t)sp;
+    }
+    __threadfence_block();
+    __syncthreads();
+    uint32_t n_cur = s_out;
+    __syncthreads();
+    while (n_cur > 0) {
+        if (threadIdx.x == 0) s_out = 0;
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < n_cur; i += blockDim.x) {
+            const int64_t sp = (int64_t)((const volatile uint32_t*)lin)[i];
+            const int64_t s2 = bu_step<K, true>(keys, n, nodes, meet, meet_far, flags, sp, info, gen);
+            if (s2 < 0) continue;
+            if (inside(s2))
+                lout[atomicAdd(&s_out, 1u)] = (uint32_t)s2;
+            else if (bound(s2))
+                xl[atomicAdd(cnt + 2, 1u)] = (uint32_t)s2;
+        }
+        __threadfence_block();
+        __syncthreads();
+        n_cur = s_out;
+        uint32_t* t = lin;
+        lin = lout;
+        lout = t;
+        __syncthreads();
+    }
+}
+
+// The tail over the spans' export list (xl, count cnt[2]); xl2 is its ping-pong partner.
+template <typename K>
+__global__ __launch_bounds__(1024) void k_bu_xtail(const K* __restrict__ keys, int64_t n, Node* nodes, Node* meet,
+                                                   int4* meet_far, uint32_t* flags, uint32_t* xl, uint32_t* xl2,
+                                                   uint32_t* cnt, SceneInfo* info) {
+    bu_finish<K>(keys, n, nodes, meet, meet_far, flags, xl, xl2, cnt[2], cnt, info);
+}
+
 // Leaf pairs of the tree over sorted keys: leaves l and l + 1 both attach to split l exactly
 // when delta(l) is above delta(l - 1) and delta(l + 1). (Instrumentation: the build does not
 // run this.)
@@ -1430,6 +1494,20 @@ int hierarchy(mmhrt_ctx_s* c, const K* keys, const uint32_t* ids) {
     // kernel rounds: as many as the last build ran before its tail, each with a grid for the list
     // that round had (the counts come back through pinned host memory one build late; they are
     // hints only: the rounds loop over any list size and the tail finishes whatever is left)
+    if (c->opt_build_span > 0) {
+        // spans of build_span chunks, one workgroup each, then the tail over their exports (at
+        // most three per span: its right boundary from the chunk pass, its two boundary splits)
+        const int64_t span = c->opt_build_span * (int64_t)bc;
+        const uint64_t spans = (n + span - 1) / span;
+        const size_t xcap = 4 * spans + 64;
+        if (int e = ensure(c, c->xlist, 2 * xcap, "span export lists")) return e;
+        hipLaunchKernelGGL(k_bu_span<K>, dim3((unsigned)spans), dim3(1024), 0, s, keys, (int64_t)n, c->nodes.p,
+                           c->meet.p, c->meet_far.p, c->flags.p, l0, c->va.p, c->vb.p, c->xlist.p, c->box_cnt.p, span,
+                           c->info.p);
+        hipLaunchKernelGGL(k_bu_xtail<K>, dim3(1), dim3(1024), 0, s, keys, (int64_t)n, c->nodes.p, c->meet.p,
+                           c->meet_far.p, c->flags.p, c->xlist.p, c->xlist.p + xcap, c->box_cnt.p, c->info.p);
+        return 0;
+    }
     const uint32_t tail_items = (uint32_t)(c->opt_build_tail > 0 ? c->opt_build_tail : 1);
     // (two rounds more than the last build needed, all of them after a build that needed none or
     // when no build has run: the tail is fastest on short lists)

@@ -59,13 +59,14 @@ def leaf_pair_form(ochild):
     return child, pairs, ~is_pair
 
 
-@pytest.mark.parametrize("chunk,ls,ri,grid,os1,wave,tail", [
-    (1024, 1, 16, 0, 0, 64, 512), (512, 1, 8, 0, 0, 64, 512), (256, 1, 32, 0, 0, 64, 512),
-    (1024, 0, 16, 0, 0, 64, 512), (256, 0, 8, 0, 0, 64, 512), (1024, 1, 32, 3, 0, 64, 512),
-    (256, 1, 16, 1, 0, 64, 512), (512, 1, 32, 7, 0, 64, 512), (1024, 1, 32, 0, 1, 64, 512),
-    (1024, 1, 16, 0, 1, 64, 512), (512, 1, 8, 5, 1, 64, 512), (1024, 1, 32, 0, 0, 0, 512),
-    (256, 1, 16, 0, 0, 32, 1 << 30), (1024, 1, 32, 0, 0, 64, 4096)])
-def test_lbvh_equals_oracle_lbvh(gpu_renderer, chunk, ls, ri, grid, os1, wave, tail):
+@pytest.mark.parametrize("chunk,ls,ri,grid,os1,wave,tail,span", [
+    (1024, 1, 16, 0, 0, 64, 512, 256), (512, 1, 8, 0, 0, 64, 512, 256), (256, 1, 32, 0, 0, 64, 512, 256),
+    (1024, 0, 16, 0, 0, 64, 512, 256), (256, 0, 8, 0, 0, 64, 512, 256), (1024, 1, 32, 3, 0, 64, 512, 256),
+    (256, 1, 16, 1, 0, 64, 512, 256), (512, 1, 32, 7, 0, 64, 512, 256), (1024, 1, 32, 0, 1, 64, 512, 256),
+    (1024, 1, 16, 0, 1, 64, 512, 256), (512, 1, 8, 5, 1, 64, 512, 256), (1024, 1, 32, 0, 0, 0, 512, 256),
+    (256, 1, 16, 0, 0, 32, 1 << 30, 0), (1024, 1, 32, 0, 0, 64, 4096, 0), (1024, 1, 32, 0, 0, 64, 512, 0),
+    (256, 1, 16, 0, 0, 64, 512, 1), (1024, 0, 8, 2, 0, 64, 512, 16)])
+def test_lbvh_equals_oracle_lbvh(gpu_renderer, chunk, ls, ri, grid, os1, wave, tail, span):
     """Both chunk passes (level-synchronous rounds, or one climbing thread per leaf) at every
     chunk size, over radix tiles of 8, 16 or 32 keys per thread, with either radix pass
     (histogram + scans + scatter, or single-pass with look-back: os1), give the oracle's tree
@@ -73,7 +74,8 @@ def test_lbvh_equals_oracle_lbvh(gpu_renderer, chunk, ls, ri, grid, os1, wave, t
     caps the persistent chunk workgroups, so that each one runs many chunks through its load
     pipeline; wave is the round size up to which one wavefront runs the rounds alone (0: never);
     tail is the straddler list size from which one workgroup finishes the tree (1 << 30: it takes
-    every straddler from the chunk pass on)."""
+    every straddler from the chunk pass on) when span is 0 (kernel rounds); span > 0 merges the
+    straddlers with one workgroup per span of that many chunks before the tail."""
     gpu_renderer.set_option("build_chunk", chunk)
     gpu_renderer.set_option("build_ls", ls)
     gpu_renderer.set_option("sort_ri", ri)
@@ -81,10 +83,11 @@ def test_lbvh_equals_oracle_lbvh(gpu_renderer, chunk, ls, ri, grid, os1, wave, t
     gpu_renderer.set_option("sort_onesweep", os1)
     gpu_renderer.set_option("build_wave", wave)
     gpu_renderer.set_option("build_tail", tail)
+    gpu_renderer.set_option("build_span", span)
     for bits in (30, 63):
         for n, dup in [(1000, False), (50000, True), (2, False), (3, True), (4097, False), (1_200_000, False)]:
             case(f"lbvh n={n} dup={dup} bits={bits} chunk={chunk} ls={ls} ri={ri} grid={grid} onesweep={os1} "
-                 f"wave={wave} tail={tail}")
+                 f"wave={wave} tail={tail} span={span}")
             g = np.random.default_rng(n + bits)
             xyzr = np.zeros((n, 4), np.float32)
             xyzr[:, :3] = g.random((n, 3)).astype(np.float32)
@@ -111,6 +114,7 @@ def test_lbvh_equals_oracle_lbvh(gpu_renderer, chunk, ls, ri, grid, os1, wave, t
     gpu_renderer.set_option("build_grid", 0)
     gpu_renderer.set_option("build_wave", 64)
     gpu_renderer.set_option("build_tail", 512)
+    gpu_renderer.set_option("build_span", 256)
 
 
 @pytest.mark.parametrize("ls", [1, 0])
