@@ -1229,68 +1229,93 @@ __global__ __launch_bounds__(1024) void k_bu_tail(const K* __restrict__ keys, in
     bu_finish<K>(keys, n, nodes, meet, meet_far, flags, lin, lout, cnt[1 + rr], cnt, info);
 }
 
+constexpr uint32_t SPAN_LDS = 2048;  // level-list entries a span workgroup keeps in LDS
+
+// The levels of one workgroup's merges over the splits completed before it: each level merges
+// the splits the level before completed, with one barrier between levels (three counters rotate:
+// read, appended, cleared). A completed split inside [lo, hi - 2] goes to the next level, any
+// other (a boundary split lo - 1 or hi - 1) to the export list xl (count *xn). The level lists
+// are LDS (s_l) when level 0 fits, else the global lists gl0 and gl1. Records are read with
+// volatile loads: the previous level's stores, fenced before the barrier, are in L2.
+template <typename K>
+__device__ void span_levels(const K* __restrict__ keys, int64_t n, Node* nodes, Node* meet, int4* meet_far,
+                            uint32_t* flags, SceneInfo* info, uint32_t gen, int64_t lo, int64_t hi, uint32_t n0,
+                            bool in_lds, uint32_t* gl0, uint32_t* gl1, uint32_t (*s_l)[SPAN_LDS], uint32_t* s_nc,
+                            uint32_t* xl, uint32_t* xn) {
+    uint32_t n_cur = n0;
+    for (int lv = 0; n_cur > 0; ++lv) {
+        uint32_t* lin = in_lds ? s_l[lv & 1] : ((lv & 1) ? gl1 : gl0);
+        uint32_t* lout = in_lds ? s_l[(lv + 1) & 1] : ((lv & 1) ? gl0 : gl1);
+        uint32_t* nc_next = &s_nc[(lv + 1) % 3];
+        if (threadIdx.x == 0) s_nc[(lv + 2) % 3] = 0;
+        for (uint32_t i = threadIdx.x; i < n_cur; i += blockDim.x) {
+            const int64_t sp = (int64_t)((const volatile uint32_t*)lin)[i];
+            const int64_t s2 = bu_step<K, true>(keys, n, nodes, meet, meet_far, flags, sp, info, gen);
+            if (s2 < 0) continue;
+            if (s2 >= lo && s2 <= hi - 2)
+                lout[atomicAdd(nc_next, 1u)] = (uint32_t)s2;
+            else
+                xl[atomicAdd(xn, 1u)] = (uint32_t)s2;
+        }
+        __threadfence_block();
+        __syncthreads();
+        n_cur = s_nc[(lv + 1) % 3];
+    }
+}
+
 // The straddlers of one span of leaves [lo, hi), one 1024-thread workgroup per span. A node whose
 // split lies inside the span (lo <= s <= hi - 2) has both children inside it, so the span's
-// workgroup merges it once both have arrived, level by level with barriers between (records are
-// read with volatile loads, as in the tail). A delivery that completes one of the span's two
-// boundary splits is exported instead, and the tail finishes the exported splits. Level 0 is
-// the chunk pass's list, which every span scans for its own splits. la and lb (n entries each)
-// hold the span's level lists at offset lo; xl is the export list (count in cnt[2]).
+// workgroup merges it once both have arrived; a delivery that completes one of the span's two
+// boundary splits is exported to xl (count cnt[2]) instead, and k_bu_top finishes the exported
+// splits. Level 0 is the chunk pass's list, which every span scans for its own splits (those at
+// hi - 1, completed on the span's right boundary by the chunk pass, are exported at once). la
+// and lb (n entries each) hold the span's level lists at offset lo when LDS cannot.
 template <typename K>
 __global__ __launch_bounds__(1024) void k_bu_span(const K* __restrict__ keys, int64_t n, Node* nodes, Node* meet,
                                                   int4* meet_far, uint32_t* flags, const uint32_t* __restrict__ list0,
                                                   uint32_t* la, uint32_t* lb, uint32_t* xl, uint32_t* cnt,
                                                   int64_t span, SceneInfo* info) {
-    __shared__ uint32_t s_out;
+    __shared__ uint32_t s_l[2][SPAN_LDS];
+    __shared__ uint32_t s_nc[3];
     const uint32_t gen = cnt[GEN_SLOT];
     const int64_t lo = (int64_t)blockIdx.x * span;
     const int64_t hi = lo + span < n ? lo + span : n;
-    auto inside = [&](int64_t sp) { return sp >= lo && sp <= hi - 2; };
-    auto bound = [&](int64_t sp) { return sp == hi - 1 || sp == lo - 1; };
-    uint32_t* lin = la + lo;
-    uint32_t* lout = lb + lo;
-    if (threadIdx.x == 0) s_out = 0;
+    if (threadIdx.x < 3) s_nc[threadIdx.x] = 0;
     __syncthreads();
     const uint32_t n0 = cnt[1];
     for (uint32_t i = threadIdx.x; i < n0; i += blockDim.x) {
         const int64_t sp = (int64_t)list0[i];
-        if (inside(sp))
-            lin[atomicAdd(&s_out, 1u)] = (uint32_t)sp;
-        else if (sp == hi - 1)  // completed on the span's right boundary by the chunk pass
+        if (sp >= lo && sp <= hi - 2) {
+            const uint32_t k = atomicAdd(&s_nc[0], 1u);
+            if (k < SPAN_LDS) s_l[0][k] = (uint32_t)sp;
+            la[lo + k] = (uint32_t)sp;
+        } else if (sp == hi - 1) {
             xl[atomicAdd(cnt + 2, 1u)] = (uint32_t)sp;
+        }
     }
     __threadfence_block();
     __syncthreads();
-    uint32_t n_cur = s_out;
-    __syncthreads();
-    while (n_cur > 0) {
-        if (threadIdx.x == 0) s_out = 0;
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i < n_cur; i += blockDim.x) {
-            const int64_t sp = (int64_t)((const volatile uint32_t*)lin)[i];
-            const int64_t s2 = bu_step<K, true>(keys, n, nodes, meet, meet_far, flags, sp, info, gen);
-            if (s2 < 0) continue;
-            if (inside(s2))
-                lout[atomicAdd(&s_out, 1u)] = (uint32_t)s2;
-            else if (bound(s2))
-                xl[atomicAdd(cnt + 2, 1u)] = (uint32_t)s2;
-        }
-        __threadfence_block();
-        __syncthreads();
-        n_cur = s_out;
-        uint32_t* t = lin;
-        lin = lout;
-        lout = t;
-        __syncthreads();
-    }
+    const uint32_t m = s_nc[0];
+    span_levels<K>(keys, n, nodes, meet, meet_far, flags, info, gen, lo, hi, m, m <= SPAN_LDS, la + lo, lb + lo, s_l,
+                   s_nc, xl, cnt + 2);
 }
 
-// The tail over the spans' export list (xl, count cnt[2]); xl2 is its ping-pong partner.
+// The top of the tree: one workgroup merges the spans' exported splits (xl, count cnt[2]) up to
+// the root; xl2 is the second global list when LDS cannot hold them.
 template <typename K>
-__global__ __launch_bounds__(1024) void k_bu_xtail(const K* __restrict__ keys, int64_t n, Node* nodes, Node* meet,
-                                                   int4* meet_far, uint32_t* flags, uint32_t* xl, uint32_t* xl2,
-                                                   uint32_t* cnt, SceneInfo* info) {
-    bu_finish<K>(keys, n, nodes, meet, meet_far, flags, xl, xl2, cnt[2], cnt, info);
+__global__ __launch_bounds__(1024) void k_bu_top(const K* __restrict__ keys, int64_t n, Node* nodes, Node* meet,
+                                                 int4* meet_far, uint32_t* flags, uint32_t* xl, uint32_t* xl2,
+                                                 uint32_t* cnt, SceneInfo* info) {
+    __shared__ uint32_t s_l[2][SPAN_LDS];
+    __shared__ uint32_t s_nc[3];
+    const uint32_t gen = cnt[GEN_SLOT];
+    const uint32_t n0 = cnt[2];
+    if (threadIdx.x < 3) s_nc[threadIdx.x] = 0;
+    for (uint32_t i = threadIdx.x; i < n0 && i < SPAN_LDS; i += blockDim.x) s_l[0][i] = xl[i];
+    __syncthreads();
+    // every split lies inside [0, n - 2]: nothing is exported (cnt[3] is an unused counter here)
+    span_levels<K>(keys, n, nodes, meet, meet_far, flags, info, gen, 0, n, n0, n0 <= SPAN_LDS, xl, xl2, s_l, s_nc,
+                   xl2, cnt + 3);
 }
 
 // Leaf pairs of the tree over sorted keys: leaves l and l + 1 both attach to split l exactly
@@ -1504,7 +1529,7 @@ int hierarchy(mmhrt_ctx_s* c, const K* keys, const uint32_t* ids) {
         hipLaunchKernelGGL(k_bu_span<K>, dim3((unsigned)spans), dim3(1024), 0, s, keys, (int64_t)n, c->nodes.p,
                            c->meet.p, c->meet_far.p, c->flags.p, l0, c->va.p, c->vb.p, c->xlist.p, c->box_cnt.p, span,
                            c->info.p);
-        hipLaunchKernelGGL(k_bu_xtail<K>, dim3(1), dim3(1024), 0, s, keys, (int64_t)n, c->nodes.p, c->meet.p,
+        hipLaunchKernelGGL(k_bu_top<K>, dim3(1), dim3(1024), 0, s, keys, (int64_t)n, c->nodes.p, c->meet.p,
                            c->meet_far.p, c->flags.p, c->xlist.p, c->xlist.p + xcap, c->box_cnt.p, c->info.p);
         return 0;
     }
