@@ -75,6 +75,8 @@ ABLATION = {
         ("lane per pixel + ballot compaction (refill at 32 idle lanes)",
          {"stack_lds": 16, "sample_lanes": 0, "refill": 32}),
         ("sample lanes, 2 workgroups per CU", {"stack_lds": 16, "refill": 0, "blocks_per_cu": 2}),
+        ("tree with leaf pairs (sample-lanes kernel with the pair branch)",
+         {"stack_lds": 16, "refill": 0, "leaf_pairs": 1}),
     ],
     "1spp": [
         ("lane per pixel, work stealing, LDS stack 16, 8x8 tiles, no compaction", {"stack_lds": 16, "refill": 0}),
@@ -84,16 +86,27 @@ ABLATION = {
         ("LDS stack 8 entries", {"stack_lds": 8, "refill": 0}),
         ("LDS stack 32 entries", {"stack_lds": 32, "refill": 0}),
         ("4 workgroups per CU", {"stack_lds": 16, "refill": 0, "blocks_per_cu": 4}),
+        ("tree without leaf pairs", {"stack_lds": 16, "refill": 0, "leaf_pairs": 0}),
     ],
 }
+# the tree each frame's rows trace unless a row says otherwise (the autotuner's choice for it)
+ABLATION_PAIRS = {"64spp": 0, "1spp": 1}
 
 
 def run_ablation(r, cam, args):
-    """Trace time (HIP events) and Mrays/s of each ABLATION variant on one LBVH of the scene."""
+    """Trace time (HIP events) and Mrays/s of each ABLATION variant on one LBVH of the scene (rebuilt
+    when a row asks for the other tree: with or without leaf pairs)."""
     from megamol_amd import _lib as L, params
-    keys = sorted({k for fr in ABLATION.values() for _, o in fr for k in o})
+    keys = sorted({k for fr in ABLATION.values() for _, o in fr for k in o} - {"leaf_pairs"})
     defaults = {k: r.get_option(k) for k in keys}
-    r.build(L.BUILD_REBUILD, args.bits)
+    pairs_default = r.get_option("leaf_pairs")
+    built = {"pairs": None}
+
+    def tree(pairs):
+        if built["pairs"] != pairs:
+            r.set_option("leaf_pairs", pairs)
+            r.build(L.BUILD_REBUILD, args.bits)
+            built["pairs"] = pairs
     frames = {
         "64spp": params(args.width, args.height, spp=args.spp, mode=L.MODE_AO, ao_distance=args.ao_distance,
                         background=(0.0, 0.0, 0.0, 1.0), flags=L.FLAG_NO_DOWNLOAD),
@@ -103,6 +116,7 @@ def run_ablation(r, cam, args):
     rows = []
     for fname, variants in ABLATION.items():
         p = frames[fname]
+        tree(ABLATION_PAIRS[fname])
         r.set_option("count_visits", 1)
         r.render_async(cam, p)
         _, _, rays = r.counters()
@@ -112,7 +126,9 @@ def run_ablation(r, cam, args):
             for k in keys:
                 r.set_option(k, defaults[k])
             for k, v in opts.items():
-                r.set_option(k, v)
+                if k != "leaf_pairs":
+                    r.set_option(k, v)
+            tree(opts.get("leaf_pairs", ABLATION_PAIRS[fname]))
             r.render_async(cam, p)
             r.sync()
             same = bool(np.array_equal(r.download(args.width, args.height, outputs=("hit_id",)).hit_id, ref_id))
@@ -126,6 +142,7 @@ def run_ablation(r, cam, args):
             log(f"{fname} | {name}: {ms:.3f} ms, {rays / (ms * 1e-3) / 1e6:.0f} Mrays/s")
     for k in keys:
         r.set_option(k, defaults[k])
+    r.set_option("leaf_pairs", pairs_default)
     print(json.dumps({"ablation": rows, "workload": f"{args.config}, {args.width}x{args.height}, AO distance "
                                                  f"{args.ao_distance}"}), flush=True)
 
