@@ -31,17 +31,6 @@ struct __align__(16) Node {
     int4 d;    // child0, child1, pair flags, height
 };
 
-// 4-wide record of node X (same index as X): the boxes and references of X's grandchildren
-// (a leaf child takes one slot itself, its pair slot is empty), X's own children and height.
-// Written top-down for the nodes at even depth; the traversal of high-spp frames starts at the
-// root's record and descends two levels per visit.
-constexpr int32_t REC4_EMPTY = (int32_t)0x80000000;
-struct __align__(16) Rec4 {
-    float4 b[6];  // slot k: lo = f[6k..6k+2], hi = f[6k+3..6k+5]
-    int4 ref;     // slot references: node index, ~leaf, or REC4_EMPTY
-    int4 meta;    // X's children (node index or ~leaf), height, 0
-};
-
 // Scene facts computed on the device by the bounds reduction.
 struct SceneInfo {
     float clo[3], chi[3];  // centre bounds
@@ -100,10 +89,6 @@ struct mmhrt_ctx_s {
     uint32_t build_gen = 0;          // generation number of the last hierarchy pass
     bool bvh_pairs = true;           // the current tree may reference leaf pairs (built with leaf_pairs on)
     uint32_t* rounds_host = nullptr; // pinned: the last build's straddler counters (BOX_SLOTS words)
-    mmhrt::DevBuf<mmhrt::Rec4> rec4;  // 4-wide records (valid while rec4_valid)
-    bool rec4_valid = false;
-    mmhrt::DevBuf<uint32_t> rec4_list;  // collapse frontiers (two halves)
-    mmhrt::DevBuf<uint32_t> rec4_cnt;
     mmhrt::DevBuf<mmhrt::Node> meet;     // per split position: the records of its two children (build scratch)
     mmhrt::DevBuf<int4> meet_far;  // per split position: the children's far range ends and parent deltas
     mmhrt::DevBuf<uint32_t> xlist;  // span pass: splits exported to the tail (two halves)
@@ -139,7 +124,6 @@ struct mmhrt_ctx_s {
     int64_t opt_sort_ri = 32;       // radix sort keys per thread per tile: 8, 16 or 32 (32 for 32-bit keys only)
     int64_t opt_sort_onesweep = 0;  // single-pass radix passes with decoupled look-back (0: histogram + scans + scatter)
     int64_t opt_build_grid = 0;     // test hook: cap on the persistent chunk workgroups (0 = as many as fit)
-    int64_t opt_bvh4_spp = 8;       // 4-wide traversal for frames of at least this many spp (0: never)
     int64_t opt_leaf_pairs = 1;     // nodes over two leaves are not stored (their parents reference the pair)
     int64_t opt_build_tail = 512;   // straddler worklist size from which one workgroup finishes the tree
     int64_t opt_build_wave = 64;    // chunk pass: rounds of at most this many (<= 64) merges run on one wavefront (0: none)
