@@ -105,6 +105,7 @@ def test_lbvh_equals_oracle_lbvh(gpu_renderer, chunk, ls, ri, grid, os1, wave, t
             np.testing.assert_array_equal(child[stored], echild[stored])
             np.testing.assert_array_equal(pairs[stored], epairs[stored])
             np.testing.assert_array_equal(box[stored].view(np.uint32), obox[stored].view(np.uint32))
+            assert gpu_renderer.get_option("leaf_pair_count") == int((~stored).sum())
             assert height[0] >= int(np.ceil(np.log2(n)))
             assert gpu_renderer.get_option("sort_fault") == 0
     gpu_renderer.set_option("sort_onesweep", 0)
@@ -137,6 +138,7 @@ def test_lbvh_without_leaf_pairs_is_the_oracle_tree(gpu_renderer, ls):
             np.testing.assert_array_equal(child, ochild)
             np.testing.assert_array_equal(box.view(np.uint32), obox.view(np.uint32))
             assert not pairs.any()
+            assert gpu_renderer.get_option("leaf_pair_count") == 0
     finally:
         gpu_renderer.set_option("leaf_pairs", 1)
         gpu_renderer.set_option("build_ls", 1)

@@ -417,3 +417,38 @@ def test_bounce_closed_forms():
     # per-sample values are 0.8 (escaped or primary miss) or 0.1 (capped); 4-sample averages lie on that grid
     grid = {round((k * 0.1 + (4 - k) * 0.8) / 4, 5) for k in range(5)}
     assert vals <= grid and len(vals) > 1
+
+
+def _delta(keys, i, bits):
+    """delta(i) of the sorted keys as the LBVH defines it: the common prefix of keys i and i + 1,
+    key width + common prefix of the indices when the keys are equal, -1 past either end."""
+    n = len(keys)
+    if i < 0 or i >= n - 1:
+        return -1
+    w = 32 if bits == 30 else 64
+    a, b = int(keys[i]), int(keys[i + 1])
+    if a == b:
+        return w + (32 - (i ^ (i + 1)).bit_length())
+    return w - (a ^ b).bit_length()
+
+
+@pytest.mark.parametrize("n,dups,bits", [(3, False, 30), (300, False, 30), (300, True, 30), (2000, True, 30),
+                                         (1000, True, 63)])
+def test_leaf_pair_count_rule_matches_the_tree(n, dups, bits):
+    """The rule the device counts leaf pairs with (leaves l and l + 1 both attach to split l when
+    delta(l) exceeds delta(l - 1) and delta(l + 1)) against the oracle tree's non-root nodes
+    whose two children are leaves; and every such node covers two adjacent leaves."""
+    g = np.random.default_rng(n + 1)
+    xyzr = np.zeros((n, 4), np.float32)
+    xyzr[:, :3] = g.random((n, 3)).astype(np.float32)
+    if dups:
+        xyzr[n // 2:, :3] = xyzr[: n - n // 2, :3]
+    xyzr[:, 3] = 0.01
+    _, child, _, keys = O.Bvh(xyzr, bits).arrays()
+    d = [_delta(keys, i, bits) for i in range(-1, n)]  # d[i + 1] = delta(i)
+    rule = sum(1 for l in range(n - 1) if d[l + 1] > d[l] and d[l + 1] > d[l + 2])
+    both = (child[:, 0] < 0) & (child[:, 1] < 0)
+    both[0] = False  # the root is always stored
+    assert rule == int(both.sum())
+    for i in np.nonzero(both)[0]:
+        assert ~child[i, 1] == ~child[i, 0] + 1
