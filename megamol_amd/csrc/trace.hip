@@ -512,10 +512,11 @@ __device__ __forceinline__ void shade_sample(const TraceArgs& A, uint32_t px, ui
 // Sample s of a pixel draws its jitter from the pixel's stream advanced by 2s steps
 // (lcg_skip), and the group folds its samples into the pixel in sample order (colour sums,
 // first strictly nearer hit, visibility, sample-0 features), so every output is bit-identical
-// to the lane-per-pixel kernels. Held to 96 VGPRs (5 wavefronts per SIMD instead of 4, no spills):
-// 64-spp AO 33.8 -> 32.6 ms, Lambert 22.9 -> 21.2 ms.
+// to the lane-per-pixel kernels. Held to 72 VGPRs (7 wavefronts per SIMD instead of 4, a few
+// registers spilled): 64-spp AO 33.8 -> 30.6 ms, Lambert 22.9 -> 20.8 ms (5 wavefronts: 32.6 and
+// 21.2 ms, 6: 31.1 and 21.3, 8: 30.8 and 21.0).
 template <int S, int MODE, bool COUNT, bool PAIRS>
-__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_trace_sl(TraceArgs A) {
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_trace_sl(TraceArgs A) {
     __shared__ int32_t lds_stack[S * TB];
     const uint32_t gtid = blockIdx.x * TB + threadIdx.x;
     LaneStack<S> st{lds_stack + threadIdx.x, A.spill + gtid, A.spill_stride, 0, A.ctr + 1};
@@ -618,11 +619,11 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(5, 8))) void
 // random streams are those of k_trace, so the results are bit-identical.
 enum : int { L_IDLE = 0, L_NEED = 1, L_PRIM = 2, L_AO = 3, L_PRIM_DONE = 4, L_AO_DONE = 5, L_OFF = 6 };
 
-// Held to 96 VGPRs (5 wavefronts per SIMD instead of 4; one spilled register): the 1-spp frame's
-// trace is bound by the latency of its dependent node fetches, and the fifth wavefront took it
-// from 1.13 to 1.07 ms.
+// Held to 72 VGPRs (7 wavefronts per SIMD instead of 4, a few registers spilled): the 1-spp
+// frame's trace is bound by the latency of its dependent node fetches, and the extra wavefronts
+// took it from 1.13 to 1.04 ms (5 wavefronts: 1.07, 6: 1.04, 8: 1.10).
 template <int S, int MODE, bool COUNT>
-__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_trace_refill(TraceArgs A,
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_trace_refill(TraceArgs A,
                                                                                                 uint32_t refill) {
     __shared__ int32_t lds_stack[S * TB];
     const uint32_t gtid = blockIdx.x * TB + threadIdx.x;
