@@ -64,9 +64,9 @@ def trace_kernel_label(r, spp, mode):
 # AO frame with the feature buffer.
 ABLATION = {
     "64spp": [
-        ("sample lanes, work stealing, LDS stack 16, 8x8 tiles", {"stack_lds": 16, "refill": 0}),
+        ("sample lanes, work stealing, LDS stack 16 (library default), 8x8 tiles", {"stack_lds": 16, "refill": 0}),
         ("LDS stack 8 entries", {"stack_lds": 8, "refill": 0}),
-        ("LDS stack 24 entries (library default)", {"stack_lds": 24, "refill": 0}),
+        ("LDS stack 24 entries", {"stack_lds": 24, "refill": 0}),
         ("LDS stack 32 entries", {"stack_lds": 32, "refill": 0}),
         ("4x16 tiles", {"stack_lds": 16, "refill": 0, "tile_w": 4}),
         ("lane per pixel (samples in a loop)", {"stack_lds": 16, "refill": 0, "sample_lanes": 0}),

@@ -277,7 +277,7 @@ MMHRT_API int32_t mmhrt_denoiser_train_step(mmhrt_ctx ctx, const uint16_t* feats
                                             uint32_t h, uint32_t w, float depth_scale, const float bg[4], float lr,
                                             int32_t apply, float* loss_out, float* grads_out);
 
-/* Tuning knobs (autotuner output): "stack_lds" (LDS stack entries per lane: 8, 16, 24 or 32, default 24),
+/* Tuning knobs (autotuner output): "stack_lds" (LDS stack entries per lane: 8, 16, 24 or 32, default 16),
  * "tile_w", "tile_h", "persistent" (0/1), "refill" (ballot refill threshold in lanes; -1, the
  * default: 16 for 1-spp frames, 0 otherwise), "blocks_per_cu" (persistent
  * workgroups per CU), "refit_period" (MMHRT_BUILD_AUTO: refits between rebuilds), "build_chunk"

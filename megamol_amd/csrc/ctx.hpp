@@ -109,7 +109,7 @@ struct mmhrt_ctx_s {
     mmhrt::DevBuf<uint32_t> counters;  // [0] tile counter, [2..7] visit counters (u64 pairs)
 
     // options
-    int64_t opt_stack_lds = 24;
+    int64_t opt_stack_lds = 16;  // 16 leaves the LDS for 7 wavefronts per SIMD
     int64_t opt_persistent = 1;
     int64_t opt_refill = -1;  // ballot-refill threshold; -1: 16 for 1-spp frames (measured 13% faster), else off
     int64_t opt_count = 0;

@@ -15,7 +15,7 @@ VARIANTS = [dict(stack_lds=16, persistent=1, refill=0), dict(stack_lds=8, persis
             dict(stack_lds=24, persistent=1, refill=64)]
 
 
-DEFAULTS = dict(stack_lds=24, persistent=1, refill=-1)  # the library's (refill -1: 16 at 1 spp, else 0)
+DEFAULTS = dict(stack_lds=16, persistent=1, refill=-1)  # the library's (refill -1: 16 at 1 spp, else 0)
 
 
 def set_opts(r, opts):
