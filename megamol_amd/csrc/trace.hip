@@ -420,8 +420,10 @@ __device__ void shade_pixel(const TraceArgs& A, uint32_t px, uint32_t py, float 
     write_pixel(A, opix, col, depth, best_t, best_leaf, aux, vis_sum);
 }
 
+// Lane per pixel, samples in a loop (every sample count). Held to 72 VGPRs (7 wavefronts per SIMD
+// instead of 4): 64-spp AO 64.3 -> 57.7 ms, 1-spp AO 1.22 -> 1.14 ms.
 template <int S, int MODE, bool COUNT, bool STEAL>
-__global__ __launch_bounds__(TB) void k_trace(TraceArgs A) {
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_trace(TraceArgs A) {
     __shared__ int32_t lds_stack[S * TB];
     const uint32_t gtid = blockIdx.x * TB + threadIdx.x;
     LaneStack<S> st{lds_stack + threadIdx.x, A.spill + gtid, A.spill_stride, 0, A.ctr + 1};
