@@ -861,8 +861,10 @@ void launch_refill(const TraceArgs& A, unsigned grid, uint32_t refill, bool coun
 // Batched trace: all jobs (cameras) of one timestep share one persistent launch. The global tile
 // index runs over the concatenated tiles of the jobs (prefix[j] = first tile of job j); each
 // wavefront steals tiles and shades them with its job's constants and output buffers.
+// Held to 72 VGPRs (7 wavefronts per SIMD instead of 4) like the single-frame kernels: the batched
+// series (python bench.py --config series) went from 149.0 to 135.7 ms per launch.
 template <int S, int MODE>
-__global__ __launch_bounds__(TB) void k_trace_batch(const TraceArgs* __restrict__ jobs,
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_trace_batch(const TraceArgs* __restrict__ jobs,
                                                     const uint32_t* __restrict__ prefix, uint32_t n_jobs,
                                                     uint32_t* ctr) {
     __shared__ int32_t lds_stack[S * TB];
