@@ -284,7 +284,7 @@ MMHRT_API int32_t mmhrt_denoiser_train_step(mmhrt_ctx ctx, const uint16_t* feats
  * (leaves per LBVH chunk workgroup: 1024, 512 or 256), "build_ls" (level-synchronous chunk
  * pass, default 1; 0: one climbing thread per leaf), "build_grid" (test hook: at most this many
  * persistent chunk workgroups, 0 = as many as fit), "sort_ri" (radix sort keys per thread per
- * tile: 8, 16 or 32), "sort_onesweep" (single-pass radix passes with look-back, default 0),
+ * tile: 8, 16 or 32, default 16), "sort_onesweep" (single-pass radix passes with look-back, default 0),
  * "leaf_pairs" (default 1: nodes over two leaves are not stored, their parents reference the
  * pair; 0: every node is stored), "build_tail" (straddler worklist size from which one
  * workgroup finishes the tree, default 512; used when "build_span" is 0), "build_span" (straddlers

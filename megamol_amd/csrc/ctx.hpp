@@ -121,7 +121,7 @@ struct mmhrt_ctx_s {
     int64_t opt_build_chunk = 1024; // leaves per LBVH chunk workgroup: 1024, 512 or 256
     int64_t opt_sample_lanes = 1;   // sample-parallel traversal for power-of-two spp (Lambert/AO)
     int64_t opt_build_ls = 1;       // level-synchronous LBVH chunk pass (0: one climbing thread per leaf)
-    int64_t opt_sort_ri = 32;       // radix sort keys per thread per tile: 8, 16 or 32 (32 for 32-bit keys only)
+    int64_t opt_sort_ri = 16;       // radix sort keys per thread per tile: 8, 16 or 32 (32 for 32-bit keys only)
     int64_t opt_sort_onesweep = 0;  // single-pass radix passes with decoupled look-back (0: histogram + scans + scatter)
     int64_t opt_build_grid = 0;     // test hook: cap on the persistent chunk workgroups (0 = as many as fit)
     int64_t opt_leaf_pairs = 1;     // nodes over two leaves are not stored (their parents reference the pair)

@@ -111,7 +111,7 @@ def test_lbvh_equals_oracle_lbvh(gpu_renderer, chunk, ls, ri, grid, os1, wave, t
     gpu_renderer.set_option("sort_onesweep", 0)
     gpu_renderer.set_option("build_chunk", 1024)
     gpu_renderer.set_option("build_ls", 1)
-    gpu_renderer.set_option("sort_ri", 32)
+    gpu_renderer.set_option("sort_ri", 16)
     gpu_renderer.set_option("build_grid", 0)
     gpu_renderer.set_option("build_wave", 64)
     gpu_renderer.set_option("build_tail", 512)
