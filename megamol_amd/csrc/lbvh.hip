@@ -380,20 +380,23 @@ __global__ __launch_bounds__(256) void k_col_sums(const uint32_t* __restrict__ h
     gsum[(size_t)blockIdx.x * 256 + threadIdx.x] = acc;
 }
 
-// One block: exclusive prefix of each digit over the groups (16 loads in flight per thread),
-// and the exclusive scan of the digit totals into row `groups`.
+// One block: exclusive prefix of each digit over the groups (32 loads in flight per thread: the
+// block walks the groups one latency at a time), and the exclusive scan of the digit totals into
+// row `groups`.
 __global__ __launch_bounds__(256) void k_col_top(uint32_t* __restrict__ gsum, uint32_t groups) {
     __shared__ uint32_t wsum[4];
     const uint32_t d = threadIdx.x, lane = __lane_id(), wave = threadIdx.x >> 6;
     uint32_t run = 0;
-    for (uint32_t g0 = 0; g0 < groups; g0 += 16) {
-        uint32_t v[16];
+    for (uint32_t g0 = 0; g0 < groups; g0 += 32) {
+        uint32_t v[32];
 #pragma unroll
-        for (int k = 0; k < 16; ++k) v[k] = g0 + k < groups ? gsum[(size_t)(g0 + k) * 256 + d] : 0u;
+        for (int k = 0; k < 32; ++k) v[k] = gsum[(size_t)(g0 + k < groups ? g0 + k : groups - 1) * 256 + d];
 #pragma unroll
-        for (int k = 0; k < 16; ++k) {
-            if (g0 + k < groups) gsum[(size_t)(g0 + k) * 256 + d] = run;
-            run += v[k];
+        for (int k = 0; k < 32; ++k) {
+            if (g0 + k < groups) {
+                gsum[(size_t)(g0 + k) * 256 + d] = run;
+                run += v[k];
+            }
         }
     }
     uint32_t inc = run;
