@@ -1525,7 +1525,10 @@ int hierarchy(mmhrt_ctx_s* c, const K* keys, const uint32_t* ids) {
     if (c->opt_build_span > 0) {
         // spans of build_span chunks, one workgroup each, then the tail over their exports (at
         // most three per span: its right boundary from the chunk pass, its two boundary splits)
-        const int64_t span = c->opt_build_span * (int64_t)bc;
+        // at most 256 spans: every span's workgroup scans the chunk pass's whole list, so larger
+        // scenes take longer spans rather than more of them
+        const int64_t nchunks = ((int64_t)n + bc - 1) / bc;
+        const int64_t span = std::max<int64_t>(c->opt_build_span, (nchunks + 255) / 256) * (int64_t)bc;
         const uint64_t spans = (n + span - 1) / span;
         const size_t xcap = 4 * spans + 64;
         if (int e = ensure(c, c->xlist, 2 * xcap, "span export lists")) return e;
