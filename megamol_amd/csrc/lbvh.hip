@@ -10,7 +10,9 @@
 //   4. hierarchy and boxes together, bottom-up, with the gather of the spheres into Morton order
 //      (leaf k) on the way: the second arrival at a parent merges and writes the whole node
 //      (Karras 2012 numbering and index tie-break for equal keys, Apetrei 2014 parent rule; LDS
-//      meeting places inside 1024-leaf chunks)
+//      meeting places inside 1024-leaf chunks; a node over two leaves is not stored, its parent
+//      references the leaf pair)
+//   5. the nodes that straddle chunks: one workgroup per span of chunks, then one for the top
 #include <algorithm>
 
 #include "ctx.hpp"
