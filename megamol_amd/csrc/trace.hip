@@ -281,8 +281,8 @@ __device__ __forceinline__ V3 random_in_unit_sphere(uint32_t& seed) {
 
 // Pixel epilogue of raygen.cu:250-272: average over samples, progressive blend, pseudo depth
 // (background on a pixel that no sample hit), then every output buffer.
-__device__ __forceinline__ void write_pixel(const TraceArgs& A, uint32_t opix, float4 col, float depth, float best_t,
-                                            int best_leaf, float aux[8], float vis_sum) {
+__device__ __forceinline__ void write_pixel_main(const TraceArgs& A, uint32_t opix, float4 col, float depth,
+                                                 float best_t, int best_leaf) {
     const FrameConsts& f = A.fc;
     float fs = (float)f.spp;
     col = make_float4(col.x / fs, col.y / fs, col.z / fs, col.w / fs);
@@ -305,15 +305,37 @@ __device__ __forceinline__ void write_pixel(const TraceArgs& A, uint32_t opix, f
     A.out_t[opix] = best_t;
     A.out_id[opix] = best_leaf >= 0 ? (int32_t)A.leaf_id[best_leaf] : -1;
     A.out_rgba8[opix] = pack_rgba8(col);
-    if (f.want_aux) {
-        aux[7] = vis_sum / fs;
-        uint32_t w[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-            w[k] = (uint32_t)__half_as_ushort(__float2half(aux[2 * k])) |
-                   ((uint32_t)__half_as_ushort(__float2half(aux[2 * k + 1])) << 16);
-        *reinterpret_cast<uint4*>(A.out_aux + (size_t)opix * 8) = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+__device__ __forceinline__ uint32_t half2_bits(float lo, float hi) {
+    return (uint32_t)__half_as_ushort(__float2half(lo)) | ((uint32_t)__half_as_ushort(__float2half(hi)) << 16);
+}
+
+// Features of a pixel (8 halves): albedo, normal and t of sample 0's hit (zeros on a miss), then
+// the mean AO visibility over the samples.
+__device__ __forceinline__ void write_pixel(const TraceArgs& A, uint32_t opix, float4 col, float depth, float best_t,
+                                            int best_leaf, float aux[8], float vis_sum) {
+    write_pixel_main(A, opix, col, depth, best_t, best_leaf);
+    if (A.fc.want_aux) {
+        aux[7] = vis_sum / (float)A.fc.spp;
+        *reinterpret_cast<uint4*>(A.out_aux + (size_t)opix * 8) = make_uint4(
+            half2_bits(aux[0], aux[1]), half2_bits(aux[2], aux[3]), half2_bits(aux[4], aux[5]), half2_bits(aux[6], aux[7]));
     }
+}
+
+// The same features written in two parts, so that a kernel need not keep them live over the
+// traversal of the remaining samples: halves 0-5 once sample 0's primary ray is resolved, halves
+// 6-7 with the pixel.
+__device__ __forceinline__ void write_aux_head(const TraceArgs& A, uint32_t opix, float a0, float a1, float a2,
+                                               float n0, float n1, float n2) {
+    if (A.fc.want_aux) {
+        uint3* q = reinterpret_cast<uint3*>(A.out_aux + (size_t)opix * 8);
+        *q = make_uint3(half2_bits(a0, a1), half2_bits(a2, n0), half2_bits(n1, n2));
+    }
+}
+__device__ __forceinline__ void write_aux_tail(const TraceArgs& A, uint32_t opix, float t0, float vis_sum) {
+    if (A.fc.want_aux)
+        *reinterpret_cast<uint32_t*>(A.out_aux + (size_t)opix * 8 + 6) = half2_bits(t0, vis_sum / (float)A.fc.spp);
 }
 
 template <int S, int MODE, bool COUNT>
@@ -645,7 +667,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(7, 8))) void
     float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
     float depth = FLT_MAX, best_t = INFINITY, vis_sum = 0.0f;
     int best_leaf_px = -1;
-    float aux[8];
+    float aux6 = 0.f;  // t of sample 0's hit (the other features are written when it is found)
     // ray state
     RayPre r;
     float tmin = 0.f, tmax = 0.f, t_best = 0.f;
@@ -681,8 +703,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(7, 8))) void
                     best_t = INFINITY;
                     best_leaf_px = -1;
                     vis_sum = 0.0f;
-#pragma unroll
-                    for (int k2 = 0; k2 < 8; ++k2) aux[k2] = 0.f;
+                    aux6 = 0.f;
                     state = L_NEED;
                 } else if (w >= total_work) {
                     state = L_OFF;
@@ -730,6 +751,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(7, 8))) void
                 float4 sc = MODE == 1 ? make_float4(f.bg[0], f.bg[1], f.bg[2], f.bg[3])
                                       : make_float4(1.0f * 0.8f, 1.0f * 0.8f, 1.0f * 0.8f, 1.0f);
                 col.x += sc.x; col.y += sc.y; col.z += sc.z; col.w += sc.w;
+                if (s == 0) write_aux_head(A, opix, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f);
                 sample_done = true;
             } else {
                 float t = t_best;
@@ -746,9 +768,8 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(7, 8))) void
                     best_leaf_px = best;
                 }
                 if (s == 0) {
-                    aux[0] = alb.x; aux[1] = alb.y; aux[2] = alb.z;
-                    aux[3] = N.x; aux[4] = N.y; aux[5] = N.z;
-                    aux[6] = t;
+                    write_aux_head(A, opix, alb.x, alb.y, alb.z, N.x, N.y, N.z);
+                    aux6 = t;
                 }
                 if (MODE == 1) {
                     uint32_t aseed = tea16(pix, 0xA0000000u + f.frame_idx * 4096u + s);
@@ -781,7 +802,8 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(7, 8))) void
             if (++s < f.spp) {
                 state = L_NEED;
             } else {
-                write_pixel(A, opix, col, depth, best_t, best_leaf_px, aux, vis_sum);
+                write_pixel_main(A, opix, col, depth, best_t, best_leaf_px);
+                write_aux_tail(A, opix, aux6, vis_sum);
                 state = L_IDLE;
             }
         }
