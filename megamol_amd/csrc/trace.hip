@@ -568,7 +568,8 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(7, 8))) void
             const uint32_t pix = py * f.width + px;
             const uint32_t seed0 = tea16(pix, f.frame_idx);
             float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
-            float depth = FLT_MAX, best_t = INFINITY, vis_sum = 0.0f;
+            // best_t is the least hit distance over the samples, the pixel's depth (FLT_MAX without a hit)
+            float best_t = INFINITY, vis_sum = 0.0f;
             int best_leaf = -1;
             float aux[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
             for (uint32_t rd = 0; rd < rounds; ++rd) {
@@ -585,21 +586,22 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(7, 8))) void
                     shade_sample<S, MODE, COUNT, PAIRS>(A, px, py, pix, s, jx, jy, pad, st, cnt, sc, ray_depth, t,
                                                         leaf, V, alb, N);
                 }
+                // visibility is 0 or 1 per sample, so its sum is exact in any order: the count of
+                // the group's unoccluded hits
+                const uint64_t gmask = L == 64u ? ~0ull : ((1ull << L) - 1ull) << base;
+                vis_sum += (float)__popcll(__ballot(leaf >= 0 && V > 0.0f) & gmask);
                 // fold the group's samples into the pixel, in sample order
                 for (uint32_t i = 0; i < L; ++i) {
                     const int src = (int)(base + i);
                     const float cx = __shfl(sc.x, src, 64), cy = __shfl(sc.y, src, 64);
                     const float cz = __shfl(sc.z, src, 64), cw = __shfl(sc.w, src, 64);
-                    const float ti = __shfl(t, src, 64), di = __shfl(ray_depth, src, 64);
-                    const float vi = __shfl(V, src, 64);
+                    const float ti = __shfl(t, src, 64);
                     const int li = __shfl(leaf, src, 64);
                     col.x += cx;
                     col.y += cy;
                     col.z += cz;
                     col.w += cw;
-                    depth = fminf(depth, di);
                     if (li >= 0) {
-                        vis_sum += vi;
                         if (ti < best_t) {
                             best_t = ti;
                             best_leaf = li;
@@ -621,7 +623,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(7, 8))) void
             }
             if (valid && sl == 0) {
                 const uint32_t opix = (py - f.win_y0) * f.win_w + (px - f.win_x0);
-                write_pixel(A, opix, col, depth, best_t, best_leaf, aux, vis_sum);
+                write_pixel(A, opix, col, best_t < INFINITY ? best_t : FLT_MAX, best_t, best_leaf, aux, vis_sum);
             }
         }
     }
@@ -665,7 +667,8 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(7, 8))) void
     // pixel state
     uint32_t px = 0, py = 0, pix = 0, opix = 0, s = 0, seed = 0;
     float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
-    float depth = FLT_MAX, best_t = INFINITY, vis_sum = 0.0f;
+    // best_t is the least hit distance over the samples, the pixel's depth (FLT_MAX without a hit)
+    float best_t = INFINITY, vis_sum = 0.0f;
     int best_leaf_px = -1;
     float aux6 = 0.f;  // t of sample 0's hit (the other features are written when it is found)
     // ray state
@@ -699,7 +702,6 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(7, 8))) void
                     s = 0;
                     seed = tea16(pix, f.frame_idx);
                     col = make_float4(0.f, 0.f, 0.f, 0.f);
-                    depth = FLT_MAX;
                     best_t = INFINITY;
                     best_leaf_px = -1;
                     vis_sum = 0.0f;
@@ -762,7 +764,6 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(7, 8))) void
                 if (dot(N, d) > 0.0f) N = neg(N);
                 N = normalize(N);
                 alb = A.col[A.leaf_id[best]];
-                depth = fminf(depth, t);
                 if (t < best_t) {
                     best_t = t;
                     best_leaf_px = best;
@@ -802,7 +803,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(7, 8))) void
             if (++s < f.spp) {
                 state = L_NEED;
             } else {
-                write_pixel_main(A, opix, col, depth, best_t, best_leaf_px);
+                write_pixel_main(A, opix, col, best_t < INFINITY ? best_t : FLT_MAX, best_t, best_leaf_px);
                 write_aux_tail(A, opix, aux6, vis_sum);
                 state = L_IDLE;
             }
