@@ -590,23 +590,34 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(7, 8))) void
                 // the group's unoccluded hits
                 const uint64_t gmask = L == 64u ? ~0ull : ((1ull << L) - 1ull) << base;
                 vis_sum += (float)__popcll(__ballot(leaf >= 0 && V > 0.0f) & gmask);
-                // fold the group's samples into the pixel, in sample order
+                // the group's closest hit, the first sample of least t (an exchange over lane ^ m
+                // stays inside the aligned group of L lanes); an earlier round wins ties
+                float rt = leaf >= 0 ? t : INFINITY;
+                int rl = leaf;
+                uint32_t rs = s;
+                for (uint32_t m = L >> 1; m > 0; m >>= 1) {
+                    const float ot = __shfl_xor(rt, (int)m, 64);
+                    const int ol = __shfl_xor(rl, (int)m, 64);
+                    const uint32_t os = __shfl_xor(rs, (int)m, 64);
+                    if (ot < rt || (ot == rt && os < rs)) {
+                        rt = ot;
+                        rl = ol;
+                        rs = os;
+                    }
+                }
+                if (rl >= 0 && rt < best_t) {
+                    best_t = rt;
+                    best_leaf = rl;
+                }
+                // colours are summed in sample order
                 for (uint32_t i = 0; i < L; ++i) {
                     const int src = (int)(base + i);
                     const float cx = __shfl(sc.x, src, 64), cy = __shfl(sc.y, src, 64);
                     const float cz = __shfl(sc.z, src, 64), cw = __shfl(sc.w, src, 64);
-                    const float ti = __shfl(t, src, 64);
-                    const int li = __shfl(leaf, src, 64);
                     col.x += cx;
                     col.y += cy;
                     col.z += cz;
                     col.w += cw;
-                    if (li >= 0) {
-                        if (ti < best_t) {
-                            best_t = ti;
-                            best_leaf = li;
-                        }
-                    }
                 }
                 if (rd == 0) {  // features of sample 0
                     const int src = (int)base;
