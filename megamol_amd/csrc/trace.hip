@@ -542,6 +542,7 @@ __device__ __forceinline__ void shade_sample(const TraceArgs& A, uint32_t px, ui
 template <int S, int MODE, bool COUNT, bool PAIRS>
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_trace_sl(TraceArgs A) {
     __shared__ int32_t lds_stack[S * TB];
+    __shared__ float4 lds_col[TB];  // the wavefront's sample colours of one round
     const uint32_t gtid = blockIdx.x * TB + threadIdx.x;
     LaneStack<S> st{lds_stack + threadIdx.x, A.spill + gtid, A.spill_stride, 0, A.ctr + 1};
     Count cnt;
@@ -609,16 +610,23 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(7, 8))) void
                     best_t = rt;
                     best_leaf = rl;
                 }
-                // colours are summed in sample order
-                for (uint32_t i = 0; i < L; ++i) {
-                    const int src = (int)(base + i);
-                    const float cx = __shfl(sc.x, src, 64), cy = __shfl(sc.y, src, 64);
-                    const float cz = __shfl(sc.z, src, 64), cw = __shfl(sc.w, src, 64);
-                    col.x += cx;
-                    col.y += cy;
-                    col.z += cz;
-                    col.w += cw;
+                // colours are summed in sample order by the group's first lane (the one that
+                // writes the pixel), from the wavefront's slice of LDS
+                float4* wcol = lds_col + (threadIdx.x - lane);
+                wcol[lane] = sc;
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                if (sl == 0) {
+                    for (uint32_t i = 0; i < L; ++i) {
+                        const float4 c = wcol[base + i];
+                        col.x += c.x;
+                        col.y += c.y;
+                        col.z += c.z;
+                        col.w += c.w;
+                    }
                 }
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
                 if (rd == 0) {  // features of sample 0
                     const int src = (int)base;
                     const int l0 = __shfl(leaf, src, 64);
