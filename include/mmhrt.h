@@ -95,7 +95,10 @@ typedef struct mmhrt_camera {
     float aspect;     /* width / height */
     float near_plane; /* projection near plane (depth conversion only) */
     float far_plane;  /* projection far plane (depth conversion only) */
-    float tile_start[2]; /* image-plane tile (Camera.h:94-99), default {0,0} */
+    float tile_start[2]; /* image-plane tile (Camera.h:94-99), default {0,0}; carried for the
+                            interface but not read: the reference's frame state has no tile and
+                            its raygen spans the full image plane (Renderer.cpp:70-93, raygen.cu:241-242).
+                            Use the render window to render part of a frame. */
     float tile_end[2];   /* default {1,1} */
 } mmhrt_camera;
 
@@ -118,8 +121,13 @@ typedef struct mmhrt_render_params {
     int32_t mode;          /* MMHRT_MODE_* */
     float ao_distance;     /* AO ray length (tmax) */
     int32_t max_bounces;   /* MMHRT_MODE_BOUNCE only */
-    uint32_t frame_idx;    /* RNG frame index; > 0 with accumulate blends with the previous frame */
-    int32_t accumulate;    /* progressive accumulation (raygen.cu:254-260) */
+    uint32_t frame_idx;    /* RNG frame index (the TEA seed of raygen.cu:182) */
+    int32_t accumulate;    /* 1: with frame_idx > 0 the frame is blended into the previous one,
+                              old + (new - old) / (frame_idx + 1), as raygen.cu:254-260 always does
+                              (the reference's raygen never reads its own accumulate flag,
+                              Renderer.cpp:162); the previous frame must be the same window of a
+                              frame of the same size (else MMHRT_E_STATE). 0: frame_idx only seeds
+                              the random streams and nothing is blended. */
     float background[4];   /* RGBA background (CallRender3D::BackgroundColor) */
     uint32_t flags;        /* MMHRT_FLAG_* */
     uint32_t window[4];    /* x0, y0, w, h: render only this pixel window of the width x height frame

@@ -38,6 +38,8 @@ float elapsed(mmhrt_ctx_s* c, int a, int b) {
     return ms;
 }
 
+}  // namespace
+
 // Camera -> per-frame constants: Renderer.cpp:136-172 (near plane of the ray model fixed at 100)
 // and the glm::frustum entries (2,2) and (3,2) that Renderer.cpp:141-146 extracts.
 void frame_consts(const mmhrt_camera* cam, const mmhrt_render_params* p, FrameConsts& f) {
@@ -82,13 +84,28 @@ void frame_consts(const mmhrt_camera* cam, const mmhrt_render_params* p, FrameCo
     }
 }
 
+// Argument and state checks shared by the render entry points.
+int check_render_args(mmhrt_ctx_s* c, const mmhrt_camera* cam, const mmhrt_render_params* p, const char* who) {
+    const std::string w(who);
+    if (!cam || !p) return fail(c, MMHRT_E_INVALID, w + ": camera or params is NULL");
+    if (p->width == 0 || p->height == 0) return fail(c, MMHRT_E_INVALID, w + ": viewport is 0x0");
+    if ((uint64_t)p->width * p->height > (1ull << 31)) return fail(c, MMHRT_E_UNSUPPORTED, w + ": viewport too large");
+    if (p->spp == 0) return fail(c, MMHRT_E_INVALID, w + ": spp must be >= 1");
+    if (p->mode < 0 || p->mode > 2) return fail(c, MMHRT_E_INVALID, w + ": unknown mode");
+    if (!c->built || c->built_n != c->n) return fail(c, MMHRT_E_STATE, w + ": build the LBVH first");
+    if (!(cam->fovy > 0.0f) || !(cam->aspect > 0.0f)) return fail(c, MMHRT_E_INVALID, w + ": bad fovy or aspect");
+    if (p->window[2] > 0 && ((uint64_t)p->window[0] + p->window[2] > p->width ||
+                             (uint64_t)p->window[1] + p->window[3] > p->height || p->window[3] == 0))
+        return fail(c, MMHRT_E_INVALID, w + ": window outside the frame");
+    return MMHRT_OK;
+}
+
 int check_ctx(mmhrt_ctx c) {
     if (!c) return MMHRT_E_INVALID;
     hipError_t e = hipSetDevice(c->device);
     if (e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
     return MMHRT_OK;
 }
-}  // namespace
 }  // namespace mmhrt
 
 using namespace mmhrt;
@@ -151,6 +168,9 @@ int32_t mmhrt_destroy(mmhrt_ctx c) {
     release(c->partial); release(c->info); release(c->tf_buf);
     release(c->fb_col); release(c->fb_depth); release(c->fb_t); release(c->fb_id); release(c->fb_rgba8);
     release(c->fb_aux); release(c->spill); release(c->counters);
+    release(c->ts_sph); release(c->ts_col); release(c->bt_col); release(c->bt_depth); release(c->bt_t);
+    release(c->bt_id); release(c->bt_rgba8); release(c->batch_ctr); release(c->batch_prefix);
+    release(c->batch_jobs);
     if (c->rounds_host) (void)hipHostFree(c->rounds_host);
     c->rounds_host = nullptr;
     if (c->dn) {
@@ -326,19 +346,15 @@ int32_t mmhrt_build(mmhrt_ctx c, int32_t policy, int32_t morton_bits) {
 
 int32_t mmhrt_render(mmhrt_ctx c, const mmhrt_camera* cam, const mmhrt_render_params* p, mmhrt_framebuffer* out) {
     if (int e = check_ctx(c)) return e;
-    if (!cam || !p) return fail(c, MMHRT_E_INVALID, "render: camera or params is NULL");
-    if (p->width == 0 || p->height == 0) return fail(c, MMHRT_E_INVALID, "render: viewport is 0x0");
-    if ((uint64_t)p->width * p->height > (1ull << 31)) return fail(c, MMHRT_E_UNSUPPORTED, "render: viewport too large");
-    if (p->spp == 0) return fail(c, MMHRT_E_INVALID, "render: spp must be >= 1");
-    if (p->mode < 0 || p->mode > 2) return fail(c, MMHRT_E_INVALID, "render: unknown mode");
-    if (!c->built || c->built_n != c->n) return fail(c, MMHRT_E_STATE, "render: build the LBVH first");
-    if (!(cam->fovy > 0.0f) || !(cam->aspect > 0.0f)) return fail(c, MMHRT_E_INVALID, "render: bad fovy or aspect");
-    if (p->window[2] > 0 && ((uint64_t)p->window[0] + p->window[2] > p->width ||
-                             (uint64_t)p->window[1] + p->window[3] > p->height || p->window[3] == 0))
-        return fail(c, MMHRT_E_INVALID, "render: window outside the frame");
-    uint32_t ow = p->window[2] > 0 ? p->window[2] : p->width, oh = p->window[2] > 0 ? p->window[3] : p->height;
-    if (p->accumulate && p->frame_idx > 0 && (c->fb_w != ow || c->fb_h != oh))
-        return fail(c, MMHRT_E_STATE, "render: accumulation needs the previous frame at the same size");
+    if (int e = check_render_args(c, cam, p, "render")) return e;
+    const bool win = p->window[2] > 0;
+    uint32_t ow = win ? p->window[2] : p->width, oh = win ? p->window[3] : p->height;
+    uint32_t ox = win ? p->window[0] : 0, oy = win ? p->window[1] : 0;
+    // blending reads the stored frame: it must be the same window of a frame of the same size
+    if (p->accumulate && p->frame_idx > 0 &&
+        (c->fb_w != ow || c->fb_h != oh || c->fb_x0 != ox || c->fb_y0 != oy || c->fb_full_w != p->width ||
+         c->fb_full_h != p->height))
+        return fail(c, MMHRT_E_STATE, "render: accumulation needs the previous frame of the same window and size");
     FrameConsts f;
     frame_consts(cam, p, f);
     if (int e = render_frame(c, f)) return e;

@@ -68,31 +68,24 @@ int32_t mmhrt_set_timesteps(mmhrt_ctx c, const float* xyzr, uint32_t timesteps, 
     size_t total = (size_t)timesteps * count;
     int e = 0;
     if ((e = ensure(c, c->ts_sph, total, "timestep spheres"))) return e;
-    if ((e = ensure(c, c->sph, count, "spheres"))) return e;
-    if ((e = ensure(c, c->col, count, "colours"))) return e;
+    if ((e = ensure(c, c->ts_col, count, "timestep colours"))) return e;
     hipError_t he = hipMemcpyAsync(c->ts_sph.p, xyzr, total * 16, hipMemcpyHostToDevice, c->stream);
     if (he != hipSuccess) return hip_fail(c, he, "set_timesteps upload");
-    // timestep 0 also becomes the context's current spheres; colours are shared by all timesteps
-    he = hipMemcpyAsync(c->sph.p, c->ts_sph.p, count * 16, hipMemcpyDeviceToDevice, c->stream);
-    if (he != hipSuccess) return hip_fail(c, he, "set_timesteps copy");
+    // the timesteps keep their own colour buffer, so that mmhrt_set_spheres (the context's
+    // spheres and colours) and the series never overwrite each other
     float4 g = make_float4(1.f, 1.f, 1.f, 1.f);
     if (global_rgba)
         g = make_float4(global_rgba[0] / 255.0f, global_rgba[1] / 255.0f, global_rgba[2] / 255.0f, global_rgba[3] / 255.0f);
     if (rgba) {
-        he = hipMemcpyAsync(c->col.p, rgba, count * 16, hipMemcpyHostToDevice, c->stream);
+        he = hipMemcpyAsync(c->ts_col.p, rgba, count * 16, hipMemcpyHostToDevice, c->stream);
         if (he != hipSuccess) return hip_fail(c, he, "set_timesteps colours");
     } else {
-        std::vector<float4> h(count, g);
-        he = hipMemcpyAsync(c->col.p, h.data(), count * 16, hipMemcpyHostToDevice, c->stream);
-        if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
-        if (he != hipSuccess) return hip_fail(c, he, "set_timesteps colours");
+        launch_fill_f4(c->stream, c->ts_col.p, g, count);
     }
     he = hipStreamSynchronize(c->stream);
     if (he != hipSuccess) return hip_fail(c, he, "set_timesteps");
     c->ts_count = timesteps;
     c->ts_n = count;
-    c->n = count;
-    c->info_valid = false;
     return MMHRT_OK;
 }
 
@@ -142,19 +135,26 @@ int32_t mmhrt_render_batch(mmhrt_ctx c, const mmhrt_job* jobs, uint32_t n_jobs, 
     if ((e = ensure(c, c->batch_prefix, (size_t)n_jobs + c->ts_count, "batch tile prefix"))) return e;
     const size_t js = batch_job_size();
     if ((e = ensure(c, c->batch_jobs, js * n_jobs, "batch jobs"))) return e;
+    // The builds read the timestep slices and their count; the context's own spheres, count and
+    // LBVH state come back when this call returns (the tree in the buffers is then the last
+    // timestep's, so the context needs a build before its next render).
+    struct Restore {
+        mmhrt_ctx_s* c;
+        uint64_t n;
+        ~Restore() {
+            c->sph_cur = nullptr;
+            c->n = n;
+            c->built = false;
+            c->info_valid = false;
+        }
+    } restore{c, c->n};
     // every buffer the per-timestep build and trace touch is allocated before any capture
     c->sph_cur = c->ts_sph.p;
     c->n = c->ts_n;
-    if ((e = build_lbvh(c, morton_bits))) {
-        c->sph_cur = nullptr;
-        return e;
-    }
+    if ((e = build_lbvh(c, morton_bits))) return e;
     {
         unsigned grid = (unsigned)std::max<int64_t>(1, c->cu_count * c->opt_blocks_per_cu);
-        if ((e = ensure(c, c->spill, (size_t)grid * 256 * 120, "traversal spill"))) {
-            c->sph_cur = nullptr;
-            return e;
-        }
+        if ((e = ensure(c, c->spill, (size_t)grid * 256 * 120, "traversal spill"))) return e;
     }
     // job table and tile prefixes (jobs of timestep t are contiguous)
     std::vector<uint8_t> hjobs(js * n_jobs);
@@ -181,10 +181,7 @@ int32_t mmhrt_render_batch(mmhrt_ctx c, const mmhrt_job* jobs, uint32_t n_jobs, 
     if (he == hipSuccess)
         he = hipMemcpyAsync(c->batch_prefix.p, hprefix.data(), hprefix.size() * 4, hipMemcpyHostToDevice, c->stream);
     if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
-    if (he != hipSuccess) {
-        c->sph_cur = nullptr;
-        return hip_fail(c, he, "render_batch upload");
-    }
+    if (he != hipSuccess) return hip_fail(c, he, "render_batch upload");
     auto issue = [&]() -> int {
         for (uint32_t t = 0; t < c->ts_count; ++t) {
             c->sph_cur = c->ts_sph.p + (size_t)t * c->ts_n;
@@ -241,9 +238,6 @@ int32_t mmhrt_render_batch(mmhrt_ctx c, const mmhrt_job* jobs, uint32_t n_jobs, 
     if (rc == 0 && ms_out) hipEventElapsedTime(ms_out, e0, e1);
     hipEventDestroy(e0);
     hipEventDestroy(e1);
-    c->sph_cur = nullptr;
-    // the context's spheres are left at timestep 0; the LBVH is that of the last timestep
-    c->built = false;
     if (rc) return rc;
     if (rgba8_out) he = hipMemcpy(rgba8_out, c->bt_rgba8.p, total_px * 4, hipMemcpyDeviceToHost);
     if (he == hipSuccess && depth_out) he = hipMemcpy(depth_out, c->bt_depth.p, total_px * 4, hipMemcpyDeviceToHost);

@@ -99,7 +99,9 @@ struct mmhrt_ctx_s {
     bool info_valid = false;
 
     // frame
-    uint32_t fb_w = 0, fb_h = 0;
+    uint32_t fb_w = 0, fb_h = 0;            // size of the last rendered window (the stored frame)
+    uint32_t fb_x0 = 0, fb_y0 = 0;          // its origin in the full frame
+    uint32_t fb_full_w = 0, fb_full_h = 0;  // the full frame it was cut from
     mmhrt::DevBuf<float4> fb_col;
     mmhrt::DevBuf<float> fb_depth, fb_t;
     mmhrt::DevBuf<int32_t> fb_id;
@@ -136,6 +138,7 @@ struct mmhrt_ctx_s {
 
     // batched time series: all timesteps resident, one hipGraph for the whole sequence
     mmhrt::DevBuf<float4> ts_sph;      // timesteps x n spheres
+    mmhrt::DevBuf<float4> ts_col;      // n colours shared by all timesteps
     uint32_t ts_count = 0;
     uint64_t ts_n = 0;
     const float4* sph_cur = nullptr;   // spheres the next build reads (a timestep slice), null = sph
@@ -189,6 +192,11 @@ void release(DevBuf<T>& b) {
     b.cap = 0;
 }
 
+// api.hip
+int check_ctx(mmhrt_ctx_s* c);
+int check_render_args(mmhrt_ctx_s* c, const mmhrt_camera* cam, const mmhrt_render_params* p, const char* who);
+void frame_consts(const mmhrt_camera* cam, const mmhrt_render_params* p, FrameConsts& f);
+
 // ingest.hip
 struct ListDesc {
     const uint8_t* vert;
@@ -203,6 +211,7 @@ int launch_convert_list(mmhrt_ctx_s* c, const ListDesc& d, const float* tf_dev, 
                         float tf_fac);
 int launch_copy_packed(mmhrt_ctx_s* c, const float4* xyzr, const float4* rgba, uint64_t n, float4 gcol);
 void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t v, uint64_t n);
+void launch_fill_f4(hipStream_t s, float4* p, float4 v, uint64_t n);
 
 // lbvh.hip
 int build_lbvh(mmhrt_ctx_s* c, int bits);
@@ -211,7 +220,20 @@ int reduce_scene(mmhrt_ctx_s* c);
 int count_leaf_pairs(mmhrt_ctx_s* c, int64_t* out);
 
 // trace.hip
+constexpr int TRACE_STACK_CAP = 128;                       // traversal stack entries per lane
+constexpr int TRACE_SPILL_PER_LANE = TRACE_STACK_CAP - 8;  // the most any LDS stack depth (>= 8) spills
+constexpr uint32_t TRACE_CTR_WORDS = 16;                   // tile counter, overflow flag, 5 u64 counters
+struct FrameOutputs {
+    float4* color;
+    float* depth;
+    float* t;
+    int32_t* hit_id;
+    uint32_t* rgba8;
+    __half* aux;
+};
 int render_frame(mmhrt_ctx_s* c, const FrameConsts& fc);
+int render_frame_into(mmhrt_ctx_s* c, const FrameConsts& fc, const FrameOutputs& out, uint32_t* ctr, int32_t* spill,
+                      uint32_t lanes, bool count);
 uint32_t tile_log2(int64_t tw);
 size_t batch_job_size();
 int batch_fill_job(mmhrt_ctx_s* c, const FrameConsts& fc, size_t out_offset, void* dst);

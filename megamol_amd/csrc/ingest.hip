@@ -10,6 +10,8 @@
 // Value semantics follow the Accessor classes (SimpleSphericalParticles.h:85-175): every field
 // is static_cast<float>; 8- and 16-bit colours are NOT normalised; the alpha of UINT8_RGB is
 // 255 and that of FLOAT_RGB is 1; lists without per-particle colour use global_rgba / 255.
+#include <algorithm>
+
 #include "ctx.hpp"
 
 namespace mmhrt {
@@ -107,7 +109,18 @@ __global__ __launch_bounds__(256) void k_copy_packed(const float4* __restrict__ 
     col[i] = rgba ? rgba[i] : gcol;
 }
 
+__global__ __launch_bounds__(256) void k_fill_f4(float4* __restrict__ p, float4 v, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
 }  // namespace
+
+void launch_fill_f4(hipStream_t s, float4* p, float4 v, uint64_t n) {
+    if (n == 0) return;
+    unsigned g = (unsigned)std::min<uint64_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_fill_f4, dim3(g), dim3(256), 0, s, p, v, n);
+}
 
 int launch_convert_list(mmhrt_ctx_s* c, const ListDesc& d, const float* tf_dev, uint32_t tf_n, float tf_vmin,
                         float tf_fac) {

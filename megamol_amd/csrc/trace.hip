@@ -24,8 +24,8 @@
 namespace mmhrt {
 namespace {
 
-constexpr int TB = 256;         // threads per block
-constexpr int STACK_CAP = 128;  // LDS + spill entries per lane (LBVH height <= 63 + 31)
+constexpr int TB = 256;                  // threads per block
+constexpr int STACK_CAP = TRACE_STACK_CAP;  // LDS + spill entries per lane (LBVH height <= 63 + 31)
 
 struct TraceArgs {
     const Node* nodes;
@@ -966,7 +966,7 @@ int batch_fill_job(mmhrt_ctx_s* c, const FrameConsts& fc, size_t out_offset, voi
     A.nodes = c->nodes.p;
     A.leaf_sph = c->leaf_sph.p;
     A.leaf_id = c->leaf_id.p;
-    A.col = c->col.p;
+    A.col = c->ts_col.p;  // the timesteps' own colours, not the context's
     A.info = c->info.p;
     A.spill = c->spill.p;
     A.spill_stride = (uint32_t)(std::max<int64_t>(1, c->cu_count * c->opt_blocks_per_cu) * TB);
@@ -995,51 +995,17 @@ uint32_t tile_log2(int64_t tw) {
     return l;
 }
 
-int render_frame(mmhrt_ctx_s* c, const FrameConsts& fc) {
-    uint32_t W = fc.win_w, H = fc.win_h;  // output size = the rendered window
-    size_t P = (size_t)W * H;
-    int e = 0;
-    if ((e = ensure(c, c->fb_col, P, "framebuffer colour"))) return e;
-    if ((e = ensure(c, c->fb_depth, P, "framebuffer depth"))) return e;
-    if ((e = ensure(c, c->fb_t, P, "framebuffer t"))) return e;
-    if ((e = ensure(c, c->fb_id, P, "framebuffer id"))) return e;
-    if ((e = ensure(c, c->fb_rgba8, P, "framebuffer rgba8"))) return e;
-    if (fc.want_aux && (e = ensure(c, c->fb_aux, P * 8, "framebuffer aux"))) return e;
-    if ((e = ensure(c, c->counters, 16, "counters"))) return e;
-    unsigned grid = (unsigned)(c->cu_count * c->opt_blocks_per_cu);
-    if (grid == 0) grid = 1;
-    uint32_t lanes = grid * TB;
+namespace {
+// Kernel choice for one frame (the same for both entry points): sample-parallel lanes for Lambert
+// and AO with spp a power of two up to 64 or a multiple of 64, the ballot-refill kernel when a
+// refill threshold is set, else the lane-per-pixel kernel.
+void dispatch_trace(mmhrt_ctx_s* c, const TraceArgs& A, unsigned grid, bool count) {
+    const FrameConsts& fc = A.fc;
     int S = (int)c->opt_stack_lds;
     if (S != 8 && S != 16 && S != 24 && S != 32) S = 16;
-    if ((e = ensure(c, c->spill, (size_t)lanes * (STACK_CAP - 8), "traversal spill"))) return e;
-    c->fb_w = W;
-    c->fb_h = H;
-    TraceArgs A;
-    A.nodes = c->nodes.p;
-    A.leaf_sph = c->leaf_sph.p;
-    A.leaf_id = c->leaf_id.p;
-    A.col = c->col.p;
-    A.info = c->info.p;
-    A.spill = c->spill.p;
-    A.spill_stride = lanes;
-    A.ctr = c->counters.p;
-    A.tw_log2 = tile_log2(c->opt_tile_w);
-    A.tiles_x = (W + (1u << A.tw_log2) - 1) >> A.tw_log2;
-    A.n_tiles = A.tiles_x * ((H + (64u >> A.tw_log2) - 1) / (64u >> A.tw_log2));
-    A.out_col = c->fb_col.p;
-    A.out_depth = c->fb_depth.p;
-    A.out_t = c->fb_t.p;
-    A.out_id = c->fb_id.p;
-    A.out_rgba8 = c->fb_rgba8.p;
-    A.out_aux = fc.want_aux ? c->fb_aux.p : nullptr;
-    A.fc = fc;
-    launch_fill_u32(c->stream, c->counters.p, 0u, 16);
-    rec(c, 6);
     bool steal = c->opt_persistent != 0;
-    bool count = c->opt_count != 0;
     const int64_t rq = c->opt_refill < 0 ? (fc.spp == 1 ? 16 : 0) : c->opt_refill;
     uint32_t refill = (uint32_t)std::max<int64_t>(0, std::min<int64_t>(64, rq));
-    // sample-parallel lanes: Lambert/AO with spp a power of two up to 64 or a multiple of 64
     const bool sl_shape = fc.spp > 1 && ((fc.spp >= 64 && fc.spp % 64 == 0) || (fc.spp < 64 && 64 % fc.spp == 0));
     if (c->opt_sample_lanes != 0 && refill == 0 && fc.mode != 2 && sl_shape) {
         switch (S) {
@@ -1063,6 +1029,69 @@ int render_frame(mmhrt_ctx_s* c, const FrameConsts& fc) {
         default: launch1<16>(A, grid, steal, count, fc.mode, c->stream); break;
         }
     }
+}
+
+void fill_scene_args(mmhrt_ctx_s* c, const FrameConsts& fc, TraceArgs& A) {
+    A.nodes = c->nodes.p;
+    A.leaf_sph = c->leaf_sph.p;
+    A.leaf_id = c->leaf_id.p;
+    A.col = c->col.p;
+    A.info = c->info.p;
+    A.tw_log2 = tile_log2(c->opt_tile_w);
+    A.tiles_x = (fc.win_w + (1u << A.tw_log2) - 1) >> A.tw_log2;
+    A.n_tiles = A.tiles_x * ((fc.win_h + (64u >> A.tw_log2) - 1) / (64u >> A.tw_log2));
+    A.fc = fc;
+}
+
+}  // namespace
+
+int render_frame(mmhrt_ctx_s* c, const FrameConsts& fc) {
+    uint32_t W = fc.win_w, H = fc.win_h;  // output size = the rendered window
+    size_t P = (size_t)W * H;
+    int e = 0;
+    if ((e = ensure(c, c->fb_col, P, "framebuffer colour"))) return e;
+    if ((e = ensure(c, c->fb_depth, P, "framebuffer depth"))) return e;
+    if ((e = ensure(c, c->fb_t, P, "framebuffer t"))) return e;
+    if ((e = ensure(c, c->fb_id, P, "framebuffer id"))) return e;
+    if ((e = ensure(c, c->fb_rgba8, P, "framebuffer rgba8"))) return e;
+    if (fc.want_aux && (e = ensure(c, c->fb_aux, P * 8, "framebuffer aux"))) return e;
+    if ((e = ensure(c, c->counters, TRACE_CTR_WORDS, "counters"))) return e;
+    unsigned grid = (unsigned)(c->cu_count * c->opt_blocks_per_cu);
+    if (grid == 0) grid = 1;
+    uint32_t lanes = grid * TB;
+    if ((e = ensure(c, c->spill, (size_t)lanes * TRACE_SPILL_PER_LANE, "traversal spill"))) return e;
+    c->fb_w = W;
+    c->fb_h = H;
+    c->fb_x0 = fc.win_x0;
+    c->fb_y0 = fc.win_y0;
+    c->fb_full_w = fc.width;
+    c->fb_full_h = fc.height;
+    FrameOutputs out{c->fb_col.p, c->fb_depth.p, c->fb_t.p, c->fb_id.p, c->fb_rgba8.p,
+                     fc.want_aux ? c->fb_aux.p : nullptr};
+    return render_frame_into(c, fc, out, c->counters.p, c->spill.p, lanes, c->opt_count != 0);
+}
+
+// The frame into the given device outputs: the persistent grid is capped at `lanes`, whose
+// traversal stacks spill into `spill` (lanes x TRACE_SPILL_PER_LANE words, lane-interleaved);
+// `ctr` (TRACE_CTR_WORDS words) takes the tile counter, the overflow flag and the instrumentation
+// counters and is cleared here by a fill kernel (safe under graph capture). Allocates nothing.
+int render_frame_into(mmhrt_ctx_s* c, const FrameConsts& fc, const FrameOutputs& out, uint32_t* ctr, int32_t* spill,
+                      uint32_t lanes, bool count) {
+    unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(c->cu_count * c->opt_blocks_per_cu, lanes / TB));
+    TraceArgs A;
+    fill_scene_args(c, fc, A);
+    A.ctr = ctr;
+    A.spill = spill;
+    A.spill_stride = grid * TB;
+    A.out_col = out.color;
+    A.out_depth = out.depth;
+    A.out_t = out.t;
+    A.out_id = out.hit_id;
+    A.out_rgba8 = out.rgba8;
+    A.out_aux = fc.want_aux ? out.aux : nullptr;
+    launch_fill_u32(c->stream, ctr, 0u, TRACE_CTR_WORDS);
+    rec(c, 6);
+    dispatch_trace(c, A, grid, count);
     rec(c, 7);
     hipError_t he = hipGetLastError();
     if (he != hipSuccess) return hip_fail(c, he, "trace launch");

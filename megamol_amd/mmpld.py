@@ -8,6 +8,9 @@ Format as parsed by MMPLDDataSource (plugins/moldyn/src/io/MMPLDDataSource.cpp):
                      [f32 global radius if vertex code in {1,3,4}],
                      [4 u8 global colour if colour code == 0 (alpha byte skipped, alpha = 255)],
                      [2 f32 intensity range if colour code in {3,7}],
+                     (these three are decided by the raw codes, even for a list whose vertex
+                     code is 0 or whose colour code is unknown; only the record sizes and the
+                     in-memory types fall back to NONE)
                      u64 particle count, [6 f32 list box if version >= 103],
                      count interleaved records of vertex bytes then colour bytes,
                      [version 101: u32 cluster count, u64 byte size, cluster bytes]
@@ -91,23 +94,24 @@ def read_frame(buf: bytes, hdr: Header, index: int) -> Frame:
     p += 4
     lists, boxes = [], []
     for _ in range(plc):
-        vt, ct = raw[p], raw[p + 1]
+        raw_vt, raw_ct = raw[p], raw[p + 1]
         p += 2
-        if vt not in VERT_SIZE_FILE:
-            vt = 0
-        if vt == 0 or ct not in COL_SIZE_FILE:
-            ct = 0
+        # types and record sizes: unknown codes read as NONE (size 0), and a list without
+        # vertices has no colours either (:89-157)
+        vt = raw_vt if raw_vt in VERT_SIZE_FILE else 0
+        ct = raw_ct if (vt != 0 and raw_ct in COL_SIZE_FILE) else 0
         vs, cs = VERT_SIZE_FILE[vt], COL_SIZE_FILE[ct]
+        # which header fields follow is decided by the raw codes (:160-178)
         grad = 0.05
-        if vt in (1, 3, 4):
+        if raw_vt in (1, 3, 4):
             (grad,) = struct.unpack_from("<f", raw, p)
             p += 4
         gcol = (192, 192, 192, 255)
         cmin, cmax = 0.0, 1.0
-        if ct == 0:
+        if raw_ct == 0:
             gcol = (raw[p], raw[p + 1], raw[p + 2], 255)
             p += 4
-        elif ct in (3, 7):
+        elif raw_ct in (3, 7):
             cmin, cmax = struct.unpack_from("<2f", raw, p)
             p += 8
         (cnt,) = struct.unpack_from("<Q", raw, p)

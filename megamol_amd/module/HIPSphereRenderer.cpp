@@ -164,7 +164,12 @@ bool HIPSphereRenderer::Render(core::view::CallRender3D& call) {
     p.mode = mode_slot_.Param<core::param::EnumParam>()->Value();
     p.max_bounces = max_bounces_slot_.Param<core::param::IntParam>()->Value();
     p.ao_distance = ao_distance_slot_.Param<core::param::FloatParam>()->Value();
-    p.accumulate = accumulate_slot_.Param<core::param::BoolParam>()->Value() ? 1 : 0;
+    // The reference blends every frame with frameIdx > 0 into the previous one whatever its
+    // accumulate parameter says (raygen.cu:254-260 never reads FrameState::accumulate, which
+    // Renderer.cpp:162 stores): the module does the same. The parameter stays for project files
+    // and, as in the reference, a change of it restarts the accumulation.
+    p.accumulate = 1;
+    const bool accumulate_param = accumulate_slot_.Param<core::param::BoolParam>()->Value();
     auto bg = call.BackgroundColor();
     p.background[0] = bg.r;
     p.background[1] = bg.g;
@@ -174,10 +179,12 @@ bool HIPSphereRenderer::Render(core::view::CallRender3D& call) {
     if (denoise) p.flags |= MMHRT_FLAG_AUX;
     // change detection resets the progressive accumulation (Renderer.cpp:93-176)
     p.frame_idx = 0;
-    if (changed || std::memcmp(&c, &last_cam_, sizeof c) != 0 || std::memcmp(&p, &last_params_, sizeof p) != 0)
+    if (changed || std::memcmp(&c, &last_cam_, sizeof c) != 0 || std::memcmp(&p, &last_params_, sizeof p) != 0 ||
+        accumulate_param != last_accumulate_param_)
         frame_idx_ = 0;
     last_cam_ = c;
     last_params_ = p;
+    last_accumulate_param_ = accumulate_param;
     p.frame_idx = frame_idx_;
     if (mmhrt_render(ctx_, &c, &p, nullptr) != MMHRT_OK) {  // out == NULL: the frame stays on the device
         Log::DefaultLog.WriteError("[HIPSphereRenderer] %s", mmhrt_last_error(ctx_));

@@ -53,6 +53,7 @@ private:
     uint32_t frame_idx_ = 0;  // progressive accumulation index (AbstractRenderer.cpp:108)
     mmhrt_camera last_cam_{};
     mmhrt_render_params last_params_{};
+    bool last_accumulate_param_ = true;
     std::string loaded_weights_;
     bool has_data_ = false;
 };

@@ -52,6 +52,17 @@ class OrcBvh(C.Structure):
 _lib = None
 
 
+def default_threads() -> int:
+    """Worker threads for the multi-threaded oracle paths: the CPUs this process may run on, at
+    most 16 (the host share of one GPU on the GPU hosts)."""
+    import os
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
 def build() -> Path:
     subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
     return LIB
@@ -90,11 +101,19 @@ def load():
                                       C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
                                       C.c_void_p]
         f.orc_bvh_build.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.POINTER(OrcBvh)]
+        f.orc_bvh_build_mt.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.c_int, C.POINTER(OrcBvh)]
+        f.orc_primary_f64_rects.restype = C.c_int
+        f.orc_primary_f64_rects.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(OrcCamera), C.POINTER(OrcParams),
+                                            C.c_void_p, C.c_uint32, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                            C.c_void_p]
         f.orc_bvh_free.argtypes = [C.POINTER(OrcBvh)]
         f.orc_render_bvh.argtypes = [C.POINTER(OrcBvh), C.c_void_p, C.POINTER(OrcCamera), C.POINTER(OrcParams), C.c_int,
                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                      C.c_void_p]
         f.orc_bvh_copy_out.argtypes = [C.POINTER(OrcBvh), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        f.orc_blend_frames.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32, C.c_void_p,
+                                       C.c_void_p, C.c_void_p]
+        f.orc_pack_rgba8.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p]
         f.orc_now.restype = C.c_double
         _lib = lib
     return _lib
@@ -142,12 +161,16 @@ def render_brute(xyzr, rgba, cam, p: OrcParams, aux=False) -> dict:
 
 
 class Bvh:
-    def __init__(self, xyzr, bits=30):
+    """The oracle's LBVH. threads: build threads (0 = default_threads(), 1 = the serial build;
+    both give the same tree)."""
+
+    def __init__(self, xyzr, bits=30, threads=0):
         self.lib = load()
         self.xyzr = np.ascontiguousarray(xyzr, np.float32)
         self.b = OrcBvh()
         t0 = self.lib.orc_now()
-        rc = self.lib.orc_bvh_build(self.xyzr.ctypes.data, len(self.xyzr), bits, C.byref(self.b))
+        rc = self.lib.orc_bvh_build_mt(self.xyzr.ctypes.data, len(self.xyzr), bits, threads or default_threads(),
+                                       C.byref(self.b))
         self.build_s = self.lib.orc_now() - t0
         if rc != 0:
             raise RuntimeError("oracle LBVH build failed")
@@ -172,7 +195,8 @@ class Bvh:
                     o[k] = None
         cnt = np.zeros(2, np.uint64)
         t0 = self.lib.orc_now()
-        self.lib.orc_render_bvh(C.byref(self.b), rgba.ctypes.data, C.byref(camera_from(cam)), C.byref(p), threads,
+        self.lib.orc_render_bvh(C.byref(self.b), rgba.ctypes.data, C.byref(camera_from(cam)), C.byref(p),
+                                threads or default_threads(),
                                 _ptr(o["rgba8"]), _ptr(o["depth"]), _ptr(o["color"]), _ptr(o["t"]),
                                 _ptr(o["hit_id"]), _ptr(o["aux"]), cnt.ctypes.data)
         o["seconds"] = self.lib.orc_now() - t0
@@ -202,6 +226,33 @@ def primary_f64(xyzr, cam, p: OrcParams, x0=0, y0=0, x1=None, y1=None):
     return ids.reshape(shp), t.reshape(shp), disc.reshape(shp), t2.reshape(shp)
 
 
+def primary_f64_rects(xyzr, cam, p: OrcParams, rects, threads=0):
+    """FP64 nearest hit of the first sample's ray for every pixel of the rectangles
+    [(x0, y0, x1, y1), ...] (multi-threaded, cone-culled: for scenes of any size). Returns one
+    (ids, t, disc, t2) tuple of 2-D arrays per rectangle."""
+    lib = load()
+    xyzr = np.ascontiguousarray(xyzr, np.float32)
+    r = np.ascontiguousarray(np.asarray(rects, np.uint32).reshape(-1, 4))
+    npx = int(sum((x1 - x0) * (y1 - y0) for x0, y0, x1, y1 in r.tolist()))
+    ids = np.zeros(max(npx, 1), np.int32)
+    t = np.zeros(max(npx, 1), np.float64)
+    disc = np.zeros(max(npx, 1), np.float64)
+    t2 = np.zeros(max(npx, 1), np.float64)
+    rc = lib.orc_primary_f64_rects(xyzr.ctypes.data, len(xyzr), C.byref(camera_from(cam)), C.byref(p),
+                                   r.ctypes.data, len(r), threads or default_threads(), ids.ctypes.data, t.ctypes.data,
+                                   disc.ctypes.data, t2.ctypes.data)
+    if rc != 0:
+        raise ValueError("rectangle outside the image")
+    out, o = [], 0
+    for x0, y0, x1, y1 in r.tolist():
+        m = (x1 - x0) * (y1 - y0)
+        shp = (y1 - y0, x1 - x0)
+        out.append((ids[o:o + m].reshape(shp), t[o:o + m].reshape(shp), disc[o:o + m].reshape(shp),
+                    t2[o:o + m].reshape(shp)))
+        o += m
+    return out
+
+
 def convert_lists(lists, tf=None, tf_min=1.0, tf_max=0.0):
     """Oracle ingest: lists are megamol_amd ParticleList objects."""
     lib = load()
@@ -222,6 +273,29 @@ def convert_lists(lists, tf=None, tf_min=1.0, tf_max=0.0):
     lib.orc_convert_lists(cl, len(lists), tfp, tfn, tf_min, tf_max, 1 if tf_min > tf_max else 0, xyzr.ctypes.data,
                           rgba.ctypes.data, C.byref(n))
     return xyzr[:n.value], rgba[:n.value]
+
+
+def blend_frames(old_col, new_frame, frame_idx, background):
+    """Progressive blend of a new frame (an oracle render dict) into the previous colour buffer
+    (raygen.cu:254-268). Returns (colour, rgba8)."""
+    lib = load()
+    new = np.ascontiguousarray(new_frame["color"], np.float32)
+    old = np.ascontiguousarray(old_col if old_col is not None else new, np.float32)
+    ids = np.ascontiguousarray(new_frame["hit_id"], np.int32)
+    bg = np.asarray(background, np.float32)
+    out = np.empty_like(new)
+    rgba8 = np.empty(ids.shape, np.uint32)
+    lib.orc_blend_frames(old.ctypes.data, new.ctypes.data, ids.ctypes.data, ids.size, frame_idx, bg.ctypes.data,
+                         out.ctypes.data, rgba8.ctypes.data)
+    return out, rgba8
+
+
+def pack_rgba8(col):
+    lib = load()
+    col = np.ascontiguousarray(col, np.float32)
+    out = np.empty(col.shape[:-1], np.uint32)
+    lib.orc_pack_rgba8(col.ctypes.data, out.size, out.ctypes.data)
+    return out
 
 
 def tea16(a, b):

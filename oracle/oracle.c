@@ -37,6 +37,39 @@ double orc_now(void) {
     return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
 }
 
+/* Run fn(arg, t, T) on T threads (T <= 1: inline). */
+typedef struct par_slot {
+    void (*fn)(void*, int, int);
+    void* arg;
+    int t, T;
+} par_slot;
+static void* par_entry(void* a) {
+    par_slot* s = a;
+    s->fn(s->arg, s->t, s->T);
+    return NULL;
+}
+static void par_run(int T, void (*fn)(void*, int, int), void* arg) {
+    if (T <= 1) {
+        fn(arg, 0, 1);
+        return;
+    }
+    pthread_t th[256];
+    par_slot sl[256];
+    for (int t = 0; t < T; ++t) {
+        sl[t] = (par_slot){fn, arg, t, T};
+        pthread_create(&th[t], NULL, par_entry, &sl[t]);
+    }
+    for (int t = 0; t < T; ++t) pthread_join(th[t], NULL);
+}
+static int threads_for(int threads, uint64_t work) {
+    if (threads <= 0) threads = (int)sysconf(_SC_NPROCESSORS_ONLN);
+    if (threads > 256) threads = 256;
+    if (work < 65536) threads = 1;
+    return threads < 1 ? 1 : threads;
+}
+#define PAR_RANGE(n, t, T, a, b) uint64_t a = (uint64_t)(((unsigned __int128)(n) * (t)) / (T)), \
+                                          b = (uint64_t)(((unsigned __int128)(n) * ((t) + 1)) / (T))
+
 /* ------------------------------------------------------------------ RNG */
 /* TEA encryption (Wheeler & Needham 1994) with an arbitrary key and cycle count. */
 void orc_tea_encrypt(uint32_t v[2], const uint32_t k[4], int cycles) {
@@ -692,6 +725,31 @@ static void store_pixel(const pixel_out* po, size_t i, uint32_t* rgba8, float* d
         for (int c = 0; c < 8; ++c) aux[8 * i + c] = f2h(po->aux[c]);
 }
 
+/* Progressive blend of raygen.cu:254-268 over a whole frame: a pixel that some sample of the new
+ * frame hit becomes lerp(old, new, 1 / (frame_idx + 1)) with lerp(a, b, t) = a + t * (b - a)
+ * (utils_device.h:152-154); a pixel that no sample hit takes the background (the override comes
+ * after the blend). frame_idx 0 keeps the new colour. rgba8 (may be NULL) receives the packed
+ * result. */
+void orc_blend_frames(const float* old_col, const float* new_col, const int32_t* hit_id, uint64_t n_px,
+                      uint32_t frame_idx, const float bg[4], float* out_col, uint32_t* rgba8) {
+    const float a = 1.0f / (float)(frame_idx + 1);
+    for (uint64_t i = 0; i < n_px; ++i) {
+        float c[4];
+        for (int k = 0; k < 4; ++k) {
+            float nv = new_col[4 * i + k];
+            c[k] = frame_idx > 0 ? old_col[4 * i + k] + a * (nv - old_col[4 * i + k]) : nv;
+            if (hit_id[i] < 0) c[k] = bg[k];
+        }
+        memcpy(out_col + 4 * i, c, sizeof c);
+        if (rgba8) rgba8[i] = to_rgba8(c);
+    }
+}
+
+/* RGBA8 packing of the framebuffer (clamp to [0,1], round half up). */
+void orc_pack_rgba8(const float* col, uint64_t n_px, uint32_t* out) {
+    for (uint64_t i = 0; i < n_px; ++i) out[i] = to_rgba8(col + 4 * i);
+}
+
 int orc_render_brute(const float* xyzr, const float* rgba, uint64_t n, const orc_camera* cam, const orc_params* p,
                      uint32_t* rgba8, float* depth, float* color, float* t_out, int32_t* id_out, uint16_t* aux) {
     orc_frame_state fs;
@@ -745,6 +803,162 @@ int orc_primary_f64(const float* xyzr, uint64_t n, const orc_camera* cam, const 
     return 0;
 }
 
+/* The same FP64 nearest hit for the pixels of k rectangles (rects: x0, y0, x1, y1 each; outputs
+ * rectangle by rectangle, row-major inside each), over very large scenes: threads split the
+ * spheres into contiguous ranges, and a sphere is tested against a rectangle's rays only if it
+ * meets the cone from the camera that holds all of them (a conservative bound: axis a, half
+ * angle th; a sphere at axial distance s and distance q from the axis can meet the cone only if
+ * q <= max(s, 0) tan th + r / cos th). Per-thread results merge in range order, so equal t goes
+ * to the lowest index as in orc_primary_f64. */
+typedef struct f64_rect {
+    uint32_t x0, y0, x1, y1;
+    size_t first; /* first output slot */
+    double axis[3], tan_th, inv_cos;
+} f64_rect;
+typedef struct f64_job {
+    const float* xyzr;
+    uint64_t n;
+    double o[3];
+    const double* dirs; /* 3 per output pixel */
+    const f64_rect* rects;
+    uint32_t k;
+    size_t npx;
+    double *best, *second, *disc; /* per thread: npx each */
+    int32_t* id;
+} f64_job;
+
+static void f64_worker(void* a, int t, int T) {
+    f64_job* J = a;
+    PAR_RANGE(J->n, t, T, i0, i1);
+    double* best = J->best + (size_t)t * J->npx;
+    double* second = J->second + (size_t)t * J->npx;
+    double* disc = J->disc + (size_t)t * J->npx;
+    int32_t* id = J->id + (size_t)t * J->npx;
+    for (size_t q = 0; q < J->npx; ++q) {
+        best[q] = second[q] = INFINITY;
+        disc[q] = 0.0;
+        id[q] = -1;
+    }
+    for (uint64_t i = i0; i < i1; ++i) {
+        double sp[4] = {J->xyzr[4 * i], J->xyzr[4 * i + 1], J->xyzr[4 * i + 2], J->xyzr[4 * i + 3]};
+        double w[3] = {sp[0] - J->o[0], sp[1] - J->o[1], sp[2] - J->o[2]};
+        double ww = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+        for (uint32_t r = 0; r < J->k; ++r) {
+            const f64_rect* R = &J->rects[r];
+            double s = w[0] * R->axis[0] + w[1] * R->axis[1] + w[2] * R->axis[2];
+            double q2 = ww - s * s;
+            double lim = (s > 0.0 ? s : 0.0) * R->tan_th + fabs(sp[3]) * R->inv_cos;
+            lim = lim * (1.0 + 1e-9) + 1e-9 * sqrt(ww) + 1e-12;
+            if (q2 > lim * lim) continue;
+            size_t q = R->first;
+            for (uint32_t y = R->y0; y < R->y1; ++y)
+                for (uint32_t x = R->x0; x < R->x1; ++x, ++q) {
+                    double tt, dd;
+                    if (orc_intersect_f64(J->o, J->dirs + 3 * q, 1e-6, sp, &tt, &dd) && tt < 1e20) {
+                        if (tt < best[q]) {
+                            second[q] = best[q];
+                            best[q] = tt;
+                            id[q] = (int32_t)i;
+                            disc[q] = dd;
+                        } else if (tt < second[q]) {
+                            second[q] = tt;
+                        }
+                    }
+                }
+        }
+    }
+}
+
+int orc_primary_f64_rects(const float* xyzr, uint64_t n, const orc_camera* cam, const orc_params* p,
+                          const uint32_t* rects, uint32_t k, int threads, int32_t* id_out, double* t_out,
+                          double* disc_out, double* t2_out) {
+    orc_frame_state fs;
+    orc_frame_state_from_camera(cam, &fs);
+    f64_rect* R = calloc(k ? k : 1, sizeof *R);
+    size_t npx = 0;
+    for (uint32_t r = 0; r < k; ++r) {
+        R[r].x0 = rects[4 * r];
+        R[r].y0 = rects[4 * r + 1];
+        R[r].x1 = rects[4 * r + 2];
+        R[r].y1 = rects[4 * r + 3];
+        if (R[r].x1 > p->width || R[r].y1 > p->height || R[r].x0 > R[r].x1 || R[r].y0 > R[r].y1) {
+            free(R);
+            return -1;
+        }
+        R[r].first = npx;
+        npx += (size_t)(R[r].x1 - R[r].x0) * (R[r].y1 - R[r].y0);
+    }
+    double* dirs = malloc((npx ? npx : 1) * 3 * sizeof(double));
+    double o[3] = {fs.center[0], fs.center[1], fs.center[2]};
+    for (uint32_t r = 0; r < k; ++r) {
+        double ax[3] = {0, 0, 0};
+        size_t q = R[r].first;
+        for (uint32_t y = R[r].y0; y < R[r].y1; ++y)
+            for (uint32_t x = R[r].x0; x < R[r].x1; ++x, ++q) {
+                /* the first sample's jittered ray, generated in FP32 as the reference does */
+                uint32_t seed = orc_tea16(y * p->width + x, p->frame_idx);
+                float rx = orc_rnd(&seed), ry = orc_rnd(&seed);
+                float of[3], df[3];
+                orc_primary_ray(&fs, p->width, p->height, x, y, rx, ry, of, df);
+                for (int a = 0; a < 3; ++a) {
+                    dirs[3 * q + a] = df[a];
+                    ax[a] += df[a];
+                }
+            }
+        double len = sqrt(ax[0] * ax[0] + ax[1] * ax[1] + ax[2] * ax[2]);
+        double cmin = 1.0;
+        for (int a = 0; a < 3; ++a) R[r].axis[a] = len > 0 ? ax[a] / len : 0.0;
+        for (size_t j = R[r].first; j < q; ++j) {
+            const double* d = dirs + 3 * j;
+            double dl = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            double c = (d[0] * R[r].axis[0] + d[1] * R[r].axis[1] + d[2] * R[r].axis[2]) / dl;
+            if (c < cmin) cmin = c;
+        }
+        if (len == 0.0 || cmin < 0.05) { /* degenerate or very wide: no culling */
+            R[r].tan_th = 1e300;
+            R[r].inv_cos = 1e300;
+        } else {
+            cmin = cmin * (1.0 - 1e-12) - 1e-12;
+            R[r].tan_th = sqrt(fmax(0.0, 1.0 - cmin * cmin)) / cmin;
+            R[r].inv_cos = 1.0 / cmin;
+        }
+    }
+    int T = threads_for(threads, n);
+    f64_job J = {xyzr, n, {o[0], o[1], o[2]}, dirs, R, k, npx, NULL, NULL, NULL, NULL};
+    size_t tot = (size_t)T * (npx ? npx : 1);
+    J.best = malloc(tot * sizeof(double));
+    J.second = malloc(tot * sizeof(double));
+    J.disc = malloc(tot * sizeof(double));
+    J.id = malloc(tot * sizeof(int32_t));
+    par_run(T, f64_worker, &J);
+    for (size_t q = 0; q < npx; ++q) {
+        double b = INFINITY, s = INFINITY, dd = 0.0;
+        int32_t id = -1;
+        for (int t = 0; t < T; ++t) { /* ranges in index order: a later range wins only if strictly nearer */
+            size_t j = (size_t)t * npx + q;
+            if (J.best[j] < b) {
+                s = fmin(b, J.second[j]);
+                b = J.best[j];
+                id = J.id[j];
+                dd = J.disc[j];
+            } else {
+                s = fmin(s, J.best[j]);
+            }
+        }
+        id_out[q] = id;
+        t_out[q] = b;
+        disc_out[q] = dd;
+        t2_out[q] = s;
+    }
+    free(J.best);
+    free(J.second);
+    free(J.disc);
+    free(J.id);
+    free(dirs);
+    free(R);
+    return 0;
+}
+
 /* --------------------------------------------------------------- LBVH */
 /* Morton codes (Morton 1966) over the centre bounds with one uniform scale, then a stable sort
  * by key, the Karras (2012, "Maximizing parallelism in the construction of BVHs, octrees and
@@ -774,113 +988,201 @@ static int delta_k(const uint64_t* keys, uint64_t n, int64_t i, int64_t j) {
     return __builtin_clzll(a ^ b);
 }
 
-int orc_bvh_build(const float* xyzr, uint64_t n, int bits, orc_bvh* out) {
-    memset(out, 0, sizeof *out);
-    if (n == 0 || n > 0x7fffffffull) return -1;
-    out->n = n;
+typedef struct build_job {
+    const float* xyzr;
+    uint64_t n, m;
+    int bits, shift;
+    float tlo[256][3], thi[256][3];
+    float lo[3], scale;
+    uint32_t maxq;
+    uint64_t *key, *key2;
+    uint32_t *idx, *idx2;
+    size_t (*hist)[256]; /* per thread digit counts, then scatter offsets */
+    orc_bvh* out;
+    int32_t* parent;
+    atomic_uchar* seen;
+} build_job;
+
+static void bj_bounds(void* a, int t, int T) {
+    build_job* J = a;
+    PAR_RANGE(J->n, t, T, i0, i1);
     float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
-    for (uint64_t i = 0; i < n; ++i)
-        for (int a = 0; a < 3; ++a) {
-            lo[a] = fminf(lo[a], xyzr[4 * i + a]);
-            hi[a] = fmaxf(hi[a], xyzr[4 * i + a]);
+    for (uint64_t i = i0; i < i1; ++i)
+        for (int ax = 0; ax < 3; ++ax) {
+            lo[ax] = fminf(lo[ax], J->xyzr[4 * i + ax]);
+            hi[ax] = fmaxf(hi[ax], J->xyzr[4 * i + ax]);
         }
-    float ext = fmaxf(fmaxf(hi[0] - lo[0], hi[1] - lo[1]), hi[2] - lo[2]);
-    float cells = bits == 63 ? 2097152.0f : 1024.0f;
-    uint32_t maxq = bits == 63 ? 2097151u : 1023u;
-    float scale = ext > 0.0f ? cells / ext : 0.0f;
-    uint64_t* key = malloc(n * 8), *key2 = malloc(n * 8);
-    uint32_t* idx = malloc(n * 4), *idx2 = malloc(n * 4);
-    for (uint64_t i = 0; i < n; ++i) {
+    memcpy(J->tlo[t], lo, sizeof lo);
+    memcpy(J->thi[t], hi, sizeof hi);
+}
+static void bj_keys(void* a, int t, int T) {
+    build_job* J = a;
+    PAR_RANGE(J->n, t, T, i0, i1);
+    for (uint64_t i = i0; i < i1; ++i) {
         uint32_t q[3];
-        for (int a = 0; a < 3; ++a) {
-            float f = (xyzr[4 * i + a] - lo[a]) * scale;
+        for (int ax = 0; ax < 3; ++ax) {
+            float f = (J->xyzr[4 * i + ax] - J->lo[ax]) * J->scale;
             uint32_t u = (uint32_t)f;
-            q[a] = u > maxq ? maxq : u;
+            q[ax] = u > J->maxq ? J->maxq : u;
         }
-        key[i] = bits == 63 ? ((expand21(q[0]) << 2) | (expand21(q[1]) << 1) | expand21(q[2]))
-                            : (uint64_t)((expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]));
-        idx[i] = (uint32_t)i;
+        J->key[i] = J->bits == 63 ? ((expand21(q[0]) << 2) | (expand21(q[1]) << 1) | expand21(q[2]))
+                                  : (uint64_t)((expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]));
+        J->idx[i] = (uint32_t)i;
     }
-    /* stable LSD radix sort, 8-bit digits */
-    int passes = bits == 63 ? 8 : 4;
-    for (int ps = 0; ps < passes; ++ps) {
-        size_t cnt[257] = {0};
-        int sh = 8 * ps;
-        for (uint64_t i = 0; i < n; ++i) cnt[((key[i] >> sh) & 0xff) + 1]++;
-        for (int b = 0; b < 256; ++b) cnt[b + 1] += cnt[b];
-        for (uint64_t i = 0; i < n; ++i) {
-            size_t dst = cnt[(key[i] >> sh) & 0xff]++;
-            key2[dst] = key[i];
-            idx2[dst] = idx[i];
-        }
-        uint64_t* tk = key; key = key2; key2 = tk;
-        uint32_t* ti = idx; idx = idx2; idx2 = ti;
+}
+static void bj_hist(void* a, int t, int T) {
+    build_job* J = a;
+    PAR_RANGE(J->n, t, T, i0, i1);
+    size_t* h = J->hist[t];
+    memset(h, 0, 256 * sizeof(size_t));
+    for (uint64_t i = i0; i < i1; ++i) h[(J->key[i] >> J->shift) & 0xff]++;
+}
+static void bj_scatter(void* a, int t, int T) {
+    build_job* J = a;
+    PAR_RANGE(J->n, t, T, i0, i1);
+    size_t* off = J->hist[t];
+    for (uint64_t i = i0; i < i1; ++i) {
+        size_t dst = off[(J->key[i] >> J->shift) & 0xff]++;
+        J->key2[dst] = J->key[i];
+        J->idx2[dst] = J->idx[i];
     }
-    free(key2);
-    free(idx2);
-    out->keys = key;
-    out->leaf_id = idx;
-    out->leaf_sph = malloc(n * 16);
-    for (uint64_t k = 0; k < n; ++k) memcpy(out->leaf_sph + 4 * k, xyzr + 4 * (size_t)idx[k], 16);
-    if (n == 1) return 0;
-    uint64_t m = n - 1;
-    out->node_box = malloc(m * 12 * sizeof(float));
-    out->node_child = malloc(m * 2 * sizeof(int32_t));
-    int32_t* parent = malloc((2 * n - 1) * sizeof(int32_t)); /* [0,m) internal, [m, m+n) leaves */
-    parent[0] = -1;
-    for (int64_t i = 0; i < (int64_t)m; ++i) {
+}
+static void bj_gather(void* a, int t, int T) {
+    build_job* J = a;
+    PAR_RANGE(J->n, t, T, k0, k1);
+    for (uint64_t k = k0; k < k1; ++k)
+        memcpy(J->out->leaf_sph + 4 * k, J->xyzr + 4 * (size_t)J->out->leaf_id[k], 16);
+}
+/* Karras (2012) node i: direction, range end by exponential + binary search, split by binary
+ * search over the common prefix of the augmented keys. */
+static void bj_nodes(void* a, int t, int T) {
+    build_job* J = a;
+    const uint64_t* key = J->out->keys;
+    uint64_t n = J->n, m = J->m;
+    PAR_RANGE(m, t, T, i0, i1);
+    for (int64_t i = (int64_t)i0; i < (int64_t)i1; ++i) {
         int dir = (delta_k(key, n, i, i + 1) - delta_k(key, n, i, i - 1)) >= 0 ? 1 : -1;
         int dmin = delta_k(key, n, i, i - dir);
         int64_t lmax = 2;
         while (delta_k(key, n, i, i + lmax * dir) > dmin) lmax *= 2;
         int64_t l = 0;
-        for (int64_t t = lmax / 2; t >= 1; t /= 2)
-            if (delta_k(key, n, i, i + (l + t) * dir) > dmin) l += t;
+        for (int64_t s = lmax / 2; s >= 1; s /= 2)
+            if (delta_k(key, n, i, i + (l + s) * dir) > dmin) l += s;
         int64_t j = i + l * dir;
         int dnode = delta_k(key, n, i, j);
-        int64_t s = 0, t = l;
+        int64_t s = 0, w = l;
         do {
-            t = (t + 1) >> 1;
-            if (delta_k(key, n, i, i + (s + t) * dir) > dnode) s += t;
-        } while (t > 1);
+            w = (w + 1) >> 1;
+            if (delta_k(key, n, i, i + (s + w) * dir) > dnode) s += w;
+        } while (w > 1);
         int64_t gamma = i + s * dir + (dir < 0 ? -1 : 0);
         int64_t first = i < j ? i : j, last = i < j ? j : i;
         int32_t left = (first == gamma) ? ~(int32_t)gamma : (int32_t)gamma;
         int32_t right = (last == gamma + 1) ? ~(int32_t)(gamma + 1) : (int32_t)(gamma + 1);
-        out->node_child[2 * i] = left;
-        out->node_child[2 * i + 1] = right;
-        parent[left < 0 ? m + (uint64_t)(~left) : (uint64_t)left] = (int32_t)i;
-        parent[right < 0 ? m + (uint64_t)(~right) : (uint64_t)right] = (int32_t)i;
+        J->out->node_child[2 * i] = left;
+        J->out->node_child[2 * i + 1] = right;
+        J->parent[left < 0 ? m + (uint64_t)(~left) : (uint64_t)left] = (int32_t)i;
+        J->parent[right < 0 ? m + (uint64_t)(~right) : (uint64_t)right] = (int32_t)i;
     }
-    /* bottom-up boxes: second arrival at a node computes its box */
-    uint8_t* seen = calloc(m, 1);
-    float* nbox = malloc(m * 6 * sizeof(float)); /* box of internal node i */
-    for (uint64_t k = 0; k < n; ++k) {
+}
+/* Bottom-up boxes: every leaf climbs; the first arrival at a node stops, the second (which
+ * finds the other child's box stored) computes the node's box and climbs on. */
+static void bj_boxes(void* a, int t, int T) {
+    build_job* J = a;
+    orc_bvh* out = J->out;
+    uint64_t m = J->m;
+    PAR_RANGE(J->n, t, T, k0, k1);
+    for (uint64_t k = k0; k < k1; ++k) {
         const float* sp = out->leaf_sph + 4 * k;
         float b[6] = {sp[0] - sp[3], sp[1] - sp[3], sp[2] - sp[3], sp[0] + sp[3], sp[1] + sp[3], sp[2] + sp[3]};
         int32_t child = ~(int32_t)k;
-        int32_t p = parent[m + k];
+        int32_t p = J->parent[m + k];
         while (p >= 0) {
             int slot = out->node_child[2 * p] == child ? 0 : 1;
             memcpy(out->node_box + 12 * (size_t)p + 6 * slot, b, sizeof b);
-            if (!seen[p]) {
-                seen[p] = 1;
-                break;
-            }
+            if (atomic_fetch_add_explicit(&J->seen[p], 1, memory_order_acq_rel) == 0) break;
             const float* c0 = out->node_box + 12 * (size_t)p;
-            for (int a = 0; a < 3; ++a) {
-                b[a] = fminf(c0[a], c0[6 + a]);
-                b[3 + a] = fmaxf(c0[3 + a], c0[9 + a]);
+            for (int ax = 0; ax < 3; ++ax) {
+                b[ax] = fminf(c0[ax], c0[6 + ax]);
+                b[3 + ax] = fmaxf(c0[3 + ax], c0[9 + ax]);
             }
-            memcpy(nbox + 6 * (size_t)p, b, sizeof b);
             child = p;
-            p = parent[p];
+            p = J->parent[p];
         }
     }
-    free(seen);
-    free(nbox);
-    free(parent);
+}
+
+int orc_bvh_build_mt(const float* xyzr, uint64_t n, int bits, int threads, orc_bvh* out) {
+    memset(out, 0, sizeof *out);
+    if (n == 0 || n > 0x7fffffffull) return -1;
+    int T = threads_for(threads, n);
+    build_job* J = calloc(1, sizeof *J);
+    J->xyzr = xyzr;
+    J->n = n;
+    J->m = n - 1;
+    J->bits = bits;
+    J->out = out;
+    out->n = n;
+    par_run(T, bj_bounds, J);
+    float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (int t = 0; t < T; ++t)
+        for (int ax = 0; ax < 3; ++ax) {
+            lo[ax] = fminf(lo[ax], J->tlo[t][ax]);
+            hi[ax] = fmaxf(hi[ax], J->thi[t][ax]);
+        }
+    float ext = fmaxf(fmaxf(hi[0] - lo[0], hi[1] - lo[1]), hi[2] - lo[2]);
+    float cells = bits == 63 ? 2097152.0f : 1024.0f;
+    memcpy(J->lo, lo, sizeof lo);
+    J->maxq = bits == 63 ? 2097151u : 1023u;
+    J->scale = ext > 0.0f ? cells / ext : 0.0f;
+    J->key = malloc(n * 8);
+    J->key2 = malloc(n * 8);
+    J->idx = malloc(n * 4);
+    J->idx2 = malloc(n * 4);
+    par_run(T, bj_keys, J);
+    /* stable LSD radix sort, 8-bit digits: per-thread counts over contiguous ranges, offsets in
+     * (digit, thread) order, so equal keys keep their input order */
+    J->hist = malloc((size_t)T * sizeof *J->hist);
+    int passes = bits == 63 ? 8 : 4;
+    for (int ps = 0; ps < passes; ++ps) {
+        J->shift = 8 * ps;
+        par_run(T, bj_hist, J);
+        size_t run = 0;
+        for (int d = 0; d < 256; ++d)
+            for (int t = 0; t < T; ++t) {
+                size_t c = J->hist[t][d];
+                J->hist[t][d] = run;
+                run += c;
+            }
+        par_run(T, bj_scatter, J);
+        uint64_t* tk = J->key; J->key = J->key2; J->key2 = tk;
+        uint32_t* ti = J->idx; J->idx = J->idx2; J->idx2 = ti;
+    }
+    free(J->hist);
+    free(J->key2);
+    free(J->idx2);
+    out->keys = J->key;
+    out->leaf_id = J->idx;
+    out->leaf_sph = malloc(n * 16);
+    par_run(T, bj_gather, J);
+    if (n > 1) {
+        uint64_t m = n - 1;
+        out->node_box = malloc(m * 12 * sizeof(float));
+        out->node_child = malloc(m * 2 * sizeof(int32_t));
+        J->parent = malloc((2 * n - 1) * sizeof(int32_t)); /* [0,m) internal, [m, m+n) leaves */
+        J->parent[0] = -1;
+        par_run(T, bj_nodes, J);
+        J->seen = calloc(m, sizeof(atomic_uchar));
+        par_run(T, bj_boxes, J);
+        free((void*)J->seen);
+        free(J->parent);
+    }
+    free(J);
     return 0;
+}
+
+int orc_bvh_build(const float* xyzr, uint64_t n, int bits, orc_bvh* out) {
+    return orc_bvh_build_mt(xyzr, n, bits, 1, out);
 }
 
 void orc_bvh_free(orc_bvh* b) {

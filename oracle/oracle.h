@@ -74,13 +74,20 @@ int orc_convert_lists(const orc_list* lists, uint32_t n_lists, const float* tf, 
                       float tf_max, int tf_auto_range, float* xyzr_out, float* rgba_out, uint64_t* n_out);
 void orc_tf_sample(const float* tf, uint32_t tf_n, float vmin, float vmax, float value, float out[4]);
 
+void orc_blend_frames(const float* old_col, const float* new_col, const int32_t* hit_id, uint64_t n_px,
+                      uint32_t frame_idx, const float bg[4], float* out_col, uint32_t* rgba8);
+void orc_pack_rgba8(const float* col, uint64_t n_px, uint32_t* out);
 int orc_render_brute(const float* xyzr, const float* rgba, uint64_t n, const orc_camera* cam, const orc_params* p,
                      uint32_t* rgba8, float* depth, float* color, float* t_out, int32_t* id_out, uint16_t* aux);
 int orc_primary_f64(const float* xyzr, uint64_t n, const orc_camera* cam, const orc_params* p, uint32_t x0,
                     uint32_t y0, uint32_t x1, uint32_t y1, int32_t* id_out, double* t_out, double* disc_out,
                     double* t2_out);
+int orc_primary_f64_rects(const float* xyzr, uint64_t n, const orc_camera* cam, const orc_params* p,
+                          const uint32_t* rects, uint32_t k, int threads, int32_t* id_out, double* t_out,
+                          double* disc_out, double* t2_out);
 
 int orc_bvh_build(const float* xyzr, uint64_t n, int bits, orc_bvh* out);
+int orc_bvh_build_mt(const float* xyzr, uint64_t n, int bits, int threads, orc_bvh* out);
 void orc_bvh_free(orc_bvh* b);
 int orc_render_bvh(const orc_bvh* b, const float* rgba, const orc_camera* cam, const orc_params* p, int threads,
                    uint32_t* rgba8, float* depth, float* color, float* t_out, int32_t* id_out, uint16_t* aux,

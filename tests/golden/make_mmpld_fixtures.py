@@ -7,6 +7,10 @@ with radius 0.1; bounding box overridden to +-2; frame time 1.23). The Perl gene
 needs the CPAN module Readonly, which this image lacks, so the files are written with
 megamol_amd.mmpld at version 102, the version MMPLD.pm writes (utils/MMPLD/MMPLD.pm:73).
 
+versions/ holds the first fixture's lists written at versions 100 and 103 (no timestamp; per-list
+boxes) and as a two-frame version 103 file, so that the version-dependent fields and the seek table
+are read back by an independent parser too (tests/golden/make_mmpldinfo_vectors.py).
+
 expected.npz holds, per fixture, the float32 xyzr and rgba that SphereGeometry::assertData
 (plugins/optix_hpg/src/optix/SphereGeometry.cpp:94-146) derives, written from the closed form
 above (no renderer code involved).
@@ -93,6 +97,22 @@ def main():
         expected[name + "_rgba"] = rgba
     np.savez(OUT / "expected.npz", **expected)
     print(f"wrote {len(TESTS)} fixtures to {OUT}")
+    write_versions()
+
+
+def write_versions():
+    """The first GenTestFiles list set at versions 100 and 103, and as two frames at 103."""
+    vd = OUT / "versions"
+    vd.mkdir(exist_ok=True)
+    first = mmpld.ListSpec(1, 4, POS, colors=COL[:, :3], global_radius=0.1, imin=0.0, imax=255.0)
+    second = mmpld.ListSpec(2, 3, POS[::-1].copy(), radius=RAD[::-1].copy(), intensity=INT, imin=10.0, imax=200.0,
+                            box=(-0.5, -0.25, 0.0, 1.5, 1.25, 1.0))
+    for ver in (100, 103):
+        mmpld.write(str(vd / f"v{ver}.mmpld"), [[first] + edge_lists()], version=ver, timestamps=[1.23],
+                    bbox=(-2, -2, -2, 2, 2, 2), clipbox=(-2.5, -2.5, -2.5, 2.5, 2.5, 2.5))
+    mmpld.write(str(vd / "v103_two_frames.mmpld"), [[first] + edge_lists(), [second, first]], version=103,
+                timestamps=[0.5, 2.75], bbox=(-2, -2, -2, 2, 2, 2), clipbox=(-2.5, -2.5, -2.5, 2.5, 2.5, 2.5))
+    print(f"wrote 3 version fixtures to {vd}")
 
 
 if __name__ == "__main__":

@@ -71,3 +71,37 @@ def test_batch_errors(gpu_renderer):
         with pytest.raises(L.MMHRTError) as e:
             r.render_batch([(0, cam, params(8, 8, mode=L.MODE_BOUNCE))])
         assert e.value.code == L.E_INVALID
+
+
+def test_batch_keeps_its_own_colours_and_the_context_state(gpu_renderer):
+    """set_timesteps, then set_spheres with another set, then render_batch: the batch renders the
+    timesteps with their own colours; afterwards the context renders its own spheres again (after
+    the build that the batch's last-timestep tree requires) and equals the oracle."""
+    import oracle as O
+    from helpers import assert_frames_equal, oracle_params
+    r = gpu_renderer
+    xs = series(5000, 3)
+    rgba_ts = np.random.default_rng(2).random((xs.shape[1], 4)).astype(np.float32)
+    other, rgba_o = scenes.random_spheres(7000, seed=31, lo=5.0, hi=15.0, rmin=0.1, rmax=0.4)
+    jobs = jobs_for(3)
+    ref = []
+    for t, cam, p in jobs:
+        r.set_spheres(xs[t], rgba_ts)
+        r.build(L.BUILD_REBUILD, 30)
+        ref.append(r.render(cam, p, outputs=("rgba8", "depth")))
+    case("batch after set_spheres of another set")
+    r.set_timesteps(xs, rgba_ts)
+    r.set_spheres(other, rgba_o)
+    out, _ = r.render_batch(jobs, refit_period=2, use_graph=True)
+    for k, ((a, d), f) in enumerate(zip(out, ref)):
+        np.testing.assert_array_equal(a, f.rgba8, err_msg=f"job {k}")
+        np.testing.assert_array_equal(d.view(np.uint32), f.depth.view(np.uint32), err_msg=f"job {k}")
+    assert r.info()[0] == len(other)
+    cam = look_at((10, 10, 40), (10, 10, 10), (0, 1, 0), 40.0, 1.0, 1.0, 200.0)
+    p = params(40, 40, spp=2, mode=L.MODE_AO, ao_distance=2.0)
+    with pytest.raises(L.MMHRTError) as e:
+        r.render(cam, p)
+    assert e.value.code == L.E_STATE
+    r.build(L.BUILD_REBUILD, 30)
+    g = r.render(cam, p)
+    assert_frames_equal(g, O.render_brute(other, rgba_o, cam, oracle_params(p)), what="context spheres after batch")

@@ -238,28 +238,40 @@ def test_refit_equals_rebuild(gpu_renderer, grid):
 
 
 def test_progressive_accumulation(gpu_renderer):
+    """raygen.cu:254-268: frame k blends into the stored frame with weight 1 / (k + 1), against
+    the oracle's blend of the oracle frames; a different window, window origin or frame size
+    cannot be blended into (E_STATE); accumulate 0 leaves frame_idx a seed only."""
     case("progressive accumulation")
     xyzr, rgba = scenes.random_spheres(2000, seed=6)
     cam = small_camera(48, 48)
     gpu_renderer.set_spheres(xyzr, rgba)
     gpu_renderer.build()
-    frames = []
-    for k in range(3):
-        p = params(48, 48, spp=1, mode=L.MODE_AO, ao_distance=0.3, frame_idx=k, accumulate=1)
-        frames.append(gpu_renderer.render(cam, p).color)
-    exp = None
-    for k in range(3):
-        o = O.render_brute(xyzr, rgba, cam, O.make_params(48, 48, 1, 1, 0.3, k))
-        c = o["color"]
-        if exp is None:
-            exp = c
-        else:
-            # raygen.cu:254-260: old + a * (new - old) before the miss override
-            a = np.float32(1.0) / np.float32(k + 1)
-            exp = exp + a * (c - exp)
-            miss = o["hit_id"] < 0
-            exp[miss] = np.array([0, 0, 0, 1], np.float32)
-        np.testing.assert_array_equal(frames[k].view(np.uint32), exp.view(np.uint32))
+    bg = (0.1, 0.2, 0.3, 1.0)
+    acc = None
+    for k in range(4):
+        p = params(48, 48, spp=1, mode=L.MODE_AO, ao_distance=0.3, frame_idx=k, accumulate=1, background=bg)
+        g = gpu_renderer.render(cam, p)
+        o = O.render_brute(xyzr, rgba, cam, O.make_params(48, 48, 1, 1, 0.3, k, bg))
+        acc, rgba8 = O.blend_frames(acc, o, k, bg)
+        np.testing.assert_array_equal(g.color.view(np.uint32), acc.view(np.uint32), err_msg=f"frame {k}")
+        np.testing.assert_array_equal(g.rgba8, rgba8, err_msg=f"frame {k} rgba8")
+    # accumulate 0: frame_idx seeds the streams, nothing is blended
+    p = params(48, 48, spp=1, mode=L.MODE_AO, ao_distance=0.3, frame_idx=5, accumulate=0, background=bg)
+    o = O.render_brute(xyzr, rgba, cam, O.make_params(48, 48, 1, 1, 0.3, 5, bg))
+    np.testing.assert_array_equal(gpu_renderer.render(cam, p).color.view(np.uint32), o["color"].view(np.uint32))
+    # the stored frame must be the same window of a frame of the same size
+    base = params(48, 48, spp=1, frame_idx=0, accumulate=1)
+    base.window[:] = [8, 8, 16, 16]
+    gpu_renderer.render(cam, base)
+    same = params(48, 48, spp=1, frame_idx=1, accumulate=1)
+    same.window[:] = [8, 8, 16, 16]
+    gpu_renderer.render(cam, same)
+    for win, size in [([0, 0, 16, 16], (48, 48)), ([8, 8, 16, 16], (64, 48)), ([8, 8, 20, 16], (48, 48))]:
+        q = params(size[0], size[1], spp=1, frame_idx=2, accumulate=1)
+        q.window[:] = win
+        with pytest.raises(L.MMHRTError) as e:
+            gpu_renderer.render(cam, q)
+        assert e.value.code == L.E_STATE, (win, size)
 
 
 def test_error_paths(gpu_renderer):

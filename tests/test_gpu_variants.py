@@ -42,11 +42,12 @@ def test_variants_bit_identical(gpu_renderer, mode):
         set_opts(gpu_renderer, DEFAULTS)
 
 
-@pytest.mark.parametrize("spp", [2, 4, 64, 128])
+@pytest.mark.parametrize("spp", [2, 4, 8, 16, 32, 64, 128])
 @pytest.mark.parametrize("mode", [L.MODE_LAMBERT, L.MODE_AO])
 def test_sample_lanes_bit_identical(gpu_renderer, mode, spp):
     """The sample-parallel kernel (lanes = samples of a pixel, jitter stream advanced by LCG
-    skip-ahead, ordered fold) against the lane-per-pixel kernel and the oracle."""
+    skip-ahead, ordered fold) against the lane-per-pixel kernel and the oracle, for every lane
+    group size it serves (L = 2 ... 64 lanes per pixel; 128 spp is two passes of 64)."""
     xyzr, rgba = scenes.random_spheres(5000, seed=14, rmin=0.005, rmax=0.03)
     w, h = (37, 29) if spp >= 64 else (101, 67)
     cam = small_camera(w, h)
@@ -66,12 +67,11 @@ def test_sample_lanes_bit_identical(gpu_renderer, mode, spp):
     a, b = frames[1], frames[0]
     for k in ("hit_id", "t", "depth", "color", "rgba8", "aux"):
         np.testing.assert_array_equal(getattr(a, k).view(np.uint8), getattr(b, k).view(np.uint8), err_msg=k)
-    if spp <= 4:
-        o = O.render_brute(xyzr, rgba, cam, oracle_params(p), aux=True)
-        assert_frames_equal(a, o, aux=True, what=f"sample lanes spp {spp}")
+    o = O.Bvh(xyzr, 30).render(rgba, cam, oracle_params(p), aux=True)
+    assert_frames_equal(a, o, aux=True, what=f"sample lanes spp {spp}")
 
 
-@pytest.mark.parametrize("spp", [1, 3, 64])
+@pytest.mark.parametrize("spp", [1, 3, 64, 128])
 @pytest.mark.parametrize("mode", [L.MODE_LAMBERT, L.MODE_AO])
 def test_trees_with_and_without_leaf_pairs_render_the_same(gpu_renderer, mode, spp):
     """A tree built with leaf pairs (nodes over two leaves not stored, their visits read the two
@@ -97,9 +97,8 @@ def test_trees_with_and_without_leaf_pairs_render_the_same(gpu_renderer, mode, s
     a, b = frames[1], frames[0]
     for k in ("hit_id", "t", "depth", "color", "rgba8", "aux"):
         np.testing.assert_array_equal(getattr(a, k).view(np.uint8), getattr(b, k).view(np.uint8), err_msg=k)
-    if spp <= 3:
-        o = O.render_brute(xyzr, rgba, cam, oracle_params(p), aux=True)
-        assert_frames_equal(a, o, aux=True, what=f"leaf pairs spp {spp}")
+    o = O.Bvh(xyzr, 30).render(rgba, cam, oracle_params(p), aux=True)
+    assert_frames_equal(a, o, aux=True, what=f"leaf pairs spp {spp}")
 
 
 def test_window_is_a_crop_of_the_full_frame(gpu_renderer):

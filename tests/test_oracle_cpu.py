@@ -452,3 +452,141 @@ def test_leaf_pair_count_rule_matches_the_tree(n, dups, bits):
     assert rule == int(both.sum())
     for i in np.nonzero(both)[0]:
         assert ~child[i, 1] == ~child[i, 0] + 1
+
+
+# ------------------------------------------------------- Lambert and miss shading
+def _sample_rays(cam, W, H, x, y, spp, frame=0):
+    """The jittered primary rays of pixel (x, y): sample s takes draws 2s and 2s + 1 of the
+    pixel's stream tea16(y * W + x, frame) (raygen.cu:182,241-242)."""
+    import ctypes as C
+    lib = O.load()
+    fs = O.frame_state(cam)
+    draws = O.lcg_stream(O.tea16(y * W + x, frame), 2 * spp)
+    out = []
+    for s in range(spp):
+        o = np.zeros(3, np.float32)
+        d = np.zeros(3, np.float32)
+        lib.orc_primary_ray(C.byref(fs), W, H, x, y, C.c_float(draws[2 * s]), C.c_float(draws[2 * s + 1]),
+                            o.ctypes.data, d.ctypes.data)
+        out.append((o.astype(np.float64), d.astype(np.float64)))
+    return out
+
+
+def _perp(d):
+    e = np.cross(d, [0.0, 1.0, 0.0] if abs(d[1]) < 0.9 else [1.0, 0.0, 0.0])
+    return e / np.linalg.norm(e)
+
+
+@pytest.mark.parametrize("h", [0.0, 0.3, 0.6, 0.9])
+def test_lambert_shading_closed_form(h):
+    """raygen.cu:135-142 with maxBounces 0: a sample that hits a sphere of albedo A at lateral
+    offset h (in radii) from its centre line gets A * (0.2 + 0.6 |N.d|), |N.d| = sqrt(1 - h^2),
+    with N the face-forwarded unit normal; h = 0 is the head-on ray, A * 0.8. The normal kept in
+    the feature buffer faces the ray."""
+    W, H, x, y = 8, 6, 3, 2
+    cam = look_at((0.2, -0.1, 4.0), (0.0, 0.0, 0.0), (0, 1, 0), 30.0, W / H, 0.1, 50.0)
+    (o, d), = _sample_rays(cam, W, H, x, y, 1)
+    r, dist = 0.5, 3.0
+    e = _perp(d)
+    c = o + dist * d + h * r * e
+    xyzr = np.array([[*c, r]], np.float32)
+    alb = np.array([[0.25, 0.5, 1.0, 1.0]], np.float32)
+    o1 = O.render_brute(xyzr, alb, cam, O.make_params(W, H, want_aux=1), aux=True)
+    assert o1["hit_id"][y, x] == 0
+    cos = math.sqrt(1.0 - h * h)
+    want = alb[0, :3].astype(np.float64) * (0.2 + 0.6 * cos)
+    np.testing.assert_allclose(o1["color"][y, x, :3], want, rtol=2e-5, atol=0)
+    assert o1["color"][y, x, 3] == 1.0
+    # the normal written to the features: (P - c) / r with P on the near side of the sphere
+    cf = xyzr[0, :3].astype(np.float64)
+    t = (cf - o) @ d - math.sqrt(max(0.0, r * r - float(np.sum(((cf - o) - ((cf - o) @ d) * d) ** 2))))
+    n_want = (o + t * d - cf) / r
+    n_got = o1["aux"][y, x, 3:6].astype(np.float64)
+    np.testing.assert_allclose(n_got, n_want, atol=2e-3)  # fp16 storage
+    assert n_got @ d < 0.0
+    assert math.isclose(float(o1["t"][y, x]), t, rel_tol=1e-5)
+
+
+def test_partial_coverage_averages_hit_shading_with_miss_white():
+    """raygen.cu:131-133,250,265-268: a sample that misses every sphere contributes
+    col * 0.8 = (0.8, 0.8, 0.8, 1) to the pixel average; only a pixel that no sample hit takes
+    the background (and pseudo depth 1)."""
+    W, H, x, y, spp = 4, 4, 1, 2, 2
+    cam = look_at((0.0, 0.0, 5.0), (0.0, 0.0, 0.0), (0, 1, 0), 40.0, 1.0, 0.1, 50.0)
+    (o, d0), (_, d1) = _sample_rays(cam, W, H, x, y, spp)
+    dist = 4.0
+    ang = math.acos(min(1.0, float(d0 @ d1)))
+    r = 0.25 * dist * math.sin(ang)  # covers sample 0's ray, far from sample 1's
+    assert r > 1e-3
+    xyzr = np.array([[*(o + dist * d0), r]], np.float32)
+    alb = np.array([[0.9, 0.3, 0.6, 1.0]], np.float32)
+    bg = (0.1, 0.2, 0.3, 0.4)
+    f = O.render_brute(xyzr, alb, cam, O.make_params(W, H, spp=spp, background=bg))
+    assert f["hit_id"][y, x] == 0
+    assert O.intersect_f32(o, d1, xyzr[0]) is None
+    want = (alb[0, :3].astype(np.float64) * 0.8 + 0.8) / 2.0
+    np.testing.assert_allclose(f["color"][y, x, :3], want, rtol=2e-5, atol=0)
+    assert f["color"][y, x, 3] == 1.0
+    assert f["depth"][y, x] < 1.0
+    # every other pixel: no sample hits -> background, depth 1, no id, t = inf
+    miss = np.ones((H, W), bool)
+    miss[y, x] = False
+    assert (f["hit_id"][miss] == -1).all() and (f["depth"][miss] == 1.0).all()
+    np.testing.assert_array_equal(f["color"][miss], np.broadcast_to(np.array(bg, np.float32), (miss.sum(), 4)))
+    assert np.isinf(f["t"][miss]).all()
+
+
+def test_progressive_blend_is_the_running_mean():
+    """raygen.cu:254-260: blending frame k with weight 1 / (k + 1) into the previous result makes
+    the running mean of the frames (to FP32 rounding); a pixel that the newest frame misses takes
+    the background whatever was accumulated."""
+    g = np.random.default_rng(3)
+    P, K = 64, 12
+    frames = [g.random((P, 4)).astype(np.float32) for _ in range(K)]
+    ids = np.zeros(P, np.int32)
+    bg = (0.5, 0.25, 0.125, 1.0)
+    acc = None
+    for k, fr in enumerate(frames):
+        acc, rgba8 = O.blend_frames(acc, {"color": fr, "hit_id": ids}, k, bg)
+        np.testing.assert_allclose(acc, np.mean(frames[:k + 1], axis=0), rtol=0, atol=3e-6)
+    np.testing.assert_array_equal(rgba8, O.pack_rgba8(acc))
+    ids[5] = -1
+    acc2, _ = O.blend_frames(acc, {"color": frames[0], "hit_id": ids}, K, bg)
+    np.testing.assert_array_equal(acc2[5], np.array(bg, np.float32))
+    # RGBA8: clamp to [0, 1], then round half up per channel, R in the low byte
+    px = O.pack_rgba8(np.array([[0.0, 1.0, 0.5, 2.0], [-1.0, 0.00196, 0.00197, 1.0]], np.float32))
+    assert px[0] == (0 | 255 << 8 | 128 << 16 | 255 << 24)
+    assert px[1] == (0 | 0 << 8 | 1 << 16 | 255 << 24)
+
+
+# ------------------------------------------------------------ scale checks of the oracle itself
+@pytest.mark.parametrize("bits", [30, 63])
+def test_parallel_lbvh_build_is_the_serial_tree(bits):
+    """The multi-threaded oracle build (per-thread radix histograms, offsets in digit-then-thread
+    order, parallel Karras nodes, atomic second-arrival boxes) gives the serial build's arrays."""
+    g = np.random.default_rng(bits)
+    n = 300_000
+    xyzr = np.zeros((n, 4), np.float32)
+    xyzr[:, :3] = g.random((n, 3)).astype(np.float32)
+    xyzr[n // 3:2 * n // 3, :3] = xyzr[:n // 3, :3]  # duplicate keys
+    xyzr[:, 3] = g.uniform(0.001, 0.004, n).astype(np.float32)
+    a, b = O.Bvh(xyzr, bits, threads=1).arrays(), O.Bvh(xyzr, bits, threads=8).arrays()
+    for u, v in zip(a, b):
+        np.testing.assert_array_equal(u.view(np.uint8), v.view(np.uint8))
+
+
+def test_cube_frame_fp32_oracle_against_fp64_everywhere():
+    """The uniform-cube workload (1M spheres, r 0.002, 1024x1024, Lambert): the oracle's FP32 LBVH
+    render against the FP64 brute-force nearest hit at every pixel. Mismatches are grazing or
+    near-tie pixels only and at most 1e-5 of the image; t within 1e-5 relative elsewhere."""
+    from helpers import fp64_check, oracle_params
+    from megamol_amd import ParticleList, _lib as L, params
+    xyzr, rgba8 = scenes.uniform_cube(1_000_000)
+    lists = [ParticleList(L.VERTDATA_FLOAT_XYZR, xyzr, L.COLDATA_UINT8_RGBA, rgba8, count=len(xyzr))]
+    xo, ro = O.convert_lists(lists)
+    cam = scenes.uniform_cube_camera(1024, 1024)
+    p = params(1024, 1024)
+    o = O.Bvh(xo, 30).render(ro, cam, oracle_params(p))
+    rects = [(x, y, x + 16, y + 16) for y in range(0, 1024, 16) for x in range(0, 1024, 16)]
+    m, n = fp64_check(xo, cam, 1024, 1024, o["hit_id"], o["t"], rects, what="cube oracle")
+    assert n == 1024 * 1024 and m <= 1e-5 * n, m
