@@ -192,30 +192,48 @@ def make_scene(args):
 
 
 def cpu_baseline(scene, args, r):
-    """CPU oracle on a bounded sample: full LBVH build + a strip of rows of the same frame."""
+    """CPU oracle on a bounded sample of the same frame: the full LBVH build (multi-threaded) and
+    `cpu_rows` rows spread evenly over the image, each timed `cpu_repeats` times (median).
+
+    Threads: every CPU this process may run on (its affinity mask), which on the GPU hosts is the
+    16-CPU share of one GPU; the machine's total count is reported beside it."""
     import oracle as O
 
-    threads = min(os.cpu_count() or 1, 16)
-    from megamol_amd import _lib as L
+    threads = O.default_threads()
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = os.cpu_count() or 1
     xyzr, rgba = O.convert_lists(scene["lists"], tf=scene["tf"])
-    t0 = time.time()
-    b = O.Bvh(xyzr, args.bits)
-    build_s = time.time() - t0
-    rows = args.cpu_rows
-    y0 = args.height // 2 - rows // 2
+    builds = []
+    b = None
+    for _ in range(2):
+        b = None
+        b = O.Bvh(xyzr, args.bits, threads=threads)
+        builds.append(b.build_s)
+    build_s = float(np.median(builds))
+    nrows = max(1, min(args.cpu_rows, args.height))
+    rows = [int((k + 0.5) * args.height / nrows) for k in range(nrows)]
     p = O.make_params(args.width, args.height, args.spp, 1 if args.mode == "ao" else 0, args.ao_distance)
-    p.row_begin, p.row_end = y0, y0 + rows
-    o = b.render(rgba, scene["cam"], p, threads=threads, outputs=("hit_id",))
-    trace_s = o["seconds"]
-    sample_px = rows * args.width
+    runs = []
+    for _ in range(max(1, args.cpu_repeats)):
+        tot = 0.0
+        for y in rows:
+            p.row_begin, p.row_end = y, y + 1
+            tot += b.render(rgba, scene["cam"], p, threads=threads, outputs=("hit_id",))["seconds"]
+        runs.append(tot)
+    trace_s = float(np.median(runs))
+    sample_px = nrows * args.width
     frac = sample_px / (args.width * args.height)
     frame_ms = 1e3 * (build_s + trace_s / frac)
-    # rays in the sample: primary per sample + AO per hit sample (hit fraction from the GPU frame)
     return dict(value=round(frame_ms, 1), unit="ms/frame (projected from sample)", cores=threads,
                 kind="port",
-                sample=(f"oracle LBVH build over all {len(xyzr)} spheres ({build_s:.2f}s, 1 thread) + "
-                        f"{rows} rows x {args.width} px x {args.spp} spp traced ({trace_s:.2f}s, {threads} threads); "
-                        f"full frame projected as build + trace/{frac:.4f}"),
+                sample=(f"oracle LBVH build over all {len(xyzr)} spheres (median of {len(builds)}: {build_s:.2f}s, "
+                        f"{threads} threads) + {nrows} rows spread over the frame x {args.width} px x {args.spp} spp "
+                        f"(median of {len(runs)} runs: {trace_s:.2f}s, {threads} threads); full frame projected as "
+                        f"build + trace/{frac:.4f}; CPUs: {affinity} in this process's affinity mask, "
+                        f"{os.cpu_count()} on the machine"),
+                build_ms=round(1e3 * build_s, 1), trace_ms_projected=round(1e3 * trace_s / frac, 1),
                 trace_seconds=trace_s, build_seconds=build_s, sample_pixels=sample_px)
 
 
@@ -413,7 +431,8 @@ def main():
     ap.add_argument("--mode", choices=["ao", "lambert"], default="ao")
     ap.add_argument("--ao-distance", type=float, default=4.0)
     ap.add_argument("--bits", type=int, default=30)
-    ap.add_argument("--cpu-rows", type=int, default=8)
+    ap.add_argument("--cpu-rows", type=int, default=12, help="CPU baseline: rows spread evenly over the frame")
+    ap.add_argument("--cpu-repeats", type=int, default=5, help="CPU baseline: timed runs of the rows (median)")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--option", action="append", default=[], help="key=value library option")
     ap.add_argument("--ablation", action="store_true",
@@ -579,8 +598,9 @@ def main():
     if not args.skip_cpu:
         log("CPU baseline ...")
         cpu = cpu_baseline(scene, args, r)
-        cpu_mrays = (args.cpu_rows * args.width * (rays / P)) / cpu["trace_seconds"] / 1e6
+        cpu_mrays = (cpu["sample_pixels"] * (rays / P)) / cpu["trace_seconds"] / 1e6
         cpu["sample"] += f"; {cpu_mrays:.2f} Mrays/s"
+        cpu["mrays_per_s"] = round(cpu_mrays, 2)
         for k in ("trace_seconds", "build_seconds", "sample_pixels"):
             cpu.pop(k)
         log(f"CPU baseline: {cpu}")

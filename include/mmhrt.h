@@ -190,7 +190,10 @@ MMHRT_API int32_t mmhrt_set_spheres_device(mmhrt_ctx ctx, const void* xyzr_dev, 
 MMHRT_API int32_t mmhrt_set_transfer_function(mmhrt_ctx ctx, const float* rgba, uint32_t n, float vmin, float vmax);
 
 /* Build (or refit) the LBVH over the current spheres: replaces optixAccelBuild
- * (SphereGeometry.cpp:197-214). morton_bits is 30 or 63. */
+ * (SphereGeometry.cpp:197-214). morton_bits is 30 or 63. With option "accel" 1 it builds the
+ * treelet P-k-d trees instead (plugins/optix_owl/src/TreeletsRenderer.cpp:95-110: the Morton
+ * order cut into treelets of option "treelet" spheres, each a left-balanced k-d tree over its
+ * spheres, a tree of boxes on top), which have no refit: every policy rebuilds them. */
 MMHRT_API int32_t mmhrt_build(mmhrt_ctx ctx, int32_t policy, int32_t morton_bits);
 
 /* Render one frame: AbstractRenderer::Render + raygen_program (AbstractRenderer.cpp:27-111,
@@ -220,6 +223,17 @@ MMHRT_API int32_t mmhrt_get_info(mmhrt_ctx ctx, uint64_t* n_spheres, float* boun
  * ~k with pair flag bit 0 (child0) or bit 1 (child1) set, and its slot is left unwritten. Any
  * pointer may be NULL. */
 MMHRT_API int32_t mmhrt_get_bvh(mmhrt_ctx ctx, float* nodes, uint32_t* leaf_ids, uint64_t* keys);
+
+/* Copy the treelet P-k-d trees (built with option "accel" 1) to host for structural checks:
+ * sizes[3] = treelet size M, treelet count T, top-tree leaves L (a power of two >= max(T, 2));
+ * spheres (n*4 floats) and ids (n, particle index) in treelet heap order: node i of treelet t at
+ * position t*M + i, children 2i+1 and 2i+2 (plugins/optix_owl/src/PKDCreate.h:86-160 recBuild,
+ * TreeletsCreate.h:117-166 prePartition_inPlace); dims (n bytes, split axis of each node);
+ * treelet_boxes (T*6: sphere bounds), treelet_rmax (T: largest radius); top_nodes ((L-1)*16
+ * floats in the layout of mmhrt_get_bvh: child ref >= 0 a top node, ~t treelet t, INT32_MIN an
+ * absent child whose box holds 3e38 everywhere). Any pointer may be NULL. */
+MMHRT_API int32_t mmhrt_get_treelets(mmhrt_ctx ctx, uint32_t* sizes, float* spheres, uint32_t* ids, uint8_t* dims,
+                                     float* treelet_boxes, float* treelet_rmax, float* top_nodes);
 
 /* Per-kernel timing of the last mmhrt_build / mmhrt_render on the context's stream, measured
  * with HIP events: times_ms[0..8] = morton, sort, hierarchy, bounds, trace, resolve, total build,
@@ -299,6 +313,9 @@ MMHRT_API int32_t mmhrt_denoiser_train_step(mmhrt_ctx ctx, const uint16_t* feats
  * merged by one workgroup per span of this many chunks before the tail, default 256; 0: kernel
  * rounds), "build_wave" (chunk-pass rounds of at most this
  * many merges, up to 64, run on one wavefront without workgroup barriers, default 64; 0: never),
+ * "accel" (structure the next build makes: 0 LBVH, the default; 1 treelet P-k-d trees, about one
+ * byte per sphere beyond the spheres instead of up to 64), "treelet" (spheres per treelet of that
+ * structure: 256, 512 or 1024, default 1024),
  * "sample_lanes" (samples of a pixel on the lanes for power-of-two spp,
  * default 1), "dn_k32" (denoiser forward on the 16x16x32 MFMA kernels, default 1),
  * "dn_conv_shape" (test hook: forced launch shape of those kernels, 0 = automatic).

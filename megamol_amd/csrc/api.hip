@@ -331,7 +331,14 @@ int32_t mmhrt_build(mmhrt_ctx c, int32_t policy, int32_t morton_bits) {
     if (morton_bits != 30 && morton_bits != 63) return fail(c, MMHRT_E_INVALID, "build: morton_bits must be 30 or 63");
     if (policy < 0 || policy > 2) return fail(c, MMHRT_E_INVALID, "build: unknown policy");
     if (c->n == 0) return fail(c, MMHRT_E_STATE, "build: no spheres set");
-    bool can_refit = c->built && c->built_n == c->n && c->bits == morton_bits;
+    if (c->opt_accel == 1) {
+        // treelet P-k-d trees keep no topology that moved spheres could refit: every policy builds
+        // (the reference's treelets renderer rebuilds on every data change, TreeletsRenderer.cpp:95-110)
+        c->refits_since_build = 0;
+        return build_pkd(c, morton_bits);
+    }
+    if (c->opt_accel != 0) return fail(c, MMHRT_E_INVALID, "build: option accel must be 0 (LBVH) or 1 (treelets)");
+    bool can_refit = c->built && c->accel == 0 && c->built_n == c->n && c->bits == morton_bits;
     if (policy == MMHRT_BUILD_REFIT) {
         if (!can_refit) return fail(c, MMHRT_E_STATE, "build: refit needs an LBVH over the same sphere count");
         return refit_lbvh(c);
@@ -410,7 +417,8 @@ int32_t mmhrt_sync(mmhrt_ctx c) {
 int32_t mmhrt_get_info(mmhrt_ctx c, uint64_t* n_spheres, float* bounds6, uint64_t* n_nodes) {
     if (int e = check_ctx(c)) return e;
     if (n_spheres) *n_spheres = c->n;
-    if (n_nodes) *n_nodes = c->built ? (c->built_n > 1 ? c->built_n - 1 : 1) : 0;
+    if (n_nodes)
+        *n_nodes = !c->built ? 0 : c->accel == 1 ? (uint64_t)c->pkd_L - 1 : (c->built_n > 1 ? c->built_n - 1 : 1);
     if (bounds6) {
         if (c->n == 0) return fail(c, MMHRT_E_STATE, "get_info: no spheres set");
         if (!c->info_valid) {
@@ -431,6 +439,7 @@ int32_t mmhrt_get_info(mmhrt_ctx c, uint64_t* n_spheres, float* bounds6, uint64_
 int32_t mmhrt_get_bvh(mmhrt_ctx c, float* nodes, uint32_t* leaf_ids, uint64_t* keys) {
     if (int e = check_ctx(c)) return e;
     if (!c->built) return fail(c, MMHRT_E_STATE, "get_bvh: nothing built");
+    if (c->accel != 0) return fail(c, MMHRT_E_STATE, "get_bvh: the current structure is not an LBVH");
     uint64_t n = c->built_n, m = n > 1 ? n - 1 : 1;
     hipError_t e = hipStreamSynchronize(c->stream);
     if (e == hipSuccess && nodes) e = hipMemcpy(nodes, c->nodes.p, m * sizeof(Node), hipMemcpyDeviceToHost);
@@ -446,6 +455,28 @@ int32_t mmhrt_get_bvh(mmhrt_ctx c, float* nodes, uint32_t* leaf_ids, uint64_t* k
         }
     }
     if (e != hipSuccess) return hip_fail(c, e, "get_bvh");
+    return MMHRT_OK;
+}
+
+int32_t mmhrt_get_treelets(mmhrt_ctx c, uint32_t* sizes, float* spheres, uint32_t* ids, uint8_t* dims,
+                           float* treelet_boxes, float* treelet_rmax, float* top_nodes) {
+    if (int e = check_ctx(c)) return e;
+    if (!c->built || c->accel != 1) return fail(c, MMHRT_E_STATE, "get_treelets: no treelet P-k-d trees built");
+    const uint64_t n = c->built_n;
+    const uint32_t T = c->pkd_T, L = c->pkd_L;
+    if (sizes) {
+        sizes[0] = c->pkd_M;
+        sizes[1] = T;
+        sizes[2] = L;
+    }
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess && spheres) e = hipMemcpy(spheres, c->leaf_sph.p, n * 16, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && ids) e = hipMemcpy(ids, c->leaf_id.p, n * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && dims) e = hipMemcpy(dims, c->pkd_dim.p, n, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && treelet_boxes) e = hipMemcpy(treelet_boxes, c->tl_box.p, (size_t)T * 24, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && treelet_rmax) e = hipMemcpy(treelet_rmax, c->tl_rmax.p, (size_t)T * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && top_nodes) e = hipMemcpy(top_nodes, c->nodes.p, (size_t)(L - 1) * sizeof(Node), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(c, e, "get_treelets");
     return MMHRT_OK;
 }
 
@@ -504,6 +535,8 @@ static int64_t* option_slot(mmhrt_ctx c, const char* key) {
     if (k == "build_wave") return &c->opt_build_wave;
     if (k == "sort_xcd") return &c->opt_sort_xcd;
     if (k == "build_span") return &c->opt_build_span;
+    if (k == "accel") return &c->opt_accel;
+    if (k == "treelet") return &c->opt_treelet;
     if (k == "stack_overflow") return &c->stack_overflow;
     if (k == "pair_visits") return &c->pair_visits;
     if (k == "leaf_fetches") return &c->leaf_fetches;

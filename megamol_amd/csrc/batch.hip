@@ -151,7 +151,9 @@ int32_t mmhrt_render_batch(mmhrt_ctx c, const mmhrt_job* jobs, uint32_t n_jobs, 
     // every buffer the per-timestep build and trace touch is allocated before any capture
     c->sph_cur = c->ts_sph.p;
     c->n = c->ts_n;
-    if ((e = build_lbvh(c, morton_bits))) return e;
+    // treelet P-k-d trees (option accel 1) are rebuilt for every timestep: they have no refit
+    const bool tl = c->opt_accel == 1;
+    if ((e = tl ? build_pkd(c, morton_bits) : build_lbvh(c, morton_bits))) return e;
     {
         unsigned grid = (unsigned)std::max<int64_t>(1, c->cu_count * c->opt_blocks_per_cu);
         if ((e = ensure(c, c->spill, (size_t)grid * 256 * 120, "traversal spill"))) return e;
@@ -185,11 +187,12 @@ int32_t mmhrt_render_batch(mmhrt_ctx c, const mmhrt_job* jobs, uint32_t n_jobs, 
     auto issue = [&]() -> int {
         for (uint32_t t = 0; t < c->ts_count; ++t) {
             c->sph_cur = c->ts_sph.p + (size_t)t * c->ts_n;
-            int r = (t % (uint32_t)refit_period == 0) ? build_lbvh(c, morton_bits) : refit_lbvh(c);
+            int r = tl ? build_pkd(c, morton_bits)
+                       : (t % (uint32_t)refit_period == 0) ? build_lbvh(c, morton_bits) : refit_lbvh(c);
             if (r) return r;
             if (count_job[t] == 0) continue;
             launch_trace_batch(c, c->batch_jobs.p + js * first_job[t], c->batch_prefix.p + first_prefix[t],
-                               count_job[t], c->batch_ctr.p + 16 * t, jobs[by_t[t][0]].params.mode);
+                               count_job[t], c->batch_ctr.p + 16 * t, jobs[by_t[t][0]].params.mode, tl);
         }
         return 0;
     };

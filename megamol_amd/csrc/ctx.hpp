@@ -95,6 +95,15 @@ struct mmhrt_ctx_s {
     mmhrt::DevBuf<uint32_t> box_cnt;  // worklist sizes of the bottom-up box rounds
     mmhrt::DevBuf<float> partial;
     mmhrt::DevBuf<mmhrt::SceneInfo> info;
+
+    // treelet P-k-d trees (accel 1): leaf_sph / leaf_id hold the spheres in treelet heap order,
+    // nodes the top tree of boxes over the treelets
+    int accel = 0;                    // structure of the current build: 0 LBVH, 1 treelet P-k-d trees
+    uint32_t pkd_M = 0, pkd_T = 0;    // treelet size (power of two) and count of the current build
+    uint32_t pkd_L = 0;               // leaves of the top tree (a power of two >= max(T, 2))
+    mmhrt::DevBuf<uint8_t> pkd_dim;   // split axis of every node
+    mmhrt::DevBuf<float> tl_box;      // T x 6: sphere bounds of each treelet
+    mmhrt::DevBuf<float> tl_rmax;     // T: largest radius of each treelet
     mmhrt::SceneInfo info_host{};
     bool info_valid = false;
 
@@ -131,6 +140,8 @@ struct mmhrt_ctx_s {
     int64_t opt_build_wave = 64;    // chunk pass: rounds of at most this many (<= 64) merges run on one wavefront (0: none)
     int64_t opt_sort_xcd = 1;       // radix scatter: each XCD takes one contiguous range of tiles (0: round robin)
     int64_t opt_build_span = 256;   // straddlers: one workgroup per span of this many chunks, then the tail (0: rounds)
+    int64_t opt_accel = 0;          // structure the next build makes: 0 LBVH, 1 treelet P-k-d trees
+    int64_t opt_treelet = 1024;     // spheres per treelet of the P-k-d structure: 256, 512 or 1024
     int64_t refits_since_build = 0;
     int64_t stack_overflow = 0;  // set by mmhrt_get_counters from the trace kernel's flag
     int64_t pair_visits = 0;     // set by mmhrt_get_counters: visits of leaf pairs (of the node visits)
@@ -219,6 +230,11 @@ int refit_lbvh(mmhrt_ctx_s* c);
 int reduce_scene(mmhrt_ctx_s* c);
 int count_leaf_pairs(mmhrt_ctx_s* c, int64_t* out);
 
+int sort_spheres(mmhrt_ctx_s* c, int bits);
+
+// pkd.hip
+int build_pkd(mmhrt_ctx_s* c, int bits);
+
 // trace.hip
 constexpr int TRACE_STACK_CAP = 128;                       // traversal stack entries per lane
 constexpr int TRACE_SPILL_PER_LANE = TRACE_STACK_CAP - 8;  // the most any LDS stack depth (>= 8) spills
@@ -238,7 +254,7 @@ uint32_t tile_log2(int64_t tw);
 size_t batch_job_size();
 int batch_fill_job(mmhrt_ctx_s* c, const FrameConsts& fc, size_t out_offset, void* dst);
 void launch_trace_batch(mmhrt_ctx_s* c, const void* jobs, const uint32_t* prefix, uint32_t n_jobs, uint32_t* ctr,
-                        int mode);
+                        int mode, bool treelets);
 
 // event record on the context stream, skipped while the stream is being captured into a graph
 inline void rec(mmhrt_ctx_s* c, int i) {

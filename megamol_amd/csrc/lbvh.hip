@@ -1613,6 +1613,30 @@ int reduce_scene(mmhrt_ctx_s* c) {
     return 0;
 }
 
+// The first half of a build, shared by the treelet P-k-d build: scene bounds, Morton codes and
+// the stable radix sort; leaf_id receives the particle of each sorted position, ka / kla the keys.
+int sort_spheres(mmhrt_ctx_s* c, int bits) {
+    uint64_t n = c->n;
+    if (n == 0) return fail(c, MMHRT_E_STATE, "build: no spheres set");
+    if (n > 0x7fffffffull) return fail(c, MMHRT_E_UNSUPPORTED, "build: more than 2^31-1 spheres");
+    c->bits = bits;
+    int e = 0;
+    if ((e = ensure(c, c->va, n, "sort values"))) return e;
+    if ((e = ensure(c, c->vb, n, "sort values"))) return e;
+    if ((e = ensure(c, c->leaf_id, n, "leaf ids"))) return e;
+    if (bits == 63) {
+        if ((e = ensure(c, c->kla, n, "keys"))) return e;
+        if ((e = ensure(c, c->klb, n, "keys"))) return e;
+    } else {
+        if ((e = ensure(c, c->ka, n, "keys"))) return e;
+        if ((e = ensure(c, c->kb, n, "keys"))) return e;
+    }
+    rec(c, 0);
+    if ((e = reduce_scene(c))) return e;
+    if (bits == 63) return morton_sort<uint64_t>(c, c->kla.p, c->klb.p, c->va.p, c->vb.p, c->leaf_id.p, n, 8);
+    return morton_sort<uint32_t>(c, c->ka.p, c->kb.p, c->va.p, c->vb.p, c->leaf_id.p, n, 4);
+}
+
 int build_lbvh(mmhrt_ctx_s* c, int bits) {
     uint64_t n = c->n;
     if (n == 0) return fail(c, MMHRT_E_STATE, "build: no spheres set");
@@ -1644,12 +1668,14 @@ int build_lbvh(mmhrt_ctx_s* c, int bits) {
     if (he != hipSuccess) return hip_fail(c, he, "LBVH build launch");
     c->built = true;
     c->built_n = n;
+    c->accel = 0;
     c->info_valid = false;
     return 0;
 }
 
 int refit_lbvh(mmhrt_ctx_s* c) {
-    if (!c->built || c->built_n != c->n) return fail(c, MMHRT_E_STATE, "refit: no matching LBVH to refit");
+    if (!c->built || c->built_n != c->n || c->accel != 0)
+        return fail(c, MMHRT_E_STATE, "refit: no matching LBVH to refit");
     rec(c, 0);
     if (int e = reduce_scene(c)) return e;
     rec(c, 1);

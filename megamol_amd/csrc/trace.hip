@@ -45,6 +45,11 @@ struct TraceArgs {
     uint32_t* out_rgba8;
     __half* out_aux;
     FrameConsts fc;
+    // treelet P-k-d trees (nodes then hold the top tree over the treelets)
+    const uint8_t* pkd_dim;  // split axis of the node at each position
+    const float* tl_rmax;    // largest radius of each treelet
+    uint32_t n_sph;          // spheres
+    uint32_t tl_log2;        // log2 of the treelet size
 };
 
 template <int S>
@@ -266,6 +271,167 @@ __device__ bool trace_any(const TraceArgs& A, const RayPre& r, float tmin, float
     return res == 2;
 }
 
+// Slab test of one child box returning the entry distance (+inf when missed) and the exit.
+__device__ __forceinline__ float slab_io(const RayPre& r, float lx, float ly, float lz, float hx, float hy, float hz,
+                                         float tmin, float tmax, float& tf_out) {
+    float x0 = __builtin_fmaf(lx, r.inv.x, -r.oil.x), x1 = __builtin_fmaf(hx, r.inv.x, -r.oih.x);
+    float y0 = __builtin_fmaf(ly, r.inv.y, -r.oil.y), y1 = __builtin_fmaf(hy, r.inv.y, -r.oih.y);
+    float z0 = __builtin_fmaf(lz, r.inv.z, -r.oil.z), z1 = __builtin_fmaf(hz, r.inv.z, -r.oih.z);
+    float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
+    float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax));
+    tf_out = tf;
+    return tn <= tf ? tn : INFINITY;
+}
+
+// Treelet P-k-d traversal (the interval logic of pkd_intersect, plugins/optix_owl/cuda_resources/
+// pkdPrograms.cu:45-159, under a top tree of boxes like the LBVH's). A reference is a top node
+// (>= 0) or the P-k-d node at position p (~p): the node's sphere is tested when the ray's
+// interval [t0, t1] meets the node's slab (split coordinate -+ the treelet's largest radius,
+// widened by the padding every slab test here carries); the near child takes
+// [t0, min(slab exit, t1, best)], the far child [max(t0, slab entry), min(t1, best)]. Closed
+// intervals, so that equal-t hits still reach the lowest particle id. A pushed P-k-d entry keeps
+// its entry distance (two stack words: t0 below the reference) and resumes with t1 = the best
+// hit at the time it is popped. A treelet is entered at its root over its box's [entry, exit].
+// Closest hit (ANY false): returns the position of the hit (or -1) and its t in t_best. ANY:
+// returns 1 at the first hit of a sphere other than position `skip`, else 0.
+template <int S, bool COUNT, bool ANY>
+__device__ int trace_tl(const TraceArgs& A, const RayPre& r, float tmin, float tmax, int skip, float& t_best,
+                        LaneStack<S>& st, Count& cnt) {
+    int best = -1;
+    t_best = tmax;
+    st.sp = 0;
+    int32_t node = 0;
+    float t0 = 0.0f, t1 = 0.0f;
+    uint32_t cur_tl = 0xffffffffu, m = 0;
+    float rmax = 0.0f;
+    const uint32_t lg = A.tl_log2;
+    const uint32_t msk = (1u << lg) - 1u;
+    for (;;) {
+        bool pop = false;
+        if (node >= 0) {
+            const float4* q = reinterpret_cast<const float4*>(A.nodes + node);
+            const float4 a = q[0], b = q[1], c = q[2];
+            const int4 d = reinterpret_cast<const int4*>(q)[3];
+            if (COUNT) cnt.visits++;
+            float tf0, tf1;
+            const float tn0 = slab_io(r, a.x, a.y, a.z, a.w, b.x, b.y, tmin, t_best, tf0);
+            const float tn1 = slab_io(r, b.z, b.w, c.x, c.y, c.z, c.w, tmin, t_best, tf1);
+            const bool h0 = tn0 != INFINITY && d.x != INT32_MIN, h1 = tn1 != INFINITY && d.y != INT32_MIN;
+            // a treelet child ~t becomes the P-k-d root at position t << lg
+            const int32_t r0 = d.x >= 0 ? d.x : (int32_t)~((uint32_t)(~d.x) << lg);
+            const int32_t r1 = d.y >= 0 ? d.y : (int32_t)~((uint32_t)(~d.y) << lg);
+            if (h0 && h1) {
+                const bool first0 = tn0 <= tn1;
+                const int32_t rn = first0 ? r0 : r1, rf = first0 ? r1 : r0;
+                const float tnf = first0 ? tn1 : tn0;
+                if (rf < 0) st.push(__float_as_int(tnf));
+                st.push(rf);
+                node = rn;
+                t0 = first0 ? tn0 : tn1;
+                t1 = first0 ? tf0 : tf1;
+            } else if (h0) {
+                node = r0;
+                t0 = tn0;
+                t1 = tf0;
+            } else if (h1) {
+                node = r1;
+                t0 = tn1;
+                t1 = tf1;
+            } else {
+                pop = true;
+            }
+        } else {
+            const uint32_t p = ~(uint32_t)node;
+            const uint32_t tl = p >> lg;
+            if (tl != cur_tl) {
+                cur_tl = tl;
+                rmax = A.tl_rmax[tl];
+                const uint32_t tb = tl << lg;
+                m = A.n_sph - tb < msk + 1u ? A.n_sph - tb : msk + 1u;
+            }
+            const uint32_t i = p & msk, bs = p - i;
+            const float4 sp = A.leaf_sph[p];
+            const int dd = A.pkd_dim[p];
+            if (COUNT) cnt.visits++;
+            const float sc = dd == 0 ? sp.x : (dd == 1 ? sp.y : sp.z);
+            const float inv = dd == 0 ? r.inv.x : (dd == 1 ? r.inv.y : r.inv.z);
+            const float oil = dd == 0 ? r.oil.x : (dd == 1 ? r.oil.y : r.oil.z);
+            const float oih = dd == 0 ? r.oih.x : (dd == 1 ? r.oih.y : r.oih.z);
+            const float tlo = __builtin_fmaf(sc - rmax, inv, -oil), thi = __builtin_fmaf(sc + rmax, inv, -oih);
+            const float tnr = fminf(tlo, thi), tfr = fmaxf(tlo, thi);
+            float lim = fminf(t1, t_best);
+            if (fmaxf(t0, tnr) <= fminf(tfr, lim) && (!ANY || (int)p != skip)) {
+                if (COUNT) cnt.tests++;
+                const float t = sphere_t(r.o, r.d, sp, tmin, tmax);
+                if (t >= 0.0f) {
+                    if (ANY) return 1;
+                    if (t < t_best) {
+                        t_best = t;
+                        best = (int)p;
+                    } else if (t == t_best && best >= 0 && A.leaf_id[p] < A.leaf_id[best]) {
+                        best = (int)p;
+                    }
+                    lim = fminf(t1, t_best);
+                }
+            }
+            const float n_t1 = fminf(tfr, lim), f_t0 = fmaxf(t0, tnr);
+            const uint32_t sg = inv < 0.0f ? 1u : 0u;
+            const uint32_t nc = 2 * i + 1 + sg, fc = 2 * i + 2 - sg;
+            const bool need_n = nc < m && t0 <= n_t1;
+            const bool need_f = fc < m && f_t0 <= lim;
+            if (need_n && need_f) {
+                st.push(__float_as_int(f_t0));
+                st.push(~(int32_t)(bs + fc));
+                node = ~(int32_t)(bs + nc);
+                t1 = n_t1;
+            } else if (need_n) {
+                node = ~(int32_t)(bs + nc);
+                t1 = n_t1;
+            } else if (need_f) {
+                node = ~(int32_t)(bs + fc);
+                t0 = f_t0;
+                t1 = lim;
+            } else {
+                pop = true;
+            }
+        }
+        if (pop) {
+            for (;;) {
+                if (st.sp == 0) return ANY ? 0 : best;
+                const int32_t v = st.pop();
+                if (v >= 0) {
+                    node = v;
+                    break;
+                }
+                const float a0 = __int_as_float(st.pop());
+                if (a0 <= t_best) {
+                    node = v;
+                    t0 = a0;
+                    t1 = t_best;
+                    break;
+                }
+            }
+        }
+    }
+}
+
+// The structure a kernel traverses: TL = treelet P-k-d trees, else the LBVH.
+template <int S, bool COUNT, bool PAIRS, bool TL>
+__device__ __forceinline__ int trace_closest_x(const TraceArgs& A, const RayPre& r, float tmin, float tmax,
+                                               float& t_best, LaneStack<S>& st, Count& cnt) {
+    if (TL) return trace_tl<S, COUNT, false>(A, r, tmin, tmax, -1, t_best, st, cnt);
+    return trace_closest<S, COUNT, PAIRS>(A, r, tmin, tmax, t_best, st, cnt);
+}
+template <int S, bool COUNT, bool PAIRS, bool TL>
+__device__ __forceinline__ bool trace_any_x(const TraceArgs& A, const RayPre& r, float tmin, float tmax, int skip,
+                                            LaneStack<S>& st, Count& cnt) {
+    if (TL) {
+        float tb;
+        return trace_tl<S, COUNT, true>(A, r, tmin, tmax, skip, tb, st, cnt) != 0;
+    }
+    return trace_any<S, COUNT, PAIRS>(A, r, tmin, tmax, skip, st, cnt);
+}
+
 // Diffuse bounce direction of raygen.cu:71-77,149-152: rejection sample in the unit cube,
 // the three draws taken x, y, z in order.
 __device__ __forceinline__ V3 random_in_unit_sphere(uint32_t& seed) {
@@ -338,7 +504,7 @@ __device__ __forceinline__ void write_aux_tail(const TraceArgs& A, uint32_t opix
         *reinterpret_cast<uint32_t*>(A.out_aux + (size_t)opix * 8 + 6) = half2_bits(t0, vis_sum / (float)A.fc.spp);
 }
 
-template <int S, int MODE, bool COUNT>
+template <int S, int MODE, bool COUNT, bool TL = false>
 __device__ void shade_pixel(const TraceArgs& A, uint32_t px, uint32_t py, float pad, LaneStack<S>& st, Count& cnt) {
     const FrameConsts& f = A.fc;
     uint32_t pix = py * f.width + px;  // full-frame index: the random stream of the pixel
@@ -358,7 +524,7 @@ __device__ void shade_pixel(const TraceArgs& A, uint32_t px, uint32_t py, float 
         RayPre r = prep(o, d, pad);
         float t;
         if (COUNT) cnt.rays++;
-        int leaf = trace_closest<S, COUNT>(A, r, 1e-6f, 1e20f, t, st, cnt);
+        int leaf = trace_closest_x<S, COUNT, true, TL>(A, r, 1e-6f, 1e20f, t, st, cnt);
         float4 sc;
         float ray_depth = FLT_MAX;
         if (leaf < 0) {
@@ -380,7 +546,7 @@ __device__ void shade_pixel(const TraceArgs& A, uint32_t px, uint32_t py, float 
                 V3 ad = ao_direction(N, aseed, s, f.spp);
                 RayPre ra = prep(P, ad, pad);
                 if (COUNT) cnt.rays++;
-                bool occ = trace_any<S, COUNT>(A, ra, 0.0f, f.ao_distance, leaf, st, cnt);
+                bool occ = trace_any_x<S, COUNT, true, TL>(A, ra, 0.0f, f.ao_distance, leaf, st, cnt);
                 float V = occ ? 0.0f : 1.0f;
                 sc = make_float4(alb.x * V, alb.y * V, alb.z * V, 1.0f);
                 vis_sum += V;
@@ -402,7 +568,7 @@ __device__ void shade_pixel(const TraceArgs& A, uint32_t px, uint32_t py, float 
                     RayPre rb = prep(so, sd, pad);
                     float tb;
                     if (COUNT) cnt.rays++;
-                    int lb = trace_closest<S, COUNT>(A, rb, 1e-3f, 1e8f, tb, st, cnt);
+                    int lb = trace_closest_x<S, COUNT, true, TL>(A, rb, 1e-3f, 1e8f, tb, st, cnt);
                     if (lb < 0) {
                         res = make_float4(cc.x * 0.8f, cc.y * 0.8f, cc.z * 0.8f, 1.0f);
                         break;
@@ -444,7 +610,7 @@ __device__ void shade_pixel(const TraceArgs& A, uint32_t px, uint32_t py, float 
 
 // Lane per pixel, samples in a loop (every sample count). Held to 72 VGPRs (7 wavefronts per SIMD
 // instead of 4): 64-spp AO 64.3 -> 57.7 ms, 1-spp AO 1.22 -> 1.14 ms.
-template <int S, int MODE, bool COUNT, bool STEAL>
+template <int S, int MODE, bool COUNT, bool STEAL, bool TL = false>
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_trace(TraceArgs A) {
     __shared__ int32_t lds_stack[S * TB];
     const uint32_t gtid = blockIdx.x * TB + threadIdx.x;
@@ -471,7 +637,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(7, 8))) void
         const uint32_t tw = 1u << A.tw_log2, th = 64u >> A.tw_log2;
         uint32_t px = A.fc.win_x0 + tx * tw + (lane & (tw - 1)), py = A.fc.win_y0 + ty * th + (lane >> A.tw_log2);
         if (px < A.fc.win_x0 + A.fc.win_w && py < A.fc.win_y0 + A.fc.win_h)
-            shade_pixel<S, MODE, COUNT>(A, px, py, pad, st, cnt);
+            shade_pixel<S, MODE, COUNT, TL>(A, px, py, pad, st, cnt);
     }
     if (COUNT) {
         atomicAdd(reinterpret_cast<unsigned long long*>(A.ctr + 2), (unsigned long long)cnt.visits);
@@ -485,7 +651,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(7, 8))) void
 // One sample of a Lambert (MODE 0) or AO (MODE 1) pixel, the arithmetic of shade_pixel: the
 // sample colour, its depth contribution, the hit (t, leaf), the visibility and, for the
 // features, albedo and normal.
-template <int S, int MODE, bool COUNT, bool PAIRS>
+template <int S, int MODE, bool COUNT, bool PAIRS, bool TL = false>
 __device__ __forceinline__ void shade_sample(const TraceArgs& A, uint32_t px, uint32_t py, uint32_t pix, uint32_t s,
                                              float jx, float jy, float pad, LaneStack<S>& st, Count& cnt,
                                              float4& sc, float& ray_depth, float& t, int& leaf, float& V,
@@ -495,7 +661,7 @@ __device__ __forceinline__ void shade_sample(const TraceArgs& A, uint32_t px, ui
     primary_ray(f, px, py, jx, jy, o, d);
     RayPre r = prep(o, d, pad);
     if (COUNT) cnt.rays++;
-    leaf = trace_closest<S, COUNT, PAIRS>(A, r, 1e-6f, 1e20f, t, st, cnt);
+    leaf = trace_closest_x<S, COUNT, PAIRS, TL>(A, r, 1e-6f, 1e20f, t, st, cnt);
     ray_depth = FLT_MAX;
     V = 0.0f;
     if (leaf < 0) {
@@ -520,7 +686,7 @@ __device__ __forceinline__ void shade_sample(const TraceArgs& A, uint32_t px, ui
         V3 ad = ao_direction(N, aseed, s, f.spp);
         RayPre ra = prep(P, ad, pad);
         if (COUNT) cnt.rays++;
-        bool occ = trace_any<S, COUNT, PAIRS>(A, ra, 0.0f, f.ao_distance, leaf, st, cnt);
+        bool occ = trace_any_x<S, COUNT, PAIRS, TL>(A, ra, 0.0f, f.ao_distance, leaf, st, cnt);
         V = occ ? 0.0f : 1.0f;
         sc = make_float4(alb.x * V, alb.y * V, alb.z * V, 1.0f);
     } else {
@@ -539,7 +705,7 @@ __device__ __forceinline__ void shade_sample(const TraceArgs& A, uint32_t px, ui
 // to the lane-per-pixel kernels. Held to 72 VGPRs (7 wavefronts per SIMD instead of 4, a few
 // registers spilled): 64-spp AO 33.8 -> 30.6 ms, Lambert 22.9 -> 20.8 ms (5 wavefronts: 32.6 and
 // 21.2 ms, 6: 31.1 and 21.3, 8: 30.8 and 21.0).
-template <int S, int MODE, bool COUNT, bool PAIRS>
+template <int S, int MODE, bool COUNT, bool PAIRS, bool TL = false>
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_trace_sl(TraceArgs A) {
     __shared__ int32_t lds_stack[S * TB];
     __shared__ float4 lds_col[TB];  // the wavefront's sample colours of one round
@@ -584,8 +750,8 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(7, 8))) void
                     uint32_t seed = lcg_skip(seed0, 2u * s);
                     float jx = rnd(seed);
                     float jy = rnd(seed);
-                    shade_sample<S, MODE, COUNT, PAIRS>(A, px, py, pix, s, jx, jy, pad, st, cnt, sc, ray_depth, t,
-                                                        leaf, V, alb, N);
+                    shade_sample<S, MODE, COUNT, PAIRS, TL>(A, px, py, pix, s, jx, jy, pad, st, cnt, sc, ray_depth,
+                                                            t, leaf, V, alb, N);
                 }
                 // visibility is 0 or 1 per sample, so its sum is exact in any order: the count of
                 // the group's unoccluded hits
@@ -885,6 +1051,24 @@ void launch_sl(const TraceArgs& A, unsigned grid, bool count, int mode, bool pai
         launch_sl_p<S, false>(A, grid, count, mode, s);
 }
 
+// Kernels over the treelet P-k-d trees.
+template <int MODE, bool COUNT>
+void launch_tl_m(const TraceArgs& A, unsigned grid, bool sl, hipStream_t s) {
+    if (sl && MODE != 2)
+        hipLaunchKernelGGL((k_trace_sl<16, MODE == 2 ? 0 : MODE, COUNT, false, true>), dim3(grid), dim3(TB), 0, s, A);
+    else
+        hipLaunchKernelGGL((k_trace<16, MODE, COUNT, true, true>), dim3(grid), dim3(TB), 0, s, A);
+}
+void launch_tl(const TraceArgs& A, unsigned grid, bool sl, bool count, int mode, hipStream_t s) {
+    if (mode == 1) {
+        if (count) launch_tl_m<1, true>(A, grid, sl, s); else launch_tl_m<1, false>(A, grid, sl, s);
+    } else if (mode == 2) {
+        if (count) launch_tl_m<2, true>(A, grid, false, s); else launch_tl_m<2, false>(A, grid, false, s);
+    } else {
+        if (count) launch_tl_m<0, true>(A, grid, sl, s); else launch_tl_m<0, false>(A, grid, sl, s);
+    }
+}
+
 template <int S>
 void launch_refill(const TraceArgs& A, unsigned grid, uint32_t refill, bool count, int mode, hipStream_t s) {
     if (mode == 1) {
@@ -905,7 +1089,7 @@ void launch_refill(const TraceArgs& A, unsigned grid, uint32_t refill, bool coun
 // wavefront steals tiles and shades them with its job's constants and output buffers.
 // Held to 72 VGPRs (7 wavefronts per SIMD instead of 4) like the single-frame kernels: the batched
 // series (python bench.py --config series) went from 149.0 to 135.7 ms per launch.
-template <int S, int MODE>
+template <int S, int MODE, bool TL = false>
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_trace_batch(const TraceArgs* __restrict__ jobs,
                                                     const uint32_t* __restrict__ prefix, uint32_t n_jobs,
                                                     uint32_t* ctr) {
@@ -930,7 +1114,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(7, 8))) void
         const uint32_t tx = lt % A.tiles_x, ty = lt / A.tiles_x;
         const uint32_t px = A.fc.win_x0 + tx * tw + (lane & (tw - 1)), py = A.fc.win_y0 + ty * th + (lane >> A.tw_log2);
         if (px < A.fc.win_x0 + A.fc.win_w && py < A.fc.win_y0 + A.fc.win_h)
-            shade_pixel<S, MODE, false>(A, px, py, pad, st, cnt);
+            shade_pixel<S, MODE, false, TL>(A, px, py, pad, st, cnt);
     }
 }
 
@@ -938,11 +1122,18 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(7, 8))) void
 
 // One batched launch over the jobs of a timestep (mode is the same for all of them).
 void launch_trace_batch(mmhrt_ctx_s* c, const void* jobs, const uint32_t* prefix, uint32_t n_jobs, uint32_t* ctr,
-                        int mode) {
+                        int mode, bool treelets) {
     unsigned grid = (unsigned)std::max<int64_t>(1, c->cu_count * c->opt_blocks_per_cu);
     const TraceArgs* J = static_cast<const TraceArgs*>(jobs);
     int S = (int)c->opt_stack_lds;
     launch_fill_u32(c->stream, ctr, 0u, 16);
+    if (treelets) {  // treelet P-k-d trees: 16-entry LDS stacks
+        if (mode == 1)
+            hipLaunchKernelGGL((k_trace_batch<16, 1, true>), dim3(grid), dim3(TB), 0, c->stream, J, prefix, n_jobs, ctr);
+        else
+            hipLaunchKernelGGL((k_trace_batch<16, 0, true>), dim3(grid), dim3(TB), 0, c->stream, J, prefix, n_jobs, ctr);
+        return;
+    }
 #define MMHRT_BATCH(SV)                                                                                     \
     if (mode == 1)                                                                                          \
         hipLaunchKernelGGL((k_trace_batch<SV, 1>), dim3(grid), dim3(TB), 0, c->stream, J, prefix, n_jobs, ctr); \
@@ -980,6 +1171,11 @@ int batch_fill_job(mmhrt_ctx_s* c, const FrameConsts& fc, size_t out_offset, voi
     A.out_id = c->bt_id.p + out_offset;
     A.out_rgba8 = c->bt_rgba8.p + out_offset;
     A.out_aux = nullptr;
+    A.pkd_dim = c->accel == 1 ? c->pkd_dim.p : nullptr;
+    A.tl_rmax = c->accel == 1 ? c->tl_rmax.p : nullptr;
+    A.n_sph = (uint32_t)c->n;
+    A.tl_log2 = 0;
+    while ((1u << A.tl_log2) < c->pkd_M) ++A.tl_log2;
     A.fc = fc;
     A.fc.want_aux = 0;
     A.fc.accumulate = 0;
@@ -1007,6 +1203,13 @@ void dispatch_trace(mmhrt_ctx_s* c, const TraceArgs& A, unsigned grid, bool coun
     const int64_t rq = c->opt_refill < 0 ? (fc.spp == 1 ? 16 : 0) : c->opt_refill;
     uint32_t refill = (uint32_t)std::max<int64_t>(0, std::min<int64_t>(64, rq));
     const bool sl_shape = fc.spp > 1 && ((fc.spp >= 64 && fc.spp % 64 == 0) || (fc.spp < 64 && 64 % fc.spp == 0));
+    if (c->accel == 1) {
+        // treelet P-k-d trees: the sample-lanes kernel where the samples fill the lanes, else
+        // lane per pixel with work stealing (16-entry LDS stacks)
+        const bool sl = c->opt_sample_lanes != 0 && fc.mode != 2 && sl_shape;
+        launch_tl(A, grid, sl, count, fc.mode, c->stream);
+        return;
+    }
     if (c->opt_sample_lanes != 0 && refill == 0 && fc.mode != 2 && sl_shape) {
         switch (S) {
         case 8: launch_sl<8>(A, grid, count, fc.mode, c->bvh_pairs, c->stream); break;
@@ -1037,6 +1240,12 @@ void fill_scene_args(mmhrt_ctx_s* c, const FrameConsts& fc, TraceArgs& A) {
     A.leaf_id = c->leaf_id.p;
     A.col = c->col.p;
     A.info = c->info.p;
+    A.pkd_dim = c->accel == 1 ? c->pkd_dim.p : nullptr;
+    A.tl_rmax = c->accel == 1 ? c->tl_rmax.p : nullptr;
+    A.n_sph = (uint32_t)c->n;
+    uint32_t lg = 0;
+    while ((1u << lg) < c->pkd_M) ++lg;
+    A.tl_log2 = lg;
     A.tw_log2 = tile_log2(c->opt_tile_w);
     A.tiles_x = (fc.win_w + (1u << A.tw_log2) - 1) >> A.tw_log2;
     A.n_tiles = A.tiles_x * ((fc.win_h + (64u >> A.tw_log2) - 1) / (64u >> A.tw_log2));

@@ -261,6 +261,25 @@ class SphereRenderer:
         height = nodes[:, 15].copy().view(np.int32)
         return nodes[:, :12].copy(), child, ids, keys, height, pairs
 
+    def get_treelets(self) -> dict:
+        """The treelet P-k-d trees of the current build (option accel 1), in the oracle's array
+        layout: sph (n,4), id (n,), dim (n,), tl_box (T,6), tl_rmax (T,), top_box (L-1,12),
+        top_child (L-1,2), plus M, T, L."""
+        sizes = np.zeros(3, np.uint32)
+        self._ck(self.lib.mmhrt_get_treelets(self.ctx, sizes.ctypes.data, None, None, None, None, None, None))
+        M, T, L = (int(x) for x in sizes)
+        n = self.info()[0]
+        sph = np.empty((n, 4), np.float32)
+        ids = np.empty(n, np.uint32)
+        dims = np.empty(n, np.uint8)
+        tl_box = np.empty((T, 6), np.float32)
+        tl_rmax = np.empty(T, np.float32)
+        top = np.empty((L - 1, 16), np.float32)
+        self._ck(self.lib.mmhrt_get_treelets(self.ctx, None, sph.ctypes.data, ids.ctypes.data, dims.ctypes.data,
+                                             tl_box.ctypes.data, tl_rmax.ctypes.data, top.ctypes.data))
+        return dict(M=M, T=T, L=L, sph=sph, id=ids, dim=dims, tl_box=tl_box, tl_rmax=tl_rmax,
+                    top_box=top[:, :12].copy(), top_child=top[:, 12:14].copy().view(np.int32))
+
     def timings(self) -> dict:
         t = (C.c_float * 9)()
         self._ck(self.lib.mmhrt_get_timings(self.ctx, t, 9))

@@ -49,6 +49,12 @@ class OrcBvh(C.Structure):
                 ("keys", C.c_void_p), ("leaf_sph", C.c_void_p)]
 
 
+class OrcPkd(C.Structure):
+    _fields_ = [("n", C.c_uint64), ("M", C.c_uint32), ("T", C.c_uint32), ("top_leaves", C.c_uint32),
+                ("sph", C.c_void_p), ("id", C.c_void_p), ("dim", C.c_void_p), ("tl_box", C.c_void_p),
+                ("tl_rmax", C.c_void_p), ("top_box", C.c_void_p), ("top_child", C.c_void_p)]
+
+
 _lib = None
 
 
@@ -115,6 +121,14 @@ def load():
                                        C.c_void_p, C.c_void_p]
         f.orc_pack_rgba8.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p]
         f.orc_now.restype = C.c_double
+        f.orc_pkd_build.restype = C.c_int
+        f.orc_pkd_build.argtypes = [C.POINTER(OrcBvh), C.c_uint32, C.c_int, C.POINTER(OrcPkd)]
+        f.orc_pkd_free.argtypes = [C.POINTER(OrcPkd)]
+        f.orc_render_pkd.argtypes = [C.POINTER(OrcPkd), C.c_void_p, C.POINTER(OrcCamera), C.POINTER(OrcParams), C.c_int,
+                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.c_void_p]
+        f.orc_pkd_subtree_size.restype = C.c_uint32
+        f.orc_pkd_subtree_size.argtypes = [C.c_uint32, C.c_uint32]
         _lib = lib
     return _lib
 
@@ -208,6 +222,62 @@ class Bvh:
             self.lib.orc_bvh_free(C.byref(self.b))
         except Exception:
             pass
+
+
+class Pkd:
+    """The oracle's treelet P-k-d trees over the Morton order of an LBVH build (treelets of M
+    consecutive spheres, each a left-balanced k-d tree in heap order; a complete tree of boxes over
+    the treelets)."""
+
+    def __init__(self, bvh: "Bvh", M=1024, threads=0):
+        self.lib = load()
+        self.bvh = bvh  # keeps the LBVH's arrays alive for the lifetime of this object
+        self.k = OrcPkd()
+        t0 = self.lib.orc_now()
+        rc = self.lib.orc_pkd_build(C.byref(bvh.b), M, threads or default_threads(), C.byref(self.k))
+        self.build_s = self.lib.orc_now() - t0
+        if rc != 0:
+            raise ValueError(f"oracle P-k-d build failed (M {M})")
+
+    def arrays(self):
+        """sph (n,4), id (n,), dim (n,), tl_box (T,6), tl_rmax (T,), top_box (L-1,12), top_child (L-1,2)."""
+        k = self.k
+        n, T, L = k.n, k.T, k.top_leaves
+
+        def arr(ptr, shape, dt):
+            m = int(np.prod(shape))
+            a = np.ctypeslib.as_array(C.cast(ptr, C.POINTER(np.ctypeslib.as_ctypes_type(dt))), (m,))
+            return a.reshape(shape).copy()
+        return dict(sph=arr(k.sph, (n, 4), np.float32), id=arr(k.id, (n,), np.uint32),
+                    dim=arr(k.dim, (n,), np.uint8), tl_box=arr(k.tl_box, (T, 6), np.float32),
+                    tl_rmax=arr(k.tl_rmax, (T,), np.float32), top_box=arr(k.top_box, (L - 1, 12), np.float32),
+                    top_child=arr(k.top_child, (L - 1, 2), np.int32))
+
+    def render(self, rgba, cam, p: OrcParams, threads=0, aux=False, outputs=None) -> dict:
+        rgba = np.ascontiguousarray(rgba, np.float32)
+        o = _outs(p.height, p.width, aux)
+        if outputs is not None:
+            for k in list(o):
+                if k not in outputs and k != "aux":
+                    o[k] = None
+        cnt = np.zeros(2, np.uint64)
+        t0 = self.lib.orc_now()
+        self.lib.orc_render_pkd(C.byref(self.k), rgba.ctypes.data, C.byref(camera_from(cam)), C.byref(p),
+                                threads or default_threads(), _ptr(o["rgba8"]), _ptr(o["depth"]), _ptr(o["color"]),
+                                _ptr(o["t"]), _ptr(o["hit_id"]), _ptr(o["aux"]), cnt.ctypes.data)
+        o["seconds"] = self.lib.orc_now() - t0
+        o["node_visits"], o["sphere_tests"] = int(cnt[0]), int(cnt[1])
+        return o
+
+    def __del__(self):
+        try:
+            self.lib.orc_pkd_free(C.byref(self.k))
+        except Exception:
+            pass
+
+
+def pkd_subtree_size(i, m):
+    return int(load().orc_pkd_subtree_size(i, m))
 
 
 def primary_f64(xyzr, cam, p: OrcParams, x0=0, y0=0, x1=None, y1=None):

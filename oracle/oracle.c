@@ -322,9 +322,10 @@ static uint16_t f2h(float f) { /* round-to-nearest-even float -> IEEE half */
 typedef struct scene_view {
     const float* xyzr; /* brute force: original order */
     uint64_t n;
-    const orc_bvh* bvh; /* NULL = brute force */
-    float pad;          /* slab padding for the LBVH traversal */
+    const orc_bvh* bvh; /* NULL = brute force (or the treelet P-k-d trees below) */
+    float pad;          /* slab padding for the LBVH and P-k-d traversals */
     uint64_t node_visits, sphere_tests;
+    const orc_pkd* pkd; /* treelet P-k-d trees (when bvh is NULL) */
 } scene_view;
 
 static int closest_brute(scene_view* s, const float o[3], const float d[3], float tmin, float tmax, float* t_out,
@@ -491,11 +492,156 @@ static int any_bvh(scene_view* s, const float o[3], const float d[3], float tmin
     }
 }
 
-/* closest hit dispatcher; the tie rule "equal t -> lowest particle id" is applied by both paths */
+/* ------------------------------------------------ treelet P-k-d traversal */
+/* Box slab test returning the entry and exit distances (box widened by pad as in slab()). */
+static int slab2(const float* box, const float o[3], const float inv[3], float pad, float tmin, float tmax, float* tn,
+                 float* tf) {
+    float t0 = -FLT_MAX, t1 = FLT_MAX;
+    for (int a = 0; a < 3; ++a) {
+        float lo = (box[a] - pad - o[a]) * inv[a];
+        float hi = (box[3 + a] + pad - o[a]) * inv[a];
+        t0 = fmaxf(t0, fminf(lo, hi));
+        t1 = fminf(t1, fmaxf(lo, hi));
+    }
+    *tn = t0;
+    *tf = t1;
+    return t0 <= t1 && t1 >= tmin && t0 <= tmax;
+}
+
+/* One treelet's P-k-d tree for a ray entering its box over [t0, t1]: the interval logic of
+ * pkd_intersect (plugins/optix_owl/cuda_resources/pkdPrograms.cu:45-159, treeletsPrograms.cu
+ * treelets_intersect): the node's sphere is tested when the ray's interval meets the node's slab
+ * (split coordinate -+ the largest radius, here the treelet's, widened by pad); the near child
+ * takes [t0, min(slab exit, t1, hit)], the far child [max(t0, slab entry), min(t1, hit)]. Closed
+ * intervals throughout (a subtree is skipped only when its interval is empty or starts beyond the
+ * best hit), so that equal-t hits still reach the lowest particle id. any: stop at the first hit
+ * of a sphere other than `skip` (returns 1). */
+static int pkd_treelet(scene_view* s, uint32_t t, const float o[3], const float d[3], const float inv[3], float tmin,
+                       float tmax, float t0, float t1, float* best, int32_t* bid, int any, int32_t skip) {
+    const orc_pkd* k = s->pkd;
+    const uint64_t base = (uint64_t)t * k->M;
+    const uint32_t m = (uint32_t)((k->n - base) < k->M ? (k->n - base) : k->M);
+    const float rmax = k->tl_rmax[t];
+    struct {
+        uint32_t node;
+        float t0, t1;
+    } st[64];
+    int sp = 0;
+    uint32_t i = 0;
+    for (;;) {
+        for (;;) {
+            const uint64_t p = base + i;
+            const float* sph = k->sph + 4 * p;
+            const int dd = k->dim[p];
+            s->node_visits++;
+            float lo = (sph[dd] - rmax - s->pad - o[dd]) * inv[dd];
+            float hi = (sph[dd] + rmax + s->pad - o[dd]) * inv[dd];
+            float tnr = fminf(lo, hi), tfr = fmaxf(lo, hi);
+            if (fmaxf(t0, tnr) <= fminf(fminf(tfr, t1), *best)) {
+                int32_t id = (int32_t)k->id[p];
+                float th;
+                if (!(any && id == skip)) {
+                    s->sphere_tests++;
+                    if (orc_intersect_f32(o, d, tmin, tmax, sph, &th)) {
+                        if (any) return 1;
+                        if (th < *best || (th == *best && id < *bid)) {
+                            *best = th;
+                            *bid = id;
+                        }
+                    }
+                }
+            }
+            float n_t1 = fminf(fminf(tfr, t1), *best);
+            float f_t0 = fmaxf(t0, tnr), f_t1 = fminf(t1, *best);
+            int sgn = inv[dd] < 0.0f;
+            uint32_t nc = 2 * i + 1 + (uint32_t)sgn, fc = 2 * i + 2 - (uint32_t)sgn;
+            int need_n = nc < m && t0 <= n_t1;
+            int need_f = fc < m && f_t0 <= f_t1;
+            if (!need_n && !need_f) break;
+            if (need_n && need_f) {
+                st[sp].node = fc;
+                st[sp].t0 = f_t0;
+                st[sp].t1 = f_t1;
+                ++sp;
+                i = nc;
+                t1 = n_t1;
+            } else if (need_n) {
+                i = nc;
+                t1 = n_t1;
+            } else {
+                i = fc;
+                t0 = f_t0;
+                t1 = f_t1;
+            }
+        }
+        for (;;) {
+            if (sp == 0) return 0;
+            --sp;
+            t0 = st[sp].t0;
+            t1 = fminf(st[sp].t1, *best);
+            i = st[sp].node;
+            if (t0 <= t1) break;
+        }
+    }
+}
+
+/* The top tree of boxes over the treelets, front to back like closest_bvh; a treelet child is
+ * traversed as soon as its box is reached. */
+static int trace_pkd(scene_view* s, const float o[3], const float d[3], float tmin, float tmax, float* t_out,
+                     int32_t* id_out, int any, int32_t skip) {
+    const orc_pkd* k = s->pkd;
+    float inv[3];
+    safe_inv(d, inv);
+    float best = tmax;
+    int32_t bid = -1;
+    int32_t stack[64];
+    int sp = 0;
+    int32_t node = 0;
+    for (;;) {
+        s->node_visits++;
+        const float* bx = k->top_box + 12 * (size_t)node;
+        const int32_t* ch = k->top_child + 2 * (size_t)node;
+        float tn[2], tf[2];
+        int hit[2];
+        for (int c = 0; c < 2; ++c)
+            hit[c] = ch[c] != INT32_MIN && slab2(bx + 6 * c, o, inv, s->pad, tmin, best, &tn[c], &tf[c]);
+        int32_t next[2];
+        int nn = 0;
+        int order0 = (hit[0] && hit[1] && tn[1] < tn[0]) ? 1 : 0;
+        for (int q = 0; q < 2; ++q) {
+            int c = q ^ order0;
+            if (!hit[c] || tn[c] > best) continue;
+            if (ch[c] < 0) {
+                if (pkd_treelet(s, (uint32_t)(~ch[c]), o, d, inv, tmin, tmax, fmaxf(tn[c], tmin), tf[c], &best, &bid,
+                                any, skip))
+                    return 1;
+            } else {
+                next[nn++] = ch[c];
+            }
+        }
+        if (nn == 2) {
+            stack[sp++] = next[1];
+            node = next[0];
+        } else if (nn == 1) {
+            node = next[0];
+        } else {
+            if (sp == 0) break;
+            node = stack[--sp];
+        }
+    }
+    if (any) return 0;
+    *t_out = best;
+    *id_out = bid;
+    return bid >= 0;
+}
+
+/* closest hit dispatcher; the tie rule "equal t -> lowest particle id" is applied by every path */
 static int closest(scene_view* s, const float o[3], const float d[3], float tmin, float tmax, float* t, int32_t* id) {
+    if (s->pkd) return trace_pkd(s, o, d, tmin, tmax, t, id, 0, -1);
     return s->bvh ? closest_bvh(s, o, d, tmin, tmax, t, id) : closest_brute(s, o, d, tmin, tmax, t, id);
 }
 static int anyhit(scene_view* s, const float o[3], const float d[3], float tmin, float tmax, int32_t skip) {
+    if (s->pkd) return trace_pkd(s, o, d, tmin, tmax, NULL, NULL, 1, skip);
     return s->bvh ? any_bvh(s, o, d, tmin, tmax, skip) : any_brute(s, o, d, tmin, tmax, skip);
 }
 
@@ -754,7 +900,7 @@ int orc_render_brute(const float* xyzr, const float* rgba, uint64_t n, const orc
                      uint32_t* rgba8, float* depth, float* color, float* t_out, int32_t* id_out, uint16_t* aux) {
     orc_frame_state fs;
     orc_frame_state_from_camera(cam, &fs);
-    scene_view s = {xyzr, n, NULL, 0.0f, 0, 0};
+    scene_view s = {xyzr, n, NULL, 0.0f, 0, 0, NULL};
     for (uint32_t y = 0; y < p->height; ++y)
         for (uint32_t x = 0; x < p->width; ++x) {
             pixel_out po;
@@ -1208,6 +1354,7 @@ int orc_bvh_copy_out(const orc_bvh* b, float* node_box, int32_t* node_child, uin
  * counter to `threads` workers (OSPRay uses TBB tiles, AbstractOSPRayRenderer.cpp:126-132). */
 typedef struct mt_job {
     const orc_bvh* b;
+    const orc_pkd* k; /* set instead of b: the treelet P-k-d trees */
     const float* xyzr;
     const float* rgba;
     const orc_frame_state* fs;
@@ -1226,7 +1373,7 @@ typedef struct mt_job {
 
 static void* mt_worker(void* arg) {
     mt_job* J = arg;
-    scene_view s = {J->xyzr, J->b->n, J->b, J->pad, 0, 0};
+    scene_view s = {J->xyzr, J->k ? J->k->n : J->b->n, J->b, J->pad, 0, 0, J->k};
     for (;;) {
         uint32_t tile = atomic_fetch_add(&J->next, 1u);
         if (tile >= J->tiles) break;
@@ -1244,21 +1391,25 @@ static void* mt_worker(void* arg) {
     return NULL;
 }
 
-int orc_render_bvh(const orc_bvh* b, const float* rgba, const orc_camera* cam, const orc_params* p, int threads,
-                   uint32_t* rgba8, float* depth, float* color, float* t_out, int32_t* id_out, uint16_t* aux,
-                   uint64_t* counters) {
+static int render_mt(const orc_bvh* b, const orc_pkd* pk, const float* rgba, const orc_camera* cam,
+                     const orc_params* p, int threads, uint32_t* rgba8, float* depth, float* color, float* t_out,
+                     int32_t* id_out, uint16_t* aux, uint64_t* counters) {
     orc_frame_state fs;
     orc_frame_state_from_camera(cam, &fs);
+    const uint64_t n = pk ? pk->n : b->n;
+    const float* ls = pk ? pk->sph : b->leaf_sph;
+    const uint32_t* lid = pk ? pk->id : b->leaf_id;
     /* particle positions in original order, rebuilt from leaf order for shading lookups */
-    float* xyzr = malloc(b->n * 16);
-    for (uint64_t k = 0; k < b->n; ++k) memcpy(xyzr + 4 * (size_t)b->leaf_id[k], b->leaf_sph + 4 * k, 16);
+    float* xyzr = malloc(n * 16);
+    for (uint64_t k = 0; k < n; ++k) memcpy(xyzr + 4 * (size_t)lid[k], ls + 4 * k, 16);
     float amax = 0.0f;
-    for (uint64_t k = 0; k < b->n; ++k)
-        for (int a = 0; a < 3; ++a) amax = fmaxf(amax, fabsf(b->leaf_sph[4 * k + a]) + fabsf(b->leaf_sph[4 * k + 3]));
+    for (uint64_t k = 0; k < n; ++k)
+        for (int a = 0; a < 3; ++a) amax = fmaxf(amax, fabsf(ls[4 * k + a]) + fabsf(ls[4 * k + 3]));
     float cmax = fmaxf(fmaxf(fabsf(cam->position[0]), fabsf(cam->position[1])), fabsf(cam->position[2]));
     mt_job J;
     memset(&J, 0, sizeof J);
-    J.b = b;
+    J.b = pk ? NULL : b;
+    J.k = pk;
     J.xyzr = xyzr;
     J.rgba = rgba;
     J.fs = &fs;
@@ -1288,4 +1439,188 @@ int orc_render_bvh(const orc_bvh* b, const float* rgba, const orc_camera* cam, c
     }
     free(xyzr);
     return 0;
+}
+
+int orc_render_bvh(const orc_bvh* b, const float* rgba, const orc_camera* cam, const orc_params* p, int threads,
+                   uint32_t* rgba8, float* depth, float* color, float* t_out, int32_t* id_out, uint16_t* aux,
+                   uint64_t* counters) {
+    return render_mt(b, NULL, rgba, cam, p, threads, rgba8, depth, color, t_out, id_out, aux, counters);
+}
+
+int orc_render_pkd(const orc_pkd* k, const float* rgba, const orc_camera* cam, const orc_params* p, int threads,
+                   uint32_t* rgba8, float* depth, float* color, float* t_out, int32_t* id_out, uint16_t* aux,
+                   uint64_t* counters) {
+    return render_mt(NULL, k, rgba, cam, p, threads, rgba8, depth, color, t_out, id_out, aux, counters);
+}
+
+/* ------------------------------------------------- treelet P-k-d build */
+/* Treelets: the Morton order cut into runs of M spheres (the reference cuts at spatial midpoints
+ * until a run has at most `threshold` particles, TreeletsCreate.h:55-123; Morton runs are cells of
+ * the same kind of midpoint subdivision, and a fixed M lets the device address a treelet by
+ * shifting). Each treelet holds a left-balanced k-d tree over its spheres in heap order
+ * (PKDCreate.h:86-160): the node's split dimension is the widest axis of its cell (arg max of
+ * the extents, ties to the lower axis), and its sphere is the one of rank |left subtree| among the
+ * subtree's spheres ordered by the coordinate on that axis (ties: the lower Morton position), so
+ * the left subtree holds the spheres below it and the right subtree those above; the children's
+ * cells are the node's cell cut at its coordinate. The reference reaches the same partition with
+ * two heaps (makeHeap / trickle); the order here fixes which of equal coordinates goes where. The
+ * root cell is the treelet's sphere bounds. */
+static uint32_t ford(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+uint32_t orc_pkd_subtree_size(uint32_t i, uint32_t m) {
+    uint64_t first = i, width = 1, cnt = 0;
+    while (first < m) {
+        uint64_t end = first + width;
+        cnt += end <= m ? width : m - first;
+        first = 2 * first + 1;
+        width *= 2;
+    }
+    return (uint32_t)cnt;
+}
+
+static int widest_axis(const float box[6]) {
+    float e0 = box[3] - box[0], e1 = box[4] - box[1], e2 = box[5] - box[2];
+    int d = 0;
+    float e = e0;
+    if (e1 > e) {
+        d = 1;
+        e = e1;
+    }
+    if (e2 > e) d = 2;
+    return d;
+}
+
+static int cmp_u64(const void* a, const void* b) {
+    uint64_t x = *(const uint64_t*)a, y = *(const uint64_t*)b;
+    return x < y ? -1 : x > y;
+}
+
+/* node i over the local indices elems[0..cnt) of the treelet's spheres sph (Morton order) */
+static void pkd_node(const float* sph, const uint32_t* ids, uint64_t* elems, uint32_t cnt, uint32_t i, uint32_t m,
+                     const float box[6], float* out_sph, uint32_t* out_id, uint8_t* out_dim) {
+    if (cnt == 0) return;
+    int d = widest_axis(box);
+    for (uint32_t q = 0; q < cnt; ++q) {
+        uint32_t j = (uint32_t)elems[q];
+        elems[q] = ((uint64_t)ford(sph[4 * (size_t)j + d]) << 32) | j;
+    }
+    qsort(elems, cnt, sizeof *elems, cmp_u64);
+    uint32_t L = 2 * i + 1 < m ? orc_pkd_subtree_size(2 * i + 1, m) : 0;
+    uint32_t j = (uint32_t)elems[L];
+    memcpy(out_sph + 4 * (size_t)i, sph + 4 * (size_t)j, 16);
+    out_id[i] = ids[j];
+    out_dim[i] = (uint8_t)d;
+    float lb[6], rb[6];
+    memcpy(lb, box, sizeof lb);
+    memcpy(rb, box, sizeof rb);
+    lb[3 + d] = rb[d] = sph[4 * (size_t)j + d];
+    for (uint32_t q = 0; q < cnt; ++q) elems[q] &= 0xffffffffull;
+    pkd_node(sph, ids, elems, L, 2 * i + 1, m, lb, out_sph, out_id, out_dim);
+    pkd_node(sph, ids, elems + L + 1, cnt - L - 1, 2 * i + 2, m, rb, out_sph, out_id, out_dim);
+}
+
+typedef struct pkd_job {
+    const orc_bvh* b;
+    orc_pkd* k;
+} pkd_job;
+
+static void pkd_worker(void* a, int t, int T) {
+    pkd_job* J = a;
+    orc_pkd* k = J->k;
+    uint64_t* elems = malloc((size_t)k->M * sizeof *elems);
+    PAR_RANGE(k->T, t, T, t0, t1);
+    for (uint64_t tl = t0; tl < t1; ++tl) {
+        uint64_t base = tl * k->M;
+        uint32_t m = (uint32_t)((k->n - base) < k->M ? (k->n - base) : k->M);
+        const float* sph = J->b->leaf_sph + 4 * base;
+        float box[6] = {FLT_MAX, FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX};
+        float rmax = 0.0f;
+        for (uint32_t j = 0; j < m; ++j) {
+            const float* q = sph + 4 * (size_t)j;
+            for (int ax = 0; ax < 3; ++ax) {
+                box[ax] = fminf(box[ax], q[ax] - q[3]);
+                box[3 + ax] = fmaxf(box[3 + ax], q[ax] + q[3]);
+            }
+            rmax = fmaxf(rmax, q[3]);
+            elems[j] = j;
+        }
+        memcpy(k->tl_box + 6 * tl, box, sizeof box);
+        k->tl_rmax[tl] = rmax;
+        pkd_node(sph, J->b->leaf_id + base, elems, m, 0, m, box, k->sph + 4 * base, k->id + base, k->dim + base);
+    }
+    free(elems);
+}
+
+/* The box that no ray reaches (every coordinate 3e38): an absent child of the top tree. */
+#define PKD_ABSENT 3.0e38f
+
+int orc_pkd_build(const orc_bvh* b, uint32_t M, int threads, orc_pkd* out) {
+    memset(out, 0, sizeof *out);
+    if (b->n == 0 || M < 2 || (M & (M - 1))) return -1;
+    out->n = b->n;
+    out->M = M;
+    out->T = (uint32_t)((b->n + M - 1) / M);
+    uint32_t L = 2;
+    while (L < out->T) L *= 2;
+    out->top_leaves = L;
+    out->sph = malloc(b->n * 16);
+    out->id = malloc(b->n * 4);
+    out->dim = malloc(b->n);
+    out->tl_box = malloc((size_t)out->T * 24);
+    out->tl_rmax = malloc((size_t)out->T * 4);
+    out->top_box = malloc((size_t)(L - 1) * 48);
+    out->top_child = malloc((size_t)(L - 1) * 8);
+    pkd_job J = {b, out};
+    par_run(threads_for(threads, b->n), pkd_worker, &J);
+    /* boxes of all 2L-1 heap nodes of the top tree, bottom up: a leaf is its treelet's box (or
+     * absent), a node the union of its present children */
+    size_t nn = 2 * (size_t)L - 1;
+    float* box = malloc(nn * 24);
+    uint8_t* present = malloc(nn);
+    for (size_t h = nn; h-- > 0;) {
+        float* bx = box + 6 * h;
+        if (h >= L - 1) {
+            size_t t = h - (L - 1);
+            present[h] = t < out->T;
+            if (present[h]) memcpy(bx, out->tl_box + 6 * t, 24);
+        } else {
+            size_t c0 = 2 * h + 1, c1 = 2 * h + 2;
+            present[h] = present[c0] || present[c1];
+            if (present[c0] && present[c1]) {
+                for (int a = 0; a < 3; ++a) {
+                    bx[a] = fminf(box[6 * c0 + a], box[6 * c1 + a]);
+                    bx[3 + a] = fmaxf(box[6 * c0 + 3 + a], box[6 * c1 + 3 + a]);
+                }
+            } else if (present[h]) {
+                memcpy(bx, box + 6 * (present[c0] ? c0 : c1), 24);
+            }
+        }
+        if (!present[h])
+            for (int a = 0; a < 6; ++a) bx[a] = PKD_ABSENT;
+    }
+    for (size_t h = 0; h < L - 1; ++h)
+        for (int c = 0; c < 2; ++c) {
+            size_t ch = 2 * h + 1 + (size_t)c;
+            memcpy(out->top_box + 12 * h + 6 * c, box + 6 * ch, 24);
+            out->top_child[2 * h + c] =
+                !present[ch] ? INT32_MIN : ch >= L - 1 ? ~(int32_t)(ch - (L - 1)) : (int32_t)ch;
+        }
+    free(box);
+    free(present);
+    return 0;
+}
+
+void orc_pkd_free(orc_pkd* k) {
+    free(k->sph);
+    free(k->id);
+    free(k->dim);
+    free(k->tl_box);
+    free(k->tl_rmax);
+    free(k->top_box);
+    free(k->top_child);
+    memset(k, 0, sizeof *k);
 }

@@ -55,6 +55,25 @@ typedef struct orc_bvh {
     float* leaf_sph;     /* n * 4: spheres in leaf order */
 } orc_bvh;
 
+/* Treelets of P-k-d trees (plugins/optix_owl/src/TreeletsCreate.h:117-166, PKDCreate.h:86-160):
+ * the Morton-sorted spheres are cut into treelets of M consecutive spheres (the last one shorter),
+ * each stored as a left-balanced k-d tree over its spheres in heap order (node i of treelet t at
+ * position t*M + i, children 2i+1 and 2i+2), with a complete binary tree of boxes over the
+ * treelets on top (top nodes 0 .. top_leaves-2 in heap order; child c >= top_leaves-1 is treelet
+ * c - (top_leaves-1), absent when >= T). */
+typedef struct orc_pkd {
+    uint64_t n;          /* spheres */
+    uint32_t M, T;       /* treelet size (power of two), treelet count */
+    uint32_t top_leaves; /* leaves of the top tree: the smallest power of two >= max(T, 2) */
+    float* sph;          /* n * 4: spheres in treelet heap order */
+    uint32_t* id;        /* n: particle index */
+    uint8_t* dim;        /* n: split dimension of the node */
+    float* tl_box;       /* T * 6: sphere bounds of each treelet (lo xyz, hi xyz) */
+    float* tl_rmax;      /* T: largest radius in each treelet */
+    float* top_box;      /* (top_leaves-1) * 12: child0 lo xyz hi xyz, child1 lo xyz hi xyz */
+    int32_t* top_child;  /* (top_leaves-1) * 2: >= 0 top node, < 0 treelet ~t, INT32_MIN absent */
+} orc_pkd;
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -93,6 +112,14 @@ int orc_render_bvh(const orc_bvh* b, const float* rgba, const orc_camera* cam, c
                    uint32_t* rgba8, float* depth, float* color, float* t_out, int32_t* id_out, uint16_t* aux,
                    uint64_t* counters);
 int orc_bvh_copy_out(const orc_bvh* b, float* node_box, int32_t* node_child, uint32_t* leaf_id, uint64_t* keys);
+
+/* Treelet P-k-d trees over the Morton order of an LBVH build (its leaf_sph / leaf_id). */
+int orc_pkd_build(const orc_bvh* b, uint32_t M, int threads, orc_pkd* out);
+void orc_pkd_free(orc_pkd* k);
+int orc_render_pkd(const orc_pkd* k, const float* rgba, const orc_camera* cam, const orc_params* p, int threads,
+                   uint32_t* rgba8, float* depth, float* color, float* t_out, int32_t* id_out, uint16_t* aux,
+                   uint64_t* counters);
+uint32_t orc_pkd_subtree_size(uint32_t i, uint32_t m);
 
 double orc_now(void);
 #ifdef __cplusplus
