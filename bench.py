@@ -237,7 +237,7 @@ def cpu_baseline(scene, args, r):
                 trace_seconds=trace_s, build_seconds=build_s, sample_pixels=sample_px)
 
 
-def denoised_leg(r, scene, args, cam, p_plain):
+def denoised_leg(r, scene, args, cam, p_plain, plain_cfg=None):
     """1-spp AO + learned denoiser, timed like the plain frame, and the image quality of both
     against a 256-spp frame of the same view (independent random stream)."""
     from megamol_amd import _lib as L, params
@@ -327,6 +327,36 @@ def denoised_leg(r, scene, args, cam, p_plain):
 
     ref = colour(params(W, H, spp=256, mode=L.MODE_AO, ao_distance=args.ao_distance, frame_idx=7777))
     plain = colour(params(W, H, spp=args.spp, mode=L.MODE_AO, ao_distance=args.ao_distance))
+    # quality against cost of the plain path: PSNR of 1 ... 128 spp against the 256-spp frame, and
+    # each frame's time (LBVH rebuild + trace, the plain frame's tree and launch)
+    curve = []
+    if not args.no_curve:
+        # the plain frame's launch and tree (its autotuned configuration; refill back to the
+        # library default: compaction for 1-spp frames only)
+        saved = {k: r.get_option(k) for k in (plain_cfg or {})}
+        saved["refill"] = r.get_option("refill")
+        for k, v in (plain_cfg or {}).items():
+            r.set_option(k, v)
+        r.set_option("refill", -1)
+        for spp_k in (1, 2, 4, 8, 16, 32, 64, 128):
+            pk = params(W, H, spp=spp_k, mode=L.MODE_AO, ao_distance=args.ao_distance, flags=L.FLAG_NO_DOWNLOAD)
+            for _ in range(2):
+                r.build(L.BUILD_REBUILD, args.bits)
+                r.render_async(cam, pk)
+            r.sync()
+            t0 = time.perf_counter()
+            reps = max(2, min(args.steps, 6))
+            for _ in range(reps):
+                r.build(L.BUILD_REBUILD, args.bits)
+                r.render_async(cam, pk)
+            r.sync()
+            ms_k = 1e3 * (time.perf_counter() - t0) / reps
+            img = colour(params(W, H, spp=spp_k, mode=L.MODE_AO, ao_distance=args.ao_distance))
+            curve.append({"spp": spp_k, "ms": round(ms_k, 3), "psnr_db": round(psnr(img, ref), 2)})
+            log(f"plain {spp_k} spp: {ms_k:.2f} ms/frame, {curve[-1]['psnr_db']} dB")
+        for k, v in saved.items():
+            r.set_option(k, v)
+        r.build(L.BUILD_REBUILD, args.bits)
     raw = colour(params(W, H, spp=1, mode=L.MODE_AO, ao_distance=args.ao_distance, flags=L.FLAG_AUX))
     raw_aux = r.download(W, H, outputs=(), aux=True).aux
     from megamol_amd.denoiser import box_filter_baseline
@@ -341,6 +371,24 @@ def denoised_leg(r, scene, args, cam, p_plain):
            "ssim_vs_256spp": {"denoised_1spp": round(ssim(den, ref), 4), "plain": round(ssim(plain, ref), 4),
                               "raw_1spp": round(ssim(raw, ref), 4), "box5_filter_1spp": round(ssim(box, ref), 4)},
            "weights": trained, "weights_meta": meta, "autotune_1spp": tuned1, "chunk_pass": chunk}
+    if curve:
+        # the plain frame of equal quality: PSNR interpolated linearly in log2(spp) between the two
+        # measured counts around the denoised frame's PSNR, and the time the same way
+        dpsnr = res["psnr_db_vs_256spp"]["denoised_1spp"]
+        eq = None
+        for a, b in zip(curve, curve[1:]):
+            if a["psnr_db"] <= dpsnr <= b["psnr_db"]:
+                w = (dpsnr - a["psnr_db"]) / max(b["psnr_db"] - a["psnr_db"], 1e-9)
+                spp_eq = 2.0 ** (np.log2(a["spp"]) + w)
+                ms_eq = a["ms"] + w * (b["ms"] - a["ms"])
+                eq = {"spp": round(float(spp_eq), 2), "plain_ms": round(float(ms_eq), 3),
+                      "plain_over_denoised": round(float(ms_eq) / ms, 2), "between_spp": [a["spp"], b["spp"]]}
+                break
+        if eq is None and curve[0]["psnr_db"] >= dpsnr:
+            eq = {"spp": 1, "plain_ms": curve[0]["ms"], "plain_over_denoised": round(curve[0]["ms"] / ms, 2),
+                  "note": "the 1-spp plain frame already reaches the denoised PSNR"}
+        res["plain_psnr_curve"] = curve
+        res["equal_psnr"] = eq
     log(f"denoised frame {ms:.2f} ms (no denoiser {ms_nodn:.2f}); PSNR {res['psnr_db_vs_256spp']}")
     if args.save_images:
         # centre crops for offline error analysis: the 256-spp mean albedo (AO distance ~0: every
@@ -355,6 +403,54 @@ def denoised_leg(r, scene, args, cam, p_plain):
                             raw=raw[sl].astype(np.float16), den=den[sl].astype(np.float16),
                             alb256=alb[sl].astype(np.float16), feats=feats[sl])
     return res
+
+
+def structure_compare(r, args, cam):
+    """The LBVH against the treelet P-k-d trees on the same scene: build time and 1-spp AO trace
+    time (HIP events), node visits per ray, and the device memory each structure keeps beyond
+    the spheres (both keep the spheres in their own order, 16 B, and a particle id, 4 B, per
+    sphere)."""
+    from megamol_amd import _lib as L, params
+
+    W, H = args.width, args.height
+    p1 = params(W, H, spp=1, mode=L.MODE_AO, ao_distance=args.ao_distance, flags=L.FLAG_NO_DOWNLOAD)
+    n = r.info()[0]
+    out, ids = {}, {}
+    for name, accel in (("lbvh", 0), ("treelets", 1)):
+        r.set_option("accel", accel)
+        bt, tt = [], []
+        for i in range(max(2, args.steps) + 1):
+            r.build(L.BUILD_REBUILD, args.bits)
+            r.render_async(cam, p1)
+            t = r.timings()
+            if i:
+                bt.append(t["build"])
+                tt.append(t["trace"])
+        r.set_option("count_visits", 1)
+        r.render_async(cam, p1)
+        visits, tests, rays = r.counters()
+        r.set_option("count_visits", 0)
+        ids[name] = r.download(W, H, outputs=("hit_id",)).hit_id.copy()
+        trace_ms = float(np.median(tt))
+        if accel == 0:
+            pairs = r.get_option("leaf_pair_count")
+            mem = {"node_slots_bytes": 64 * (n - 1), "stored_node_bytes": 64 * (n - 1 - pairs)}
+        else:
+            M = r.get_option("treelet")
+            T = (n + M - 1) // M
+            Lt = 2
+            while Lt < T:
+                Lt *= 2
+            mem = {"top_node_bytes": 64 * (Lt - 1), "split_axis_bytes": n, "treelet_record_bytes": 28 * T}
+        mem["total_beyond_spheres_and_ids"] = (mem.get("node_slots_bytes", 0) + mem.get("top_node_bytes", 0)
+                                              + mem.get("split_axis_bytes", 0) + mem.get("treelet_record_bytes", 0))
+        out[name] = {"build_ms": round(float(np.median(bt)), 3), "trace_1spp_ms": round(trace_ms, 3),
+                     "mrays_per_s": round(rays / (trace_ms * 1e-3) / 1e6, 1), "visits_per_ray": round(visits / rays, 2),
+                     "tests_per_ray": round(tests / rays, 2), "memory": mem}
+        log(f"structure {name}: {out[name]}")
+    r.set_option("accel", 0)
+    out["same_hit_ids"] = bool(np.array_equal(ids["lbvh"], ids["treelets"]))
+    return out
 
 
 def run_series(args):
@@ -417,6 +513,9 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="fcc/cube configs: keep the library's default launch knobs instead of tuning them")
     ap.add_argument("--no-denoise", action="store_true")
+    ap.add_argument("--no-curve", action="store_true", help="skip the plain path's PSNR-vs-spp curve")
+    ap.add_argument("--structures", action="store_true",
+                    help="also compare the LBVH with the treelet P-k-d trees (build, 1-spp AO trace, memory)")
     ap.add_argument("--weights", type=str,
                     default=os.path.join(ROOT, "megamol_amd", "weights", "denoiser_fcc.bin"))
     ap.add_argument("--train-pairs", type=int, default=256)
@@ -574,10 +673,14 @@ def main():
 
     dn = None
     if mode == L.MODE_AO and args.config in ("fcc", "clusters") and not args.no_denoise:
-        dn = denoised_leg(r, scene, args, cam, p)
+        dn = denoised_leg(r, scene, args, cam, p, tuned["config"] if tuned else None)
         dn["plain_over_denoised"] = round(ms / dn["ms"], 2)
         dn["target_8x_at_38db"] = bool(dn["plain_over_denoised"] >= 8.0 and
                                        dn["psnr_db_vs_256spp"]["denoised_1spp"] >= 38.0)
+
+    structures = None
+    if args.structures:
+        structures = structure_compare(r, args, cam)
 
     orbit = None
     if args.config == "clusters" and args.orbit > 0:
@@ -656,6 +759,7 @@ def main():
                                  "kernel": kernel_name, "algorithmic_bytes": int(algo_bytes)},
         "cpu_baseline": cpu,
         "orbit": orbit,
+        "structures": structures,
     }
     print(json.dumps(out), flush=True)
     r.close()

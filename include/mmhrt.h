@@ -318,7 +318,9 @@ MMHRT_API int32_t mmhrt_denoiser_train_step(mmhrt_ctx ctx, const uint16_t* feats
  * structure: 256, 512 or 1024, default 1024),
  * "sample_lanes" (samples of a pixel on the lanes for power-of-two spp,
  * default 1), "dn_k32" (denoiser forward on the 16x16x32 MFMA kernels, default 1),
- * "dn_conv_shape" (test hook: forced launch shape of those kernels, 0 = automatic).
+ * "dn_conv_shape" (test hook: forced launch shape of those kernels, 0 = automatic),
+ * "fail_alloc" (test hook: the k-th device allocation from now fails with MMHRT_E_OOM, as a real
+ * failure would, and the option returns to 0; 0, the default, injects nothing).
  * Diagnostics: "count_visits" (0/1) and, read-only in effect, "stack_overflow", "pair_visits",
  * "leaf_fetches" (refreshed by mmhrt_get_counters), "sort_fault", and "leaf_pair_count" (get
  * only: the leaf pairs of the current tree, counted on the device from its sorted keys). */

@@ -83,14 +83,19 @@ def build(verbose: bool = False, jobs: int | None = None) -> Path:
 
 def build_module_test() -> Path:
     """The MegaMol renderer module compiled against the in-repo stand-ins of the MegaMol types,
-    with a driver that renders through it (host C++ only, links libmmhrt.so)."""
+    with a driver that renders through it and the headless project runner (host C++ only, links
+    libmmhrt.so)."""
     mod = PKG / "module"
     outdir = mod / "_build"
     outdir.mkdir(exist_ok=True)
     exe = outdir / "module_test"
-    _run(["g++", "-std=c++17", "-O2", "-Wall", "-I", str(mod / "mmstub"), "-I", str(mod), "-I",
-          str(PKG.parent / "include"), str(mod / "module_test.cpp"), str(mod / "HIPSphereRenderer.cpp"), "-L",
-          str(LIB), "-lmmhrt", "-Wl,-rpath,$ORIGIN/../../lib", "-o", str(exe)])
+    inc = ["-I", str(mod / "mmstub"), "-I", str(mod), "-I", str(PKG.parent / "include")]
+    link = ["-L", str(LIB), "-lmmhrt", "-Wl,-rpath,$ORIGIN/../../lib"]
+    _run(["g++", "-std=c++17", "-O2", "-Wall", *inc, str(mod / "module_test.cpp"), str(mod / "HIPSphereRenderer.cpp"),
+          *link, "-o", str(exe)])
+    # the headless project runner: MMPLD source -> renderer module -> screenshots
+    _run(["g++", "-std=c++17", "-O2", "-Wall", *inc, str(mod / "headless.cpp"), str(mod / "MMPLDSource.cpp"),
+          str(mod / "HIPSphereRenderer.cpp"), *link, "-o", str(outdir / "headless")])
     return exe
 
 

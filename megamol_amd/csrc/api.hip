@@ -530,12 +530,14 @@ static int64_t* option_slot(mmhrt_ctx c, const char* key) {
     if (k == "sort_ri") return &c->opt_sort_ri;
     if (k == "build_grid") return &c->opt_build_grid;
     if (k == "sort_onesweep") return &c->opt_sort_onesweep;
+    if (k == "sort_digit") return &c->opt_sort_digit;
     if (k == "leaf_pairs") return &c->opt_leaf_pairs;
     if (k == "build_tail") return &c->opt_build_tail;
     if (k == "build_wave") return &c->opt_build_wave;
     if (k == "sort_xcd") return &c->opt_sort_xcd;
     if (k == "build_span") return &c->opt_build_span;
     if (k == "accel") return &c->opt_accel;
+    if (k == "fail_alloc") return &c->opt_fail_alloc;
     if (k == "treelet") return &c->opt_treelet;
     if (k == "stack_overflow") return &c->stack_overflow;
     if (k == "pair_visits") return &c->pair_visits;

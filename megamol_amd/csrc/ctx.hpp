@@ -134,12 +134,14 @@ struct mmhrt_ctx_s {
     int64_t opt_build_ls = 1;       // level-synchronous LBVH chunk pass (0: one climbing thread per leaf)
     int64_t opt_sort_ri = 16;       // radix sort keys per thread per tile: 8, 16 or 32 (32 for 32-bit keys only)
     int64_t opt_sort_onesweep = 0;  // single-pass radix passes with decoupled look-back (0: histogram + scans + scatter)
+    int64_t opt_sort_digit = 0;     // radix digit width: 0 wide (10 bits for 30-bit keys, 11 for 63-bit), 8
     int64_t opt_build_grid = 0;     // test hook: cap on the persistent chunk workgroups (0 = as many as fit)
     int64_t opt_leaf_pairs = 1;     // nodes over two leaves are not stored (their parents reference the pair)
     int64_t opt_build_tail = 512;   // straddler worklist size from which one workgroup finishes the tree
     int64_t opt_build_wave = 64;    // chunk pass: rounds of at most this many (<= 64) merges run on one wavefront (0: none)
     int64_t opt_sort_xcd = 1;       // radix scatter: each XCD takes one contiguous range of tiles (0: round robin)
     int64_t opt_build_span = 256;   // straddlers: one workgroup per span of this many chunks, then the tail (0: rounds)
+    int64_t opt_fail_alloc = 0;     // test hook: the k-th device allocation from now fails with E_OOM (0: none)
     int64_t opt_accel = 0;          // structure the next build makes: 0 LBVH, 1 treelet P-k-d trees
     int64_t opt_treelet = 1024;     // spheres per treelet of the P-k-d structure: 256, 512 or 1024
     int64_t refits_since_build = 0;
@@ -180,6 +182,8 @@ int hip_fail(mmhrt_ctx_s* c, hipError_t e, const char* what);
 template <typename T>
 int ensure(mmhrt_ctx_s* c, DevBuf<T>& b, size_t n, const char* what) {
     if (b.cap >= n && b.p) return 0;
+    if (c->opt_fail_alloc > 0 && --c->opt_fail_alloc == 0)  // injected failure (test hook "fail_alloc")
+        return fail(c, 3, std::string("device allocation failed: ") + what + " (injected)");
     if (b.p) {
         hipError_t e = hipFree(b.p);
         b.p = nullptr;

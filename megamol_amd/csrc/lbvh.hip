@@ -26,11 +26,12 @@ constexpr int RB = 256;      // threads per block
 
 __device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << __lane_id()) - 1ull; }
 
-// Mask of the lanes of this wave whose (valid) digit equals ours.
+// Mask of the lanes of this wave whose (valid) DB-bit digit equals ours.
+template <int DB = 8>
 __device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid) {
     uint64_t m = __ballot(valid);
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
+    for (int b = 0; b < DB; ++b) {
         uint64_t bm = __ballot((d >> b) & 1u);
         m &= ((d >> b) & 1u) ? bm : ~bm;
     }
@@ -191,13 +192,13 @@ __device__ __forceinline__ uint64_t morton63(const float4 s, const SceneInfo* in
 // The Morton codes of one radix tile (RT spheres, laid out as the sort's tiles are) and the
 // tile's histogram of the first 8-bit digit: the first radix pass then needs no histogram pass,
 // and no value array either, since its values are the particle indices themselves.
-template <typename K, int RI>
+template <typename K, int RI, int DB>
 __global__ __launch_bounds__(RB) void k_morton_tile(const float4* __restrict__ sph, uint64_t n,
                                                     const SceneInfo* __restrict__ info, K* __restrict__ keys,
                                                     uint32_t* __restrict__ hist) {
-    constexpr int RT = RB * RI, RW = RT / (RB / 64), SB = 8;
-    __shared__ uint32_t h[256];
-    h[threadIdx.x] = 0;
+    constexpr int RT = RB * RI, RW = RT / (RB / 64), SB = 8, ND = 1 << DB;
+    __shared__ uint32_t h[ND];
+    for (int d = threadIdx.x; d < ND; d += RB) h[d] = 0;
     __syncthreads();
     const float sc = info->scale;
     const uint64_t base = (uint64_t)blockIdx.x * RT + (uint64_t)(threadIdx.x >> 6) * RW + __lane_id();
@@ -218,32 +219,32 @@ __global__ __launch_bounds__(RB) void k_morton_tile(const float4* __restrict__ s
                 key = morton30(sb[j], info, sc);
             if (i < n) {
                 keys[i] = key;
-                hist_add(h, (uint32_t)(key & 255u));
+                hist_add(h, (uint32_t)(key & (K)(ND - 1)));
             }
         }
     }
     __syncthreads();
-    hist[(size_t)blockIdx.x * 256 + threadIdx.x] = h[threadIdx.x];
+    for (int d = threadIdx.x; d < ND; d += RB) hist[(size_t)blockIdx.x * ND + d] = h[d];
 }
 
-template <typename K, int RI>
+template <typename K, int RI, int DB>
 __global__ __launch_bounds__(RB) void k_radix_hist(const K* __restrict__ keys, uint64_t n, int shift,
                                                    uint32_t* __restrict__ hist) {
-    constexpr int RT = RB * RI, RW = RT / (RB / 64);
-    __shared__ uint32_t h[256];
-    h[threadIdx.x] = 0;
+    constexpr int RT = RB * RI, RW = RT / (RB / 64), ND = 1 << DB;
+    __shared__ uint32_t h[ND];
+    for (int d = threadIdx.x; d < ND; d += RB) h[d] = 0;
     const uint64_t base = (uint64_t)blockIdx.x * RT + (uint64_t)(threadIdx.x >> 6) * RW + __lane_id();
     // unconditional loads at clamped indices (a guarded load becomes a branch and a wait per
-    // key); keys past n count as digit 256, which is not counted
+    // key); keys past n count as digit ND, which is not counted
     uint32_t d[RI];
 #pragma unroll
     for (int r = 0; r < RI; ++r) {
         uint64_t i = base + (uint64_t)r * 64;
-        d[r] = (uint32_t)((keys[i < n ? i : n - 1] >> shift) & 255u);
+        d[r] = (uint32_t)((keys[i < n ? i : n - 1] >> shift) & (K)(ND - 1));
     }
 #pragma unroll
     for (int r = 0; r < RI; ++r)
-        if (base + (uint64_t)r * 64 >= n) d[r] = 256u;
+        if (base + (uint64_t)r * 64 >= n) d[r] = (uint32_t)ND;
     __syncthreads();
     // LDS atomics; the lanes whose digit equals lane 0's add as one (Morton digits of
     // neighbouring keys are often equal, and the LDS unit serialises same-address lanes)
@@ -252,11 +253,11 @@ __global__ __launch_bounds__(RB) void k_radix_hist(const K* __restrict__ keys, u
         const uint32_t d0 = __builtin_amdgcn_readfirstlane(d[r]);
         const bool eq = d[r] == d0;
         const uint64_t m = __ballot(eq);
-        if (__lane_id() == 0 && d0 < 256u) atomicAdd(&h[d0], (uint32_t)__popcll(m));
-        if (!eq && d[r] < 256u) atomicAdd(&h[d[r]], 1u);
+        if (__lane_id() == 0 && d0 < (uint32_t)ND) atomicAdd(&h[d0], (uint32_t)__popcll(m));
+        if (!eq && d[r] < (uint32_t)ND) atomicAdd(&h[d[r]], 1u);
     }
     __syncthreads();
-    hist[(size_t)blockIdx.x * 256 + threadIdx.x] = h[threadIdx.x];
+    for (int dd = threadIdx.x; dd < ND; dd += RB) hist[(size_t)blockIdx.x * ND + dd] = h[dd];
 }
 
 // Stable scatter of one tile for one 8-bit digit (vin null: the values are the key indices). Each wave ranks its segment with ballots and
@@ -264,16 +265,16 @@ __global__ __launch_bounds__(RB) void k_radix_hist(const K* __restrict__ keys, u
 // digits turns the per-wave counts into offsets, the keys are placed in digit-sorted order in
 // LDS and finally written out in digit-contiguous runs, so that consecutive threads write
 // consecutive addresses.
-template <typename K, int RI>
+template <typename K, int RI, int DB>
 __global__ __launch_bounds__(RB) void k_radix_scatter(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                       K* __restrict__ kout, uint32_t* __restrict__ vout, uint64_t n,
                                                       int shift, const uint32_t* __restrict__ hist,
                                                       const uint32_t* __restrict__ offs, int xcd) {
-    constexpr int RT = RB * RI, RW = RT / (RB / 64);
+    constexpr int RT = RB * RI, RW = RT / (RB / 64), ND = 1 << DB, DPT = ND / RB;  // digits per thread
     __shared__ K s_key[RT];
     __shared__ uint32_t s_val[RT];
-    __shared__ uint32_t wc[RB / 64][256];  // per-wave running digit counts, then per-wave offsets
-    __shared__ uint32_t goff[256];         // global offset of the digit's run minus its local start
+    __shared__ uint16_t wc[RB / 64][ND];  // per-wave running digit counts, then per-wave offsets (< RT)
+    __shared__ uint32_t goff[ND];         // global offset of the digit's run minus its local start
     __shared__ uint32_t wsum[RB / 64];
     // xcd: workgroups are dealt to the 8 XCDs in turn, so workgroup w takes tile (w mod 8) * per +
     // w / 8 and each XCD scatters one contiguous range of tiles: the runs that consecutive tiles
@@ -306,40 +307,60 @@ __global__ __launch_bounds__(RB) void k_radix_scatter(const K* __restrict__ kin,
 #pragma unroll
         for (int r = 0; r < RI; ++r) val[r] = (uint32_t)(base + (uint64_t)r * 64);
     }
-    // tile-local digit starts: exclusive scan of this tile's digit counts (thread = digit)
-    const uint32_t cnt = hist[(size_t)tile * 256 + threadIdx.x];
-    uint32_t inc = cnt;
+    // tile-local digit starts: exclusive scan of this tile's digit counts; thread t owns the DPT
+    // consecutive digits from t * DPT
+    uint32_t cnt[DPT];
+    uint32_t tsum = 0;
+#pragma unroll
+    for (int j = 0; j < DPT; ++j) {
+        cnt[j] = hist[(size_t)tile * ND + threadIdx.x * DPT + j];
+        tsum += cnt[j];
+    }
+    uint32_t inc = tsum;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
         uint32_t y = __shfl_up(inc, o, 64);
         if (lane >= (uint32_t)o) inc += y;
     }
     if (lane == 63) wsum[wave] = inc;
+    for (int d = threadIdx.x; d < ND; d += RB)
 #pragma unroll
-    for (int w = 0; w < RB / 64; ++w) wc[w][threadIdx.x] = 0;
+        for (int w = 0; w < RB / 64; ++w) wc[w][d] = 0;
     __syncthreads();
     uint32_t wb = 0;
     for (int w = 0; w < wave; ++w) wb += wsum[w];
-    const uint32_t lstart = wb + inc - cnt;
-    goff[threadIdx.x] = offs[(size_t)tile * 256 + threadIdx.x] - lstart;
+    uint32_t lstart[DPT];
+    {
+        uint32_t run = wb + inc - tsum;
+#pragma unroll
+        for (int j = 0; j < DPT; ++j) {
+            const int d = threadIdx.x * DPT + j;
+            lstart[j] = run;
+            goff[d] = offs[(size_t)tile * ND + d] - run;
+            run += cnt[j];
+        }
+    }
     // ranks inside the wave's segment
     uint32_t rank[RI];
 #pragma unroll
     for (int r = 0; r < RI; ++r) {
         bool valid = base + (uint64_t)r * 64 < n;
-        uint32_t d = (uint32_t)((key[r] >> shift) & 255u);
-        uint64_t peers = match_digit(d, valid);
+        uint32_t d = (uint32_t)((key[r] >> shift) & (K)(ND - 1));
+        uint64_t peers = match_digit<DB>(d, valid);
         uint32_t before = valid ? wc[wave][d] : 0u;
         rank[r] = before + (uint32_t)__popcll(peers & lanemask_lt());
-        if (valid && (__ffsll((unsigned long long)peers) - 1) == (int)lane) wc[wave][d] = before + (uint32_t)__popcll(peers);
+        if (valid && (__ffsll((unsigned long long)peers) - 1) == (int)lane)
+            wc[wave][d] = (uint16_t)(before + (uint32_t)__popcll(peers));
     }
     __syncthreads();
-    {  // thread = digit: per-wave counts -> tile-local offsets of each wave's run
-        uint32_t acc = lstart;
+#pragma unroll
+    for (int j = 0; j < DPT; ++j) {  // per-wave counts -> tile-local offsets of each wave's run
+        const int d = threadIdx.x * DPT + j;
+        uint32_t acc = lstart[j];
 #pragma unroll
         for (int w = 0; w < RB / 64; ++w) {
-            uint32_t c = wc[w][threadIdx.x];
-            wc[w][threadIdx.x] = acc;
+            uint32_t c = wc[w][d];
+            wc[w][d] = (uint16_t)acc;
             acc += c;
         }
     }
@@ -347,7 +368,7 @@ __global__ __launch_bounds__(RB) void k_radix_scatter(const K* __restrict__ kin,
 #pragma unroll
     for (int r = 0; r < RI; ++r) {
         if (base + (uint64_t)r * 64 < n) {
-            uint32_t d = (uint32_t)((key[r] >> shift) & 255u);
+            uint32_t d = (uint32_t)((key[r] >> shift) & (K)(ND - 1));
             uint32_t pos = wc[wave][d] + rank[r];
             s_key[pos] = key[r];
             s_val[pos] = val[r];
@@ -357,7 +378,7 @@ __global__ __launch_bounds__(RB) void k_radix_scatter(const K* __restrict__ kin,
     const uint32_t nvalid = (uint32_t)((n - tbase) < (uint64_t)RT ? (n - tbase) : (uint64_t)RT);
     for (uint32_t i = threadIdx.x; i < nvalid; i += RB) {
         K k = s_key[i];
-        uint32_t d = (uint32_t)((k >> shift) & 255u);
+        uint32_t d = (uint32_t)((k >> shift) & (K)(ND - 1));
         uint32_t g = goff[d] + i;
         kout[g] = k;
         vout[g] = s_val[i];
@@ -371,37 +392,46 @@ __global__ __launch_bounds__(RB) void k_radix_scatter(const K* __restrict__ kin,
 constexpr int CG = 64;  // tiles per column group
 
 __global__ __launch_bounds__(256) void k_col_sums(const uint32_t* __restrict__ hist, uint32_t tiles,
-                                                  uint32_t* __restrict__ gsum) {
+                                                  uint32_t* __restrict__ gsum, uint32_t nd) {
     const uint32_t t0 = blockIdx.x * CG;
+    const uint32_t d = blockIdx.y * 256 + threadIdx.x;  // digit
     uint32_t v[CG];
 #pragma unroll
-    for (int k = 0; k < CG; ++k) v[k] = t0 + k < tiles ? hist[(size_t)(t0 + k) * 256 + threadIdx.x] : 0u;
+    for (int k = 0; k < CG; ++k) v[k] = t0 + k < tiles ? hist[(size_t)(t0 + k) * nd + d] : 0u;
     uint32_t acc = 0;
 #pragma unroll
     for (int k = 0; k < CG; ++k) acc += v[k];
-    gsum[(size_t)blockIdx.x * 256 + threadIdx.x] = acc;
+    gsum[(size_t)blockIdx.x * nd + d] = acc;
 }
 
 // One block: exclusive prefix of each digit over the groups (32 loads in flight per thread: the
 // block walks the groups one latency at a time), and the exclusive scan of the digit totals into
 // row `groups`.
-__global__ __launch_bounds__(256) void k_col_top(uint32_t* __restrict__ gsum, uint32_t groups) {
+__global__ __launch_bounds__(256) void k_col_top(uint32_t* __restrict__ gsum, uint32_t groups, uint32_t nd) {
     __shared__ uint32_t wsum[4];
-    const uint32_t d = threadIdx.x, lane = __lane_id(), wave = threadIdx.x >> 6;
-    uint32_t run = 0;
-    for (uint32_t g0 = 0; g0 < groups; g0 += 32) {
-        uint32_t v[32];
+    const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+    const uint32_t dpt = nd / 256;  // digits per thread: d = threadIdx.x * dpt + j
+    uint32_t tot[8];
+    uint32_t tsum = 0;
+    for (uint32_t j = 0; j < dpt; ++j) {
+        const uint32_t d = threadIdx.x * dpt + j;
+        uint32_t run = 0;
+        for (uint32_t g0 = 0; g0 < groups; g0 += 32) {
+            uint32_t v[32];
 #pragma unroll
-        for (int k = 0; k < 32; ++k) v[k] = gsum[(size_t)(g0 + k < groups ? g0 + k : groups - 1) * 256 + d];
+            for (int k = 0; k < 32; ++k) v[k] = gsum[(size_t)(g0 + k < groups ? g0 + k : groups - 1) * nd + d];
 #pragma unroll
-        for (int k = 0; k < 32; ++k) {
-            if (g0 + k < groups) {
-                gsum[(size_t)(g0 + k) * 256 + d] = run;
-                run += v[k];
+            for (int k = 0; k < 32; ++k) {
+                if (g0 + k < groups) {
+                    gsum[(size_t)(g0 + k) * nd + d] = run;
+                    run += v[k];
+                }
             }
         }
+        tot[j] = run;
+        tsum += run;
     }
-    uint32_t inc = run;
+    uint32_t inc = tsum;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
         uint32_t y = __shfl_up(inc, o, 64);
@@ -411,20 +441,25 @@ __global__ __launch_bounds__(256) void k_col_top(uint32_t* __restrict__ gsum, ui
     __syncthreads();
     uint32_t base = 0;
     for (uint32_t w = 0; w < wave; ++w) base += wsum[w];
-    gsum[(size_t)groups * 256 + d] = base + inc - run;  // keys with a smaller digit
+    uint32_t run = base + inc - tsum;
+    for (uint32_t j = 0; j < dpt; ++j) {
+        gsum[(size_t)groups * nd + threadIdx.x * dpt + j] = run;  // keys with a smaller digit
+        run += tot[j];
+    }
 }
 
 __global__ __launch_bounds__(256) void k_col_offsets(const uint32_t* __restrict__ hist, uint32_t tiles,
                                                      uint32_t groups, const uint32_t* __restrict__ gsum,
-                                                     uint32_t* __restrict__ offs) {
+                                                     uint32_t* __restrict__ offs, uint32_t nd) {
     const uint32_t t0 = blockIdx.x * CG;
+    const uint32_t d = blockIdx.y * 256 + threadIdx.x;  // digit
     uint32_t v[CG];
 #pragma unroll
-    for (int k = 0; k < CG; ++k) v[k] = t0 + k < tiles ? hist[(size_t)(t0 + k) * 256 + threadIdx.x] : 0u;
-    uint32_t run = gsum[(size_t)blockIdx.x * 256 + threadIdx.x] + gsum[(size_t)groups * 256 + threadIdx.x];
+    for (int k = 0; k < CG; ++k) v[k] = t0 + k < tiles ? hist[(size_t)(t0 + k) * nd + d] : 0u;
+    uint32_t run = gsum[(size_t)blockIdx.x * nd + d] + gsum[(size_t)groups * nd + d];
 #pragma unroll
     for (int k = 0; k < CG; ++k) {
-        if (t0 + k < tiles) offs[(size_t)(t0 + k) * 256 + threadIdx.x] = run;
+        if (t0 + k < tiles) offs[(size_t)(t0 + k) * nd + d] = run;
         run += v[k];
     }
 }
@@ -993,7 +1028,17 @@ __global__ __launch_bounds__(BC) __attribute__((amdgpu_waves_per_eu(8, 8))) void
             s_side[slot] = (uint8_t)side;
             // the merge happens in the next round, after a barrier: a relaxed count suffices
             uint32_t old = __hip_atomic_fetch_add(&s_cnt[slot], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (old == 1u) lst_next[atomicAdd(nl_next, 1u)] = (uint16_t)slot;
+            // completed splits join the next list with one counter update per wavefront (lanes
+            // adding to one LDS address one at a time serialised a round's appends)
+            const bool app = old == 1u;
+            const uint64_t am = __ballot(app);
+            if (am) {
+                const int leader = __ffsll((unsigned long long)am) - 1;
+                uint32_t b0 = 0;
+                if ((int)__lane_id() == leader) b0 = atomicAdd(nl_next, (uint32_t)__popcll(am));
+                b0 = __shfl(b0, leader, 64);
+                if (app) lst_next[b0 + (uint32_t)__popcll(am & lanemask_lt())] = (uint16_t)slot;
+            }
         };
         auto gref = [&](uint16_t rr) -> int32_t {
             return (rr & 0x8000u) ? ~(cs + (int32_t)(rr & 0x7fffu)) : cs + (int32_t)rr;
@@ -1097,26 +1142,24 @@ template <typename K, bool VOL>
 __device__ __forceinline__ int64_t bu_step(const K* __restrict__ keys, int64_t n, Node* nodes, Node* meet,
                                            int4* meet_far, uint32_t* flags, int64_t s, SceneInfo* info,
                                            uint32_t gen) {
+    // The two records at s were written by an earlier kernel (visible since this launch began) or,
+    // within one workgroup's levels (VOL), by this workgroup before the barrier that ended the
+    // previous level: waves of a workgroup run on one CU, whose vector L1 they share, so plain
+    // loads see those stores. (System-scope loads here bypassed both cache levels one dependent
+    // access at a time: 20 round trips to memory per merge.)
+    (void)VOL;
     float f[12];
     int dv[4];
     int fr[4];  // far0, far1, dx0, dx1
-    if constexpr (VOL) {
-        const volatile float* src = reinterpret_cast<const volatile float*>(meet + s);
-#pragma unroll
-        for (int q = 0; q < 12; ++q) f[q] = src[q];
-        const volatile int* di = reinterpret_cast<const volatile int*>(&meet[s].d);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) dv[q] = di[q];
-        const volatile int* fi = reinterpret_cast<const volatile int*>(meet_far + s);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) fr[q] = fi[q];
-    } else {
-        const float* src = reinterpret_cast<const float*>(meet + s);
-#pragma unroll
-        for (int q = 0; q < 12; ++q) f[q] = src[q];
-        int4 d = meet[s].d;
+    {
+        const float4* src = reinterpret_cast<const float4*>(meet + s);
+        const float4 a = src[0], b = src[1], c = src[2];
+        const int4 d = meet[s].d;
+        const int4 ff = meet_far[s];
+        f[0] = a.x; f[1] = a.y; f[2] = a.z; f[3] = a.w;
+        f[4] = b.x; f[5] = b.y; f[6] = b.z; f[7] = b.w;
+        f[8] = c.x; f[9] = c.y; f[10] = c.z; f[11] = c.w;
         dv[0] = d.x; dv[1] = d.y; dv[2] = d.z; dv[3] = d.w;
-        int4 ff = meet_far[s];
         fr[0] = ff.x;
         fr[1] = ff.y;
         fr[2] = ff.z;
@@ -1165,7 +1208,7 @@ __device__ void bu_finish(const K* __restrict__ keys, int64_t n, Node* nodes, No
         for (uint32_t base = 0; base < n_cur; base += blockDim.x) {
             uint32_t i = base + threadIdx.x;
             if (i < n_cur) {
-                int64_t s = (int64_t)((const volatile uint32_t*)lin)[i];
+                int64_t s = (int64_t)lin[i];
                 int64_t s2 = bu_step<K, true>(keys, n, nodes, meet, meet_far, flags, s, info, gen);
                 if (s2 >= 0) lout[atomicAdd(&s_out, 1u)] = (uint32_t)s2;
             }
@@ -1240,8 +1283,9 @@ constexpr uint32_t SPAN_LDS = 2048;  // level-list entries a span workgroup keep
 // the splits the level before completed, with one barrier between levels (three counters rotate:
 // read, appended, cleared). A completed split inside [lo, hi - 2] goes to the next level, any
 // other (a boundary split lo - 1 or hi - 1) to the export list xl (count *xn). The level lists
-// are LDS (s_l) when level 0 fits, else the global lists gl0 and gl1. Records are read with
-// volatile loads: the previous level's stores, fenced before the barrier, are in L2.
+// are LDS (s_l) when level 0 fits, else the global lists gl0 and gl1. Records and lists are read
+// with plain loads: the previous level's stores, completed before the barrier, are visible to
+// every wave of the workgroup.
 template <typename K>
 __device__ void span_levels(const K* __restrict__ keys, int64_t n, Node* nodes, Node* meet, int4* meet_far,
                             uint32_t* flags, SceneInfo* info, uint32_t gen, int64_t lo, int64_t hi, uint32_t n0,
@@ -1254,7 +1298,7 @@ __device__ void span_levels(const K* __restrict__ keys, int64_t n, Node* nodes, 
         uint32_t* nc_next = &s_nc[(lv + 1) % 3];
         if (threadIdx.x == 0) s_nc[(lv + 2) % 3] = 0;
         for (uint32_t i = threadIdx.x; i < n_cur; i += blockDim.x) {
-            const int64_t sp = (int64_t)((const volatile uint32_t*)lin)[i];
+            const int64_t sp = (int64_t)lin[i];
             const int64_t s2 = bu_step<K, true>(keys, n, nodes, meet, meet_far, flags, sp, info, gen);
             if (s2 < 0) continue;
             if (s2 >= lo && s2 <= hi - 2)
@@ -1368,26 +1412,27 @@ inline unsigned blocks_for(uint64_t n, unsigned b) { return (unsigned)((n + b - 
 // over `passes` 8-bit digits; the sorted keys end in ka, the sorted values in vfinal (the last
 // pass writes them there). The Morton pass also produces the first digit's tile histograms, and
 // the first pass takes the particle indices as its values.
-template <typename K, int RI>
-int morton_sort_t(mmhrt_ctx_s* c, K* ka, K* kb, uint32_t* va, uint32_t* vb, uint32_t* vfinal, uint64_t n,
-                  int passes) {
-    constexpr int RT = RB * RI;
+template <typename K, int RI, int DB>
+int morton_sort_t(mmhrt_ctx_s* c, K* ka, K* kb, uint32_t* va, uint32_t* vb, uint32_t* vfinal, uint64_t n) {
+    constexpr int RT = RB * RI, ND = 1 << DB;
+    const int passes = ((sizeof(K) == 8 ? 63 : 30) + DB - 1) / DB;
     uint32_t tiles = (uint32_t)((n + RT - 1) / RT);
-    uint64_t hlen = (uint64_t)tiles * 256;
+    uint64_t hlen = (uint64_t)tiles * ND;
     uint32_t groups = (tiles + CG - 1) / CG;
-    const bool os = c->opt_sort_onesweep != 0 && n < (1ull << 30);
+    const bool os = DB == 8 && c->opt_sort_onesweep != 0 && n < (1ull << 30);
     if (int e = ensure(c, c->hist, hlen * (os ? std::max(2, passes) : 2), "radix histogram")) return e;
-    if (int e = ensure(c, c->scan_tmp, std::max((size_t)(groups + 1) * 256, (size_t)passes * 512 + 16),
+    if (int e = ensure(c, c->scan_tmp, std::max((size_t)(groups + 1) * ND, (size_t)passes * 512 + 16),
                        "radix column sums"))
         return e;
     uint32_t* hist = c->hist.p;
     uint32_t* offs = c->hist.p + hlen;
+    // an odd number of passes starts from kb, so that the sorted keys end in ka
+    K* src = (passes & 1) ? kb : ka;
+    K* dst = (passes & 1) ? ka : kb;
     rec(c, 1);
-    hipLaunchKernelGGL((k_morton_tile<K, RI>), dim3(tiles), dim3(RB), 0, c->stream, cur_spheres(c), n, c->info.p, ka,
-                       hist);
+    hipLaunchKernelGGL((k_morton_tile<K, RI, DB>), dim3(tiles), dim3(RB), 0, c->stream, cur_spheres(c), n, c->info.p,
+                       src, hist);
     rec(c, 2);
-    K* src = ka;
-    K* dst = kb;
     uint32_t* vs = nullptr;  // first pass: values are the indices
     uint32_t* vd = vb;
     if (os) {
@@ -1410,39 +1455,45 @@ int morton_sort_t(mmhrt_ctx_s* c, K* ka, K* kb, uint32_t* va, uint32_t* vb, uint
             hipLaunchKernelGGL((k_radix_onesweep<K, RI>), dim3(tiles), dim3(RB), 0, c->stream, src, vs, dst,
                                p == passes - 1 ? vfinal : vd, n, 8 * p, dstart + p * 256, c->hist.p + hlen * p,
                                tickets + p, c->sort_fault.p);
-            src = dst;
-            dst = (p & 1) ? kb : ka;
+            std::swap(src, dst);
             vs = vd;
-            vd = (p & 1) ? vb : va;
+            vd = vd == vb ? va : vb;
         }
         return 0;
     }
     for (int p = 0; p < passes; ++p) {
-        int shift = 8 * p;
+        int shift = DB * p;
         if (p > 0)
-            hipLaunchKernelGGL((k_radix_hist<K, RI>), dim3(tiles), dim3(RB), 0, c->stream, src, n, shift, hist);
-        hipLaunchKernelGGL(k_col_sums, dim3(groups), dim3(256), 0, c->stream, hist, tiles, c->scan_tmp.p);
-        hipLaunchKernelGGL(k_col_top, dim3(1), dim3(256), 0, c->stream, c->scan_tmp.p, groups);
-        hipLaunchKernelGGL(k_col_offsets, dim3(groups), dim3(256), 0, c->stream, hist, tiles, groups, c->scan_tmp.p,
-                           offs);
+            hipLaunchKernelGGL((k_radix_hist<K, RI, DB>), dim3(tiles), dim3(RB), 0, c->stream, src, n, shift, hist);
+        hipLaunchKernelGGL(k_col_sums, dim3(groups, ND / 256), dim3(256), 0, c->stream, hist, tiles, c->scan_tmp.p,
+                           (uint32_t)ND);
+        hipLaunchKernelGGL(k_col_top, dim3(1), dim3(256), 0, c->stream, c->scan_tmp.p, groups, (uint32_t)ND);
+        hipLaunchKernelGGL(k_col_offsets, dim3(groups, ND / 256), dim3(256), 0, c->stream, hist, tiles, groups,
+                           c->scan_tmp.p, offs, (uint32_t)ND);
         const bool xcd = c->opt_sort_xcd != 0;
-        hipLaunchKernelGGL((k_radix_scatter<K, RI>), dim3(xcd ? 8 * ((tiles + 7) / 8) : tiles), dim3(RB), 0,
+        hipLaunchKernelGGL((k_radix_scatter<K, RI, DB>), dim3(xcd ? 8 * ((tiles + 7) / 8) : tiles), dim3(RB), 0,
                            c->stream, src, vs, dst, p == passes - 1 ? vfinal : vd, n, shift, hist, offs, (int)xcd);
-        // ping-pong: pass p writes (kb, vb) when p is even, (ka, va) when odd
-        src = dst;
-        dst = (p & 1) ? kb : ka;
+        std::swap(src, dst);
         vs = vd;
-        vd = (p & 1) ? vb : va;
+        vd = vd == vb ? va : vb;
     }
-    return 0;  // passes is even: the keys are back in ka
+    return 0;  // the keys are in ka
+}
+
+// Digits of 8 bits (option sort_digit 8: 4 passes for 30-bit keys, 8 for 63-bit), or wide digits
+// (the default: 3 passes of 10 bits, 6 passes of 11 bits).
+template <typename K, int RI>
+int morton_sort_d(mmhrt_ctx_s* c, K* ka, K* kb, uint32_t* va, uint32_t* vb, uint32_t* vfinal, uint64_t n) {
+    if (c->opt_sort_digit == 8 || c->opt_sort_onesweep != 0)
+        return morton_sort_t<K, RI, 8>(c, ka, kb, va, vb, vfinal, n);
+    return morton_sort_t<K, RI, sizeof(K) == 8 ? 11 : 10>(c, ka, kb, va, vb, vfinal, n);
 }
 
 template <typename K>
-int morton_sort(mmhrt_ctx_s* c, K* ka, K* kb, uint32_t* va, uint32_t* vb, uint32_t* vfinal, uint64_t n,
-                int passes) {
-    if (c->opt_sort_ri == 8) return morton_sort_t<K, 8>(c, ka, kb, va, vb, vfinal, n, passes);
-    if (c->opt_sort_ri == 32 && sizeof(K) == 4) return morton_sort_t<K, 32>(c, ka, kb, va, vb, vfinal, n, passes);
-    return morton_sort_t<K, 16>(c, ka, kb, va, vb, vfinal, n, passes);
+int morton_sort(mmhrt_ctx_s* c, K* ka, K* kb, uint32_t* va, uint32_t* vb, uint32_t* vfinal, uint64_t n) {
+    if (c->opt_sort_ri == 8) return morton_sort_d<K, 8>(c, ka, kb, va, vb, vfinal, n);
+    if (c->opt_sort_ri == 32 && sizeof(K) == 4) return morton_sort_d<K, 32>(c, ka, kb, va, vb, vfinal, n);
+    return morton_sort_d<K, 16>(c, ka, kb, va, vb, vfinal, n);
 }
 
 // Persistent grid of the chunk pass: as many workgroups as fit on the device at once.
@@ -1568,8 +1619,7 @@ template <typename K>
 int build_typed(mmhrt_ctx_s* c, K* ka, K* kb) {
     uint64_t n = c->n;
     hipStream_t s = c->stream;
-    int passes = sizeof(K) == 8 ? 8 : 4;
-    if (int e = morton_sort<K>(c, ka, kb, c->va.p, c->vb.p, c->leaf_id.p, n, passes)) return e;
+    if (int e = morton_sort<K>(c, ka, kb, c->va.p, c->vb.p, c->leaf_id.p, n)) return e;
     rec(c, 3);
     rec(c, 4);  // the leaf gather runs inside the hierarchy pass
     // the sort's last pass wrote its values (the leaf -> particle map) straight into leaf_id
@@ -1633,8 +1683,8 @@ int sort_spheres(mmhrt_ctx_s* c, int bits) {
     }
     rec(c, 0);
     if ((e = reduce_scene(c))) return e;
-    if (bits == 63) return morton_sort<uint64_t>(c, c->kla.p, c->klb.p, c->va.p, c->vb.p, c->leaf_id.p, n, 8);
-    return morton_sort<uint32_t>(c, c->ka.p, c->kb.p, c->va.p, c->vb.p, c->leaf_id.p, n, 4);
+    if (bits == 63) return morton_sort<uint64_t>(c, c->kla.p, c->klb.p, c->va.p, c->vb.p, c->leaf_id.p, n);
+    return morton_sort<uint32_t>(c, c->ka.p, c->kb.p, c->va.p, c->vb.p, c->leaf_id.p, n);
 }
 
 int build_lbvh(mmhrt_ctx_s* c, int bits) {

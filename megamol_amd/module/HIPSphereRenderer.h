@@ -33,6 +33,9 @@ public:
     HIPSphereRenderer();
     ~HIPSphereRenderer() override;
 
+    // Index the next Render blends at (0: a fresh frame); AbstractRenderer.cpp:108 advances it.
+    uint32_t ProgressiveFrameIndex() const { return frame_idx_; }
+
 protected:
     bool create() override;
     void release() override;

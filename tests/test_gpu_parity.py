@@ -59,14 +59,15 @@ def leaf_pair_form(ochild):
     return child, pairs, ~is_pair
 
 
-@pytest.mark.parametrize("chunk,ls,ri,grid,os1,wave,tail,span", [
-    (1024, 1, 16, 0, 0, 64, 512, 256), (512, 1, 8, 0, 0, 64, 512, 256), (256, 1, 32, 0, 0, 64, 512, 256),
-    (1024, 0, 16, 0, 0, 64, 512, 256), (256, 0, 8, 0, 0, 64, 512, 256), (1024, 1, 32, 3, 0, 64, 512, 256),
-    (256, 1, 16, 1, 0, 64, 512, 256), (512, 1, 32, 7, 0, 64, 512, 256), (1024, 1, 32, 0, 1, 64, 512, 256),
-    (1024, 1, 16, 0, 1, 64, 512, 256), (512, 1, 8, 5, 1, 64, 512, 256), (1024, 1, 32, 0, 0, 0, 512, 256),
-    (256, 1, 16, 0, 0, 32, 1 << 30, 0), (1024, 1, 32, 0, 0, 64, 4096, 0), (1024, 1, 32, 0, 0, 64, 512, 0),
-    (256, 1, 16, 0, 0, 64, 512, 1), (1024, 0, 8, 2, 0, 64, 512, 16)])
-def test_lbvh_equals_oracle_lbvh(gpu_renderer, chunk, ls, ri, grid, os1, wave, tail, span):
+@pytest.mark.parametrize("chunk,ls,ri,grid,os1,wave,tail,span,dig", [
+    (1024, 1, 16, 0, 0, 64, 512, 256, 0), (512, 1, 8, 0, 0, 64, 512, 256, 0), (256, 1, 32, 0, 0, 64, 512, 256, 0),
+    (1024, 0, 16, 0, 0, 64, 512, 256, 0), (256, 0, 8, 0, 0, 64, 512, 256, 0), (1024, 1, 32, 3, 0, 64, 512, 256, 0),
+    (256, 1, 16, 1, 0, 64, 512, 256, 0), (512, 1, 32, 7, 0, 64, 512, 256, 0), (1024, 1, 32, 0, 1, 64, 512, 256, 0),
+    (1024, 1, 16, 0, 1, 64, 512, 256, 0), (512, 1, 8, 5, 1, 64, 512, 256, 0), (1024, 1, 32, 0, 0, 0, 512, 256, 0),
+    (256, 1, 16, 0, 0, 32, 1 << 30, 0, 0), (1024, 1, 32, 0, 0, 64, 4096, 0, 0), (1024, 1, 32, 0, 0, 64, 512, 0, 0),
+    (256, 1, 16, 0, 0, 64, 512, 1, 0), (1024, 0, 8, 2, 0, 64, 512, 16, 0), (1024, 1, 16, 0, 0, 64, 512, 256, 8),
+    (512, 1, 32, 0, 0, 64, 512, 256, 8), (256, 1, 8, 0, 0, 64, 512, 256, 8)])
+def test_lbvh_equals_oracle_lbvh(gpu_renderer, chunk, ls, ri, grid, os1, wave, tail, span, dig):
     """Both chunk passes (level-synchronous rounds, or one climbing thread per leaf) at every
     chunk size, over radix tiles of 8, 16 or 32 keys per thread, with either radix pass
     (histogram + scans + scatter, or single-pass with look-back: os1), give the oracle's tree
@@ -75,7 +76,10 @@ def test_lbvh_equals_oracle_lbvh(gpu_renderer, chunk, ls, ri, grid, os1, wave, t
     pipeline; wave is the round size up to which one wavefront runs the rounds alone (0: never);
     tail is the straddler list size from which one workgroup finishes the tree (1 << 30: it takes
     every straddler from the chunk pass on) when span is 0 (kernel rounds); span > 0 merges the
-    straddlers with one workgroup per span of that many chunks before the tail."""
+    straddlers with one workgroup per span of that many chunks before the tail. dig is the radix
+    digit width: 0 the wide digits (3 passes of 10 bits for 30-bit keys, 6 of 11 bits for 63-bit
+    keys), 8 the 8-bit passes (single-pass radix passes always take 8-bit digits)."""
+    gpu_renderer.set_option("sort_digit", dig)
     gpu_renderer.set_option("build_chunk", chunk)
     gpu_renderer.set_option("build_ls", ls)
     gpu_renderer.set_option("sort_ri", ri)
@@ -87,7 +91,7 @@ def test_lbvh_equals_oracle_lbvh(gpu_renderer, chunk, ls, ri, grid, os1, wave, t
     for bits in (30, 63):
         for n, dup in [(1000, False), (50000, True), (2, False), (3, True), (4097, False), (1_200_000, False)]:
             case(f"lbvh n={n} dup={dup} bits={bits} chunk={chunk} ls={ls} ri={ri} grid={grid} onesweep={os1} "
-                 f"wave={wave} tail={tail} span={span}")
+                 f"wave={wave} tail={tail} span={span} digit={dig}")
             g = np.random.default_rng(n + bits)
             xyzr = np.zeros((n, 4), np.float32)
             xyzr[:, :3] = g.random((n, 3)).astype(np.float32)
@@ -109,6 +113,7 @@ def test_lbvh_equals_oracle_lbvh(gpu_renderer, chunk, ls, ri, grid, os1, wave, t
             assert height[0] >= int(np.ceil(np.log2(n)))
             assert gpu_renderer.get_option("sort_fault") == 0
     gpu_renderer.set_option("sort_onesweep", 0)
+    gpu_renderer.set_option("sort_digit", 0)
     gpu_renderer.set_option("build_chunk", 1024)
     gpu_renderer.set_option("build_ls", 1)
     gpu_renderer.set_option("sort_ri", 16)
@@ -345,3 +350,31 @@ def test_device_framebuffer_is_the_downloaded_frame(gpu_renderer):
     np.testing.assert_array_equal(get(d.depth, np.float32, n).reshape(50, 70).view(np.uint32),
                                   fr.depth.view(np.uint32))
     np.testing.assert_array_equal(get(d.aux, np.uint16, 8 * n).reshape(50, 70, 8), aux.view(np.uint16))
+
+
+def test_allocation_failure_is_reported():
+    """An allocation failure (injected from the host with the fail_alloc hook) returns E_OOM with
+    the buffer named, leaves the context usable, and the next attempt succeeds."""
+    from megamol_amd import SphereRenderer
+    case("injected allocation failure")
+    xyzr, rgba = scenes.random_spheres(5000, seed=3)
+    cam = small_camera(32, 24)
+    p = params(32, 24, spp=1, mode=L.MODE_AO, ao_distance=0.2)
+    with SphereRenderer(0) as r:
+        for k in (1, 2, 3):  # the first, second or third allocation of the upload fails
+            r.set_option("fail_alloc", k)
+            with pytest.raises(L.MMHRTError) as e:
+                r.set_spheres(xyzr, rgba)
+                r.build(L.BUILD_REBUILD, 30)
+                r.render(cam, p)
+            assert e.value.code == L.E_OOM and "injected" in str(e.value)
+            assert r.get_option("fail_alloc") == 0
+        r.set_spheres(xyzr, rgba)
+        r.set_option("fail_alloc", 1)  # the build's first allocation
+        with pytest.raises(L.MMHRTError) as e:
+            r.build(L.BUILD_REBUILD, 30)
+        assert e.value.code == L.E_OOM
+        r.build(L.BUILD_REBUILD, 30)
+        g = r.render(cam, p)
+        o = O.render_brute(xyzr, rgba, cam, oracle_params(p))
+        assert_frames_equal(g, o, what="after injected failures")
