@@ -209,6 +209,22 @@ MMHRT_API int32_t mmhrt_download(mmhrt_ctx ctx, mmhrt_framebuffer* out);
  * mmhrt_destroy; the frame may still be in flight on `stream`. */
 MMHRT_API int32_t mmhrt_get_device_framebuffer(mmhrt_ctx ctx, mmhrt_device_framebuffer* out);
 
+/* Render one frame into device buffers the caller owns, allocating nothing. No counterpart in the
+ * reference: for callers that cannot allocate device memory (work under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). The frame is the one mmhrt_render produces
+ * (same kernels, same bits); out holds device pointers to colour (float4), depth, t (float),
+ * hit_id (int32), rgba8 (uint32) and, with MMHRT_FLAG_AUX, aux (8 fp16 per pixel) of
+ * out->width x out->height pixels, which must be the render window (stream is not read). With
+ * accumulate and frame_idx > 0 the colour buffer must hold the previous frame. The call is
+ * asynchronous on the context's stream (mmhrt_sync waits for it); it needs a build, and the
+ * scene, the structure and out must stay alive until it completes. The trace's scratch is a
+ * fixed device array of 62.9 MB per GPU (tile counter, flags and the traversal stacks of at most
+ * 131,072 persistent lanes: the persistent grid is capped to two workgroups per CU, so this entry
+ * point traces slower than mmhrt_render), held until the process ends and shared by every call on
+ * that GPU: calls that overlap on one GPU race. Frames larger than 2^31 pixels are rejected. */
+MMHRT_API int32_t mmhrt_render_noalloc(mmhrt_ctx ctx, const mmhrt_camera* cam, const mmhrt_render_params* params,
+                                       const mmhrt_device_framebuffer* out);
+
 /* Wait for all queued work on the context's stream. */
 MMHRT_API int32_t mmhrt_sync(mmhrt_ctx ctx);
 

@@ -85,6 +85,7 @@ SIGNATURES = {
     "mmhrt_sync": (C.c_int32, [C.c_void_p]),
     "mmhrt_get_info": (C.c_int32, [C.c_void_p, C.POINTER(C.c_uint64), C.c_void_p, C.POINTER(C.c_uint64)]),
     "mmhrt_get_bvh": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "mmhrt_render_noalloc": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "mmhrt_get_treelets": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                        C.c_void_p, C.c_void_p]),
     "mmhrt_get_timings": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32]),

@@ -145,6 +145,10 @@ int32_t mmhrt_create(int32_t device, mmhrt_ctx* out) {
     if (c->cu_count <= 0) c->cu_count = 1;
     for (int i = 0; i < 12; ++i) hipEventCreate(&c->tm.ev[i]);
     c->tm.init = true;
+    // the device address of this device's scratch array for mmhrt_render_noalloc; on failure that
+    // entry point reports the reason and everything else works as before
+    if (noalloc_init(c) != 0) c->noalloc_err = c->err;
+    c->err.clear();
     // pinned words through which a build tells the next one its straddler rounds (all ones: none yet)
     if (hipHostMalloc((void**)&c->rounds_host, BOX_SLOTS * sizeof(uint32_t), hipHostMallocDefault) == hipSuccess)
         memset(c->rounds_host, 0xff, BOX_SLOTS * sizeof(uint32_t));
@@ -369,6 +373,28 @@ int32_t mmhrt_render(mmhrt_ctx c, const mmhrt_camera* cam, const mmhrt_render_pa
     for (int i = 0; i < 4; ++i) c->last_bg[i] = p->background[i];
     if (out && !(p->flags & MMHRT_FLAG_NO_DOWNLOAD)) return mmhrt_download(c, out);
     return MMHRT_OK;
+}
+
+int32_t mmhrt_render_noalloc(mmhrt_ctx c, const mmhrt_camera* cam, const mmhrt_render_params* p,
+                             const mmhrt_device_framebuffer* out) {
+    if (int e = check_ctx(c)) return e;
+    if (int e = check_render_args(c, cam, p, "render_noalloc")) return e;
+    if (!c->noalloc_err.empty()) return fail(c, MMHRT_E_STATE, "render_noalloc: " + c->noalloc_err);
+    if (!out || !out->color || !out->depth || !out->t || !out->hit_id || !out->rgba8)
+        return fail(c, MMHRT_E_INVALID, "render_noalloc: out needs device colour, depth, t, hit id and rgba8 buffers");
+    if ((p->flags & MMHRT_FLAG_AUX) && !out->aux)
+        return fail(c, MMHRT_E_INVALID, "render_noalloc: MMHRT_FLAG_AUX needs an aux buffer");
+    const bool win = p->window[2] > 0;
+    const uint32_t ow = win ? p->window[2] : p->width, oh = win ? p->window[3] : p->height;
+    if (out->width != ow || out->height != oh)
+        return fail(c, MMHRT_E_INVALID, "render_noalloc: out->width x out->height must be the render window (" +
+                                            std::to_string(ow) + "x" + std::to_string(oh) + ")");
+    FrameConsts f;
+    frame_consts(cam, p, f);
+    FrameOutputs o{static_cast<float4*>(out->color), static_cast<float*>(out->depth), static_cast<float*>(out->t),
+                   static_cast<int32_t*>(out->hit_id), static_cast<uint32_t*>(out->rgba8),
+                   static_cast<__half*>(out->aux)};
+    return render_noalloc(c, f, o);
 }
 
 int32_t mmhrt_download(mmhrt_ctx c, mmhrt_framebuffer* out) {

@@ -58,6 +58,7 @@ struct mmhrt_ctx_s {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
+    std::string noalloc_err;  // why mmhrt_render_noalloc cannot run on this device ("" = it can)
     int cu_count = 0;
 
     // scene (original particle order)
@@ -255,6 +256,11 @@ int render_frame(mmhrt_ctx_s* c, const FrameConsts& fc);
 int render_frame_into(mmhrt_ctx_s* c, const FrameConsts& fc, const FrameOutputs& out, uint32_t* ctr, int32_t* spill,
                       uint32_t lanes, bool count);
 uint32_t tile_log2(int64_t tw);
+
+// noalloc.hip
+int noalloc_init(mmhrt_ctx_s* c);
+int render_noalloc(mmhrt_ctx_s* c, const FrameConsts& fc, const FrameOutputs& out);
+size_t noalloc_scratch_bytes();
 size_t batch_job_size();
 int batch_fill_job(mmhrt_ctx_s* c, const FrameConsts& fc, size_t out_offset, void* dst);
 void launch_trace_batch(mmhrt_ctx_s* c, const void* jobs, const uint32_t* prefix, uint32_t n_jobs, uint32_t* ctr,
