@@ -147,6 +147,38 @@ def run_ablation(r, cam, args):
                                                  f"{args.ao_distance}"}), flush=True)
 
 
+BUILD_SWEEP = {"build_chunk": (1024, 512, 256), "sort_ri": (8, 16, 32), "sort_digit": (0, 8), "build_wave": (64, 0),
+               "leaf_pairs": (1, 0)}
+
+
+def run_build_sweep(r, args):
+    """LBVH build time (HIP events, median of `steps` rebuilds) for each build option changed one at a time from
+    the library defaults, on the loaded scene."""
+    from megamol_amd import _lib as L
+    defaults = {k: r.get_option(k) for k in BUILD_SWEEP}
+    rows = []
+
+    def measure():
+        r.build(L.BUILD_REBUILD, args.bits)
+        r.sync()
+        t = []
+        for _ in range(max(3, args.steps)):
+            r.build(L.BUILD_REBUILD, args.bits)
+            t.append(r.timings())
+        return {k: round(float(np.median([x[k] for x in t])), 3) for k in ("build", "morton", "sort", "boxes", "chunk")}
+    rows.append({"options": dict(defaults), **measure()})
+    log(f"build defaults {defaults}: {rows[-1]}")
+    for k, vals in BUILD_SWEEP.items():
+        for v in vals:
+            if v == defaults[k]:
+                continue
+            r.set_option(k, v)
+            rows.append({"options": {k: v}, **measure()})
+            log(f"build {k}={v}: {rows[-1]}")
+            r.set_option(k, defaults[k])
+    print(json.dumps({"build_sweep": rows, "spheres": r.info()[0], "bits": args.bits}), flush=True)
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -405,6 +437,59 @@ def denoised_leg(r, scene, args, cam, p_plain, plain_cfg=None):
     return res
 
 
+def pipelined_frames(scene, args, cam, opts_plain, opts_dn, weights, depth_scale):
+    """Frames in flight: K contexts on the one GPU, each with the scene resident and its own stream,
+    take the frames of a sequence in turn, so that one frame's LBVH build (latency-bound rounds)
+    runs beside another's trace and denoiser. Returns ms per frame for the plain 64-spp frame and
+    the denoised 1-spp frame, each frame still a full rebuild + trace (+ denoise)."""
+    from megamol_amd import SphereRenderer, _lib as L, params
+    from megamol_amd.denoiser import Denoiser
+
+    K = args.pipelined
+    ctxs = []
+    for _ in range(K):
+        r = SphereRenderer(0)
+        if scene["tf"] is not None:
+            r.set_transfer_function(scene["tf"])
+        r.set_lists(scene["lists"])
+        d = Denoiser(r, seed=1234)
+        if weights:
+            d.load(weights)
+        ctxs.append((r, d))
+    p64 = params(args.width, args.height, spp=args.spp, mode=L.MODE_AO, ao_distance=args.ao_distance,
+                 background=(0.0, 0.0, 0.0, 1.0), flags=L.FLAG_NO_DOWNLOAD)
+    p1 = params(args.width, args.height, spp=1, mode=L.MODE_AO, ao_distance=args.ao_distance,
+                flags=L.FLAG_AUX | L.FLAG_NO_DOWNLOAD)
+    out = {"frames_in_flight": K}
+    for name, p, opts, dn in (("plain", p64, opts_plain, False), ("denoised", p1, opts_dn, True)):
+        for r, _ in ctxs:
+            for k, v in (opts or {}).items():
+                r.set_option(k, v)
+
+        def frame(i):
+            r, d = ctxs[i % K]
+            r.build(L.BUILD_REBUILD, args.bits)
+            r.render_async(cam, p)
+            if dn:
+                d.denoise_frame(depth_scale)
+        for i in range(2 * K):
+            frame(i)
+        for r, _ in ctxs:
+            r.sync()
+        n = max(args.steps, 4) * K
+        t0 = time.perf_counter()
+        for i in range(n):
+            frame(i)
+        for r, _ in ctxs:
+            r.sync()
+        out[f"{name}_ms_per_frame"] = round(1e3 * (time.perf_counter() - t0) / n, 3)
+        log(f"pipelined {name}: {out[f'{name}_ms_per_frame']} ms/frame with {K} frames in flight")
+    out["plain_over_denoised"] = round(out["plain_ms_per_frame"] / out["denoised_ms_per_frame"], 2)
+    for r, _ in ctxs:
+        r.close()
+    return out
+
+
 def structure_compare(r, args, cam):
     """The LBVH against the treelet P-k-d trees on the same scene: build time and 1-spp AO trace
     time (HIP events), node visits per ray, and the device memory each structure keeps beyond
@@ -537,6 +622,10 @@ def main():
     ap.add_argument("--ablation", action="store_true",
                     help="only trace the 64-spp and 1-spp AO frames once per traversal variant (one choice "
                          "changed at a time) and print their Mrays/s")
+    ap.add_argument("--pipelined", type=int, default=0, metavar="K",
+                    help="also time both frames with K frames in flight (K contexts on the GPU, one stream each)")
+    ap.add_argument("--build-sweep", action="store_true",
+                    help="only time the LBVH build with each build option changed one at a time")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB five times (calibrates the HBM "
                          "byte counters of a profiler against a kernel of known traffic)")
@@ -585,6 +674,10 @@ def main():
     cam = scene["cam"]
     if args.ablation:
         run_ablation(r, cam, args)
+        r.close()
+        return
+    if args.build_sweep:
+        run_build_sweep(r, args)
         r.close()
         return
 
@@ -678,6 +771,14 @@ def main():
         dn["target_8x_at_38db"] = bool(dn["plain_over_denoised"] >= 8.0 and
                                        dn["psnr_db_vs_256spp"]["denoised_1spp"] >= 38.0)
 
+    pipelined = None
+    if args.pipelined > 1 and dn is not None:
+        lo, hi = scene["xyzr"][:, :3].min(0), scene["xyzr"][:, :3].max(0)
+        ds = float(np.linalg.norm(np.array(cam.position[:]) - 0.5 * (lo + hi)))
+        opts_dn = (dn.get("autotune_1spp") or {}).get("config")
+        pipelined = pipelined_frames(scene, args, cam, tuned["config"] if tuned else None, opts_dn,
+                                     args.weights if args.weights and os.path.exists(args.weights) else None, ds)
+
     structures = None
     if args.structures:
         structures = structure_compare(r, args, cam)
@@ -760,6 +861,7 @@ def main():
         "cpu_baseline": cpu,
         "orbit": orbit,
         "structures": structures,
+        "pipelined": pipelined,
     }
     print(json.dumps(out), flush=True)
     r.close()

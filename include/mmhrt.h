@@ -323,6 +323,8 @@ MMHRT_API int32_t mmhrt_denoiser_train_step(mmhrt_ctx ctx, const uint16_t* feats
  * pass, default 1; 0: one climbing thread per leaf), "build_grid" (test hook: at most this many
  * persistent chunk workgroups, 0 = as many as fit), "sort_ri" (radix sort keys per thread per
  * tile: 8, 16 or 32, default 16), "sort_onesweep" (single-pass radix passes with look-back, default 0),
+ * "sort_digit" (radix digit width: 8, the default, or 0 for wide digits: three 10-bit passes over
+ * 30-bit keys, six 11-bit passes over 63-bit keys),
  * "leaf_pairs" (default 1: nodes over two leaves are not stored, their parents reference the
  * pair; 0: every node is stored), "build_tail" (straddler worklist size from which one
  * workgroup finishes the tree, default 512; used when "build_span" is 0), "build_span" (straddlers

@@ -135,7 +135,7 @@ struct mmhrt_ctx_s {
     int64_t opt_build_ls = 1;       // level-synchronous LBVH chunk pass (0: one climbing thread per leaf)
     int64_t opt_sort_ri = 16;       // radix sort keys per thread per tile: 8, 16 or 32 (32 for 32-bit keys only)
     int64_t opt_sort_onesweep = 0;  // single-pass radix passes with decoupled look-back (0: histogram + scans + scatter)
-    int64_t opt_sort_digit = 0;     // radix digit width: 0 wide (10 bits for 30-bit keys, 11 for 63-bit), 8
+    int64_t opt_sort_digit = 8;     // radix digit width: 8, or 0 for wide digits (10 bits for 30-bit keys, 11 for 63-bit)
     int64_t opt_build_grid = 0;     // test hook: cap on the persistent chunk workgroups (0 = as many as fit)
     int64_t opt_leaf_pairs = 1;     // nodes over two leaves are not stored (their parents reference the pair)
     int64_t opt_build_tail = 512;   // straddler worklist size from which one workgroup finishes the tree

@@ -46,13 +46,14 @@ class OrcList(C.Structure):
 
 class OrcBvh(C.Structure):
     _fields_ = [("n", C.c_uint64), ("node_box", C.c_void_p), ("node_child", C.c_void_p), ("leaf_id", C.c_void_p),
-                ("keys", C.c_void_p), ("leaf_sph", C.c_void_p)]
+                ("keys", C.c_void_p), ("leaf_sph", C.c_void_p), ("xyzr_in", C.c_void_p), ("amax", C.c_float)]
 
 
 class OrcPkd(C.Structure):
     _fields_ = [("n", C.c_uint64), ("M", C.c_uint32), ("T", C.c_uint32), ("top_leaves", C.c_uint32),
                 ("sph", C.c_void_p), ("id", C.c_void_p), ("dim", C.c_void_p), ("tl_box", C.c_void_p),
-                ("tl_rmax", C.c_void_p), ("top_box", C.c_void_p), ("top_child", C.c_void_p)]
+                ("tl_rmax", C.c_void_p), ("top_box", C.c_void_p), ("top_child", C.c_void_p), ("xyzr_in", C.c_void_p),
+                ("amax", C.c_float)]
 
 
 _lib = None

@@ -1138,7 +1138,7 @@ typedef struct build_job {
     const float* xyzr;
     uint64_t n, m;
     int bits, shift;
-    float tlo[256][3], thi[256][3];
+    float tlo[256][3], thi[256][3], tamax[256];
     float lo[3], scale;
     uint32_t maxq;
     uint64_t *key, *key2;
@@ -1153,13 +1153,16 @@ static void bj_bounds(void* a, int t, int T) {
     build_job* J = a;
     PAR_RANGE(J->n, t, T, i0, i1);
     float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    float am = 0.0f;
     for (uint64_t i = i0; i < i1; ++i)
         for (int ax = 0; ax < 3; ++ax) {
             lo[ax] = fminf(lo[ax], J->xyzr[4 * i + ax]);
             hi[ax] = fmaxf(hi[ax], J->xyzr[4 * i + ax]);
+            am = fmaxf(am, fabsf(J->xyzr[4 * i + ax]) + fabsf(J->xyzr[4 * i + 3]));
         }
     memcpy(J->tlo[t], lo, sizeof lo);
     memcpy(J->thi[t], hi, sizeof hi);
+    J->tamax[t] = am;
 }
 static void bj_keys(void* a, int t, int T) {
     build_job* J = a;
@@ -1269,13 +1272,17 @@ int orc_bvh_build_mt(const float* xyzr, uint64_t n, int bits, int threads, orc_b
     J->bits = bits;
     J->out = out;
     out->n = n;
+    out->xyzr_in = xyzr;
     par_run(T, bj_bounds, J);
     float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
-    for (int t = 0; t < T; ++t)
+    out->amax = 0.0f;
+    for (int t = 0; t < T; ++t) {
         for (int ax = 0; ax < 3; ++ax) {
             lo[ax] = fminf(lo[ax], J->tlo[t][ax]);
             hi[ax] = fmaxf(hi[ax], J->thi[t][ax]);
         }
+        out->amax = fmaxf(out->amax, J->tamax[t]);
+    }
     float ext = fmaxf(fmaxf(hi[0] - lo[0], hi[1] - lo[1]), hi[2] - lo[2]);
     float cells = bits == 63 ? 2097152.0f : 1024.0f;
     memcpy(J->lo, lo, sizeof lo);
@@ -1396,15 +1403,9 @@ static int render_mt(const orc_bvh* b, const orc_pkd* pk, const float* rgba, con
                      int32_t* id_out, uint16_t* aux, uint64_t* counters) {
     orc_frame_state fs;
     orc_frame_state_from_camera(cam, &fs);
-    const uint64_t n = pk ? pk->n : b->n;
-    const float* ls = pk ? pk->sph : b->leaf_sph;
-    const uint32_t* lid = pk ? pk->id : b->leaf_id;
-    /* particle positions in original order, rebuilt from leaf order for shading lookups */
-    float* xyzr = malloc(n * 16);
-    for (uint64_t k = 0; k < n; ++k) memcpy(xyzr + 4 * (size_t)lid[k], ls + 4 * k, 16);
-    float amax = 0.0f;
-    for (uint64_t k = 0; k < n; ++k)
-        for (int a = 0; a < 3; ++a) amax = fmaxf(amax, fabsf(ls[4 * k + a]) + fabsf(ls[4 * k + 3]));
+    /* particle positions in original order (for shading lookups): the build's input */
+    const float* xyzr = pk ? pk->xyzr_in : b->xyzr_in;
+    const float amax = pk ? pk->amax : b->amax;
     float cmax = fmaxf(fmaxf(fabsf(cam->position[0]), fabsf(cam->position[1])), fabsf(cam->position[2]));
     mt_job J;
     memset(&J, 0, sizeof J);
@@ -1437,7 +1438,7 @@ static int render_mt(const orc_bvh* b, const orc_pkd* pk, const float* rgba, con
         counters[0] = atomic_load(&J.visits);
         counters[1] = atomic_load(&J.tests);
     }
-    free(xyzr);
+    
     return 0;
 }
 
@@ -1562,6 +1563,8 @@ int orc_pkd_build(const orc_bvh* b, uint32_t M, int threads, orc_pkd* out) {
     memset(out, 0, sizeof *out);
     if (b->n == 0 || M < 2 || (M & (M - 1))) return -1;
     out->n = b->n;
+    out->xyzr_in = b->xyzr_in;
+    out->amax = b->amax;
     out->M = M;
     out->T = (uint32_t)((b->n + M - 1) / M);
     uint32_t L = 2;

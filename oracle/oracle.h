@@ -53,6 +53,8 @@ typedef struct orc_bvh {
     uint32_t* leaf_id;   /* n: particle index of leaf k */
     uint64_t* keys;      /* n: Morton key of leaf k */
     float* leaf_sph;     /* n * 4: spheres in leaf order */
+    const float* xyzr_in; /* the spheres the build read, in particle order (not owned: the caller keeps them) */
+    float amax;           /* max over spheres of |coordinate| + radius (the slab padding's scale) */
 } orc_bvh;
 
 /* Treelets of P-k-d trees (plugins/optix_owl/src/TreeletsCreate.h:117-166, PKDCreate.h:86-160):
@@ -72,6 +74,8 @@ typedef struct orc_pkd {
     float* tl_rmax;      /* T: largest radius in each treelet */
     float* top_box;      /* (top_leaves-1) * 12: child0 lo xyz hi xyz, child1 lo xyz hi xyz */
     int32_t* top_child;  /* (top_leaves-1) * 2: >= 0 top node, < 0 treelet ~t, INT32_MIN absent */
+    const float* xyzr_in; /* as orc_bvh */
+    float amax;
 } orc_pkd;
 
 #ifdef __cplusplus

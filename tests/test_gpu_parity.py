@@ -60,13 +60,13 @@ def leaf_pair_form(ochild):
 
 
 @pytest.mark.parametrize("chunk,ls,ri,grid,os1,wave,tail,span,dig", [
-    (1024, 1, 16, 0, 0, 64, 512, 256, 0), (512, 1, 8, 0, 0, 64, 512, 256, 0), (256, 1, 32, 0, 0, 64, 512, 256, 0),
-    (1024, 0, 16, 0, 0, 64, 512, 256, 0), (256, 0, 8, 0, 0, 64, 512, 256, 0), (1024, 1, 32, 3, 0, 64, 512, 256, 0),
-    (256, 1, 16, 1, 0, 64, 512, 256, 0), (512, 1, 32, 7, 0, 64, 512, 256, 0), (1024, 1, 32, 0, 1, 64, 512, 256, 0),
-    (1024, 1, 16, 0, 1, 64, 512, 256, 0), (512, 1, 8, 5, 1, 64, 512, 256, 0), (1024, 1, 32, 0, 0, 0, 512, 256, 0),
-    (256, 1, 16, 0, 0, 32, 1 << 30, 0, 0), (1024, 1, 32, 0, 0, 64, 4096, 0, 0), (1024, 1, 32, 0, 0, 64, 512, 0, 0),
-    (256, 1, 16, 0, 0, 64, 512, 1, 0), (1024, 0, 8, 2, 0, 64, 512, 16, 0), (1024, 1, 16, 0, 0, 64, 512, 256, 8),
-    (512, 1, 32, 0, 0, 64, 512, 256, 8), (256, 1, 8, 0, 0, 64, 512, 256, 8)])
+    (1024, 1, 16, 0, 0, 64, 512, 256, 8), (512, 1, 8, 0, 0, 64, 512, 256, 8), (256, 1, 32, 0, 0, 64, 512, 256, 8),
+    (1024, 0, 16, 0, 0, 64, 512, 256, 8), (256, 0, 8, 0, 0, 64, 512, 256, 8), (1024, 1, 32, 3, 0, 64, 512, 256, 8),
+    (256, 1, 16, 1, 0, 64, 512, 256, 8), (512, 1, 32, 7, 0, 64, 512, 256, 8), (1024, 1, 32, 0, 1, 64, 512, 256, 8),
+    (1024, 1, 16, 0, 1, 64, 512, 256, 8), (512, 1, 8, 5, 1, 64, 512, 256, 8), (1024, 1, 32, 0, 0, 0, 512, 256, 8),
+    (256, 1, 16, 0, 0, 32, 1 << 30, 0, 8), (1024, 1, 32, 0, 0, 64, 4096, 0, 8), (1024, 1, 32, 0, 0, 64, 512, 0, 8),
+    (256, 1, 16, 0, 0, 64, 512, 1, 8), (1024, 0, 8, 2, 0, 64, 512, 16, 8), (1024, 1, 16, 0, 0, 64, 512, 256, 0),
+    (512, 1, 32, 0, 0, 64, 512, 256, 0), (256, 1, 8, 0, 0, 64, 512, 256, 0)])
 def test_lbvh_equals_oracle_lbvh(gpu_renderer, chunk, ls, ri, grid, os1, wave, tail, span, dig):
     """Both chunk passes (level-synchronous rounds, or one climbing thread per leaf) at every
     chunk size, over radix tiles of 8, 16 or 32 keys per thread, with either radix pass
@@ -77,8 +77,8 @@ def test_lbvh_equals_oracle_lbvh(gpu_renderer, chunk, ls, ri, grid, os1, wave, t
     tail is the straddler list size from which one workgroup finishes the tree (1 << 30: it takes
     every straddler from the chunk pass on) when span is 0 (kernel rounds); span > 0 merges the
     straddlers with one workgroup per span of that many chunks before the tail. dig is the radix
-    digit width: 0 the wide digits (3 passes of 10 bits for 30-bit keys, 6 of 11 bits for 63-bit
-    keys), 8 the 8-bit passes (single-pass radix passes always take 8-bit digits)."""
+    digit width: 8 the 8-bit passes, 0 the wide digits (3 passes of 10 bits for 30-bit keys, 6 of
+    11 bits for 63-bit keys; single-pass radix passes always take 8-bit digits)."""
     gpu_renderer.set_option("sort_digit", dig)
     gpu_renderer.set_option("build_chunk", chunk)
     gpu_renderer.set_option("build_ls", ls)
@@ -113,7 +113,7 @@ def test_lbvh_equals_oracle_lbvh(gpu_renderer, chunk, ls, ri, grid, os1, wave, t
             assert height[0] >= int(np.ceil(np.log2(n)))
             assert gpu_renderer.get_option("sort_fault") == 0
     gpu_renderer.set_option("sort_onesweep", 0)
-    gpu_renderer.set_option("sort_digit", 0)
+    gpu_renderer.set_option("sort_digit", 8)
     gpu_renderer.set_option("build_chunk", 1024)
     gpu_renderer.set_option("build_ls", 1)
     gpu_renderer.set_option("sort_ri", 16)
