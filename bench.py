@@ -35,10 +35,15 @@ HBM_PEAK_GBS = 8000.0
 # plain 64-spp frame, for the 50M FCC workload
 DRAM_PROFILE_CHUNK = os.path.join(ROOT, "profiles", "k_bu_chunk_ls_fcc50m_dram.md")
 DRAM_PROFILE = os.path.join(ROOT, "profiles", "k_trace_sl_fcc50m_ao64_dram.md")
+# rocprofv3 --pmc SQ_INSTS_VALU summary of the plain frame's trace kernel (the same workload)
+VALU_PROFILE = os.path.join(ROOT, "profiles", "k_trace_sl_fcc50m_ao64_valu.md")
+# VALU issue roof: 256 CUs x 4 SIMDs, one wave64 instruction per SIMD per 4 cycles, 2.4 GHz peak clock
+VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
 
 
-def dram_traffic(workload_key, path=DRAM_PROFILE):
-    """DRAM bytes per launch recorded in a committed counter summary for this workload, or None."""
+def dram_traffic(workload_key, path=DRAM_PROFILE, field="dram_bytes_per_launch"):
+    """A per-launch figure (DRAM bytes by default) recorded in a committed counter summary for this
+    workload, or None."""
     try:
         txt = open(path).read()
     except OSError:
@@ -46,7 +51,7 @@ def dram_traffic(workload_key, path=DRAM_PROFILE):
     if workload_key not in txt:
         return None
     for line in txt.splitlines():
-        if line.startswith("dram_bytes_per_launch:"):
+        if line.startswith(field + ":"):
             return int(line.split(":")[1].strip())
     return None
 
@@ -809,6 +814,19 @@ def main():
             cpu.pop(k)
         log(f"CPU baseline: {cpu}")
 
+    # the plain frame's trace kernel against the roof it runs into: VALU instruction issue (its node
+    # fetches are mostly L2 / MALL hits, so algorithmic bytes over HBM would exceed 1)
+    valu = (dram_traffic(f"cells {args.cells}, 1920x1080, spp {args.spp}, mode {args.mode}", VALU_PROFILE,
+                         "valu_instructions_per_launch") if args.config == "fcc" else None)
+    plain_trace_roof = {"kernel": kernel_name, "algorithmic_bytes": int(algo_bytes),
+                        "node_and_sphere_gbs": round(achieved, 1), "dram_traffic": traffic,
+                        "dram_frac_of_hbm": (round(traffic / (trace_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                                             if traffic else None)}
+    if valu:
+        ach = valu / (trace_ms * 1e-3)
+        plain_trace_roof.update({"bound": "valu_issue", "achieved": round(ach / 1e9, 1),
+                                 "peak": round(VALU_ISSUE_PEAK / 1e9, 1), "unit": "G wave-instructions/s",
+                                 "frac": round(ach / VALU_ISSUE_PEAK, 4), "valu_instructions": int(valu)})
     value = dn["ms"] if dn else ms
     # roofline of the dominant kernel of the frame that `value` times: the chunk pass of the
     # denoised frame's build (the plain frame's trace kernel when there is no denoised frame)
@@ -853,11 +871,7 @@ def main():
         "trace_work": split,
         "autotune": tuned,
         "roofline": roofline,
-        "roofline_plain_trace": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                                 "dram_frac": (round(traffic / (trace_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
-                                               if traffic else None),
-                                 "kernel": kernel_name, "algorithmic_bytes": int(algo_bytes)},
+        "roofline_plain_trace": plain_trace_roof,
         "cpu_baseline": cpu,
         "orbit": orbit,
         "structures": structures,
