@@ -82,6 +82,48 @@ __device__ __forceinline__ void store_blocks(const ConvArgs& a, const f32x4 (&ac
     }
 }
 
+// The same epilogue for one output row with a fixed number of stores: every store is a buffer
+// store over the row (bytes [0, W * channels * size)), and a pixel or channel outside it gets an
+// offset past the end, which the hardware drops. No store sits in a branch, so every path issues
+// exactly PB * NB * 4 of them, and a wait for loads issued before them can count them
+// (vmcnt(PB * NB * 4)) instead of waiting for the stores too. row_ok false drops the whole row.
+template <int PB, int NB, bool F32>
+__device__ __forceinline__ void store_row_fixed(const ConvArgs& a, const f32x4 (&acc)[PB][NB],
+                                                const float (&bias)[NB], int b, int y, bool row_ok, int x0, int q,
+                                                int lr) {
+    const int yc = F32 ? (a.YfC ? a.YfC : a.CO) : a.CO;
+    const size_t rowo = ((size_t)b * a.H + (row_ok ? y : 0)) * a.W;
+    // the row is the same for the whole wavefront: keep the descriptor in scalar registers
+    const uint64_t ba = F32 ? (uint64_t)(a.Yf + rowo * yc) : (uint64_t)(a.Y + rowo * a.CO);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)ba);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(ba >> 32));
+    void* base = (void*)(((uint64_t)hi << 32) | lo);
+    const int row_bytes = __builtin_amdgcn_readfirstlane(row_ok ? a.W * yc * (F32 ? 4 : 2) : 0);
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(base, 0, row_bytes, 0x00020000);
+    constexpr int kOut = 0x40000000;  // an offset past any row
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        const int co = j * 16 + lr;
+#pragma unroll
+        for (int pb = 0; pb < PB; ++pb) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                float v = acc[pb][j][e] + bias[j];
+                if (a.relu) v = fmaxf(v, 0.0f);
+                const int x = x0 + pb * 16 + q * 4 + e;
+                const bool ok = x < a.W && co < yc;
+                if constexpr (F32) {
+                    const int off = ok ? (x * yc + co) * 4 : kOut;
+                    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), rs, off, 0, 0);
+                } else {
+                    const int off = ok ? (x * yc + co) * 2 : kOut;
+                    __builtin_amdgcn_raw_buffer_store_b16(f2bf(v), rs, off, 0, 0);
+                }
+            }
+        }
+    }
+}
+
 union Frag {
     bf16x4 v;
     uint2 u;
@@ -395,24 +437,26 @@ constexpr int DT_SLOTS = LT_ROWS * LT_PX * 4;    // 1584 16-byte slots of a tile
 constexpr int DT_INST = (DT_SLOTS + 255) / 256 * 4;  // 28 wave-wide loads: 7 per wave
 constexpr int DT_ALLOC = DT_INST * 64;           // slots allocated per buffer
 
-template <int NB>
+template <int NB, bool F32>
 __global__ __launch_bounds__(256) void k_conv_fwd_dma(ConvArgs a) {
     __shared__ uint4 tA[DT_ALLOC];
     __shared__ uint4 tB[DT_ALLOC];
     constexpr int NWI = (9 * NB * 64 + 255) / 256;
     __shared__ uint4 s_w[NWI * 256];  // padded: the weight stores are unconditional
-    const int CI = a.C1 + a.C2;
-    const int nslice = 1;  // CI <= 32
+    const int CI = a.C1 + a.C2;  // <= 32: one slice
     const int runs = (a.W + 63) / 64;
     const int rblocks = (a.H + 3) / 4;
     const int ntiles = a.B * rblocks * runs;
-    const int NBT = a.CO / 16;  // == NB
-    const int wave = threadIdx.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: the store rows
     const int lane = __lane_id();
     const int q = lane >> 4, lr = lane & 15;
     const uint4* wl4 = reinterpret_cast<const uint4*>(a.Wl);
-    int tl = (int)blockIdx.x, sl = 0;
+    int tl = (int)blockIdx.x;
     if (tl >= ntiles) return;
+    // s_waitcnt immediate (gfx9 layout: vmcnt bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8) that
+    // waits until at most kStores vector memory operations and no LDS operation are outstanding
+    constexpr int kStores = 4 * NB * 4;
+    constexpr int kWaitLoads = (kStores & 15) | ((kStores >> 4) << 14) | (7 << 4);  // and lgkmcnt(0)
     float bias[NB];
 #pragma unroll
     for (int j = 0; j < NB; ++j) bias[j] = a.bias[j * 16 + lr];
@@ -474,46 +518,52 @@ __global__ __launch_bounds__(256) void k_conv_fwd_dma(ConvArgs a) {
                     acc[pb][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[pb].v, cw[j].v, acc[pb][j], 0, 0, 0);
         }
     };
-    issue(tA, tl, sl);
+    issue(tA, tl, 0);
     f32x4 acc[4][NB];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < NB; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    // one step: wait for this item (the barrier's wait), start the next item into the other
-    // buffer, compute this one; the buffers are distinct arrays, so the compiler knows that reads
-    // of one need not wait for loads into the other
+    // one step: wait for this item, start the next item into the other buffer, compute this one
+    // and store it. The barrier opening a step is the only one: before it each wave waits for its
+    // own loads into `cur` (issued a whole step earlier; the stores issued after them have a fixed
+    // count, so the wait does not include them), and after it every wave is also done reading
+    // `nxt` (it read it in the previous step's compute, whose results it has already stored). The
+    // buffers are distinct arrays, so the compiler knows that reads of one need not wait for loads
+    // into the other.
     auto step = [&](const uint4* cur, uint4* nxt) -> bool {
-        __syncthreads();
-        int ntl = tl, nsl = sl + 1;
-        if (nsl == nslice) {
-            nsl = 0;
-            ntl = tl + (int)gridDim.x;
-        }
+        // this wave's loads into cur were followed by exactly kStores stores (none before the
+        // first step, where the wait is already met); vector memory operations complete in
+        // issue order, so vmcnt(kStores) means the loads have landed
+        // (a plain barrier: __syncthreads' fence would also wait for the stores). The signal
+        // fences keep the compiler from moving LDS accesses across it.
+        __builtin_amdgcn_s_waitcnt(kWaitLoads);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_barrier();
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        const int ntl = tl + (int)gridDim.x;
         const bool more = ntl < ntiles;
         // issued unconditionally (the last item reloads itself) so that every path has the same
         // number of loads in flight and the compiler's waits can count them
-        issue(nxt, more ? ntl : tl, more ? nsl : sl);
+        issue(nxt, more ? ntl : tl, 0);
         compute(cur, acc);
-        if (sl == nslice - 1) {  // the tile is complete
-            int b, x0, y0;
-            tile_origin(tl, b, x0, y0);
-            const int y = y0 + wave;
-            if (y < a.H) store_blocks<4, NB>(a, acc, bias, 0, NBT, b, y, x0, q, lr);
+        int b, x0, y0;
+        tile_origin(tl, b, x0, y0);
+        const int y = y0 + wave;
+        store_row_fixed<4, NB, F32>(a, acc, bias, b, y, y < a.H, x0, q, lr);
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int j = 0; j < NB; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-        }
-        __syncthreads();  // every wave is done reading cur before it is refilled
+            for (int j = 0; j < NB; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
         tl = ntl;
-        sl = nsl;
         return more;
     };
     while (true) {
         if (!step(tA, tB)) break;
         if (!step(tB, tA)) break;
     }
+    // the reload issued by the last step lands in LDS: wait for it before the workgroup ends
+    __syncthreads();
 }
 
 // ------------------------------------------------------------------ backward data
@@ -1008,12 +1058,12 @@ static size_t param_offset(int l) {
 // (shape % 10: 4, 2 or 1) and, for 4, channel blocks per wave (shape / 10, 0 = automatic).
 // Persistent grid of the LDS-DMA kernel: the workgroups that fit on the device at once (queried
 // once per instance), at most one per tile.
-template <int NB>
+template <int NB, bool F32>
 void launch_dma(hipStream_t s, const ConvArgs& a, long tiles) {
     static int fit = 0;
     if (fit == 0) {
         int per_cu = 0, dev = 0, cus = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_conv_fwd_dma<NB>, 256, 0) != hipSuccess ||
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_conv_fwd_dma<NB, F32>, 256, 0) != hipSuccess ||
             per_cu < 1)
             per_cu = 1;
         if (hipGetDevice(&dev) != hipSuccess ||
@@ -1022,7 +1072,7 @@ void launch_dma(hipStream_t s, const ConvArgs& a, long tiles) {
         fit = per_cu * cus;
     }
     const long g = tiles < fit ? tiles : fit;
-    hipLaunchKernelGGL((k_conv_fwd_dma<NB>), dim3((unsigned)g), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_conv_fwd_dma<NB, F32>), dim3((unsigned)g), dim3(256), 0, s, a);
 }
 
 static void conv_fwd(hipStream_t s, const ConvArgs& a, bool k32, int shape) {
@@ -1036,12 +1086,14 @@ static void conv_fwd(hipStream_t s, const ConvArgs& a, bool k32, int shape) {
         int pbk = 4;
         // the LDS-DMA double-buffered kernel (persistent) for single-slice layers of up to 32
         // output channels: the default for them (shape 0) and shape 6
-        if ((shape == 0 || shape == 6) && a.C1 + a.C2 <= 32 && NBT <= 2) {
+        if ((shape == 0 || shape == 6) && a.C1 + a.C2 <= 32 && NBT <= (a.Yf ? 1 : 2)) {
             const long tiles = (long)a.B * ((a.H + 3) / 4) * ((a.W + 63) / 64);
-            if (NBT == 1)
-                launch_dma<1>(s, a, tiles);
+            if (a.Yf)
+                launch_dma<1, true>(s, a, tiles);  // the output layer (16 channels)
+            else if (NBT == 1)
+                launch_dma<1, false>(s, a, tiles);
             else
-                launch_dma<2>(s, a, tiles);
+                launch_dma<2, false>(s, a, tiles);
             return;
         }
         // the LDS-staged kernel for every layer (it beats the per-wavefront kernel at all four
