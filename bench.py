@@ -544,8 +544,10 @@ def structure_compare(r, args, cam):
 
 
 def run_series(args):
-    """Batched time series: timesteps x 1-6 cameras (640x480 .. 1920x1080) in one hipGraph launch,
-    refit/rebuild policy and launch knobs picked by the autotuner."""
+    """Batched time series: timesteps x 1-6 cameras (640x480 .. 1920x1080) in one hipGraph launch
+    (every timestep's structure, then one trace launch over all jobs), refit/rebuild policy and
+    launch knobs picked by the autotuner; the same series with one trace launch per timestep is
+    timed beside it."""
     from megamol_amd import SphereRenderer, _lib as L, autotune, params, scenes
 
     t0 = time.time()
@@ -575,6 +577,16 @@ def run_series(args):
         _, ms = r.render_batch(jobs, refit_period=cfg["refit_period"], use_graph=True)
         ms_all.append(ms)
     ms = float(np.median(ms_all))
+    # the same series with one trace launch per timestep (right after its build) instead of one
+    # launch over all timesteps
+    r.set_option("batch_launch", 0)
+    ms_per_ts = []
+    for _ in range(max(1, args.steps)):
+        _, m2 = r.render_batch(jobs, refit_period=cfg["refit_period"], use_graph=True)
+        ms_per_ts.append(m2)
+    r.set_option("batch_launch", 1)
+    ms_ts = float(np.median(ms_per_ts))
+    log(f"one launch over all timesteps {ms:.2f} ms, one launch per timestep {ms_ts:.2f} ms")
     rays = px * args.spp * (2 if mode == L.MODE_AO else 1)
     out = {"metric": "batched series ms per launch (timesteps x cameras in one hipGraph)", "value": round(ms, 3),
            "unit": "ms/launch", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
@@ -584,6 +596,7 @@ def run_series(args):
                                   f"{px} pixels, mode {args.mode}, {args.spp} spp",
                       "autotuned": cfg},
            "frames": len(jobs), "ms_per_frame": round(ms / len(jobs), 3),
+           "launch_per_timestep_ms": round(ms_ts, 3),
            "mrays_per_s_upper": round(rays / (ms * 1e-3) / 1e6, 1)}
     print(json.dumps(out), flush=True)
     r.close()

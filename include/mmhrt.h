@@ -266,9 +266,11 @@ MMHRT_API int32_t mmhrt_get_counters(mmhrt_ctx ctx, uint64_t* node_visits, uint6
 /* ---- batched time series ---------------------------------------------------
  * No single counterpart in the reference, which renders one frame per Render call and rebuilds
  * its BVH on every frame change (SphereGeometry.cpp:252-257). All timesteps stay resident; per
- * timestep the LBVH is rebuilt (t % refit_period == 0) or refitted, then all cameras of that
- * timestep are traced by one persistent launch; use_graph captures the whole sequence into one
- * hipGraph and launches it once. */
+ * timestep the LBVH is rebuilt (t % refit_period == 0) or refitted into a structure of its own,
+ * then one persistent launch per mode (Lambert, AO) traces every job of the series; with option
+ * "batch_launch" 0 each timestep's jobs are traced right after its build instead (one launch per
+ * timestep, whose jobs must then share the mode; no per-timestep structures are kept). use_graph
+ * captures the whole sequence into one hipGraph and launches it once. */
 typedef struct mmhrt_job {
     uint32_t timestep;
     uint32_t reserved;
@@ -333,7 +335,8 @@ MMHRT_API int32_t mmhrt_denoiser_train_step(mmhrt_ctx ctx, const uint16_t* feats
  * many merges, up to 64, run on one wavefront without workgroup barriers, default 64; 0: never),
  * "accel" (structure the next build makes: 0 LBVH, the default; 1 treelet P-k-d trees, about one
  * byte per sphere beyond the spheres instead of up to 64), "treelet" (spheres per treelet of that
- * structure: 256, 512 or 1024, default 1024),
+ * structure: 256, 512 or 1024, default 1024), "batch_launch" (mmhrt_render_batch: 1, the
+ * default, one trace launch per mode over all timesteps; 0 one launch per timestep),
  * "sample_lanes" (samples of a pixel on the lanes for power-of-two spp,
  * default 1), "dn_k32" (denoiser forward on the 16x16x32 MFMA kernels, default 1),
  * "dn_conv_shape" (test hook: forced launch shape of those kernels, 0 = automatic),

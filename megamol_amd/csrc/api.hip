@@ -175,6 +175,11 @@ int32_t mmhrt_destroy(mmhrt_ctx c) {
     release(c->ts_sph); release(c->ts_col); release(c->bt_col); release(c->bt_depth); release(c->bt_t);
     release(c->bt_id); release(c->bt_rgba8); release(c->batch_ctr); release(c->batch_prefix);
     release(c->batch_jobs);
+    for (auto& sl : c->ts_slots) {
+        release(sl.nodes); release(sl.leaf); release(sl.id); release(sl.info); release(sl.dim); release(sl.box);
+        release(sl.rmax);
+    }
+    c->ts_slots.clear();
     if (c->rounds_host) (void)hipHostFree(c->rounds_host);
     c->rounds_host = nullptr;
     if (c->dn) {
@@ -565,6 +570,7 @@ static int64_t* option_slot(mmhrt_ctx c, const char* key) {
     if (k == "accel") return &c->opt_accel;
     if (k == "fail_alloc") return &c->opt_fail_alloc;
     if (k == "treelet") return &c->opt_treelet;
+    if (k == "batch_launch") return &c->opt_batch_launch;
     if (k == "stack_overflow") return &c->stack_overflow;
     if (k == "pair_visits") return &c->pair_visits;
     if (k == "leaf_fetches") return &c->leaf_fetches;

@@ -3,9 +3,11 @@
 // No single counterpart in the reference: MegaMol renders one frame per Render call and rebuilds
 // its OptiX BVH whenever the frame changes (SphereGeometry.cpp:252-257, always
 // OPTIX_BUILD_OPERATION_BUILD). Here all timesteps stay resident in HBM; for each timestep the
-// LBVH is rebuilt or refitted (refit keeps the topology and recomputes the boxes) and all of its
-// cameras are traced by one persistent launch that steals tiles across the cameras. The whole
-// sequence is captured once into a hipGraph and replayed.
+// LBVH is rebuilt or refitted (refit keeps the topology and recomputes the boxes) into a structure
+// of its own, and then one persistent launch traces every (timestep, camera) job of the series,
+// stealing tiles across all of them (option batch_launch 0: one launch per timestep, right after
+// its build, over the structure buffers the timesteps share). The whole sequence is captured
+// once into a hipGraph and launched.
 #include <string.h>
 
 #include <algorithm>
@@ -117,12 +119,37 @@ int32_t mmhrt_render_batch(mmhrt_ctx c, const mmhrt_job* jobs, uint32_t n_jobs, 
         by_t[J.timestep].push_back(j);
     }
     if (total_px > (1ull << 32)) return fail(c, MMHRT_E_UNSUPPORTED, "render_batch: outputs too large");
-    for (uint32_t t = 0; t < c->ts_count; ++t) {
-        int mode = -1;
-        for (uint32_t j : by_t[t]) {
-            if (mode >= 0 && jobs[j].params.mode != mode)
-                return fail(c, MMHRT_E_INVALID, "render_batch: jobs of one timestep must share the mode");
-            mode = jobs[j].params.mode;
+    const uint32_t T = c->ts_count;
+    // one launch: all jobs of one mode (Lambert, AO) in one persistent launch after every
+    // timestep's structure is built; otherwise one launch per timestep after its build, whose jobs
+    // then share the mode
+    const bool one = c->opt_batch_launch != 0;
+    if (!one) {
+        for (uint32_t t = 0; t < T; ++t) {
+            int mode = -1;
+            for (uint32_t j : by_t[t]) {
+                if (mode >= 0 && jobs[j].params.mode != mode)
+                    return fail(c, MMHRT_E_INVALID, "render_batch: jobs of one timestep must share the mode");
+                mode = jobs[j].params.mode;
+            }
+        }
+    }
+    // launch groups: consecutive jobs in launch order (by timestep, or by mode then timestep)
+    std::vector<uint32_t> order;
+    struct Launch {
+        uint32_t first, count, prefix, timestep;
+        int mode;
+    };
+    std::vector<Launch> launches;
+    for (int m = 0; m < (one ? 2 : 1); ++m) {
+        for (uint32_t t = 0; t < T; ++t) {
+            for (uint32_t j : by_t[t]) {
+                if (one && jobs[j].params.mode != m) continue;
+                if (launches.empty() || (one ? launches.back().mode != m : launches.back().timestep != t))
+                    launches.push_back(Launch{(uint32_t)order.size(), 0, 0, t, jobs[j].params.mode});
+                order.push_back(j);
+                ++launches.back().count;
+            }
         }
     }
     int e = 0;
@@ -131,8 +158,7 @@ int32_t mmhrt_render_batch(mmhrt_ctx c, const mmhrt_job* jobs, uint32_t n_jobs, 
     if ((e = ensure(c, c->bt_t, total_px, "batch t"))) return e;
     if ((e = ensure(c, c->bt_id, total_px, "batch id"))) return e;
     if ((e = ensure(c, c->bt_rgba8, total_px, "batch rgba8"))) return e;
-    if ((e = ensure(c, c->batch_ctr, (size_t)c->ts_count * 16, "batch counters"))) return e;
-    if ((e = ensure(c, c->batch_prefix, (size_t)n_jobs + c->ts_count, "batch tile prefix"))) return e;
+    if ((e = ensure(c, c->batch_ctr, launches.size() * 16, "batch counters"))) return e;
     const size_t js = batch_job_size();
     if ((e = ensure(c, c->batch_jobs, js * n_jobs, "batch jobs"))) return e;
     // The builds read the timestep slices and their count; the context's own spheres, count and
@@ -158,24 +184,61 @@ int32_t mmhrt_render_batch(mmhrt_ctx c, const mmhrt_job* jobs, uint32_t n_jobs, 
         unsigned grid = (unsigned)std::max<int64_t>(1, c->cu_count * c->opt_blocks_per_cu);
         if ((e = ensure(c, c->spill, (size_t)grid * 256 * 120, "traversal spill"))) return e;
     }
-    // job table and tile prefixes (jobs of timestep t are contiguous)
+    // One launch reads every timestep's structure, so each timestep but the last builds into a
+    // slot of its own (the last one into the context's buffers), and each rebuild but the last
+    // keeps its leaf order (which its refits share) in a slot too. bind(t) swaps timestep t's
+    // buffers into the context, so that the build and the job table see them; a second call
+    // swaps them back.
+    const uint32_t period = tl ? 1u : (uint32_t)refit_period;
+    const uint32_t G = (T - 1) / period + 1;  // rebuild groups
+    if (one && T > 1) {
+        if (c->ts_slots.size() < T - 1) c->ts_slots.resize(T - 1);
+        for (uint32_t t = 0; t + 1 < T; ++t) {
+            SeriesSlot& sl = c->ts_slots[t];
+            if ((e = ensure(c, sl.nodes, c->nodes.cap, "series nodes"))) return e;
+            if ((e = ensure(c, sl.leaf, c->leaf_sph.cap, "series leaf spheres"))) return e;
+            if ((e = ensure(c, sl.info, 1, "series scene info"))) return e;
+            if (t + 1 < G && (e = ensure(c, sl.id, c->leaf_id.cap, "series leaf ids"))) return e;
+            if (tl) {
+                if ((e = ensure(c, sl.dim, c->pkd_dim.cap, "series split axes"))) return e;
+                if ((e = ensure(c, sl.box, c->tl_box.cap, "series treelet boxes"))) return e;
+                if ((e = ensure(c, sl.rmax, c->tl_rmax.cap, "series treelet radii"))) return e;
+            }
+        }
+    }
+    auto bind = [&](uint32_t t) {
+        if (!one) return;
+        if (t + 1 < T) {
+            SeriesSlot& sl = c->ts_slots[t];
+            std::swap(c->nodes, sl.nodes);
+            std::swap(c->leaf_sph, sl.leaf);
+            std::swap(c->info, sl.info);
+            if (tl) {
+                std::swap(c->pkd_dim, sl.dim);
+                std::swap(c->tl_box, sl.box);
+                std::swap(c->tl_rmax, sl.rmax);
+            }
+        }
+        const uint32_t g = t / period;
+        if (g + 1 < G) std::swap(c->leaf_id, c->ts_slots[g].id);
+    };
+    // job table and tile prefixes (the jobs of a launch are contiguous, each launch's prefix
+    // array starts at 0)
     std::vector<uint8_t> hjobs(js * n_jobs);
     std::vector<uint32_t> hprefix;
-    std::vector<uint32_t> first_job(c->ts_count), first_prefix(c->ts_count), count_job(c->ts_count);
-    uint32_t slot = 0;
-    for (uint32_t t = 0; t < c->ts_count; ++t) {
-        first_job[t] = slot;
-        first_prefix[t] = (uint32_t)hprefix.size();
-        count_job[t] = (uint32_t)by_t[t].size();
+    for (Launch& L : launches) {
+        L.prefix = (uint32_t)hprefix.size();
         uint32_t acc = 0;
         hprefix.push_back(0);
-        for (uint32_t j : by_t[t]) {
+        for (uint32_t k = L.first; k < L.first + L.count; ++k) {
+            const mmhrt_job& J = jobs[order[k]];
             FrameConsts f;
-            frame_consts_batch(&jobs[j].camera, &jobs[j].params, f);
-            int tiles = batch_fill_job(c, f, jobs[j].out_offset, hjobs.data() + js * slot);
+            frame_consts_batch(&J.camera, &J.params, f);
+            bind(J.timestep);
+            const int tiles = batch_fill_job(c, f, J.out_offset, hjobs.data() + js * k);
+            bind(J.timestep);
             acc += (uint32_t)tiles;
             hprefix.push_back(acc);
-            ++slot;
         }
     }
     if ((e = ensure(c, c->batch_prefix, hprefix.size(), "batch tile prefix"))) return e;
@@ -184,16 +247,23 @@ int32_t mmhrt_render_batch(mmhrt_ctx c, const mmhrt_job* jobs, uint32_t n_jobs, 
         he = hipMemcpyAsync(c->batch_prefix.p, hprefix.data(), hprefix.size() * 4, hipMemcpyHostToDevice, c->stream);
     if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
     if (he != hipSuccess) return hip_fail(c, he, "render_batch upload");
+    auto launch = [&](size_t li) {
+        const Launch& L = launches[li];
+        launch_trace_batch(c, c->batch_jobs.p + js * L.first, c->batch_prefix.p + L.prefix, L.count,
+                           c->batch_ctr.p + 16 * li, L.mode, tl);
+    };
     auto issue = [&]() -> int {
-        for (uint32_t t = 0; t < c->ts_count; ++t) {
+        size_t li = 0;
+        for (uint32_t t = 0; t < T; ++t) {
             c->sph_cur = c->ts_sph.p + (size_t)t * c->ts_n;
+            bind(t);
             int r = tl ? build_pkd(c, morton_bits)
                        : (t % (uint32_t)refit_period == 0) ? build_lbvh(c, morton_bits) : refit_lbvh(c);
+            bind(t);
             if (r) return r;
-            if (count_job[t] == 0) continue;
-            launch_trace_batch(c, c->batch_jobs.p + js * first_job[t], c->batch_prefix.p + first_prefix[t],
-                               count_job[t], c->batch_ctr.p + 16 * t, jobs[by_t[t][0]].params.mode, tl);
+            if (!one && li < launches.size() && launches[li].timestep == t) launch(li++);
         }
+        for (; one && li < launches.size(); ++li) launch(li);
         return 0;
     };
     hipEvent_t e0, e1;

@@ -47,6 +47,19 @@ struct DevBuf {
     size_t cap = 0;  // elements
 };
 
+// The structure of one timestep of a batched series, kept until the single trace launch over all
+// timesteps has read it: the LBVH (or treelets' top tree) nodes, leaf spheres, scene facts and, for
+// treelets, the split axes, treelet boxes and radii; `id` holds the leaf order of one rebuild (its
+// refits share it).
+struct SeriesSlot {
+    DevBuf<Node> nodes;
+    DevBuf<float4> leaf;
+    DevBuf<uint32_t> id;
+    DevBuf<SceneInfo> info;
+    DevBuf<uint8_t> dim;
+    DevBuf<float> box, rmax;
+};
+
 struct Timers {
     hipEvent_t ev[12];
     bool init = false;
@@ -145,6 +158,7 @@ struct mmhrt_ctx_s {
     int64_t opt_fail_alloc = 0;     // test hook: the k-th device allocation from now fails with E_OOM (0: none)
     int64_t opt_accel = 0;          // structure the next build makes: 0 LBVH, 1 treelet P-k-d trees
     int64_t opt_treelet = 1024;     // spheres per treelet of the P-k-d structure: 256, 512 or 1024
+    int64_t opt_batch_launch = 1;   // batched series: 1 one trace launch over all timesteps (per mode), 0 one per timestep
     int64_t refits_since_build = 0;
     int64_t stack_overflow = 0;  // set by mmhrt_get_counters from the trace kernel's flag
     int64_t pair_visits = 0;     // set by mmhrt_get_counters: visits of leaf pairs (of the node visits)
@@ -162,6 +176,7 @@ struct mmhrt_ctx_s {
     mmhrt::DevBuf<int32_t> bt_id;
     mmhrt::DevBuf<uint32_t> bt_rgba8, batch_ctr, batch_prefix;
     mmhrt::DevBuf<uint8_t> batch_jobs;
+    std::vector<mmhrt::SeriesSlot> ts_slots;  // per timestep but the last (one-launch series)
 
     // learned denoiser (created on first use)
     mmhrt::dn::Denoiser* dn = nullptr;

@@ -1084,9 +1084,10 @@ void launch_refill(const TraceArgs& A, unsigned grid, uint32_t refill, bool coun
     }
 }
 
-// Batched trace: all jobs (cameras) of one timestep share one persistent launch. The global tile
-// index runs over the concatenated tiles of the jobs (prefix[j] = first tile of job j); each
-// wavefront steals tiles and shades them with its job's constants and output buffers.
+// Batched trace: the jobs (timestep, camera) of a series, each with its own timestep's structure
+// in its constants, share one persistent launch. The global tile index runs over the concatenated
+// tiles of the jobs (prefix[j] = first tile of job j); each wavefront steals tiles and shades them
+// with its job's constants and output buffers.
 // Held to 72 VGPRs (7 wavefronts per SIMD instead of 4) like the single-frame kernels: the batched
 // series (python bench.py --config series) went from 149.0 to 135.7 ms per launch.
 template <int S, int MODE, bool TL = false>
@@ -1104,8 +1105,16 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(7, 8))) void
         if (lane == 0) t0 = atomicAdd(ctr, 1u);
         const uint32_t tile = __shfl(t0, 0, 64);
         if (tile >= total) break;
-        uint32_t j = 0;
-        while (j + 1 < n_jobs && prefix[j + 1] <= tile) ++j;
+        // the job holding the tile: the last j with prefix[j] <= tile (binary search: a launch
+        // over a whole series holds hundreds of jobs)
+        uint32_t j = 0, hi = n_jobs - 1;
+        while (j < hi) {
+            const uint32_t mid = (j + hi + 1) >> 1;
+            if (prefix[mid] <= tile)
+                j = mid;
+            else
+                hi = mid - 1;
+        }
         const TraceArgs& A = jobs[j];
         const float amax = A.info->absmax;
         const float pad = (fmaxf(A.fc.pad, amax) + amax) * (1.0f / 131072.0f);
@@ -1120,7 +1129,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(7, 8))) void
 
 }  // namespace
 
-// One batched launch over the jobs of a timestep (mode is the same for all of them).
+// One batched launch over jobs that share the mode.
 void launch_trace_batch(mmhrt_ctx_s* c, const void* jobs, const uint32_t* prefix, uint32_t n_jobs, uint32_t* ctr,
                         int mode, bool treelets) {
     unsigned grid = (unsigned)std::max<int64_t>(1, c->cu_count * c->opt_blocks_per_cu);

@@ -46,15 +46,21 @@ def test_batch_equals_single_renders(gpu_renderer):
         f = r.render(cam, p, outputs=("rgba8", "depth"))
         ref.append((f.rgba8, f.depth))
     r.set_timesteps(xs, rgba)
-    for use_graph in (False, True):
-        for period in (1, 3, 100):
-            case(f"batch graph={use_graph} refit_period={period}")
-            for rep in range(2):  # a graph launched again must give the same frames
-                out, ms = r.render_batch(jobs, refit_period=period, use_graph=use_graph)
-                assert ms > 0
-                for k, ((a, d), (ra, rd)) in enumerate(zip(out, ref)):
-                    np.testing.assert_array_equal(a, ra, err_msg=f"job {k} rgba graph={use_graph} period={period}")
-                    np.testing.assert_array_equal(d.view(np.uint32), rd.view(np.uint32), err_msg=f"job {k} depth")
+    # batch_launch 1: every timestep's structure kept, one launch per mode over the whole series
+    # (the jobs alternate Lambert and AO by timestep: two launches); 0: one launch per timestep
+    for one in (1, 0):
+        r.set_option("batch_launch", one)
+        for use_graph in (False, True):
+            for period in (1, 3, 100):
+                case(f"batch launch={one} graph={use_graph} refit_period={period}")
+                for rep in range(2):  # a graph launched again must give the same frames
+                    out, ms = r.render_batch(jobs, refit_period=period, use_graph=use_graph)
+                    assert ms > 0
+                    for k, ((a, d), (ra, rd)) in enumerate(zip(out, ref)):
+                        np.testing.assert_array_equal(a, ra, err_msg=f"job {k} rgba launch={one} graph={use_graph} "
+                                                                     f"period={period}")
+                        np.testing.assert_array_equal(d.view(np.uint32), rd.view(np.uint32), err_msg=f"job {k} depth")
+    r.set_option("batch_launch", 1)
 
 
 def test_batch_errors(gpu_renderer):
