@@ -289,7 +289,8 @@ __device__ __forceinline__ float slab_io(const RayPre& r, float lx, float ly, fl
 // interval [t0, t1] meets the node's slab (split coordinate -+ the treelet's largest radius,
 // widened by the padding every slab test here carries); the near child takes
 // [t0, min(slab exit, t1, best)], the far child [max(t0, slab entry), min(t1, best)]. Closed
-// intervals, so that equal-t hits still reach the lowest particle id. A pushed P-k-d entry keeps
+// intervals, so that equal-t hits still reach the lowest particle id. Before its exact test a
+// sphere must also meet the ray in its own box. A pushed P-k-d entry keeps
 // its entry distance (two stack words: t0 below the reference) and resumes with t1 = the best
 // hit at the time it is popped. A treelet is entered at its root over its box's [entry, exit].
 // Closest hit (ANY false): returns the position of the hit (or -1) and its t in t_best. ANY:
@@ -360,7 +361,12 @@ __device__ int trace_tl(const TraceArgs& A, const RayPre& r, float tmin, float t
             const float tlo = __builtin_fmaf(sc - rmax, inv, -oil), thi = __builtin_fmaf(sc + rmax, inv, -oih);
             const float tnr = fminf(tlo, thi), tfr = fmaxf(tlo, thi);
             float lim = fminf(t1, t_best);
-            if (fmaxf(t0, tnr) <= fminf(tfr, lim) && (!ANY || (int)p != skip)) {
+            // a sphere in the node's interval is tested against its own box first (centre -+
+            // radius: the leaf box an LBVH traversal tests before its sphere), a filter that
+            // changes no result: most spheres in a slab lie off the ray in the other two axes
+            if (fmaxf(t0, tnr) <= fminf(tfr, lim) && (!ANY || (int)p != skip) &&
+                slab(r, sp.x - sp.w, sp.y - sp.w, sp.z - sp.w, sp.x + sp.w, sp.y + sp.w, sp.z + sp.w, tmin,
+                     t_best) != INFINITY) {
                 if (COUNT) cnt.tests++;
                 const float t = sphere_t(r.o, r.d, sp, tmin, tmax);
                 if (t >= 0.0f) {
