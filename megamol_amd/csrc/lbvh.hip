@@ -933,9 +933,11 @@ __global__ __launch_bounds__(BC) __attribute__((amdgpu_waves_per_eu(8, 8))) void
     __shared__ uint32_t s_cnt[BC];
     __shared__ uint16_t s_lst[2][BC];   // splits completed in the previous / current round
     __shared__ uint32_t s_nl[3];        // list sizes: read in round r, appended in r + 1, cleared in r + 2
+    __shared__ uint32_t s_nw[2];        // list sizes of the rounds wavefront 0 runs alone
     __shared__ int32_t s_pend[2];       // boundary splits delivered to in this chunk (left, right)
     __shared__ uint32_t s_list[256];
     __shared__ uint32_t s_n, s_base;
+    __shared__ float4 s_nb[BC / 64];    // the sphere of each wavefront's first leaf (its left neighbour's partner)
     SplitList ap{s_list, &s_n, &s_base};
     const uint32_t gen = cnt[GEN_SLOT];
     const int pair_h = (int)info->pair_h;
@@ -993,6 +995,7 @@ __global__ __launch_bounds__(BC) __attribute__((amdgpu_waves_per_eu(8, 8))) void
             }
         }
         s_cnt[tid] = 0;
+        if ((tid & 63) == 0) s_nb[tid >> 6] = sp_cur;
         __syncthreads();
         // deliver node X = [l, r] (box b, height h, dl = delta(l - 1), dr = delta(r); its
         // children's boxes f and references kd when internal) to its parent
@@ -1043,13 +1046,42 @@ __global__ __launch_bounds__(BC) __attribute__((amdgpu_waves_per_eu(8, 8))) void
         auto gref = [&](uint16_t rr) -> int32_t {
             return (rr & 0x8000u) ? ~(cs + (int32_t)(rr & 0x7fffu)) : cs + (int32_t)rr;
         };
-        // the leaves
+        // the leaves. Leaves k and k + 1 whose parent is split k (delta(k) above delta(k - 1) and
+        // delta(k + 1)) are merged here by the left leaf's thread, exactly as a round would merge
+        // them, so that the most numerous merges (the nodes over two leaves) take no round and no
+        // LDS records; the right leaf's thread delivers nothing. The right sphere comes from the next
+        // lane, or for a wavefront's last lane from the next wavefront's first (s_nb). A pair whose
+        // split is a chunk boundary is left to the global meeting place like any boundary split.
         if (k < ce) {
+            const int dkm1 = s_delta[tid], dk = s_delta[tid + 1];
+            const int dkp1 = s_delta[tid + 2 <= BC ? tid + 2 : BC];  // read only when k + 1 < ce
+            const int dkm2 = s_delta[tid >= 1 ? tid - 1 : 0];         // read only when k - 1 >= cs
+            const bool pair_l = k + 1 < ce && dk > dkm1 && dk > dkp1;
+            const bool pair_r = k - 1 >= cs && dkm1 > dkm2 && dkm1 > dk;
+            float4 nb;
+            nb.x = __shfl_down(sp_cur.x, 1, 64);
+            nb.y = __shfl_down(sp_cur.y, 1, 64);
+            nb.z = __shfl_down(sp_cur.z, 1, 64);
+            nb.w = __shfl_down(sp_cur.w, 1, 64);
+            if ((tid & 63) == 63 && pair_l) nb = s_nb[(tid >> 6) + 1];  // k + 1 < ce: a later wavefront's leaf
             float b[6];
             leaf_box(sp_cur, b);
-            float f0[12];
-            deliver(k, k, s_delta[tid], s_delta[tid + 1], b, 0, false, f0, make_int2(0, 0), 0, k, &s_nl[0],
-                    s_lst[0]);
+            if (pair_l) {
+                float f[12], u[6];
+                leaf_box(nb, f + 6);
+#pragma unroll
+                for (int a = 0; a < 6; ++a) f[a] = b[a];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    u[a] = fminf(f[a], f[6 + a]);
+                    u[3 + a] = fmaxf(f[3 + a], f[9 + a]);
+                }
+                const int2 kd = make_int2(~k, ~(k + 1));
+                deliver(k, k + 1, dkm1, dkp1, u, 1, true, f, kd, pair_flags(kd, 0, 0), 0, &s_nl[0], s_lst[0]);
+            } else if (!pair_r) {
+                float f0[12];
+                deliver(k, k, dkm1, dk, b, 0, false, f0, make_int2(0, 0), 0, k, &s_nl[0], s_lst[0]);
+            }
         }
         // merge the two children parked at a completed in-chunk split and deliver the node
         auto merge = [&](int slot, uint32_t* nl_next, uint16_t* lst_next) {
@@ -1077,15 +1109,19 @@ __global__ __launch_bounds__(BC) __attribute__((amdgpu_waves_per_eu(8, 8))) void
             if (count <= (uint32_t)wave_items) {
                 // the last levels on wavefront 0 alone: one wavefront's LDS accesses complete in issue
                 // order, so its rounds need no workgroup barrier, only a fence that keeps the compiler
-                // from moving LDS accesses across a round
+                // from moving LDS accesses across a round. Its rounds count in s_nw, never in s_nl:
+                // the other wavefronts may still be reading s_nl[c0] for this round's decision, and a
+                // wavefront that read it after a clear would leave the loop early, skip the barrier
+                // below and run the arrival counts while wavefront 0 still merges.
                 if (tid < 64) {
                     uint32_t cw = count;
                     for (int r2 = rd; cw != 0; ++r2) {
-                        if (tid == 0) s_nl[(r2 + 2) % 3] = 0;
+                        uint32_t* nc = &s_nw[r2 & 1];  // read (into cw) at the end of round r2 - 1 at the latest
+                        if (tid == 0) *nc = 0;
                         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                        if ((uint32_t)tid < cw) merge(s_lst[r2 & 1][tid], &s_nl[(r2 + 1) % 3], s_lst[(r2 + 1) & 1]);
+                        if ((uint32_t)tid < cw) merge(s_lst[r2 & 1][tid], nc, s_lst[(r2 + 1) & 1]);
                         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                        cw = s_nl[(r2 + 1) % 3];
+                        cw = *nc;
                     }
                 }
                 __syncthreads();
