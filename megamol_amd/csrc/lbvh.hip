@@ -927,10 +927,10 @@ __global__ __launch_bounds__(BC) __attribute__((amdgpu_waves_per_eu(8, 8))) void
     __shared__ uint16_t s_ref[BC][2];   // chunk-relative child references
     __shared__ uint16_t s_far[BC][2];   // chunk-relative far range ends
     __shared__ int8_t s_delta[BC + 1];  // delta(cs - 1 + j)
-    __shared__ uint8_t s_h[BC][2];
-    __shared__ int8_t s_dx[BC][2];      // per side: delta(l - 1) of a left child, delta(r) of a right child
-    __shared__ uint8_t s_side[BC];
-    __shared__ uint32_t s_cnt[BC];
+    // per side: the child's height (low byte) and the one delta its parent needs (high byte: delta(l - 1)
+    // of a left child, delta(r) of a right child); both sides are read with one 4-byte load
+    __shared__ uint16_t s_hd[BC][2];
+    __shared__ uint32_t s_cnt[BC];      // arrivals: + 1 for side 0, + 0x10000 for side 1
     __shared__ uint16_t s_lst[2][BC];   // splits completed in the previous / current round
     __shared__ uint32_t s_nl[3];        // list sizes: read in round r, appended in r + 1, cleared in r + 2
     __shared__ uint32_t s_nw[2];        // list sizes of the rounds wavefront 0 runs alone
@@ -1035,13 +1035,14 @@ __global__ __launch_bounds__(BC) __attribute__((amdgpu_waves_per_eu(8, 8))) void
             for (int a = 0; a < 6; ++a) s_box[slot][side * 6 + a] = b[a];
             s_ref[slot][side] = rrel;
             s_far[slot][side] = (uint16_t)(far - cs);
-            s_h[slot][side] = (uint8_t)h;
-            s_dx[slot][side] = (int8_t)(left ? dl : dr);  // the parent's own dl (left) or dr (right)
-            s_side[slot] = (uint8_t)side;
-            // the merge happens in the next round, after a barrier: a relaxed count suffices
-            uint32_t old = __hip_atomic_fetch_add(&s_cnt[slot], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            // the parent's own dl (left) or dr (right) beside the height
+            s_hd[slot][side] = (uint16_t)((uint32_t)h | ((uint32_t)(uint8_t)(int8_t)(left ? dl : dr) << 8));
+            // the merge happens in the next round, after a barrier: a relaxed count suffices. Each
+            // side adds its own unit, so the count also tells which side a lone arrival came from.
+            const uint32_t old = __hip_atomic_fetch_add(&s_cnt[slot], side ? 0x10000u : 1u, __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_WORKGROUP);
             // (a wavefront-aggregated append, one counter update per wavefront, measured 2% slower)
-            if (old == 1u) lst_next[atomicAdd(nl_next, 1u)] = (uint16_t)slot;
+            if (old != 0u) lst_next[atomicAdd(nl_next, 1u)] = (uint16_t)slot;
         };
         auto gref = [&](uint16_t rr) -> int32_t {
             return (rr & 0x8000u) ? ~(cs + (int32_t)(rr & 0x7fffu)) : cs + (int32_t)rr;
@@ -1089,10 +1090,11 @@ __global__ __launch_bounds__(BC) __attribute__((amdgpu_waves_per_eu(8, 8))) void
 #pragma unroll
             for (int a = 0; a < 12; ++a) f[a] = s_box[slot][a];
             const int2 kd = make_int2(gref(s_ref[slot][0]), gref(s_ref[slot][1]));
-            const int h0 = s_h[slot][0], h1 = s_h[slot][1];
+            const uint32_t hd = *reinterpret_cast<const uint32_t*>(s_hd[slot]);
+            const int h0 = (int)(hd & 0xffu), h1 = (int)((hd >> 16) & 0xffu);
             const int h = 1 + (h0 > h1 ? h0 : h1);
             const int32_t l = cs + (int32_t)s_far[slot][0], r = cs + (int32_t)s_far[slot][1];
-            const int dl = s_dx[slot][0], dr = s_dx[slot][1];
+            const int dl = (int)(int8_t)(hd >> 8), dr = (int)(int8_t)(hd >> 24);
             float b[6];
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
@@ -1149,14 +1151,16 @@ __global__ __launch_bounds__(BC) __attribute__((amdgpu_waves_per_eu(8, 8))) void
         }
         const int nsplit = ce - cs - 1;
         for (int slot = tid; slot < nsplit; slot += BC) {
-            if (s_cnt[slot] != 1u) continue;
-            const int side = s_side[slot];
+            const uint32_t ca = s_cnt[slot];
+            if (ca != 1u && ca != 0x10000u) continue;
+            const int side = ca == 1u ? 0 : 1;
             const int32_t sp = cs + slot;
             float b[6];
 #pragma unroll
             for (int a = 0; a < 6; ++a) b[a] = s_box[slot][side * 6 + a];
-            meet_put(meet + sp, meet_far + sp, side, b, gref(s_ref[slot][side]), s_h[slot][side],
-                     cs + (int32_t)s_far[slot][side], s_dx[slot][side]);
+            const uint32_t hd = s_hd[slot][side];
+            meet_put(meet + sp, meet_far + sp, side, b, gref(s_ref[slot][side]), (int)(hd & 0xffu),
+                     cs + (int32_t)s_far[slot][side], (int)(int8_t)(hd >> 8));
             if (atomicExch(flags + sp, gen) == gen) ap.push((uint32_t)sp, list, cnt + 1);
         }
         __syncthreads();
