@@ -34,12 +34,15 @@ typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
 
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
-// round to nearest even
-__device__ __forceinline__ uint16_t f2bf(float f) {
-    const uint32_t u = __float_as_uint(f);
-    const uint32_t r = (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
-    const uint32_t nan = (u >> 16) | 0x40u;  // NaN stays NaN
-    return (uint16_t)(((u & 0x7fffffffu) > 0x7f800000u) ? nan : r);  // a select, no branch
+// round to nearest even: one v_cvt_pk_bf16_f32 (gfx950), which rounds as the mode register says
+// (nearest even, denormals kept) and keeps NaN a NaN; a shift-and-add rounding costs 6 VALU
+// instructions per value, which dominated the convolution epilogues
+__device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+// two values in one conversion: lo in bits 0-15, hi in bits 16-31
+__device__ __forceinline__ uint32_t f2bf2(float lo, float hi) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2_t){lo, hi}, bf16x2_t));
 }
 
 // Epilogue of 16x16 MFMA output blocks: lane (q, lr) holds channel co = 16 nb + lr of pixels
@@ -420,19 +423,19 @@ __global__ __launch_bounds__(256) void k_conv_fwd_lds(ConvArgs a) {
     store_blocks<4, NB>(a, acc, bias, 0, NBT, b, y, x0, q, lr);
 }
 
-// The same convolution with the input brought into LDS by direct global->LDS loads
-// (global_load_lds_dwordx4: 16 bytes per lane into consecutive LDS slots), double buffered: a
-// persistent workgroup walks its (tile, slice) items and issues the next item's loads before it
-// computes the current one, without holding the data in registers. The tile is unpadded, and a
+// The same convolution with the input brought into LDS by direct loads (buffer_load_dwordx4 ...
+// lds: 16 bytes per lane into consecutive LDS slots), double buffered: a persistent workgroup walks
+// its tiles and issues the next tile's loads before it computes the current one, without holding
+// the data in registers. The input is X1 only (C2 = 0) and at most 2 GB, so that the buffer
+// offsets fit 31 bits (the launch checks both). The tile is unpadded, and a
 // pixel's four 8-channel groups are stored XOR-swizzled (slot 4P + (g ^ ((P >> 1) & 3)) for tile
 // pixel P), so that the 8 consecutive pixels of an A-fragment read phase cover all 32 banks.
-// Pieces outside the image or past CI load 16 zero bytes from dn_zero16 instead, so every wave
+// Pieces outside the image or past CI load 16 zero bytes (out-of-range buffer loads), so every wave
 // issues the same fixed number of loads per item, and the compiler can count them when it waits
 // for anything else. The kernel serves single-slice layers (CI <= 32) of up to 32 output
 // channels, whose weights sit in LDS for the whole launch. (A multi-slice variant that loaded
 // each slice's weights into registers per item measured 2.4x slower than the LDS-staged kernel
 // on the 80-channel decoder layer: the weight latency it exposes per item is not hidden.)
-__device__ uint4 dn_zero16[1];                   // 16 zero bytes (static storage starts zeroed)
 constexpr int DT_SLOTS = LT_ROWS * LT_PX * 4;    // 1584 16-byte slots of a tile slice
 constexpr int DT_INST = (DT_SLOTS + 255) / 256 * 4;  // 28 wave-wide loads: 7 per wave
 constexpr int DT_ALLOC = DT_INST * 64;           // slots allocated per buffer
@@ -477,24 +480,35 @@ __global__ __launch_bounds__(256) void k_conv_fwd_dma(ConvArgs a) {
         b = r2 / rblocks;
         x0 = run * 64;
     };
-    // loads of item (t, slice) into buf: wave w issues wave-wide loads w, w + 4, ..., 7 of them
-    auto issue = [&](uint4* buf, int t, int slice) {
+    // Loads of a tile into buf: wave w issues the wave-wide loads w, w + 4, ..., 7 of them, as
+    // buffer loads over the input tensor. A slot's piece (tile row, pixel, 8-channel group) is the
+    // same for every tile, so its byte offset from the tile's corner and its place are computed once;
+    // per tile only the corner and the image bounds change. A piece outside the image or past CI
+    // gets an offset past the tensor, which the hardware answers with 16 zero bytes.
+    constexpr int NSLOT = DT_INST / 4;
+    int rel[NSLOT];    // byte offset of the piece from pixel (y0 - 1, x0 - 1), channel 0
+    int where[NSLOT];  // tile row | pixel << 8, or -1 for a slot past the tile or past CI
+#pragma unroll
+    for (int k = 0; k < NSLOT; ++k) {
+        const int S = (wave + 4 * k) * 64 + lane;
+        const int P = S >> 2, g = (S & 3) ^ ((P >> 1) & 3);
+        const int ry = P / LT_PX, pxl = P - ry * LT_PX;
+        rel[k] = ((ry * a.W + pxl) * a.C1 + g * 8) * 2;
+        where[k] = (P < LT_ROWS * LT_PX && g * 8 < CI) ? (ry | (pxl << 8)) : -1;
+    }
+    const __amdgpu_buffer_rsrc_t src =
+        __builtin_amdgcn_make_buffer_rsrc((void*)a.X1, 0, a.B * a.H * a.W * a.C1 * 2, 0x00020000);
+    auto issue = [&](uint4* buf, int t) {
         int b, x0, y0;
         tile_origin(t, b, x0, y0);
+        const int corner = (((b * a.H + y0 - 1) * a.W) + x0 - 1) * a.C1 * 2;
 #pragma unroll
-        for (int k = 0; k < DT_INST / 4; ++k) {
-            const int qi = wave + 4 * k;
-            const int S = qi * 64 + lane;
-            const int P = S >> 2, g = (S & 3) ^ ((P >> 1) & 3);
-            const int ry = P / LT_PX, pxl = P - ry * LT_PX;
-            const int yy = y0 - 1 + ry, xx = x0 - 1 + pxl, ci0 = slice * 32 + g * 8;
-            const bool ok = P < LT_ROWS * LT_PX && ci0 < CI && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
-            const int yc = ok ? yy : 0, xc = ok ? xx : 0, cc = ok ? ci0 : 0;
-            const uint16_t* src =
-                cc >= a.C1 ? a.X2 + (((size_t)b * (a.H >> 1) + (yc >> 1)) * (a.W >> 1) + (xc >> 1)) * a.C2 + (cc - a.C1)
-                           : a.X1 + (((size_t)b * a.H + yc) * a.W + xc) * a.C1 + cc;
-            const void* gsrc = ok ? (const void*)src : (const void*)dn_zero16;
-            __builtin_amdgcn_global_load_lds(gsrc, (__attribute__((address_space(3))) void*)(buf + qi * 64), 16, 0, 0);
+        for (int k = 0; k < NSLOT; ++k) {
+            const int yy = y0 - 1 + (where[k] & 0xff), xx = x0 - 1 + ((where[k] >> 8) & 0xff);
+            const bool ok = where[k] >= 0 && (unsigned)yy < (unsigned)a.H && (unsigned)xx < (unsigned)a.W;
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                src, (__attribute__((address_space(3))) void*)(buf + (wave + 4 * k) * 64), 16,
+                ok ? corner + rel[k] : (int)0x80000000, 0, 0, 0);
         }
     };
     auto compute = [&](const uint4* buf, f32x4 (&acc)[4][NB]) {
@@ -518,7 +532,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd_dma(ConvArgs a) {
                     acc[pb][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[pb].v, cw[j].v, acc[pb][j], 0, 0, 0);
         }
     };
-    issue(tA, tl, 0);
+    issue(tA, tl);
     f32x4 acc[4][NB];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -545,7 +559,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd_dma(ConvArgs a) {
         const bool more = ntl < ntiles;
         // issued unconditionally (the last item reloads itself) so that every path has the same
         // number of loads in flight and the compiler's waits can count them
-        issue(nxt, more ? ntl : tl, 0);
+        issue(nxt, more ? ntl : tl);
         compute(cur, acc);
         int b, x0, y0;
         tile_origin(tl, b, x0, y0);
@@ -1086,7 +1100,8 @@ static void conv_fwd(hipStream_t s, const ConvArgs& a, bool k32, int shape) {
         int pbk = 4;
         // the LDS-DMA double-buffered kernel (persistent) for single-slice layers of up to 32
         // output channels: the default for them (shape 0) and shape 6
-        if ((shape == 0 || shape == 6) && a.C1 + a.C2 <= 32 && NBT <= (a.Yf ? 1 : 2)) {
+        if ((shape == 0 || shape == 6) && a.C2 == 0 && a.C1 <= 32 && NBT <= (a.Yf ? 1 : 2) &&
+            (double)a.B * a.H * a.W * a.C1 * 2 < 2147483648.0) {
             const long tiles = (long)a.B * ((a.H + 3) / 4) * ((a.W + 63) / 64);
             if (a.Yf)
                 launch_dma<1, true>(s, a, tiles);  // the output layer (16 channels)
