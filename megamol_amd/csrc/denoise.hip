@@ -311,7 +311,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd32(ConvArgs a) {
 constexpr int LT_PX = 66, LT_ROWS = 6, LT_CP = 40;
 
 template <int NB>
-__global__ __launch_bounds__(256) void k_conv_fwd_lds(ConvArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_conv_fwd_lds(ConvArgs a) {
     // up to 3 output-channel blocks the slice's weights (9 * NB KB) sit in LDS beside the tile,
     // so that the tap loop reads LDS only; wider layers read them from global memory, one tap ahead
     constexpr bool WLDS = NB <= 3;
@@ -334,7 +334,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd_lds(ConvArgs a) {
     const int b = blk / rblocks;
     const int x0 = run * 64, y0 = rb * 4;
     const int NBT = a.CO / 16;  // == NB: the launch picks the instance
-    const int wave = threadIdx.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: the store row
     const int lane = __lane_id();
     const int q = lane >> 4, lr = lane & 15;
     f32x4 acc[4][NB];
@@ -346,27 +346,57 @@ __global__ __launch_bounds__(256) void k_conv_fwd_lds(ConvArgs a) {
 #pragma unroll
     for (int j = 0; j < NB; ++j) bias[j] = a.bias[(j < NBT ? j : 0) * 16 + lr];
     const uint4* wl4 = reinterpret_cast<const uint4*>(a.Wl);
-    for (int sl = 0; sl < nslice; ++sl) {
-        // stage the slice: 6 rows x 66 pixels x 4 groups of 8 channels (and its weights). Every
-        // load is issued before any is stored, at a clamped (valid) address; pieces outside the
-        // image or past CI are zeroed by a mask when stored.
-        uint4 v[NI];
-        uint32_t okm = 0;
+    // A piece's place in the tile (row, pixel, 8-channel group) is fixed per thread; its source is
+    // X1 (channels below C1) or the half-resolution X2 (nearest upsampled), read with buffer loads:
+    // a piece outside the image or past CI gets an offset past its tensor and reads zeros, so no
+    // load is guarded and no mask is needed.
+    int where[NI];  // tile row | pixel << 8 | group << 16, or -1 past the tile
 #pragma unroll
-        for (int k = 0; k < NI; ++k) {
-            const int i = threadIdx.x + k * 256;
-            const int ii = i < NT ? i : 0;
-            const int g8 = ii & 3, pxl = (ii >> 2) % LT_PX, ry = (ii >> 2) / LT_PX;
-            const int yy = y0 - 1 + ry, xx = x0 - 1 + pxl, ci0 = sl * 32 + g8 * 8;
-            const bool ok = ci0 < CI && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
-            okm |= (ok ? 1u : 0u) << k;
-            const int yc = yy < 0 ? 0 : (yy >= a.H ? a.H - 1 : yy);
-            const int xc = xx < 0 ? 0 : (xx >= a.W ? a.W - 1 : xx);
-            const int cc = ci0 < CI ? ci0 : 0;
-            const uint16_t* src =
-                cc >= a.C1 ? a.X2 + (((size_t)b * (a.H >> 1) + (yc >> 1)) * (a.W >> 1) + (xc >> 1)) * a.C2 + (cc - a.C1)
-                           : a.X1 + (((size_t)b * a.H + yc) * a.W + xc) * a.C1 + cc;
-            v[k] = *reinterpret_cast<const uint4*>(src);
+    for (int k = 0; k < NI; ++k) {
+        const int i = threadIdx.x + k * 256;
+        where[k] = i < NT ? (((i >> 2) / LT_PX) | (((i >> 2) % LT_PX) << 8) | ((i & 3) << 16)) : -1;
+    }
+    const __amdgpu_buffer_rsrc_t r1 =
+        __builtin_amdgcn_make_buffer_rsrc((void*)a.X1, 0, a.B * a.H * a.W * a.C1 * 2, 0x00020000);
+    const __amdgpu_buffer_rsrc_t r2 = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(a.X2 ? a.X2 : a.X1), 0, a.X2 ? a.B * (a.H >> 1) * (a.W >> 1) * a.C2 * 2 : 0, 0x00020000);
+    constexpr int kOut = (int)0x80000000;
+    for (int sl = 0; sl < nslice; ++sl) {
+        // stage the slice: 6 rows x 66 pixels x 4 groups of 8 channels (and its weights); every
+        // load is issued before any is stored
+        const bool from1 = sl * 32 < a.C1, from2 = sl * 32 + 32 > a.C1 && a.C2 > 0;  // uniform
+        auto off1 = [&](int yy, int xx, int ci0) { return (((b * a.H + yy) * a.W + xx) * a.C1 + ci0) * 2; };
+        auto off2 = [&](int yy, int xx, int ci0) {
+            return (((b * (a.H >> 1) + (yy >> 1)) * (a.W >> 1) + (xx >> 1)) * a.C2 + ci0 - a.C1) * 2;
+        };
+        uint4 v[NI];
+        if (!(from1 && from2)) {  // the whole slice from one tensor: one load per piece
+            const __amdgpu_buffer_rsrc_t rs = from1 ? r1 : r2;
+#pragma unroll
+            for (int k = 0; k < NI; ++k) {
+                const int w = where[k];
+                const int yy = y0 - 1 + (w & 0xff), xx = x0 - 1 + ((w >> 8) & 0xff);
+                const int ci0 = sl * 32 + ((w >> 16) & 3) * 8;
+                const bool in = w >= 0 && (unsigned)yy < (unsigned)a.H && (unsigned)xx < (unsigned)a.W && ci0 < CI;
+                const int o = in ? (from1 ? off1(yy, xx, ci0) : off2(yy, xx, ci0)) : kOut;
+                v[k] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rs, o, 0, 0));
+            }
+        } else {  // a slice across C1 (C1 not a multiple of 32): both loads issued, then combined
+            uint4 v2[NI];
+#pragma unroll
+            for (int k = 0; k < NI; ++k) {
+                const int w = where[k];
+                const int yy = y0 - 1 + (w & 0xff), xx = x0 - 1 + ((w >> 8) & 0xff);
+                const int ci0 = sl * 32 + ((w >> 16) & 3) * 8;
+                const bool in = w >= 0 && (unsigned)yy < (unsigned)a.H && (unsigned)xx < (unsigned)a.W;
+                v[k] = __builtin_bit_cast(
+                    uint4, __builtin_amdgcn_raw_buffer_load_b128(r1, in && ci0 < a.C1 ? off1(yy, xx, ci0) : kOut, 0, 0));
+                v2[k] = __builtin_bit_cast(
+                    uint4, __builtin_amdgcn_raw_buffer_load_b128(
+                               r2, in && ci0 >= a.C1 && ci0 < CI ? off2(yy, xx, ci0) : kOut, 0, 0));
+            }
+#pragma unroll
+            for (int k = 0; k < NI; ++k) v[k] = make_uint4(v[k].x | v2[k].x, v[k].y | v2[k].y, v[k].z | v2[k].z, v[k].w | v2[k].w);
         }
         uint4 wv[NW > 0 ? NW : 1];
 #pragma unroll
@@ -380,9 +410,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd_lds(ConvArgs a) {
         for (int k = 0; k < NI; ++k) {
             const int i = threadIdx.x + k * 256;  // pieces past NT land in the padding rows
             const int g8 = i & 3, pxl = (i >> 2) % LT_PX, ry = (i >> 2) / LT_PX;
-            const uint32_t m = 0u - ((okm >> k) & 1u);
-            *reinterpret_cast<uint4*>(tile + (ry * LT_PX + pxl) * LT_CP + g8 * 8) =
-                make_uint4(v[k].x & m, v[k].y & m, v[k].z & m, v[k].w & m);
+            *reinterpret_cast<uint4*>(tile + (ry * LT_PX + pxl) * LT_CP + g8 * 8) = v[k];
         }
 #pragma unroll
         for (int k = 0; k < NW; ++k) s_w[threadIdx.x + k * 256] = wv[k];
@@ -419,8 +447,10 @@ __global__ __launch_bounds__(256) void k_conv_fwd_lds(ConvArgs a) {
         __syncthreads();
     }
     const int y = y0 + wave;
-    if (y >= a.H) return;
-    store_blocks<4, NB>(a, acc, bias, 0, NBT, b, y, x0, q, lr);
+    if (a.Yf)
+        store_row_fixed<4, NB, true>(a, acc, bias, b, y, y < a.H, x0, q, lr);
+    else
+        store_row_fixed<4, NB, false>(a, acc, bias, b, y, y < a.H, x0, q, lr);
 }
 
 // The same convolution with the input brought into LDS by direct loads (buffer_load_dwordx4 ...
@@ -1113,7 +1143,10 @@ static void conv_fwd(hipStream_t s, const ConvArgs& a, bool k32, int shape) {
         }
         // the LDS-staged kernel for every layer (it beats the per-wavefront kernel at all four
         // resolutions); shape 5 forces it, other nonzero shapes force the per-wavefront kernel
-        const bool lds = shape == 5 || shape == 6 || (shape == 0 && NBT <= 6);
+        // (its buffer loads and stores take 31-bit offsets: tensors under 2 GB)
+        const bool small = (double)a.B * a.H * a.W * std::max(std::max(a.C1, a.CO * (a.Yf ? 2 : 1)), 1) * 2 <
+                           2147483648.0;
+        const bool lds = small && (shape == 5 || shape == 6 || (shape == 0 && NBT <= 6));
         if (lds) {
             const long blocks = (long)a.B * ((a.H + 3) / 4) * ((a.W + 63) / 64);
             switch (NBT) {
