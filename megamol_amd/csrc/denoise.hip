@@ -39,6 +39,7 @@ __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint
 // instructions per value, which dominated the convolution epilogues
 __device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));  // loaded and stored as one value (never a memcpy)
 typedef float f32x2_t __attribute__((ext_vector_type(2)));
 // two values in one conversion: lo in bits 0-15, hi in bits 16-31
 __device__ __forceinline__ uint32_t f2bf2(float lo, float hi) {
@@ -361,15 +362,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const __amdgpu_buffer_rsrc_t r2 = __builtin_amdgcn_make_buffer_rsrc(
         (void*)(a.X2 ? a.X2 : a.X1), 0, a.X2 ? a.B * (a.H >> 1) * (a.W >> 1) * a.C2 * 2 : 0, 0x00020000);
     constexpr int kOut = (int)0x80000000;
-    for (int sl = 0; sl < nslice; ++sl) {
-        // stage the slice: 6 rows x 66 pixels x 4 groups of 8 channels (and its weights); every
-        // load is issued before any is stored
+    // stage a slice into registers: 6 rows x 66 pixels x 4 groups of 8 channels and, WLDS, its
+    // weights; every load is issued before any is stored
+    uint4 v[NI];
+    u32x4 wv[NW > 0 ? NW : 1];
+    auto off1 = [&](int yy, int xx, int ci0) { return (((b * a.H + yy) * a.W + xx) * a.C1 + ci0) * 2; };
+    auto off2 = [&](int yy, int xx, int ci0) {
+        return (((b * (a.H >> 1) + (yy >> 1)) * (a.W >> 1) + (xx >> 1)) * a.C2 + ci0 - a.C1) * 2;
+    };
+    auto stage = [&](int sl) {
         const bool from1 = sl * 32 < a.C1, from2 = sl * 32 + 32 > a.C1 && a.C2 > 0;  // uniform
-        auto off1 = [&](int yy, int xx, int ci0) { return (((b * a.H + yy) * a.W + xx) * a.C1 + ci0) * 2; };
-        auto off2 = [&](int yy, int xx, int ci0) {
-            return (((b * (a.H >> 1) + (yy >> 1)) * (a.W >> 1) + (xx >> 1)) * a.C2 + ci0 - a.C1) * 2;
-        };
-        uint4 v[NI];
         if (!(from1 && from2)) {  // the whole slice from one tensor: one load per piece
             const __amdgpu_buffer_rsrc_t rs = from1 ? r1 : r2;
 #pragma unroll
@@ -396,16 +398,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                                r2, in && ci0 >= a.C1 && ci0 < CI ? off2(yy, xx, ci0) : kOut, 0, 0));
             }
 #pragma unroll
-            for (int k = 0; k < NI; ++k) v[k] = make_uint4(v[k].x | v2[k].x, v[k].y | v2[k].y, v[k].z | v2[k].z, v[k].w | v2[k].w);
+            for (int k = 0; k < NI; ++k)
+                v[k] = make_uint4(v[k].x | v2[k].x, v[k].y | v2[k].y, v[k].z | v2[k].z, v[k].w | v2[k].w);
         }
-        uint4 wv[NW > 0 ? NW : 1];
+        const u32x4* w4 = reinterpret_cast<const u32x4*>(a.Wl);
 #pragma unroll
         for (int k = 0; k < NW; ++k) {
             const int i = threadIdx.x + k * 256;
             const int ii = i < NWT ? i : 0;
             const int t = ii / (NB * 64), r = ii - t * (NB * 64);  // r = nb * 64 + lane
-            wv[k] = wl4[((size_t)t * nslice + sl) * (NB * 64) + r];
+            wv[k] = w4[((size_t)t * nslice + sl) * (NB * 64) + r];
         }
+    };
+    // with the weights in LDS the next slice's loads are issued right after a slice is stored and
+    // land while its taps run; wider layers stream their weights from global memory during the
+    // taps, whose waits would then include those loads, so they stage each slice just before
+    // storing it
+    if constexpr (WLDS) stage(0);
+    for (int sl = 0; sl < nslice; ++sl) {
+        if constexpr (!WLDS) stage(sl);
 #pragma unroll
         for (int k = 0; k < NI; ++k) {
             const int i = threadIdx.x + k * 256;  // pieces past NT land in the padding rows
@@ -413,8 +424,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             *reinterpret_cast<uint4*>(tile + (ry * LT_PX + pxl) * LT_CP + g8 * 8) = v[k];
         }
 #pragma unroll
-        for (int k = 0; k < NW; ++k) s_w[threadIdx.x + k * 256] = wv[k];
+        for (int k = 0; k < NW; ++k) reinterpret_cast<u32x4*>(s_w)[threadIdx.x + k * 256] = wv[k];
         __syncthreads();
+        if constexpr (WLDS)
+            if (sl + 1 < nslice) stage(sl + 1);
         auto load_w = [&](int t, Frag8 (&fb)[NB]) {
 #pragma unroll
             for (int j = 0; j < NB; ++j) {
