@@ -408,6 +408,8 @@ def denoised_leg(r, scene, args, cam, p_plain, plain_cfg=None):
            "ssim_vs_256spp": {"denoised_1spp": round(ssim(den, ref), 4), "plain": round(ssim(plain, ref), 4),
                               "raw_1spp": round(ssim(raw, ref), 4), "box5_filter_1spp": round(ssim(box, ref), 4)},
            "weights": trained, "weights_meta": meta, "autotune_1spp": tuned1, "chunk_pass": chunk}
+    if args.footprint > 0:
+        res["footprint"] = footprint_variant(r, cam, args, ref, raw_aux, den, ms, colour)
     if curve:
         # the plain frame of equal quality: PSNR interpolated linearly in log2(spp) between the two
         # measured counts around the denoised frame's PSNR, and the time the same way
@@ -440,6 +442,56 @@ def denoised_leg(r, scene, args, cam, p_plain, plain_cfg=None):
                             raw=raw[sl].astype(np.float16), den=den[sl].astype(np.float16),
                             alb256=alb[sl].astype(np.float16), feats=feats[sl])
     return res
+
+
+def footprint_variant(r, cam, args, ref, raw_aux, den, ms, colour):
+    """The albedo-aliasing ceiling of a one-sample frame and a variant past it: a primary-only
+    pass of K jittered samples per pixel (AO distance ~0, so every sample on a sphere is
+    unoccluded and the pixel colour is the mean over the samples of the albedo, or the black
+    background on a miss: the pixel's footprint albedo with its coverage) replaces the one-sample
+    albedo the denoised frame is modulated with. The network is not retrained: its output is
+    divided by the sample-0 albedo (the visibility it estimates) and multiplied by the footprint
+    albedo; sample-0 misses take the mean estimated visibility of the hit pixels in their 5x5
+    window. Reported beside the strict 1-spp frame with the footprint pass's cost added."""
+    from scipy.ndimage import uniform_filter
+    from megamol_amd import _lib as L, params
+    from megamol_amd.denoiser import psnr
+    W, H, K = args.width, args.height, args.footprint
+    pf = params(W, H, spp=K, mode=L.MODE_AO, ao_distance=1e-7, frame_idx=4242, flags=L.FLAG_NO_DOWNLOAD)
+    for _ in range(2):
+        r.render_async(cam, pf)
+    r.sync()
+    t0 = time.perf_counter()
+    reps = max(2, min(args.steps, 6))
+    for _ in range(reps):
+        r.render_async(cam, pf)
+    r.sync()
+    fp_ms = 1e3 * (time.perf_counter() - t0) / reps
+    fp = colour(pf)
+    # the reference's footprint albedo (256 samples) and, where it is nonzero, the reference's
+    # visibility (the 256-spp AO colour over it)
+    alb256 = colour(params(W, H, spp=256, mode=L.MODE_AO, ao_distance=1e-7, frame_idx=7777))
+    a = raw_aux.astype(np.float32)
+    hit = a[..., 6] > 0
+    alb1 = a[..., 0:3] * hit[..., None]
+    eps = 1e-4
+    vref = np.where(alb256 > eps, ref / np.maximum(alb256, eps), 0.0)
+    vnet = np.where(alb1 > eps, den / np.maximum(alb1, eps), 0.0).mean(-1)
+    vnet = np.clip(vnet, 0.0, 1.0) * hit
+    num = uniform_filter(vnet, 5, mode="constant")
+    cnt = uniform_filter(hit.astype(np.float32), 5, mode="constant")
+    vfill = np.where(cnt > 0, num / np.maximum(cnt, 1e-6), 0.0)
+    v = np.where(hit, vnet, vfill)
+    variant = fp * v[..., None]
+    out = {"samples": K, "footprint_pass_ms": round(fp_ms, 3),
+           "frame_ms": round(ms + fp_ms, 3),
+           "psnr_db_vs_256spp": {
+               "one_sample_albedo_x_reference_visibility": round(psnr(alb1 * vref, ref), 2),
+               "footprint_albedo_x_reference_visibility": round(psnr(fp * vref, ref), 2),
+               "variant_footprint_albedo_x_network_visibility": round(psnr(variant, ref), 2),
+               "strict_denoised_1spp": round(psnr(den, ref), 2)}}
+    log(f"footprint variant: {out}")
+    return out
 
 
 def pipelined_frames(scene, args, cam, opts_plain, opts_dn, weights, depth_scale):
@@ -613,6 +665,8 @@ def main():
     ap.add_argument("--series-n", type=int, default=4_000_000)
     ap.add_argument("--series-t", type=int, default=64)
     ap.add_argument("--autotune", action="store_true", help="series config: tune refit period and launch knobs")
+    ap.add_argument("--footprint", type=int, default=8,
+                    help="samples of the primary-only footprint pass of the labelled variant (0: skip it)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="fcc/cube configs: keep the library's default launch knobs instead of tuning them")
     ap.add_argument("--no-denoise", action="store_true")
@@ -841,6 +895,30 @@ def main():
                                  "peak": round(VALU_ISSUE_PEAK / 1e9, 1), "unit": "G wave-instructions/s",
                                  "frac": round(ach / VALU_ISSUE_PEAK, 4), "valu_instructions": int(valu)})
     value = dn["ms"] if dn else ms
+    # the host boundary, outside `value` (whose inputs are resident in HBM): the MultiParticleDataCall
+    # lists uploaded again (pageable host memory -> device, decoded there; what a data change costs)
+    # and the CPU framebuffer (RGBA8 + depth) downloaded, each timed on the wall clock around a
+    # synchronised call
+    host_io = None
+    if args.config == "fcc":
+        ups, downs = [], []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            r.set_lists(scene["lists"], data_hash=1)
+            ups.append((time.perf_counter() - t0) * 1e3)
+        r.build(L.BUILD_REBUILD, args.bits)
+        r.render(cam, p, outputs=())
+        for _ in range(5):
+            t0 = time.perf_counter()
+            r.download(args.width, args.height, outputs=("rgba8", "depth"))
+            downs.append((time.perf_counter() - t0) * 1e3)
+        up, down = float(np.median(ups)), float(np.median(downs))
+        nbytes = sum(int(l.vert.nbytes) + (int(l.col.nbytes) if l.col is not None else 0) for l in scene["lists"])
+        host_io = {"upload_ms": round(up, 2), "upload_bytes": nbytes, "upload_gb_per_s": round(nbytes / up / 1e6, 1),
+                   "download_ms": round(down, 3), "download_bytes": args.width * args.height * 8,
+                   "frame_with_upload_and_download_ms": round(value + up + down, 2),
+                   "frame_with_download_ms": round(value + down, 3)}
+        log(f"host boundary: {host_io}")
     # roofline of the dominant kernel of the frame that `value` times: the chunk pass of the
     # denoised frame's build (the plain frame's trace kernel when there is no denoised frame)
     if dn:
@@ -889,6 +967,7 @@ def main():
         "orbit": orbit,
         "structures": structures,
         "pipelined": pipelined,
+        "host_io": host_io,
     }
     print(json.dumps(out), flush=True)
     r.close()
