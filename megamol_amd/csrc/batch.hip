@@ -164,6 +164,8 @@ int32_t mmhrt_render_batch(mmhrt_ctx c, const mmhrt_job* jobs, uint32_t n_jobs, 
     // The builds read the timestep slices and their count; the context's own spheres, count and
     // LBVH state come back when this call returns (the tree in the buffers is then the last
     // timestep's, so the context needs a build before its next render).
+    // The per-timestep structures are released too: they are rebuilt by every call, and holding
+    // them (one LBVH per timestep) would keep memory a later, larger scene needs.
     struct Restore {
         mmhrt_ctx_s* c;
         uint64_t n;
@@ -172,6 +174,16 @@ int32_t mmhrt_render_batch(mmhrt_ctx c, const mmhrt_job* jobs, uint32_t n_jobs, 
             c->n = n;
             c->built = false;
             c->info_valid = false;
+            for (auto& sl : c->ts_slots) {
+                release(sl.nodes);
+                release(sl.leaf);
+                release(sl.id);
+                release(sl.info);
+                release(sl.dim);
+                release(sl.box);
+                release(sl.rmax);
+            }
+            c->ts_slots.clear();
         }
     } restore{c, c->n};
     // every buffer the per-timestep build and trace touch is allocated before any capture
@@ -192,17 +204,20 @@ int32_t mmhrt_render_batch(mmhrt_ctx c, const mmhrt_job* jobs, uint32_t n_jobs, 
     const uint32_t period = tl ? 1u : (uint32_t)refit_period;
     const uint32_t G = (T - 1) / period + 1;  // rebuild groups
     if (one && T > 1) {
+        // sized for this series (the context's own buffers may be larger, from an earlier scene)
+        const uint64_t n = c->ts_n;
+        const size_t n_nodes = tl ? (size_t)c->pkd_L - 1 : (size_t)(n > 1 ? n - 1 : 1);
         if (c->ts_slots.size() < T - 1) c->ts_slots.resize(T - 1);
         for (uint32_t t = 0; t + 1 < T; ++t) {
             SeriesSlot& sl = c->ts_slots[t];
-            if ((e = ensure(c, sl.nodes, c->nodes.cap, "series nodes"))) return e;
-            if ((e = ensure(c, sl.leaf, c->leaf_sph.cap, "series leaf spheres"))) return e;
+            if ((e = ensure(c, sl.nodes, n_nodes, "series nodes"))) return e;
+            if ((e = ensure(c, sl.leaf, n, "series leaf spheres"))) return e;
             if ((e = ensure(c, sl.info, 1, "series scene info"))) return e;
-            if (t + 1 < G && (e = ensure(c, sl.id, c->leaf_id.cap, "series leaf ids"))) return e;
+            if (t + 1 < G && (e = ensure(c, sl.id, n, "series leaf ids"))) return e;
             if (tl) {
-                if ((e = ensure(c, sl.dim, c->pkd_dim.cap, "series split axes"))) return e;
-                if ((e = ensure(c, sl.box, c->tl_box.cap, "series treelet boxes"))) return e;
-                if ((e = ensure(c, sl.rmax, c->tl_rmax.cap, "series treelet radii"))) return e;
+                if ((e = ensure(c, sl.dim, n, "series split axes"))) return e;
+                if ((e = ensure(c, sl.box, (size_t)c->pkd_T * 6, "series treelet boxes"))) return e;
+                if ((e = ensure(c, sl.rmax, c->pkd_T, "series treelet radii"))) return e;
             }
         }
     }
