@@ -554,8 +554,10 @@ __global__ __launch_bounds__(256) void k_conv_fwd_dma(ConvArgs a) {
                 ok ? corner + rel[k] : (int)0x80000000, 0, 0, 0);
         }
     };
+    // the 9 taps unrolled: in a rolled loop the compiler moved the accumulators between AGPRs on every
+    // iteration (24 v_accvgpr moves per tap, as many VALU instructions as the tap's address math)
     auto compute = [&](const uint4* buf, f32x4 (&acc)[4][NB]) {
-#pragma unroll 1
+#pragma unroll
         for (int t = 0; t < 9; ++t) {
             Frag8 cw[NB];
 #pragma unroll
