@@ -439,7 +439,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         };
         Frag8 fw[NB];
         load_w(0, fw);
-#pragma unroll 1
+        // A rolled tap loop costs accumulator moves between AGPRs on every iteration; the
+        // unrolled one costs registers. NB <= 3 tiles are held to two workgroups per CU by
+        // their LDS anyway, so they unroll; NB >= 4 would drop from four waves per SIMD to two.
+        constexpr int kTapUnroll = NB <= 3 ? 9 : 1;
+#pragma unroll kTapUnroll
         for (int t = 0; t < 9; ++t) {
             Frag8 cw[NB];
 #pragma unroll
