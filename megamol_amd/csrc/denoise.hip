@@ -86,46 +86,78 @@ __device__ __forceinline__ void store_blocks(const ConvArgs& a, const f32x4 (&ac
     }
 }
 
-// The same epilogue for one output row with a fixed number of stores: every store is a buffer
-// store over the row (bytes [0, W * channels * size)), and a pixel or channel outside it gets an
-// offset past the end, which the hardware drops. No store sits in a branch, so every path issues
-// exactly PB * NB * 4 of them, and a wait for loads issued before them can count them
-// (vmcnt(PB * NB * 4)) instead of waiting for the stores too. row_ok false drops the whole row.
-template <int PB, int NB, bool F32>
-__device__ __forceinline__ void store_row_fixed(const ConvArgs& a, const f32x4 (&acc)[PB][NB],
-                                                const float (&bias)[NB], int b, int y, bool row_ok, int x0, int q,
-                                                int lr) {
+// Epilogue of one output row for blocks computed as weights x activations (the weight fragment
+// as the MFMA's A operand): lane (q, lr) holds channels 16 nb + 4q + e, e < 4, of pixel
+// x0 + 16 pb + lr, so its four values are adjacent in NHWC and leave in one 8-byte (bf16) or
+// 16-byte (fp32) store: PB * NB stores per lane (pixels as the A operand would give one channel
+// of 4 pixels per lane and 4x the stores). The accumulators started from the bias (the first
+// MFMA's C operand), so nothing is added here. ReLU on bf16 output is one packed 16-bit integer
+// max per channel pair after rounding: a bf16 with the sign bit set is a negative int16, and
+// rounding keeps the sign, so max(round(v), 0) = round(max(v, 0)). Channel counts written are
+// multiples of 4 (16 NB, or 4 for the fp32 output layer). Every store is a buffer store over the
+// row (bytes [0, W * channels * size)), and a pixel or channel outside it gets an offset past the
+// end, which the hardware drops. No store sits in a branch, so every path issues exactly PB * NB
+// of them, and a wait for loads issued before them can count them instead of waiting for the
+// stores too. row_ok false drops the whole row.
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t relu_bf16x2(uint32_t v) {
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(s16x2, v), (s16x2){0, 0}));
+}
+template <int PB, int NB, bool F32, bool RELU>
+__device__ __forceinline__ void store_row_t(const ConvArgs& a, const f32x4 (&acc)[PB][NB], int b, int y, bool row_ok,
+                                            int x0, int q, int lr) {
     const int yc = F32 ? (a.YfC ? a.YfC : a.CO) : a.CO;
     const size_t rowo = ((size_t)b * a.H + (row_ok ? y : 0)) * a.W;
-    // the row is the same for the whole wavefront: keep the descriptor in scalar registers
     const uint64_t ba = F32 ? (uint64_t)(a.Yf + rowo * yc) : (uint64_t)(a.Y + rowo * a.CO);
     const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)ba);
     const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(ba >> 32));
     void* base = (void*)(((uint64_t)hi << 32) | lo);
-    const int row_bytes = __builtin_amdgcn_readfirstlane(row_ok ? a.W * yc * (F32 ? 4 : 2) : 0);
+    constexpr int S = F32 ? 4 : 2;  // bytes per value
+    const int row_bytes = __builtin_amdgcn_readfirstlane(row_ok ? a.W * yc * S : 0);
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(base, 0, row_bytes, 0x00020000);
     constexpr int kOut = 0x40000000;  // an offset past any row
 #pragma unroll
-    for (int j = 0; j < NB; ++j) {
-        const int co = j * 16 + lr;
+    for (int pb = 0; pb < PB; ++pb) {
+        const int x = x0 + pb * 16 + lr;
+        const int px = x < a.W ? x * yc * S : kOut;
 #pragma unroll
-        for (int pb = 0; pb < PB; ++pb) {
+        for (int j = 0; j < NB; ++j) {
+            const int co = j * 16 + q * 4;
+            const int off = co < yc ? px + co * S : kOut;
+            const f32x4 v = acc[pb][j];
+            if constexpr (F32) {
+                float w[4];
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                float v = acc[pb][j][e] + bias[j];
-                if (a.relu) v = fmaxf(v, 0.0f);
-                const int x = x0 + pb * 16 + q * 4 + e;
-                const bool ok = x < a.W && co < yc;
-                if constexpr (F32) {
-                    const int off = ok ? (x * yc + co) * 4 : kOut;
-                    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), rs, off, 0, 0);
-                } else {
-                    const int off = ok ? (x * yc + co) * 2 : kOut;
-                    __builtin_amdgcn_raw_buffer_store_b16(f2bf(v), rs, off, 0, 0);
+                for (int e = 0; e < 4; ++e) w[e] = RELU ? fmaxf(v[e], 0.0f) : v[e];
+                __builtin_amdgcn_raw_buffer_store_b128(
+                    (u32x4){__float_as_uint(w[0]), __float_as_uint(w[1]), __float_as_uint(w[2]), __float_as_uint(w[3])},
+                    rs, off, 0, 0);
+            } else {
+                uint32_t p0 = f2bf2(v[0], v[1]), p1 = f2bf2(v[2], v[3]);
+                if constexpr (RELU) {
+                    p0 = relu_bf16x2(p0);
+                    p1 = relu_bf16x2(p1);
                 }
+                __builtin_amdgcn_raw_buffer_store_b64((u32x2){p0, p1}, rs, off, 0, 0);
             }
         }
     }
+}
+template <int PB, int NB, bool F32>
+__device__ __forceinline__ void store_row_t(const ConvArgs& a, const f32x4 (&acc)[PB][NB], int b, int y, bool row_ok,
+                                            int x0, int q, int lr) {
+    if (a.relu)
+        store_row_t<PB, NB, F32, true>(a, acc, b, y, row_ok, x0, q, lr);
+    else
+        store_row_t<PB, NB, F32, false>(a, acc, b, y, row_ok, x0, q, lr);
+}
+
+// bias of the channels 16 nb + 4q + e a lane holds in the layout above
+template <int NB>
+__device__ __forceinline__ void load_bias_t(const ConvArgs& a, int q, f32x4 (&bias)[NB]) {
+#pragma unroll
+    for (int j = 0; j < NB; ++j) bias[j] = *reinterpret_cast<const f32x4*>(a.bias + j * 16 + q * 4);
 }
 
 union Frag {
@@ -334,18 +366,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const int rb = blk % rblocks;
     const int b = blk / rblocks;
     const int x0 = run * 64, y0 = rb * 4;
-    const int NBT = a.CO / 16;  // == NB: the launch picks the instance
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: the store row
     const int lane = __lane_id();
     const int q = lane >> 4, lr = lane & 15;
-    f32x4 acc[4][NB];
+    f32x4 acc[4][NB];  // start from the bias (channels 16 j + 4q + e of the lane's pixel)
+    {
+        f32x4 bias[NB];
+        load_bias_t<NB>(a, q, bias);
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < NB; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    float bias[NB];
-#pragma unroll
-    for (int j = 0; j < NB; ++j) bias[j] = a.bias[(j < NBT ? j : 0) * 16 + lr];
+            for (int j = 0; j < NB; ++j) acc[i][j] = bias[j];
+    }
     const uint4* wl4 = reinterpret_cast<const uint4*>(a.Wl);
     // A piece's place in the tile (row, pixel, 8-channel group) is fixed per thread; its source is
     // X1 (channels below C1) or the half-resolution X2 (nearest upsampled), read with buffer loads:
@@ -459,15 +491,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             for (int j = 0; j < NB; ++j)
 #pragma unroll
                 for (int pb = 0; pb < 4; ++pb)
-                    acc[pb][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[pb].v, cw[j].v, acc[pb][j], 0, 0, 0);
+                    acc[pb][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cw[j].v, fa[pb].v, acc[pb][j], 0, 0, 0);
         }
         __syncthreads();
     }
     const int y = y0 + wave;
     if (a.Yf)
-        store_row_fixed<4, NB, true>(a, acc, bias, b, y, y < a.H, x0, q, lr);
+        store_row_t<4, NB, true>(a, acc, b, y, y < a.H, x0, q, lr);
     else
-        store_row_fixed<4, NB, false>(a, acc, bias, b, y, y < a.H, x0, q, lr);
+        store_row_t<4, NB, false>(a, acc, b, y, y < a.H, x0, q, lr);
 }
 
 // The same convolution with the input brought into LDS by direct loads (buffer_load_dwordx4 ...
@@ -505,11 +537,10 @@ __global__ __launch_bounds__(256) void k_conv_fwd_dma(ConvArgs a) {
     if (tl >= ntiles) return;
     // s_waitcnt immediate (gfx9 layout: vmcnt bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8) that
     // waits until at most kStores vector memory operations and no LDS operation are outstanding
-    constexpr int kStores = 4 * NB * 4;
+    constexpr int kStores = 4 * NB;  // store_row_t: one store per 16-pixel block and channel block
     constexpr int kWaitLoads = (kStores & 15) | ((kStores >> 4) << 14) | (7 << 4);  // and lgkmcnt(0)
-    float bias[NB];
-#pragma unroll
-    for (int j = 0; j < NB; ++j) bias[j] = a.bias[j * 16 + lr];
+    f32x4 bias[NB];
+    load_bias_t<NB>(a, q, bias);
     {  // the layer's weights, once (slot = (t * NB + nb) * 64 + lane)
         uint4 wv[NWI];
 #pragma unroll
@@ -578,15 +609,17 @@ __global__ __launch_bounds__(256) void k_conv_fwd_dma(ConvArgs a) {
             for (int j = 0; j < NB; ++j)
 #pragma unroll
                 for (int pb = 0; pb < 4; ++pb)
-                    acc[pb][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[pb].v, cw[j].v, acc[pb][j], 0, 0, 0);
+                    acc[pb][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cw[j].v, fa[pb].v, acc[pb][j], 0, 0, 0);
         }
     };
     issue(tA, tl);
-    f32x4 acc[4][NB];
+    // the first item's loads are followed by no stores: wait for them outright (vmcnt(0))
+    __builtin_amdgcn_s_waitcnt(7 << 4);
+    f32x4 acc[4][NB];  // every item starts from the bias, which the first tap's MFMAs take as C
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < NB; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int j = 0; j < NB; ++j) acc[i][j] = bias[j];
     // one step: wait for this item, start the next item into the other buffer, compute this one
     // and store it. The barrier opening a step is the only one: before it each wave waits for its
     // own loads into `cur` (issued a whole step earlier; the stores issued after them have a fixed
@@ -595,9 +628,9 @@ __global__ __launch_bounds__(256) void k_conv_fwd_dma(ConvArgs a) {
     // buffers are distinct arrays, so the compiler knows that reads of one need not wait for loads
     // into the other.
     auto step = [&](const uint4* cur, uint4* nxt) -> bool {
-        // this wave's loads into cur were followed by exactly kStores stores (none before the
-        // first step, where the wait is already met); vector memory operations complete in
-        // issue order, so vmcnt(kStores) means the loads have landed
+        // this wave's loads into cur were followed by exactly kStores stores (before the first
+        // step there are none, and the wait after the first issue below covers it); vector memory
+        // operations complete in issue order, so vmcnt(kStores) means the loads have landed
         // (a plain barrier: __syncthreads' fence would also wait for the stores). The signal
         // fences keep the compiler from moving LDS accesses across it.
         __builtin_amdgcn_s_waitcnt(kWaitLoads);
@@ -613,11 +646,11 @@ __global__ __launch_bounds__(256) void k_conv_fwd_dma(ConvArgs a) {
         int b, x0, y0;
         tile_origin(tl, b, x0, y0);
         const int y = y0 + wave;
-        store_row_fixed<4, NB, F32>(a, acc, bias, b, y, y < a.H, x0, q, lr);
+        store_row_t<4, NB, F32>(a, acc, b, y, y < a.H, x0, q, lr);
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
-            for (int j = 0; j < NB; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+            for (int j = 0; j < NB; ++j) acc[i][j] = bias[j];
         tl = ntl;
         return more;
     };
