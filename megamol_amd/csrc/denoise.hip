@@ -460,46 +460,80 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         __syncthreads();
         if constexpr (WLDS)
             if (sl + 1 < nslice) stage(sl + 1);
-        auto load_w = [&](int t, Frag8 (&fb)[NB]) {
+        if constexpr (WLDS) {
+            // wavefront w computes pixels x0 + 16w .. + 15 of all 4 rows (acc[row][nb]): the
+            // fragment of one input row and tap column feeds every output row it reaches (up to 3),
+            // so a slice takes 18 activation and 9 NB weight fragment reads for 36 NB MFMAs (one row
+            // x 64 pixels per wavefront took 36 and 9 NB)
 #pragma unroll
-            for (int j = 0; j < NB; ++j) {
-                if constexpr (WLDS)
-                    fb[j].u = s_w[(t * NB + j) * 64 + lane];
-                else
-                    fb[j].u = wl4[(((size_t)t * nslice + sl) * NB + j) * 64 + lane];
+            for (int dx = 0; dx < 3; ++dx) {
+                Frag8 cw[3][NB];
+#pragma unroll
+                for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+                    for (int j = 0; j < NB; ++j) cw[dy][j].u = s_w[((dy * 3 + dx) * NB + j) * 64 + lane];
+#pragma unroll
+                for (int ry = 0; ry < LT_ROWS; ++ry) {
+                    Frag8 fa;
+                    fa.u = *reinterpret_cast<const uint4*>(tile + (ry * LT_PX + wave * 16 + lr + dx) * LT_CP + q * 8);
+#pragma unroll
+                    for (int orow = 0; orow < 4; ++orow) {
+                        const int dy = ry - orow;  // input row y0 - 1 + ry = output row y0 + orow + dy - 1
+                        if (dy < 0 || dy > 2) continue;
+#pragma unroll
+                        for (int j = 0; j < NB; ++j)
+                            acc[orow][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cw[dy][j].v, fa.v, acc[orow][j], 0, 0, 0);
+                    }
+                }
             }
-        };
-        Frag8 fw[NB];
-        load_w(0, fw);
-        // A rolled tap loop costs accumulator moves between AGPRs on every iteration; the
-        // unrolled one costs registers. NB <= 3 tiles are held to two workgroups per CU by
-        // their LDS anyway, so they unroll; NB >= 4 would drop from four waves per SIMD to two.
-        constexpr int kTapUnroll = NB <= 3 ? 9 : 1;
-#pragma unroll kTapUnroll
-        for (int t = 0; t < 9; ++t) {
-            Frag8 cw[NB];
+        } else {
+            // wavefront w computes row y0 + w (acc[pb][nb]: pixels x0 + 16 pb ..), its weights
+            // read from global memory one tap ahead; the tap loop stays rolled (unrolled, its
+            // registers would cost these kernels half their occupancy)
+            auto load_w = [&](int t, Frag8 (&fb)[NB]) {
 #pragma unroll
-            for (int j = 0; j < NB; ++j) cw[j] = fw[j];
-            if (t < 8) load_w(t + 1, fw);
-            const int ry = wave + t / 3;  // tile row of input row y + t/3 - 1
-            const int dx = t % 3;         // tile pixel offset = output pixel - x0 + dx
-            Frag8 fa[4];
+                for (int j = 0; j < NB; ++j) fb[j].u = wl4[(((size_t)t * nslice + sl) * NB + j) * 64 + lane];
+            };
+            Frag8 fw[NB];
+            load_w(0, fw);
+#pragma unroll 1
+            for (int t = 0; t < 9; ++t) {
+                Frag8 cw[NB];
 #pragma unroll
-            for (int pb = 0; pb < 4; ++pb)
-                fa[pb].u = *reinterpret_cast<const uint4*>(tile + (ry * LT_PX + pb * 16 + lr + dx) * LT_CP + q * 8);
-#pragma unroll
-            for (int j = 0; j < NB; ++j)
+                for (int j = 0; j < NB; ++j) cw[j] = fw[j];
+                if (t < 8) load_w(t + 1, fw);
+                const int ry = wave + t / 3;  // tile row of input row y + t/3 - 1
+                const int dx = t % 3;         // tile pixel offset = output pixel - x0 + dx
+                Frag8 fa[4];
 #pragma unroll
                 for (int pb = 0; pb < 4; ++pb)
-                    acc[pb][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cw[j].v, fa[pb].v, acc[pb][j], 0, 0, 0);
+                    fa[pb].u = *reinterpret_cast<const uint4*>(tile + (ry * LT_PX + pb * 16 + lr + dx) * LT_CP + q * 8);
+#pragma unroll
+                for (int j = 0; j < NB; ++j)
+#pragma unroll
+                    for (int pb = 0; pb < 4; ++pb)
+                        acc[pb][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cw[j].v, fa[pb].v, acc[pb][j], 0, 0, 0);
+            }
         }
         __syncthreads();
     }
-    const int y = y0 + wave;
-    if (a.Yf)
-        store_row_t<4, NB, true>(a, acc, b, y, y < a.H, x0, q, lr);
-    else
-        store_row_t<4, NB, false>(a, acc, b, y, y < a.H, x0, q, lr);
+    if constexpr (WLDS) {  // acc[row]: the wavefront's 16 pixels of row y0 + row
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int y = y0 + r;
+            const f32x4(&ar)[1][NB] = *reinterpret_cast<const f32x4(*)[1][NB]>(&acc[r]);
+            if (a.Yf)
+                store_row_t<1, NB, true>(a, ar, b, y, y < a.H, x0 + 16 * wave, q, lr);
+            else
+                store_row_t<1, NB, false>(a, ar, b, y, y < a.H, x0 + 16 * wave, q, lr);
+        }
+    } else {  // acc[pb]: pixels x0 + 16 pb .. of row y0 + wave
+        const int y = y0 + wave;
+        if (a.Yf)
+            store_row_t<4, NB, true>(a, acc, b, y, y < a.H, x0, q, lr);
+        else
+            store_row_t<4, NB, false>(a, acc, b, y, y < a.H, x0, q, lr);
+    }
 }
 
 // The same convolution with the input brought into LDS by direct loads (buffer_load_dwordx4 ...
