@@ -17,7 +17,9 @@
 // B holds column l%16, k = 4*(l/16)..+3; the 4 results of a lane are rows 4*(l/16)..+3 of
 // column l%16. v_mfma_f32_16x16x32_bf16 is the same with 8 k values per lane (k = 8*(l/16)..+7).
 // The forward pass uses the latter over K = 9 taps x CI input channels taken 32 at a time (a
-// lane's 8 values are 8 consecutive channels of one tap, one 16-byte load).
+// lane's 8 values are 8 consecutive channels of one tap, one 16-byte load). The LDS-staged and
+// DMA forward kernels take the weights as A and 16 pixels as B, so that a lane's 4 results are 4
+// adjacent channels of one pixel (one store).
 #include <math.h>
 
 #include <algorithm>
@@ -335,12 +337,13 @@ __global__ __launch_bounds__(256) void k_conv_fwd32(ConvArgs a) {
     store_blocks<PB, NB>(a, acc, bias, nb0, NBT, b, y, x0, q, lr);
 }
 
-// The same convolution staged through LDS, for the full-resolution layers. A workgroup (4
-// wavefronts) computes 4 rows x 64 pixels x all output channels. For each 32-channel slice of
-// the input it stages rows y0-1..y0+4, pixels x0-1..x0+64 in LDS (the decoder's [skip,
-// upsampled] concatenation is resolved here, once per element), then every wavefront runs the 9
-// taps of its row out of LDS. Pixels are padded to 40 channels in LDS so that the 16 lanes of an
-// A-fragment read land on distinct banks. Weights: [tap][slice][co/16][lane][8], zero past CI.
+// The same convolution staged through LDS, at every resolution. A workgroup (4 wavefronts)
+// computes 4 rows x 64 pixels x all output channels. For each 32-channel slice of the input it
+// stages rows y0-1..y0+4, pixels x0-1..x0+64 in LDS (the decoder's [skip, upsampled]
+// concatenation is resolved here, once per element), then the wavefronts run the 9 taps out of
+// LDS: up to 48 output channels each takes 16 pixels of all 4 rows, wider layers one row each.
+// Pixels are padded to 40 channels in LDS so that the 16 lanes of an activation-fragment read
+// land on distinct banks. Weights: [tap][slice][co/16][lane][8], zero past CI.
 constexpr int LT_PX = 66, LT_ROWS = 6, LT_CP = 40;
 
 template <int NB>
@@ -542,7 +545,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 // the data in registers. The input is X1 only (C2 = 0) and at most 2 GB, so that the buffer
 // offsets fit 31 bits (the launch checks both). The tile is unpadded, and a
 // pixel's four 8-channel groups are stored XOR-swizzled (slot 4P + (g ^ ((P >> 1) & 3)) for tile
-// pixel P), so that the 8 consecutive pixels of an A-fragment read phase cover all 32 banks.
+// pixel P), so that the 8 consecutive pixels of an activation-fragment read phase cover all 32 banks.
 // Pieces outside the image or past CI load 16 zero bytes (out-of-range buffer loads), so every wave
 // issues the same fixed number of loads per item, and the compiler can count them when it waits
 // for anything else. The kernel serves single-slice layers (CI <= 32) of up to 32 output
