@@ -898,31 +898,25 @@ __global__ __launch_bounds__(BC) void k_bu_chunk(const K* __restrict__ keys, con
     ap.flush(list, cnt + 1);
 }
 
-// The chunk pass, level-synchronous: the same tree, numbering and meeting rules as
-// k_bu_chunk, but the climb runs in rounds over a compact LDS list of the splits whose two
+// The chunk pass, level-synchronous and persistent: the same tree, numbering and meeting rules as
+// k_bu_chunk, but the climb runs in rounds over compact LDS lists of the splits whose two
 // children have arrived, so a wavefront's lanes stay busy instead of following the few long
-// climbs. Round 0 delivers the leaves; round r merges the splits completed in round r - 1,
-// writes their nodes and delivers them to their parents. LDS records are chunk-relative 16-bit
-// (a reference is the leaf or node index minus the chunk start, bit 15 marking a leaf).
-// n >= 2.
-// The chunk pass, level-synchronous and persistent: workgroup g takes chunks g, g + G, g + 2G, ...
-// (G = the grid) and also gathers the leaf spheres into Morton order (leaf_sph[k] = sph[ids[k]]). Round 0 delivers the leaves, each later round merges
-// the splits that the previous round completed; completed splits are kept in compact LDS lists,
-// so the active items stay packed in the first waves. Rounds touch LDS only: deliveries to the
-// two boundary splits of a chunk write their global meeting record at once but take their
-// arrival count after the rounds, together with the in-chunk splits left with one arrival. The
-// loads of the next chunk's spheres and keys (and of the ids one chunk further) are issued just
-// before those arrival counts, so that the one wait for global memory per chunk covers both.
-// n >= 2.
+// climbs. Workgroup g takes chunks g, g + G, g + 2G, ... (G = the grid) and also gathers the leaf
+// spheres into Morton order (leaf_sph[k] = sph[ids[k]]). Round 0 delivers the leaves; round r
+// merges the splits completed in round r - 1, writes their nodes and delivers them to their
+// parents, and the active items stay packed in the first waves. LDS records are chunk-relative
+// 16-bit (a reference is the leaf or node index minus the chunk start, bit 15 marking a leaf).
+// Rounds touch LDS only: deliveries to the two boundary splits of a chunk write their global
+// meeting record at once but take their arrival count after the rounds, together with the
+// in-chunk splits left with one arrival. The loads of the next chunk's spheres and keys (and of
+// the ids one chunk further) are issued just before those arrival counts, so that the one wait
+// for global memory per chunk covers both. n >= 2.
 // Two 1024-thread workgroups per CU (their LDS allows two) are 8 waves per SIMD: at most 64 VGPRs.
 template <typename K, int BC>
 __global__ __launch_bounds__(BC) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_bu_chunk_ls(
-    const K* __restrict__ keys, const uint32_t* __restrict__ ids,
-                                                    const float4* __restrict__ sph, float4* __restrict__ leaf_sph,
-                                                    int64_t n64,
-                                                    Node* __restrict__ nodes, Node* meet, int4* meet_far,
-                                                    uint32_t* flags, uint32_t* list, uint32_t* cnt, SceneInfo* info,
-                                                    int wave_items) {
+    const K* __restrict__ keys, const uint32_t* __restrict__ ids, const float4* __restrict__ sph,
+    float4* __restrict__ leaf_sph, int64_t n64, Node* __restrict__ nodes, Node* meet, int4* meet_far, uint32_t* flags,
+    uint32_t* list, uint32_t* cnt, SceneInfo* info, int wave_items) {
     __shared__ float s_box[BC][12];     // the two child boxes parked at each in-chunk split
     __shared__ uint16_t s_ref[BC][2];   // chunk-relative child references
     __shared__ uint16_t s_far[BC][2];   // chunk-relative far range ends
