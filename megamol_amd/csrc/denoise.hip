@@ -344,40 +344,53 @@ __global__ __launch_bounds__(256) void k_conv_fwd32(ConvArgs a) {
 // LDS: up to 48 output channels each takes 16 pixels of all 4 rows, wider layers one row each.
 // Pixels are padded to 40 channels in LDS so that the 16 lanes of an activation-fragment read
 // land on distinct banks. Weights: [tap][slice][co/16][lane][8], zero past CI.
-constexpr int LT_PX = 66, LT_ROWS = 6, LT_CP = 40;
+constexpr int LT_PX = 66, LT_CP = 40;
+// output rows of a tile: 8 up to 32 output channels (a wavefront's 16 pixels x 8 rows x NB blocks of
+// accumulators fit its registers beside the staged slice), 4 above
+__host__ __device__ constexpr int lds_tile_rows(int nb) { return nb <= 2 ? 8 : 4; }
 
 template <int NB>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_conv_fwd_lds(ConvArgs a) {
     // up to 3 output-channel blocks the slice's weights (9 * NB KB) sit in LDS beside the tile,
     // so that the tap loop reads LDS only; wider layers read them from global memory, one tap ahead
     constexpr bool WLDS = NB <= 3;
-    constexpr int NT = LT_ROWS * LT_PX * 4;  // 16-byte pieces of a slice
+    constexpr int ROWS = lds_tile_rows(NB);  // output rows of a tile
+    constexpr int TR = ROWS + 2;              // staged rows
+    constexpr int NT = TR * LT_PX * 4;        // 16-byte pieces of a slice
     constexpr int NI = (NT + 255) / 256;
     constexpr int NWT = 9 * NB * 64;         // 16-byte weight fragments of a slice
     constexpr int NW = WLDS ? (NWT + 255) / 256 : 0;
     // both arrays are padded to whole rounds of 256 pieces, so that every staging store is
-    // unconditional (a guarded store lets the compiler sink its load into the branch)
-    __shared__ __attribute__((aligned(16))) uint16_t tile[(NI * 256 / (LT_PX * 4) + 1) * LT_PX * LT_CP];
+    // unconditional (a guarded store lets the compiler sink its load into the branch). With the
+    // weights in LDS the tile is unpadded (LT_PX-pixel rows of 4 16-byte groups) and a pixel's
+    // groups are XOR-swizzled (slot 4P + (g ^ ((P >> 1) & 3)) for tile pixel P), so that the 8
+    // consecutive pixels of a fragment read phase cover all 32 banks; wider layers keep pixels
+    // padded to LT_CP channels.
+    __shared__ __attribute__((aligned(16))) uint16_t
+        tile[WLDS ? NI * 256 * 8 : (NI * 256 / (LT_PX * 4) + 1) * LT_PX * LT_CP];
     __shared__ uint4 s_w[WLDS ? NW * 256 : 1];
     const int CI = a.C1 + a.C2;
     const int nslice = (CI + 31) / 32;
     const int runs = (a.W + 63) / 64;
-    const int rblocks = (a.H + 3) / 4;
+    const int rblocks = (a.H + ROWS - 1) / ROWS;
     int blk = blockIdx.x;
     const int run = blk % runs;
     blk /= runs;
     const int rb = blk % rblocks;
     const int b = blk / rblocks;
-    const int x0 = run * 64, y0 = rb * 4;
+    const int x0 = run * 64, y0 = rb * ROWS;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: the store row
     const int lane = __lane_id();
     const int q = lane >> 4, lr = lane & 15;
-    f32x4 acc[4][NB];  // start from the bias (channels 16 j + 4q + e of the lane's pixel)
+    // acc[row][nb] (WLDS: the wavefront's 16 pixels of each row) or acc[pb][nb] (one row); they start
+    // from the bias (channels 16 nb + 4q + e of the lane's pixel)
+    constexpr int NA = WLDS ? ROWS : 4;
+    f32x4 acc[NA][NB];
     {
         f32x4 bias[NB];
         load_bias_t<NB>(a, q, bias);
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < NA; ++i)
 #pragma unroll
             for (int j = 0; j < NB; ++j) acc[i][j] = bias[j];
     }
@@ -454,9 +467,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         if constexpr (!WLDS) stage(sl);
 #pragma unroll
         for (int k = 0; k < NI; ++k) {
-            const int i = threadIdx.x + k * 256;  // pieces past NT land in the padding rows
-            const int g8 = i & 3, pxl = (i >> 2) % LT_PX, ry = (i >> 2) / LT_PX;
-            *reinterpret_cast<uint4*>(tile + (ry * LT_PX + pxl) * LT_CP + g8 * 8) = v[k];
+            const int i = threadIdx.x + k * 256;  // pieces past NT land in the padding
+            const int g8 = i & 3, P = i >> 2;      // P = tile row * LT_PX + pixel
+            if constexpr (WLDS)
+                reinterpret_cast<uint4*>(tile)[P * 4 + (g8 ^ ((P >> 1) & 3))] = v[k];
+            else
+                *reinterpret_cast<uint4*>(tile + P * LT_CP + g8 * 8) = v[k];
         }
 #pragma unroll
         for (int k = 0; k < NW; ++k) reinterpret_cast<u32x4*>(s_w)[threadIdx.x + k * 256] = wv[k];
@@ -464,10 +480,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         if constexpr (WLDS)
             if (sl + 1 < nslice) stage(sl + 1);
         if constexpr (WLDS) {
-            // wavefront w computes pixels x0 + 16w .. + 15 of all 4 rows (acc[row][nb]): the
+            // wavefront w computes pixels x0 + 16w .. + 15 of all ROWS rows (acc[row][nb]): the
             // fragment of one input row and tap column feeds every output row it reaches (up to 3),
-            // so a slice takes 18 activation and 9 NB weight fragment reads for 36 NB MFMAs (one row
-            // x 64 pixels per wavefront took 36 and 9 NB)
+            // so a 4-row slice takes 18 activation and 9 NB weight fragment reads for 36 NB MFMAs
+            // (one row x 64 pixels per wavefront took 36 and 9 NB), an 8-row one 30 and 9 NB for 72 NB
 #pragma unroll
             for (int dx = 0; dx < 3; ++dx) {
                 Frag8 cw[3][NB];
@@ -476,11 +492,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
                     for (int j = 0; j < NB; ++j) cw[dy][j].u = s_w[((dy * 3 + dx) * NB + j) * 64 + lane];
 #pragma unroll
-                for (int ry = 0; ry < LT_ROWS; ++ry) {
+                for (int ry = 0; ry < TR; ++ry) {
+                    const int P = ry * LT_PX + wave * 16 + lr + dx;
                     Frag8 fa;
-                    fa.u = *reinterpret_cast<const uint4*>(tile + (ry * LT_PX + wave * 16 + lr + dx) * LT_CP + q * 8);
+                    fa.u = reinterpret_cast<const uint4*>(tile)[P * 4 + (q ^ ((P >> 1) & 3))];
 #pragma unroll
-                    for (int orow = 0; orow < 4; ++orow) {
+                    for (int orow = 0; orow < ROWS; ++orow) {
                         const int dy = ry - orow;  // input row y0 - 1 + ry = output row y0 + orow + dy - 1
                         if (dy < 0 || dy > 2) continue;
 #pragma unroll
@@ -522,7 +539,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     }
     if constexpr (WLDS) {  // acc[row]: the wavefront's 16 pixels of row y0 + row
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
+        for (int r = 0; r < ROWS; ++r) {
             const int y = y0 + r;
             const f32x4(&ar)[1][NB] = *reinterpret_cast<const f32x4(*)[1][NB]>(&acc[r]);
             if (a.Yf)
@@ -552,7 +569,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 // channels, whose weights sit in LDS for the whole launch. (A multi-slice variant that loaded
 // each slice's weights into registers per item measured 2.4x slower than the LDS-staged kernel
 // on the 80-channel decoder layer: the weight latency it exposes per item is not hidden.)
-constexpr int DT_SLOTS = LT_ROWS * LT_PX * 4;    // 1584 16-byte slots of a tile slice
+constexpr int DT_ROWS = 6;                      // staged rows of a 4-row tile
+constexpr int DT_SLOTS = DT_ROWS * LT_PX * 4;    // 1584 16-byte slots of a tile slice
 constexpr int DT_INST = (DT_SLOTS + 255) / 256 * 4;  // 28 wave-wide loads: 7 per wave
 constexpr int DT_ALLOC = DT_INST * 64;           // slots allocated per buffer
 
@@ -609,7 +627,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd_dma(ConvArgs a) {
         const int P = S >> 2, g = (S & 3) ^ ((P >> 1) & 3);
         const int ry = P / LT_PX, pxl = P - ry * LT_PX;
         rel[k] = ((ry * a.W + pxl) * a.C1 + g * 8) * 2;
-        where[k] = (P < LT_ROWS * LT_PX && g * 8 < CI) ? (ry | (pxl << 8)) : -1;
+        where[k] = (P < DT_ROWS * LT_PX && g * 8 < CI) ? (ry | (pxl << 8)) : -1;
     }
     const __amdgpu_buffer_rsrc_t src =
         __builtin_amdgcn_make_buffer_rsrc((void*)a.X1, 0, a.B * a.H * a.W * a.C1 * 2, 0x00020000);
@@ -1237,7 +1255,8 @@ static void conv_fwd(hipStream_t s, const ConvArgs& a, bool k32, int shape) {
                            2147483648.0;
         const bool lds = small && (shape == 5 || shape == 6 || (shape == 0 && NBT <= 6));
         if (lds) {
-            const long blocks = (long)a.B * ((a.H + 3) / 4) * ((a.W + 63) / 64);
+            const int rows = lds_tile_rows(NBT);
+            const long blocks = (long)a.B * ((a.H + rows - 1) / rows) * ((a.W + 63) / 64);
             switch (NBT) {
             case 1: hipLaunchKernelGGL((k_conv_fwd_lds<1>), dim3(blocks), dim3(256), 0, s, a); break;
             case 2: hipLaunchKernelGGL((k_conv_fwd_lds<2>), dim3(blocks), dim3(256), 0, s, a); break;
