@@ -246,6 +246,65 @@ __device__ __forceinline__ int step_any(const TraceArgs& A, const RayPre& r, flo
     return 0;
 }
 
+// One node visit of either kind of ray, for wavefronts that hold both: a closest-hit ray (ao
+// false; t_best its current bound, as in step_closest) or an any-hit ray (ao true; t_best holds
+// its tmax and is never lowered, leaf `skip` is ignored, as in step_any). Both kinds run one
+// instruction stream, where a lane-divergent choice between the two steps ran both bodies with
+// half the lanes masked off. The arithmetic, the visit order and the tie rule are those of the
+// two steps, so the results are bit-identical. Returns 0 to continue, 1 when the ray is finished
+// (for any-hit: without a hit), 2 for an any-hit ray's hit.
+template <int S, bool COUNT, bool PAIRS = true>
+__device__ __forceinline__ int step_mixed(const TraceArgs& A, const RayPre& r, float tmin, float tmax, float& t_best,
+                                          int& best, int32_t& node, LaneStack<S>& st, Count& cnt, int skip, bool ao) {
+    const bool pair_visit = PAIRS && node < 0;
+    const Visit v = fetch<COUNT, PAIRS>(A, node, cnt);
+    float tn0 = slab(r, v.a.x, v.a.y, v.a.z, v.a.w, v.b.x, v.b.y, tmin, t_best);
+    float tn1 = slab(r, v.b.z, v.b.w, v.c.x, v.c.y, v.c.z, v.c.w, tmin, t_best);
+    const int32_t ch0 = v.d.x, ch1 = v.d.y;
+    const bool leaf0 = ch0 < 0 && !(PAIRS && (v.d.z & 1)), leaf1 = ch1 < 0 && !(PAIRS && (v.d.z & 2));
+    const bool lf0 = tn0 != INFINITY && leaf0 && ~ch0 != skip, lf1 = tn1 != INFINITY && leaf1 && ~ch1 != skip;
+    float4 sp0 = make_float4(0.f, 0.f, 0.f, 0.f), sp1 = sp0;
+    if (lf0) sp0 = A.leaf_sph[~ch0];
+    if (lf1) sp1 = A.leaf_sph[~ch1];
+    if (COUNT && !pair_visit) cnt.leaf_fetches += (lf0 ? 1 : 0) + (lf1 ? 1 : 0);
+    bool hit = false;
+    auto test = [&](int leaf, float4 sp) {
+        const float t = sphere_t(r.o, r.d, sp, tmin, tmax);
+        if (COUNT) cnt.tests++;
+        if (t >= 0.0f) {
+            if (ao) {
+                hit = true;
+            } else if (t < t_best) {
+                t_best = t;
+                best = leaf;
+            } else if (t == t_best && best >= 0 && A.leaf_id[leaf] < A.leaf_id[best]) {
+                best = leaf;
+            }
+        }
+    };
+    if (lf0) test(~ch0, sp0);
+    if (hit) return 2;
+    if (lf1) test(~ch1, sp1);
+    if (hit) return 2;
+    // leaf children are never descended (a skipped leaf neither)
+    if (leaf0) tn0 = INFINITY;
+    if (leaf1) tn1 = INFINITY;
+    const bool h0 = tn0 <= t_best, h1 = tn1 <= t_best;
+    if (h0 && h1) {
+        const bool first0 = tn0 <= tn1;
+        st.push(first0 ? ch1 : ch0);
+        node = first0 ? ch0 : ch1;
+    } else if (h0) {
+        node = ch0;
+    } else if (h1) {
+        node = ch1;
+    } else {
+        if (st.sp == 0) return 1;
+        node = st.pop();
+    }
+    return 0;
+}
+
 // Closest hit: returns the leaf index of the nearest reported intersection (or -1) and its t.
 template <int S, bool COUNT, bool PAIRS = true>
 __device__ int trace_closest(const TraceArgs& A, const RayPre& r, float tmin, float tmax, float& t_best,
@@ -925,13 +984,16 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(7, 8))) void
         if (!live) break;
         // 3. traversal: one node visit per active lane per iteration
         while (act) {
-            if (state == L_PRIM) {
-                if (step_closest<S, COUNT>(A, r, tmin, tmax, t_best, best, node, st, cnt)) state = L_PRIM_DONE;
-            } else if (state == L_AO) {
-                int res = step_any<S, COUNT>(A, r, tmin, tmax, skip, node, st, cnt);
+            if (state == L_PRIM || state == L_AO) {  // one step for both kinds (primary and AO lanes mix here)
+                const bool ao = state == L_AO;
+                const int res = step_mixed<S, COUNT>(A, r, tmin, tmax, t_best, best, node, st, cnt, ao ? skip : -1, ao);
                 if (res) {
-                    state = L_AO_DONE;
-                    best = res;  // 2 = occluded
+                    if (ao) {
+                        state = L_AO_DONE;
+                        best = res;  // 2 = occluded
+                    } else {
+                        state = L_PRIM_DONE;
+                    }
                 }
             }
             act = __ballot(state == L_PRIM || state == L_AO);
@@ -971,6 +1033,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(7, 8))) void
                     r = prep(Pp, ad, pad);
                     tmin = 0.0f;
                     tmax = f.ao_distance;
+                    t_best = tmax;  // an any-hit ray's bound in step_mixed
                     skip = best;
                     node = 0;
                     st.sp = 0;
