@@ -895,11 +895,12 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(7, 8))) void
 // random streams are those of k_trace, so the results are bit-identical.
 enum : int { L_IDLE = 0, L_NEED = 1, L_PRIM = 2, L_AO = 3, L_PRIM_DONE = 4, L_AO_DONE = 5, L_OFF = 6 };
 
-// Held to 72 VGPRs (7 wavefronts per SIMD instead of 4, a few registers spilled): the 1-spp
-// frame's trace is bound by the latency of its dependent node fetches, and the extra wavefronts
-// took it from 1.13 to 1.04 ms (5 wavefronts: 1.07, 6: 1.04, 8: 1.10).
+// Held to 5 wavefronts per SIMD (at most 102 VGPRs; it takes 95 and spills nothing). With one step
+// for both kinds of ray the kernel is shorter, and the 1-spp frame's trace went from 1.02 ms at 7
+// wavefronts (72 VGPRs and 96 bytes of spills per lane) to 0.92 ms (6 wavefronts, 60 bytes of
+// spills: 0.91 ms). Before that step, 7 wavefronts had been the fastest (5: 1.07, 7: 1.04 ms).
 template <int S, int MODE, bool COUNT>
-__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_trace_refill(TraceArgs A,
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_trace_refill(TraceArgs A,
                                                                                                 uint32_t refill) {
     __shared__ int32_t lds_stack[S * TB];
     const uint32_t gtid = blockIdx.x * TB + threadIdx.x;
