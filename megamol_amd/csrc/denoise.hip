@@ -108,7 +108,7 @@ __device__ __forceinline__ uint32_t relu_bf16x2(uint32_t v) {
 }
 template <int PB, int NB, bool F32, bool RELU>
 __device__ __forceinline__ void store_row_t(const ConvArgs& a, const f32x4 (&acc)[PB][NB], int b, int y, bool row_ok,
-                                            int x0, int q, int lr) {
+                                            int x0, int q, int lr, int c0) {
     const int yc = F32 ? (a.YfC ? a.YfC : a.CO) : a.CO;
     const size_t rowo = ((size_t)b * a.H + (row_ok ? y : 0)) * a.W;
     const uint64_t ba = F32 ? (uint64_t)(a.Yf + rowo * yc) : (uint64_t)(a.Y + rowo * a.CO);
@@ -125,7 +125,7 @@ __device__ __forceinline__ void store_row_t(const ConvArgs& a, const f32x4 (&acc
         const int px = x < a.W ? x * yc * S : kOut;
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
-            const int co = j * 16 + q * 4;
+            const int co = c0 + j * 16 + q * 4;
             const int off = co < yc ? px + co * S : kOut;
             const f32x4 v = acc[pb][j];
             if constexpr (F32) {
@@ -148,18 +148,18 @@ __device__ __forceinline__ void store_row_t(const ConvArgs& a, const f32x4 (&acc
 }
 template <int PB, int NB, bool F32>
 __device__ __forceinline__ void store_row_t(const ConvArgs& a, const f32x4 (&acc)[PB][NB], int b, int y, bool row_ok,
-                                            int x0, int q, int lr) {
+                                            int x0, int q, int lr, int c0 = 0) {
     if (a.relu)
-        store_row_t<PB, NB, F32, true>(a, acc, b, y, row_ok, x0, q, lr);
+        store_row_t<PB, NB, F32, true>(a, acc, b, y, row_ok, x0, q, lr, c0);
     else
-        store_row_t<PB, NB, F32, false>(a, acc, b, y, row_ok, x0, q, lr);
+        store_row_t<PB, NB, F32, false>(a, acc, b, y, row_ok, x0, q, lr, c0);
 }
 
-// bias of the channels 16 nb + 4q + e a lane holds in the layout above
+// bias of the channels c0 + 16 nb + 4q + e a lane holds in the layout above
 template <int NB>
-__device__ __forceinline__ void load_bias_t(const ConvArgs& a, int q, f32x4 (&bias)[NB]) {
+__device__ __forceinline__ void load_bias_t(const ConvArgs& a, int q, f32x4 (&bias)[NB], int c0 = 0) {
 #pragma unroll
-    for (int j = 0; j < NB; ++j) bias[j] = *reinterpret_cast<const f32x4*>(a.bias + j * 16 + q * 4);
+    for (int j = 0; j < NB; ++j) bias[j] = *reinterpret_cast<const f32x4*>(a.bias + c0 + j * 16 + q * 4);
 }
 
 union Frag {
@@ -379,6 +379,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const int rb = blk % rblocks;
     const int b = blk / rblocks;
     const int x0 = run * 64, y0 = rb * ROWS;
+    // output channels: group blockIdx.y of NB 16-channel blocks out of NBT (a layer wider than the
+    // kernel's NB runs as several groups, each with its slice of the weights in LDS)
+    const int NBT = a.CO / 16;
+    const int c0 = (int)blockIdx.y * NB * 16;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: the store row
     const int lane = __lane_id();
     const int q = lane >> 4, lr = lane & 15;
@@ -388,7 +392,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     f32x4 acc[NA][NB];
     {
         f32x4 bias[NB];
-        load_bias_t<NB>(a, q, bias);
+        load_bias_t<NB>(a, q, bias, c0);
 #pragma unroll
         for (int i = 0; i < NA; ++i)
 #pragma unroll
@@ -455,7 +459,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             const int i = threadIdx.x + k * 256;
             const int ii = i < NWT ? i : 0;
             const int t = ii / (NB * 64), r = ii - t * (NB * 64);  // r = nb * 64 + lane
-            wv[k] = w4[((size_t)t * nslice + sl) * (NB * 64) + r];
+            wv[k] = w4[((size_t)t * nslice + sl) * (NBT * 64) + (int)blockIdx.y * NB * 64 + r];
         }
     };
     // with the weights in LDS the next slice's loads are issued right after a slice is stored and
@@ -512,7 +516,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             // registers would cost these kernels half their occupancy)
             auto load_w = [&](int t, Frag8 (&fb)[NB]) {
 #pragma unroll
-                for (int j = 0; j < NB; ++j) fb[j].u = wl4[(((size_t)t * nslice + sl) * NB + j) * 64 + lane];
+                for (int j = 0; j < NB; ++j)
+                    fb[j].u = wl4[(((size_t)t * nslice + sl) * NBT + (int)blockIdx.y * NB + j) * 64 + lane];
             };
             Frag8 fw[NB];
             load_w(0, fw);
@@ -543,16 +548,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             const int y = y0 + r;
             const f32x4(&ar)[1][NB] = *reinterpret_cast<const f32x4(*)[1][NB]>(&acc[r]);
             if (a.Yf)
-                store_row_t<1, NB, true>(a, ar, b, y, y < a.H, x0 + 16 * wave, q, lr);
+                store_row_t<1, NB, true>(a, ar, b, y, y < a.H, x0 + 16 * wave, q, lr, c0);
             else
-                store_row_t<1, NB, false>(a, ar, b, y, y < a.H, x0 + 16 * wave, q, lr);
+                store_row_t<1, NB, false>(a, ar, b, y, y < a.H, x0 + 16 * wave, q, lr, c0);
         }
     } else {  // acc[pb]: pixels x0 + 16 pb .. of row y0 + wave
         const int y = y0 + wave;
         if (a.Yf)
-            store_row_t<4, NB, true>(a, acc, b, y, y < a.H, x0, q, lr);
+            store_row_t<4, NB, true>(a, acc, b, y, y < a.H, x0, q, lr, c0);
         else
-            store_row_t<4, NB, false>(a, acc, b, y, y < a.H, x0, q, lr);
+            store_row_t<4, NB, false>(a, acc, b, y, y < a.H, x0, q, lr, c0);
     }
 }
 
@@ -1249,21 +1254,33 @@ static void conv_fwd(hipStream_t s, const ConvArgs& a, bool k32, int shape) {
             return;
         }
         // the LDS-staged kernel for every layer (it beats the per-wavefront kernel at all four
-        // resolutions); shape 5 forces it, other nonzero shapes force the per-wavefront kernel
-        // (its buffer loads and stores take 31-bit offsets: tensors under 2 GB)
+        // resolutions); shapes 5 and 7 force it (7 without output-channel groups), other nonzero
+        // shapes but 6 force the per-wavefront kernel (its buffer loads and stores take 31-bit
+        // offsets: tensors under 2 GB)
         const bool small = (double)a.B * a.H * a.W * std::max(std::max(a.C1, a.CO * (a.Yf ? 2 : 1)), 1) * 2 <
                            2147483648.0;
-        const bool lds = small && (shape == 5 || shape == 6 || (shape == 0 && NBT <= 6));
+        const bool lds = small && (shape == 5 || shape == 6 || shape == 7 || (shape == 0 && NBT <= 6));
         if (lds) {
-            const int rows = lds_tile_rows(NBT);
+            // 96-channel layers run as two groups of 48 output channels (grid.y), each with its slice
+            // of the weights in LDS: the eighth-resolution level then fills the device and no tap
+            // waits for weights from global memory (21 -> 13 us a layer). 64-channel layers measured
+            // slower as two groups of 32 (27 -> 31 us: the same number of workgroups, each staging
+            // the whole input). Shape 7 keeps every layer whole (weights streamed per tap).
+            int nb = NBT, groups = 1;
+            if (shape != 7 && NBT == 6) {
+                nb = 3;
+                groups = 2;
+            }
+            const int rows = lds_tile_rows(nb);
             const long blocks = (long)a.B * ((a.H + rows - 1) / rows) * ((a.W + 63) / 64);
-            switch (NBT) {
-            case 1: hipLaunchKernelGGL((k_conv_fwd_lds<1>), dim3(blocks), dim3(256), 0, s, a); break;
-            case 2: hipLaunchKernelGGL((k_conv_fwd_lds<2>), dim3(blocks), dim3(256), 0, s, a); break;
-            case 3: hipLaunchKernelGGL((k_conv_fwd_lds<3>), dim3(blocks), dim3(256), 0, s, a); break;
-            case 4: hipLaunchKernelGGL((k_conv_fwd_lds<4>), dim3(blocks), dim3(256), 0, s, a); break;
-            case 5: hipLaunchKernelGGL((k_conv_fwd_lds<5>), dim3(blocks), dim3(256), 0, s, a); break;
-            default: hipLaunchKernelGGL((k_conv_fwd_lds<6>), dim3(blocks), dim3(256), 0, s, a); break;
+            const dim3 grid((unsigned)blocks, (unsigned)groups);
+            switch (nb) {
+            case 1: hipLaunchKernelGGL((k_conv_fwd_lds<1>), grid, dim3(256), 0, s, a); break;
+            case 2: hipLaunchKernelGGL((k_conv_fwd_lds<2>), grid, dim3(256), 0, s, a); break;
+            case 3: hipLaunchKernelGGL((k_conv_fwd_lds<3>), grid, dim3(256), 0, s, a); break;
+            case 4: hipLaunchKernelGGL((k_conv_fwd_lds<4>), grid, dim3(256), 0, s, a); break;
+            case 5: hipLaunchKernelGGL((k_conv_fwd_lds<5>), grid, dim3(256), 0, s, a); break;
+            default: hipLaunchKernelGGL((k_conv_fwd_lds<6>), grid, dim3(256), 0, s, a); break;
             }
             return;
         }

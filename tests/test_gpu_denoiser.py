@@ -31,13 +31,14 @@ def dnz(gpu_renderer):
     return d
 
 
-@pytest.mark.parametrize("k32", [0, 1, 14, 24, 34, 2, 4, 5, 6])
+@pytest.mark.parametrize("k32", [0, 1, 14, 24, 34, 2, 4, 5, 6, 7])
 @pytest.mark.parametrize("shape", [(1, 16, 16), (2, 24, 40), (1, 37, 70)])
 def test_forward_matches_reference(dnz, shape, k32):
     """k32: 0 = 16x16x16 kernels, 1 = 16x16x32 with the automatic launch shape, 5 = the LDS-staged
-    kernel, 6 = the LDS-DMA double-buffered kernel for single-slice layers (the LDS-staged one for
-    the others), other values force the 16x16x32 launch shape (channel blocks * 10 + pixel blocks
-    per wavefront)."""
+    kernel (96-channel layers as two groups of 48 output channels), 7 = the LDS-staged
+    kernel without channel groups, 6 = the LDS-DMA double-buffered kernel for single-slice layers
+    (the LDS-staged one for the others), other values force the 16x16x32 launch shape (channel
+    blocks * 10 + pixel blocks per wavefront)."""
     case(f"denoiser forward {shape} k32={k32}")
     dnz.r.set_option("dn_k32", 1 if k32 else 0)
     dnz.r.set_option("dn_conv_shape", k32 if k32 > 1 else 0)
