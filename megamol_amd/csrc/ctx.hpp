@@ -152,7 +152,7 @@ struct mmhrt_ctx_s {
     int64_t opt_build_grid = 0;     // test hook: cap on the persistent chunk workgroups (0 = as many as fit)
     int64_t opt_leaf_pairs = 1;     // nodes over two leaves are not stored (their parents reference the pair)
     int64_t opt_build_tail = 512;   // straddler worklist size from which one workgroup finishes the tree
-    int64_t opt_build_wave = 64;    // chunk pass: rounds of at most this many (<= 64) merges run on one wavefront (0: none)
+    int64_t opt_build_wave = 64;    // chunk pass: from this many merges on, lanes climb instead of rounds (0: never)
     int64_t opt_sort_xcd = 1;       // radix scatter: each XCD takes one contiguous range of tiles (0: round robin)
     int64_t opt_build_span = 256;   // straddlers: one workgroup per span of this many chunks, then the tail (0: rounds)
     int64_t opt_fail_alloc = 0;     // test hook: the k-th device allocation from now fails with E_OOM (0: none)

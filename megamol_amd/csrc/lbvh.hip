@@ -927,7 +927,6 @@ __global__ __launch_bounds__(BC) __attribute__((amdgpu_waves_per_eu(8, 8))) void
     __shared__ uint32_t s_cnt[BC];      // arrivals: + 1 for side 0, + 0x10000 for side 1
     __shared__ uint16_t s_lst[2][BC];   // splits completed in the previous / current round
     __shared__ uint32_t s_nl[3];        // list sizes: read in round r, appended in r + 1, cleared in r + 2
-    __shared__ uint32_t s_nw[2];        // list sizes of the rounds wavefront 0 runs alone
     __shared__ int32_t s_pend[2];       // boundary splits delivered to in this chunk (left, right)
     __shared__ uint32_t s_list[256];
     __shared__ uint32_t s_n, s_base;
@@ -992,13 +991,16 @@ __global__ __launch_bounds__(BC) __attribute__((amdgpu_waves_per_eu(8, 8))) void
         if ((tid & 63) == 0) s_nb[tid >> 6] = sp_cur;
         __syncthreads();
         // deliver node X = [l, r] (box b, height h, dl = delta(l - 1), dr = delta(r); its
-        // children's boxes f and references kd when internal) to its parent
+        // children's boxes f and references kd when internal) to its parent. When this completes an
+        // in-chunk split, the split goes to lst_next, or with lst_next null (a climbing lane) its
+        // slot is returned for this lane to merge next; otherwise -1.
         auto deliver = [&](int32_t l, int32_t r, int dl, int dr, const float b[6], int h, bool internal,
-                           const float f[12], int2 kd, int pf, int32_t leaf, uint32_t* nl_next, uint16_t* lst_next) {
+                           const float f[12], int2 kd, int pf, int32_t leaf, uint32_t* nl_next,
+                           uint16_t* lst_next, bool xw = false) -> int {
             if (l == 0 && r == n - 1) {  // the root
                 store_node(nodes, f, kd, pf, h);
                 info->height = (uint32_t)h;
-                return;
+                return -1;
             }
             if (internal && h == pair_h) {  // a leaf pair: delivered as its first leaf (height 1), not stored
                 internal = false;
@@ -1022,7 +1024,7 @@ __global__ __launch_bounds__(BC) __attribute__((amdgpu_waves_per_eu(8, 8))) void
             if (sp < cs || sp + 1 >= ce) {  // a boundary split: sp = cs - 1 or ce - 1
                 meet_put(meet + sp, meet_far + sp, side, b, ref, h, far, left ? dl : dr);
                 s_pend[sp < cs ? 0 : 1] = sp;
-                return;
+                return -1;
             }
             const int slot = sp - cs;
 #pragma unroll
@@ -1031,12 +1033,27 @@ __global__ __launch_bounds__(BC) __attribute__((amdgpu_waves_per_eu(8, 8))) void
             s_far[slot][side] = (uint16_t)(far - cs);
             // the parent's own dl (left) or dr (right) beside the height
             s_hd[slot][side] = (uint16_t)((uint32_t)h | ((uint32_t)(uint8_t)(int8_t)(left ? dl : dr) << 8));
-            // the merge happens in the next round, after a barrier: a relaxed count suffices. Each
-            // side adds its own unit, so the count also tells which side a lone arrival came from.
-            const uint32_t old = __hip_atomic_fetch_add(&s_cnt[slot], side ? 0x10000u : 1u, __ATOMIC_RELAXED,
-                                                        __HIP_MEMORY_SCOPE_WORKGROUP);
+            // a round's merge happens in the next round, after a barrier, and a climbing lane reads
+            // the record right after the count. Within one wavefront LDS executes its accesses in
+            // issue order: a relaxed count suffices, with compiler fences that keep the record's
+            // writes before it and the climb's reads after it. Lanes of several wavefronts climbing
+            // together (xw) take an acquire-release count. Each side adds its own unit, so the count
+            // also tells which side a lone arrival came from.
+            uint32_t old;
+            if (xw) {
+                old = __hip_atomic_fetch_add(&s_cnt[slot], side ? 0x10000u : 1u, __ATOMIC_ACQ_REL,
+                                             __HIP_MEMORY_SCOPE_WORKGROUP);
+            } else {
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                old = __hip_atomic_fetch_add(&s_cnt[slot], side ? 0x10000u : 1u, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_WORKGROUP);
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            }
+            if (old == 0u) return -1;
+            if (lst_next == nullptr) return slot;
             // (a wavefront-aggregated append, one counter update per wavefront, measured 2% slower)
-            if (old != 0u) lst_next[atomicAdd(nl_next, 1u)] = (uint16_t)slot;
+            lst_next[atomicAdd(nl_next, 1u)] = (uint16_t)slot;
+            return -1;
         };
         auto gref = [&](uint16_t rr) -> int32_t {
             return (rr & 0x8000u) ? ~(cs + (int32_t)(rr & 0x7fffu)) : cs + (int32_t)rr;
@@ -1079,7 +1096,7 @@ __global__ __launch_bounds__(BC) __attribute__((amdgpu_waves_per_eu(8, 8))) void
             }
         }
         // merge the two children parked at a completed in-chunk split and deliver the node
-        auto merge = [&](int slot, uint32_t* nl_next, uint16_t* lst_next) {
+        auto merge = [&](int slot, uint32_t* nl_next, uint16_t* lst_next, bool xw = false) -> int {
             float f[12];
 #pragma unroll
             for (int a = 0; a < 12; ++a) f[a] = s_box[slot][a];
@@ -1095,7 +1112,7 @@ __global__ __launch_bounds__(BC) __attribute__((amdgpu_waves_per_eu(8, 8))) void
                 b[a] = fminf(f[a], f[6 + a]);
                 b[3 + a] = fmaxf(f[3 + a], f[9 + a]);
             }
-            deliver(l, r, dl, dr, b, h, true, f, kd, pair_flags(kd, h0, h1), 0, nl_next, lst_next);
+            return deliver(l, r, dl, dr, b, h, true, f, kd, pair_flags(kd, h0, h1), 0, nl_next, lst_next, xw);
         };
         __syncthreads();
         for (int rd = 0;; ++rd) {
@@ -1103,22 +1120,16 @@ __global__ __launch_bounds__(BC) __attribute__((amdgpu_waves_per_eu(8, 8))) void
             const uint32_t count = s_nl[c0];
             if (count == 0) break;
             if (count <= (uint32_t)wave_items) {
-                // the last levels on wavefront 0 alone: one wavefront's LDS accesses complete in issue
-                // order, so its rounds need no workgroup barrier, only a fence that keeps the compiler
-                // from moving LDS accesses across a round. Its rounds count in s_nw, never in s_nl:
-                // the other wavefronts may still be reading s_nl[c0] for this round's decision, and a
-                // wavefront that read it after a clear would leave the loop early, skip the barrier
-                // below and run the arrival counts while wavefront 0 still merges.
-                if (tid < 64) {
-                    uint32_t cw = count;
-                    for (int r2 = rd; cw != 0; ++r2) {
-                        uint32_t* nc = &s_nw[r2 & 1];  // read (into cw) at the end of round r2 - 1 at the latest
-                        if (tid == 0) *nc = 0;
-                        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                        if ((uint32_t)tid < cw) merge(s_lst[r2 & 1][tid], nc, s_lst[(r2 + 1) & 1]);
-                        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                        cw = *nc;
-                    }
+                // the last levels without rounds: each lane of the first waves takes one of the last
+                // splits and climbs while its delivery completes the parent (within one wavefront LDS
+                // executes accesses in issue order, so the other half of a parent, written before its
+                // count by a lane of the same wavefront, is there when the count says so; lanes of
+                // several wavefronts use acquire-release counts). No list, counter or fence per
+                // level, and no workgroup barrier until the chunk's top.
+                const bool xw = count > 64u;
+                if ((uint32_t)tid < ((count + 63u) & ~63u)) {
+                    int slot = (uint32_t)tid < count ? (int)s_lst[rd & 1][tid] : -1;
+                    while (slot >= 0) slot = merge(slot, nullptr, nullptr, xw);
                 }
                 __syncthreads();
                 break;
@@ -1551,7 +1562,7 @@ void launch_chunk_ls(mmhrt_ctx_s* c, const K* keys, const uint32_t* ids, uint32_
     hipLaunchKernelGGL((k_bu_chunk_ls<K, BC>), dim3(chunk_grid<K, BC>(c, n)), dim3(BC), 0, c->stream, keys, ids,
                        cur_spheres(c), c->leaf_sph.p, (int64_t)n, c->nodes.p,
                        c->meet.p, c->meet_far.p, c->flags.p, list, c->box_cnt.p, c->info.p,
-                       (int)std::max<int64_t>(0, std::min<int64_t>(c->opt_build_wave, 64)));
+                       (int)std::max<int64_t>(0, std::min<int64_t>(c->opt_build_wave, BC)));
 }
 
 // Hierarchy and boxes over the sorted keys, gathering the leaf spheres on the way: ids[k] is

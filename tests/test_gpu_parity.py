@@ -66,14 +66,18 @@ def leaf_pair_form(ochild):
     (1024, 1, 16, 0, 1, 64, 512, 256, 8), (512, 1, 8, 5, 1, 64, 512, 256, 8), (1024, 1, 32, 0, 0, 0, 512, 256, 8),
     (256, 1, 16, 0, 0, 32, 1 << 30, 0, 8), (1024, 1, 32, 0, 0, 64, 4096, 0, 8), (1024, 1, 32, 0, 0, 64, 512, 0, 8),
     (256, 1, 16, 0, 0, 64, 512, 1, 8), (1024, 0, 8, 2, 0, 64, 512, 16, 8), (1024, 1, 16, 0, 0, 64, 512, 256, 0),
-    (512, 1, 32, 0, 0, 64, 512, 256, 0), (256, 1, 8, 0, 0, 64, 512, 256, 0)])
+    (512, 1, 32, 0, 0, 64, 512, 256, 0), (256, 1, 8, 0, 0, 64, 512, 256, 0),
+    (1024, 1, 16, 0, 0, 1024, 512, 256, 8), (512, 1, 16, 3, 0, 256, 512, 256, 8),
+    (1024, 1, 32, 0, 0, 128, 512, 0, 8)])
 def test_lbvh_equals_oracle_lbvh(gpu_renderer, chunk, ls, ri, grid, os1, wave, tail, span, dig):
     """Both chunk passes (level-synchronous rounds, or one climbing thread per leaf) at every
     chunk size, over radix tiles of 8, 16 or 32 keys per thread, with either radix pass
     (histogram + scans + scatter, or single-pass with look-back: os1), give the oracle's tree
     node for node (sorted keys and leaf order included; leaf pairs in their stored form). grid > 0
     caps the persistent chunk workgroups, so that each one runs many chunks through its load
-    pipeline; wave is the round size up to which one wavefront runs the rounds alone (0: never);
+    pipeline; wave is the list size from which the remaining merges climb without rounds, one lane
+    per split (one wavefront up to 64, several wavefronts with acquire-release counts above; 0:
+    rounds to the top);
     tail is the straddler list size from which one workgroup finishes the tree (1 << 30: it takes
     every straddler from the chunk pass on) when span is 0 (kernel rounds); span > 0 merges the
     straddlers with one workgroup per span of that many chunks before the tail. dig is the radix
