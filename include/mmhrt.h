@@ -331,8 +331,9 @@ MMHRT_API int32_t mmhrt_denoiser_train_step(mmhrt_ctx ctx, const uint16_t* feats
  * pair; 0: every node is stored), "build_tail" (straddler worklist size from which one
  * workgroup finishes the tree, default 512; used when "build_span" is 0), "build_span" (straddlers
  * merged by one workgroup per span of this many chunks before the tail, default 256; 0: kernel
- * rounds), "build_wave" (chunk-pass rounds of at most this
- * many merges, up to 64, run on one wavefront without workgroup barriers, default 64; 0: never),
+ * rounds), "build_wave" (chunk pass: once a round would merge at most this many splits, the
+ * rest of the chunk climbs without rounds, one lane per split; default 64, one wavefront; up to
+ * the chunk size; 0: rounds to the top),
  * "accel" (structure the next build makes: 0 LBVH, the default; 1 treelet P-k-d trees, about one
  * byte per sphere beyond the spheres instead of up to 64), "treelet" (spheres per treelet of that
  * structure: 256, 512 or 1024, default 1024), "batch_launch" (mmhrt_render_batch: 1, the
