@@ -1017,10 +1017,12 @@ __global__ void k_frame_input(const float4* __restrict__ col, const __half* __re
                                (uint16_t)q.z, (uint16_t)(q.z >> 16), (uint16_t)q.w, (uint16_t)(q.w >> 16)};
         const uint16_t c0 = __half_as_ushort(__float2half(c.x)), c1 = __half_as_ushort(__float2half(c.y)),
                        c2 = __half_as_ushort(__float2half(c.z));
-        uint2* d = reinterpret_cast<uint2*>(raw + p * 12);  // raw: colour 3, aux 8, 0
-        d[0] = make_uint2((uint32_t)c0 | ((uint32_t)c1 << 16), (uint32_t)c2 | ((uint32_t)h[0] << 16));
-        d[1] = make_uint2((uint32_t)h[1] | ((uint32_t)h[2] << 16), (uint32_t)h[3] | ((uint32_t)h[4] << 16));
-        d[2] = make_uint2((uint32_t)h[5] | ((uint32_t)h[6] << 16), (uint32_t)h[7]);
+        if (raw) {  // raw: colour 3, aux 8, 0 (a rendered frame's compose reads the aux buffer instead)
+            uint2* d = reinterpret_cast<uint2*>(raw + p * 12);
+            d[0] = make_uint2((uint32_t)c0 | ((uint32_t)c1 << 16), (uint32_t)c2 | ((uint32_t)h[0] << 16));
+            d[1] = make_uint2((uint32_t)h[1] | ((uint32_t)h[2] << 16), (uint32_t)h[3] | ((uint32_t)h[4] << 16));
+            d[2] = make_uint2((uint32_t)h[5] | ((uint32_t)h[6] << 16), (uint32_t)h[7]);
+        }
         auto hf = [](uint16_t v) { return __half2float(__ushort_as_half(v)); };
         const float t = hf(h[6]);
         const bool hit = t > 0.0f;
@@ -1044,8 +1046,9 @@ __global__ void k_frame_input(const float4* __restrict__ col, const __half* __re
 // Network output (fp32 [B,Hp,Wp,16]: v in channel 0, colour residual in 1-3) -> colour
 // albedo * v + residual on hit pixels, background elsewhere. With a target, also the squared
 // error and d(loss)/d(out).
-__global__ __launch_bounds__(256) void k_compose(const float* __restrict__ out, const uint16_t* __restrict__ raw, int B,
-                                                 int H, int W, int Hp, int Wp, float4 bg,
+__global__ __launch_bounds__(256) void k_compose(const float* __restrict__ out, const uint16_t* __restrict__ feat,
+                                                 int fstride, int fa, int ft, int B, int H, int W, int Hp, int Wp,
+                                                 float4 bg,
                                                  const float* __restrict__ target, float inv_count,
                                                  float* __restrict__ color, uint16_t* __restrict__ dout,
                                                  float* __restrict__ loss, float4* __restrict__ fb_col,
@@ -1061,10 +1064,10 @@ __global__ __launch_bounds__(256) void k_compose(const float* __restrict__ out, 
         float g0 = 0.0f, gr[3] = {0.0f, 0.0f, 0.0f};
         if (x < W && y < H) {
             size_t q = ((size_t)b * H + y) * W + x;
-            const uint16_t* s = raw + q * 12;
-            float t = __half2float(__ushort_as_half(s[9]));
-            float al[3] = {__half2float(__ushort_as_half(s[3])), __half2float(__ushort_as_half(s[4])),
-                           __half2float(__ushort_as_half(s[5]))};
+            const uint16_t* s = feat + q * fstride;
+            float t = __half2float(__ushort_as_half(s[ft]));
+            float al[3] = {__half2float(__ushort_as_half(s[fa])), __half2float(__ushort_as_half(s[fa + 1])),
+                           __half2float(__ushort_as_half(s[fa + 2]))};
             const float4 o4 = reinterpret_cast<const float4*>(out)[i];  // visibility, colour residual
             const float v = o4.x, res[3] = {o4.y, o4.z, o4.w};
             float c[3];
@@ -1356,12 +1359,15 @@ static void conv_bwd_weight(hipStream_t s, const ConvWgtArgs& a, int cu) {
 }
 
 
-void launch_compose(hipStream_t s, const float* out, const uint16_t* raw, int B, int H, int W, int Hp, int Wp,
-                    float4 bg, const float* target, float inv_count, float* color, uint16_t* dout, float* loss,
-                    float4* fb_col, uint32_t* fb_rgba8) {
+// feat: the per-pixel features, either the 12-half raw records (colour 3, albedo 3, normal 3, t,
+// coverage, 0: stride 12, albedo at 3, t at 9) or a rendered frame's 8-half aux buffer (albedo 3,
+// normal 3, t, coverage: stride 8, albedo at 0, t at 6)
+void launch_compose(hipStream_t s, const float* out, const uint16_t* feat, int fstride, int fa, int ft, int B, int H,
+                    int W, int Hp, int Wp, float4 bg, const float* target, float inv_count, float* color,
+                    uint16_t* dout, float* loss, float4* fb_col, uint32_t* fb_rgba8) {
     long n = (long)B * Hp * Wp;
-    hipLaunchKernelGGL(k_compose, dim3(nblk(n)), dim3(256), 0, s, out, raw, B, H, W, Hp, Wp, bg, target, inv_count,
-                       color, dout, loss, fb_col, fb_rgba8);
+    hipLaunchKernelGGL(k_compose, dim3(nblk(n)), dim3(256), 0, s, out, feat, fstride, fa, ft, B, H, W, Hp, Wp, bg,
+                       target, inv_count, color, dout, loss, fb_col, fb_rgba8);
 }
 void zero(hipStream_t s, float* p, long n) {
     hipLaunchKernelGGL(k_zero_f32, dim3((unsigned)std::min<long>(nblk(n), 4096)), dim3(256), 0, s, p, n);
@@ -1579,7 +1585,8 @@ inline int ceil8(int v) { return (v + 7) & ~7; }
 // fb_col / fb_rgba8: compose straight into a rendered frame's framebuffer (B = 1) instead of the
 // colour buffer
 int dn_run(mmhrt_ctx c, int B, int H, int W, float depth_scale, const float bg[4], const float* target, float* loss,
-           bool train, float4* fb_col = nullptr, uint32_t* fb_rgba8 = nullptr, bool packed = false) {
+           bool train, float4* fb_col = nullptr, uint32_t* fb_rgba8 = nullptr, bool packed = false,
+           const __half* aux8 = nullptr) {
     Denoiser& d = *c->dn;
     int Hp = ceil8(H), Wp = ceil8(W);
     if (int e = d.ensure_acts(c, B, Hp, Wp, train)) return e;
@@ -1596,9 +1603,11 @@ int dn_run(mmhrt_ctx c, int B, int H, int W, float depth_scale, const float bg[4
     d.forward(s, B, Hp, Wp);
     float4 bgc = make_float4(bg ? bg[0] : 0.f, bg ? bg[1] : 0.f, bg ? bg[2] : 0.f, bg ? bg[3] : 1.f);
     if (loss) mmhrt::launch_fill_u32(s, reinterpret_cast<uint32_t*>(d.scalar.p), 0u, 1);
-    mmhrt::dn::launch_compose(s, d.out.p, d.raw.p, B, H, W, Hp, Wp, bgc, target, 1.0f / (float)(P * 3),
-                              fb_col ? nullptr : d.color.p, train ? d.gbuf.p : nullptr, loss ? d.scalar.p : nullptr,
-                              fb_col, fb_rgba8);
+    // a rendered frame's albedo and hit distance come straight from its aux buffer (no raw copy)
+    const uint16_t* feat = aux8 ? reinterpret_cast<const uint16_t*>(aux8) : d.raw.p;
+    mmhrt::dn::launch_compose(s, d.out.p, feat, aux8 ? 8 : 12, aux8 ? 0 : 3, aux8 ? 6 : 9, B, H, W, Hp, Wp, bgc,
+                              target, 1.0f / (float)(P * 3), fb_col ? nullptr : d.color.p, train ? d.gbuf.p : nullptr,
+                              loss ? d.scalar.p : nullptr, fb_col, fb_rgba8);
     return 0;
 }
 }  // namespace
@@ -1667,14 +1676,13 @@ int32_t mmhrt_denoise(mmhrt_ctx c, float depth_scale) {
         return mmhrt::fail(c, MMHRT_E_STATE, "denoise: render a frame with MMHRT_FLAG_AUX first");
     Denoiser& d = *c->dn;
     int H = (int)c->fb_h, W = (int)c->fb_w;
-    size_t P = (size_t)H * W;
-    if (int e = mmhrt::ensure(c, d.raw, P * 12, "denoiser features")) return e;
     const int Hp = ceil8(H), Wp = ceil8(W);
     if (int e = d.ensure_acts(c, 1, Hp, Wp, false)) return e;
     const float inv_depth = depth_scale > 0.0f ? 1.0f / depth_scale : 1.0f;
     hipLaunchKernelGGL(mmhrt::dn::k_frame_input, dim3((unsigned)(((size_t)Hp * Wp + 255) / 256)), dim3(256), 0,
-                       c->stream, c->fb_col.p, c->fb_aux.p, H, W, Hp, Wp, inv_depth, d.raw.p, d.x0);
-    if (int e = dn_run(c, 1, H, W, depth_scale, c->last_bg, nullptr, nullptr, false, c->fb_col.p, c->fb_rgba8.p, true))
+                       c->stream, c->fb_col.p, c->fb_aux.p, H, W, Hp, Wp, inv_depth, nullptr, d.x0);
+    if (int e = dn_run(c, 1, H, W, depth_scale, c->last_bg, nullptr, nullptr, false, c->fb_col.p, c->fb_rgba8.p, true,
+                       c->fb_aux.p))
         return e;
     hipError_t he = hipGetLastError();
     return he == hipSuccess ? MMHRT_OK : mmhrt::hip_fail(c, he, "denoise launch");

@@ -16,9 +16,9 @@ struct LayerSpec {
 };
 extern const LayerSpec kLayers[kNumLayers];
 size_t param_count();
-void launch_compose(hipStream_t s, const float* out, const uint16_t* raw, int B, int H, int W, int Hp, int Wp,
-                    float4 bg, const float* target, float inv_count, float* color, uint16_t* dout, float* loss,
-                    float4* fb_col, uint32_t* fb_rgba8);
+void launch_compose(hipStream_t s, const float* out, const uint16_t* feat, int fstride, int fa, int ft, int B, int H,
+                    int W, int Hp, int Wp, float4 bg, const float* target, float inv_count, float* color,
+                    uint16_t* dout, float* loss, float4* fb_col, uint32_t* fb_rgba8);
 void zero(hipStream_t s, float* p, long n);
 void adam(hipStream_t s, float* p, const float* g, float* m, float* v, long n, float lr, float b1, float b2, float eps,
           float c1, float c2);
