@@ -153,7 +153,7 @@ def run_ablation(r, cam, args):
 
 
 BUILD_SWEEP = {"build_chunk": (1024, 512, 256), "sort_ri": (8, 16, 32), "sort_digit": (0, 8),
-               "build_wave": (64, 0, 128, 256, 1024),
+               "build_wave": (64, 0, 128, 256, 1024), "build_span": (256, 64, 128, 512, 1024),
                "leaf_pairs": (1, 0)}
 
 

@@ -168,7 +168,7 @@ int32_t mmhrt_destroy(mmhrt_ctx c) {
     release(c->meet); release(c->meet_far); release(c->ka); release(c->kb); release(c->kla); release(c->klb);
     release(c->va); release(c->vb); release(c->hist); release(c->scan_tmp); release(c->sort_fault);
     release(c->pair_count); release(c->xlist);
-    release(c->leaf_sph); release(c->leaf_id); release(c->nodes); release(c->flags); release(c->box_cnt);
+    release(c->leaf_sph); release(c->leaf_id); release(c->nodes); release(c->flags); release(c->flags2); release(c->box_cnt);
     release(c->partial); release(c->info); release(c->tf_buf);
     release(c->fb_col); release(c->fb_depth); release(c->fb_t); release(c->fb_id); release(c->fb_rgba8);
     release(c->fb_aux); release(c->spill); release(c->counters);

@@ -100,6 +100,7 @@ struct mmhrt_ctx_s {
     mmhrt::DevBuf<uint32_t> leaf_id;
     mmhrt::DevBuf<mmhrt::Node> nodes;
     mmhrt::DevBuf<uint32_t> flags;   // arrivals at the global meeting places (generation numbers)
+    mmhrt::DevBuf<uint32_t> flags2;  // the chunk pass's arrivals at chunk boundaries, per side (generation numbers)
     uint32_t build_gen = 0;          // generation number of the last hierarchy pass
     bool bvh_pairs = true;           // the current tree may reference leaf pairs (built with leaf_pairs on)
     uint32_t* rounds_host = nullptr; // pinned: the last build's straddler counters (BOX_SLOTS words)

@@ -916,7 +916,7 @@ template <typename K, int BC>
 __global__ __launch_bounds__(BC) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_bu_chunk_ls(
     const K* __restrict__ keys, const uint32_t* __restrict__ ids, const float4* __restrict__ sph,
     float4* __restrict__ leaf_sph, int64_t n64, Node* __restrict__ nodes, Node* meet, int4* meet_far, uint32_t* flags,
-    uint32_t* list, uint32_t* cnt, SceneInfo* info, int wave_items) {
+    uint32_t* flags2, uint32_t* list, uint32_t* cnt, SceneInfo* info, int wave_items) {
     __shared__ float s_box[BC][12];     // the two child boxes parked at each in-chunk split
     __shared__ uint16_t s_ref[BC][2];   // chunk-relative child references
     __shared__ uint16_t s_far[BC][2];   // chunk-relative far range ends
@@ -1149,10 +1149,17 @@ __global__ __launch_bounds__(BC) __attribute__((amdgpu_waves_per_eu(8, 8))) void
         const uint32_t id_next2 = load_id(ch + 2 * G);
         // arrival counts at the global meeting places: the boundary splits delivered to, and the
         // in-chunk splits with one arrival (their other child straddles the chunk), whose parked
-        // child moves to the global record first
+        // child moves to the global record first. With flags2 no count is exchanged: no other
+        // child of these splits can arrive in this pass (a straddler arrives in a later pass, a
+        // neighbour chunk's node at a boundary split only here), so the arrivals are recorded
+        // with plain stores, a boundary split's per side (flags2[2k + side] for split k * BC - 1),
+        // and the later passes tell which splits the chunks completed.
         if (tid < 2 && s_pend[tid] >= 0) {
             const int32_t sp = s_pend[tid];
-            if (atomicExch(flags + sp, gen) == gen) ap.push((uint32_t)sp, list, cnt + 1);
+            if (flags2)  // left boundary (tid 0): its right side; right boundary: its left side
+                flags2[2 * (ch + tid) + 1 - tid] = gen;
+            else if (atomicExch(flags + sp, gen) == gen)
+                ap.push((uint32_t)sp, list, cnt + 1);
         }
         const int nsplit = ce - cs - 1;
         for (int slot = tid; slot < nsplit; slot += BC) {
@@ -1166,10 +1173,22 @@ __global__ __launch_bounds__(BC) __attribute__((amdgpu_waves_per_eu(8, 8))) void
             const uint32_t hd = s_hd[slot][side];
             meet_put(meet + sp, meet_far + sp, side, b, gref(s_ref[slot][side]), (int)(hd & 0xffu),
                      cs + (int32_t)s_far[slot][side], (int)(int8_t)(hd >> 8));
-            if (atomicExch(flags + sp, gen) == gen) ap.push((uint32_t)sp, list, cnt + 1);
+            if (flags2)
+                flags[sp] = gen;
+            else if (atomicExch(flags + sp, gen) == gen)
+                ap.push((uint32_t)sp, list, cnt + 1);
         }
-        __syncthreads();
-        if (s_n > 192u) ap.flush(list, cnt + 1);  // s_n is read by every thread after the barrier
+        if (flags2) {
+            // only LDS needs ordering here: the stores above are for later passes, and the next
+            // chunk's loads are waited for where they are used
+            __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            __builtin_amdgcn_s_barrier();
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        } else {
+            __syncthreads();
+            if (s_n > 192u) ap.flush(list, cnt + 1);  // s_n is read by every thread after the barrier
+        }
         // rotate the pipeline
         id_cur = id_next;
         id_next = id_next2;
@@ -1183,10 +1202,14 @@ __global__ __launch_bounds__(BC) __attribute__((amdgpu_waves_per_eu(8, 8))) void
 
 // Merge the two records at split s, write the node, deliver it to its parent's meeting place.
 // Returns the parent's split when this delivery completes it, else -1.
+// flags2 (with chunk, the chunk pass's leaves per chunk): that pass's arrivals at the chunk
+// boundary splits k * chunk - 1, stored per side as flags2[2k + side] = gen rather than
+// exchanged at flags, so a delivery there also completes the split when the other side's entry
+// holds gen. Null: every earlier arrival was exchanged at flags.
 template <typename K, bool VOL>
 __device__ __forceinline__ int64_t bu_step(const K* __restrict__ keys, int64_t n, Node* nodes, Node* meet,
                                            int4* meet_far, uint32_t* flags, int64_t s, SceneInfo* info,
-                                           uint32_t gen) {
+                                           uint32_t gen, const uint32_t* flags2 = nullptr, int64_t chunk = 1) {
     // The two records at s were written by an earlier kernel (visible since this launch began) or,
     // within one workgroup's levels (VOL), by this workgroup before the barrier that ended the
     // previous level: waves of a workgroup run on one CU, whose vector L1 they share, so plain
@@ -1237,7 +1260,10 @@ __device__ __forceinline__ int64_t bu_step(const K* __restrict__ keys, int64_t n
     }
     const int64_t s2 = left ? r : l - 1;
     meet_put(meet + s2, meet_far + s2, left ? 0 : 1, b, ref, h, (int32_t)(left ? l : r), left ? dl : dr);
-    return atomicExch(flags + s2, gen) == gen ? s2 : -1;
+    const bool bnd = flags2 != nullptr && (s2 + 1) % chunk == 0;
+    const uint32_t other = bnd ? flags2[((s2 + 1) / chunk) * 2 + (left ? 1 : 0)] : 0u;
+    const uint32_t x = atomicExch(flags + s2, gen);
+    return (x == gen || (bnd && other == gen)) ? s2 : -1;
 }
 
 template <typename K>
@@ -1335,7 +1361,7 @@ template <typename K>
 __device__ void span_levels(const K* __restrict__ keys, int64_t n, Node* nodes, Node* meet, int4* meet_far,
                             uint32_t* flags, SceneInfo* info, uint32_t gen, int64_t lo, int64_t hi, uint32_t n0,
                             bool in_lds, uint32_t* gl0, uint32_t* gl1, uint32_t (*s_l)[SPAN_LDS], uint32_t* s_nc,
-                            uint32_t* xl, uint32_t* xn) {
+                            uint32_t* xl, uint32_t* xn, const uint32_t* flags2, int64_t chunk) {
     uint32_t n_cur = n0;
     for (int lv = 0; n_cur > 0; ++lv) {
         uint32_t* lin = in_lds ? s_l[lv & 1] : ((lv & 1) ? gl1 : gl0);
@@ -1344,7 +1370,7 @@ __device__ void span_levels(const K* __restrict__ keys, int64_t n, Node* nodes, 
         if (threadIdx.x == 0) s_nc[(lv + 2) % 3] = 0;
         for (uint32_t i = threadIdx.x; i < n_cur; i += blockDim.x) {
             const int64_t sp = (int64_t)lin[i];
-            const int64_t s2 = bu_step<K, true>(keys, n, nodes, meet, meet_far, flags, sp, info, gen);
+            const int64_t s2 = bu_step<K, true>(keys, n, nodes, meet, meet_far, flags, sp, info, gen, flags2, chunk);
             if (s2 < 0) continue;
             if (s2 >= lo && s2 <= hi - 2)
                 lout[atomicAdd(nc_next, 1u)] = (uint32_t)s2;
@@ -1361,14 +1387,17 @@ __device__ void span_levels(const K* __restrict__ keys, int64_t n, Node* nodes, 
 // split lies inside the span (lo <= s <= hi - 2) has both children inside it, so the span's
 // workgroup merges it once both have arrived; a delivery that completes one of the span's two
 // boundary splits is exported to xl (count cnt[2]) instead, and k_bu_top finishes the exported
-// splits. Level 0 is the chunk pass's list, which every span scans for its own splits (those at
-// hi - 1, completed on the span's right boundary by the chunk pass, are exported at once). la
-// and lb (n entries each) hold the span's level lists at offset lo when LDS cannot.
+// splits. Level 0 is what the chunk pass completed: with flags2 (its per-side arrival records at
+// the boundaries of its chunks, chunk leaves each; span is a multiple of chunk) the chunk
+// boundaries inside the span with both sides recorded, else its list, which every span scans for
+// its own splits. Those at hi - 1, completed on the span's right boundary, are exported at once.
+// la and lb (n entries each) hold the span's level lists at offset lo when LDS cannot.
 template <typename K>
 __global__ __launch_bounds__(1024) void k_bu_span(const K* __restrict__ keys, int64_t n, Node* nodes, Node* meet,
-                                                  int4* meet_far, uint32_t* flags, const uint32_t* __restrict__ list0,
-                                                  uint32_t* la, uint32_t* lb, uint32_t* xl, uint32_t* cnt,
-                                                  int64_t span, SceneInfo* info) {
+                                                  int4* meet_far, uint32_t* flags, const uint32_t* __restrict__ flags2,
+                                                  int64_t chunk, const uint32_t* __restrict__ list0, uint32_t* la,
+                                                  uint32_t* lb, uint32_t* xl, uint32_t* cnt, int64_t span,
+                                                  SceneInfo* info) {
     __shared__ uint32_t s_l[2][SPAN_LDS];
     __shared__ uint32_t s_nc[3];
     const uint32_t gen = cnt[GEN_SLOT];
@@ -1376,30 +1405,43 @@ __global__ __launch_bounds__(1024) void k_bu_span(const K* __restrict__ keys, in
     const int64_t hi = lo + span < n ? lo + span : n;
     if (threadIdx.x < 3) s_nc[threadIdx.x] = 0;
     __syncthreads();
-    const uint32_t n0 = cnt[1];
-    for (uint32_t i = threadIdx.x; i < n0; i += blockDim.x) {
-        const int64_t sp = (int64_t)list0[i];
-        if (sp >= lo && sp <= hi - 2) {
+    auto level0 = [&](int64_t sp) {
+        if (sp <= hi - 2) {
             const uint32_t k = atomicAdd(&s_nc[0], 1u);
             if (k < SPAN_LDS) s_l[0][k] = (uint32_t)sp;
             la[lo + k] = (uint32_t)sp;
-        } else if (sp == hi - 1) {
+        } else {
             xl[atomicAdd(cnt + 2, 1u)] = (uint32_t)sp;
+        }
+    };
+    if (flags2) {
+        const int64_t last = hi - 1 < n - 2 ? hi - 1 : n - 2;
+        for (int64_t sp = lo + chunk - 1 + (int64_t)threadIdx.x * chunk; sp <= last; sp += (int64_t)blockDim.x * chunk) {
+            const int64_t k = (sp + 1) / chunk;
+            if (flags2[2 * k] == gen && flags2[2 * k + 1] == gen) level0(sp);
+        }
+    } else {
+        const uint32_t n0 = cnt[1];
+        for (uint32_t i = threadIdx.x; i < n0; i += blockDim.x) {
+            const int64_t sp = (int64_t)list0[i];
+            if (sp >= lo && sp <= hi - 1) level0(sp);
         }
     }
     __threadfence_block();
     __syncthreads();
     const uint32_t m = s_nc[0];
     span_levels<K>(keys, n, nodes, meet, meet_far, flags, info, gen, lo, hi, m, m <= SPAN_LDS, la + lo, lb + lo, s_l,
-                   s_nc, xl, cnt + 2);
+                   s_nc, xl, cnt + 2, flags2, chunk);
 }
 
 // The top of the tree: one workgroup merges the spans' exported splits (xl, count cnt[2]) up to
-// the root; xl2 is the second global list when LDS cannot hold them.
+// the root; xl2 is the second global list when LDS cannot hold them. flags2 and chunk as for
+// k_bu_span.
 template <typename K>
 __global__ __launch_bounds__(1024) void k_bu_top(const K* __restrict__ keys, int64_t n, Node* nodes, Node* meet,
-                                                 int4* meet_far, uint32_t* flags, uint32_t* xl, uint32_t* xl2,
-                                                 uint32_t* cnt, SceneInfo* info) {
+                                                 int4* meet_far, uint32_t* flags, const uint32_t* __restrict__ flags2,
+                                                 int64_t chunk, uint32_t* xl, uint32_t* xl2, uint32_t* cnt,
+                                                 SceneInfo* info) {
     __shared__ uint32_t s_l[2][SPAN_LDS];
     __shared__ uint32_t s_nc[3];
     const uint32_t gen = cnt[GEN_SLOT];
@@ -1409,7 +1451,7 @@ __global__ __launch_bounds__(1024) void k_bu_top(const K* __restrict__ keys, int
     __syncthreads();
     // every split lies inside [0, n - 2]: nothing is exported (cnt[3] is an unused counter here)
     span_levels<K>(keys, n, nodes, meet, meet_far, flags, info, gen, 0, n, n0, n0 <= SPAN_LDS, xl, xl2, s_l, s_nc,
-                   xl2, cnt + 3);
+                   xl2, cnt + 3, flags2, chunk);
 }
 
 // Leaf pairs of the tree over sorted keys: leaves l and l + 1 both attach to split l exactly
@@ -1557,11 +1599,11 @@ unsigned chunk_grid(mmhrt_ctx_s* c, uint64_t n) {
 }
 
 template <typename K, int BC>
-void launch_chunk_ls(mmhrt_ctx_s* c, const K* keys, const uint32_t* ids, uint32_t* list) {
+void launch_chunk_ls(mmhrt_ctx_s* c, const K* keys, const uint32_t* ids, uint32_t* flags2, uint32_t* list) {
     const uint64_t n = c->n;
     hipLaunchKernelGGL((k_bu_chunk_ls<K, BC>), dim3(chunk_grid<K, BC>(c, n)), dim3(BC), 0, c->stream, keys, ids,
                        cur_spheres(c), c->leaf_sph.p, (int64_t)n, c->nodes.p,
-                       c->meet.p, c->meet_far.p, c->flags.p, list, c->box_cnt.p, c->info.p,
+                       c->meet.p, c->meet_far.p, c->flags.p, flags2, list, c->box_cnt.p, c->info.p,
                        (int)std::max<int64_t>(0, std::min<int64_t>(c->opt_build_wave, BC)));
 }
 
@@ -1602,14 +1644,25 @@ int hierarchy(mmhrt_ctx_s* c, const K* keys, const uint32_t* ids) {
     // leaves per chunk workgroup: 1024 (two per CU), 512 or 256; level-synchronous rounds unless
     // the option build_ls is 0
     const int bc = c->opt_build_chunk == 256 ? 256 : (c->opt_build_chunk == 512 ? 512 : 1024);
+    const int64_t nchunks = ((int64_t)n + bc - 1) / bc;
+    // per-side arrival records at the chunk boundaries: the level-synchronous chunk pass followed
+    // by span passes stores its boundary arrivals there instead of exchanging them
+    uint32_t* flags2 = nullptr;
+    if (c->opt_build_ls != 0 && c->opt_build_span > 0) {
+        const uint64_t m2 = 2 * (uint64_t)(nchunks + 1);
+        const bool fresh2 = !c->flags2.p || c->flags2.cap < m2;
+        if (int e = ensure(c, c->flags2, m2, "chunk boundary arrivals")) return e;
+        if (fresh2) launch_fill_u32(s, c->flags2.p, 0xffffffffu, c->flags2.cap);
+        flags2 = c->flags2.p;
+    }
     rec(c, 8);
     if (c->opt_build_ls != 0) {
         if (bc == 256)
-            launch_chunk_ls<K, 256>(c, keys, ids, l0);
+            launch_chunk_ls<K, 256>(c, keys, ids, flags2, l0);
         else if (bc == 512)
-            launch_chunk_ls<K, 512>(c, keys, ids, l0);
+            launch_chunk_ls<K, 512>(c, keys, ids, flags2, l0);
         else
-            launch_chunk_ls<K, 1024>(c, keys, ids, l0);
+            launch_chunk_ls<K, 1024>(c, keys, ids, flags2, l0);
     } else if (bc == 256)
         hipLaunchKernelGGL((k_bu_chunk<K, 256>), dim3(blocks_for(n, 256)), dim3(256), 0, s, keys, c->leaf_sph.p,
                            (int64_t)n, c->nodes.p, c->meet.p, c->meet_far.p, c->flags.p, l0, c->box_cnt.p, c->info.p);
@@ -1625,19 +1678,21 @@ int hierarchy(mmhrt_ctx_s* c, const K* keys, const uint32_t* ids) {
     // hints only: the rounds loop over any list size and the tail finishes whatever is left)
     if (c->opt_build_span > 0) {
         // spans of build_span chunks, one workgroup each, then the tail over their exports (at
-        // most three per span: its right boundary from the chunk pass, its two boundary splits)
-        // at most 256 spans: every span's workgroup scans the chunk pass's whole list, so larger
-        // scenes take longer spans rather than more of them
-        const int64_t nchunks = ((int64_t)n + bc - 1) / bc;
-        const int64_t span = std::max<int64_t>(c->opt_build_span, (nchunks + 255) / 256) * (int64_t)bc;
+        // most three per span: its right boundary from the chunk pass, its two boundary splits).
+        // Without flags2 at most 256 spans: every span's workgroup then scans the chunk pass's
+        // whole list, so larger scenes take longer spans rather than more of them
+        const int64_t span_chunks =
+            flags2 ? c->opt_build_span : std::max<int64_t>(c->opt_build_span, (nchunks + 255) / 256);
+        const int64_t span = span_chunks * (int64_t)bc;
         const uint64_t spans = (n + span - 1) / span;
         const size_t xcap = 4 * spans + 64;
         if (int e = ensure(c, c->xlist, 2 * xcap, "span export lists")) return e;
         hipLaunchKernelGGL(k_bu_span<K>, dim3((unsigned)spans), dim3(1024), 0, s, keys, (int64_t)n, c->nodes.p,
-                           c->meet.p, c->meet_far.p, c->flags.p, l0, c->va.p, c->vb.p, c->xlist.p, c->box_cnt.p, span,
-                           c->info.p);
+                           c->meet.p, c->meet_far.p, c->flags.p, flags2, (int64_t)bc, l0, c->va.p, c->vb.p,
+                           c->xlist.p, c->box_cnt.p, span, c->info.p);
         hipLaunchKernelGGL(k_bu_top<K>, dim3(1), dim3(1024), 0, s, keys, (int64_t)n, c->nodes.p, c->meet.p,
-                           c->meet_far.p, c->flags.p, c->xlist.p, c->xlist.p + xcap, c->box_cnt.p, c->info.p);
+                           c->meet_far.p, c->flags.p, flags2, (int64_t)bc, c->xlist.p, c->xlist.p + xcap,
+                           c->box_cnt.p, c->info.p);
         return 0;
     }
     const uint32_t tail_items = (uint32_t)(c->opt_build_tail > 0 ? c->opt_build_tail : 1);

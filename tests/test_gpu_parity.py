@@ -80,7 +80,9 @@ def test_lbvh_equals_oracle_lbvh(gpu_renderer, chunk, ls, ri, grid, os1, wave, t
     rounds to the top);
     tail is the straddler list size from which one workgroup finishes the tree (1 << 30: it takes
     every straddler from the chunk pass on) when span is 0 (kernel rounds); span > 0 merges the
-    straddlers with one workgroup per span of that many chunks before the tail. dig is the radix
+    straddlers with one workgroup per span of that many chunks before the tail (after the
+    level-synchronous chunk pass, which then stores its boundary arrivals per side instead of
+    exchanging them, span 1 makes every chunk boundary a span boundary). dig is the radix
     digit width: 8 the 8-bit passes, 0 the wide digits (3 passes of 10 bits for 30-bit keys, 6 of
     11 bits for 63-bit keys; single-pass radix passes always take 8-bit digits)."""
     gpu_renderer.set_option("sort_digit", dig)
