@@ -457,6 +457,52 @@ __global__ __launch_bounds__(256) void k_col_digits(uint32_t* __restrict__ gsum,
     }
 }
 
+// 8-bit digits: each digit's exclusive prefix over the groups and its total (row `groups`), in
+// place of k_col_top and k_col_digits. Block b takes digits 64 b .. 64 b + 63 with SCAN_T
+// threads per digit, each over a contiguous run of groups: one load latency per walk, where the
+// one-block scan walks the groups 32 at a time. The digit totals are scanned by k_col_offsets.
+constexpr int SCAN_T = 16;
+__global__ __launch_bounds__(1024) void k_col_scan(uint32_t* __restrict__ gsum, uint32_t groups) {
+    __shared__ uint32_t part[SCAN_T][64];
+    const uint32_t dl = threadIdx.x & 63u, q = threadIdx.x >> 6;
+    const uint32_t d = blockIdx.x * 64u + dl;
+    const uint32_t gpt = (groups + SCAN_T - 1) / SCAN_T;
+    const uint32_t g0 = q * gpt;
+    const uint32_t g1 = g0 + gpt < groups ? g0 + gpt : groups;
+    // loads at clamped rows, masked where summed (a guarded load compiles to a branch and a wait)
+    auto row = [&](uint32_t g) -> uint32_t { return g < g1 ? g : g1 - 1; };
+    uint32_t sum = 0;
+    for (uint32_t g = g0; g < g1; g += 16) {
+        uint32_t v[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) v[k] = gsum[(size_t)row(g + k) * 256 + d];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) sum += g + k < g1 ? v[k] : 0u;
+    }
+    part[q][dl] = sum;
+    __syncthreads();
+    uint32_t run = 0;
+    for (uint32_t j = 0; j < q; ++j) run += part[j][dl];
+    if (q == SCAN_T - 1) gsum[(size_t)groups * 256 + d] = run + sum;
+    for (uint32_t g = g0; g < g1; g += 16) {
+        uint32_t v[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) v[k] = gsum[(size_t)row(g + k) * 256 + d];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            if (g + k < g1) {
+                gsum[(size_t)(g + k) * 256 + d] = run;
+                run += v[k];
+            }
+        }
+    }
+}
+
+// Each tile's offsets: its group's prefix plus the running count down the group's tiles, plus the
+// digit's base. SCAN: the digit totals in row `groups` are not scanned yet (k_col_scan); every
+// block scans them itself (nd = 256, one digit per thread), 1 KB each, instead of a kernel of its
+// own doing it once.
+template <bool SCAN>
 __global__ __launch_bounds__(256) void k_col_offsets(const uint32_t* __restrict__ hist, uint32_t tiles,
                                                      uint32_t groups, const uint32_t* __restrict__ gsum,
                                                      uint32_t* __restrict__ offs, uint32_t nd) {
@@ -465,7 +511,25 @@ __global__ __launch_bounds__(256) void k_col_offsets(const uint32_t* __restrict_
     uint32_t v[CG];
 #pragma unroll
     for (int k = 0; k < CG; ++k) v[k] = t0 + k < tiles ? hist[(size_t)(t0 + k) * nd + d] : 0u;
-    uint32_t run = gsum[(size_t)blockIdx.x * nd + d] + gsum[(size_t)groups * nd + d];
+    uint32_t base;
+    if constexpr (SCAN) {
+        __shared__ uint32_t wsum[4];
+        const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+        const uint32_t tot = gsum[(size_t)groups * nd + d];
+        uint32_t inc = tot;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(inc, o, 64);
+            if (lane >= (uint32_t)o) inc += y;
+        }
+        if (lane == 63) wsum[wave] = inc;
+        __syncthreads();
+        base = inc - tot;
+        for (uint32_t w = 0; w < wave; ++w) base += wsum[w];
+    } else {
+        base = gsum[(size_t)groups * nd + d];
+    }
+    uint32_t run = gsum[(size_t)blockIdx.x * nd + d] + base;
 #pragma unroll
     for (int k = 0; k < CG; ++k) {
         if (t0 + k < tiles) offs[(size_t)(t0 + k) * nd + d] = run;
@@ -1554,10 +1618,17 @@ int morton_sort_t(mmhrt_ctx_s* c, K* ka, K* kb, uint32_t* va, uint32_t* vb, uint
             hipLaunchKernelGGL((k_radix_hist<K, RI, DB>), dim3(tiles), dim3(RB), 0, c->stream, src, n, shift, hist);
         hipLaunchKernelGGL(k_col_sums, dim3(groups, ND / 256), dim3(256), 0, c->stream, hist, tiles, c->scan_tmp.p,
                            (uint32_t)ND);
-        hipLaunchKernelGGL(k_col_top, dim3(ND / 256), dim3(256), 0, c->stream, c->scan_tmp.p, groups, (uint32_t)ND);
-        hipLaunchKernelGGL(k_col_digits, dim3(1), dim3(256), 0, c->stream, c->scan_tmp.p, groups, (uint32_t)ND);
-        hipLaunchKernelGGL(k_col_offsets, dim3(groups, ND / 256), dim3(256), 0, c->stream, hist, tiles, groups,
-                           c->scan_tmp.p, offs, (uint32_t)ND);
+        if (ND == 256) {
+            hipLaunchKernelGGL(k_col_scan, dim3(4), dim3(1024), 0, c->stream, c->scan_tmp.p, groups);
+            hipLaunchKernelGGL(k_col_offsets<true>, dim3(groups), dim3(256), 0, c->stream, hist, tiles, groups,
+                               c->scan_tmp.p, offs, (uint32_t)ND);
+        } else {
+            hipLaunchKernelGGL(k_col_top, dim3(ND / 256), dim3(256), 0, c->stream, c->scan_tmp.p, groups,
+                               (uint32_t)ND);
+            hipLaunchKernelGGL(k_col_digits, dim3(1), dim3(256), 0, c->stream, c->scan_tmp.p, groups, (uint32_t)ND);
+            hipLaunchKernelGGL(k_col_offsets<false>, dim3(groups, ND / 256), dim3(256), 0, c->stream, hist, tiles,
+                               groups, c->scan_tmp.p, offs, (uint32_t)ND);
+        }
         const bool xcd = c->opt_sort_xcd != 0;
         hipLaunchKernelGGL((k_radix_scatter<K, RI, DB>), dim3(xcd ? 8 * ((tiles + 7) / 8) : tiles), dim3(RB), 0,
                            c->stream, src, vs, dst, p == passes - 1 ? vfinal : vd, n, shift, hist, offs, (int)xcd);
