@@ -1559,6 +1559,17 @@ __global__ void k_fill(uint32_t* p, uint32_t v, uint64_t n) {
 
 inline unsigned blocks_for(uint64_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
+// Per hierarchy pass: list counters cleared, the pass's generation number, the leaf-pair height
+// (one launch instead of three fills).
+__global__ void k_build_init(uint32_t* cnt, uint32_t gen, uint32_t* pair_h, uint32_t pv) {
+    const uint32_t i = threadIdx.x;
+    if (i < (uint32_t)(BOX_ROUNDS + 3)) cnt[i] = 0u;
+    if (i == 0) {
+        cnt[GEN_SLOT] = gen;
+        *pair_h = pv;
+    }
+}
+
 // Morton codes of the current spheres into ka, then the LSD radix sort of (key, particle index)
 // over `passes` 8-bit digits; the sorted keys end in ka, the sorted values in vfinal (the last
 // pass writes them there). The Morton pass also produces the first digit's tile histograms, and
@@ -1704,9 +1715,8 @@ int hierarchy(mmhrt_ctx_s* c, const K* keys, const uint32_t* ids) {
     // each pass takes a new generation; builds captured into a graph keep theirs, so such a
     // graph is launched once (the batched time series), never replayed
     if (++c->build_gen == 0xffffffffu) c->build_gen = 1;
-    launch_fill_u32(s, c->box_cnt.p, 0u, BOX_ROUNDS + 3);
-    launch_fill_u32(s, c->box_cnt.p + GEN_SLOT, c->build_gen, 1);
-    launch_fill_u32(s, &c->info.p->pair_h, c->opt_leaf_pairs ? 1u : 0u, 1);
+    hipLaunchKernelGGL(k_build_init, dim3(1), dim3(64), 0, s, c->box_cnt.p, c->build_gen, &c->info.p->pair_h,
+                       c->opt_leaf_pairs ? 1u : 0u);
     c->bvh_pairs = c->opt_leaf_pairs != 0;
     // worklists of the straddler rounds: the spare key buffer (free once the sort is done) and
     // the spare value buffer
