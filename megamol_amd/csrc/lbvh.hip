@@ -260,11 +260,21 @@ __global__ __launch_bounds__(RB) void k_radix_hist(const K* __restrict__ keys, u
     for (int dd = threadIdx.x; dd < ND; dd += RB) hist[(size_t)blockIdx.x * ND + dd] = h[dd];
 }
 
-// Stable scatter of one tile for one 8-bit digit (vin null: the values are the key indices). Each wave ranks its segment with ballots and
-// a running per-(wave, digit) count in LDS (no barrier between rounds), one pass over the 256
-// digits turns the per-wave counts into offsets, the keys are placed in digit-sorted order in
-// LDS and finally written out in digit-contiguous runs, so that consecutive threads write
-// consecutive addresses.
+// Barrier for work that shares only LDS: waits for this wave's LDS accesses, not for its global
+// loads and stores, so loads issued before it stay in flight.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_barrier();
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+// Stable scatter of one tile for one digit (vin null: the values are the key indices). Each wave
+// ranks its segment with ballots and a running per-(wave, digit) count in LDS (no barrier between
+// rounds), one pass over the digits turns the per-wave counts into offsets, the keys are placed in
+// digit-sorted order in LDS and finally written out in digit-contiguous runs, so that consecutive
+// threads write consecutive addresses. The tile's digit counts and global offsets are loaded with
+// its keys, and the barriers order LDS only (no global memory passes between the threads).
 template <typename K, int RI, int DB>
 __global__ __launch_bounds__(RB) void k_radix_scatter(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                       K* __restrict__ kout, uint32_t* __restrict__ vout, uint64_t n,
@@ -289,31 +299,45 @@ __global__ __launch_bounds__(RB) void k_radix_scatter(const K* __restrict__ kin,
     const uint32_t lane = __lane_id();
     const uint64_t tbase = (uint64_t)tile * RT;
     const uint64_t base = tbase + (uint64_t)wave * RW + lane;
+    // loads: a whole tile from one base address at constant offsets, the last one at clamped
+    // indices (validity is applied where the keys are used)
     K key[RI];
     uint32_t val[RI];
+    if (tbase + RT <= n) {
+        const K* kp = kin + base;
 #pragma unroll
-    for (int r = 0; r < RI; ++r) {
-        uint64_t i = base + (uint64_t)r * 64;
-        bool valid = i < n;
-        key[r] = valid ? kin[i] : (K)0;
-    }
-    if (vin) {
+        for (int r = 0; r < RI; ++r) key[r] = kp[r * 64];
+        if (vin) {
+            const uint32_t* vp = vin + base;
+#pragma unroll
+            for (int r = 0; r < RI; ++r) val[r] = vp[r * 64];
+        }
+    } else {
 #pragma unroll
         for (int r = 0; r < RI; ++r) {
-            uint64_t i = base + (uint64_t)r * 64;
-            val[r] = i < n ? vin[i] : 0u;
+            const uint64_t i = base + (uint64_t)r * 64;
+            key[r] = kin[i < n ? i : n - 1];
         }
-    } else {  // the first pass: the values are the indices
+        if (vin) {
+#pragma unroll
+            for (int r = 0; r < RI; ++r) {
+                const uint64_t i = base + (uint64_t)r * 64;
+                val[r] = vin[i < n ? i : n - 1];
+            }
+        }
+    }
+    if (!vin) {  // the first pass: the values are the indices
 #pragma unroll
         for (int r = 0; r < RI; ++r) val[r] = (uint32_t)(base + (uint64_t)r * 64);
     }
     // tile-local digit starts: exclusive scan of this tile's digit counts; thread t owns the DPT
     // consecutive digits from t * DPT
-    uint32_t cnt[DPT];
+    uint32_t cnt[DPT], ofv[DPT];
     uint32_t tsum = 0;
 #pragma unroll
     for (int j = 0; j < DPT; ++j) {
         cnt[j] = hist[(size_t)tile * ND + threadIdx.x * DPT + j];
+        ofv[j] = offs[(size_t)tile * ND + threadIdx.x * DPT + j];
         tsum += cnt[j];
     }
     uint32_t inc = tsum;
@@ -326,7 +350,7 @@ __global__ __launch_bounds__(RB) void k_radix_scatter(const K* __restrict__ kin,
     for (int d = threadIdx.x; d < ND; d += RB)
 #pragma unroll
         for (int w = 0; w < RB / 64; ++w) wc[w][d] = 0;
-    __syncthreads();
+    lds_barrier();
     uint32_t wb = 0;
     for (int w = 0; w < wave; ++w) wb += wsum[w];
     uint32_t lstart[DPT];
@@ -336,7 +360,7 @@ __global__ __launch_bounds__(RB) void k_radix_scatter(const K* __restrict__ kin,
         for (int j = 0; j < DPT; ++j) {
             const int d = threadIdx.x * DPT + j;
             lstart[j] = run;
-            goff[d] = offs[(size_t)tile * ND + d] - run;
+            goff[d] = ofv[j] - run;
             run += cnt[j];
         }
     }
@@ -352,7 +376,7 @@ __global__ __launch_bounds__(RB) void k_radix_scatter(const K* __restrict__ kin,
         if (valid && (__ffsll((unsigned long long)peers) - 1) == (int)lane)
             wc[wave][d] = (uint16_t)(before + (uint32_t)__popcll(peers));
     }
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int j = 0; j < DPT; ++j) {  // per-wave counts -> tile-local offsets of each wave's run
         const int d = threadIdx.x * DPT + j;
@@ -364,7 +388,7 @@ __global__ __launch_bounds__(RB) void k_radix_scatter(const K* __restrict__ kin,
             acc += c;
         }
     }
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int r = 0; r < RI; ++r) {
         if (base + (uint64_t)r * 64 < n) {
@@ -374,7 +398,7 @@ __global__ __launch_bounds__(RB) void k_radix_scatter(const K* __restrict__ kin,
             s_val[pos] = val[r];
         }
     }
-    __syncthreads();
+    lds_barrier();
     const uint32_t nvalid = (uint32_t)((n - tbase) < (uint64_t)RT ? (n - tbase) : (uint64_t)RT);
     for (uint32_t i = threadIdx.x; i < nvalid; i += RB) {
         K k = s_key[i];
