@@ -38,6 +38,15 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid) {
     return m;
 }
 
+// Barrier for work that shares only LDS: waits for this wave's LDS accesses, not for its global
+// loads and stores, so loads and stores issued before it stay in flight.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_barrier();
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
 // ---------------------------------------------------------------- scene reduction
 constexpr int NRED = 13;
 
@@ -196,18 +205,25 @@ template <typename K, int RI, int DB>
 __global__ __launch_bounds__(RB) void k_morton_tile(const float4* __restrict__ sph, uint64_t n,
                                                     const SceneInfo* __restrict__ info, K* __restrict__ keys,
                                                     uint32_t* __restrict__ hist) {
-    constexpr int RT = RB * RI, RW = RT / (RB / 64), SB = 8, ND = 1 << DB;
+    constexpr int RT = RB * RI, RW = RT / (RB / 64), SB = RI < 16 ? RI : 16, ND = 1 << DB;
     __shared__ uint32_t h[ND];
     for (int d = threadIdx.x; d < ND; d += RB) h[d] = 0;
     __syncthreads();
     const float sc = info->scale;
     const uint64_t base = (uint64_t)blockIdx.x * RT + (uint64_t)(threadIdx.x >> 6) * RW + __lane_id();
+    const bool whole = (uint64_t)(blockIdx.x + 1) * RT <= n;  // uniform
     for (int r0 = 0; r0 < RI; r0 += SB) {
-        float4 sb[SB];  // SB sphere loads in flight, at clamped indices
+        float4 sb[SB];  // SB sphere loads in flight: a whole tile's at constant offsets, else at clamped indices
+        if (whole) {
+            const float4* sp = sph + base;
 #pragma unroll
-        for (int j = 0; j < SB; ++j) {
-            const uint64_t i = base + (uint64_t)(r0 + j) * 64;
-            sb[j] = sph[i < n ? i : n - 1];
+            for (int j = 0; j < SB; ++j) sb[j] = sp[(r0 + j) * 64];
+        } else {
+#pragma unroll
+            for (int j = 0; j < SB; ++j) {
+                const uint64_t i = base + (uint64_t)(r0 + j) * 64;
+                sb[j] = sph[i < n ? i : n - 1];
+            }
         }
 #pragma unroll
         for (int j = 0; j < SB; ++j) {
@@ -217,13 +233,13 @@ __global__ __launch_bounds__(RB) void k_morton_tile(const float4* __restrict__ s
                 key = morton63(sb[j], info, sc);
             else
                 key = morton30(sb[j], info, sc);
-            if (i < n) {
+            if (whole || i < n) {
                 keys[i] = key;
                 hist_add(h, (uint32_t)(key & (K)(ND - 1)));
             }
         }
     }
-    __syncthreads();
+    lds_barrier();  // the histogram is in LDS; the key stores drain meanwhile
     for (int d = threadIdx.x; d < ND; d += RB) hist[(size_t)blockIdx.x * ND + d] = h[d];
 }
 
@@ -258,15 +274,6 @@ __global__ __launch_bounds__(RB) void k_radix_hist(const K* __restrict__ keys, u
     }
     __syncthreads();
     for (int dd = threadIdx.x; dd < ND; dd += RB) hist[(size_t)blockIdx.x * ND + dd] = h[dd];
-}
-
-// Barrier for work that shares only LDS: waits for this wave's LDS accesses, not for its global
-// loads and stores, so loads issued before it stay in flight.
-__device__ __forceinline__ void lds_barrier() {
-    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    __builtin_amdgcn_s_barrier();
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
 // Stable scatter of one tile for one digit (vin null: the values are the key indices). Each wave
