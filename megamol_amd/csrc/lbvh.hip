@@ -817,7 +817,10 @@ __device__ __forceinline__ int split_delta(const K* __restrict__ keys, int64_t n
 // Straddling nodes are a small fraction: the ancestors of one node per chunk boundary.
 // Arrival at a global meeting place: every split receives exactly two arrivals per build, so an
 // exchange of the build's generation number gen into flags[s] tells them apart (the second one
-// reads gen back) without clearing the 4n-byte array between builds; it starts as all ones.
+// reads gen back) without clearing the 4n-byte array between builds; it starts as all ones. The
+// level-synchronous chunk pass ahead of span passes exchanges nothing: no split it delivers to
+// globally can get its other arrival in that pass except from the neighbouring chunk at a chunk
+// boundary, so it stores gen (per side in flags2 at boundaries) and the later passes read it.
 constexpr int GEN_SLOT = BOX_ROUNDS + 3;  // box_cnt slot holding the generation number
 constexpr int ROUNDS_SLOT = BOX_ROUNDS + 2;  // box_cnt slot: kernel rounds this build needed
 
